@@ -1,0 +1,125 @@
+"""ctypes front of the FP64 CPU oracle (blfid_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg, always as the checker, never as the thing measured or shipped.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liborc.so")
+_lib = None
+
+_d = ctypes.POINTER(ctypes.c_double)
+_i = ctypes.POINTER(ctypes.c_int)
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = ctypes.CDLL(LIB)
+        L.orc_powder_fid.restype = ctypes.c_int
+        L.orc_powder_fid.argtypes = [ctypes.c_int, _i, ctypes.c_int, _d, ctypes.c_int, _d,
+                                     ctypes.c_int, _d, ctypes.c_double, ctypes.c_double,
+                                     ctypes.c_double, ctypes.c_int, ctypes.c_double,
+                                     ctypes.c_int, _d, _d, _d, _d, _d, _d, ctypes.c_int,
+                                     ctypes.c_int, _d, _d]
+        L.orc_hamiltonian.argtypes = [ctypes.c_int, _i, ctypes.c_int, _d, ctypes.c_int, _d,
+                                      ctypes.c_int, _d] + [ctypes.c_double] * 5 + [_d]
+        L.orc_eig_herm_x.argtypes = [ctypes.c_int, _d, _d, _d]
+        L.orc_eig_normal_x.argtypes = [ctypes.c_int, _d, _d, _d]
+        L.orc_expm_herm_x.argtypes = [ctypes.c_int, _d, ctypes.c_double, ctypes.c_double, _d]
+        _lib = L
+    return _lib
+
+
+def _p(a, t=_d):
+    return a.ctypes.data_as(t)
+
+
+def _cm(M):
+    """complex128 column-major copy viewed as float64."""
+    return np.asfortranarray(np.asarray(M, dtype=np.complex128)).ravel(order="F").view(np.float64)
+
+
+def _sys(sy):
+    iso, csa, dip, jc = sy.tables()
+    iso = np.ascontiguousarray(iso, dtype=np.int32)
+    return (iso, len(sy.csa), csa if csa.size else np.zeros(1), len(sy.dip),
+            dip if dip.size else np.zeros(1), len(sy.jc), jc if jc.size else np.zeros(1))
+
+
+def powder_fid(sy, wr, rotor_deg, sw, npts, powder, rho, det, t0=0.0, normalize=True,
+               nthreads=None):
+    """Powder-summed FID (complex128[npts]) and the adjusted sweep width."""
+    L = lib()
+    iso, nc, csa, nd, dip, nj, jc = _sys(sy)
+    r = _cm(rho)
+    d = _cm(det)
+    out = np.zeros(2 * max(npts, 1), dtype=np.float64)
+    swo = np.zeros(1)
+    th = np.ascontiguousarray(powder.theta, np.float64)
+    ph = np.ascontiguousarray(powder.phi, np.float64)
+    ga = np.ascontiguousarray(powder.gamma, np.float64)
+    w = np.ascontiguousarray(powder.weight, np.float64)
+    nthreads = nthreads or os.cpu_count() or 1
+    st = L.orc_powder_fid(sy.spins, _p(iso, _i), nc, _p(csa), nd, _p(dip), nj, _p(jc),
+                          float(wr), float(rotor_deg), float(sw), int(npts), float(t0), len(th),
+                          _p(th), _p(ph), _p(ga), _p(w), _p(r), _p(d), int(bool(normalize)),
+                          int(nthreads), _p(out), _p(swo))
+    if st != 0:
+        raise RuntimeError(f"oracle failed with status {st}")
+    return out[: 2 * npts].view(np.complex128).copy(), float(swo[0])
+
+
+def hamiltonian(sy, theta, phi, gamma, rotor_deg, rotor_phase):
+    L = lib()
+    iso, nc, csa, nd, dip, nj, jc = _sys(sy)
+    N = sy.N
+    H = np.zeros(2 * N * N)
+    L.orc_hamiltonian(sy.spins, _p(iso, _i), nc, _p(csa), nd, _p(dip), nj, _p(jc), theta, phi,
+                      gamma, rotor_deg, rotor_phase, _p(H))
+    return H.view(np.complex128).reshape(N, N, order="F")
+
+
+def eig_herm(A):
+    N = A.shape[0]
+    a = _cm(A)
+    w = np.zeros(N)
+    V = np.zeros(2 * N * N)
+    st = lib().orc_eig_herm_x(N, _p(a), _p(w), _p(V))
+    if st:
+        raise RuntimeError(st)
+    return w, V.view(np.complex128).reshape(N, N, order="F")
+
+
+def eig_normal(A):
+    N = A.shape[0]
+    a = _cm(A)
+    lam = np.zeros(2 * N)
+    Q = np.zeros(2 * N * N)
+    st = lib().orc_eig_normal_x(N, _p(a), _p(lam), _p(Q))
+    if st:
+        raise RuntimeError(st)
+    return lam.view(np.complex128).copy(), Q.view(np.complex128).reshape(N, N, order="F")
+
+
+def expm_herm(H, mul):
+    N = H.shape[0]
+    h = _cm(H)
+    U = np.zeros(2 * N * N)
+    st = lib().orc_expm_herm_x(N, _p(h), float(np.real(mul)), float(np.imag(mul)), _p(U))
+    if st:
+        raise RuntimeError(st)
+    return U.view(np.complex128).reshape(N, N, order="F")

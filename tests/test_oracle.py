@@ -1,0 +1,210 @@
+"""Pins the FP64 CPU oracle before anything is compared against it.
+
+Each check is chosen so that a plausible mistake in the restatement fails it:
+  * linear algebra   -> numpy/scipy (LAPACK) on random and degenerate inputs
+  * Hamiltonian      -> an independent numpy restatement (Kronecker operators, elementary
+                        rotations) and geometric closed forms (axial P2, magic-angle average)
+  * static path      -> the reference's stored output examples/solid-2.0/examples/static/simpSTA
+  * gamma-COMPUTE    -> explicit step propagation (oracle/bruteforce.py), rosym and general rho,
+                        even and odd n (integer-division scale), wrap-around of the rotor period
+  * closed forms     -> fid[0] and a single-spin MAS FID from a scalar phase sum
+"""
+import numpy as np
+import pytest
+import scipy.linalg
+
+from blochlib_amd.powder import Powder
+from blochlib_amd.spinsys import SolidSys
+from oracle import bruteforce as B
+
+MAGIC = float(np.degrees(np.arccos(1 / np.sqrt(3))))
+
+
+def rand_herm(rng, N):
+    A = rng.normal(size=(N, N)) + 1j * rng.normal(size=(N, N))
+    return (A + A.conj().T) / 2
+
+
+@pytest.mark.parametrize("N", [1, 2, 3, 5, 8, 16, 33])
+def test_eig_herm_matches_lapack(oracle, N):
+    rng = np.random.default_rng(N)
+    A = rand_herm(rng, N)
+    w, V = oracle.eig_herm(A)
+    assert np.allclose(np.sort(w), np.linalg.eigvalsh(A), atol=1e-12 * N)
+    assert np.allclose(V.conj().T @ V, np.eye(N), atol=1e-13 * N)
+    assert np.allclose(V @ np.diag(w) @ V.conj().T, A, atol=1e-12 * N)
+
+
+def test_eig_herm_degenerate(oracle):
+    rng = np.random.default_rng(7)
+    Q, _ = np.linalg.qr(rng.normal(size=(12, 12)) + 1j * rng.normal(size=(12, 12)))
+    d = np.array([1, 1, 1, -2, -2, 0.5, 3, 3, 3, 3, 0, 0], float)
+    A = Q @ np.diag(d) @ Q.conj().T
+    w, V = oracle.eig_herm(A)
+    assert np.allclose(np.sort(w), np.sort(d), atol=1e-12)
+    assert np.allclose(V @ np.diag(w) @ V.conj().T, A, atol=1e-12)
+
+
+@pytest.mark.parametrize("N", [1, 2, 4, 7, 16, 40])
+def test_eig_normal_unitary(oracle, N):
+    rng = np.random.default_rng(100 + N)
+    U = scipy.linalg.expm(-1j * 3.0 * rand_herm(rng, N))
+    lam, W = oracle.eig_normal(U)
+    assert np.allclose(W.conj().T @ W, np.eye(N), atol=1e-13 * N)
+    assert np.allclose(U @ W, W * lam, atol=1e-12 * N)
+    ref = np.linalg.eigvals(U)
+    assert np.allclose(np.sort_complex(lam), np.sort_complex(ref), atol=1e-12 * N)
+
+
+def test_eig_normal_degenerate_blocks(oracle):
+    rng = np.random.default_rng(3)
+    Q, _ = np.linalg.qr(rng.normal(size=(10, 10)) + 1j * rng.normal(size=(10, 10)))
+    ph = np.exp(1j * np.array([0.3, 0.3, 0.3, -1.0, -1.0, 2.0, np.pi, np.pi, 0.0, 0.0]))
+    U = Q @ np.diag(ph) @ Q.conj().T
+    lam, W = oracle.eig_normal(U)
+    assert np.allclose(W.conj().T @ W, np.eye(10), atol=1e-12)
+    assert np.allclose(U @ W, W * lam, atol=1e-12)
+
+
+@pytest.mark.parametrize("N", [2, 4, 9, 32])
+def test_expm_matches_scipy(oracle, N):
+    rng = np.random.default_rng(200 + N)
+    H = rand_herm(rng, N)
+    mul = -1j * 0.37
+    U = oracle.expm_herm(H, mul)
+    assert np.allclose(U, scipy.linalg.expm(mul * H), atol=1e-12)
+
+
+SYS3 = """numspin 3
+T 1H 0
+T 1H 1
+T 13C 2
+C 1000 5000 0.5 0 10 20 30
+C -1000 3000 0.2 1 40 50 60
+C 300 7000 0.9 2
+C 0 4000 0 2 5 6 7
+C 250 0 0 1
+D -20000 0 1 0 30 40
+D 5000 1 2 10 70 20
+J 140 0 2
+J 50 0 1 1
+"""
+
+
+def test_hamiltonian_matches_independent_numpy(oracle):
+    sy = SolidSys.from_text(SYS3)
+    rng = np.random.default_rng(11)
+    for _ in range(6):
+        th, ph, ga = rng.uniform(0, np.pi), rng.uniform(0, 2 * np.pi), rng.uniform(0, 2 * np.pi)
+        for rot, al in [(MAGIC, rng.uniform(0, 6.3)), (0.0, 0.0), (23.0, rng.uniform(0, 6.3))]:
+            H1 = oracle.hamiltonian(sy, th, ph, ga, rot, al)
+            H2 = B.hamiltonian(sy, th, ph, ga, rot, al)
+            assert np.allclose(H1, H2, atol=1e-9), (rot, al)
+            assert np.allclose(H1, H1.conj().T)
+
+
+def test_axial_csa_is_delta_p2(oracle):
+    """eta = 0: zz = delta * P2(cos angle(tensor axis, B0)), computed from unit vectors."""
+    sy = SolidSys.from_text("numspin 1\nC 0 7000 0 0 20 35 50\n")
+    rng = np.random.default_rng(5)
+    for _ in range(5):
+        th, ph, ga = rng.uniform(0, np.pi), rng.uniform(0, 2 * np.pi), rng.uniform(0, 2 * np.pi)
+        H = oracle.hamiltonian(sy, th, ph, ga, 0.0, 0.0)
+        # tensor axis in the crystal frame = third column of E (E maps PAS -> crystal)
+        E = B.euler(*np.radians([20, 35, 50]))
+        axis_c = E[:, 2]
+        # lab z in the crystal frame = third row of the powder rotation
+        z_c = B.euler(ph, th, ga)[2]
+        c = axis_c @ z_c
+        zz = 7000 * (3 * c * c - 1) / 2
+        assert np.isclose(H[0, 0].real, 0.5 * zz, atol=1e-9)
+
+
+def test_magic_angle_rotor_average_vanishes(oracle):
+    """Average of the anisotropic term over a rotor cycle is P2(cos beta_r) * (axis part):
+    it vanishes at the magic angle for every orientation and eta."""
+    sy = SolidSys.from_text("numspin 1\nC 0 9000 0.7 0 12 77 140\n")
+    phases = np.linspace(0, 2 * np.pi, 64, endpoint=False)
+    for th, ph, ga in [(0.3, 1.1, 2.0), (2.0, 4.0, 0.5)]:
+        avg = np.mean([oracle.hamiltonian(sy, th, ph, ga, MAGIC, a)[0, 0].real for a in phases])
+        assert abs(avg) < 1e-9
+        avg30 = np.mean([oracle.hamiltonian(sy, th, ph, ga, 30.0, a)[0, 0].real for a in phases])
+        assert abs(avg30) > 1.0
+
+
+def test_static_fid_matches_reference_output(oracle):
+    """examples/solid-2.0/examples/static/static-basic.sim -> simpSTA (6 significant digits,
+    produced by the reference's complex64 diagonal path)."""
+    sy = SolidSys.from_text("numspin 2\nT 1H 0\nT 1H 1\nC 5000  2134 0 0\nC -5000 2789 0.5 1\n")
+    fid, sw = oracle.powder_fid(sy, 0.0, 0.0, 40000.0, 512, Powder.read("rep2000"),
+                                sy.op("Ix"), sy.op("Ip"), normalize=True)
+    ref = np.loadtxt("tests/golden/simpSTA.txt")
+    fr = ref[:, 4] + 1j * ref[:, 5]
+    assert sw == 40000.0
+    assert np.abs(fid - fr).max() < 2e-6
+    assert np.linalg.norm(fid - fr) / np.linalg.norm(fr) < 1e-5
+
+
+@pytest.mark.parametrize("sw", [80000.0, 70000.0, 120000.0])
+@pytest.mark.parametrize("rho_name", ["Ix", "Iz+0.3*Ix_0"])
+def test_gamma_compute_equals_explicit_stepping(oracle, sw, rho_name):
+    sy = SolidSys.from_text(SYS3)
+    rho, det = sy.op(rho_name), sy.op("Ip")
+    wr, npts = 10000.0, 27
+    for th, ph, ga in [(0.7, 1.9, 2.3), (2.9, 0.2, 5.5)]:
+        f1, swo = oracle.powder_fid(sy, wr, MAGIC, sw, npts, Powder.single(th, ph, ga), rho, det,
+                                    normalize=False, nthreads=1)
+        f2 = B.gamma_compute_equivalent(sy, th, ph, ga, MAGIC, wr, sw, npts, rho, det)
+        assert swo == int(np.floor(sw / wr + 0.5)) * wr
+        assert np.abs(f1 - f2).max() < 1e-11 * max(1.0, np.abs(f2).max())
+
+
+def test_period_phase_is_reference_not_forward(oracle):
+    """The reference applies whole rotor periods through U_T^-1 (see bruteforce docstring):
+    the forward-propagated signal agrees only before the first wrap."""
+    sy = SolidSys.from_text(SYS3)
+    rho, det = sy.op("Ix"), sy.op("Ip")
+    f1, _ = oracle.powder_fid(sy, 10000.0, MAGIC, 80000.0, 20, Powder.single(0.7, 1.9, 2.3), rho,
+                              det, normalize=False, nthreads=1)
+    fw = B.gamma_compute_equivalent(sy, 0.7, 1.9, 2.3, MAGIC, 10000.0, 80000.0, 20, rho, det,
+                                    forward=True)
+    assert np.isclose(f1[0], fw[0])
+    assert np.abs(f1 - fw).max() > 1e-3
+
+
+def test_fid0_closed_form_and_powder_sum(oracle):
+    """fid[0] = sum(w) * (Tr(D^2) + Tr(D D^H)) * 2 / Tr(D D^H) for rho = (D+D^H)/2 (= 2 for
+    D = Fp), and the powder sum is linear in the weights."""
+    sy = SolidSys.from_text(SYS3)
+    pw = Powder.read("ZCW_3_1154").subset(np.arange(0, 1154, 97))
+    rho, det = sy.op("Ix"), sy.op("Ip")
+    f, _ = oracle.powder_fid(sy, 10000.0, MAGIC, 40000.0, 9, pw, rho, det, normalize=True)
+    assert np.isclose(f[0], 2.0 * pw.weight.sum(), atol=1e-12)
+    pw2 = Powder(pw.theta, pw.phi, pw.gamma, 2.5 * pw.weight)
+    f2, _ = oracle.powder_fid(sy, 10000.0, MAGIC, 40000.0, 9, pw2, rho, det, normalize=True,
+                              nthreads=3)
+    assert np.allclose(f2, 2.5 * f, atol=1e-12)
+
+
+def test_single_spin_mas_closed_form(oracle):
+    """One spin, CSA only: H(t) = w(t) Iz commutes, so each start index i gives
+    2 * 0.5 * exp(i 2 pi tau * phase sum) with the period part entering as U_T^-1."""
+    sy = SolidSys.from_text("numspin 1\nC 800 6000 0.4 0 10 40 70\n")
+    wr, sw, npts = 5000.0, 40000.0, 21
+    n = 8
+    tau = 1 / wr / n
+    th, ph, ga = 1.1, 0.4, 2.2
+    w = np.array([2 * oracle.hamiltonian(sy, th, ph, ga, MAGIC, 2 * np.pi * wr * (k + 0.5) * tau)[0, 0].real
+                  for k in range(n)])
+    cum = np.concatenate([[0.0], np.cumsum(w)])  # A_k phase for |+><-|
+    fid = np.zeros(npts, complex)
+    for i in range(n):
+        for t in range(npts):
+            M = (i + t) // n
+            j = (i + t) % n
+            phase = cum[j + 1] - M * cum[n] - cum[i + 1]
+            fid[t] += np.exp(2j * np.pi * tau * phase)
+    fid *= 2 * 0.5 / (n // 2)
+    f1, _ = oracle.powder_fid(sy, wr, MAGIC, sw, npts, Powder.single(th, ph, ga), sy.op("Ix"),
+                              sy.op("Ip"), normalize=False, nthreads=1)
+    assert np.abs(f1 - fid).max() < 1e-12
