@@ -1,0 +1,37 @@
+// blfid_dev.h -- device-side plan tables shared by the host planner and the gfx950 kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#define DEV_TERM_STRIDE 10 /* Cxx Cyy Czz Cxy Cxz Cyz  fdiag  fflip  i  j(-1 for CSA) */
+#define DEV_MAX_TERMS 128
+#define DEV_MAX_SPINS 12
+#define DEV_MAX_BLOCK 80 /* largest Hilbert-space block handled by k_eig's LDS tile */
+
+struct DevPlan {
+    int S, N;      // spins, Hilbert dimension
+    int n;         // steps per rotor period (1 for the static path)
+    int npts;      // FID points
+    int nterms;    // anisotropic interaction terms
+    int nblocks;   // Hilbert-space blocks
+    int npairs;    // coupled (I,J) block pairs of det / rho_eff
+    long Lsum;     // sum of b^2 over blocks: per-(crystallite,step) matrix storage
+    long Etot;     // sum of b_I*b_J over pairs: correlation elements per crystallite
+    double theta;  // 2*pi*tau
+    const double *terms;   // nterms x DEV_TERM_STRIDE
+    const double *dconst;  // N: isotropic diagonal of H (-iso_csa m_i + J m_i m_j)
+    const double *ffconst; // S x S: constant flip-flop coefficients (strong J / 2)
+    const double *u3;      // n x 3: row 2 of cartSpin at each step midpoint
+    const double *RP;      // K x 9: cartPowder of each crystallite (row-major)
+    const int *blk_dim;    // nblocks
+    const int *blk_soff;   // nblocks: offset of the block in the state list (sum of b)
+    const long *blk_uoff;  // nblocks: offset of the block in Lsum storage
+    const int *blk_states; // N: basis-state index of each block-local row
+    const int *pair_I, *pair_J;
+    const long *pair_eoff; // element offset of each pair (sum of bI*bJ)
+    const int *pair_doff;  // start of each pair's det CSR row pointer (bI+1 entries)
+    const int *pair_roff;  // start of each pair's rho_eff CSR row pointer (bJ+1 entries)
+    const int *d_rowptr, *d_col;
+    const double *d_val;   // complex, interleaved
+    const int *r_rowptr, *r_col;
+    const double *r_val;   // complex, interleaved
+};

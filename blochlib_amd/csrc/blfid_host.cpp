@@ -1,0 +1,797 @@
+// blfid_host.cpp -- planner and C ABI of the MI355X powder FID engine (include/blfid.h).
+//
+// The planner lowers a tensor-described SolidSys (src/QMspins/sys.h:59-62) to device tables:
+//   * anisotropic terms: crystal-frame tensor C = R_PAS diag(PAS) R_PAS^T (csa.cc:276-284,
+//     dip.cc:159-167) with the spin factor of their diagonal (Iz_i, T20 homo/hetero,
+//     spin_ten.cc:211-236, 431-452) and flip-flop parts;
+//   * isotropic diagonal (-iso Iz, csa.cc:296-300; J Iz Iz, j.cc:135-137) and strong-J
+//     flip-flops (spin_ten.cc:455-465);
+//   * the Hilbert-space blocks: connected components of the flip-flop graph over Zeeman
+//     states.  H(t) never couples two blocks, so U, A, W and the gamma-COMPUTE sums are done
+//     per block and per coupled block pair;
+//   * the rotor direction at each step midpoint: tmid = (t2-t1)/2 + t1 with t1, t2 advanced
+//     by tau exactly as compute::calcUFID (compute.h:257-274) and sys.cc:753-755.
+// Operators rho and det are split into sparse block-pair parts; rosym is detected with the
+// reference's exact comparison ro == 0.5*(det + det^H) (compute.h:75-83).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <map>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/blfid.h"
+#include "blfid_dev.h"
+
+typedef std::complex<double> cplx;
+
+extern "C" {
+int blfid_dev_launch_exp(int M, const DevPlan *P, int Kb, int c0, const int *grp, int ngrp,
+                         double *scratch, size_t slot, int nslots, double2 *bufU, hipStream_t st);
+int blfid_dev_launch_dyson(int M, const DevPlan *P, int Kb, const int *grp, int ngrp,
+                           const double2 *bufU, double2 *bufA, hipStream_t st);
+int blfid_dev_launch_eig(const DevPlan *P, int Kb, const int *grp, int ngrp, const double2 *src,
+                         size_t src_stride, int mode, double2 *bufW, double *outv, hipStream_t st);
+int blfid_dev_launch_bmul(int M, const DevPlan *P, int Kb, const int *grp, int ngrp,
+                          const double2 *bufA, const double2 *bufW, double2 *bufB, hipStream_t st);
+int blfid_dev_launch_qr(int M, const DevPlan *P, int Kb, const int *pgrp, int ngrp,
+                        const double2 *bufB, const double *thetas, double2 *scratch, size_t slot,
+                        int nslots, double2 *Qt, double2 *Rt, hipStream_t st);
+int blfid_dev_launch_corr(const DevPlan *P, int Kb, const double2 *Qt, const double2 *Rt,
+                          const double *thetas, const int *er, const int *es, double *fidc,
+                          hipStream_t st);
+int blfid_dev_launch_reduce(int Kb, int c0, int npts, const double *w, double scale,
+                            const double *fidc, double *acc, double *comp, hipStream_t st);
+int blfid_dev_launch_finish(int npts, const double *acc, double nre, double nim, double *out,
+                            hipStream_t st);
+int blfid_dev_launch_zero(double *p, size_t n, hipStream_t st);
+int blfid_dev_launch_prep(int K, const double *th, const double *ph, const double *ga,
+                          double *RP, hipStream_t st);
+int blfid_dev_launch_static_h(const DevPlan *P, int Kb, int c0, const int *grp, int ngrp,
+                              double *scratch, size_t slot, int nslots, double2 *bufH,
+                              hipStream_t st);
+int blfid_dev_launch_static_fid(int M, const DevPlan *P, int Kb, const int *pgrp, int ngrp,
+                                const double2 *bufW, const double *lam, double2 *scratch,
+                                size_t slot, int nslots, double dt, double *fidc, hipStream_t st);
+}
+
+static thread_local std::string g_err;
+
+static blfid_status fail(blfid_status s, const std::string &msg) {
+    g_err = msg;
+    return s;
+}
+
+#define HIPCHK(expr)                                                                          \
+    do {                                                                                      \
+        hipError_t _e = (expr);                                                               \
+        if (_e != hipSuccess)                                                                 \
+            return fail(_e == hipErrorOutOfMemory ? BLFID_ERR_NOMEM : BLFID_ERR_DEVICE,       \
+                        std::string(#expr) + ": " + hipGetErrorString(_e));                   \
+    } while (0)
+
+#define LAUNCHCHK(expr)                                                                       \
+    do {                                                                                      \
+        int _e = (expr);                                                                      \
+        if (_e != 0)                                                                          \
+            return fail(BLFID_ERR_DEVICE, std::string(#expr) + ": " +                         \
+                                              hipGetErrorString((hipError_t)_e));             \
+    } while (0)
+
+static const double PI2 = 6.283185307179586476925286766559;
+static const double DEG2RAD = 0.017453292519943295769236907684886;
+
+namespace {
+
+void rot3(double a, double b, double g, double R[9]) {
+    double cg = cos(g), cb = cos(b), ca = cos(a), sg = sin(g), sb = sin(b), sa = sin(a);
+    R[0] = ca * cb * cg - sa * sg; R[1] = sa * cb * cg + ca * sg; R[2] = -sb * cg;
+    R[3] = -ca * cb * sg - sa * cg; R[4] = -sa * cb * sg + ca * cg; R[5] = sb * sg;
+    R[6] = ca * sb; R[7] = sa * sb; R[8] = cb;
+}
+
+// C = R diag(p) R^T with R = rotationMatrix3D(Euler in degrees): 6 unique components
+void crystal_tensor(double pxx, double pyy, double pzz, double a, double b, double g,
+                    double C6[6]) {
+    double R[9];
+    rot3(a * DEG2RAD, b * DEG2RAD, g * DEG2RAD, R);
+    double p[3] = {pxx, pyy, pzz};
+    double C[9];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            double s = 0;
+            for (int k = 0; k < 3; k++) s += R[3 * i + k] * p[k] * R[3 * j + k];
+            C[3 * i + j] = s;
+        }
+    C6[0] = C[0]; C6[1] = C[4]; C6[2] = C[8];
+    C6[3] = 0.5 * (C[1] + C[3]); C6[4] = 0.5 * (C[2] + C[6]); C6[5] = 0.5 * (C[5] + C[7]);
+}
+
+template <typename T> struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+    hipError_t alloc(size_t cnt) {
+        if (cnt <= n && p) return hipSuccess;
+        release();
+        n = cnt;
+        if (cnt == 0) return hipSuccess;
+        return hipMalloc((void **)&p, cnt * sizeof(T));
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    hipError_t upload(const std::vector<T> &v) {
+        hipError_t e = alloc(v.size() ? v.size() : 1);
+        if (e != hipSuccess || v.empty()) return e;
+        return hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+    }
+};
+
+struct Block {
+    std::vector<int> states;
+};
+
+struct Pair {
+    int I, J;
+};
+
+} // namespace
+
+struct blfid_plan {
+    // input
+    int S = 0, N = 0;
+    std::vector<int> iso;
+    std::vector<double> csa, dip, jc;
+    double wr = 0, rotor_deg = 0, sw = 0, t0 = 0;
+    int npts = 0;
+    bool spinning = false;
+    // derived
+    int n = 1;
+    double sw_adj = 0, tau = 0;
+    std::vector<double> terms, dconst, ffconst, u3;
+    std::vector<Block> blocks;
+    std::vector<int> blk_dim, blk_soff, blk_states;
+    std::vector<long> blk_uoff;
+    long Lsum = 0;
+    int bmax = 0;
+    std::map<int, std::vector<int>> blk_groups; // M -> blocks
+    // operators
+    bool have_ops = false;
+    int rosym = 0;
+    cplx trdd = 0;
+    std::vector<Pair> pairs;
+    std::vector<long> pair_eoff;
+    std::vector<int> pair_doff, pair_roff, d_rowptr, d_col, r_rowptr, r_col, elem_r, elem_s;
+    std::vector<double> d_val, r_val;
+    long Etot = 0;
+    std::map<int, std::vector<int>> pair_groups; // M -> pairs
+    // powder
+    int K = 0;
+    // device
+    DevBuf<double> d_terms, d_dconst, d_ffconst, d_u3, d_RP, d_th, d_ph, d_ga, d_w;
+    DevBuf<int> d_blk_dim, d_blk_soff, d_blk_states, d_pair_I, d_pair_J, d_pair_doff,
+        d_pair_roff, d_drow, d_dcol, d_rrow, d_rcol, d_er, d_es;
+    DevBuf<long> d_blk_uoff, d_pair_eoff;
+    DevBuf<double> d_dval, d_rval;
+    std::map<int, DevBuf<int>> d_bgrp, d_pgrp;
+    DevBuf<double2> bufU, bufA, bufW, bufQt, bufRt, scrQ;
+    DevBuf<double> thetas, scrE, fidc, acc, comp, fid_out;
+    double budget = 0;
+    int Kb = 0;
+    hipStream_t last_stream = nullptr;
+    bool ran = false;
+    DevPlan dp{};
+};
+
+// ------------------------------------------------------------------------------ planning
+static blfid_status build_tables(blfid_plan *pl) {
+    const int S = pl->S, N = pl->N;
+    pl->terms.clear();
+    pl->dconst.assign(N, 0.0);
+    pl->ffconst.assign(S * S, 0.0);
+    auto mz = [&](int s, int i) { return ((s >> (S - 1 - i)) & 1) ? -0.5 : 0.5; };
+    const double sq3 = std::pow(3., 0.5);
+    // flip-flop adjacency for the block structure
+    std::vector<std::vector<char>> ffpair(S, std::vector<char>(S, 0));
+    for (size_t c = 0; c < pl->csa.size() / 7; c++) {
+        const double *p = &pl->csa[7 * c];
+        double iso = p[0], del = p[1], eta = p[2];
+        int on = (int)p[3];
+        if (iso != 0) {
+            double isoterm = (1. / sq3) * (-sq3 * iso);
+            for (int s = 0; s < N; s++) pl->dconst[s] += isoterm * mz(s, on);
+        }
+        if (iso != 0 && del == 0.0) continue;
+        double t[DEV_TERM_STRIDE];
+        crystal_tensor(del * (eta - 1.0) / 2.0, -del * (eta + 1.0) / 2.0, 1.0 * del, p[4], p[5],
+                       p[6], t);
+        t[6] = 1.0;
+        t[7] = 0.0;
+        t[8] = on;
+        t[9] = -1;
+        pl->terms.insert(pl->terms.end(), t, t + DEV_TERM_STRIDE);
+    }
+    for (size_t d = 0; d < pl->dip.size() / 6; d++) {
+        const double *p = &pl->dip[6 * d];
+        double del = p[0];
+        int i = (int)p[1], j = (int)p[2];
+        double s6 = std::pow(6.0, 0.5);
+        double t[DEV_TERM_STRIDE];
+        crystal_tensor(-0.5 * del * s6, -0.5 * del * s6, 1.0 * del * s6, p[3], p[4], p[5], t);
+        bool homo = pl->iso[i] == pl->iso[j];
+        if (homo) {
+            double f = std::pow(1. / 6., 0.5);
+            t[6] = f * 2.0;
+            t[7] = f * (-0.5);
+            if (del != 0.0) ffpair[i][j] = ffpair[j][i] = 1;
+        } else {
+            t[6] = std::pow(4. / 6., 0.5);
+            t[7] = 0.0;
+        }
+        t[8] = i;
+        t[9] = j;
+        pl->terms.insert(pl->terms.end(), t, t + DEV_TERM_STRIDE);
+    }
+    for (size_t q = 0; q < pl->jc.size() / 4; q++) {
+        const double *p = &pl->jc[4 * q];
+        double iso = p[0];
+        int i = (int)p[1], j = (int)p[2], strong = (int)p[3];
+        double f = iso * sq3 * (1. / sq3);
+        for (int s = 0; s < N; s++) pl->dconst[s] += f * mz(s, i) * mz(s, j);
+        if (strong) {
+            pl->ffconst[i * S + j] += 0.5 * f;
+            pl->ffconst[j * S + i] += 0.5 * f;
+            if (f != 0.0) ffpair[i][j] = ffpair[j][i] = 1;
+        }
+    }
+    if ((int)(pl->terms.size() / DEV_TERM_STRIDE) > DEV_MAX_TERMS)
+        return fail(BLFID_ERR_UNSUPPORTED, "too many interactions (limit 128 anisotropic terms)");
+    // blocks: union-find over states, edges = flip-flops of coupled pairs
+    std::vector<int> par(N);
+    std::iota(par.begin(), par.end(), 0);
+    std::function<int(int)> find = [&](int x) {
+        while (par[x] != x) x = par[x] = par[par[x]];
+        return x;
+    };
+    for (int i = 0; i < S; i++)
+        for (int j = i + 1; j < S; j++) {
+            if (!ffpair[i][j]) continue;
+            int bi = 1 << (S - 1 - i), bj = 1 << (S - 1 - j);
+            for (int s = 0; s < N; s++) {
+                if (((s & bi) != 0) != ((s & bj) != 0)) {
+                    int a = find(s), b = find(s ^ bi ^ bj);
+                    if (a != b) par[a] = b;
+                }
+            }
+        }
+    std::map<int, int> root2blk;
+    pl->blocks.clear();
+    for (int s = 0; s < N; s++) {
+        int r = find(s);
+        auto it = root2blk.find(r);
+        if (it == root2blk.end()) {
+            root2blk[r] = (int)pl->blocks.size();
+            pl->blocks.push_back(Block());
+            it = root2blk.find(r);
+        }
+        pl->blocks[it->second].states.push_back(s);
+    }
+    pl->blk_dim.clear(); pl->blk_soff.clear(); pl->blk_uoff.clear(); pl->blk_states.clear();
+    pl->blk_groups.clear();
+    long uo = 0;
+    int so = 0;
+    pl->bmax = 0;
+    for (size_t b = 0; b < pl->blocks.size(); b++) {
+        int d = (int)pl->blocks[b].states.size();
+        pl->blk_dim.push_back(d);
+        pl->blk_soff.push_back(so);
+        pl->blk_uoff.push_back(uo);
+        for (int s : pl->blocks[b].states) pl->blk_states.push_back(s);
+        so += d;
+        uo += (long)d * d;
+        pl->bmax = std::max(pl->bmax, d);
+        int M = std::min(5, (d + 15) / 16);
+        pl->blk_groups[M].push_back((int)b);
+    }
+    pl->Lsum = uo;
+    if (pl->bmax > DEV_MAX_BLOCK)
+        return fail(BLFID_ERR_UNSUPPORTED,
+                    "largest Hilbert-space block " + std::to_string(pl->bmax) +
+                        " exceeds the engine's limit of " + std::to_string(DEV_MAX_BLOCK));
+    // rotor direction (row 2 of cartSpin) at each step midpoint
+    pl->u3.clear();
+    if (pl->spinning) {
+        double br = pl->rotor_deg * DEG2RAD;
+        double t1 = pl->t0, t2 = pl->t0 + pl->tau;
+        for (int k = 0; k < pl->n; k++) {
+            double tmid = (t2 - t1) / 2.0 + t1;
+            double al = tmid * pl->wr * PI2;
+            pl->u3.push_back(std::sin(al) * std::sin(br));
+            pl->u3.push_back(-std::cos(al) * std::sin(br));
+            pl->u3.push_back(std::cos(br));
+            t1 += pl->tau;
+            t2 += pl->tau;
+        }
+    } else {
+        // static: Hamiltonian(0, dt) with wr = 0 -> rotor phase 0 (oneFID.h:297-299)
+        double br = pl->rotor_deg * DEG2RAD;
+        if (br == 0.0) {
+            pl->u3 = {0.0, 0.0, 1.0};
+        } else {
+            pl->u3 = {0.0, -std::sin(br), std::cos(br)};
+        }
+    }
+    return BLFID_OK;
+}
+
+static blfid_status upload_tables(blfid_plan *pl) {
+    HIPCHK(pl->d_terms.upload(pl->terms));
+    HIPCHK(pl->d_dconst.upload(pl->dconst));
+    HIPCHK(pl->d_ffconst.upload(pl->ffconst));
+    HIPCHK(pl->d_u3.upload(pl->u3));
+    HIPCHK(pl->d_blk_dim.upload(pl->blk_dim));
+    HIPCHK(pl->d_blk_soff.upload(pl->blk_soff));
+    HIPCHK(pl->d_blk_states.upload(pl->blk_states));
+    HIPCHK(pl->d_blk_uoff.upload(pl->blk_uoff));
+    for (auto &kv : pl->blk_groups) HIPCHK(pl->d_bgrp[kv.first].upload(kv.second));
+    return BLFID_OK;
+}
+
+static void refresh_devplan(blfid_plan *pl) {
+    DevPlan &d = pl->dp;
+    d.S = pl->S;
+    d.N = pl->N;
+    d.n = pl->n;
+    d.npts = pl->npts;
+    d.nterms = (int)(pl->terms.size() / DEV_TERM_STRIDE);
+    d.nblocks = (int)pl->blocks.size();
+    d.npairs = (int)pl->pairs.size();
+    d.Lsum = pl->Lsum;
+    d.Etot = pl->Etot;
+    d.theta = PI2 * pl->tau;
+    d.terms = pl->d_terms.p;
+    d.dconst = pl->d_dconst.p;
+    d.ffconst = pl->d_ffconst.p;
+    d.u3 = pl->d_u3.p;
+    d.RP = pl->d_RP.p;
+    d.blk_dim = pl->d_blk_dim.p;
+    d.blk_soff = pl->d_blk_soff.p;
+    d.blk_uoff = pl->d_blk_uoff.p;
+    d.blk_states = pl->d_blk_states.p;
+    d.pair_I = pl->d_pair_I.p;
+    d.pair_J = pl->d_pair_J.p;
+    d.pair_eoff = pl->d_pair_eoff.p;
+    d.pair_doff = pl->d_pair_doff.p;
+    d.pair_roff = pl->d_pair_roff.p;
+    d.d_rowptr = pl->d_drow.p;
+    d.d_col = pl->d_dcol.p;
+    d.d_val = pl->d_dval.p;
+    d.r_rowptr = pl->d_rrow.p;
+    d.r_col = pl->d_rcol.p;
+    d.r_val = pl->d_rval.p;
+}
+
+// ------------------------------------------------------------------------------ C ABI
+extern "C" {
+
+const char *blfid_last_error(void) { return g_err.c_str(); }
+const char *blfid_version(void) { return "blfid-mi355x 0.1 (gfx950, FP64)"; }
+
+blfid_status blfid_plan_create(const blfid_system *sys, double wr_hz, double rotor_deg,
+                               double sw_hz, int npts, double t0, blfid_precision prec,
+                               blfid_plan **out) {
+    if (!out) return fail(BLFID_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    if (!sys) return fail(BLFID_ERR_INVALID, "sys is NULL");
+    if (prec != BLFID_FP64) return fail(BLFID_ERR_UNSUPPORTED, "only BLFID_FP64 is implemented");
+    if (sys->nspins < 1 || sys->nspins > DEV_MAX_SPINS)
+        return fail(BLFID_ERR_INVALID, "nspins must be in [1, 12]");
+    if (npts <= 0)
+        return fail(BLFID_ERR_INVALID,
+                    "Fid size was '0' please use 'npts(int)' set set it (oneFID.h:78-83)");
+    if (sys->ncsa < 0 || sys->ndip < 0 || sys->nj < 0)
+        return fail(BLFID_ERR_INVALID, "negative interaction count");
+    const int S = sys->nspins;
+    for (int c = 0; c < sys->ncsa; c++) {
+        const double *p = sys->csa + 7 * c;
+        if (p[2] > 1.0 || p[2] < 0.0)
+            return fail(BLFID_ERR_INVALID, " 'eta' set to a negative or value >1...");
+        if ((int)p[3] < 0 || (int)p[3] >= S) return fail(BLFID_ERR_INVALID, "CSA spin out of range");
+    }
+    for (int d = 0; d < sys->ndip; d++) {
+        const double *p = sys->dip + 6 * d;
+        int i = (int)p[1], j = (int)p[2];
+        if (i < 0 || j < 0 || i >= S || j >= S || i == j)
+            return fail(BLFID_ERR_INVALID, "dipole spin index out of range");
+    }
+    for (int q = 0; q < sys->nj; q++) {
+        const double *p = sys->jc + 4 * q;
+        int i = (int)p[1], j = (int)p[2];
+        if (i < 0 || j < 0 || i >= S || j >= S || i == j)
+            return fail(BLFID_ERR_INVALID, "J spin index out of range");
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return fail(BLFID_ERR_DEVICE, "no HIP device available");
+    blfid_plan *pl = new blfid_plan();
+    pl->S = S;
+    pl->N = 1 << S;
+    pl->iso.assign(sys->isotope, sys->isotope + S);
+    pl->csa.assign(sys->csa, sys->csa + 7 * sys->ncsa);
+    pl->dip.assign(sys->dip, sys->dip + 6 * sys->ndip);
+    pl->jc.assign(sys->jc, sys->jc + 4 * sys->nj);
+    pl->wr = wr_hz;
+    pl->rotor_deg = rotor_deg;
+    pl->sw = sw_hz;
+    pl->t0 = t0;
+    pl->npts = npts;
+    pl->spinning = !(wr_hz == 0.0 || rotor_deg == 0.0);
+    if (pl->spinning) {
+        int n = (int)std::floor(sw_hz / wr_hz + 0.5);
+        if (n < 1) {
+            delete pl;
+            return fail(BLFID_ERR_INVALID, "sw/wr rounds to zero steps per rotor period");
+        }
+        pl->n = n;
+        pl->sw_adj = (double)(n * wr_hz);
+        double tmin = t0, tmax = t0 + 1.0 / wr_hz;
+        pl->tau = (tmax - tmin) / n;
+        if (pl->tau <= 0) {
+            delete pl;
+            return fail(BLFID_ERR_INVALID, " your time for the propgator is negative");
+        }
+    } else {
+        if (sw_hz <= 0) {
+            delete pl;
+            return fail(BLFID_ERR_INVALID, "sweep width must be positive");
+        }
+        pl->n = 1;
+        pl->sw_adj = sw_hz;
+        pl->tau = 1.0 / sw_hz;
+    }
+    blfid_status st = build_tables(pl);
+    if (st == BLFID_OK) st = upload_tables(pl);
+    if (st != BLFID_OK) {
+        delete pl;
+        return st;
+    }
+    refresh_devplan(pl);
+    *out = pl;
+    return BLFID_OK;
+}
+
+void blfid_plan_destroy(blfid_plan *pl) {
+    if (!pl) return;
+    if (pl->last_stream) (void)hipStreamSynchronize(pl->last_stream);
+    else (void)hipDeviceSynchronize();
+    delete pl; // DevBuf destructors free the device memory
+}
+
+double blfid_plan_sw(const blfid_plan *pl) { return pl ? pl->sw_adj : 0.0; }
+int blfid_plan_steps(const blfid_plan *pl) { return pl ? pl->n : 0; }
+int blfid_plan_dim(const blfid_plan *pl) { return pl ? pl->N : 0; }
+
+blfid_status blfid_plan_set_memory_budget(blfid_plan *pl, double bytes) {
+    if (!pl || bytes < 0) return fail(BLFID_ERR_INVALID, "bad plan or budget");
+    pl->budget = bytes;
+    return BLFID_OK;
+}
+
+blfid_status blfid_plan_set_operators(blfid_plan *pl, const double *rho_d, const double *det_d) {
+    if (!pl || !rho_d || !det_d) return fail(BLFID_ERR_INVALID, "NULL argument");
+    const int N = pl->N;
+    const cplx *rho = (const cplx *)rho_d, *det = (const cplx *)det_d;
+    auto R = [&](int i, int j) { return rho[i + (size_t)j * N]; };
+    auto D = [&](int i, int j) { return det[i + (size_t)j * N]; };
+    int rosym = 1;
+    for (int i = 0; i < N && rosym; i++)
+        for (int j = 0; j < N; j++)
+            if (R(i, j) != 0.5 * (D(i, j) + std::conj(D(j, i)))) {
+                rosym = 0;
+                break;
+            }
+    pl->rosym = rosym;
+    cplx tr = 0;
+    for (int i = 0; i < N; i++)
+        for (int j = 0; j < N; j++) tr += D(i, j) * std::conj(D(i, j));
+    pl->trdd = tr;
+    // rho_eff: det + det^H for the rosym spinning case (compute.h:381-382), rho otherwise
+    auto RE = [&](int i, int j) {
+        if (pl->spinning && rosym) return D(i, j) + std::conj(D(j, i));
+        return R(i, j);
+    };
+    // state -> (block, local index)
+    std::vector<int> sblk(N), sloc(N);
+    for (size_t b = 0; b < pl->blocks.size(); b++)
+        for (size_t a = 0; a < pl->blocks[b].states.size(); a++) {
+            sblk[pl->blocks[b].states[a]] = (int)b;
+            sloc[pl->blocks[b].states[a]] = (int)a;
+        }
+    const int nb = (int)pl->blocks.size();
+    std::vector<char> dnz(nb * nb, 0), rnz(nb * nb, 0);
+    for (int i = 0; i < N; i++)
+        for (int j = 0; j < N; j++) {
+            if (D(i, j) != 0.0) dnz[sblk[i] * nb + sblk[j]] = 1;
+            if (RE(i, j) != 0.0) rnz[sblk[i] * nb + sblk[j]] = 1;
+        }
+    pl->pairs.clear();
+    for (int I = 0; I < nb; I++)
+        for (int J = 0; J < nb; J++)
+            if (dnz[I * nb + J] && rnz[J * nb + I]) pl->pairs.push_back({I, J});
+    pl->pair_eoff.clear(); pl->pair_doff.clear(); pl->pair_roff.clear();
+    pl->d_rowptr.clear(); pl->d_col.clear(); pl->d_val.clear();
+    pl->r_rowptr.clear(); pl->r_col.clear(); pl->r_val.clear();
+    pl->elem_r.clear(); pl->elem_s.clear();
+    pl->pair_groups.clear();
+    long eo = 0;
+    for (size_t p = 0; p < pl->pairs.size(); p++) {
+        int I = pl->pairs[p].I, J = pl->pairs[p].J;
+        const auto &sI = pl->blocks[I].states, &sJ = pl->blocks[J].states;
+        int bI = (int)sI.size(), bJ = (int)sJ.size();
+        pl->pair_eoff.push_back(eo);
+        eo += (long)bI * bJ;
+        pl->pair_doff.push_back((int)pl->d_rowptr.size());
+        for (int a = 0; a < bI; a++) {
+            pl->d_rowptr.push_back((int)pl->d_col.size());
+            for (int b = 0; b < bJ; b++) {
+                cplx v = D(sI[a], sJ[b]);
+                if (v != 0.0) {
+                    pl->d_col.push_back(b);
+                    pl->d_val.push_back(v.real());
+                    pl->d_val.push_back(v.imag());
+                }
+            }
+        }
+        pl->d_rowptr.push_back((int)pl->d_col.size());
+        pl->pair_roff.push_back((int)pl->r_rowptr.size());
+        for (int b = 0; b < bJ; b++) {
+            pl->r_rowptr.push_back((int)pl->r_col.size());
+            for (int a = 0; a < bI; a++) {
+                cplx v = RE(sJ[b], sI[a]);
+                if (v != 0.0) {
+                    pl->r_col.push_back(a);
+                    pl->r_val.push_back(v.real());
+                    pl->r_val.push_back(v.imag());
+                }
+            }
+        }
+        pl->r_rowptr.push_back((int)pl->r_col.size());
+        for (int s = 0; s < bJ; s++)
+            for (int r = 0; r < bI; r++) {
+                pl->elem_r.push_back(pl->blk_soff[I] + r);
+                pl->elem_s.push_back(pl->blk_soff[J] + s);
+            }
+        int M = std::min(5, (std::max(bI, bJ) + 15) / 16);
+        pl->pair_groups[M].push_back((int)p);
+    }
+    pl->Etot = eo;
+    std::vector<int> pI, pJ;
+    for (auto &pp : pl->pairs) {
+        pI.push_back(pp.I);
+        pJ.push_back(pp.J);
+    }
+    HIPCHK(pl->d_pair_I.upload(pI));
+    HIPCHK(pl->d_pair_J.upload(pJ));
+    HIPCHK(pl->d_pair_eoff.upload(pl->pair_eoff));
+    HIPCHK(pl->d_pair_doff.upload(pl->pair_doff));
+    HIPCHK(pl->d_pair_roff.upload(pl->pair_roff));
+    HIPCHK(pl->d_drow.upload(pl->d_rowptr));
+    HIPCHK(pl->d_dcol.upload(pl->d_col));
+    HIPCHK(pl->d_dval.upload(pl->d_val));
+    HIPCHK(pl->d_rrow.upload(pl->r_rowptr));
+    HIPCHK(pl->d_rcol.upload(pl->r_col));
+    HIPCHK(pl->d_rval.upload(pl->r_val));
+    HIPCHK(pl->d_er.upload(pl->elem_r));
+    HIPCHK(pl->d_es.upload(pl->elem_s));
+    for (auto &kv : pl->pair_groups) HIPCHK(pl->d_pgrp[kv.first].upload(kv.second));
+    pl->have_ops = true;
+    refresh_devplan(pl);
+    return BLFID_OK;
+}
+
+blfid_status blfid_plan_set_powder(blfid_plan *pl, const double *theta, const double *phi,
+                                   const double *gamma, const double *weight, int K) {
+    if (!pl || K < 0 || (K > 0 && (!theta || !phi || !gamma || !weight)))
+        return fail(BLFID_ERR_INVALID, "bad powder arguments");
+    pl->K = K;
+    if (K == 0) return BLFID_OK;
+    std::vector<double> th(theta, theta + K), ph(phi, phi + K), ga(gamma, gamma + K),
+        w(weight, weight + K);
+    HIPCHK(pl->d_th.upload(th));
+    HIPCHK(pl->d_ph.upload(ph));
+    HIPCHK(pl->d_ga.upload(ga));
+    HIPCHK(pl->d_w.upload(w));
+    HIPCHK(pl->d_RP.alloc((size_t)9 * K));
+    LAUNCHCHK(blfid_dev_launch_prep(K, pl->d_th.p, pl->d_ph.p, pl->d_ga.p, pl->d_RP.p, nullptr));
+    HIPCHK(hipDeviceSynchronize());
+    refresh_devplan(pl);
+    return BLFID_OK;
+}
+
+static const int EXP_SLOTS = 2048;
+static const int QR_SLOTS = 2048;
+
+static double bytes_per_crystallite(const blfid_plan *pl) {
+    double b = 0;
+    if (pl->spinning) {
+        b += 2.0 * pl->n * pl->Lsum * 16.0;  // U/B and A
+        b += 2.0 * pl->n * pl->Etot * 16.0;  // Qt, Rt
+        b += pl->Lsum * 16.0 + pl->N * 8.0;  // W, thetas
+    } else {
+        b += 2.0 * pl->Lsum * 16.0 + pl->N * 8.0;
+    }
+    b += pl->npts * 16.0;
+    return b;
+}
+
+blfid_status blfid_plan_stats(const blfid_plan *pl, double *out) {
+    if (!pl || !out) return fail(BLFID_ERR_INVALID, "NULL argument");
+    double N = pl->N, n = pl->n, P = pl->npts;
+    // executed FP64 flops per crystallite-step (unpadded block sizes; complex MAC = 8)
+    double s3 = 0, pairs = 0, pairsQ = 0;
+    for (int d : pl->blk_dim) s3 += (double)d * d * d;
+    for (auto &pp : pl->pairs) {
+        double bI = pl->blk_dim[pp.I], bJ = pl->blk_dim[pp.J];
+        pairs += bI * bJ;
+        pairsQ += bI * bJ * (bI + bJ);
+    }
+    double ex = 0;
+    if (pl->spinning) {
+        ex += 2.0 * 14.0 * s3;  // exp: 14 real GEMMs (no squaring counted)
+        ex += 8.0 * s3;         // Dyson
+        ex += 8.0 * s3;         // B = A W
+        ex += 8.0 * pairsQ;     // Q and R
+        ex += 8.0 * n * pairs;  // correlation
+        ex += 8.0 * P / n * pairs; // FID
+    }
+    out[0] = ex;
+    out[1] = 48.0 * N * N * N + 14.0 * n * N * N + 14.0 * P * N * N / n;
+    out[2] = (double)pl->blocks.size();
+    out[3] = (double)pl->bmax;
+    out[4] = bytes_per_crystallite(pl);
+    return BLFID_OK;
+}
+
+static blfid_status ensure_buffers(blfid_plan *pl, int Kb) {
+    const size_t L = (size_t)pl->Lsum;
+    const size_t bm2 = (size_t)pl->bmax * pl->bmax;
+    if (pl->spinning) {
+        HIPCHK(pl->bufU.alloc((size_t)Kb * pl->n * L));
+        HIPCHK(pl->bufA.alloc((size_t)Kb * pl->n * L));
+        HIPCHK(pl->bufQt.alloc(std::max<size_t>(1, (size_t)Kb * pl->n * pl->Etot)));
+        HIPCHK(pl->bufRt.alloc(std::max<size_t>(1, (size_t)Kb * pl->n * pl->Etot)));
+        HIPCHK(pl->scrE.alloc((size_t)EXP_SLOTS * 5 * bm2));
+    } else {
+        HIPCHK(pl->bufU.alloc((size_t)Kb * L));
+        HIPCHK(pl->scrE.alloc((size_t)EXP_SLOTS * bm2));
+    }
+    HIPCHK(pl->bufW.alloc((size_t)Kb * L));
+    HIPCHK(pl->thetas.alloc((size_t)Kb * pl->N));
+    HIPCHK(pl->scrQ.alloc((size_t)QR_SLOTS * 3 * bm2));
+    HIPCHK(pl->fidc.alloc((size_t)Kb * 2 * pl->npts));
+    HIPCHK(pl->acc.alloc((size_t)2 * pl->npts));
+    HIPCHK(pl->comp.alloc((size_t)2 * pl->npts));
+    HIPCHK(pl->fid_out.alloc((size_t)2 * pl->npts));
+    return BLFID_OK;
+}
+
+blfid_status blfid_plan_run(blfid_plan *pl, int normalize, void *stream) {
+    if (!pl) return fail(BLFID_ERR_INVALID, "plan is NULL");
+    if (!pl->have_ops) return fail(BLFID_ERR_INVALID, "operators not set");
+    hipStream_t st = (hipStream_t)stream;
+    pl->last_stream = st;
+    const int K = pl->K;
+    // batch size from the memory budget
+    double budget = pl->budget;
+    if (budget <= 0) {
+        size_t fre = 0, tot = 0;
+        HIPCHK(hipMemGetInfo(&fre, &tot));
+        budget = 0.4 * (double)fre;
+    }
+    double per = bytes_per_crystallite(pl);
+    int Kb = (int)std::min<double>(std::max(1, K), std::floor(budget / per));
+    if (Kb < 1) return fail(BLFID_ERR_NOMEM, "one crystallite does not fit the memory budget");
+    if (pl->Kb != Kb) {
+        pl->bufU.release(); pl->bufA.release(); pl->bufQt.release(); pl->bufRt.release();
+        pl->bufW.release(); pl->thetas.release(); pl->fidc.release();
+        pl->Kb = Kb;
+    }
+    blfid_status s = ensure_buffers(pl, Kb);
+    if (s != BLFID_OK) return s;
+    refresh_devplan(pl);
+    const DevPlan *P = &pl->dp;
+    LAUNCHCHK(blfid_dev_launch_zero(pl->acc.p, 2 * pl->npts, st));
+    LAUNCHCHK(blfid_dev_launch_zero(pl->comp.p, 2 * pl->npts, st));
+    const size_t bm2 = (size_t)pl->bmax * pl->bmax;
+    const int ff = pl->rosym == 1 ? 2 : 1;
+    const double scale = pl->spinning ? 1.0 / (double)(pl->n / ff) : 1.0;
+    for (int c0 = 0; c0 < K; c0 += Kb) {
+        int kb = std::min(Kb, K - c0);
+        LAUNCHCHK(blfid_dev_launch_zero(pl->fidc.p, (size_t)kb * 2 * pl->npts, st));
+        if (pl->spinning) {
+            for (auto &kv : pl->blk_groups)
+                LAUNCHCHK(blfid_dev_launch_exp(kv.first, P, kb, c0, pl->d_bgrp[kv.first].p,
+                                               (int)kv.second.size(), pl->scrE.p, bm2, EXP_SLOTS,
+                                               pl->bufU.p, st));
+            for (auto &kv : pl->blk_groups)
+                LAUNCHCHK(blfid_dev_launch_dyson(kv.first, P, kb, pl->d_bgrp[kv.first].p,
+                                                 (int)kv.second.size(), pl->bufU.p, pl->bufA.p, st));
+            for (auto &kv : pl->blk_groups)
+                LAUNCHCHK(blfid_dev_launch_eig(P, kb, pl->d_bgrp[kv.first].p, (int)kv.second.size(),
+                                               pl->bufA.p + (size_t)(pl->n - 1) * pl->Lsum,
+                                               (size_t)pl->n * pl->Lsum, 0, pl->bufW.p,
+                                               pl->thetas.p, st));
+            for (auto &kv : pl->blk_groups)
+                LAUNCHCHK(blfid_dev_launch_bmul(kv.first, P, kb, pl->d_bgrp[kv.first].p,
+                                                (int)kv.second.size(), pl->bufA.p, pl->bufW.p,
+                                                pl->bufU.p, st));
+            for (auto &kv : pl->pair_groups)
+                LAUNCHCHK(blfid_dev_launch_qr(kv.first, P, kb, pl->d_pgrp[kv.first].p,
+                                              (int)kv.second.size(), pl->bufU.p, pl->thetas.p,
+                                              pl->scrQ.p, bm2, QR_SLOTS, pl->bufQt.p, pl->bufRt.p,
+                                              st));
+            if (pl->Etot > 0)
+                LAUNCHCHK(blfid_dev_launch_corr(P, kb, pl->bufQt.p, pl->bufRt.p, pl->thetas.p,
+                                                pl->d_er.p, pl->d_es.p, pl->fidc.p, st));
+        } else {
+            for (auto &kv : pl->blk_groups)
+                LAUNCHCHK(blfid_dev_launch_static_h(P, kb, c0, pl->d_bgrp[kv.first].p,
+                                                    (int)kv.second.size(), pl->scrE.p, bm2,
+                                                    EXP_SLOTS, pl->bufU.p, st));
+            for (auto &kv : pl->blk_groups)
+                LAUNCHCHK(blfid_dev_launch_eig(P, kb, pl->d_bgrp[kv.first].p, (int)kv.second.size(),
+                                               pl->bufU.p, (size_t)pl->Lsum, 1, pl->bufW.p,
+                                               pl->thetas.p, st));
+            for (auto &kv : pl->pair_groups)
+                LAUNCHCHK(blfid_dev_launch_static_fid(kv.first, P, kb, pl->d_pgrp[kv.first].p,
+                                                      (int)kv.second.size(), pl->bufW.p,
+                                                      pl->thetas.p, pl->scrQ.p, bm2, QR_SLOTS,
+                                                      1.0 / pl->sw, pl->fidc.p, st));
+        }
+        LAUNCHCHK(blfid_dev_launch_reduce(kb, c0, pl->npts, pl->d_w.p, scale, pl->fidc.p,
+                                          pl->acc.p, pl->comp.p, st));
+    }
+    cplx norm = normalize ? pl->trdd : cplx(1.0, 0.0);
+    if (normalize && norm == 0.0) return fail(BLFID_ERR_INVALID, "Tr(det det^H) is zero");
+    LAUNCHCHK(blfid_dev_launch_finish(pl->npts, pl->acc.p, norm.real(), norm.imag(),
+                                      pl->fid_out.p, st));
+    pl->ran = true;
+    return BLFID_OK;
+}
+
+blfid_status blfid_plan_fetch(blfid_plan *pl, double *out) {
+    if (!pl || !out) return fail(BLFID_ERR_INVALID, "NULL argument");
+    if (!pl->ran) return fail(BLFID_ERR_INVALID, "plan has not been run");
+    HIPCHK(hipStreamSynchronize(pl->last_stream));
+    HIPCHK(hipMemcpy(out, pl->fid_out.p, sizeof(double) * 2 * pl->npts, hipMemcpyDeviceToHost));
+    return BLFID_OK;
+}
+
+const double *blfid_plan_device_fid(const blfid_plan *pl) { return pl ? pl->fid_out.p : nullptr; }
+
+blfid_status blfid_powder_fid(blfid_plan *pl, const double *theta, const double *phi,
+                              const double *gamma, const double *weight, int K,
+                              const double *rho, const double *det, int normalize,
+                              double *fid_out) {
+    blfid_status s = blfid_plan_set_operators(pl, rho, det);
+    if (s != BLFID_OK) return s;
+    s = blfid_plan_set_powder(pl, theta, phi, gamma, weight, K);
+    if (s != BLFID_OK) return s;
+    s = blfid_plan_run(pl, normalize, nullptr);
+    if (s != BLFID_OK) return s;
+    return blfid_plan_fetch(pl, fid_out);
+}
+
+} // extern "C"

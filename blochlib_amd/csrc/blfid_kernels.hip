@@ -1,0 +1,918 @@
+// blfid_kernels.hip -- gfx950 kernels of the powder-averaged MAS FID engine (FP64).
+//
+// The work of one spin system is organised by the block structure of its Hamiltonian: every
+// SolidSys term (CSA, secular dipolar, J; src/QMspins/sys.cc:749-770) commutes with the
+// Zeeman structure that the host finds (connected components of the flip-flop graph), so all
+// propagators are block diagonal and every matrix below is one block (dimension b <= N).
+//
+// Pipeline for a batch of crystallites (host: blfid_host.cpp):
+//   k_prep      rotation of each crystallite (cartPowder, space_ten.cc:578-618)
+//   k_exp       H_k block at the step midpoint (sys.cc:749-770) and U_k = exp(-i 2 pi tau H_k)
+//               by scaling-and-squaring of the degree-15 Taylor cos/sin pair, real FP64
+//               (replaces Mexp's eigen route, matexp.h:239-252; H is real symmetric)
+//   k_dyson     A_k = U_k A_{k-1} along the rotor period (compute.h:257-274)
+//   k_eig       U_T = A_n = W diag(lambda) W^H by joint Jacobi on (U+U^H)/2, (U-U^H)/2i
+//               (replaces diag(Uf) -> corth/comqr3, compute.h:317)
+//   k_bmul      B_k = A_k W
+//   k_qr        Q_k = B^H det B and R_k = B^H rho_eff B on the coupled block pairs, with the
+//               gamma-COMPUTE twiddles z^-j / z^i folded in (compute.h:378-427)
+//   k_corr      F_p = sum_i Qt_{(i+p)%n} Rt_i (cyclic correlation) and
+//               fid[t] += sum_rs F_{t%n}(r,s) z_rs^t (compute.h:402-457)
+//   k_reduce    FID += w_c * fid_c / (n/ff) [/ Tr(D D^H)], Kahan-compensated, in crystallite
+//               order (oneFID.h:396-407, sequencerun_funcs.cc:698-702)
+// and, for wr == 0 or rotor == 0, the static FID (oneFID.h:292-336):
+//   k_static_h  H(0, dt) block;  k_eig (Hermitian case);  k_static_fid.
+//
+// All matrices are column-major, complex values are double2 (re, im).
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "blfid_dev.h"
+
+typedef double2 cd;
+
+// ------------------------------------------------------------------------------ helpers
+__device__ __forceinline__ cd cmk(double a, double b) { return make_double2(a, b); }
+__device__ __forceinline__ cd cadd(cd a, cd b) { return cmk(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ cd csub(cd a, cd b) { return cmk(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ cd cmul(cd a, cd b) {
+    return cmk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ cd cconj(cd a) { return cmk(a.x, -a.y); }
+__device__ __forceinline__ cd cscale(cd a, double s) { return cmk(a.x * s, a.y * s); }
+__device__ __forceinline__ void cfma(cd &c, cd a, cd b) {
+    c.x = fma(a.x, b.x, c.x);
+    c.x = fma(-a.y, b.y, c.x);
+    c.y = fma(a.x, b.y, c.y);
+    c.y = fma(a.y, b.x, c.y);
+}
+__device__ __forceinline__ cd cexpi(double ph) {
+    double s, c;
+    sincos(ph, &s, &c);
+    return cmk(c, s);
+}
+
+template <typename T> struct Arith;
+template <> struct Arith<double> {
+    static __device__ __forceinline__ double zero() { return 0.0; }
+    static __device__ __forceinline__ double cj(double a) { return a; }
+    static __device__ __forceinline__ void mac(double &c, double a, double b) { c = fma(a, b, c); }
+    static __device__ __forceinline__ double add(double a, double b) { return a + b; }
+};
+template <> struct Arith<cd> {
+    static __device__ __forceinline__ cd zero() { return cmk(0.0, 0.0); }
+    static __device__ __forceinline__ cd cj(cd a) { return cconj(a); }
+    static __device__ __forceinline__ void mac(cd &c, cd a, cd b) { cfma(c, a, b); }
+    static __device__ __forceinline__ cd add(cd a, cd b) { return cadd(a, b); }
+};
+
+// ---------------------------------------------------------------- workgroup GEMM (256 thr)
+// C[m x n] = op(A)[m x k] * op(B)[k x n] (+ diag * I); op = N or conjugate transpose.
+// Output tiles of 16M x 16M, thread (tx,ty) of a 16x16 grid owns rows tx+16i, cols ty+16j.
+// K is walked in slices of 16 staged through LDS with zero fill, so unpadded sizes work.
+// C must not alias A or B.  `smem` holds 2 * 16 * 16M elements of T.
+template <int M, typename T>
+__device__ void wg_gemm(int m, int n, int k, const T *__restrict__ A, int lda, bool ca,
+                        const T *__restrict__ B, int ldb, bool cb, T *__restrict__ C, int ldc,
+                        T diag, T *smem) {
+    constexpr int TM = 16 * M, KS = 16;
+    T *As = smem;
+    T *Bs = smem + KS * TM;
+    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+    for (int i0 = 0; i0 < m; i0 += TM) {
+        for (int j0 = 0; j0 < n; j0 += TM) {
+            T acc[M][M];
+#pragma unroll
+            for (int i = 0; i < M; i++)
+#pragma unroll
+                for (int j = 0; j < M; j++) acc[i][j] = Arith<T>::zero();
+            for (int k0 = 0; k0 < k; k0 += KS) {
+                __syncthreads();
+                for (int idx = tid; idx < TM * KS; idx += 256) {
+                    int r, q;
+                    if (!ca) { r = idx % TM; q = idx / TM; }
+                    else { q = idx % KS; r = idx / KS; }
+                    int gr = i0 + r, gq = k0 + q;
+                    T v = Arith<T>::zero();
+                    if (gr < m && gq < k)
+                        v = ca ? Arith<T>::cj(A[gq + (size_t)gr * lda]) : A[gr + (size_t)gq * lda];
+                    As[q * TM + r] = v;
+                }
+                for (int idx = tid; idx < TM * KS; idx += 256) {
+                    int c, q;
+                    if (!cb) { q = idx % KS; c = idx / KS; }
+                    else { c = idx % TM; q = idx / TM; }
+                    int gc = j0 + c, gq = k0 + q;
+                    T v = Arith<T>::zero();
+                    if (gc < n && gq < k)
+                        v = cb ? Arith<T>::cj(B[gc + (size_t)gq * ldb]) : B[gq + (size_t)gc * ldb];
+                    Bs[q * TM + c] = v;
+                }
+                __syncthreads();
+#pragma unroll 4
+                for (int q = 0; q < KS; q++) {
+                    T a[M], b[M];
+#pragma unroll
+                    for (int i = 0; i < M; i++) a[i] = As[q * TM + tx + 16 * i];
+#pragma unroll
+                    for (int j = 0; j < M; j++) b[j] = Bs[q * TM + ty + 16 * j];
+#pragma unroll
+                    for (int i = 0; i < M; i++)
+#pragma unroll
+                        for (int j = 0; j < M; j++) Arith<T>::mac(acc[i][j], a[i], b[j]);
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < M; i++)
+#pragma unroll
+                for (int j = 0; j < M; j++) {
+                    int r = i0 + tx + 16 * i, c = j0 + ty + 16 * j;
+                    if (r < m && c < n) {
+                        T v = acc[i][j];
+                        if (r == c) v = Arith<T>::add(v, diag);
+                        C[r + (size_t)c * ldc] = v;
+                    }
+                }
+        }
+    }
+    __syncthreads();
+}
+
+// ------------------------------------------------------------------------------ k_prep
+// cartPowder = rotationMatrix3D(phi, theta, gamma) (space_ten.cc:578-618, set by
+// SolidSys::setPowderAngles(th, ph, ga) -> Rotations::setPowderAngles(ph, th, ga)).
+__global__ void k_prep(int K, const double *__restrict__ th, const double *__restrict__ ph,
+                       const double *__restrict__ ga, double *__restrict__ RP) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= K) return;
+    double a = ph[c], b = th[c], g = ga[c];
+    double sa, ca, sb, cb, sg, cg;
+    sincos(a, &sa, &ca);
+    sincos(b, &sb, &cb);
+    sincos(g, &sg, &cg);
+    double *R = RP + 9 * (size_t)c;
+    R[0] = ca * cb * cg - sa * sg; R[1] = sa * cb * cg + ca * sg; R[2] = -sb * cg;
+    R[3] = -ca * cb * sg - sa * cg; R[4] = -sa * cb * sg + ca * cg; R[5] = sb * sg;
+    R[6] = ca * sb; R[7] = sa * sb; R[8] = cb;
+}
+
+// ------------------------------------------------------------------- Hamiltonian block build
+// zz of every anisotropic term for lab direction v = R_P^T u, u = row 2 of cartSpin
+// (csa.cc:316-319, dip.cc:196-199): zz_t = v^T C_t v, C_t in the crystal frame.
+__device__ void build_h_block(const DevPlan &P, const double *RPc, const double *u, int blk,
+                              double *H, int ld, double *s_zz, double *s_ff, double *s_v) {
+    const int tid = threadIdx.x;
+    if (tid == 0) {
+        for (int a = 0; a < 3; a++)
+            s_v[a] = RPc[0 * 3 + a] * u[0] + RPc[1 * 3 + a] * u[1] + RPc[2 * 3 + a] * u[2];
+    }
+    __syncthreads();
+    const double vx = s_v[0], vy = s_v[1], vz = s_v[2];
+    for (int t = tid; t < P.nterms; t += blockDim.x) {
+        const double *c = P.terms + DEV_TERM_STRIDE * t;
+        s_zz[t] = c[0] * vx * vx + c[1] * vy * vy + c[2] * vz * vz +
+                  2.0 * (c[3] * vx * vy + c[4] * vx * vz + c[5] * vy * vz);
+    }
+    for (int q = tid; q < P.S * P.S; q += blockDim.x) s_ff[q] = P.ffconst[q];
+    __syncthreads();
+    if (tid == 0) {
+        for (int t = 0; t < P.nterms; t++) {
+            const double *c = P.terms + DEV_TERM_STRIDE * t;
+            if (c[7] != 0.0) {
+                int i = (int)c[8], j = (int)c[9];
+                s_ff[i * P.S + j] += s_zz[t] * c[7];
+                s_ff[j * P.S + i] += s_zz[t] * c[7];
+            }
+        }
+    }
+    __syncthreads();
+    const int b = P.blk_dim[blk];
+    const int *st = P.blk_states + P.blk_soff[blk];
+    for (int e = tid; e < b * b; e += blockDim.x) {
+        int ra = e % b, cb_ = e / b;
+        int ga = st[ra], gb = st[cb_];
+        double h = 0.0;
+        if (ra == cb_) {
+            h = P.dconst[ga];
+            for (int t = 0; t < P.nterms; t++) {
+                const double *c = P.terms + DEV_TERM_STRIDE * t;
+                int i = (int)c[8], j = (int)c[9];
+                double mi = ((ga >> (P.S - 1 - i)) & 1) ? -0.5 : 0.5;
+                double f = c[6] * mi;
+                if (j >= 0) f *= ((ga >> (P.S - 1 - j)) & 1) ? -0.5 : 0.5;
+                h += s_zz[t] * f;
+            }
+        } else {
+            int x = ga ^ gb;
+            if (__popc(x) == 2 && (ga & x) != 0 && (ga & x) != x) {
+                int p1 = 31 - __clz(x);
+                int p0 = __ffs(x) - 1;
+                int i = P.S - 1 - p1, j = P.S - 1 - p0;
+                h = s_ff[i * P.S + j];
+            }
+        }
+        H[ra + (size_t)cb_ * ld] = h;
+    }
+    __syncthreads();
+}
+
+// ------------------------------------------------------------------------------ k_exp
+// One task = (crystallite c, step k, block) of a block-size group.  U = exp(-i theta H),
+// theta = 2 pi tau, as cos(X) - i sin(X) with X = theta H / 2^s, ||X||_inf <= 1/2,
+// cos and sin/x by Horner in Y = X^2 to degree 7 (Taylor error < 1e-18), then s squarings
+// (C,S) -> (C^2 - S^2, 2 C S).
+template <int M>
+__global__ void __launch_bounds__(256) k_exp(DevPlan P, int Kb, int c0, const int *__restrict__ grp,
+                                             int ngrp, double *__restrict__ scratch, size_t slot,
+                                             cd *__restrict__ bufU) {
+    __shared__ double smem[2 * 16 * 16 * M];
+    __shared__ double s_zz[DEV_MAX_TERMS];
+    __shared__ double s_ff[DEV_MAX_SPINS * DEV_MAX_SPINS];
+    __shared__ double s_v[4];
+    __shared__ double s_red[256];
+    const int tid = threadIdx.x;
+    double *X = scratch + slot * 5 * blockIdx.x;
+    double *Y = X + slot, *Rc = Y + slot, *Rs = Rc + slot, *T = Rs + slot;
+    const long ntask = (long)Kb * P.n * ngrp;
+    const double theta = P.theta;
+    for (long task = blockIdx.x; task < ntask; task += gridDim.x) {
+        const int g = (int)(task % ngrp);
+        const long ck = task / ngrp;
+        const int k = (int)(ck % P.n), c = (int)(ck / P.n);
+        const int blk = grp[g];
+        const int b = P.blk_dim[blk];
+        build_h_block(P, P.RP + 9 * (size_t)(c0 + c), P.u3 + 3 * k, blk, X, b, s_zz, s_ff, s_v);
+        // inf-norm
+        double rmax = 0.0;
+        for (int r = tid; r < b; r += 256) {
+            double s = 0.0;
+            for (int q = 0; q < b; q++) s += fabs(X[r + (size_t)q * b]);
+            rmax = fmax(rmax, s);
+        }
+        s_red[tid] = rmax;
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if (tid < w) s_red[tid] = fmax(s_red[tid], s_red[tid + w]);
+            __syncthreads();
+        }
+        const double nrm = s_red[0] * theta;
+        __syncthreads();
+        int sq = 0;
+        while (nrm / (double)(1 << sq) > 0.5 && sq < 30) sq++;
+        const double sc = theta / (double)(1 << sq);
+        for (int e = tid; e < b * b; e += 256) X[e] *= sc;
+        __syncthreads();
+        wg_gemm<M, double>(b, b, b, X, b, false, X, b, false, Y, b, 0.0, smem);
+        // cos: c_j = (-1)^j / (2j)!,  sinc: s_j = (-1)^j / (2j+1)!
+        const double cc[8] = {1.0, -1.0 / 2, 1.0 / 24, -1.0 / 720, 1.0 / 40320, -1.0 / 3628800,
+                              1.0 / 479001600, -1.0 / 87178291200.0};
+        const double ss[8] = {1.0, -1.0 / 6, 1.0 / 120, -1.0 / 5040, 1.0 / 362880,
+                              -1.0 / 39916800, 1.0 / 6227020800.0, -1.0 / 1307674368000.0};
+        for (int e = tid; e < b * b; e += 256) {
+            int r = e % b, q = e / b;
+            double y = Y[e], d = (r == q) ? 1.0 : 0.0;
+            Rc[e] = cc[7] * y + cc[6] * d;
+            Rs[e] = ss[7] * y + ss[6] * d;
+        }
+        __syncthreads();
+        for (int j = 5; j >= 0; j--) {
+            wg_gemm<M, double>(b, b, b, Rc, b, false, Y, b, false, T, b, cc[j], smem);
+            double *tmp = Rc; Rc = T; T = tmp;
+        }
+        for (int j = 5; j >= 0; j--) {
+            wg_gemm<M, double>(b, b, b, Rs, b, false, Y, b, false, T, b, ss[j], smem);
+            double *tmp = Rs; Rs = T; T = tmp;
+        }
+        // S = X * Rs -> T
+        wg_gemm<M, double>(b, b, b, X, b, false, Rs, b, false, T, b, 0.0, smem);
+        double *Cm = Rc, *Sm = T;
+        double *f1 = X, *f2 = Y, *f3 = Rs;
+        for (int it = 0; it < sq; it++) {
+            wg_gemm<M, double>(b, b, b, Cm, b, false, Cm, b, false, f1, b, 0.0, smem);
+            wg_gemm<M, double>(b, b, b, Sm, b, false, Sm, b, false, f2, b, 0.0, smem);
+            wg_gemm<M, double>(b, b, b, Cm, b, false, Sm, b, false, f3, b, 0.0, smem);
+            for (int e = tid; e < b * b; e += 256) {
+                Cm[e] = f1[e] - f2[e];
+                Sm[e] = 2.0 * f3[e];
+            }
+            __syncthreads();
+        }
+        cd *U = bufU + ((size_t)c * P.n + k) * P.Lsum + P.blk_uoff[blk];
+        for (int e = tid; e < b * b; e += 256) U[e] = cmk(Cm[e], -Sm[e]);
+        // restore the scratch pointer order for the next task
+        X = scratch + slot * 5 * blockIdx.x;
+        Y = X + slot; Rc = Y + slot; Rs = Rc + slot; T = Rs + slot;
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------------------ k_dyson
+// One task = (crystallite, block) chain: A_1 = U_1, A_{k+1} = U_{k+1} A_k (compute.h:267).
+template <int M>
+__global__ void __launch_bounds__(256) k_dyson(DevPlan P, int Kb, const int *__restrict__ grp,
+                                               int ngrp, const cd *__restrict__ bufU,
+                                               cd *__restrict__ bufA) {
+    __shared__ cd smem[2 * 16 * 16 * M];
+    const long ntask = (long)Kb * ngrp;
+    for (long task = blockIdx.x; task < ntask; task += gridDim.x) {
+        const int g = (int)(task % ngrp), c = (int)(task / ngrp);
+        const int blk = grp[g], b = P.blk_dim[blk];
+        const size_t base = (size_t)c * P.n * P.Lsum + P.blk_uoff[blk];
+        const cd *U0 = bufU + base;
+        cd *A0 = bufA + base;
+        for (int e = threadIdx.x; e < b * b; e += 256) A0[e] = U0[e];
+        __syncthreads();
+        for (int k = 1; k < P.n; k++) {
+            const cd *Uk = bufU + base + (size_t)k * P.Lsum;
+            const cd *Ap = bufA + base + (size_t)(k - 1) * P.Lsum;
+            cd *Ak = bufA + base + (size_t)k * P.Lsum;
+            wg_gemm<M, cd>(b, b, b, Uk, b, false, Ap, b, false, Ak, b, cmk(0, 0), smem);
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------ k_eig
+// Joint Jacobi diagonalisation of a normal matrix Mx = H1 + i H2 (H1, H2 Hermitian,
+// commuting): each 2x2 rotation is the Cardoso-Souloumiac angle for the pair (H1, H2)
+// (SIAM J. Matrix Anal. Appl. 17 (1996) 161), pairs in round-robin order, b/2 rotations per
+// round.  Mx lives in LDS; W accumulates in global memory.
+// mode 0: Mx = A_n block (unitary) -> lambda_r = Mx_rr, out phase = arg(lambda_r)
+// mode 1: Mx = H block (Hermitian, static path) -> out value = Re(Mx_rr)
+#define EIG_MAXB 80
+__global__ void __launch_bounds__(256) k_eig(DevPlan P, int Kb, const int *__restrict__ grp,
+                                             int ngrp, const cd *__restrict__ src,
+                                             size_t src_stride, int mode, cd *__restrict__ bufW,
+                                             double *__restrict__ outv) {
+    __shared__ cd Ms[EIG_MAXB * (EIG_MAXB + 1)];
+    __shared__ double rc[EIG_MAXB / 2];
+    __shared__ cd rs[EIG_MAXB / 2];
+    __shared__ int rp[EIG_MAXB / 2], rq[EIG_MAXB / 2];
+    __shared__ int s_any;
+    const int tid = threadIdx.x;
+    const long ntask = (long)Kb * ngrp;
+    for (long task = blockIdx.x; task < ntask; task += gridDim.x) {
+        const int g = (int)(task % ngrp), c = (int)(task / ngrp);
+        const int blk = grp[g], b = P.blk_dim[blk];
+        const int ld = b + 1;
+        const cd *S0 = src + (size_t)c * src_stride + P.blk_uoff[blk];
+        cd *W = bufW + (size_t)c * P.Lsum + P.blk_uoff[blk];
+        for (int e = tid; e < b * b; e += 256) {
+            int r = e % b, q = e / b;
+            Ms[r + q * ld] = S0[e];
+            W[e] = cmk(r == q ? 1.0 : 0.0, 0.0);
+        }
+        __syncthreads();
+        const int bb = (b + 1) & ~1; // even count, index b is a dummy when b is odd
+        const int npair = bb / 2;
+        for (int sweep = 0; sweep < 40 && b > 1; sweep++) {
+            if (tid == 0) s_any = 0;
+            __syncthreads();
+            for (int round = 0; round < bb - 1; round++) {
+                if (tid < npair) {
+                    // circle method: position 0 fixed, positions 1..bb-1 rotate
+                    int pa = tid, pb = bb - 1 - tid;
+                    int ia = (pa == 0) ? 0 : ((pa - 1 + round) % (bb - 1)) + 1;
+                    int ib = (pb == 0) ? 0 : ((pb - 1 + round) % (bb - 1)) + 1;
+                    int p = min(ia, ib), q = max(ia, ib);
+                    double cth = 1.0;
+                    cd sth = cmk(0, 0);
+                    if (q < b) {
+                        cd mpp = Ms[p + p * ld], mqq = Ms[q + q * ld];
+                        cd mpq = Ms[p + q * ld], mqp = Ms[q + p * ld];
+                        // H1 = (M + M^H)/2, H2 = (M - M^H)/(2i)
+                        cd h1pq = cscale(cadd(mpq, cconj(mqp)), 0.5);
+                        cd d2 = csub(mpq, cconj(mqp));           // = 2i H2pq
+                        cd h2pq = cmk(0.5 * d2.y, -0.5 * d2.x); // d2 / (2i)
+                        double off = fabs(h1pq.x) + fabs(h1pq.y) + fabs(h2pq.x) + fabs(h2pq.y);
+                        double scale = fabs(mpp.x) + fabs(mpp.y) + fabs(mqq.x) + fabs(mqq.y) + 1e-300;
+                        if (off > 1e-17 * scale) {
+                            double h1[3] = {mpp.x - mqq.x, 2.0 * h1pq.x, 2.0 * h1pq.y};
+                            double h2[3] = {mpp.y - mqq.y, 2.0 * h2pq.x, 2.0 * h2pq.y};
+                            double ka = h1[0] * h1[0] + h1[1] * h1[1] + h1[2] * h1[2];
+                            double kd = h2[0] * h2[0] + h2[1] * h2[1] + h2[2] * h2[2];
+                            double kb = h1[0] * h2[0] + h1[1] * h2[1] + h1[2] * h2[2];
+                            double hm = 0.5 * (ka + kd);
+                            double lam = hm + sqrt(0.25 * (ka - kd) * (ka - kd) + kb * kb);
+                            double a1 = lam - kd, b1 = kb, a2 = kb, b2 = lam - ka;
+                            double al, be;
+                            if (a1 * a1 + b1 * b1 >= a2 * a2 + b2 * b2) { al = a1; be = b1; }
+                            else { al = a2; be = b2; }
+                            if (al == 0.0 && be == 0.0) { al = 1.0; be = 0.0; }
+                            double ux = al * h1[0] + be * h2[0];
+                            double uy = al * h1[1] + be * h2[1];
+                            double uz = al * h1[2] + be * h2[2];
+                            if (ux < 0) { ux = -ux; uy = -uy; uz = -uz; }
+                            double r = sqrt(ux * ux + uy * uy + uz * uz);
+                            if (r > 0.0) {
+                                cth = sqrt((ux + r) / (2.0 * r));
+                                double den = sqrt(2.0 * r * (ux + r));
+                                sth = cmk(uy / den, -uz / den);
+                                if (fabs(sth.x) + fabs(sth.y) > 1e-15) s_any = 1;
+                            }
+                        }
+                    }
+                    rc[tid] = cth;
+                    rs[tid] = sth;
+                    rp[tid] = p;
+                    rq[tid] = q;
+                }
+                __syncthreads();
+                // rows: M <- R^H M, R = [[c, -conj(s)], [s, c]] on (p, q)
+                for (int e = tid; e < npair * b; e += 256) {
+                    int pi = e % npair, col = e / npair;
+                    int p = rp[pi], q = rq[pi];
+                    if (q >= b) continue;
+                    double cth = rc[pi];
+                    cd sth = rs[pi];
+                    cd xp = Ms[p + col * ld], xq = Ms[q + col * ld];
+                    Ms[p + col * ld] = cadd(cscale(xp, cth), cmul(cconj(sth), xq));
+                    Ms[q + col * ld] = csub(cscale(xq, cth), cmul(sth, xp));
+                }
+                __syncthreads();
+                // columns: M <- M R, W <- W R
+                for (int e = tid; e < npair * b; e += 256) {
+                    int pi = e % npair, row = e / npair;
+                    int p = rp[pi], q = rq[pi];
+                    if (q >= b) continue;
+                    double cth = rc[pi];
+                    cd sth = rs[pi];
+                    cd xp = Ms[row + p * ld], xq = Ms[row + q * ld];
+                    Ms[row + p * ld] = cadd(cscale(xp, cth), cmul(sth, xq));
+                    Ms[row + q * ld] = csub(cscale(xq, cth), cmul(cconj(sth), xp));
+                    cd wp = W[row + (size_t)p * b], wq = W[row + (size_t)q * b];
+                    W[row + (size_t)p * b] = cadd(cscale(wp, cth), cmul(sth, wq));
+                    W[row + (size_t)q * b] = csub(cscale(wq, cth), cmul(cconj(sth), wp));
+                }
+                __syncthreads();
+            }
+            if (!s_any) break;
+        }
+        double *o = outv + (size_t)c * P.N + P.blk_soff[blk];
+        for (int r = tid; r < b; r += 256) {
+            cd l = Ms[r + r * ld];
+            o[r] = (mode == 0) ? atan2(l.y, l.x) : l.x;
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------------------ k_bmul
+// B_k = A_k W for every (crystallite, step, block) of a group; writes into bufB.
+template <int M>
+__global__ void __launch_bounds__(256) k_bmul(DevPlan P, int Kb, const int *__restrict__ grp,
+                                              int ngrp, const cd *__restrict__ bufA,
+                                              const cd *__restrict__ bufW, cd *__restrict__ bufB) {
+    __shared__ cd smem[2 * 16 * 16 * M];
+    const long ntask = (long)Kb * P.n * ngrp;
+    for (long task = blockIdx.x; task < ntask; task += gridDim.x) {
+        const int g = (int)(task % ngrp);
+        const long ck = task / ngrp;
+        const int k = (int)(ck % P.n), c = (int)(ck / P.n);
+        const int blk = grp[g], b = P.blk_dim[blk];
+        const size_t off = ((size_t)c * P.n + k) * P.Lsum + P.blk_uoff[blk];
+        const cd *W = bufW + (size_t)c * P.Lsum + P.blk_uoff[blk];
+        wg_gemm<M, cd>(b, b, b, bufA + off, b, false, W, b, false, bufB + off, b, cmk(0, 0), smem);
+    }
+}
+
+// ------------------------------------------------------------------------------ k_qr
+// One task = (crystallite, step, block pair (I,J)).  With B = A_{k+1} W (per block):
+//   Q(r,s) = sum conj(B_I(a,r)) D_IJ(a,b) B_J(b,s)        r in I, s in J
+//   R(s,r) = sum conj(B_J(b,s)) rho_JI(b,a) B_I(a,r)
+// (rho_eff = det + det^H when rho == (det+det^H)/2, compute.h:381-382).  Stored as the
+// twiddled series Qt_k(e) = Q z^-k, Rt_k(e) = R(s,r) z^k, z = exp(i (th_r - th_s)/n),
+// element e = r + s*b_I at offset pair_eoff, layout [c][k][e].
+template <int M>
+__global__ void __launch_bounds__(256) k_qr(DevPlan P, int Kb, const int *__restrict__ pgrp,
+                                            int ngrp, const cd *__restrict__ bufB,
+                                            const double *__restrict__ thetas,
+                                            cd *__restrict__ scratch, size_t slot,
+                                            cd *__restrict__ Qt, cd *__restrict__ Rt) {
+    __shared__ cd smem[2 * 16 * 16 * M];
+    const int tid = threadIdx.x;
+    cd *T1 = scratch + slot * 2 * blockIdx.x;
+    cd *G = T1 + slot;
+    const long ntask = (long)Kb * P.n * ngrp;
+    for (long task = blockIdx.x; task < ntask; task += gridDim.x) {
+        const int g = (int)(task % ngrp);
+        const long ck = task / ngrp;
+        const int k = (int)(ck % P.n), c = (int)(ck / P.n);
+        const int pr = pgrp[g];
+        const int I = P.pair_I[pr], J = P.pair_J[pr];
+        const int bI = P.blk_dim[I], bJ = P.blk_dim[J];
+        const cd *BI = bufB + ((size_t)c * P.n + k) * P.Lsum + P.blk_uoff[I];
+        const cd *BJ = bufB + ((size_t)c * P.n + k) * P.Lsum + P.blk_uoff[J];
+        const double *thI = thetas + (size_t)c * P.N + P.blk_soff[I];
+        const double *thJ = thetas + (size_t)c * P.N + P.blk_soff[J];
+        const size_t eo = ((size_t)c * P.n + k) * P.Etot + P.pair_eoff[pr];
+        // T1 = D_IJ B_J  (bI x bJ), sparse rows of D_IJ
+        {
+            const int *rp = P.d_rowptr + P.pair_doff[pr];
+            const int *ci = P.d_col;
+            const cd *dv = (const cd *)P.d_val;
+            for (int e = tid; e < bI * bJ; e += 256) {
+                int a = e % bI, s = e / bI;
+                cd acc = cmk(0, 0);
+                for (int z = rp[a]; z < rp[a + 1]; z++) cfma(acc, dv[z], BJ[ci[z] + (size_t)s * bJ]);
+                T1[e] = acc;
+            }
+        }
+        __syncthreads();
+        // G = B_I^H T1 = Q (bI x bJ)
+        wg_gemm<M, cd>(bI, bJ, bI, BI, bI, true, T1, bI, false, G, bI, cmk(0, 0), smem);
+        const double inv_n = 1.0 / (double)P.n;
+        for (int e = tid; e < bI * bJ; e += 256) {
+            int r = e % bI, s = e / bI;
+            double ph = (thI[r] - thJ[s]) * inv_n * (double)k;
+            Qt[eo + e] = cmul(G[e], cexpi(-ph));
+        }
+        __syncthreads();
+        // T1' = rho_JI B_I (bJ x bI) ; R = B_J^H T1' (bJ x bI)
+        {
+            const int *rp = P.r_rowptr + P.pair_roff[pr];
+            const int *ci = P.r_col;
+            const cd *rv = (const cd *)P.r_val;
+            for (int e = tid; e < bJ * bI; e += 256) {
+                int bq = e % bJ, r = e / bJ;
+                cd acc = cmk(0, 0);
+                for (int z = rp[bq]; z < rp[bq + 1]; z++) cfma(acc, rv[z], BI[ci[z] + (size_t)r * bI]);
+                T1[e] = acc;
+            }
+        }
+        __syncthreads();
+        wg_gemm<M, cd>(bJ, bI, bJ, BJ, bJ, true, T1, bJ, false, G, bJ, cmk(0, 0), smem);
+        for (int e = tid; e < bI * bJ; e += 256) {
+            int r = e % bI, s = e / bI;
+            double ph = (thI[r] - thJ[s]) * inv_n * (double)k;
+            Rt[eo + e] = cmul(G[s + (size_t)r * bJ], cexpi(ph));
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------------------ k_corr
+// One workgroup = (crystallite, chunk of CH elements).  The Qt/Rt series of the chunk are
+// staged in LDS; thread (el, pg) computes F_p(el) for p in [8 pg, 8 pg + 8):
+//   F_p(e) = sum_i Rt_i(e) Qt_{(i+p)%n}(e)                      (compute.h:406-427)
+// then F replaces Qt in LDS and thread t forms fid[t] = sum_e F_{t%n}(e) exp(i t phi_e),
+// phi_e = (th_r - th_s)/n, added atomically to fid_c[c][t]      (compute.h:432-457).
+// CH = min(CORR_SPAN / n, 256 / ceil(n/8)) is computed identically on the host.
+#define CORR_SPAN 2048
+#define CORR_PPT 8
+__device__ __host__ inline int corr_chunk(int n) {
+    int npg = (n + CORR_PPT - 1) / CORR_PPT;
+    int a = CORR_SPAN / n, b = 256 / npg;
+    int ch = a < b ? a : b;
+    return ch < 1 ? 1 : ch;
+}
+__global__ void __launch_bounds__(256) k_corr(DevPlan P, int Kb, const cd *__restrict__ Qt,
+                                              const cd *__restrict__ Rt,
+                                              const double *__restrict__ thetas,
+                                              const int *__restrict__ elem_r,
+                                              const int *__restrict__ elem_s,
+                                              double *__restrict__ fidc) {
+    __shared__ cd sQ[CORR_SPAN];
+    __shared__ cd sR[CORR_SPAN];
+    __shared__ double sphi[256];
+    const int n = P.n, tid = threadIdx.x;
+    const int CH = corr_chunk(n);
+    const int npg = (n + CORR_PPT - 1) / CORR_PPT;
+    const int nchunk = (int)((P.Etot + CH - 1) / CH);
+    const long ntask = (long)Kb * nchunk;
+    for (long task = blockIdx.x; task < ntask; task += gridDim.x) {
+        const int ch = (int)(task % nchunk), c = (int)(task / nchunk);
+        const long e0 = (long)ch * CH;
+        for (int idx = tid; idx < CH * n; idx += 256) {
+            int el = idx % CH, k = idx / CH;
+            long e = e0 + el;
+            cd q = cmk(0, 0), r = cmk(0, 0);
+            if (e < P.Etot) {
+                size_t o = ((size_t)c * n + k) * P.Etot + e;
+                q = Qt[o];
+                r = Rt[o];
+            }
+            sQ[idx] = q;
+            sR[idx] = r;
+        }
+        for (int el = tid; el < CH; el += 256) {
+            long e = e0 + el;
+            double ph = 0.0;
+            if (e < P.Etot) {
+                const double *th = thetas + (size_t)c * P.N;
+                ph = (th[elem_r[e]] - th[elem_s[e]]) / (double)n;
+            }
+            sphi[el] = ph;
+        }
+        __syncthreads();
+        cd F[CORR_PPT];
+        const int el = tid % CH, pg = tid / CH;
+        const int p0 = pg * CORR_PPT;
+        const bool active = pg < npg;
+#pragma unroll
+        for (int q = 0; q < CORR_PPT; q++) F[q] = cmk(0, 0);
+        if (active) {
+            cd win[CORR_PPT];
+#pragma unroll
+            for (int q = 0; q < CORR_PPT; q++) win[q] = sQ[((p0 + q) % n) * CH + el];
+            for (int i = 0; i < n; i++) {
+                cd r = sR[i * CH + el];
+#pragma unroll
+                for (int q = 0; q < CORR_PPT; q++) cfma(F[q], r, win[q]);
+#pragma unroll
+                for (int q = 0; q < CORR_PPT - 1; q++) win[q] = win[q + 1];
+                win[CORR_PPT - 1] = sQ[((i + p0 + CORR_PPT) % n) * CH + el];
+            }
+        }
+        __syncthreads();
+        if (active) {
+#pragma unroll
+            for (int q = 0; q < CORR_PPT; q++)
+                if (p0 + q < n) sQ[(p0 + q) * CH + el] = F[q];
+        }
+        __syncthreads();
+        for (int t = tid; t < P.npts; t += 256) {
+            const int p = t % n;
+            cd acc = cmk(0, 0);
+            for (int e = 0; e < CH; e++) {
+                cd f = sQ[p * CH + e];
+                if (f.x == 0.0 && f.y == 0.0) continue;
+                cfma(acc, f, cexpi((double)t * sphi[e]));
+            }
+            if (acc.x != 0.0 || acc.y != 0.0) {
+                atomicAdd(&fidc[2 * ((size_t)c * P.npts + t)], acc.x);
+                atomicAdd(&fidc[2 * ((size_t)c * P.npts + t) + 1], acc.y);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------------------ k_reduce
+// acc[t] += sum_c w_c * scale * fid_c[t], Kahan-compensated, crystallites in order.
+__global__ void k_reduce(int Kb, int c0, int npts, const double *__restrict__ w, double scale,
+                         const double *__restrict__ fidc, double *__restrict__ acc,
+                         double *__restrict__ comp) {
+    int q = blockIdx.x * blockDim.x + threadIdx.x; // re/im component index
+    if (q >= 2 * npts) return;
+    double s = acc[q], cmp = comp[q];
+    for (int c = 0; c < Kb; c++) {
+        double y = w[c0 + c] * scale * fidc[(size_t)c * 2 * npts + q] - cmp;
+        double t = s + y;
+        cmp = (t - s) - y;
+        s = t;
+    }
+    acc[q] = s;
+    comp[q] = cmp;
+}
+
+__global__ void k_finish(int npts, const double *__restrict__ acc, double2 norm,
+                         double *__restrict__ out) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= npts) return;
+    cd v = cmk(acc[2 * t], acc[2 * t + 1]);
+    // v / norm (complex division; norm = Tr(D D^H) is real positive in practice)
+    double d = norm.x * norm.x + norm.y * norm.y;
+    cd r = cmk((v.x * norm.x + v.y * norm.y) / d, (v.y * norm.x - v.x * norm.y) / d);
+    out[2 * t] = r.x;
+    out[2 * t + 1] = r.y;
+}
+
+__global__ void k_zero(double *p, size_t n) {
+    size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    for (; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0.0;
+}
+
+// ------------------------------------------------------------------------------ static path
+// H(0, dt) block of every crystallite (wr = 0: rotor phase 0), complex copy for k_eig.
+__global__ void __launch_bounds__(256) k_static_h(DevPlan P, int Kb, int c0, const int *__restrict__ grp,
+                                                  int ngrp, double *__restrict__ scratch,
+                                                  size_t slot, cd *__restrict__ bufH) {
+    __shared__ double s_zz[DEV_MAX_TERMS];
+    __shared__ double s_ff[DEV_MAX_SPINS * DEV_MAX_SPINS];
+    __shared__ double s_v[4];
+    double *X = scratch + slot * blockIdx.x;
+    const long ntask = (long)Kb * ngrp;
+    for (long task = blockIdx.x; task < ntask; task += gridDim.x) {
+        const int g = (int)(task % ngrp), c = (int)(task / ngrp);
+        const int blk = grp[g], b = P.blk_dim[blk];
+        build_h_block(P, P.RP + 9 * (size_t)(c0 + c), P.u3, blk, X, b, s_zz, s_ff, s_v);
+        cd *H = bufH + (size_t)c * P.Lsum + P.blk_uoff[blk];
+        for (int e = threadIdx.x; e < b * b; e += 256) H[e] = cmk(X[e], 0.0);
+        __syncthreads();
+    }
+}
+
+// Static FID of one (crystallite, block pair): with V the eigenvectors (bufW) and lambda the
+// eigenvalues, A(i,j) = Do(i,j) sig0(j,i), kept when |A|^2 > 1e-10 (oneFID.h:317-325), and
+// fid[t] += A(i,j) exp(-i 2 pi dt t (lambda_j - lambda_i)) (conj(e_i) e_j)^t.
+template <int M>
+__global__ void __launch_bounds__(256) k_static_fid(DevPlan P, int Kb, const int *__restrict__ pgrp,
+                                                    int ngrp, const cd *__restrict__ bufW,
+                                                    const double *__restrict__ lam,
+                                                    cd *__restrict__ scratch, size_t slot,
+                                                    double dt, double *__restrict__ fidc) {
+    __shared__ cd smem[2 * 16 * 16 * M];
+    const int tid = threadIdx.x;
+    cd *T1 = scratch + slot * 3 * blockIdx.x;
+    cd *Gq = T1 + slot, *Gr = Gq + slot;
+    const long ntask = (long)Kb * ngrp;
+    for (long task = blockIdx.x; task < ntask; task += gridDim.x) {
+        const int g = (int)(task % ngrp), c = (int)(task / ngrp);
+        const int pr = pgrp[g];
+        const int I = P.pair_I[pr], J = P.pair_J[pr];
+        const int bI = P.blk_dim[I], bJ = P.blk_dim[J];
+        const cd *VI = bufW + (size_t)c * P.Lsum + P.blk_uoff[I];
+        const cd *VJ = bufW + (size_t)c * P.Lsum + P.blk_uoff[J];
+        const double *lI = lam + (size_t)c * P.N + P.blk_soff[I];
+        const double *lJ = lam + (size_t)c * P.N + P.blk_soff[J];
+        {
+            const int *rp = P.d_rowptr + P.pair_doff[pr];
+            const cd *dv = (const cd *)P.d_val;
+            for (int e = tid; e < bI * bJ; e += 256) {
+                int a = e % bI, s = e / bI;
+                cd acc = cmk(0, 0);
+                for (int z = rp[a]; z < rp[a + 1]; z++) cfma(acc, dv[z], VJ[P.d_col[z] + (size_t)s * bJ]);
+                T1[e] = acc;
+            }
+        }
+        __syncthreads();
+        wg_gemm<M, cd>(bI, bJ, bI, VI, bI, true, T1, bI, false, Gq, bI, cmk(0, 0), smem);
+        {
+            const int *rp = P.r_rowptr + P.pair_roff[pr];
+            const cd *rv = (const cd *)P.r_val;
+            for (int e = tid; e < bJ * bI; e += 256) {
+                int bq = e % bJ, r = e / bJ;
+                cd acc = cmk(0, 0);
+                for (int z = rp[bq]; z < rp[bq + 1]; z++) cfma(acc, rv[z], VI[P.r_col[z] + (size_t)r * bI]);
+                T1[e] = acc;
+            }
+        }
+        __syncthreads();
+        wg_gemm<M, cd>(bJ, bI, bJ, VJ, bJ, true, T1, bJ, false, Gr, bJ, cmk(0, 0), smem);
+        // A(i,j) = Do(i,j) sig0(j,i): i in I, j in J
+        for (int e = tid; e < bI * bJ; e += 256) {
+            int r = e % bI, s = e / bI;
+            cd a = cmul(Gq[e], Gr[s + (size_t)r * bJ]);
+            if (a.x * a.x + a.y * a.y > 1.0e-10) T1[e] = a;
+            else T1[e] = cmk(0, 0);
+        }
+        __syncthreads();
+        const double w0 = -2.0 * 3.14159265358979323846 * dt;
+        for (int t = tid; t < P.npts; t += 256) {
+            cd acc = cmk(0, 0);
+            for (int e = 0; e < bI * bJ; e++) {
+                cd a = T1[e];
+                if (a.x == 0.0 && a.y == 0.0) continue;
+                int r = e % bI, s = e / bI;
+                cfma(acc, a, cexpi(w0 * (double)t * (lJ[s] - lI[r])));
+            }
+            atomicAdd(&fidc[2 * ((size_t)c * P.npts + t)], acc.x);
+            atomicAdd(&fidc[2 * ((size_t)c * P.npts + t) + 1], acc.y);
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------------------ launchers
+#define GRID_CAP 4096
+
+extern "C" int blfid_dev_launch_exp(int M, const DevPlan *P, int Kb, int c0, const int *grp,
+                                    int ngrp, double *scratch, size_t slot, int nslots, cd *bufU,
+                                    hipStream_t st) {
+    long ntask = (long)Kb * P->n * ngrp;
+    int grid = (int)(ntask < nslots ? ntask : nslots);
+    if (grid <= 0) return 0;
+    switch (M) {
+    case 1: k_exp<1><<<grid, 256, 0, st>>>(*P, Kb, c0, grp, ngrp, scratch, slot, bufU); break;
+    case 2: k_exp<2><<<grid, 256, 0, st>>>(*P, Kb, c0, grp, ngrp, scratch, slot, bufU); break;
+    case 3: k_exp<3><<<grid, 256, 0, st>>>(*P, Kb, c0, grp, ngrp, scratch, slot, bufU); break;
+    case 4: k_exp<4><<<grid, 256, 0, st>>>(*P, Kb, c0, grp, ngrp, scratch, slot, bufU); break;
+    default: k_exp<5><<<grid, 256, 0, st>>>(*P, Kb, c0, grp, ngrp, scratch, slot, bufU); break;
+    }
+    return (int)hipGetLastError();
+}
+
+extern "C" int blfid_dev_launch_dyson(int M, const DevPlan *P, int Kb, const int *grp, int ngrp,
+                                      const cd *bufU, cd *bufA, hipStream_t st) {
+    long ntask = (long)Kb * ngrp;
+    int grid = (int)(ntask < GRID_CAP ? ntask : GRID_CAP);
+    if (grid <= 0) return 0;
+    switch (M) {
+    case 1: k_dyson<1><<<grid, 256, 0, st>>>(*P, Kb, grp, ngrp, bufU, bufA); break;
+    case 2: k_dyson<2><<<grid, 256, 0, st>>>(*P, Kb, grp, ngrp, bufU, bufA); break;
+    case 3: k_dyson<3><<<grid, 256, 0, st>>>(*P, Kb, grp, ngrp, bufU, bufA); break;
+    case 4: k_dyson<4><<<grid, 256, 0, st>>>(*P, Kb, grp, ngrp, bufU, bufA); break;
+    default: k_dyson<5><<<grid, 256, 0, st>>>(*P, Kb, grp, ngrp, bufU, bufA); break;
+    }
+    return (int)hipGetLastError();
+}
+
+extern "C" int blfid_dev_launch_eig(const DevPlan *P, int Kb, const int *grp, int ngrp,
+                                    const cd *src, size_t src_stride, int mode, cd *bufW,
+                                    double *outv, hipStream_t st) {
+    long ntask = (long)Kb * ngrp;
+    int grid = (int)(ntask < GRID_CAP ? ntask : GRID_CAP);
+    if (grid <= 0) return 0;
+    k_eig<<<grid, 256, 0, st>>>(*P, Kb, grp, ngrp, src, src_stride, mode, bufW, outv);
+    return (int)hipGetLastError();
+}
+
+extern "C" int blfid_dev_launch_bmul(int M, const DevPlan *P, int Kb, const int *grp, int ngrp,
+                                     const cd *bufA, const cd *bufW, cd *bufB, hipStream_t st) {
+    long ntask = (long)Kb * P->n * ngrp;
+    int grid = (int)(ntask < GRID_CAP ? ntask : GRID_CAP);
+    if (grid <= 0) return 0;
+    switch (M) {
+    case 1: k_bmul<1><<<grid, 256, 0, st>>>(*P, Kb, grp, ngrp, bufA, bufW, bufB); break;
+    case 2: k_bmul<2><<<grid, 256, 0, st>>>(*P, Kb, grp, ngrp, bufA, bufW, bufB); break;
+    case 3: k_bmul<3><<<grid, 256, 0, st>>>(*P, Kb, grp, ngrp, bufA, bufW, bufB); break;
+    case 4: k_bmul<4><<<grid, 256, 0, st>>>(*P, Kb, grp, ngrp, bufA, bufW, bufB); break;
+    default: k_bmul<5><<<grid, 256, 0, st>>>(*P, Kb, grp, ngrp, bufA, bufW, bufB); break;
+    }
+    return (int)hipGetLastError();
+}
+
+extern "C" int blfid_dev_launch_qr(int M, const DevPlan *P, int Kb, const int *pgrp, int ngrp,
+                                   const cd *bufB, const double *thetas, cd *scratch, size_t slot,
+                                   int nslots, cd *Qt, cd *Rt, hipStream_t st) {
+    long ntask = (long)Kb * P->n * ngrp;
+    int grid = (int)(ntask < nslots ? ntask : nslots);
+    if (grid <= 0) return 0;
+    switch (M) {
+    case 1: k_qr<1><<<grid, 256, 0, st>>>(*P, Kb, pgrp, ngrp, bufB, thetas, scratch, slot, Qt, Rt); break;
+    case 2: k_qr<2><<<grid, 256, 0, st>>>(*P, Kb, pgrp, ngrp, bufB, thetas, scratch, slot, Qt, Rt); break;
+    case 3: k_qr<3><<<grid, 256, 0, st>>>(*P, Kb, pgrp, ngrp, bufB, thetas, scratch, slot, Qt, Rt); break;
+    case 4: k_qr<4><<<grid, 256, 0, st>>>(*P, Kb, pgrp, ngrp, bufB, thetas, scratch, slot, Qt, Rt); break;
+    default: k_qr<5><<<grid, 256, 0, st>>>(*P, Kb, pgrp, ngrp, bufB, thetas, scratch, slot, Qt, Rt); break;
+    }
+    return (int)hipGetLastError();
+}
+
+extern "C" int blfid_dev_launch_corr(const DevPlan *P, int Kb, const cd *Qt, const cd *Rt,
+                                     const double *thetas, const int *er, const int *es,
+                                     double *fidc, hipStream_t st) {
+    int CH = corr_chunk(P->n);
+    long nchunk = (P->Etot + CH - 1) / CH;
+    long ntask = (long)Kb * nchunk;
+    int grid = (int)(ntask < GRID_CAP ? ntask : GRID_CAP);
+    if (grid <= 0) return 0;
+    k_corr<<<grid, 256, 0, st>>>(*P, Kb, Qt, Rt, thetas, er, es, fidc);
+    return (int)hipGetLastError();
+}
+
+extern "C" int blfid_dev_launch_reduce(int Kb, int c0, int npts, const double *w, double scale,
+                                       const double *fidc, double *acc, double *comp,
+                                       hipStream_t st) {
+    int n2 = 2 * npts;
+    k_reduce<<<(n2 + 255) / 256, 256, 0, st>>>(Kb, c0, npts, w, scale, fidc, acc, comp);
+    return (int)hipGetLastError();
+}
+
+extern "C" int blfid_dev_launch_finish(int npts, const double *acc, double nre, double nim,
+                                       double *out, hipStream_t st) {
+    k_finish<<<(npts + 255) / 256, 256, 0, st>>>(npts, acc, make_double2(nre, nim), out);
+    return (int)hipGetLastError();
+}
+
+extern "C" int blfid_dev_launch_zero(double *p, size_t n, hipStream_t st) {
+    if (n == 0) return 0;
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 65535) blocks = 65535;
+    k_zero<<<(int)blocks, 256, 0, st>>>(p, n);
+    return (int)hipGetLastError();
+}
+
+extern "C" int blfid_dev_launch_prep(int K, const double *th, const double *ph, const double *ga,
+                                     double *RP, hipStream_t st) {
+    if (K <= 0) return 0;
+    k_prep<<<(K + 255) / 256, 256, 0, st>>>(K, th, ph, ga, RP);
+    return (int)hipGetLastError();
+}
+
+extern "C" int blfid_dev_launch_static_h(const DevPlan *P, int Kb, int c0, const int *grp,
+                                         int ngrp, double *scratch, size_t slot, int nslots,
+                                         cd *bufH, hipStream_t st) {
+    long ntask = (long)Kb * ngrp;
+    int grid = (int)(ntask < nslots ? ntask : nslots);
+    if (grid <= 0) return 0;
+    k_static_h<<<grid, 256, 0, st>>>(*P, Kb, c0, grp, ngrp, scratch, slot, bufH);
+    return (int)hipGetLastError();
+}
+
+extern "C" int blfid_dev_launch_static_fid(int M, const DevPlan *P, int Kb, const int *pgrp,
+                                           int ngrp, const cd *bufW, const double *lam,
+                                           cd *scratch, size_t slot, int nslots, double dt,
+                                           double *fidc, hipStream_t st) {
+    long ntask = (long)Kb * ngrp;
+    int grid = (int)(ntask < nslots ? ntask : nslots);
+    if (grid <= 0) return 0;
+    switch (M) {
+    case 1: k_static_fid<1><<<grid, 256, 0, st>>>(*P, Kb, pgrp, ngrp, bufW, lam, scratch, slot, dt, fidc); break;
+    case 2: k_static_fid<2><<<grid, 256, 0, st>>>(*P, Kb, pgrp, ngrp, bufW, lam, scratch, slot, dt, fidc); break;
+    case 3: k_static_fid<3><<<grid, 256, 0, st>>>(*P, Kb, pgrp, ngrp, bufW, lam, scratch, slot, dt, fidc); break;
+    case 4: k_static_fid<4><<<grid, 256, 0, st>>>(*P, Kb, pgrp, ngrp, bufW, lam, scratch, slot, dt, fidc); break;
+    default: k_static_fid<5><<<grid, 256, 0, st>>>(*P, Kb, pgrp, ngrp, bufW, lam, scratch, slot, dt, fidc); break;
+    }
+    return (int)hipGetLastError();
+}

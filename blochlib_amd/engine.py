@@ -1,0 +1,230 @@
+"""ctypes binding of the engine's C ABI (include/blfid.h) and the oneFID-style front end.
+
+The shared library is ``blochlib_amd/libblfid.so`` (built by ``python -m blochlib_amd.build``).
+There is no fallback: if the library is missing or no HIP device is present, calls raise.
+
+Front end (mirrors src/QMspins/driver/oneFID.h:52-218):
+
+    fid = OneFID(sys, npts=256, sw=640e3, wr=10e3, rotor=54.7356)
+    sig = fid.FID(powder, ro, det)             # whole powder on the GPU, oneFID.h:200-208
+    sig1 = fid.FID(ro, det, theta, phi, gamma) # one crystallite, oneFID.h:210-218
+    fid.sw()                                   # adjusted sweep width n*wr
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from .powder import Powder
+from .spinsys import BlochLibError, SolidSys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libblfid.so")
+
+BLFID_OK = 0
+STATUS = {0: "OK", 1: "INVALID", 2: "NOMEM", 3: "DEVICE", 4: "UNSUPPORTED", 5: "NOCONV"}
+
+_d = ctypes.POINTER(ctypes.c_double)
+_i = ctypes.POINTER(ctypes.c_int)
+
+
+class EngineError(BlochLibError):
+    def __init__(self, status, msg):
+        super().__init__(f"blfid {STATUS.get(status, status)}: {msg}")
+        self.status = status
+
+
+class _System(ctypes.Structure):
+    _fields_ = [("nspins", ctypes.c_int), ("isotope", _i), ("ncsa", ctypes.c_int),
+                ("csa", _d), ("ndip", ctypes.c_int), ("dip", _d), ("nj", ctypes.c_int),
+                ("jc", _d)]
+
+
+_lib = None
+
+
+def lib():
+    """Load libblfid.so; raises if it is missing (no CPU fallback exists)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise EngineError(3, f"{LIBPATH} is not built; run python -m blochlib_amd.build")
+        L = ctypes.CDLL(LIBPATH)
+        vp = ctypes.c_void_p
+        L.blfid_plan_create.argtypes = [ctypes.POINTER(_System), ctypes.c_double, ctypes.c_double,
+                                        ctypes.c_double, ctypes.c_int, ctypes.c_double,
+                                        ctypes.c_int, ctypes.POINTER(vp)]
+        L.blfid_plan_destroy.argtypes = [vp]
+        L.blfid_plan_destroy.restype = None
+        L.blfid_plan_sw.argtypes = [vp]
+        L.blfid_plan_sw.restype = ctypes.c_double
+        L.blfid_plan_steps.argtypes = [vp]
+        L.blfid_plan_dim.argtypes = [vp]
+        L.blfid_plan_set_operators.argtypes = [vp, _d, _d]
+        L.blfid_plan_set_powder.argtypes = [vp, _d, _d, _d, _d, ctypes.c_int]
+        L.blfid_plan_run.argtypes = [vp, ctypes.c_int, vp]
+        L.blfid_plan_fetch.argtypes = [vp, _d]
+        L.blfid_plan_device_fid.argtypes = [vp]
+        L.blfid_plan_device_fid.restype = vp
+        L.blfid_powder_fid.argtypes = [vp, _d, _d, _d, _d, ctypes.c_int, _d, _d, ctypes.c_int, _d]
+        L.blfid_plan_stats.argtypes = [vp, _d]
+        L.blfid_plan_set_memory_budget.argtypes = [vp, ctypes.c_double]
+        L.blfid_last_error.restype = ctypes.c_char_p
+        L.blfid_version.restype = ctypes.c_char_p
+        _lib = L
+    return _lib
+
+
+def _check(st):
+    if st != BLFID_OK:
+        raise EngineError(st, lib().blfid_last_error().decode())
+
+
+def _p(a, t=_d):
+    return a.ctypes.data_as(t)
+
+
+def _cm(M, N):
+    M = np.asarray(M, dtype=np.complex128)
+    if M.shape != (N, N):
+        raise BlochLibError(f"operator must be {N}x{N}, got {M.shape}")
+    return np.ascontiguousarray(M.ravel(order="F").view(np.float64))
+
+
+class Plan:
+    """One spin system + acquisition on the device (blfid_plan_*)."""
+
+    def __init__(self, sy: SolidSys, wr: float, rotor_deg: float, sw: float, npts: int,
+                 t0: float = 0.0):
+        L = lib()
+        iso, csa, dip, jc = sy.tables()
+        self._keep = [np.ascontiguousarray(iso, np.int32),
+                      csa if csa.size else np.zeros(1), dip if dip.size else np.zeros(1),
+                      jc if jc.size else np.zeros(1)]
+        s = _System(sy.spins, _p(self._keep[0], _i), len(sy.csa), _p(self._keep[1]), len(sy.dip),
+                    _p(self._keep[2]), len(sy.jc), _p(self._keep[3]))
+        h = ctypes.c_void_p()
+        _check(L.blfid_plan_create(ctypes.byref(s), float(wr), float(rotor_deg), float(sw),
+                                   int(npts), float(t0), 0, ctypes.byref(h)))
+        self.h = h
+        self.N = sy.N
+        self.npts = int(npts)
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            lib().blfid_plan_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def sw(self) -> float:
+        return lib().blfid_plan_sw(self.h)
+
+    @property
+    def steps(self) -> int:
+        return lib().blfid_plan_steps(self.h)
+
+    def set_operators(self, rho, det):
+        self._r = _cm(rho, self.N)
+        self._dt = _cm(det, self.N)
+        _check(lib().blfid_plan_set_operators(self.h, _p(self._r), _p(self._dt)))
+
+    def set_powder(self, pw: Powder):
+        self._pw = [np.ascontiguousarray(a, np.float64) for a in
+                    (pw.theta, pw.phi, pw.gamma, pw.weight)]
+        _check(lib().blfid_plan_set_powder(self.h, *[_p(a) for a in self._pw], len(pw)))
+
+    def set_memory_budget(self, nbytes: float):
+        _check(lib().blfid_plan_set_memory_budget(self.h, float(nbytes)))
+
+    def run(self, normalize: bool = True, stream: int | None = None):
+        _check(lib().blfid_plan_run(self.h, int(bool(normalize)),
+                                    ctypes.c_void_p(stream) if stream else None))
+
+    def fetch(self) -> np.ndarray:
+        out = np.zeros(2 * self.npts)
+        _check(lib().blfid_plan_fetch(self.h, _p(out)))
+        return out.view(np.complex128).copy()
+
+    def device_fid_ptr(self) -> int:
+        return lib().blfid_plan_device_fid(self.h) or 0
+
+    def stats(self) -> dict:
+        out = np.zeros(5)
+        _check(lib().blfid_plan_stats(self.h, _p(out)))
+        return {"flops_per_cs": out[0], "dense_flops_per_cs": out[1], "blocks": int(out[2]),
+                "max_block": int(out[3]), "bytes_per_crystallite": out[4]}
+
+    def powder_fid(self, pw: Powder, rho, det, normalize=True) -> np.ndarray:
+        self.set_operators(rho, det)
+        self.set_powder(pw)
+        self.run(normalize)
+        return self.fetch()
+
+
+def powder_fid(sy, wr, rotor_deg, sw, npts, pw, rho, det, t0=0.0, normalize=True):
+    """FID(powder&, ro, det) on the GPU; returns (fid, adjusted sw)."""
+    p = Plan(sy, wr, rotor_deg, sw, npts, t0)
+    try:
+        f = p.powder_fid(pw, rho, det, normalize)
+        return f, p.sw
+    finally:
+        p.close()
+
+
+class OneFID:
+    """oneFID<SolidSys, hmatrix> front end (src/QMspins/driver/oneFID.h:52-218)."""
+
+    def __init__(self, sys: SolidSys, npts: int, sw: float = 20000.0, wr: float = 0.0,
+                 rotb: float = 0.0, startT: float = 0.0):
+        self.function_ = sys
+        self._npts = int(npts)
+        self.sw_ = float(sw)
+        self.wr_ = float(wr)
+        self.rotb_ = float(rotb)
+        self.startT_ = float(startT)
+
+    # setters/getters of oneFID.h:130-177
+    def setFunction(self, sys): self.function_ = sys
+    def setWr(self, v): self.wr_ = float(v)
+    def setRotorSpeed(self, v): self.wr_ = float(v)
+    def setRotorAngle(self, v): self.rotb_ = float(v)
+    def setRotorBeta(self, v): self.rotb_ = float(v)
+    def setSw(self, v): self.sw_ = float(v)
+    def setSweepWidth(self, v): self.sw_ = float(v)
+    def setSize(self, n): self._npts = int(n)
+    def setNpts(self, n): self._npts = int(n)
+    def setStartTime(self, t): self.startT_ = float(t)
+    def wr(self): return self.wr_
+    def rotorAngle(self): return self.rotb_
+    def sw(self): return self.sw_
+    def npts(self): return self._npts
+    def size(self): return self._npts
+    def startTime(self): return self.startT_
+
+    def FID(self, *args, normalize=False):
+        """FID(powder, ro, det) or FID(ro, det, theta=0, phi=0, gamma=0).  Like the library
+        call (oneFID.h:396-402) the powder sum is NOT divided by Tr(det det^H) unless
+        normalize=True (solid-2.0's doFID does divide, sequencerun_funcs.cc:698-702)."""
+        if self._npts <= 0:
+            raise BlochLibError("Fid size was '0' please use 'npts(int)' set set it")
+        if isinstance(args[0], Powder):
+            pw, ro, det = args[0], args[1], args[2]
+        else:
+            ro, det = args[0], args[1]
+            th, ph, ga = (list(args[2:5]) + [0.0, 0.0, 0.0])[:3]
+            pw = Powder.single(th, ph, ga, 1.0)
+        p = Plan(self.function_, self.wr_, self.rotb_, self.sw_, self._npts, self.startT_)
+        try:
+            f = p.powder_fid(pw, ro, det, normalize)
+            self.sw_ = p.sw
+            return f
+        finally:
+            p.close()

@@ -1,0 +1,121 @@
+/*
+ * blfid.h -- C ABI of the MI355X powder-averaged MAS FID engine.
+ *
+ * This is the boundary a BlochLib caller binds instead of the solid-2.0 FID path.  It replaces
+ *   oneFID<SolidSys, M>::FID(powder&, ro, det)   src/QMspins/driver/oneFID.h:200-208, 339-410
+ *   oneFID<SolidSys, M>::FID(ro, det, th, ph, ga) src/QMspins/driver/oneFID.h:210-218, 419-431
+ *   compute<SolidSys, M>  (gamma-COMPUTE)         src/QMspins/driver/compute.h:57-461
+ *   SequenceRun::doFID 1-D fid() branch           examples/solid-2.0/src/sequencerun_funcs.cc:690-704
+ * and the MPI powder reduction it sits in        src/QMspins/driver/oneFID.h:351-407,
+ *                                                 examples/solid-2.0/src/sequencerun.cc:435-508.
+ *
+ * Conventions (all as in BlochLib):
+ *   - Hilbert space of S spin-1/2 nuclei, N = 2^S; spin 0 is the leftmost Kronecker factor and
+ *     m = +1/2 comes first (src/QMspins/multispinop.cc:51-70, singlespinop.cc:142-155).
+ *   - Operator matrices (rho, det) are N x N complex, column-major, re/im interleaved doubles
+ *     (BlochLib Full matrices are column-major, src/container/matrix/fmatrix.h:97-121).
+ *   - Frequencies in Hz, times in s, powder angles in radians, Euler angles of interaction
+ *     tensors and the rotor angle in degrees (csa.cc:276, oneFID.h:421).
+ *   - Powder orientation k is (theta[k], phi[k], gamma[k]) with weight w[k]; a powder file row
+ *     "a b g w" is phi=a, theta=b, gamma=g (src/QMspins/driver/powder.cc:333-389).
+ *
+ * Errors: every call returns a blfid_status; blfid_last_error() gives a thread-local message.
+ * A plan is not thread safe; distinct plans may be used from distinct threads.
+ */
+#ifndef BLFID_H
+#define BLFID_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    BLFID_OK = 0,
+    BLFID_ERR_INVALID = 1,     /* bad argument (BLEXCEPTION cases of the reference) */
+    BLFID_ERR_NOMEM = 2,       /* host or device allocation failed / over the HBM budget */
+    BLFID_ERR_DEVICE = 3,      /* HIP runtime error or no device */
+    BLFID_ERR_UNSUPPORTED = 4, /* valid input outside what this engine implements */
+    BLFID_ERR_NOCONV = 5       /* eigen-solver did not converge */
+} blfid_status;
+
+typedef enum {
+    BLFID_FP64 = 0 /* complex128 everywhere: oneFID<SolidSys, hmatrix> (oneFID.h:52) */
+} blfid_precision;
+
+/* Tensor-described SolidSys (src/QMspins/sys.h:59-62, read by sys.cc:120-204).
+ *   isotope[S]       small integer label per spin; equal labels mean homonuclear
+ *                    (T_Dip's A(on1)==A(on2) test, spin_ten.cc:441)
+ *   csa[7*ncsa]      iso del eta spin alpha beta gamma      (C line, csa.cc:400-440)
+ *   dip[6*ndip]      del spin1 spin2 alpha beta gamma       (D line, dip.cc:285-323)
+ *   jc[4*nj]         iso spin1 spin2 strong                 (J line, j.cc:171-202) */
+typedef struct {
+    int nspins;
+    const int *isotope;
+    int ncsa;
+    const double *csa;
+    int ndip;
+    const double *dip;
+    int nj;
+    const double *jc;
+} blfid_system;
+
+typedef struct blfid_plan blfid_plan;
+
+/* Plan for one spin system and one acquisition: replaces constructing
+ * oneFID(sys, npts, sw, wr, rotor_deg, t0) (oneFID.h:103-108) + compute(sys, wr, sw, t0, t0+1/wr)
+ * (compute.h:213-235).  wr == 0 or rotor_deg == 0 selects the static FID (oneFID.h:216). */
+blfid_status blfid_plan_create(const blfid_system *sys, double wr_hz, double rotor_deg,
+                               double sw_hz, int npts, double t0, blfid_precision prec,
+                               blfid_plan **out);
+void blfid_plan_destroy(blfid_plan *plan);
+
+/* Adjusted sweep width n*wr (compute.h:96, oneFID.h:429), steps per rotor period n, N. */
+double blfid_plan_sw(const blfid_plan *plan);
+int blfid_plan_steps(const blfid_plan *plan);
+int blfid_plan_dim(const blfid_plan *plan);
+
+/* Detection and initial density operators (oneFID::FID(.., ro, det), oneFID.h:200-218). */
+blfid_status blfid_plan_set_operators(blfid_plan *plan, const double *rho, const double *det);
+
+/* Powder set, uploaded to device memory once (the powder class, powder.h:52-162). */
+blfid_status blfid_plan_set_powder(blfid_plan *plan, const double *theta, const double *phi,
+                                   const double *gamma, const double *weight, int K);
+
+/* Run the whole powder on the device, asynchronously on `stream` (a hipStream_t, or NULL for
+ * the default stream).  normalize != 0 divides by Tr(det det^H) as solid-2.0's doFID does
+ * (sequencerun_funcs.cc:698-702); normalize == 0 is oneFID::FID(powder&,..) (oneFID.h:396-402).
+ * The result stays on the device until blfid_plan_fetch. */
+blfid_status blfid_plan_run(blfid_plan *plan, int normalize, void *stream);
+
+/* Wait for the last run and copy its FID (2*npts doubles, re/im interleaved) to host. */
+blfid_status blfid_plan_fetch(blfid_plan *plan, double *fid_out);
+
+/* Device pointer of the last run's FID (2*npts doubles), valid until the next run/destroy. */
+const double *blfid_plan_device_fid(const blfid_plan *plan);
+
+/* One-call form of oneFID::FID(powder&, ro, det): set operators + powder, run, fetch. */
+blfid_status blfid_powder_fid(blfid_plan *plan, const double *theta, const double *phi,
+                              const double *gamma, const double *weight, int K,
+                              const double *rho, const double *det, int normalize,
+                              double *fid_out);
+
+/* Work accounting of the plan for the current operators (per crystallite):
+ * out[0] executed flops per crystallite-step, out[1] reference dense flops per
+ * crystallite-step (48N^3 + 14nN^2 + 14PN^2/n), out[2] number of Hilbert-space blocks,
+ * out[3] largest block, out[4] device bytes per crystallite of working storage. */
+blfid_status blfid_plan_stats(const blfid_plan *plan, double *out5);
+
+/* Limit device working memory of a plan (bytes); crystallites are processed in batches that
+ * fit.  0 = default (a fraction of free HBM). */
+blfid_status blfid_plan_set_memory_budget(blfid_plan *plan, double bytes);
+
+/* Thread-local description of the last error. */
+const char *blfid_last_error(void);
+
+/* Engine version string. */
+const char *blfid_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BLFID_H */

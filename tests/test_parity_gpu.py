@@ -1,0 +1,123 @@
+"""GPU parity: the HIP engine (through the C ABI) against the FP64 CPU oracle.
+
+Tolerance: relative L2 <= 1e-9 for the FP64 engine on these sizes (the engine's exp is a
+Taylor/scaling-squaring evaluation and its eigen-solver is joint Jacobi, so agreement is to
+rounding, not bitwise).  The north-star bar is 1e-5.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from blochlib_amd import engine, workloads
+from blochlib_amd.powder import Powder, zcw
+from blochlib_amd.spinsys import SolidSys
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-9
+MAGIC = workloads.MAGIC_DEG
+
+
+def rel_l2(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def compare(name, sy, wr, rotor, sw, npts, pw, rho, det, t0=0.0, normalize=True, tol=TOL):
+    print(f"[parity] {name}: S={sy.spins} K={len(pw)} wr={wr} sw={sw} npts={npts}", flush=True)
+    g, swg = engine.powder_fid(sy, wr, rotor, sw, npts, pw, rho, det, t0=t0, normalize=normalize)
+    c, swc = O.powder_fid(sy, wr, rotor, sw, npts, pw, rho, det, t0=t0, normalize=normalize)
+    assert swg == swc
+    err = rel_l2(g, c)
+    print(f"[parity] {name}: rel-L2 {err:.3e}", flush=True)
+    assert err < tol, (name, err)
+    return g, c
+
+
+SYS3 = """numspin 3
+T 1H 0
+T 1H 1
+T 13C 2
+C 1000 5000 0.5 0 10 20 30
+C -1000 3000 0.2 1 40 50 60
+C 300 7000 0.9 2
+D -20000 0 1 0 30 40
+D 5000 1 2 10 70 20
+J 140 0 2
+J 50 0 1 1
+"""
+
+
+def test_config1_system_zcw_subset():
+    w = workloads.config1()
+    pw = w.powder().subset(np.arange(0, 1154, 23))
+    rho, det = w.operators()
+    compare("config1-subset", w.system, w.wr, w.rotor, w.sw, w.npts, pw, rho, det)
+
+
+@pytest.mark.parametrize("sw", [80000.0, 70000.0])
+@pytest.mark.parametrize("rho_name", ["Ix", "Iz+0.3*Ix_0"])
+def test_hetero_strongJ_rosym_and_general_rho(sw, rho_name):
+    sy = SolidSys.from_text(SYS3)
+    pw = zcw(1154).subset(np.arange(0, 1154, 97))
+    compare(f"sys3-{rho_name}-sw{sw}", sy, 10000.0, MAGIC, sw, 40, pw, sy.op(rho_name),
+            sy.op("Ip"), normalize=False)
+
+
+def test_start_time_and_other_rotor_angle():
+    sy = SolidSys.from_text(SYS3)
+    pw = zcw(1154).subset(np.arange(5, 1154, 131))
+    compare("sys3-t0", sy, 7000.0, 30.0, 7000.0 * 12, 50, pw, sy.op("Ix"), sy.op("Ip"), t0=3.1e-5)
+
+
+@pytest.mark.parametrize("S", [4, 5, 6])
+def test_clusters(S):
+    sy = workloads.small_cluster(S, 77 + S)
+    pw = zcw(1154).subset(np.arange(0, 1154, 191))
+    compare(f"cluster{S}", sy, 15000.0, MAGIC, 15000.0 * 16, 64, pw, sy.op("Ix"), sy.op("Ip"))
+
+
+@pytest.mark.parametrize("S", [7, 8])
+def test_large_blocks(S):
+    sy = workloads.small_cluster(S, 300 + S)
+    pw = zcw(1154).subset([3, 400])
+    compare(f"cluster{S}", sy, 15000.0, MAGIC, 15000.0 * 8, 24, pw, sy.op("Ix"), sy.op("Ip"))
+
+
+def test_static_path_matches_reference_output():
+    sy = SolidSys.from_text("numspin 2\nT 1H 0\nT 1H 1\nC 5000  2134 0 0\nC -5000 2789 0.5 1\n")
+    g, c = compare("static-simpSTA", sy, 0.0, 0.0, 40000.0, 512, Powder.read("rep2000"),
+                   sy.op("Ix"), sy.op("Ip"))
+    ref = np.loadtxt("tests/golden/simpSTA.txt")
+    fr = ref[:, 4] + 1j * ref[:, 5]
+    assert np.abs(g - fr).max() < 2e-6
+
+
+def test_static_dipolar_rotor_tilted():
+    sy = SolidSys.from_text(SYS3)
+    pw = zcw(1154).subset(np.arange(0, 1154, 61))
+    compare("static-sys3-rotor30", sy, 0.0, 30.0, 50000.0, 64, pw, sy.op("Ix"), sy.op("Ip"))
+
+
+def test_empty_powder_and_single_point():
+    sy = SolidSys.from_text(SYS3)
+    p = engine.Plan(sy, 10000.0, MAGIC, 80000.0, 1)
+    f = p.powder_fid(Powder(np.zeros(0), np.zeros(0), np.zeros(0), np.zeros(0)), sy.op("Ix"),
+                     sy.op("Ip"))
+    assert f.shape == (1,) and f[0] == 0
+    g, c = compare("npts1", sy, 10000.0, MAGIC, 80000.0, 1, zcw(1154).subset([0, 1, 2]),
+                   sy.op("Ix"), sy.op("Ip"))
+    assert np.isclose(g[0], 2.0 * 3 / 1154)
+
+
+def test_repeat_runs_are_consistent_and_batches_agree():
+    w = workloads.config1()
+    pw = w.powder().subset(np.arange(0, 1154, 7))
+    rho, det = w.operators()
+    p = engine.Plan(w.system, w.wr, w.rotor, w.sw, w.npts)
+    a = p.powder_fid(pw, rho, det)
+    p.set_memory_budget(3 * p.stats()["bytes_per_crystallite"])  # forces batches of 3
+    p.run(True)
+    b = p.fetch()
+    assert rel_l2(b, a) < 1e-13
