@@ -149,6 +149,14 @@ struct Pair {
 
 } // namespace
 
+struct blfid_workspace {
+    char *base = nullptr;
+    size_t bytes = 0;
+    hipEvent_t ev = nullptr;
+    hipStream_t last = nullptr;
+    bool used = false;
+};
+
 struct blfid_plan {
     // input
     int S = 0, N = 0;
@@ -186,14 +194,60 @@ struct blfid_plan {
     DevBuf<long> d_blk_uoff, d_pair_eoff;
     DevBuf<double> d_dval, d_rval;
     std::map<int, DevBuf<int>> d_bgrp, d_pgrp;
-    DevBuf<double2> bufU, bufA, bufW, bufQt, bufRt, scrQ;
-    DevBuf<double> thetas, scrE, fidc, acc, comp, fid_out;
+    // working storage (carved from the plan's own arena or a shared workspace)
+    DevBuf<char> own_ws;
+    blfid_workspace *ws = nullptr;
+    double2 *bufU = nullptr, *bufA = nullptr, *bufW = nullptr, *bufQt = nullptr,
+            *bufRt = nullptr, *scrQ = nullptr;
+    double *thetas = nullptr, *scrE = nullptr, *fidc = nullptr;
+    DevBuf<double> acc, comp, fid_out;
     double budget = 0;
     int Kb = 0;
     hipStream_t last_stream = nullptr;
     bool ran = false;
     DevPlan dp{};
+    // profiling
+    bool prof = false;
+    struct Ev { int phase; hipEvent_t a, b; double flops; };
+    std::vector<Ev> pending;
+    double ph_ms[BLFID_NSLOT] = {0}, ph_flops[BLFID_NSLOT] = {0};
+    int ph_n[BLFID_NSLOT] = {0};
+    ~blfid_plan() {
+        for (auto &e : pending) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
+    }
 };
+
+// Records HIP events around one launch on `st` when profiling is on.
+struct PhaseScope {
+    blfid_plan *pl; int phase; double flops; hipStream_t st; hipEvent_t a = nullptr, b = nullptr;
+    PhaseScope(blfid_plan *p, int ph, double fl, hipStream_t s) : pl(p), phase(ph), flops(fl), st(s) {
+        if (pl->prof) {
+            (void)hipEventCreate(&a);
+            (void)hipEventCreate(&b);
+            (void)hipEventRecord(a, st);
+        }
+    }
+    ~PhaseScope() {
+        if (pl->prof) {
+            (void)hipEventRecord(b, st);
+            pl->pending.push_back({phase, a, b, flops});
+        }
+    }
+};
+
+static void collect_profile(blfid_plan *pl) {
+    for (auto &e : pl->pending) {
+        float ms = 0.f;
+        (void)hipEventSynchronize(e.b);
+        (void)hipEventElapsedTime(&ms, e.a, e.b);
+        pl->ph_ms[e.phase] += ms;
+        pl->ph_n[e.phase] += 1;
+        pl->ph_flops[e.phase] += e.flops;
+        (void)hipEventDestroy(e.a);
+        (void)hipEventDestroy(e.b);
+    }
+    pl->pending.clear();
+}
 
 // ------------------------------------------------------------------------------ planning
 static blfid_status build_tables(blfid_plan *pl) {
@@ -620,20 +674,29 @@ blfid_status blfid_plan_set_powder(blfid_plan *pl, const double *theta, const do
     return BLFID_OK;
 }
 
+static double group_b3(const blfid_plan *pl, const std::vector<int> &g) {
+    double s = 0;
+    for (int b : g) s += (double)pl->blk_dim[b] * pl->blk_dim[b] * pl->blk_dim[b];
+    return s;
+}
+static double pair_flops(const blfid_plan *pl, const std::vector<int> &g) {
+    double s = 0;
+    for (int p : g) {
+        double bI = pl->blk_dim[pl->pairs[p].I], bJ = pl->blk_dim[pl->pairs[p].J];
+        s += 8.0 * (bI * bI * bJ + bJ * bJ * bI);
+    }
+    return s;
+}
+
 static const int EXP_SLOTS = 2048;
 static const int QR_SLOTS = 2048;
 
+static size_t layout_bytes(const blfid_plan *pl, int Kb, size_t off[10]);
+static size_t fixed_bytes(const blfid_plan *pl);
+
 static double bytes_per_crystallite(const blfid_plan *pl) {
-    double b = 0;
-    if (pl->spinning) {
-        b += 2.0 * pl->n * pl->Lsum * 16.0;  // U/B and A
-        b += 2.0 * pl->n * pl->Etot * 16.0;  // Qt, Rt
-        b += pl->Lsum * 16.0 + pl->N * 8.0;  // W, thetas
-    } else {
-        b += 2.0 * pl->Lsum * 16.0 + pl->N * 8.0;
-    }
-    b += pl->npts * 16.0;
-    return b;
+    size_t off[10];
+    return (double)(layout_bytes(pl, 1, off) - fixed_bytes(pl));
 }
 
 blfid_status blfid_plan_stats(const blfid_plan *pl, double *out) {
@@ -661,30 +724,56 @@ blfid_status blfid_plan_stats(const blfid_plan *pl, double *out) {
     out[2] = (double)pl->blocks.size();
     out[3] = (double)pl->bmax;
     out[4] = bytes_per_crystallite(pl);
+    out[5] = (double)fixed_bytes(pl) + 4096.0;
     return BLFID_OK;
 }
 
-static blfid_status ensure_buffers(blfid_plan *pl, int Kb) {
-    const size_t L = (size_t)pl->Lsum;
-    const size_t bm2 = (size_t)pl->bmax * pl->bmax;
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// Byte layout of the working storage for a batch of Kb crystallites (fixed scratch first).
+static size_t layout_bytes(const blfid_plan *pl, int Kb, size_t off[10]) {
+    const size_t L = (size_t)pl->Lsum, bm2 = (size_t)pl->bmax * pl->bmax;
+    const size_t c16 = sizeof(double2), d8 = sizeof(double);
+    size_t sz[10];
     if (pl->spinning) {
-        HIPCHK(pl->bufU.alloc((size_t)Kb * pl->n * L));
-        HIPCHK(pl->bufA.alloc((size_t)Kb * pl->n * L));
-        HIPCHK(pl->bufQt.alloc(std::max<size_t>(1, (size_t)Kb * pl->n * pl->Etot)));
-        HIPCHK(pl->bufRt.alloc(std::max<size_t>(1, (size_t)Kb * pl->n * pl->Etot)));
-        HIPCHK(pl->scrE.alloc((size_t)EXP_SLOTS * 5 * bm2));
+        sz[0] = (size_t)Kb * pl->n * L * c16;                              // U / B
+        sz[1] = (size_t)Kb * pl->n * L * c16;                              // A
+        sz[2] = std::max<size_t>(1, (size_t)Kb * pl->n * pl->Etot) * c16;  // Qt
+        sz[3] = std::max<size_t>(1, (size_t)Kb * pl->n * pl->Etot) * c16;  // Rt
+        sz[4] = (size_t)EXP_SLOTS * 5 * bm2 * d8;                         // exp scratch
     } else {
-        HIPCHK(pl->bufU.alloc((size_t)Kb * L));
-        HIPCHK(pl->scrE.alloc((size_t)EXP_SLOTS * bm2));
+        sz[0] = (size_t)Kb * L * c16;
+        sz[1] = 0; sz[2] = 0; sz[3] = 0;
+        sz[4] = (size_t)EXP_SLOTS * bm2 * d8;
     }
-    HIPCHK(pl->bufW.alloc((size_t)Kb * L));
-    HIPCHK(pl->thetas.alloc((size_t)Kb * pl->N));
-    HIPCHK(pl->scrQ.alloc((size_t)QR_SLOTS * 3 * bm2));
-    HIPCHK(pl->fidc.alloc((size_t)Kb * 2 * pl->npts));
-    HIPCHK(pl->acc.alloc((size_t)2 * pl->npts));
-    HIPCHK(pl->comp.alloc((size_t)2 * pl->npts));
-    HIPCHK(pl->fid_out.alloc((size_t)2 * pl->npts));
-    return BLFID_OK;
+    sz[5] = (size_t)Kb * L * c16;                 // W
+    sz[6] = (size_t)Kb * pl->N * d8;              // thetas / eigenvalues
+    sz[7] = (size_t)QR_SLOTS * 3 * bm2 * c16;     // qr scratch
+    sz[8] = (size_t)Kb * 2 * pl->npts * d8;       // per-crystallite FIDs
+    sz[9] = 0;
+    size_t o = 0;
+    for (int i = 0; i < 10; i++) {
+        off[i] = o;
+        o += align256(sz[i]);
+    }
+    return o;
+}
+
+static size_t fixed_bytes(const blfid_plan *pl) {
+    size_t off[10];
+    return layout_bytes(pl, 0, off);
+}
+
+static void carve(blfid_plan *pl, char *base, const size_t off[10]) {
+    pl->bufU = (double2 *)(base + off[0]);
+    pl->bufA = (double2 *)(base + off[1]);
+    pl->bufQt = (double2 *)(base + off[2]);
+    pl->bufRt = (double2 *)(base + off[3]);
+    pl->scrE = (double *)(base + off[4]);
+    pl->bufW = (double2 *)(base + off[5]);
+    pl->thetas = (double *)(base + off[6]);
+    pl->scrQ = (double2 *)(base + off[7]);
+    pl->fidc = (double *)(base + off[8]);
 }
 
 blfid_status blfid_plan_run(blfid_plan *pl, int normalize, void *stream) {
@@ -693,23 +782,37 @@ blfid_status blfid_plan_run(blfid_plan *pl, int normalize, void *stream) {
     hipStream_t st = (hipStream_t)stream;
     pl->last_stream = st;
     const int K = pl->K;
-    // batch size from the memory budget
-    double budget = pl->budget;
-    if (budget <= 0) {
-        size_t fre = 0, tot = 0;
-        HIPCHK(hipMemGetInfo(&fre, &tot));
-        budget = 0.4 * (double)fre;
+    // batch size from the workspace / memory budget
+    size_t off[10];
+    const double per = (double)(layout_bytes(pl, 1, off) - fixed_bytes(pl));
+    const double fixed = (double)fixed_bytes(pl) + 4096.0;
+    double avail;
+    if (pl->ws) {
+        avail = (double)pl->ws->bytes;
+    } else {
+        avail = pl->budget;
+        if (avail <= 0) {
+            size_t fre = 0, tot = 0;
+            HIPCHK(hipMemGetInfo(&fre, &tot));
+            avail = 0.4 * (double)(fre + pl->own_ws.n);
+        }
     }
-    double per = bytes_per_crystallite(pl);
-    int Kb = (int)std::min<double>(std::max(1, K), std::floor(budget / per));
+    int Kb = (int)std::min<double>(std::max(1, K), std::floor((avail - fixed) / per));
     if (Kb < 1) return fail(BLFID_ERR_NOMEM, "one crystallite does not fit the memory budget");
-    if (pl->Kb != Kb) {
-        pl->bufU.release(); pl->bufA.release(); pl->bufQt.release(); pl->bufRt.release();
-        pl->bufW.release(); pl->thetas.release(); pl->fidc.release();
-        pl->Kb = Kb;
+    pl->Kb = Kb;
+    size_t need = layout_bytes(pl, Kb, off);
+    char *base;
+    if (pl->ws) {
+        base = pl->ws->base;
+        if (pl->ws->used && pl->ws->last != st) HIPCHK(hipStreamWaitEvent(st, pl->ws->ev, 0));
+    } else {
+        HIPCHK(pl->own_ws.alloc(need));
+        base = pl->own_ws.p;
     }
-    blfid_status s = ensure_buffers(pl, Kb);
-    if (s != BLFID_OK) return s;
+    carve(pl, base, off);
+    HIPCHK(pl->acc.alloc((size_t)2 * pl->npts));
+    HIPCHK(pl->comp.alloc((size_t)2 * pl->npts));
+    HIPCHK(pl->fid_out.alloc((size_t)2 * pl->npts));
     refresh_devplan(pl);
     const DevPlan *P = &pl->dp;
     LAUNCHCHK(blfid_dev_launch_zero(pl->acc.p, 2 * pl->npts, st));
@@ -719,55 +822,131 @@ blfid_status blfid_plan_run(blfid_plan *pl, int normalize, void *stream) {
     const double scale = pl->spinning ? 1.0 / (double)(pl->n / ff) : 1.0;
     for (int c0 = 0; c0 < K; c0 += Kb) {
         int kb = std::min(Kb, K - c0);
-        LAUNCHCHK(blfid_dev_launch_zero(pl->fidc.p, (size_t)kb * 2 * pl->npts, st));
+        LAUNCHCHK(blfid_dev_launch_zero(pl->fidc, (size_t)kb * 2 * pl->npts, st));
         if (pl->spinning) {
-            for (auto &kv : pl->blk_groups)
+            for (auto &kv : pl->blk_groups) {
+                PhaseScope ps(pl, 0 * 8 + kv.first, kb * (double)pl->n * group_b3(pl, kv.second) * 28.0, st);
                 LAUNCHCHK(blfid_dev_launch_exp(kv.first, P, kb, c0, pl->d_bgrp[kv.first].p,
-                                               (int)kv.second.size(), pl->scrE.p, bm2, EXP_SLOTS,
-                                               pl->bufU.p, st));
-            for (auto &kv : pl->blk_groups)
+                                               (int)kv.second.size(), pl->scrE, bm2, EXP_SLOTS,
+                                               pl->bufU, st));
+            }
+            for (auto &kv : pl->blk_groups) {
+                PhaseScope ps(pl, 1 * 8 + kv.first, kb * (double)(pl->n - 1) * group_b3(pl, kv.second) * 8.0, st);
                 LAUNCHCHK(blfid_dev_launch_dyson(kv.first, P, kb, pl->d_bgrp[kv.first].p,
-                                                 (int)kv.second.size(), pl->bufU.p, pl->bufA.p, st));
-            for (auto &kv : pl->blk_groups)
+                                                 (int)kv.second.size(), pl->bufU, pl->bufA, st));
+            }
+            for (auto &kv : pl->blk_groups) {
+                PhaseScope ps(pl, 2 * 8, 0.0, st);
                 LAUNCHCHK(blfid_dev_launch_eig(P, kb, pl->d_bgrp[kv.first].p, (int)kv.second.size(),
-                                               pl->bufA.p + (size_t)(pl->n - 1) * pl->Lsum,
-                                               (size_t)pl->n * pl->Lsum, 0, pl->bufW.p,
-                                               pl->thetas.p, st));
-            for (auto &kv : pl->blk_groups)
+                                               pl->bufA + (size_t)(pl->n - 1) * pl->Lsum,
+                                               (size_t)pl->n * pl->Lsum, 0, pl->bufW,
+                                               pl->thetas, st));
+            }
+            for (auto &kv : pl->blk_groups) {
+                PhaseScope ps(pl, 3 * 8 + kv.first, kb * (double)pl->n * group_b3(pl, kv.second) * 8.0, st);
                 LAUNCHCHK(blfid_dev_launch_bmul(kv.first, P, kb, pl->d_bgrp[kv.first].p,
-                                                (int)kv.second.size(), pl->bufA.p, pl->bufW.p,
-                                                pl->bufU.p, st));
-            for (auto &kv : pl->pair_groups)
+                                                (int)kv.second.size(), pl->bufA, pl->bufW,
+                                                pl->bufU, st));
+            }
+            for (auto &kv : pl->pair_groups) {
+                PhaseScope ps(pl, 4 * 8 + kv.first, kb * (double)pl->n * pair_flops(pl, kv.second), st);
                 LAUNCHCHK(blfid_dev_launch_qr(kv.first, P, kb, pl->d_pgrp[kv.first].p,
-                                              (int)kv.second.size(), pl->bufU.p, pl->thetas.p,
-                                              pl->scrQ.p, bm2, QR_SLOTS, pl->bufQt.p, pl->bufRt.p,
+                                              (int)kv.second.size(), pl->bufU, pl->thetas,
+                                              pl->scrQ, bm2, QR_SLOTS, pl->bufQt, pl->bufRt,
                                               st));
-            if (pl->Etot > 0)
-                LAUNCHCHK(blfid_dev_launch_corr(P, kb, pl->bufQt.p, pl->bufRt.p, pl->thetas.p,
-                                                pl->d_er.p, pl->d_es.p, pl->fidc.p, st));
+            }
+            if (pl->Etot > 0) {
+                PhaseScope ps(pl, 5 * 8, kb * 8.0 * (double)pl->Etot * ((double)pl->n * pl->n + pl->npts), st);
+                LAUNCHCHK(blfid_dev_launch_corr(P, kb, pl->bufQt, pl->bufRt, pl->thetas,
+                                                pl->d_er.p, pl->d_es.p, pl->fidc, st));
+            }
         } else {
             for (auto &kv : pl->blk_groups)
                 LAUNCHCHK(blfid_dev_launch_static_h(P, kb, c0, pl->d_bgrp[kv.first].p,
-                                                    (int)kv.second.size(), pl->scrE.p, bm2,
-                                                    EXP_SLOTS, pl->bufU.p, st));
+                                                    (int)kv.second.size(), pl->scrE, bm2,
+                                                    EXP_SLOTS, pl->bufU, st));
             for (auto &kv : pl->blk_groups)
                 LAUNCHCHK(blfid_dev_launch_eig(P, kb, pl->d_bgrp[kv.first].p, (int)kv.second.size(),
-                                               pl->bufU.p, (size_t)pl->Lsum, 1, pl->bufW.p,
-                                               pl->thetas.p, st));
-            for (auto &kv : pl->pair_groups)
+                                               pl->bufU, (size_t)pl->Lsum, 1, pl->bufW,
+                                               pl->thetas, st));
+            for (auto &kv : pl->pair_groups) {
+                PhaseScope ps(pl, 7 * 8 + kv.first, 0.0, st);
                 LAUNCHCHK(blfid_dev_launch_static_fid(kv.first, P, kb, pl->d_pgrp[kv.first].p,
-                                                      (int)kv.second.size(), pl->bufW.p,
-                                                      pl->thetas.p, pl->scrQ.p, bm2, QR_SLOTS,
-                                                      1.0 / pl->sw, pl->fidc.p, st));
+                                                      (int)kv.second.size(), pl->bufW,
+                                                      pl->thetas, pl->scrQ, bm2, QR_SLOTS,
+                                                      1.0 / pl->sw, pl->fidc, st));
+            }
         }
-        LAUNCHCHK(blfid_dev_launch_reduce(kb, c0, pl->npts, pl->d_w.p, scale, pl->fidc.p,
-                                          pl->acc.p, pl->comp.p, st));
+        {
+            PhaseScope ps(pl, 6 * 8, 4.0 * kb * 2.0 * pl->npts, st);
+            LAUNCHCHK(blfid_dev_launch_reduce(kb, c0, pl->npts, pl->d_w.p, scale, pl->fidc,
+                                              pl->acc.p, pl->comp.p, st));
+        }
     }
     cplx norm = normalize ? pl->trdd : cplx(1.0, 0.0);
     if (normalize && norm == 0.0) return fail(BLFID_ERR_INVALID, "Tr(det det^H) is zero");
     LAUNCHCHK(blfid_dev_launch_finish(pl->npts, pl->acc.p, norm.real(), norm.imag(),
                                       pl->fid_out.p, st));
+    if (pl->ws) {
+        HIPCHK(hipEventRecord(pl->ws->ev, st));
+        pl->ws->last = st;
+        pl->ws->used = true;
+    }
     pl->ran = true;
+    return BLFID_OK;
+}
+
+blfid_status blfid_workspace_create(double bytes, blfid_workspace **out) {
+    if (!out || bytes <= 0) return fail(BLFID_ERR_INVALID, "bad workspace arguments");
+    *out = nullptr;
+    blfid_workspace *w = new blfid_workspace();
+    w->bytes = (size_t)bytes;
+    hipError_t e = hipMalloc((void **)&w->base, w->bytes);
+    if (e != hipSuccess) {
+        delete w;
+        return fail(e == hipErrorOutOfMemory ? BLFID_ERR_NOMEM : BLFID_ERR_DEVICE,
+                    std::string("workspace hipMalloc: ") + hipGetErrorString(e));
+    }
+    e = hipEventCreateWithFlags(&w->ev, hipEventDisableTiming);
+    if (e != hipSuccess) {
+        (void)hipFree(w->base);
+        delete w;
+        return fail(BLFID_ERR_DEVICE, "workspace event");
+    }
+    *out = w;
+    return BLFID_OK;
+}
+
+void blfid_workspace_destroy(blfid_workspace *w) {
+    if (!w) return;
+    (void)hipDeviceSynchronize();
+    (void)hipEventDestroy(w->ev);
+    (void)hipFree(w->base);
+    delete w;
+}
+
+blfid_status blfid_plan_set_workspace(blfid_plan *pl, blfid_workspace *w) {
+    if (!pl) return fail(BLFID_ERR_INVALID, "plan is NULL");
+    pl->ws = w;
+    if (w) pl->own_ws.release();
+    return BLFID_OK;
+}
+
+blfid_status blfid_plan_set_profiling(blfid_plan *pl, int on) {
+    if (!pl) return fail(BLFID_ERR_INVALID, "plan is NULL");
+    pl->prof = on != 0;
+    for (int i = 0; i < BLFID_NSLOT; i++) { pl->ph_ms[i] = 0; pl->ph_n[i] = 0; pl->ph_flops[i] = 0; }
+    return BLFID_OK;
+}
+
+blfid_status blfid_plan_phase_times(blfid_plan *pl, double *ms, int *launches, double *flops) {
+    if (!pl || !ms || !launches || !flops) return fail(BLFID_ERR_INVALID, "NULL argument");
+    collect_profile(pl);
+    for (int i = 0; i < BLFID_NSLOT; i++) {
+        ms[i] = pl->ph_ms[i];
+        launches[i] = pl->ph_n[i];
+        flops[i] = pl->ph_flops[i];
+    }
     return BLFID_OK;
 }
 

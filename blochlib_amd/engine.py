@@ -71,6 +71,12 @@ def lib():
         L.blfid_powder_fid.argtypes = [vp, _d, _d, _d, _d, ctypes.c_int, _d, _d, ctypes.c_int, _d]
         L.blfid_plan_stats.argtypes = [vp, _d]
         L.blfid_plan_set_memory_budget.argtypes = [vp, ctypes.c_double]
+        L.blfid_workspace_create.argtypes = [ctypes.c_double, ctypes.POINTER(vp)]
+        L.blfid_workspace_destroy.argtypes = [vp]
+        L.blfid_workspace_destroy.restype = None
+        L.blfid_plan_set_workspace.argtypes = [vp, vp]
+        L.blfid_plan_set_profiling.argtypes = [vp, ctypes.c_int]
+        L.blfid_plan_phase_times.argtypes = [vp, _d, _i, _d]
         L.blfid_last_error.restype = ctypes.c_char_p
         L.blfid_version.restype = ctypes.c_char_p
         _lib = L
@@ -91,6 +97,27 @@ def _cm(M, N):
     if M.shape != (N, N):
         raise BlochLibError(f"operator must be {N}x{N}, got {M.shape}")
     return np.ascontiguousarray(M.ravel(order="F").view(np.float64))
+
+
+class Workspace:
+    """Device working storage shared by plans (blfid_workspace_*)."""
+
+    def __init__(self, nbytes: float):
+        h = ctypes.c_void_p()
+        _check(lib().blfid_workspace_create(float(nbytes), ctypes.byref(h)))
+        self.h = h
+        self.nbytes = float(nbytes)
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            lib().blfid_workspace_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Plan:
@@ -141,6 +168,10 @@ class Plan:
                     (pw.theta, pw.phi, pw.gamma, pw.weight)]
         _check(lib().blfid_plan_set_powder(self.h, *[_p(a) for a in self._pw], len(pw)))
 
+    def set_workspace(self, ws: "Workspace | None"):
+        self._ws = ws
+        _check(lib().blfid_plan_set_workspace(self.h, ws.h if ws is not None else None))
+
     def set_memory_budget(self, nbytes: float):
         _check(lib().blfid_plan_set_memory_budget(self.h, float(nbytes)))
 
@@ -157,10 +188,33 @@ class Plan:
         return lib().blfid_plan_device_fid(self.h) or 0
 
     def stats(self) -> dict:
-        out = np.zeros(5)
+        out = np.zeros(6)
         _check(lib().blfid_plan_stats(self.h, _p(out)))
         return {"flops_per_cs": out[0], "dense_flops_per_cs": out[1], "blocks": int(out[2]),
-                "max_block": int(out[3]), "bytes_per_crystallite": out[4]}
+                "max_block": int(out[3]), "bytes_per_crystallite": out[4],
+                "fixed_bytes": out[5]}
+
+    PHASES = ("exp", "dyson", "eig", "bmul", "qr", "corr", "reduce", "static")
+    KERNELS = ("k_exp", "k_dyson", "k_eig", "k_bmul", "k_qr", "k_corr", "k_reduce",
+               "k_static_fid")
+
+    def set_profiling(self, on: bool = True):
+        _check(lib().blfid_plan_set_profiling(self.h, int(bool(on))))
+
+    def phase_times(self) -> dict:
+        """{kernel name: {ms, launches, flops}} for every slot that ran, e.g. 'k_exp<5>'."""
+        ms = np.zeros(64)
+        n = np.zeros(64, dtype=np.int32)
+        fl = np.zeros(64)
+        _check(lib().blfid_plan_phase_times(self.h, _p(ms), _p(n, _i), _p(fl)))
+        out = {}
+        for slot in range(64):
+            if n[slot]:
+                ph, M = divmod(slot, 8)
+                name = self.KERNELS[ph] + (f"<{M}>" if M else "")
+                out[name] = {"ms": float(ms[slot]), "launches": int(n[slot]),
+                             "flops": float(fl[slot])}
+        return out
 
     def powder_fid(self, pw: Powder, rho, det, normalize=True) -> np.ndarray:
         self.set_operators(rho, det)

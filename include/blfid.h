@@ -102,12 +102,36 @@ blfid_status blfid_powder_fid(blfid_plan *plan, const double *theta, const doubl
 /* Work accounting of the plan for the current operators (per crystallite):
  * out[0] executed flops per crystallite-step, out[1] reference dense flops per
  * crystallite-step (48N^3 + 14nN^2 + 14PN^2/n), out[2] number of Hilbert-space blocks,
- * out[3] largest block, out[4] device bytes per crystallite of working storage. */
-blfid_status blfid_plan_stats(const blfid_plan *plan, double *out5);
+ * out[3] largest block, out[4] device bytes per crystallite of working storage,
+ * out[5] fixed device bytes of working storage independent of the batch size. */
+blfid_status blfid_plan_stats(const blfid_plan *plan, double *out6);
 
 /* Limit device working memory of a plan (bytes); crystallites are processed in batches that
  * fit.  0 = default (a fraction of free HBM). */
 blfid_status blfid_plan_set_memory_budget(blfid_plan *plan, double bytes);
+
+/* Device workspace shared by several plans (crystallite batches are sized to fit it).  A run
+ * on a stream other than the workspace's previous user first waits for that user's work, so
+ * plans sharing a workspace serialise safely.  Without a workspace a plan grows its own,
+ * sized by blfid_plan_set_memory_budget.  The workspace must outlive the plans using it. */
+typedef struct blfid_workspace blfid_workspace;
+blfid_status blfid_workspace_create(double bytes, blfid_workspace **out);
+void blfid_workspace_destroy(blfid_workspace *ws);
+blfid_status blfid_plan_set_workspace(blfid_plan *plan, blfid_workspace *ws);
+
+/* Per-kernel timing with HIP events recorded on the run's stream around every launch
+ * (enable before blfid_plan_run; costs a few microseconds per launch).  Phases:
+ *   0 exp (H build + propagator)   1 dyson   2 period eig   3 B = A W   4 Q/R similarity
+ *   5 correlation + FID            6 powder reduce            7 static FID
+ * Templated kernels come in tile variants M = 1..5 (16M x 16M output tiles).
+ * blfid_plan_phase_times waits for the last run and returns, per slot phase*8 + M (M = 0 for
+ * untemplated kernels), the summed kernel time in ms, the number of launches and the executed
+ * algorithmic FP64 flops, accumulated since profiling was enabled.  Arrays hold BLFID_NSLOT
+ * entries. */
+#define BLFID_NPHASE 8
+#define BLFID_NSLOT 64
+blfid_status blfid_plan_set_profiling(blfid_plan *plan, int on);
+blfid_status blfid_plan_phase_times(blfid_plan *plan, double *ms, int *launches, double *flops);
 
 /* Thread-local description of the last error. */
 const char *blfid_last_error(void);

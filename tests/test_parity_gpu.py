@@ -117,7 +117,31 @@ def test_repeat_runs_are_consistent_and_batches_agree():
     rho, det = w.operators()
     p = engine.Plan(w.system, w.wr, w.rotor, w.sw, w.npts)
     a = p.powder_fid(pw, rho, det)
-    p.set_memory_budget(3 * p.stats()["bytes_per_crystallite"])  # forces batches of 3
+    st = p.stats()
+    p.set_memory_budget(st["fixed_bytes"] + 3.5 * st["bytes_per_crystallite"])  # batches of 3
     p.run(True)
     b = p.fetch()
     assert rel_l2(b, a) < 1e-13
+
+
+def test_shared_workspace_two_plans_small_batches():
+    """Two plans borrowing one workspace small enough to force several crystallite batches."""
+    w1 = workloads.config1()
+    sy2 = workloads.small_cluster(5, 5)
+    pw = zcw(1154).subset(np.arange(0, 1154, 53))
+    p1 = engine.Plan(w1.system, w1.wr, w1.rotor, w1.sw, w1.npts)
+    p2 = engine.Plan(sy2, 15000.0, MAGIC, 15000.0 * 16, 64)
+    need = max(p1.stats()["bytes_per_crystallite"], p2.stats()["bytes_per_crystallite"])
+    fixed = max(p1.stats()["fixed_bytes"], p2.stats()["fixed_bytes"])
+    ws = engine.Workspace(fixed + 4.5 * need)
+    for p, sy, (wr, sw, npts) in [(p1, w1.system, (w1.wr, w1.sw, w1.npts)),
+                                  (p2, sy2, (15000.0, 15000.0 * 16, 64))]:
+        p.set_workspace(ws)
+        got = p.powder_fid(pw, sy.op("Ix"), sy.op("Ip"))
+        ref, _ = O.powder_fid(sy, wr, MAGIC if p is p2 else w1.rotor, sw, npts, pw, sy.op("Ix"),
+                              sy.op("Ip"))
+        print(f"[parity] workspace plan S={sy.spins}: rel-L2 {rel_l2(got, ref):.3e}", flush=True)
+        assert rel_l2(got, ref) < TOL
+    p1.close()
+    p2.close()
+    ws.close()
