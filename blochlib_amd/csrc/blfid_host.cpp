@@ -41,8 +41,8 @@ int blfid_dev_launch_eig(const DevPlan *P, int Kb, const int *grp, int ngrp, con
 int blfid_dev_launch_bmul(int M, const DevPlan *P, int Kb, const int *grp, int ngrp,
                           const double2 *bufA, const double2 *bufW, double2 *bufB, hipStream_t st);
 int blfid_dev_launch_qr(int M, const DevPlan *P, int Kb, const int *pgrp, int ngrp,
-                        const double2 *bufB, const double *thetas, double2 *scratch, size_t slot,
-                        int nslots, double2 *Qt, double2 *Rt, hipStream_t st);
+                        const double2 *bufB, double2 *scratch, size_t slot, int nslots,
+                        double2 *Qt, double2 *Rt, hipStream_t st);
 int blfid_dev_launch_corr(const DevPlan *P, int Kb, const double2 *Qt, const double2 *Rt,
                           const double *thetas, const int *er, const int *es, double *fidc,
                           hipStream_t st);
@@ -712,7 +712,7 @@ blfid_status blfid_plan_stats(const blfid_plan *pl, double *out) {
     }
     double ex = 0;
     if (pl->spinning) {
-        ex += 2.0 * 14.0 * s3;  // exp: 14 real GEMMs (no squaring counted)
+        ex += 2.0 * 7.0 * s3;   // exp: 7 real GEMMs (no squaring counted)
         ex += 8.0 * s3;         // Dyson
         ex += 8.0 * s3;         // B = A W
         ex += 8.0 * pairsQ;     // Q and R
@@ -740,7 +740,7 @@ static size_t layout_bytes(const blfid_plan *pl, int Kb, size_t off[10]) {
         sz[1] = (size_t)Kb * pl->n * L * c16;                              // A
         sz[2] = std::max<size_t>(1, (size_t)Kb * pl->n * pl->Etot) * c16;  // Qt
         sz[3] = std::max<size_t>(1, (size_t)Kb * pl->n * pl->Etot) * c16;  // Rt
-        sz[4] = (size_t)EXP_SLOTS * 5 * bm2 * d8;                         // exp scratch
+        sz[4] = (size_t)EXP_SLOTS * 9 * bm2 * d8;                         // exp scratch
     } else {
         sz[0] = (size_t)Kb * L * c16;
         sz[1] = 0; sz[2] = 0; sz[3] = 0;
@@ -825,7 +825,7 @@ blfid_status blfid_plan_run(blfid_plan *pl, int normalize, void *stream) {
         LAUNCHCHK(blfid_dev_launch_zero(pl->fidc, (size_t)kb * 2 * pl->npts, st));
         if (pl->spinning) {
             for (auto &kv : pl->blk_groups) {
-                PhaseScope ps(pl, 0 * 8 + kv.first, kb * (double)pl->n * group_b3(pl, kv.second) * 28.0, st);
+                PhaseScope ps(pl, 0 * 8 + kv.first, kb * (double)pl->n * group_b3(pl, kv.second) * 14.0, st);
                 LAUNCHCHK(blfid_dev_launch_exp(kv.first, P, kb, c0, pl->d_bgrp[kv.first].p,
                                                (int)kv.second.size(), pl->scrE, bm2, EXP_SLOTS,
                                                pl->bufU, st));
@@ -851,9 +851,8 @@ blfid_status blfid_plan_run(blfid_plan *pl, int normalize, void *stream) {
             for (auto &kv : pl->pair_groups) {
                 PhaseScope ps(pl, 4 * 8 + kv.first, kb * (double)pl->n * pair_flops(pl, kv.second), st);
                 LAUNCHCHK(blfid_dev_launch_qr(kv.first, P, kb, pl->d_pgrp[kv.first].p,
-                                              (int)kv.second.size(), pl->bufU, pl->thetas,
-                                              pl->scrQ, bm2, QR_SLOTS, pl->bufQt, pl->bufRt,
-                                              st));
+                                              (int)kv.second.size(), pl->bufU, pl->scrQ, bm2,
+                                              QR_SLOTS, pl->bufQt, pl->bufRt, st));
             }
             if (pl->Etot > 0) {
                 PhaseScope ps(pl, 5 * 8, kb * 8.0 * (double)pl->Etot * ((double)pl->n * pl->n + pl->npts), st);

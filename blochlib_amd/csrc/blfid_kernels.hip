@@ -67,67 +67,125 @@ template <> struct Arith<cd> {
     static __device__ __forceinline__ cd add(cd a, cd b) { return cadd(a, b); }
 };
 
-// ---------------------------------------------------------------- workgroup GEMM (256 thr)
+// ---------------------------------------------------------------- workgroup GEMM
 // C[m x n] = op(A)[m x k] * op(B)[k x n] (+ diag * I); op = N or conjugate transpose.
-// Output tiles of 16M x 16M, thread (tx,ty) of a 16x16 grid owns rows tx+16i, cols ty+16j.
-// K is walked in slices of 16 staged through LDS with zero fill, so unpadded sizes work.
-// C must not alias A or B.  `smem` holds 2 * 16 * 16M elements of T.
-template <int M, typename T>
+// NT threads (256 or 64) form a 16 x (NT/16) grid; output tiles are 16M x 16M, thread (tx,ty)
+// owns rows tx + 16 i and columns ty + (NT/16) j.  K is walked in slices of 16 staged through
+// LDS with zero fill (unpadded sizes work); the next slice is prefetched into registers while
+// the current one is consumed.  C must not alias A or B.  `smem` holds 2*16*16M elements.
+template <int M, int NT, typename T>
+__device__ __forceinline__ void gemm_load_slice(int m, int n, int k, int i0, int j0, int k0,
+                                                const T *__restrict__ A, int lda, bool ca,
+                                                const T *__restrict__ B, int ldb, bool cb,
+                                                T (&ra)[16 * M * 16 / NT], T (&rb)[16 * M * 16 / NT]) {
+    constexpr int TM = 16 * M, KS = 16, LPT = TM * KS / NT;
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int l = 0; l < LPT; l++) {
+        int idx = tid + NT * l;
+        int r, q;
+        if (!ca) { r = idx % TM; q = idx / TM; }
+        else { q = idx % KS; r = idx / KS; }
+        int gr = i0 + r, gq = k0 + q;
+        T v = Arith<T>::zero();
+        if (gr < m && gq < k)
+            v = ca ? Arith<T>::cj(A[gq + (size_t)gr * lda]) : A[gr + (size_t)gq * lda];
+        ra[l] = v;
+    }
+#pragma unroll
+    for (int l = 0; l < LPT; l++) {
+        int idx = tid + NT * l;
+        int c, q;
+        if (!cb) { q = idx % KS; c = idx / KS; }
+        else { c = idx % TM; q = idx / TM; }
+        int gc = j0 + c, gq = k0 + q;
+        T v = Arith<T>::zero();
+        if (gc < n && gq < k)
+            v = cb ? Arith<T>::cj(B[gc + (size_t)gq * ldb]) : B[gq + (size_t)gc * ldb];
+        rb[l] = v;
+    }
+}
+
+template <int M, int NT, typename T>
 __device__ void wg_gemm(int m, int n, int k, const T *__restrict__ A, int lda, bool ca,
                         const T *__restrict__ B, int ldb, bool cb, T *__restrict__ C, int ldc,
                         T diag, T *smem) {
-    constexpr int TM = 16 * M, KS = 16;
+    constexpr int TM = 16 * M, KS = 16, TY = NT / 16, MJ = TM / TY, LPT = TM * KS / NT;
+    // register prefetch of the next K slice unless the accumulators already fill the VGPRs
+    constexpr bool PF = (M * MJ * sizeof(T)) <= 256;
     T *As = smem;
     T *Bs = smem + KS * TM;
     const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
     for (int i0 = 0; i0 < m; i0 += TM) {
         for (int j0 = 0; j0 < n; j0 += TM) {
-            T acc[M][M];
+            T acc[M][MJ];
 #pragma unroll
             for (int i = 0; i < M; i++)
 #pragma unroll
-                for (int j = 0; j < M; j++) acc[i][j] = Arith<T>::zero();
+                for (int j = 0; j < MJ; j++) acc[i][j] = Arith<T>::zero();
+            T ra[PF ? LPT : 1], rb[PF ? LPT : 1];
+            if constexpr (PF)
+                gemm_load_slice<M, NT, T>(m, n, k, i0, j0, 0, A, lda, ca, B, ldb, cb, ra, rb);
             for (int k0 = 0; k0 < k; k0 += KS) {
                 __syncthreads();
-                for (int idx = tid; idx < TM * KS; idx += 256) {
-                    int r, q;
-                    if (!ca) { r = idx % TM; q = idx / TM; }
-                    else { q = idx % KS; r = idx / KS; }
-                    int gr = i0 + r, gq = k0 + q;
-                    T v = Arith<T>::zero();
-                    if (gr < m && gq < k)
-                        v = ca ? Arith<T>::cj(A[gq + (size_t)gr * lda]) : A[gr + (size_t)gq * lda];
-                    As[q * TM + r] = v;
-                }
-                for (int idx = tid; idx < TM * KS; idx += 256) {
-                    int c, q;
-                    if (!cb) { q = idx % KS; c = idx / KS; }
-                    else { c = idx % TM; q = idx / TM; }
-                    int gc = j0 + c, gq = k0 + q;
-                    T v = Arith<T>::zero();
-                    if (gc < n && gq < k)
-                        v = cb ? Arith<T>::cj(B[gc + (size_t)gq * ldb]) : B[gq + (size_t)gc * ldb];
-                    Bs[q * TM + c] = v;
+                if constexpr (PF) {
+#pragma unroll
+                    for (int l = 0; l < LPT; l++) {
+                        int idx = tid + NT * l;
+                        int r, q;
+                        if (!ca) { r = idx % TM; q = idx / TM; }
+                        else { q = idx % KS; r = idx / KS; }
+                        As[q * TM + r] = ra[l];
+                        int c2, q2;
+                        if (!cb) { q2 = idx % KS; c2 = idx / KS; }
+                        else { c2 = idx % TM; q2 = idx / TM; }
+                        Bs[q2 * TM + c2] = rb[l];
+                    }
+                } else {
+                    for (int idx = tid; idx < TM * KS; idx += NT) {
+                        int r, q;
+                        if (!ca) { r = idx % TM; q = idx / TM; }
+                        else { q = idx % KS; r = idx / KS; }
+                        int gr = i0 + r, gq = k0 + q;
+                        T v = Arith<T>::zero();
+                        if (gr < m && gq < k)
+                            v = ca ? Arith<T>::cj(A[gq + (size_t)gr * lda]) : A[gr + (size_t)gq * lda];
+                        As[q * TM + r] = v;
+                    }
+                    for (int idx = tid; idx < TM * KS; idx += NT) {
+                        int c, q;
+                        if (!cb) { q = idx % KS; c = idx / KS; }
+                        else { c = idx % TM; q = idx / TM; }
+                        int gc = j0 + c, gq = k0 + q;
+                        T v = Arith<T>::zero();
+                        if (gc < n && gq < k)
+                            v = cb ? Arith<T>::cj(B[gc + (size_t)gq * ldb]) : B[gq + (size_t)gc * ldb];
+                        Bs[q * TM + c] = v;
+                    }
                 }
                 __syncthreads();
+                if constexpr (PF) {
+                    if (k0 + KS < k)
+                        gemm_load_slice<M, NT, T>(m, n, k, i0, j0, k0 + KS, A, lda, ca, B, ldb, cb, ra, rb);
+                }
 #pragma unroll 4
                 for (int q = 0; q < KS; q++) {
-                    T a[M], b[M];
+                    T a[M], b[MJ];
 #pragma unroll
                     for (int i = 0; i < M; i++) a[i] = As[q * TM + tx + 16 * i];
 #pragma unroll
-                    for (int j = 0; j < M; j++) b[j] = Bs[q * TM + ty + 16 * j];
+                    for (int j = 0; j < MJ; j++) b[j] = Bs[q * TM + ty + TY * j];
 #pragma unroll
                     for (int i = 0; i < M; i++)
 #pragma unroll
-                        for (int j = 0; j < M; j++) Arith<T>::mac(acc[i][j], a[i], b[j]);
+                        for (int j = 0; j < MJ; j++) Arith<T>::mac(acc[i][j], a[i], b[j]);
                 }
             }
 #pragma unroll
             for (int i = 0; i < M; i++)
 #pragma unroll
-                for (int j = 0; j < M; j++) {
-                    int r = i0 + tx + 16 * i, c = j0 + ty + 16 * j;
+                for (int j = 0; j < MJ; j++) {
+                    int r = i0 + tx + 16 * i, c = j0 + ty + TY * j;
                     if (r < m && c < n) {
                         T v = acc[i][j];
                         if (r == c) v = Arith<T>::add(v, diag);
@@ -138,6 +196,9 @@ __device__ void wg_gemm(int m, int n, int k, const T *__restrict__ A, int lda, b
     }
     __syncthreads();
 }
+
+// Threads per workgroup for a tile variant: 64 for the 16 x 16 tiles of small blocks.
+template <int M> struct NThr { static constexpr int v = (M == 1) ? 64 : 256; };
 
 // ------------------------------------------------------------------------------ k_prep
 // cartPowder = rotationMatrix3D(phi, theta, gamma) (space_ten.cc:578-618, set by
@@ -219,40 +280,50 @@ __device__ void build_h_block(const DevPlan &P, const double *RPc, const double 
 
 // ------------------------------------------------------------------------------ k_exp
 // One task = (crystallite c, step k, block) of a block-size group.  U = exp(-i theta H),
-// theta = 2 pi tau, as cos(X) - i sin(X) with X = theta H / 2^s, ||X||_inf <= 1/2,
-// cos and sin/x by Horner in Y = X^2 to degree 7 (Taylor error < 1e-18), then s squarings
-// (C,S) -> (C^2 - S^2, 2 C S).
+// theta = 2 pi tau, as cos(X) - i sin(X) with X = theta H / 2^s, ||X||_inf <= 1/2 and the
+// degree-15 Taylor pair evaluated by Paterson-Stockmeyer in Y = X^2:
+//   p(Y) = (a0 + a1 Y + a2 Y^2 + a3 Y^3) + Y^4 (a4 + a5 Y + a6 Y^2 + a7 Y^3)
+// for cos (a_j = (-1)^j/(2j)!) and sin(X)/X (a_j = (-1)^j/(2j+1)!): 7 real GEMMs
+// (Y, Y^2, Y^3, Y^4, two Y^4 products, X * sinc), truncation error < 1e-18, then s squarings
+// (C,S) -> (C^2 - S^2, 2 C S).  Nine b x b scratch matrices per workgroup slot.
 template <int M>
-__global__ void __launch_bounds__(256) k_exp(DevPlan P, int Kb, int c0, const int *__restrict__ grp,
-                                             int ngrp, double *__restrict__ scratch, size_t slot,
-                                             cd *__restrict__ bufU) {
+__global__ void __launch_bounds__(NThr<M>::v) k_exp(DevPlan P, int Kb, int c0,
+                                                    const int *__restrict__ grp, int ngrp,
+                                                    double *__restrict__ scratch, size_t slot,
+                                                    cd *__restrict__ bufU) {
+    constexpr int NT = NThr<M>::v;
     __shared__ double smem[2 * 16 * 16 * M];
     __shared__ double s_zz[DEV_MAX_TERMS];
     __shared__ double s_ff[DEV_MAX_SPINS * DEV_MAX_SPINS];
     __shared__ double s_v[4];
-    __shared__ double s_red[256];
+    __shared__ double s_red[NT];
     const int tid = threadIdx.x;
-    double *X = scratch + slot * 5 * blockIdx.x;
-    double *Y = X + slot, *Rc = Y + slot, *Rs = Rc + slot, *T = Rs + slot;
+    double *X = scratch + slot * 9 * blockIdx.x;
+    double *Y = X + slot, *Y2 = Y + slot, *Y3 = Y2 + slot, *Y4 = Y3 + slot;
+    double *Hc = Y4 + slot, *Hs = Hc + slot, *Cm = Hs + slot, *Ps = Cm + slot;
     const long ntask = (long)Kb * P.n * ngrp;
     const double theta = P.theta;
+    const double cc[8] = {1.0, -1.0 / 2, 1.0 / 24, -1.0 / 720, 1.0 / 40320, -1.0 / 3628800,
+                          1.0 / 479001600, -1.0 / 87178291200.0};
+    const double ss[8] = {1.0, -1.0 / 6, 1.0 / 120, -1.0 / 5040, 1.0 / 362880,
+                          -1.0 / 39916800, 1.0 / 6227020800.0, -1.0 / 1307674368000.0};
     for (long task = blockIdx.x; task < ntask; task += gridDim.x) {
         const int g = (int)(task % ngrp);
         const long ck = task / ngrp;
         const int k = (int)(ck % P.n), c = (int)(ck / P.n);
         const int blk = grp[g];
         const int b = P.blk_dim[blk];
+        const int bb = b * b;
         build_h_block(P, P.RP + 9 * (size_t)(c0 + c), P.u3 + 3 * k, blk, X, b, s_zz, s_ff, s_v);
-        // inf-norm
         double rmax = 0.0;
-        for (int r = tid; r < b; r += 256) {
-            double s = 0.0;
-            for (int q = 0; q < b; q++) s += fabs(X[r + (size_t)q * b]);
-            rmax = fmax(rmax, s);
+        for (int r = tid; r < b; r += NT) {
+            double sm = 0.0;
+            for (int q = 0; q < b; q++) sm += fabs(X[r + (size_t)q * b]);
+            rmax = fmax(rmax, sm);
         }
         s_red[tid] = rmax;
         __syncthreads();
-        for (int w = 128; w > 0; w >>= 1) {
+        for (int w = NT / 2; w > 0; w >>= 1) {
             if (tid < w) s_red[tid] = fmax(s_red[tid], s_red[tid + w]);
             __syncthreads();
         }
@@ -261,48 +332,42 @@ __global__ void __launch_bounds__(256) k_exp(DevPlan P, int Kb, int c0, const in
         int sq = 0;
         while (nrm / (double)(1 << sq) > 0.5 && sq < 30) sq++;
         const double sc = theta / (double)(1 << sq);
-        for (int e = tid; e < b * b; e += 256) X[e] *= sc;
+        for (int e = tid; e < bb; e += NT) X[e] *= sc;
         __syncthreads();
-        wg_gemm<M, double>(b, b, b, X, b, false, X, b, false, Y, b, 0.0, smem);
-        // cos: c_j = (-1)^j / (2j)!,  sinc: s_j = (-1)^j / (2j+1)!
-        const double cc[8] = {1.0, -1.0 / 2, 1.0 / 24, -1.0 / 720, 1.0 / 40320, -1.0 / 3628800,
-                              1.0 / 479001600, -1.0 / 87178291200.0};
-        const double ss[8] = {1.0, -1.0 / 6, 1.0 / 120, -1.0 / 5040, 1.0 / 362880,
-                              -1.0 / 39916800, 1.0 / 6227020800.0, -1.0 / 1307674368000.0};
-        for (int e = tid; e < b * b; e += 256) {
-            int r = e % b, q = e / b;
-            double y = Y[e], d = (r == q) ? 1.0 : 0.0;
-            Rc[e] = cc[7] * y + cc[6] * d;
-            Rs[e] = ss[7] * y + ss[6] * d;
+        wg_gemm<M, NT, double>(b, b, b, X, b, false, X, b, false, Y, b, 0.0, smem);
+        wg_gemm<M, NT, double>(b, b, b, Y, b, false, Y, b, false, Y2, b, 0.0, smem);
+        wg_gemm<M, NT, double>(b, b, b, Y2, b, false, Y, b, false, Y3, b, 0.0, smem);
+        wg_gemm<M, NT, double>(b, b, b, Y2, b, false, Y2, b, false, Y4, b, 0.0, smem);
+        for (int e = tid; e < bb; e += NT) {
+            double d = (e % b == e / b) ? 1.0 : 0.0;
+            double y = Y[e], y2 = Y2[e], y3 = Y3[e];
+            Hc[e] = cc[4] * d + cc[5] * y + cc[6] * y2 + cc[7] * y3;
+            Hs[e] = ss[4] * d + ss[5] * y + ss[6] * y2 + ss[7] * y3;
         }
         __syncthreads();
-        for (int j = 5; j >= 0; j--) {
-            wg_gemm<M, double>(b, b, b, Rc, b, false, Y, b, false, T, b, cc[j], smem);
-            double *tmp = Rc; Rc = T; T = tmp;
+        wg_gemm<M, NT, double>(b, b, b, Y4, b, false, Hc, b, false, Cm, b, 0.0, smem);
+        wg_gemm<M, NT, double>(b, b, b, Y4, b, false, Hs, b, false, Ps, b, 0.0, smem);
+        for (int e = tid; e < bb; e += NT) {
+            double d = (e % b == e / b) ? 1.0 : 0.0;
+            double y = Y[e], y2 = Y2[e], y3 = Y3[e];
+            Cm[e] += cc[0] * d + cc[1] * y + cc[2] * y2 + cc[3] * y3;
+            Ps[e] += ss[0] * d + ss[1] * y + ss[2] * y2 + ss[3] * y3;
         }
-        for (int j = 5; j >= 0; j--) {
-            wg_gemm<M, double>(b, b, b, Rs, b, false, Y, b, false, T, b, ss[j], smem);
-            double *tmp = Rs; Rs = T; T = tmp;
-        }
-        // S = X * Rs -> T
-        wg_gemm<M, double>(b, b, b, X, b, false, Rs, b, false, T, b, 0.0, smem);
-        double *Cm = Rc, *Sm = T;
-        double *f1 = X, *f2 = Y, *f3 = Rs;
+        __syncthreads();
+        double *Sm = Y4;
+        wg_gemm<M, NT, double>(b, b, b, X, b, false, Ps, b, false, Sm, b, 0.0, smem);
         for (int it = 0; it < sq; it++) {
-            wg_gemm<M, double>(b, b, b, Cm, b, false, Cm, b, false, f1, b, 0.0, smem);
-            wg_gemm<M, double>(b, b, b, Sm, b, false, Sm, b, false, f2, b, 0.0, smem);
-            wg_gemm<M, double>(b, b, b, Cm, b, false, Sm, b, false, f3, b, 0.0, smem);
-            for (int e = tid; e < b * b; e += 256) {
-                Cm[e] = f1[e] - f2[e];
-                Sm[e] = 2.0 * f3[e];
+            wg_gemm<M, NT, double>(b, b, b, Cm, b, false, Cm, b, false, Y, b, 0.0, smem);
+            wg_gemm<M, NT, double>(b, b, b, Sm, b, false, Sm, b, false, Y2, b, 0.0, smem);
+            wg_gemm<M, NT, double>(b, b, b, Cm, b, false, Sm, b, false, Y3, b, 0.0, smem);
+            for (int e = tid; e < bb; e += NT) {
+                Cm[e] = Y[e] - Y2[e];
+                Sm[e] = 2.0 * Y3[e];
             }
             __syncthreads();
         }
         cd *U = bufU + ((size_t)c * P.n + k) * P.Lsum + P.blk_uoff[blk];
-        for (int e = tid; e < b * b; e += 256) U[e] = cmk(Cm[e], -Sm[e]);
-        // restore the scratch pointer order for the next task
-        X = scratch + slot * 5 * blockIdx.x;
-        Y = X + slot; Rc = Y + slot; Rs = Rc + slot; T = Rs + slot;
+        for (int e = tid; e < bb; e += NT) U[e] = cmk(Cm[e], -Sm[e]);
         __syncthreads();
     }
 }
@@ -310,9 +375,10 @@ __global__ void __launch_bounds__(256) k_exp(DevPlan P, int Kb, int c0, const in
 // ------------------------------------------------------------------------------ k_dyson
 // One task = (crystallite, block) chain: A_1 = U_1, A_{k+1} = U_{k+1} A_k (compute.h:267).
 template <int M>
-__global__ void __launch_bounds__(256) k_dyson(DevPlan P, int Kb, const int *__restrict__ grp,
-                                               int ngrp, const cd *__restrict__ bufU,
-                                               cd *__restrict__ bufA) {
+__global__ void __launch_bounds__(NThr<M>::v) k_dyson(DevPlan P, int Kb, const int *__restrict__ grp,
+                                                      int ngrp, const cd *__restrict__ bufU,
+                                                      cd *__restrict__ bufA) {
+    constexpr int NT = NThr<M>::v;
     __shared__ cd smem[2 * 16 * 16 * M];
     const long ntask = (long)Kb * ngrp;
     for (long task = blockIdx.x; task < ntask; task += gridDim.x) {
@@ -321,13 +387,13 @@ __global__ void __launch_bounds__(256) k_dyson(DevPlan P, int Kb, const int *__r
         const size_t base = (size_t)c * P.n * P.Lsum + P.blk_uoff[blk];
         const cd *U0 = bufU + base;
         cd *A0 = bufA + base;
-        for (int e = threadIdx.x; e < b * b; e += 256) A0[e] = U0[e];
+        for (int e = threadIdx.x; e < b * b; e += NT) A0[e] = U0[e];
         __syncthreads();
         for (int k = 1; k < P.n; k++) {
             const cd *Uk = bufU + base + (size_t)k * P.Lsum;
             const cd *Ap = bufA + base + (size_t)(k - 1) * P.Lsum;
             cd *Ak = bufA + base + (size_t)k * P.Lsum;
-            wg_gemm<M, cd>(b, b, b, Uk, b, false, Ap, b, false, Ak, b, cmk(0, 0), smem);
+            wg_gemm<M, NT, cd>(b, b, b, Uk, b, false, Ap, b, false, Ak, b, cmk(0, 0), smem);
         }
     }
 }
@@ -460,9 +526,11 @@ __global__ void __launch_bounds__(256) k_eig(DevPlan P, int Kb, const int *__res
 // ------------------------------------------------------------------------------ k_bmul
 // B_k = A_k W for every (crystallite, step, block) of a group; writes into bufB.
 template <int M>
-__global__ void __launch_bounds__(256) k_bmul(DevPlan P, int Kb, const int *__restrict__ grp,
-                                              int ngrp, const cd *__restrict__ bufA,
-                                              const cd *__restrict__ bufW, cd *__restrict__ bufB) {
+__global__ void __launch_bounds__(NThr<M>::v) k_bmul(DevPlan P, int Kb, const int *__restrict__ grp,
+                                                     int ngrp, const cd *__restrict__ bufA,
+                                                     const cd *__restrict__ bufW,
+                                                     cd *__restrict__ bufB) {
+    constexpr int NT = NThr<M>::v;
     __shared__ cd smem[2 * 16 * 16 * M];
     const long ntask = (long)Kb * P.n * ngrp;
     for (long task = blockIdx.x; task < ntask; task += gridDim.x) {
@@ -472,7 +540,7 @@ __global__ void __launch_bounds__(256) k_bmul(DevPlan P, int Kb, const int *__re
         const int blk = grp[g], b = P.blk_dim[blk];
         const size_t off = ((size_t)c * P.n + k) * P.Lsum + P.blk_uoff[blk];
         const cd *W = bufW + (size_t)c * P.Lsum + P.blk_uoff[blk];
-        wg_gemm<M, cd>(b, b, b, bufA + off, b, false, W, b, false, bufB + off, b, cmk(0, 0), smem);
+        wg_gemm<M, NT, cd>(b, b, b, bufA + off, b, false, W, b, false, bufB + off, b, cmk(0, 0), smem);
     }
 }
 
@@ -480,15 +548,14 @@ __global__ void __launch_bounds__(256) k_bmul(DevPlan P, int Kb, const int *__re
 // One task = (crystallite, step, block pair (I,J)).  With B = A_{k+1} W (per block):
 //   Q(r,s) = sum conj(B_I(a,r)) D_IJ(a,b) B_J(b,s)        r in I, s in J
 //   R(s,r) = sum conj(B_J(b,s)) rho_JI(b,a) B_I(a,r)
-// (rho_eff = det + det^H when rho == (det+det^H)/2, compute.h:381-382).  Stored as the
-// twiddled series Qt_k(e) = Q z^-k, Rt_k(e) = R(s,r) z^k, z = exp(i (th_r - th_s)/n),
-// element e = r + s*b_I at offset pair_eoff, layout [c][k][e].
+// (rho_eff = det + det^H when rho == (det+det^H)/2, compute.h:381-382).  Stored per element
+// e = r + s*b_I at offset pair_eoff, layout [c][k][e]: Qs = Q(r,s), Rs = R(s,r).
 template <int M>
-__global__ void __launch_bounds__(256) k_qr(DevPlan P, int Kb, const int *__restrict__ pgrp,
-                                            int ngrp, const cd *__restrict__ bufB,
-                                            const double *__restrict__ thetas,
-                                            cd *__restrict__ scratch, size_t slot,
-                                            cd *__restrict__ Qt, cd *__restrict__ Rt) {
+__global__ void __launch_bounds__(NThr<M>::v) k_qr(DevPlan P, int Kb, const int *__restrict__ pgrp,
+                                                   int ngrp, const cd *__restrict__ bufB,
+                                                   cd *__restrict__ scratch, size_t slot,
+                                                   cd *__restrict__ Qt, cd *__restrict__ Rt) {
+    constexpr int NT = NThr<M>::v;
     __shared__ cd smem[2 * 16 * 16 * M];
     const int tid = threadIdx.x;
     cd *T1 = scratch + slot * 2 * blockIdx.x;
@@ -503,15 +570,12 @@ __global__ void __launch_bounds__(256) k_qr(DevPlan P, int Kb, const int *__rest
         const int bI = P.blk_dim[I], bJ = P.blk_dim[J];
         const cd *BI = bufB + ((size_t)c * P.n + k) * P.Lsum + P.blk_uoff[I];
         const cd *BJ = bufB + ((size_t)c * P.n + k) * P.Lsum + P.blk_uoff[J];
-        const double *thI = thetas + (size_t)c * P.N + P.blk_soff[I];
-        const double *thJ = thetas + (size_t)c * P.N + P.blk_soff[J];
         const size_t eo = ((size_t)c * P.n + k) * P.Etot + P.pair_eoff[pr];
-        // T1 = D_IJ B_J  (bI x bJ), sparse rows of D_IJ
         {
             const int *rp = P.d_rowptr + P.pair_doff[pr];
             const int *ci = P.d_col;
             const cd *dv = (const cd *)P.d_val;
-            for (int e = tid; e < bI * bJ; e += 256) {
+            for (int e = tid; e < bI * bJ; e += NT) {
                 int a = e % bI, s = e / bI;
                 cd acc = cmk(0, 0);
                 for (int z = rp[a]; z < rp[a + 1]; z++) cfma(acc, dv[z], BJ[ci[z] + (size_t)s * bJ]);
@@ -519,21 +583,12 @@ __global__ void __launch_bounds__(256) k_qr(DevPlan P, int Kb, const int *__rest
             }
         }
         __syncthreads();
-        // G = B_I^H T1 = Q (bI x bJ)
-        wg_gemm<M, cd>(bI, bJ, bI, BI, bI, true, T1, bI, false, G, bI, cmk(0, 0), smem);
-        const double inv_n = 1.0 / (double)P.n;
-        for (int e = tid; e < bI * bJ; e += 256) {
-            int r = e % bI, s = e / bI;
-            double ph = (thI[r] - thJ[s]) * inv_n * (double)k;
-            Qt[eo + e] = cmul(G[e], cexpi(-ph));
-        }
-        __syncthreads();
-        // T1' = rho_JI B_I (bJ x bI) ; R = B_J^H T1' (bJ x bI)
+        wg_gemm<M, NT, cd>(bI, bJ, bI, BI, bI, true, T1, bI, false, Qt + eo, bI, cmk(0, 0), smem);
         {
             const int *rp = P.r_rowptr + P.pair_roff[pr];
             const int *ci = P.r_col;
             const cd *rv = (const cd *)P.r_val;
-            for (int e = tid; e < bJ * bI; e += 256) {
+            for (int e = tid; e < bJ * bI; e += NT) {
                 int bq = e % bJ, r = e / bJ;
                 cd acc = cmk(0, 0);
                 for (int z = rp[bq]; z < rp[bq + 1]; z++) cfma(acc, rv[z], BI[ci[z] + (size_t)r * bI]);
@@ -541,22 +596,26 @@ __global__ void __launch_bounds__(256) k_qr(DevPlan P, int Kb, const int *__rest
             }
         }
         __syncthreads();
-        wg_gemm<M, cd>(bJ, bI, bJ, BJ, bJ, true, T1, bJ, false, G, bJ, cmk(0, 0), smem);
-        for (int e = tid; e < bI * bJ; e += 256) {
+        // R^T(r,s) = sum_b T1(b,r) conj(B_J(b,s)) = (T1^T conj(B_J))(r,s): computed as
+        // G = B_J^H T1 (bJ x bI) then transposed into Rt
+        wg_gemm<M, NT, cd>(bJ, bI, bJ, BJ, bJ, true, T1, bJ, false, G, bJ, cmk(0, 0), smem);
+        for (int e = tid; e < bI * bJ; e += NT) {
             int r = e % bI, s = e / bI;
-            double ph = (thI[r] - thJ[s]) * inv_n * (double)k;
-            Rt[eo + e] = cmul(G[s + (size_t)r * bJ], cexpi(ph));
+            Rt[eo + e] = G[s + (size_t)r * bJ];
         }
         __syncthreads();
     }
 }
 
 // ------------------------------------------------------------------------------ k_corr
-// One workgroup = (crystallite, chunk of CH elements).  The Qt/Rt series of the chunk are
-// staged in LDS; thread (el, pg) computes F_p(el) for p in [8 pg, 8 pg + 8):
-//   F_p(e) = sum_i Rt_i(e) Qt_{(i+p)%n}(e)                      (compute.h:406-427)
-// then F replaces Qt in LDS and thread t forms fid[t] = sum_e F_{t%n}(e) exp(i t phi_e),
-// phi_e = (th_r - th_s)/n, added atomically to fid_c[c][t]      (compute.h:432-457).
+// One workgroup = (crystallite, chunk of CH elements).  Staging applies the gamma-COMPUTE
+// twiddles, Qt_k = Q_k z^-k and Rt_k = R_k(s,r) z^k with z = exp(i phi_e),
+// phi_e = (th_r - th_s)/n (compute.h:406-427 written as exp(-i ind2 tau w) = z^-(i+p)%n z^i),
+// each thread walking a run of k with one sincos and complex multiplies.  Thread (el, pg)
+// then computes F_p(el) for p in [8 pg, 8 pg + 8):
+//   F_p(e) = sum_i Rt_i(e) Qt_{(i+p)%n}(e)
+// F replaces Qt in LDS and thread t forms fid[t] = sum_e F_{t%n}(e) z_e^t (compute.h:432-457),
+// z_e^t advanced by z_e^256, added atomically to fid_c[c][t].
 // CH = min(CORR_SPAN / n, 256 / ceil(n/8)) is computed identically on the host.
 #define CORR_SPAN 2048
 #define CORR_PPT 8
@@ -575,26 +634,17 @@ __global__ void __launch_bounds__(256) k_corr(DevPlan P, int Kb, const cd *__res
     __shared__ cd sQ[CORR_SPAN];
     __shared__ cd sR[CORR_SPAN];
     __shared__ double sphi[256];
+    __shared__ cd sstep[256];
     const int n = P.n, tid = threadIdx.x;
     const int CH = corr_chunk(n);
     const int npg = (n + CORR_PPT - 1) / CORR_PPT;
+    const int nseg = 256 / CH;
+    const int kpt = (n + nseg - 1) / nseg;
     const int nchunk = (int)((P.Etot + CH - 1) / CH);
     const long ntask = (long)Kb * nchunk;
     for (long task = blockIdx.x; task < ntask; task += gridDim.x) {
         const int ch = (int)(task % nchunk), c = (int)(task / nchunk);
         const long e0 = (long)ch * CH;
-        for (int idx = tid; idx < CH * n; idx += 256) {
-            int el = idx % CH, k = idx / CH;
-            long e = e0 + el;
-            cd q = cmk(0, 0), r = cmk(0, 0);
-            if (e < P.Etot) {
-                size_t o = ((size_t)c * n + k) * P.Etot + e;
-                q = Qt[o];
-                r = Rt[o];
-            }
-            sQ[idx] = q;
-            sR[idx] = r;
-        }
         for (int el = tid; el < CH; el += 256) {
             long e = e0 + el;
             double ph = 0.0;
@@ -603,6 +653,30 @@ __global__ void __launch_bounds__(256) k_corr(DevPlan P, int Kb, const cd *__res
                 ph = (th[elem_r[e]] - th[elem_s[e]]) / (double)n;
             }
             sphi[el] = ph;
+            sstep[el] = cexpi(256.0 * ph);
+        }
+        __syncthreads();
+        {
+            const int el = tid % CH, seg = tid / CH;
+            const long e = e0 + el;
+            const int ka = seg * kpt, kb = min(n, ka + kpt);
+            if (seg < nseg && ka < kb) {
+                const double ph = sphi[el];
+                cd w = cexpi(-(double)ka * ph);   // z^-k
+                const cd z = cexpi(ph);
+                const cd zi = cconj(z);
+                for (int k = ka; k < kb; k++) {
+                    cd q = cmk(0, 0), r = cmk(0, 0);
+                    if (e < P.Etot) {
+                        size_t o = ((size_t)c * n + k) * P.Etot + e;
+                        q = Qt[o];
+                        r = Rt[o];
+                    }
+                    sQ[k * CH + el] = cmul(q, w);
+                    sR[k * CH + el] = cmul(r, cconj(w));
+                    w = cmul(w, zi);
+                }
+            }
         }
         __syncthreads();
         cd F[CORR_PPT];
@@ -631,17 +705,33 @@ __global__ void __launch_bounds__(256) k_corr(DevPlan P, int Kb, const cd *__res
                 if (p0 + q < n) sQ[(p0 + q) * CH + el] = F[q];
         }
         __syncthreads();
-        for (int t = tid; t < P.npts; t += 256) {
-            const int p = t % n;
-            cd acc = cmk(0, 0);
-            for (int e = 0; e < CH; e++) {
-                cd f = sQ[p * CH + e];
-                if (f.x == 0.0 && f.y == 0.0) continue;
-                cfma(acc, f, cexpi((double)t * sphi[e]));
-            }
-            if (acc.x != 0.0 || acc.y != 0.0) {
-                atomicAdd(&fidc[2 * ((size_t)c * P.npts + t)], acc.x);
-                atomicAdd(&fidc[2 * ((size_t)c * P.npts + t) + 1], acc.y);
+        for (int tb = 0; tb < P.npts; tb += 2048) {
+            const int t0 = tb + tid;
+            if (t0 < P.npts) {
+                const int nt = min(8, (P.npts - t0 + 255) / 256); // t = t0 + 256 m
+                cd acc[8];
+#pragma unroll
+                for (int m = 0; m < 8; m++) acc[m] = cmk(0, 0);
+                for (int e = 0; e < CH; e++) {
+                    if (e0 + e >= P.Etot) break;
+                    cd w = cexpi((double)t0 * sphi[e]);
+                    const cd st = sstep[e];
+#pragma unroll
+                    for (int m = 0; m < 8; m++) {
+                        if (m < nt) {
+                            cfma(acc[m], sQ[((t0 + 256 * m) % n) * CH + e], w);
+                            w = cmul(w, st);
+                        }
+                    }
+                }
+#pragma unroll
+                for (int m = 0; m < 8; m++) {
+                    if (m < nt) {
+                        const int t = t0 + 256 * m;
+                        atomicAdd(&fidc[2 * ((size_t)c * P.npts + t)], acc[m].x);
+                        atomicAdd(&fidc[2 * ((size_t)c * P.npts + t) + 1], acc[m].y);
+                    }
+                }
             }
         }
         __syncthreads();
@@ -707,11 +797,12 @@ __global__ void __launch_bounds__(256) k_static_h(DevPlan P, int Kb, int c0, con
 // eigenvalues, A(i,j) = Do(i,j) sig0(j,i), kept when |A|^2 > 1e-10 (oneFID.h:317-325), and
 // fid[t] += A(i,j) exp(-i 2 pi dt t (lambda_j - lambda_i)) (conj(e_i) e_j)^t.
 template <int M>
-__global__ void __launch_bounds__(256) k_static_fid(DevPlan P, int Kb, const int *__restrict__ pgrp,
+__global__ void __launch_bounds__(NThr<M>::v) k_static_fid(DevPlan P, int Kb, const int *__restrict__ pgrp,
                                                     int ngrp, const cd *__restrict__ bufW,
                                                     const double *__restrict__ lam,
                                                     cd *__restrict__ scratch, size_t slot,
                                                     double dt, double *__restrict__ fidc) {
+    constexpr int NT = NThr<M>::v;
     __shared__ cd smem[2 * 16 * 16 * M];
     const int tid = threadIdx.x;
     cd *T1 = scratch + slot * 3 * blockIdx.x;
@@ -729,7 +820,7 @@ __global__ void __launch_bounds__(256) k_static_fid(DevPlan P, int Kb, const int
         {
             const int *rp = P.d_rowptr + P.pair_doff[pr];
             const cd *dv = (const cd *)P.d_val;
-            for (int e = tid; e < bI * bJ; e += 256) {
+            for (int e = tid; e < bI * bJ; e += NT) {
                 int a = e % bI, s = e / bI;
                 cd acc = cmk(0, 0);
                 for (int z = rp[a]; z < rp[a + 1]; z++) cfma(acc, dv[z], VJ[P.d_col[z] + (size_t)s * bJ]);
@@ -737,11 +828,11 @@ __global__ void __launch_bounds__(256) k_static_fid(DevPlan P, int Kb, const int
             }
         }
         __syncthreads();
-        wg_gemm<M, cd>(bI, bJ, bI, VI, bI, true, T1, bI, false, Gq, bI, cmk(0, 0), smem);
+        wg_gemm<M, NT, cd>(bI, bJ, bI, VI, bI, true, T1, bI, false, Gq, bI, cmk(0, 0), smem);
         {
             const int *rp = P.r_rowptr + P.pair_roff[pr];
             const cd *rv = (const cd *)P.r_val;
-            for (int e = tid; e < bJ * bI; e += 256) {
+            for (int e = tid; e < bJ * bI; e += NT) {
                 int bq = e % bJ, r = e / bJ;
                 cd acc = cmk(0, 0);
                 for (int z = rp[bq]; z < rp[bq + 1]; z++) cfma(acc, rv[z], VI[P.r_col[z] + (size_t)r * bI]);
@@ -749,9 +840,9 @@ __global__ void __launch_bounds__(256) k_static_fid(DevPlan P, int Kb, const int
             }
         }
         __syncthreads();
-        wg_gemm<M, cd>(bJ, bI, bJ, VJ, bJ, true, T1, bJ, false, Gr, bJ, cmk(0, 0), smem);
+        wg_gemm<M, NT, cd>(bJ, bI, bJ, VJ, bJ, true, T1, bJ, false, Gr, bJ, cmk(0, 0), smem);
         // A(i,j) = Do(i,j) sig0(j,i): i in I, j in J
-        for (int e = tid; e < bI * bJ; e += 256) {
+        for (int e = tid; e < bI * bJ; e += NT) {
             int r = e % bI, s = e / bI;
             cd a = cmul(Gq[e], Gr[s + (size_t)r * bJ]);
             if (a.x * a.x + a.y * a.y > 1.0e-10) T1[e] = a;
@@ -759,7 +850,7 @@ __global__ void __launch_bounds__(256) k_static_fid(DevPlan P, int Kb, const int
         }
         __syncthreads();
         const double w0 = -2.0 * 3.14159265358979323846 * dt;
-        for (int t = tid; t < P.npts; t += 256) {
+        for (int t = tid; t < P.npts; t += NT) {
             cd acc = cmk(0, 0);
             for (int e = 0; e < bI * bJ; e++) {
                 cd a = T1[e];
@@ -784,7 +875,7 @@ extern "C" int blfid_dev_launch_exp(int M, const DevPlan *P, int Kb, int c0, con
     int grid = (int)(ntask < nslots ? ntask : nslots);
     if (grid <= 0) return 0;
     switch (M) {
-    case 1: k_exp<1><<<grid, 256, 0, st>>>(*P, Kb, c0, grp, ngrp, scratch, slot, bufU); break;
+    case 1: k_exp<1><<<grid, 64, 0, st>>>(*P, Kb, c0, grp, ngrp, scratch, slot, bufU); break;
     case 2: k_exp<2><<<grid, 256, 0, st>>>(*P, Kb, c0, grp, ngrp, scratch, slot, bufU); break;
     case 3: k_exp<3><<<grid, 256, 0, st>>>(*P, Kb, c0, grp, ngrp, scratch, slot, bufU); break;
     case 4: k_exp<4><<<grid, 256, 0, st>>>(*P, Kb, c0, grp, ngrp, scratch, slot, bufU); break;
@@ -799,7 +890,7 @@ extern "C" int blfid_dev_launch_dyson(int M, const DevPlan *P, int Kb, const int
     int grid = (int)(ntask < GRID_CAP ? ntask : GRID_CAP);
     if (grid <= 0) return 0;
     switch (M) {
-    case 1: k_dyson<1><<<grid, 256, 0, st>>>(*P, Kb, grp, ngrp, bufU, bufA); break;
+    case 1: k_dyson<1><<<grid, 64, 0, st>>>(*P, Kb, grp, ngrp, bufU, bufA); break;
     case 2: k_dyson<2><<<grid, 256, 0, st>>>(*P, Kb, grp, ngrp, bufU, bufA); break;
     case 3: k_dyson<3><<<grid, 256, 0, st>>>(*P, Kb, grp, ngrp, bufU, bufA); break;
     case 4: k_dyson<4><<<grid, 256, 0, st>>>(*P, Kb, grp, ngrp, bufU, bufA); break;
@@ -824,7 +915,7 @@ extern "C" int blfid_dev_launch_bmul(int M, const DevPlan *P, int Kb, const int 
     int grid = (int)(ntask < GRID_CAP ? ntask : GRID_CAP);
     if (grid <= 0) return 0;
     switch (M) {
-    case 1: k_bmul<1><<<grid, 256, 0, st>>>(*P, Kb, grp, ngrp, bufA, bufW, bufB); break;
+    case 1: k_bmul<1><<<grid, 64, 0, st>>>(*P, Kb, grp, ngrp, bufA, bufW, bufB); break;
     case 2: k_bmul<2><<<grid, 256, 0, st>>>(*P, Kb, grp, ngrp, bufA, bufW, bufB); break;
     case 3: k_bmul<3><<<grid, 256, 0, st>>>(*P, Kb, grp, ngrp, bufA, bufW, bufB); break;
     case 4: k_bmul<4><<<grid, 256, 0, st>>>(*P, Kb, grp, ngrp, bufA, bufW, bufB); break;
@@ -834,17 +925,17 @@ extern "C" int blfid_dev_launch_bmul(int M, const DevPlan *P, int Kb, const int 
 }
 
 extern "C" int blfid_dev_launch_qr(int M, const DevPlan *P, int Kb, const int *pgrp, int ngrp,
-                                   const cd *bufB, const double *thetas, cd *scratch, size_t slot,
-                                   int nslots, cd *Qt, cd *Rt, hipStream_t st) {
+                                   const cd *bufB, cd *scratch, size_t slot, int nslots, cd *Qt,
+                                   cd *Rt, hipStream_t st) {
     long ntask = (long)Kb * P->n * ngrp;
     int grid = (int)(ntask < nslots ? ntask : nslots);
     if (grid <= 0) return 0;
     switch (M) {
-    case 1: k_qr<1><<<grid, 256, 0, st>>>(*P, Kb, pgrp, ngrp, bufB, thetas, scratch, slot, Qt, Rt); break;
-    case 2: k_qr<2><<<grid, 256, 0, st>>>(*P, Kb, pgrp, ngrp, bufB, thetas, scratch, slot, Qt, Rt); break;
-    case 3: k_qr<3><<<grid, 256, 0, st>>>(*P, Kb, pgrp, ngrp, bufB, thetas, scratch, slot, Qt, Rt); break;
-    case 4: k_qr<4><<<grid, 256, 0, st>>>(*P, Kb, pgrp, ngrp, bufB, thetas, scratch, slot, Qt, Rt); break;
-    default: k_qr<5><<<grid, 256, 0, st>>>(*P, Kb, pgrp, ngrp, bufB, thetas, scratch, slot, Qt, Rt); break;
+    case 1: k_qr<1><<<grid, 64, 0, st>>>(*P, Kb, pgrp, ngrp, bufB, scratch, slot, Qt, Rt); break;
+    case 2: k_qr<2><<<grid, 256, 0, st>>>(*P, Kb, pgrp, ngrp, bufB, scratch, slot, Qt, Rt); break;
+    case 3: k_qr<3><<<grid, 256, 0, st>>>(*P, Kb, pgrp, ngrp, bufB, scratch, slot, Qt, Rt); break;
+    case 4: k_qr<4><<<grid, 256, 0, st>>>(*P, Kb, pgrp, ngrp, bufB, scratch, slot, Qt, Rt); break;
+    default: k_qr<5><<<grid, 256, 0, st>>>(*P, Kb, pgrp, ngrp, bufB, scratch, slot, Qt, Rt); break;
     }
     return (int)hipGetLastError();
 }
@@ -908,7 +999,7 @@ extern "C" int blfid_dev_launch_static_fid(int M, const DevPlan *P, int Kb, cons
     int grid = (int)(ntask < nslots ? ntask : nslots);
     if (grid <= 0) return 0;
     switch (M) {
-    case 1: k_static_fid<1><<<grid, 256, 0, st>>>(*P, Kb, pgrp, ngrp, bufW, lam, scratch, slot, dt, fidc); break;
+    case 1: k_static_fid<1><<<grid, 64, 0, st>>>(*P, Kb, pgrp, ngrp, bufW, lam, scratch, slot, dt, fidc); break;
     case 2: k_static_fid<2><<<grid, 256, 0, st>>>(*P, Kb, pgrp, ngrp, bufW, lam, scratch, slot, dt, fidc); break;
     case 3: k_static_fid<3><<<grid, 256, 0, st>>>(*P, Kb, pgrp, ngrp, bufW, lam, scratch, slot, dt, fidc); break;
     case 4: k_static_fid<4><<<grid, 256, 0, st>>>(*P, Kb, pgrp, ngrp, bufW, lam, scratch, slot, dt, fidc); break;
