@@ -331,16 +331,47 @@ static blfid_status build_tables(blfid_plan *pl) {
             }
         }
     std::map<int, int> root2blk;
-    pl->blocks.clear();
+    std::vector<Block> comps;
     for (int s = 0; s < N; s++) {
         int r = find(s);
         auto it = root2blk.find(r);
         if (it == root2blk.end()) {
-            root2blk[r] = (int)pl->blocks.size();
-            pl->blocks.push_back(Block());
+            root2blk[r] = (int)comps.size();
+            comps.push_back(Block());
             it = root2blk.find(r);
         }
-        pl->blocks[it->second].states.push_back(s);
+        comps[it->second].states.push_back(s);
+    }
+    // Components smaller than one 16-wide tile are packed (first fit, largest first) into
+    // shared blocks of at most 16 states: H stays block diagonal inside such a block, so
+    // exp, products and the eigenbasis are unchanged, and tiny components stop costing a
+    // workgroup each.
+    pl->blocks.clear();
+    std::vector<int> order(comps.size());
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        return comps[a].states.size() > comps[b].states.size();
+    });
+    std::vector<int> binfill;
+    std::vector<int> small_bins;
+    for (int ci : order) {
+        const auto &st = comps[ci].states;
+        if (st.size() >= 16) {
+            pl->blocks.push_back(comps[ci]);
+            continue;
+        }
+        int placed = -1;
+        for (size_t q = 0; q < small_bins.size(); q++)
+            if (binfill[q] + (int)st.size() <= 16) { placed = (int)q; break; }
+        if (placed < 0) {
+            small_bins.push_back((int)pl->blocks.size());
+            binfill.push_back(0);
+            pl->blocks.push_back(Block());
+            placed = (int)small_bins.size() - 1;
+        }
+        auto &dst = pl->blocks[small_bins[placed]].states;
+        dst.insert(dst.end(), st.begin(), st.end());
+        binfill[placed] += (int)st.size();
     }
     pl->blk_dim.clear(); pl->blk_soff.clear(); pl->blk_uoff.clear(); pl->blk_states.clear();
     pl->blk_groups.clear();
@@ -356,7 +387,7 @@ static blfid_status build_tables(blfid_plan *pl) {
         so += d;
         uo += (long)d * d;
         pl->bmax = std::max(pl->bmax, d);
-        int M = std::min(5, (d + 15) / 16);
+        int M = std::min(4, (d + 15) / 16);
         pl->blk_groups[M].push_back((int)b);
     }
     pl->Lsum = uo;
@@ -627,7 +658,7 @@ blfid_status blfid_plan_set_operators(blfid_plan *pl, const double *rho_d, const
                 pl->elem_r.push_back(pl->blk_soff[I] + r);
                 pl->elem_s.push_back(pl->blk_soff[J] + s);
             }
-        int M = std::min(5, (std::max(bI, bJ) + 15) / 16);
+        int M = std::min(4, (std::max(bI, bJ) + 15) / 16);
         pl->pair_groups[M].push_back((int)p);
     }
     pl->Etot = eo;
