@@ -387,7 +387,7 @@ static blfid_status build_tables(blfid_plan *pl) {
         so += d;
         uo += (long)d * d;
         pl->bmax = std::max(pl->bmax, d);
-        int M = std::min(4, (d + 15) / 16);
+        int M = std::min(5, (d + 15) / 16);
         pl->blk_groups[M].push_back((int)b);
     }
     pl->Lsum = uo;
@@ -658,7 +658,7 @@ blfid_status blfid_plan_set_operators(blfid_plan *pl, const double *rho_d, const
                 pl->elem_r.push_back(pl->blk_soff[I] + r);
                 pl->elem_s.push_back(pl->blk_soff[J] + s);
             }
-        int M = std::min(4, (std::max(bI, bJ) + 15) / 16);
+        int M = std::min(5, (std::max(bI, bJ) + 15) / 16);
         pl->pair_groups[M].push_back((int)p);
     }
     pl->Etot = eo;

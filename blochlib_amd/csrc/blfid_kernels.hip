@@ -73,10 +73,10 @@ template <> struct Arith<cd> {
 // owns rows tx + 16 i and columns ty + (NT/16) j.  K is walked in slices of 16 staged through
 // LDS with zero fill (unpadded sizes work); the next slice is prefetched into registers while
 // the current one is consumed.  C must not alias A or B.  `smem` holds 2*16*16M elements.
-template <int M, int NT, typename T, bool ca, bool cb>
+template <int M, int NT, typename T>
 __device__ __forceinline__ void gemm_load_slice(int m, int n, int k, int i0, int j0, int k0,
-                                                const T *__restrict__ A, int lda,
-                                                const T *__restrict__ B, int ldb,
+                                                const T *__restrict__ A, int lda, bool ca,
+                                                const T *__restrict__ B, int ldb, bool cb,
                                                 T (&ra)[16 * M * 16 / NT], T (&rb)[16 * M * 16 / NT]) {
     constexpr int TM = 16 * M, KS = 16, LPT = TM * KS / NT;
     const int tid = threadIdx.x;
@@ -106,115 +106,92 @@ __device__ __forceinline__ void gemm_load_slice(int m, int n, int k, int i0, int
     }
 }
 
-// MFMA step on one 16x16 sub-tile: v_mfma_f64_16x16x4f64 with lane l holding
-// A[l%16][l/16], B[l/16][l%16] and D[4j + l/16][l%16] in register j.  The operands are
-// passed swapped (x from op(B), y from op(A)) so that D = C^T: lanes 0..15 then walk rows
-// of the column-major C and the stores coalesce.
-typedef double mf4 __attribute__((ext_vector_type(4)));
-template <typename T> struct MfmaAcc;
-template <> struct MfmaAcc<double> {
-    mf4 v;
-    __device__ __forceinline__ void zero() { v = mf4{0.0, 0.0, 0.0, 0.0}; }
-    __device__ __forceinline__ void step(double a, double b) {
-        v = __builtin_amdgcn_mfma_f64_16x16x4f64(b, a, v, 0, 0, 0);
-    }
-    __device__ __forceinline__ double get(int j) const { return v[j]; }
-};
-template <> struct MfmaAcc<cd> {
-    mf4 re, im;
-    __device__ __forceinline__ void zero() {
-        re = mf4{0.0, 0.0, 0.0, 0.0};
-        im = re;
-    }
-    __device__ __forceinline__ void step(cd a, cd b) {
-        re = __builtin_amdgcn_mfma_f64_16x16x4f64(b.x, a.x, re, 0, 0, 0);
-        re = __builtin_amdgcn_mfma_f64_16x16x4f64(b.y, -a.y, re, 0, 0, 0);
-        im = __builtin_amdgcn_mfma_f64_16x16x4f64(b.y, a.x, im, 0, 0, 0);
-        im = __builtin_amdgcn_mfma_f64_16x16x4f64(b.x, a.y, im, 0, 0, 0);
-    }
-    __device__ __forceinline__ cd get(int j) const { return cmk(re[j], im[j]); }
-};
-
-// C[m x n] = op(A)[m x k] * op(B)[k x n] (+ diag * I) on FP64 matrix cores.  Output tiles of
-// 16M x 16M split into 16x16 sub-tiles (those inside C) dealt round-robin to the NT/64
-// waves; K walked in
-// slices of 16 staged through LDS (zero fill makes unpadded sizes work), the next slice
-// prefetched into registers while the current one is consumed.  C must not alias A or B.
-// `smem` holds 2 * 16 * 16M elements of T.
-template <int M, int NT, typename T, bool ca = false, bool cb = false>
-__device__ void wg_gemm(int m, int n, int k, const T *__restrict__ A, int lda,
-                        const T *__restrict__ B, int ldb, T *__restrict__ C, int ldc,
+template <int M, int NT, typename T>
+__device__ void wg_gemm(int m, int n, int k, const T *__restrict__ A, int lda, bool ca,
+                        const T *__restrict__ B, int ldb, bool cb, T *__restrict__ C, int ldc,
                         T diag, T *smem) {
-    constexpr int TM = 16 * M, KS = 16, LPT = TM * KS / NT;
-    constexpr int NW = NT / 64, NST = M * M, SPW = (NST + NW - 1) / NW;
-    // register prefetch only while the accumulators leave room for it
-    constexpr bool PF = SPW * sizeof(MfmaAcc<T>) <= 64 * sizeof(double);
+    constexpr int TM = 16 * M, KS = 16, TY = NT / 16, MJ = TM / TY, LPT = TM * KS / NT;
+    // register prefetch of the next K slice unless the accumulators already fill the VGPRs
+    constexpr bool PF = (M * MJ * sizeof(T)) <= 256;
     T *As = smem;
     T *Bs = smem + KS * TM;
-    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const int lr = lane & 15, lk = lane >> 4;
+    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
     for (int i0 = 0; i0 < m; i0 += TM) {
         for (int j0 = 0; j0 < n; j0 += TM) {
-            // only the 16x16 sub-tiles that intersect C are computed
-            const int mt = min(M, (m - i0 + 15) / 16), nt = min(M, (n - j0 + 15) / 16);
-            MfmaAcc<T> acc[SPW];
+            T acc[M][MJ];
 #pragma unroll
-            for (int s = 0; s < SPW; s++) acc[s].zero();
-            T ra[LPT], rb[LPT];
+            for (int i = 0; i < M; i++)
+#pragma unroll
+                for (int j = 0; j < MJ; j++) acc[i][j] = Arith<T>::zero();
+            T ra[PF ? LPT : 1], rb[PF ? LPT : 1];
             if constexpr (PF)
-                gemm_load_slice<M, NT, T, ca, cb>(m, n, k, i0, j0, 0, A, lda, B, ldb, ra, rb);
+                gemm_load_slice<M, NT, T>(m, n, k, i0, j0, 0, A, lda, ca, B, ldb, cb, ra, rb);
             for (int k0 = 0; k0 < k; k0 += KS) {
-                if constexpr (!PF)
-                    gemm_load_slice<M, NT, T, ca, cb>(m, n, k, i0, j0, k0, A, lda, B, ldb, ra, rb);
                 __syncthreads();
+                if constexpr (PF) {
 #pragma unroll
-                for (int l = 0; l < LPT; l++) {
-                    int idx = tid + NT * l;
-                    int r, q;
-                    if (!ca) { r = idx % TM; q = idx / TM; }
-                    else { q = idx % KS; r = idx / KS; }
-                    As[q * TM + r] = ra[l];
-                    int c2, q2;
-                    if (!cb) { q2 = idx % KS; c2 = idx / KS; }
-                    else { c2 = idx % TM; q2 = idx / TM; }
-                    Bs[q2 * TM + c2] = rb[l];
+                    for (int l = 0; l < LPT; l++) {
+                        int idx = tid + NT * l;
+                        int r, q;
+                        if (!ca) { r = idx % TM; q = idx / TM; }
+                        else { q = idx % KS; r = idx / KS; }
+                        As[q * TM + r] = ra[l];
+                        int c2, q2;
+                        if (!cb) { q2 = idx % KS; c2 = idx / KS; }
+                        else { c2 = idx % TM; q2 = idx / TM; }
+                        Bs[q2 * TM + c2] = rb[l];
+                    }
+                } else {
+                    for (int idx = tid; idx < TM * KS; idx += NT) {
+                        int r, q;
+                        if (!ca) { r = idx % TM; q = idx / TM; }
+                        else { q = idx % KS; r = idx / KS; }
+                        int gr = i0 + r, gq = k0 + q;
+                        T v = Arith<T>::zero();
+                        if (gr < m && gq < k)
+                            v = ca ? Arith<T>::cj(A[gq + (size_t)gr * lda]) : A[gr + (size_t)gq * lda];
+                        As[q * TM + r] = v;
+                    }
+                    for (int idx = tid; idx < TM * KS; idx += NT) {
+                        int c, q;
+                        if (!cb) { q = idx % KS; c = idx / KS; }
+                        else { c = idx % TM; q = idx / TM; }
+                        int gc = j0 + c, gq = k0 + q;
+                        T v = Arith<T>::zero();
+                        if (gc < n && gq < k)
+                            v = cb ? Arith<T>::cj(B[gc + (size_t)gq * ldb]) : B[gq + (size_t)gc * ldb];
+                        Bs[q * TM + c] = v;
+                    }
                 }
                 __syncthreads();
                 if constexpr (PF) {
                     if (k0 + KS < k)
-                        gemm_load_slice<M, NT, T, ca, cb>(m, n, k, i0, j0, k0 + KS, A, lda, B, ldb, ra, rb);
+                        gemm_load_slice<M, NT, T>(m, n, k, i0, j0, k0 + KS, A, lda, ca, B, ldb, cb, ra, rb);
                 }
-#pragma unroll 1
-                for (int kk = 0; kk < KS / 4; kk++) {
-                    const int q = kk * 4 + lk;
+#pragma unroll 4
+                for (int q = 0; q < KS; q++) {
+                    T a[M], b[MJ];
 #pragma unroll
-                    for (int s = 0; s < SPW; s++) {
-                        const int st = wave + s * NW;
-                        if (st < mt * nt) {
-                            const int ti = st % mt, tj = st / mt;
-                            T a = As[q * TM + ti * 16 + lr];
-                            T b = Bs[q * TM + tj * 16 + lr];
-                            acc[s].step(a, b);
-                        }
-                    }
+                    for (int i = 0; i < M; i++) a[i] = As[q * TM + tx + 16 * i];
+#pragma unroll
+                    for (int j = 0; j < MJ; j++) b[j] = Bs[q * TM + ty + TY * j];
+#pragma unroll
+                    for (int i = 0; i < M; i++)
+#pragma unroll
+                        for (int j = 0; j < MJ; j++) Arith<T>::mac(acc[i][j], a[i], b[j]);
                 }
             }
 #pragma unroll
-            for (int s = 0; s < SPW; s++) {
-                const int st = wave + s * NW;
-                if (st < mt * nt) {
-                    const int ti = st % mt, tj = st / mt;
+            for (int i = 0; i < M; i++)
 #pragma unroll
-                    for (int j = 0; j < 4; j++) {
-                        int r = i0 + ti * 16 + lr, c = j0 + tj * 16 + 4 * j + lk;
-                        if (r < m && c < n) {
-                            T v = acc[s].get(j);
-                            if (r == c) v = Arith<T>::add(v, diag);
-                            C[r + (size_t)c * ldc] = v;
-                        }
+                for (int j = 0; j < MJ; j++) {
+                    int r = i0 + tx + 16 * i, c = j0 + ty + TY * j;
+                    if (r < m && c < n) {
+                        T v = acc[i][j];
+                        if (r == c) v = Arith<T>::add(v, diag);
+                        C[r + (size_t)c * ldc] = v;
                     }
                 }
-            }
         }
     }
     __syncthreads();
@@ -357,10 +334,10 @@ __global__ void __launch_bounds__(NThr<M>::v) k_exp(DevPlan P, int Kb, int c0,
         const double sc = theta / (double)(1 << sq);
         for (int e = tid; e < bb; e += NT) X[e] *= sc;
         __syncthreads();
-        wg_gemm<M, NT, double>(b, b, b, X, b, X, b, Y, b, 0.0, smem);
-        wg_gemm<M, NT, double>(b, b, b, Y, b, Y, b, Y2, b, 0.0, smem);
-        wg_gemm<M, NT, double>(b, b, b, Y2, b, Y, b, Y3, b, 0.0, smem);
-        wg_gemm<M, NT, double>(b, b, b, Y2, b, Y2, b, Y4, b, 0.0, smem);
+        wg_gemm<M, NT, double>(b, b, b, X, b, false, X, b, false, Y, b, 0.0, smem);
+        wg_gemm<M, NT, double>(b, b, b, Y, b, false, Y, b, false, Y2, b, 0.0, smem);
+        wg_gemm<M, NT, double>(b, b, b, Y2, b, false, Y, b, false, Y3, b, 0.0, smem);
+        wg_gemm<M, NT, double>(b, b, b, Y2, b, false, Y2, b, false, Y4, b, 0.0, smem);
         for (int e = tid; e < bb; e += NT) {
             double d = (e % b == e / b) ? 1.0 : 0.0;
             double y = Y[e], y2 = Y2[e], y3 = Y3[e];
@@ -368,8 +345,8 @@ __global__ void __launch_bounds__(NThr<M>::v) k_exp(DevPlan P, int Kb, int c0,
             Hs[e] = ss[4] * d + ss[5] * y + ss[6] * y2 + ss[7] * y3;
         }
         __syncthreads();
-        wg_gemm<M, NT, double>(b, b, b, Y4, b, Hc, b, Cm, b, 0.0, smem);
-        wg_gemm<M, NT, double>(b, b, b, Y4, b, Hs, b, Ps, b, 0.0, smem);
+        wg_gemm<M, NT, double>(b, b, b, Y4, b, false, Hc, b, false, Cm, b, 0.0, smem);
+        wg_gemm<M, NT, double>(b, b, b, Y4, b, false, Hs, b, false, Ps, b, 0.0, smem);
         for (int e = tid; e < bb; e += NT) {
             double d = (e % b == e / b) ? 1.0 : 0.0;
             double y = Y[e], y2 = Y2[e], y3 = Y3[e];
@@ -378,11 +355,11 @@ __global__ void __launch_bounds__(NThr<M>::v) k_exp(DevPlan P, int Kb, int c0,
         }
         __syncthreads();
         double *Sm = Y4;
-        wg_gemm<M, NT, double>(b, b, b, X, b, Ps, b, Sm, b, 0.0, smem);
+        wg_gemm<M, NT, double>(b, b, b, X, b, false, Ps, b, false, Sm, b, 0.0, smem);
         for (int it = 0; it < sq; it++) {
-            wg_gemm<M, NT, double>(b, b, b, Cm, b, Cm, b, Y, b, 0.0, smem);
-            wg_gemm<M, NT, double>(b, b, b, Sm, b, Sm, b, Y2, b, 0.0, smem);
-            wg_gemm<M, NT, double>(b, b, b, Cm, b, Sm, b, Y3, b, 0.0, smem);
+            wg_gemm<M, NT, double>(b, b, b, Cm, b, false, Cm, b, false, Y, b, 0.0, smem);
+            wg_gemm<M, NT, double>(b, b, b, Sm, b, false, Sm, b, false, Y2, b, 0.0, smem);
+            wg_gemm<M, NT, double>(b, b, b, Cm, b, false, Sm, b, false, Y3, b, 0.0, smem);
             for (int e = tid; e < bb; e += NT) {
                 Cm[e] = Y[e] - Y2[e];
                 Sm[e] = 2.0 * Y3[e];
@@ -416,7 +393,7 @@ __global__ void __launch_bounds__(NThr<M>::v) k_dyson(DevPlan P, int Kb, const i
             const cd *Uk = bufU + base + (size_t)k * P.Lsum;
             const cd *Ap = bufA + base + (size_t)(k - 1) * P.Lsum;
             cd *Ak = bufA + base + (size_t)k * P.Lsum;
-            wg_gemm<M, NT, cd>(b, b, b, Uk, b, Ap, b, Ak, b, cmk(0, 0), smem);
+            wg_gemm<M, NT, cd>(b, b, b, Uk, b, false, Ap, b, false, Ak, b, cmk(0, 0), smem);
         }
     }
 }
@@ -563,7 +540,7 @@ __global__ void __launch_bounds__(NThr<M>::v) k_bmul(DevPlan P, int Kb, const in
         const int blk = grp[g], b = P.blk_dim[blk];
         const size_t off = ((size_t)c * P.n + k) * P.Lsum + P.blk_uoff[blk];
         const cd *W = bufW + (size_t)c * P.Lsum + P.blk_uoff[blk];
-        wg_gemm<M, NT, cd>(b, b, b, bufA + off, b, W, b, bufB + off, b, cmk(0, 0), smem);
+        wg_gemm<M, NT, cd>(b, b, b, bufA + off, b, false, W, b, false, bufB + off, b, cmk(0, 0), smem);
     }
 }
 
@@ -606,7 +583,7 @@ __global__ void __launch_bounds__(NThr<M>::v) k_qr(DevPlan P, int Kb, const int 
             }
         }
         __syncthreads();
-        wg_gemm<M, NT, cd, true, false>(bI, bJ, bI, BI, bI, T1, bI, Qt + eo, bI, cmk(0, 0), smem);
+        wg_gemm<M, NT, cd>(bI, bJ, bI, BI, bI, true, T1, bI, false, Qt + eo, bI, cmk(0, 0), smem);
         {
             const int *rp = P.r_rowptr + P.pair_roff[pr];
             const int *ci = P.r_col;
@@ -621,7 +598,7 @@ __global__ void __launch_bounds__(NThr<M>::v) k_qr(DevPlan P, int Kb, const int 
         __syncthreads();
         // R^T(r,s) = sum_b T1(b,r) conj(B_J(b,s)) = (T1^T conj(B_J))(r,s): computed as
         // G = B_J^H T1 (bJ x bI) then transposed into Rt
-        wg_gemm<M, NT, cd, true, false>(bJ, bI, bJ, BJ, bJ, T1, bJ, G, bJ, cmk(0, 0), smem);
+        wg_gemm<M, NT, cd>(bJ, bI, bJ, BJ, bJ, true, T1, bJ, false, G, bJ, cmk(0, 0), smem);
         for (int e = tid; e < bI * bJ; e += NT) {
             int r = e % bI, s = e / bI;
             Rt[eo + e] = G[s + (size_t)r * bJ];
@@ -864,7 +841,7 @@ __global__ void __launch_bounds__(NThr<M>::v) k_static_fid(DevPlan P, int Kb, co
             }
         }
         __syncthreads();
-        wg_gemm<M, NT, cd, true, false>(bI, bJ, bI, VI, bI, T1, bI, Gq, bI, cmk(0, 0), smem);
+        wg_gemm<M, NT, cd>(bI, bJ, bI, VI, bI, true, T1, bI, false, Gq, bI, cmk(0, 0), smem);
         {
             const int *rp = P.r_rowptr + P.pair_roff[pr];
             const cd *rv = (const cd *)P.r_val;
@@ -876,7 +853,7 @@ __global__ void __launch_bounds__(NThr<M>::v) k_static_fid(DevPlan P, int Kb, co
             }
         }
         __syncthreads();
-        wg_gemm<M, NT, cd, true, false>(bJ, bI, bJ, VJ, bJ, T1, bJ, Gr, bJ, cmk(0, 0), smem);
+        wg_gemm<M, NT, cd>(bJ, bI, bJ, VJ, bJ, true, T1, bJ, false, Gr, bJ, cmk(0, 0), smem);
         // A(i,j) = Do(i,j) sig0(j,i): i in I, j in J
         for (int e = tid; e < bI * bJ; e += NT) {
             int r = e % bI, s = e / bI;
@@ -914,7 +891,8 @@ extern "C" int blfid_dev_launch_exp(int M, const DevPlan *P, int Kb, int c0, con
     case 1: k_exp<1><<<grid, 64, 0, st>>>(*P, Kb, c0, grp, ngrp, scratch, slot, bufU); break;
     case 2: k_exp<2><<<grid, 256, 0, st>>>(*P, Kb, c0, grp, ngrp, scratch, slot, bufU); break;
     case 3: k_exp<3><<<grid, 256, 0, st>>>(*P, Kb, c0, grp, ngrp, scratch, slot, bufU); break;
-    default: k_exp<4><<<grid, 256, 0, st>>>(*P, Kb, c0, grp, ngrp, scratch, slot, bufU); break;
+    case 4: k_exp<4><<<grid, 256, 0, st>>>(*P, Kb, c0, grp, ngrp, scratch, slot, bufU); break;
+    default: k_exp<5><<<grid, 256, 0, st>>>(*P, Kb, c0, grp, ngrp, scratch, slot, bufU); break;
     }
     return (int)hipGetLastError();
 }
@@ -928,7 +906,8 @@ extern "C" int blfid_dev_launch_dyson(int M, const DevPlan *P, int Kb, const int
     case 1: k_dyson<1><<<grid, 64, 0, st>>>(*P, Kb, grp, ngrp, bufU, bufA); break;
     case 2: k_dyson<2><<<grid, 256, 0, st>>>(*P, Kb, grp, ngrp, bufU, bufA); break;
     case 3: k_dyson<3><<<grid, 256, 0, st>>>(*P, Kb, grp, ngrp, bufU, bufA); break;
-    default: k_dyson<4><<<grid, 256, 0, st>>>(*P, Kb, grp, ngrp, bufU, bufA); break;
+    case 4: k_dyson<4><<<grid, 256, 0, st>>>(*P, Kb, grp, ngrp, bufU, bufA); break;
+    default: k_dyson<5><<<grid, 256, 0, st>>>(*P, Kb, grp, ngrp, bufU, bufA); break;
     }
     return (int)hipGetLastError();
 }
@@ -952,7 +931,8 @@ extern "C" int blfid_dev_launch_bmul(int M, const DevPlan *P, int Kb, const int 
     case 1: k_bmul<1><<<grid, 64, 0, st>>>(*P, Kb, grp, ngrp, bufA, bufW, bufB); break;
     case 2: k_bmul<2><<<grid, 256, 0, st>>>(*P, Kb, grp, ngrp, bufA, bufW, bufB); break;
     case 3: k_bmul<3><<<grid, 256, 0, st>>>(*P, Kb, grp, ngrp, bufA, bufW, bufB); break;
-    default: k_bmul<4><<<grid, 256, 0, st>>>(*P, Kb, grp, ngrp, bufA, bufW, bufB); break;
+    case 4: k_bmul<4><<<grid, 256, 0, st>>>(*P, Kb, grp, ngrp, bufA, bufW, bufB); break;
+    default: k_bmul<5><<<grid, 256, 0, st>>>(*P, Kb, grp, ngrp, bufA, bufW, bufB); break;
     }
     return (int)hipGetLastError();
 }
@@ -967,7 +947,8 @@ extern "C" int blfid_dev_launch_qr(int M, const DevPlan *P, int Kb, const int *p
     case 1: k_qr<1><<<grid, 64, 0, st>>>(*P, Kb, pgrp, ngrp, bufB, scratch, slot, Qt, Rt); break;
     case 2: k_qr<2><<<grid, 256, 0, st>>>(*P, Kb, pgrp, ngrp, bufB, scratch, slot, Qt, Rt); break;
     case 3: k_qr<3><<<grid, 256, 0, st>>>(*P, Kb, pgrp, ngrp, bufB, scratch, slot, Qt, Rt); break;
-    default: k_qr<4><<<grid, 256, 0, st>>>(*P, Kb, pgrp, ngrp, bufB, scratch, slot, Qt, Rt); break;
+    case 4: k_qr<4><<<grid, 256, 0, st>>>(*P, Kb, pgrp, ngrp, bufB, scratch, slot, Qt, Rt); break;
+    default: k_qr<5><<<grid, 256, 0, st>>>(*P, Kb, pgrp, ngrp, bufB, scratch, slot, Qt, Rt); break;
     }
     return (int)hipGetLastError();
 }
@@ -1034,7 +1015,8 @@ extern "C" int blfid_dev_launch_static_fid(int M, const DevPlan *P, int Kb, cons
     case 1: k_static_fid<1><<<grid, 64, 0, st>>>(*P, Kb, pgrp, ngrp, bufW, lam, scratch, slot, dt, fidc); break;
     case 2: k_static_fid<2><<<grid, 256, 0, st>>>(*P, Kb, pgrp, ngrp, bufW, lam, scratch, slot, dt, fidc); break;
     case 3: k_static_fid<3><<<grid, 256, 0, st>>>(*P, Kb, pgrp, ngrp, bufW, lam, scratch, slot, dt, fidc); break;
-    default: k_static_fid<4><<<grid, 256, 0, st>>>(*P, Kb, pgrp, ngrp, bufW, lam, scratch, slot, dt, fidc); break;
+    case 4: k_static_fid<4><<<grid, 256, 0, st>>>(*P, Kb, pgrp, ngrp, bufW, lam, scratch, slot, dt, fidc); break;
+    default: k_static_fid<5><<<grid, 256, 0, st>>>(*P, Kb, pgrp, ngrp, bufW, lam, scratch, slot, dt, fidc); break;
     }
     return (int)hipGetLastError();
 }
