@@ -5,7 +5,8 @@
 #define DEV_TERM_STRIDE 10 /* Cxx Cyy Czz Cxy Cxz Cyz  fdiag  fflip  i  j(-1 for CSA) */
 #define DEV_MAX_TERMS 128
 #define DEV_MAX_SPINS 12
-#define DEV_MAX_BLOCK 80 /* largest Hilbert-space block handled by k_eig's LDS tile */
+#define DEV_MAX_BLOCK 256 /* largest Hilbert-space block (k_eig pair tables) */
+#define DEV_LDS_BLOCK 80  /* blocks up to this size keep k_eig's matrix in LDS */
 
 struct DevPlan {
     int S, N;      // spins, Hilbert dimension

@@ -37,7 +37,8 @@ int blfid_dev_launch_exp(int M, const DevPlan *P, int Kb, int c0, const int *grp
 int blfid_dev_launch_dyson(int M, const DevPlan *P, int Kb, const int *grp, int ngrp,
                            const double2 *bufU, double2 *bufA, hipStream_t st);
 int blfid_dev_launch_eig(const DevPlan *P, int Kb, const int *grp, int ngrp, const double2 *src,
-                         size_t src_stride, int mode, double2 *bufW, double *outv, hipStream_t st);
+                         size_t src_stride, int mode, double2 *bufW, double *outv, int big,
+                         double2 *gscr, size_t gslot, int nslots, hipStream_t st);
 int blfid_dev_launch_bmul(int M, const DevPlan *P, int Kb, const int *grp, int ngrp,
                           const double2 *bufA, const double2 *bufW, double2 *bufB, hipStream_t st);
 int blfid_dev_launch_qr(int M, const DevPlan *P, int Kb, const int *pgrp, int ngrp,
@@ -175,6 +176,8 @@ struct blfid_plan {
     long Lsum = 0;
     int bmax = 0;
     std::map<int, std::vector<int>> blk_groups; // M -> blocks
+    std::vector<int> eig_small, eig_big;        // blocks whose eigen-solve fits LDS / not
+    int exp_slots = 0, qr_slots = 0, eig_slots = 0;
     // operators
     bool have_ops = false;
     int rosym = 0;
@@ -194,12 +197,14 @@ struct blfid_plan {
     DevBuf<long> d_blk_uoff, d_pair_eoff;
     DevBuf<double> d_dval, d_rval;
     std::map<int, DevBuf<int>> d_bgrp, d_pgrp;
+    DevBuf<int> d_eig_small, d_eig_big;
     // working storage (carved from the plan's own arena or a shared workspace)
     DevBuf<char> own_ws;
     blfid_workspace *ws = nullptr;
     double2 *bufU = nullptr, *bufA = nullptr, *bufW = nullptr, *bufQt = nullptr,
             *bufRt = nullptr, *scrQ = nullptr;
     double *thetas = nullptr, *scrE = nullptr, *fidc = nullptr;
+    double2 *scrEig = nullptr;
     DevBuf<double> acc, comp, fid_out;
     double budget = 0;
     int Kb = 0;
@@ -391,6 +396,18 @@ static blfid_status build_tables(blfid_plan *pl) {
         pl->blk_groups[M].push_back((int)b);
     }
     pl->Lsum = uo;
+    pl->eig_small.clear();
+    pl->eig_big.clear();
+    for (size_t b = 0; b < pl->blocks.size(); b++)
+        (pl->blk_dim[b] <= DEV_LDS_BLOCK ? pl->eig_small : pl->eig_big).push_back((int)b);
+    {
+        // concurrent workgroup slots of scratch, sized so each scratch region stays ~2.5 GB
+        const double bm2 = (double)pl->bmax * pl->bmax;
+        auto clampi = [](double v, int lo, int hi) { return (int)std::max<double>(lo, std::min<double>(hi, v)); };
+        pl->exp_slots = clampi(std::floor(2.5e9 / (9.0 * bm2 * 8.0)), 64, 8192);
+        pl->qr_slots = clampi(std::floor(2.5e9 / (3.0 * bm2 * 16.0)), 64, 8192);
+        pl->eig_slots = pl->eig_big.empty() ? 0 : clampi(std::floor(1.0e9 / (bm2 * 16.0)), 64, 2048);
+    }
     if (pl->bmax > DEV_MAX_BLOCK)
         return fail(BLFID_ERR_UNSUPPORTED,
                     "largest Hilbert-space block " + std::to_string(pl->bmax) +
@@ -431,6 +448,8 @@ static blfid_status upload_tables(blfid_plan *pl) {
     HIPCHK(pl->d_blk_states.upload(pl->blk_states));
     HIPCHK(pl->d_blk_uoff.upload(pl->blk_uoff));
     for (auto &kv : pl->blk_groups) HIPCHK(pl->d_bgrp[kv.first].upload(kv.second));
+    HIPCHK(pl->d_eig_small.upload(pl->eig_small));
+    HIPCHK(pl->d_eig_big.upload(pl->eig_big));
     return BLFID_OK;
 }
 
@@ -719,8 +738,6 @@ static double pair_flops(const blfid_plan *pl, const std::vector<int> &g) {
     return s;
 }
 
-static const int EXP_SLOTS = 2048;
-static const int QR_SLOTS = 2048;
 
 static size_t layout_bytes(const blfid_plan *pl, int Kb, size_t off[10]);
 static size_t fixed_bytes(const blfid_plan *pl);
@@ -771,17 +788,17 @@ static size_t layout_bytes(const blfid_plan *pl, int Kb, size_t off[10]) {
         sz[1] = (size_t)Kb * pl->n * L * c16;                              // A
         sz[2] = std::max<size_t>(1, (size_t)Kb * pl->n * pl->Etot) * c16;  // Qt
         sz[3] = std::max<size_t>(1, (size_t)Kb * pl->n * pl->Etot) * c16;  // Rt
-        sz[4] = (size_t)EXP_SLOTS * 9 * bm2 * d8;                         // exp scratch
+        sz[4] = (size_t)pl->exp_slots * 9 * bm2 * d8;                     // exp scratch
     } else {
         sz[0] = (size_t)Kb * L * c16;
         sz[1] = 0; sz[2] = 0; sz[3] = 0;
-        sz[4] = (size_t)EXP_SLOTS * bm2 * d8;
+        sz[4] = (size_t)pl->exp_slots * bm2 * d8;
     }
     sz[5] = (size_t)Kb * L * c16;                 // W
     sz[6] = (size_t)Kb * pl->N * d8;              // thetas / eigenvalues
-    sz[7] = (size_t)QR_SLOTS * 3 * bm2 * c16;     // qr scratch
+    sz[7] = (size_t)pl->qr_slots * 3 * bm2 * c16; // qr scratch
     sz[8] = (size_t)Kb * 2 * pl->npts * d8;       // per-crystallite FIDs
-    sz[9] = 0;
+    sz[9] = (size_t)pl->eig_slots * bm2 * c16;    // eigen-solver scratch for blocks > LDS
     size_t o = 0;
     for (int i = 0; i < 10; i++) {
         off[i] = o;
@@ -805,6 +822,7 @@ static void carve(blfid_plan *pl, char *base, const size_t off[10]) {
     pl->thetas = (double *)(base + off[6]);
     pl->scrQ = (double2 *)(base + off[7]);
     pl->fidc = (double *)(base + off[8]);
+    pl->scrEig = (double2 *)(base + off[9]);
 }
 
 blfid_status blfid_plan_run(blfid_plan *pl, int normalize, void *stream) {
@@ -858,7 +876,7 @@ blfid_status blfid_plan_run(blfid_plan *pl, int normalize, void *stream) {
             for (auto &kv : pl->blk_groups) {
                 PhaseScope ps(pl, 0 * 8 + kv.first, kb * (double)pl->n * group_b3(pl, kv.second) * 14.0, st);
                 LAUNCHCHK(blfid_dev_launch_exp(kv.first, P, kb, c0, pl->d_bgrp[kv.first].p,
-                                               (int)kv.second.size(), pl->scrE, bm2, EXP_SLOTS,
+                                               (int)kv.second.size(), pl->scrE, bm2, pl->exp_slots,
                                                pl->bufU, st));
             }
             for (auto &kv : pl->blk_groups) {
@@ -866,12 +884,15 @@ blfid_status blfid_plan_run(blfid_plan *pl, int normalize, void *stream) {
                 LAUNCHCHK(blfid_dev_launch_dyson(kv.first, P, kb, pl->d_bgrp[kv.first].p,
                                                  (int)kv.second.size(), pl->bufU, pl->bufA, st));
             }
-            for (auto &kv : pl->blk_groups) {
-                PhaseScope ps(pl, 2 * 8, 0.0, st);
-                LAUNCHCHK(blfid_dev_launch_eig(P, kb, pl->d_bgrp[kv.first].p, (int)kv.second.size(),
+            for (int big = 0; big < 2; big++) {
+                const std::vector<int> &lst = big ? pl->eig_big : pl->eig_small;
+                if (lst.empty()) continue;
+                PhaseScope ps(pl, 2 * 8 + big, 0.0, st);
+                LAUNCHCHK(blfid_dev_launch_eig(P, kb, big ? pl->d_eig_big.p : pl->d_eig_small.p,
+                                               (int)lst.size(),
                                                pl->bufA + (size_t)(pl->n - 1) * pl->Lsum,
-                                               (size_t)pl->n * pl->Lsum, 0, pl->bufW,
-                                               pl->thetas, st));
+                                               (size_t)pl->n * pl->Lsum, 0, pl->bufW, pl->thetas,
+                                               big, pl->scrEig, bm2, pl->eig_slots, st));
             }
             for (auto &kv : pl->blk_groups) {
                 PhaseScope ps(pl, 3 * 8 + kv.first, kb * (double)pl->n * group_b3(pl, kv.second) * 8.0, st);
@@ -883,7 +904,7 @@ blfid_status blfid_plan_run(blfid_plan *pl, int normalize, void *stream) {
                 PhaseScope ps(pl, 4 * 8 + kv.first, kb * (double)pl->n * pair_flops(pl, kv.second), st);
                 LAUNCHCHK(blfid_dev_launch_qr(kv.first, P, kb, pl->d_pgrp[kv.first].p,
                                               (int)kv.second.size(), pl->bufU, pl->scrQ, bm2,
-                                              QR_SLOTS, pl->bufQt, pl->bufRt, st));
+                                              pl->qr_slots, pl->bufQt, pl->bufRt, st));
             }
             if (pl->Etot > 0) {
                 PhaseScope ps(pl, 5 * 8, kb * 8.0 * (double)pl->Etot * ((double)pl->n * pl->n + pl->npts), st);
@@ -894,16 +915,20 @@ blfid_status blfid_plan_run(blfid_plan *pl, int normalize, void *stream) {
             for (auto &kv : pl->blk_groups)
                 LAUNCHCHK(blfid_dev_launch_static_h(P, kb, c0, pl->d_bgrp[kv.first].p,
                                                     (int)kv.second.size(), pl->scrE, bm2,
-                                                    EXP_SLOTS, pl->bufU, st));
-            for (auto &kv : pl->blk_groups)
-                LAUNCHCHK(blfid_dev_launch_eig(P, kb, pl->d_bgrp[kv.first].p, (int)kv.second.size(),
-                                               pl->bufU, (size_t)pl->Lsum, 1, pl->bufW,
-                                               pl->thetas, st));
+                                                    pl->exp_slots, pl->bufU, st));
+            for (int big = 0; big < 2; big++) {
+                const std::vector<int> &lst = big ? pl->eig_big : pl->eig_small;
+                if (lst.empty()) continue;
+                LAUNCHCHK(blfid_dev_launch_eig(P, kb, big ? pl->d_eig_big.p : pl->d_eig_small.p,
+                                               (int)lst.size(), pl->bufU, (size_t)pl->Lsum, 1,
+                                               pl->bufW, pl->thetas, big, pl->scrEig, bm2,
+                                               pl->eig_slots, st));
+            }
             for (auto &kv : pl->pair_groups) {
                 PhaseScope ps(pl, 7 * 8 + kv.first, 0.0, st);
                 LAUNCHCHK(blfid_dev_launch_static_fid(kv.first, P, kb, pl->d_pgrp[kv.first].p,
                                                       (int)kv.second.size(), pl->bufW,
-                                                      pl->thetas, pl->scrQ, bm2, QR_SLOTS,
+                                                      pl->thetas, pl->scrQ, bm2, pl->qr_slots,
                                                       1.0 / pl->sw, pl->fidc, st));
             }
         }

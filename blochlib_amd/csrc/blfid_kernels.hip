@@ -406,21 +406,27 @@ __global__ void __launch_bounds__(NThr<M>::v) k_dyson(DevPlan P, int Kb, const i
 // mode 0: Mx = A_n block (unitary) -> lambda_r = Mx_rr, out phase = arg(lambda_r)
 // mode 1: Mx = H block (Hermitian, static path) -> out value = Re(Mx_rr)
 #define EIG_MAXB 80
+#define EIG_MAXPAIR 128 /* blocks up to 256 */
+// BIG = false: the block lives in LDS (b <= EIG_MAXB); BIG = true: in a global scratch
+// slot of b*b complex per workgroup (any b <= 2*EIG_MAXPAIR), same algorithm.
+template <bool BIG>
 __global__ void __launch_bounds__(256) k_eig(DevPlan P, int Kb, const int *__restrict__ grp,
                                              int ngrp, const cd *__restrict__ src,
                                              size_t src_stride, int mode, cd *__restrict__ bufW,
-                                             double *__restrict__ outv) {
-    __shared__ cd Ms[EIG_MAXB * (EIG_MAXB + 1)];
-    __shared__ double rc[EIG_MAXB / 2];
-    __shared__ cd rs[EIG_MAXB / 2];
-    __shared__ int rp[EIG_MAXB / 2], rq[EIG_MAXB / 2];
+                                             double *__restrict__ outv, cd *__restrict__ gscr,
+                                             size_t gslot) {
+    __shared__ cd Mlds[BIG ? 1 : EIG_MAXB * (EIG_MAXB + 1)];
+    __shared__ double rc[EIG_MAXPAIR];
+    __shared__ cd rs[EIG_MAXPAIR];
+    __shared__ int rp[EIG_MAXPAIR], rq[EIG_MAXPAIR];
     __shared__ int s_any;
     const int tid = threadIdx.x;
+    cd *Ms = BIG ? gscr + gslot * blockIdx.x : Mlds;
     const long ntask = (long)Kb * ngrp;
     for (long task = blockIdx.x; task < ntask; task += gridDim.x) {
         const int g = (int)(task % ngrp), c = (int)(task / ngrp);
         const int blk = grp[g], b = P.blk_dim[blk];
-        const int ld = b + 1;
+        const int ld = BIG ? b : b + 1;
         const cd *S0 = src + (size_t)c * src_stride + P.blk_uoff[blk];
         cd *W = bufW + (size_t)c * P.Lsum + P.blk_uoff[blk];
         for (int e = tid; e < b * b; e += 256) {
@@ -435,9 +441,9 @@ __global__ void __launch_bounds__(256) k_eig(DevPlan P, int Kb, const int *__res
             if (tid == 0) s_any = 0;
             __syncthreads();
             for (int round = 0; round < bb - 1; round++) {
-                if (tid < npair) {
+                for (int pi = tid; pi < npair; pi += 256) {
                     // circle method: position 0 fixed, positions 1..bb-1 rotate
-                    int pa = tid, pb = bb - 1 - tid;
+                    int pa = pi, pb = bb - 1 - pi;
                     int ia = (pa == 0) ? 0 : ((pa - 1 + round) % (bb - 1)) + 1;
                     int ib = (pb == 0) ? 0 : ((pb - 1 + round) % (bb - 1)) + 1;
                     int p = min(ia, ib), q = max(ia, ib);
@@ -478,10 +484,10 @@ __global__ void __launch_bounds__(256) k_eig(DevPlan P, int Kb, const int *__res
                             }
                         }
                     }
-                    rc[tid] = cth;
-                    rs[tid] = sth;
-                    rp[tid] = p;
-                    rq[tid] = q;
+                    rc[pi] = cth;
+                    rs[pi] = sth;
+                    rp[pi] = p;
+                    rq[pi] = q;
                 }
                 __syncthreads();
                 // rows: M <- R^H M, R = [[c, -conj(s)], [s, c]] on (p, q)
@@ -914,11 +920,14 @@ extern "C" int blfid_dev_launch_dyson(int M, const DevPlan *P, int Kb, const int
 
 extern "C" int blfid_dev_launch_eig(const DevPlan *P, int Kb, const int *grp, int ngrp,
                                     const cd *src, size_t src_stride, int mode, cd *bufW,
-                                    double *outv, hipStream_t st) {
+                                    double *outv, int big, cd *gscr, size_t gslot, int nslots,
+                                    hipStream_t st) {
     long ntask = (long)Kb * ngrp;
-    int grid = (int)(ntask < GRID_CAP ? ntask : GRID_CAP);
+    int cap = big ? nslots : GRID_CAP;
+    int grid = (int)(ntask < cap ? ntask : cap);
     if (grid <= 0) return 0;
-    k_eig<<<grid, 256, 0, st>>>(*P, Kb, grp, ngrp, src, src_stride, mode, bufW, outv);
+    if (big) k_eig<true><<<grid, 256, 0, st>>>(*P, Kb, grp, ngrp, src, src_stride, mode, bufW, outv, gscr, gslot);
+    else k_eig<false><<<grid, 256, 0, st>>>(*P, Kb, grp, ngrp, src, src_stride, mode, bufW, outv, gscr, gslot);
     return (int)hipGetLastError();
 }
 

@@ -1,0 +1,54 @@
+"""The C++ front end (include/blfid/solid.hpp) compiled against libblfid.so like a BlochLib
+program would be, run on the CPU (host logic, typed error without a device) and on the GPU
+(result equals the oracle)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from blochlib_amd import powder as P
+from blochlib_amd.spinsys import SolidSys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "tests", "cpp", "frontend_check")
+SPINS = ("numspin 2\nT 1H 0\nT 1H 1\nC 1000 5000 0.5 0 10 20 30\nC -1000 5000 0.5 1 40 50 60\n"
+         "D -20000 0 1\n")
+
+
+def build_binary():
+    src = os.path.join(ROOT, "tests", "cpp", "frontend_check.cpp")
+    lib = os.path.join(ROOT, "blochlib_amd")
+    if os.path.exists(BIN) and os.path.getmtime(BIN) > max(
+            os.path.getmtime(src), os.path.getmtime(os.path.join(ROOT, "include", "blfid", "solid.hpp"))):
+        return BIN
+    subprocess.run(["g++", "-std=c++17", "-O1", "-I", os.path.join(ROOT, "include"), src, "-o", BIN,
+                    "-L", lib, "-lblfid", f"-Wl,-rpath,{lib}"], check=True)
+    return BIN
+
+
+def test_frontend_host_side():
+    b = build_binary()
+    import torch
+    r = subprocess.run([b, "cpu", P.CRYSTALS], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "OK" in r.stdout
+    if not torch.cuda.is_available():
+        assert "engine status without device: 3" in r.stdout
+
+
+@pytest.mark.gpu
+def test_frontend_gpu_matches_oracle():
+    from oracle import oracle as O
+    b = build_binary()
+    r = subprocess.run([b, "gpu", P.CRYSTALS], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.strip().splitlines()
+    sw = float(lines[0].split()[1])
+    fid = np.array([complex(*map(float, l.split())) for l in lines[1:]])
+    sy = SolidSys.from_text(SPINS)
+    pw = P.zcw(1154).subset(np.arange(0, 1154, 29))
+    ref, swr = O.powder_fid(sy, 10000.0, 54.7356, 640000.0, 256, pw, sy.op("Ix"), sy.op("Ip"),
+                            normalize=False)
+    assert sw == swr
+    assert np.linalg.norm(fid - ref) / np.linalg.norm(ref) < 1e-9

@@ -145,3 +145,31 @@ def test_shared_workspace_two_plans_small_batches():
     p1.close()
     p2.close()
     ws.close()
+
+
+def test_blocks_beyond_lds_nine_spins():
+    """9 spins: magnetisation blocks of 126 states, beyond the LDS eigen-solver tile (80)."""
+    sy = workloads.small_cluster(9, 409)
+    pw = zcw(1154).subset([7])
+    compare("cluster9", sy, 20000.0, MAGIC, 20000.0 * 4, 8, pw, sy.op("Ix"), sy.op("Ip"))
+
+
+def test_ten_spin_dense_shape_invariants():
+    """The 10-spin (N=1024, blocks up to 252) workload at its real step count and FID length
+    on a few crystallites: fid[0] equals its closed form 2*sum(w) for rho = Fx, det = Fp, the
+    signal stays bounded by it, and batching the crystallites does not change the result."""
+    w = workloads.config3()
+    pw = w.powder().subset([0, 300, 700, 1100])
+    rho, det = w.operators()
+    print("[parity] config3 shape invariants", flush=True)
+    p = engine.Plan(w.system, w.wr, w.rotor, w.sw, w.npts)
+    st = p.stats()
+    assert st["max_block"] == 252
+    a = p.powder_fid(pw, rho, det)
+    assert np.isclose(a[0].real, 2.0 * pw.weight.sum(), rtol=1e-12) and abs(a[0].imag) < 1e-12
+    assert np.all(np.isfinite(a)) and np.abs(a).max() <= abs(a[0]) * (1 + 1e-9)
+    p.set_memory_budget(st["fixed_bytes"] + 2.5 * st["bytes_per_crystallite"])
+    p.run(True)
+    b = p.fetch()
+    assert rel_l2(b, a) < 1e-13
+    p.close()
