@@ -739,6 +739,95 @@ int orc_powder_fid(int S, const int *isotopes, int ncsa, const double *csa, int 
     return st;
 }
 
+/* ------------------------------------------------------------------------------------ */
+/* point-to-point FID with rf (solid-2.0 "ptop" over a rotor-period sub-sequence)       */
+/* ------------------------------------------------------------------------------------ */
+/* Propogation::propogator (examples/solid-2.0/src/propogation.cc:240-304): for each step
+ * k of the period, hh = SolidSys::Hamiltonian(t1,t2,wr) + PulseData::Pulse (pulsedata.cc:
+ * 110-125: amp (cos(ph) Ix_i + sin(ph) Iy_i) + offset Iz_i on every spin of the pulsed
+ * isotope), U *= Mexp(hh, +i tau 2 pi) with U *= X meaning U = X U.  The ptop doFID branch
+ * (sequencerun_funcs.cc:641-652 + sequencerun.cc:164-196) records Tr(ro det) then applies
+ * ro <- U ro U^H once per point.  rf[k*4*niso + 4*q + {0,1,2,3}] = isotope id, amp (Hz),
+ * phase (rad), offset (Hz) for pulse channel q at step k. */
+static void add_rf(const orc_sys *sy, const double *rfk, int nrf, cplx *H) {
+    const int S = sy->S, N = sy->N;
+    for (int q = 0; q < nrf; q++) {
+        int iso = (int)rfk[4 * q];
+        double amp = rfk[4 * q + 1], ph = rfk[4 * q + 2], off = rfk[4 * q + 3];
+        if (amp == 0.0 && off == 0.0) continue;
+        for (int i = 0; i < S; i++) {
+            if (sy->iso[i] != iso) continue;
+            int bit = 1 << (S - 1 - i);
+            for (int s = 0; s < N; s++) {
+                int t = s ^ bit;
+                int down = (s & bit) != 0; /* column state has m = -1/2 */
+                /* Ix(t,s) = 1/2; Iy(t,s) = +i/2 when s has m=+1/2 (t has m=-1/2), -i/2 else
+                 * (sps_Iy, singlespinop.cc:113-132) */
+                cplx ix = 0.5, iy = down ? CMPLX(0.0, -0.5) : CMPLX(0.0, 0.5);
+                H[t + (size_t)s * N] += amp * (cos(ph) * ix + sin(ph) * iy);
+            }
+            for (int s = 0; s < N; s++) H[s + (size_t)s * N] += off * mz(S, s, i);
+        }
+    }
+}
+
+int orc_ptop_fid(int S, const int *isotopes, int ncsa, const double *csa, int ndip,
+                 const double *dip, int nj, const double *jc, double wr, double rotor_deg,
+                 int nsteps, int npts, double t0, int nrf, const double *rf, int K,
+                 const double *theta, const double *phi, const double *gamma, const double *w,
+                 const double *rho, const double *det, int normalize, double *fid_out) {
+    if (S < 1 || S > 12 || nsteps < 1 || npts < 0 || K < 0 || wr <= 0.0) return ORC_EINVAL;
+    orc_sys sy = {S, 1 << S, isotopes, ncsa, csa, ndip, dip, nj, jc};
+    const int N = sy.N;
+    const size_t NN = (size_t)N * N;
+    const cplx *R0 = (const cplx *)rho, *D = (const cplx *)det;
+    cplx *H = malloc(sizeof(cplx) * NN), *X = malloc(sizeof(cplx) * NN), *U = malloc(sizeof(cplx) * NN);
+    cplx *T = malloc(sizeof(cplx) * NN), *r = malloc(sizeof(cplx) * NN), *acc = calloc(npts > 0 ? npts : 1, sizeof(cplx));
+    if (!H || !X || !U || !T || !r || !acc) { free(H); free(X); free(U); free(T); free(r); free(acc); return ORC_ENOMEM; }
+    double tau = (1.0 / wr) / nsteps;
+    int st = ORC_OK;
+    for (int c = 0; c < K && st == ORC_OK; c++) {
+        double cp[9];
+        rot3(phi[c], theta[c], gamma[c], cp);
+        for (size_t q = 0; q < NN; q++) U[q] = 0;
+        for (int i = 0; i < N; i++) U[i + (size_t)i * N] = 1.0;
+        double t1 = t0, t2 = t0 + tau;
+        for (int k = 0; k < nsteps; k++) {
+            double tmid = (t2 - t1) / 2.0 + t1;
+            build_hamiltonian(&sy, cp, rotor_deg * DEG2RAD, tmid * wr * PI2, H);
+            add_rf(&sy, rf + (size_t)k * 4 * nrf, nrf, H);
+            st = orc_expm_herm(N, H, CMPLX(0.0, 1.0) * tau * PI2, X);
+            if (st) break;
+            zgemm(N, 'N', 'N', X, U, T);
+            memcpy(U, T, sizeof(cplx) * NN);
+            t1 += tau;
+            t2 += tau;
+        }
+        if (st) break;
+        memcpy(r, R0, sizeof(cplx) * NN);
+        for (int m = 0; m < npts; m++) {
+            cplx tr = 0;
+            for (int i = 0; i < N; i++)
+                for (int j = 0; j < N; j++) tr += EL(r, i, j, N) * EL(D, j, i, N);
+            acc[m] += w[c] * tr;
+            if (m + 1 < npts) {
+                zgemm(N, 'N', 'C', r, U, T);   /* r U^H */
+                zgemm(N, 'N', 'N', U, T, r);   /* U r U^H */
+            }
+        }
+    }
+    cplx trdd = 0;
+    for (int i = 0; i < N; i++)
+        for (int j = 0; j < N; j++) trdd += EL(D, i, j, N) * conj(EL(D, i, j, N));
+    for (int m = 0; m < npts; m++) {
+        cplx v = normalize ? acc[m] / trdd : acc[m];
+        fid_out[2 * m] = creal(v);
+        fid_out[2 * m + 1] = cimag(v);
+    }
+    free(H); free(X); free(U); free(T); free(r); free(acc);
+    return st;
+}
+
 /* Test hooks (exported): the pieces above on their own. */
 int orc_hamiltonian(int S, const int *isotopes, int ncsa, const double *csa, int ndip,
                     const double *dip, int nj, const double *jc, double theta, double phi,

@@ -145,3 +145,35 @@ def static_fid(sy, theta, phi, gamma, rotor_deg, sw, npts, rho, det):
         fid[t] = np.trace(det @ r)
         r = U @ r @ U.conj().T
     return fid
+
+
+def rf_hamiltonian(sy, rf_k):
+    """PulseData::Pulse (pulsedata.cc:110-125) for one step: rf_k = [(isotope, amp, phase,
+    offset)]: amp (cos(ph) Ix_i + sin(ph) Iy_i) + offset Iz_i on spins of that isotope."""
+    S = sy.spins
+    H = np.zeros((sy.N, sy.N), dtype=np.complex128)
+    for name, amp, ph, off in rf_k:
+        for i in range(S):
+            if sy.isotopes[i] == name:
+                H += amp * (np.cos(ph) * single_spin_op(S, "x", i) + np.sin(ph) * single_spin_op(S, "y", i))
+                H += off * single_spin_op(S, "z", i)
+    return H
+
+
+def ptop_equivalent(sy, theta, phi, gamma, rotor_deg, wr, nsteps, npts, rho, det, rf=(), t0=0.0):
+    """One crystallite: U = prod_k exp(+i 2 pi tau (H_k + H_rf,k)) (propogation.cc:240-304,
+    note the + sign), fid[m] = Tr(rho_m det), rho_{m+1} = U rho_m U^H (sequencerun.cc:164-196)."""
+    tau = 1.0 / wr / nsteps
+    U = np.eye(sy.N, dtype=np.complex128)
+    for k in range(nsteps):
+        tmid = t0 + (k + 0.5) * tau
+        H = hamiltonian(sy, theta, phi, gamma, rotor_deg, 2 * np.pi * wr * tmid)
+        H = H + rf_hamiltonian(sy, [(n_, np.broadcast_to(a, nsteps)[k], np.broadcast_to(p, nsteps)[k],
+                                     np.broadcast_to(o, nsteps)[k]) for n_, a, p, o in rf])
+        U = scipy.linalg.expm(2j * np.pi * tau * H) @ U
+    r = rho.astype(np.complex128)
+    fid = np.zeros(npts, dtype=np.complex128)
+    for m in range(npts):
+        fid[m] = np.trace(r @ det)
+        r = U @ r @ U.conj().T
+    return fid

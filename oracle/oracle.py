@@ -35,6 +35,11 @@ def lib():
                                      ctypes.c_double, ctypes.c_int, ctypes.c_double,
                                      ctypes.c_int, _d, _d, _d, _d, _d, _d, ctypes.c_int,
                                      ctypes.c_int, _d, _d]
+        L.orc_ptop_fid.restype = ctypes.c_int
+        L.orc_ptop_fid.argtypes = [ctypes.c_int, _i, ctypes.c_int, _d, ctypes.c_int, _d,
+                                   ctypes.c_int, _d, ctypes.c_double, ctypes.c_double,
+                                   ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, _d,
+                                   ctypes.c_int, _d, _d, _d, _d, _d, _d, ctypes.c_int, _d]
         L.orc_hamiltonian.argtypes = [ctypes.c_int, _i, ctypes.c_int, _d, ctypes.c_int, _d,
                                       ctypes.c_int, _d] + [ctypes.c_double] * 5 + [_d]
         L.orc_eig_herm_x.argtypes = [ctypes.c_int, _d, _d, _d]
@@ -123,3 +128,39 @@ def expm_herm(H, mul):
     if st:
         raise RuntimeError(st)
     return U.view(np.complex128).reshape(N, N, order="F")
+
+
+def rf_table(sy, rf, nsteps):
+    """rf: list of (isotope name, amp[nsteps], phase_rad[nsteps], offset[nsteps]) ->
+    flat [step][channel][iso id, amp, phase, offset] table."""
+    ids = {}
+    for n_ in sy.isotopes:
+        ids.setdefault(n_, len(ids))
+    tab = np.zeros((nsteps, max(len(rf), 1), 4))
+    for q, (name, amp, ph, off) in enumerate(rf):
+        tab[:, q, 0] = ids.get(name, -1)
+        tab[:, q, 1] = np.broadcast_to(amp, nsteps)
+        tab[:, q, 2] = np.broadcast_to(ph, nsteps)
+        tab[:, q, 3] = np.broadcast_to(off, nsteps)
+    return np.ascontiguousarray(tab.ravel()), max(len(rf), 1)
+
+
+def ptop_fid(sy, wr, rotor_deg, nsteps, npts, powder, rho, det, rf=(), t0=0.0, normalize=True):
+    """Rotor-synchronised point-to-point FID with rf over one rotor period per point."""
+    L = lib()
+    iso, nc, csa, nd, dip, nj, jc = _sys(sy)
+    tab, nrf = rf_table(sy, list(rf), nsteps)
+    r = _cm(rho)
+    d = _cm(det)
+    out = np.zeros(2 * max(npts, 1))
+    th = np.ascontiguousarray(powder.theta, np.float64)
+    ph = np.ascontiguousarray(powder.phi, np.float64)
+    ga = np.ascontiguousarray(powder.gamma, np.float64)
+    w = np.ascontiguousarray(powder.weight, np.float64)
+    st = L.orc_ptop_fid(sy.spins, _p(iso, _i), nc, _p(csa), nd, _p(dip), nj, _p(jc), float(wr),
+                        float(rotor_deg), int(nsteps), int(npts), float(t0), nrf, _p(tab),
+                        len(th), _p(th), _p(ph), _p(ga), _p(w), _p(r), _p(d),
+                        int(bool(normalize)), _p(out))
+    if st != 0:
+        raise RuntimeError(f"oracle failed with status {st}")
+    return out[: 2 * npts].view(np.complex128).copy()
