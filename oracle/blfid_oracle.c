@@ -745,7 +745,7 @@ int orc_powder_fid(int S, const int *isotopes, int ncsa, const double *csa, int 
 /* Propogation::propogator (examples/solid-2.0/src/propogation.cc:240-304): for each step
  * k of the period, hh = SolidSys::Hamiltonian(t1,t2,wr) + PulseData::Pulse (pulsedata.cc:
  * 110-125: amp (cos(ph) Ix_i + sin(ph) Iy_i) + offset Iz_i on every spin of the pulsed
- * isotope), U *= Mexp(hh, +i tau 2 pi) with U *= X meaning U = X U.  The ptop doFID branch
+ * isotope, only for channels with amp != 0), U *= Mexp(hh, +i tau 2 pi) with U *= X meaning U = X U.  The ptop doFID branch
  * (sequencerun_funcs.cc:641-652 + sequencerun.cc:164-196) records Tr(ro det) then applies
  * ro <- U ro U^H once per point.  rf[k*4*niso + 4*q + {0,1,2,3}] = isotope id, amp (Hz),
  * phase (rad), offset (Hz) for pulse channel q at step k. */
@@ -754,7 +754,8 @@ static void add_rf(const orc_sys *sy, const double *rfk, int nrf, cplx *H) {
     for (int q = 0; q < nrf; q++) {
         int iso = (int)rfk[4 * q];
         double amp = rfk[4 * q + 1], ph = rfk[4 * q + 2], off = rfk[4 * q + 3];
-        if (amp == 0.0 && off == 0.0) continue;
+        /* a channel with zero amplitude adds nothing, its offset included (pulsedata.cc:114) */
+        if (amp == 0.0) continue;
         for (int i = 0; i < S; i++) {
             if (sy->iso[i] != iso) continue;
             int bit = 1 << (S - 1 - i);

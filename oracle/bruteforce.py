@@ -149,10 +149,13 @@ def static_fid(sy, theta, phi, gamma, rotor_deg, sw, npts, rho, det):
 
 def rf_hamiltonian(sy, rf_k):
     """PulseData::Pulse (pulsedata.cc:110-125) for one step: rf_k = [(isotope, amp, phase,
-    offset)]: amp (cos(ph) Ix_i + sin(ph) Iy_i) + offset Iz_i on spins of that isotope."""
+    offset)]: amp (cos(ph) Ix_i + sin(ph) Iy_i) + offset Iz_i on spins of that isotope; a
+    channel with amp == 0 adds nothing, not even its offset."""
     S = sy.spins
     H = np.zeros((sy.N, sy.N), dtype=np.complex128)
     for name, amp, ph, off in rf_k:
+        if amp == 0.0:
+            continue
         for i in range(S):
             if sy.isotopes[i] == name:
                 H += amp * (np.cos(ph) * single_spin_op(S, "x", i) + np.sin(ph) * single_spin_op(S, "y", i))

@@ -8,6 +8,9 @@ Each check is chosen so that a plausible mistake in the restatement fails it:
   * gamma-COMPUTE    -> explicit step propagation (oracle/bruteforce.py), rosym and general rho,
                         even and odd n (integer-division scale), wrap-around of the rotor period
   * closed forms     -> fid[0] and a single-spin MAS FID from a scalar phase sum
+  * ptop with rf     -> explicit numpy propagation with the rf Hamiltonian built from Kronecker
+                        operators, on- and off-resonance nutation closed forms (propagator
+                        sign, phase sense, offsets, the zero-amplitude channel rule)
 """
 import numpy as np
 import pytest
@@ -234,7 +237,10 @@ def _xix(nsteps, amp, period_segments=4):
 def test_ptop_matches_explicit_numpy(oracle, phase_shift):
     sy = SolidSys.from_text(HET)
     rf = _xix(16, 50000.0)
-    rf = [(n, a, p + phase_shift, o) for n, a, p, o in rf] + [("13C", np.zeros(16), np.zeros(16),
+    # 13C channel: amplitude only on the second half of the period (its offset counts only
+    # where the amplitude is non-zero, pulsedata.cc:114)
+    camp = np.where(np.arange(16) >= 8, 7000.0, 0.0)
+    rf = [(n, a, p + phase_shift, o) for n, a, p, o in rf] + [("13C", camp, np.full(16, 0.3),
                                                                np.full(16, 300.0))]
     rho, det = sy.op("Ix_0"), sy.op("Ip_0")
     for th, ph, ga in [(0.4, 1.2, 2.0), (2.2, 5.0, 0.3)]:
@@ -260,3 +266,20 @@ def test_ptop_single_spin_nutation_closed_form(oracle):
     # exp(+i phi Iy) Iz exp(-i phi Iy) = Iz cos phi - Ix sin phi  -> Tr(. Ix) = -sin(phi)/2
     # (the + sign of the propagator exponent turns the nutation the other way round)
     assert np.allclose(fy, -0.5 * np.sin(2 * np.pi * a * m / wr), atol=1e-13)
+
+
+def test_ptop_offset_closed_forms(oracle):
+    """One spin, rf amp a and offset o for the whole period: U = exp(+i 2 pi T (a Ix + o Iz))
+    precesses about n = (a, 0, o)/w_eff, so Tr(Iz_m Iz) = 1/2 (n_z^2 + (1 - n_z^2) cos(2 pi
+    w_eff m T)).  A channel with a == 0 contributes nothing, its offset included."""
+    sy = SolidSys.from_text("numspin 1\nC 0 0 0 0\n")
+    a, o, wr, n = 2300.0, 1700.0, 10000.0, 8
+    m = np.arange(11)
+    f = oracle.ptop_fid(sy, wr, MAGIC, n, 11, Powder.single(), sy.op("Iz"), sy.op("Iz"),
+                        rf=[("1H", np.full(n, a), np.zeros(n), np.full(n, o))], normalize=False)
+    weff = np.hypot(a, o)
+    nz2 = (o / weff) ** 2
+    assert np.allclose(f, 0.5 * (nz2 + (1 - nz2) * np.cos(2 * np.pi * weff * m / wr)), atol=1e-13)
+    f0 = oracle.ptop_fid(sy, wr, MAGIC, n, 11, Powder.single(), sy.op("Ix"), sy.op("Ix"),
+                         rf=[("1H", np.zeros(n), np.zeros(n), np.full(n, o))], normalize=False)
+    assert np.allclose(f0, 0.5, atol=1e-14)
