@@ -14,6 +14,10 @@ Prints one JSON line (rank 0): metric, value (crystallite-steps/s), FIDs/s, per-
 p50/p99 inside a step, accuracy against the FP64 oracle, the roofline of the dominant kernel
 (HIP events around its launches during the timed steps) and a CPU baseline (the oracle,
 i.e. the reference algorithm restated in C, on a bounded sample, one thread per sample).
+
+--workload ptop measures the pulse-program path instead: 13C + 6 1H (N=128) under MAS
+12.5 kHz with 100 kHz phase-alternating 1H rf, 256 steps per period, ZCW 1154, 2048
+rotor-synchronised points, one FID per CSA parameter set (--sets of them per step).
 """
 from __future__ import annotations
 
@@ -94,6 +98,43 @@ def cpu_baseline(wls, sample_mode: str):
     }
 
 
+def cpu_baseline_ptop(w):
+    """Oracle (explicit Dyson product with rf, then rho <- U rho U^H per point, FP64, one
+    thread) on one crystallite of one parameter set: the period propagator and the per-point
+    cost are timed separately (16 and 48 points) and combined for the full point count."""
+    from oracle import oracle as O
+    pw = w.powder().subset([0])
+    rho, det = w.operators()
+    t = []
+    for npts in (16, 48):
+        t0 = time.perf_counter()
+        O.ptop_fid(w.system, w.wr, w.rotor, w.steps, npts, pw, rho, det, rf=w.rf)
+        t.append(time.perf_counter() - t0)
+    per_point = max((t[1] - t[0]) / 32.0, 0.0)
+    fixed = max(t[0] - 16 * per_point, 0.0)
+    sec = fixed + w.npts * per_point
+    return {
+        "value": w.steps / sec,
+        "unit": "crystallite-steps/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": ("oracle (explicit rf Dyson product + point-by-point U rho U^H, FP64, dense "
+                   f"N=128) on 1 crystallite of parameter set 0, timed at 16 and 48 points: "
+                   f"{fixed:.3f} s for the period propagator + {per_point * 1e3:.2f} ms per "
+                   f"point, combined for {w.npts} points = {sec:.2f} s per crystallite"),
+    }
+
+
+def accuracy_ptop(wls, ncryst=2, npts=64):
+    from oracle import oracle as O
+    w = wls[0]
+    pw = w.powder().subset(np.arange(ncryst) * 577 % w.powder_size)
+    rho, det = w.operators()
+    g = engine.ptop_fid(w.system, w.wr, w.rotor, w.steps, npts, pw, rho, det, rf=w.rf)
+    c = O.ptop_fid(w.system, w.wr, w.rotor, w.steps, npts, pw, rho, det, rf=w.rf)
+    return float(np.linalg.norm(g - c) / np.linalg.norm(c))
+
+
 def accuracy(wls, limit_spins=6, ncryst=2):
     """rel-L2 of the engine vs the FP64 oracle on a few crystallites of the small clusters."""
     from oracle import oracle as O
@@ -121,6 +162,8 @@ def main():
     ap.add_argument("--cpu-sample", choices=["full", "quick", "none"], default="full")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--workspace-gb", type=float, default=120.0)
+    ap.add_argument("--workload", choices=["batched", "ptop"], default="batched")
+    ap.add_argument("--sets", type=int, default=8, help="parameter sets per step (ptop)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this engine runs on one GPU; multi-GPU runs are not part of this build "
@@ -129,12 +172,16 @@ def main():
         sys.exit(2)
 
     H = hip()
-    wls = workloads.config2(args.clusters)
+    ptop = args.workload == "ptop"
+    wls = workloads.config4(args.sets) if ptop else workloads.config2(args.clusters)
     ws = engine.Workspace(args.workspace_gb * 1e9)
     plans = []
     t0 = time.perf_counter()
     for w in wls:
-        p = engine.Plan(w.system, w.wr, w.rotor, w.sw, w.npts)
+        if ptop:
+            p = engine.Plan.ptop(w.system, w.wr, w.rotor, w.steps, w.npts, w.rf)
+        else:
+            p = engine.Plan(w.system, w.wr, w.rotor, w.sw, w.npts)
         rho, det = w.operators()
         p.set_operators(rho, det)
         p.set_powder(w.powder())
@@ -200,7 +247,17 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {
+        "config": ({
+            "workload": ("heteronuclear rf: 13C + 6 1H (N=128), MAS 12.5 kHz, magic angle, "
+                         "256 steps/period, 100 kHz 0/180 phase-alternating 1H rf (8 cycles per "
+                         "period), ZCW 1154, 2048 rotor-synchronised points, rho=Ix_0, "
+                         "detect=Ip_0, complex128, one FID per seeded CSA set"),
+            "sets": len(wls),
+            "crystallites": wls[0].powder_size,
+            "steps_per_period": wls[0].steps,
+            "npts": wls[0].npts,
+            "crystallite_steps_per_step": cs_per_step,
+        } if ptop else {
             "workload": ("batched uneven sizes: 64 seeded 1H clusters of 4-8 spins (N=16-256), "
                          "MAS 15 kHz, magic angle, 128 steps/period, ZCW 3722, 1024-point FID, "
                          "rho=Ix, detect=I+, complex128"),
@@ -209,11 +266,11 @@ def main():
             "steps_per_period": wls[0].steps,
             "npts": wls[0].npts,
             "crystallite_steps_per_step": cs_per_step,
-        },
+        }),
         "fids_per_s": len(wls) * args.steps / elapsed,
         "latency_ms_per_fid": {"p50": float(np.percentile(lat, 50)),
                                "p99": float(np.percentile(lat, 99)),
-                               "note": "per-cluster powder FID service time inside a step"},
+                               "note": "per-plan (cluster or parameter set) powder FID service time inside a step"},
         "kernels": {k: {"ms_total": v["ms"], "launches": v["launches"],
                         "tflops": v["flops"] / max(v["ms"], 1e-9) / 1e9}
                     for k, v in sorted(agg.items(), key=lambda kv: -kv[1]["ms"])},
@@ -231,9 +288,9 @@ def main():
         },
     }
     if not args.no_accuracy:
-        out["rel_l2_vs_oracle"] = accuracy(wls)
+        out["rel_l2_vs_oracle"] = accuracy_ptop(wls) if ptop else accuracy(wls)
     if args.cpu_sample != "none":
-        cb = cpu_baseline(wls, args.cpu_sample)
+        cb = cpu_baseline_ptop(wls[0]) if ptop else cpu_baseline(wls, args.cpu_sample)
         out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
         out["speedup_vs_cpu_core"] = value / cb["value"]
     print(json.dumps(out), flush=True)

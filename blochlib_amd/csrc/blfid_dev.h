@@ -35,4 +35,10 @@ struct DevPlan {
     const double *d_val;   // complex, interleaved
     const int *r_rowptr, *r_col;
     const double *r_val;   // complex, interleaved
+    // point-to-point propagation with rf
+    int usign;             // step propagator exp(usign * i theta H): -1 gamma-COMPUTE, +1 ptop
+    int akeep;             // Dyson products kept per crystallite (n, or 2 when only A_n is used)
+    int nrfiso;            // pulsed isotopes
+    const int *spin_rf;    // S: pulsed-isotope slot of each spin or -1
+    const double *rf;      // n x nrfiso x 3: amp/2, phase, offset of each step
 };

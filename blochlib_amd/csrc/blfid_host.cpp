@@ -59,7 +59,8 @@ int blfid_dev_launch_static_h(const DevPlan *P, int Kb, int c0, const int *grp, 
                               hipStream_t st);
 int blfid_dev_launch_static_fid(int M, const DevPlan *P, int Kb, const int *pgrp, int ngrp,
                                 const double2 *bufW, const double *lam, double2 *scratch,
-                                size_t slot, int nslots, double dt, double *fidc, hipStream_t st);
+                                size_t slot, int nslots, double w0, int prune, double *fidc,
+                                hipStream_t st);
 }
 
 static thread_local std::string g_err;
@@ -166,6 +167,11 @@ struct blfid_plan {
     double wr = 0, rotor_deg = 0, sw = 0, t0 = 0;
     int npts = 0;
     bool spinning = false;
+    bool ptop = false;          // point-to-point pulse-program FID (blfid_plan_create_ptop)
+    // rf of a ptop plan: slot per pulsed isotope, n x nrfiso x (amp/2, phase, offset)
+    int nrfiso = 0;
+    std::vector<int> spin_rf;
+    std::vector<double> rf;
     // derived
     int n = 1;
     double sw_adj = 0, tau = 0;
@@ -197,7 +203,8 @@ struct blfid_plan {
     DevBuf<long> d_blk_uoff, d_pair_eoff;
     DevBuf<double> d_dval, d_rval;
     std::map<int, DevBuf<int>> d_bgrp, d_pgrp;
-    DevBuf<int> d_eig_small, d_eig_big;
+    DevBuf<int> d_eig_small, d_eig_big, d_spin_rf;
+    DevBuf<double> d_rf;
     // working storage (carved from the plan's own arena or a shared workspace)
     DevBuf<char> own_ws;
     blfid_workspace *ws = nullptr;
@@ -324,6 +331,19 @@ static blfid_status build_tables(blfid_plan *pl) {
         while (par[x] != x) x = par[x] = par[par[x]];
         return x;
     };
+    // rf couples the two Zeeman states of every pulsed spin (single flips)
+    for (int i = 0; i < S && pl->nrfiso > 0; i++) {
+        int q = pl->spin_rf[i];
+        if (q < 0) continue;
+        bool on = false;
+        for (int k = 0; k < pl->n && !on; k++) on = pl->rf[(size_t)3 * (k * pl->nrfiso + q)] != 0.0;
+        if (!on) continue;
+        int bi = 1 << (S - 1 - i);
+        for (int s = 0; s < N; s++) {
+            int a = find(s), b = find(s ^ bi);
+            if (a != b) par[a] = b;
+        }
+    }
     for (int i = 0; i < S; i++)
         for (int j = i + 1; j < S; j++) {
             if (!ffpair[i][j]) continue;
@@ -414,7 +434,7 @@ static blfid_status build_tables(blfid_plan *pl) {
                         " exceeds the engine's limit of " + std::to_string(DEV_MAX_BLOCK));
     // rotor direction (row 2 of cartSpin) at each step midpoint
     pl->u3.clear();
-    if (pl->spinning) {
+    if (pl->spinning || pl->ptop) {
         double br = pl->rotor_deg * DEG2RAD;
         double t1 = pl->t0, t2 = pl->t0 + pl->tau;
         for (int k = 0; k < pl->n; k++) {
@@ -450,6 +470,8 @@ static blfid_status upload_tables(blfid_plan *pl) {
     for (auto &kv : pl->blk_groups) HIPCHK(pl->d_bgrp[kv.first].upload(kv.second));
     HIPCHK(pl->d_eig_small.upload(pl->eig_small));
     HIPCHK(pl->d_eig_big.upload(pl->eig_big));
+    HIPCHK(pl->d_spin_rf.upload(pl->spin_rf));
+    HIPCHK(pl->d_rf.upload(pl->rf));
     return BLFID_OK;
 }
 
@@ -485,6 +507,11 @@ static void refresh_devplan(blfid_plan *pl) {
     d.r_rowptr = pl->d_rrow.p;
     d.r_col = pl->d_rcol.p;
     d.r_val = pl->d_rval.p;
+    d.usign = pl->ptop ? 1 : -1;
+    d.akeep = pl->ptop ? 2 : pl->n;
+    d.nrfiso = pl->nrfiso;
+    d.spin_rf = pl->d_spin_rf.p;
+    d.rf = pl->d_rf.p;
 }
 
 // ------------------------------------------------------------------------------ C ABI
@@ -493,9 +520,12 @@ extern "C" {
 const char *blfid_last_error(void) { return g_err.c_str(); }
 const char *blfid_version(void) { return "blfid-mi355x 0.1 (gfx950, FP64)"; }
 
-blfid_status blfid_plan_create(const blfid_system *sys, double wr_hz, double rotor_deg,
-                               double sw_hz, int npts, double t0, blfid_precision prec,
-                               blfid_plan **out) {
+}
+
+// Argument checks shared by both plan kinds (the BLEXCEPTION cases of oneFID.h:78-83,
+// csa.cc:413-418 and the spin-index checks of the interaction readers).
+static blfid_status check_system(const blfid_system *sys, int npts, blfid_precision prec,
+                                 blfid_plan **out) {
     if (!out) return fail(BLFID_ERR_INVALID, "out is NULL");
     *out = nullptr;
     if (!sys) return fail(BLFID_ERR_INVALID, "sys is NULL");
@@ -529,6 +559,12 @@ blfid_status blfid_plan_create(const blfid_system *sys, double wr_hz, double rot
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
         return fail(BLFID_ERR_DEVICE, "no HIP device available");
+    return BLFID_OK;
+}
+
+static blfid_plan *new_plan(const blfid_system *sys, double wr_hz, double rotor_deg, int npts,
+                            double t0) {
+    const int S = sys->nspins;
     blfid_plan *pl = new blfid_plan();
     pl->S = S;
     pl->N = 1 << S;
@@ -538,9 +574,32 @@ blfid_status blfid_plan_create(const blfid_system *sys, double wr_hz, double rot
     pl->jc.assign(sys->jc, sys->jc + 4 * sys->nj);
     pl->wr = wr_hz;
     pl->rotor_deg = rotor_deg;
-    pl->sw = sw_hz;
     pl->t0 = t0;
     pl->npts = npts;
+    return pl;
+}
+
+static blfid_status finish_plan(blfid_plan *pl, blfid_plan **out) {
+    blfid_status st = build_tables(pl);
+    if (st == BLFID_OK) st = upload_tables(pl);
+    if (st != BLFID_OK) {
+        delete pl;
+        return st;
+    }
+    refresh_devplan(pl);
+    *out = pl;
+    return BLFID_OK;
+}
+
+extern "C" {
+
+blfid_status blfid_plan_create(const blfid_system *sys, double wr_hz, double rotor_deg,
+                               double sw_hz, int npts, double t0, blfid_precision prec,
+                               blfid_plan **out) {
+    blfid_status cs = check_system(sys, npts, prec, out);
+    if (cs != BLFID_OK) return cs;
+    blfid_plan *pl = new_plan(sys, wr_hz, rotor_deg, npts, t0);
+    pl->sw = sw_hz;
     pl->spinning = !(wr_hz == 0.0 || rotor_deg == 0.0);
     if (pl->spinning) {
         int n = (int)std::floor(sw_hz / wr_hz + 0.5);
@@ -565,15 +624,83 @@ blfid_status blfid_plan_create(const blfid_system *sys, double wr_hz, double rot
         pl->sw_adj = sw_hz;
         pl->tau = 1.0 / sw_hz;
     }
-    blfid_status st = build_tables(pl);
-    if (st == BLFID_OK) st = upload_tables(pl);
-    if (st != BLFID_OK) {
-        delete pl;
-        return st;
+    return finish_plan(pl, out);
+}
+
+blfid_status blfid_plan_create_ptop(const blfid_system *sys, double wr_hz, double rotor_deg,
+                                    int steps, int npts, double t0, int nrf, const blfid_rf *rf,
+                                    blfid_precision prec, blfid_plan **out) {
+    blfid_status cs = check_system(sys, npts, prec, out);
+    if (cs != BLFID_OK) return cs;
+    if (!(wr_hz > 0.0) || !std::isfinite(wr_hz))
+        return fail(BLFID_ERR_INVALID, "ptop needs a positive rotor speed (one period per point)");
+    if (steps < 1) return fail(BLFID_ERR_INVALID, "steps per rotor period must be >= 1");
+    if (nrf < 0 || (nrf > 0 && !rf)) return fail(BLFID_ERR_INVALID, "bad rf channel list");
+    for (int c = 0; c < nrf; c++)
+        if (!rf[c].amp_hz || !rf[c].phase_rad || !rf[c].offset_hz)
+            return fail(BLFID_ERR_INVALID, "rf channel with a NULL table");
+    const int S = sys->nspins;
+    // one slot per pulsed isotope that labels at least one spin
+    std::vector<int> slot_iso;
+    std::vector<int> spin_rf(S, -1);
+    for (int c = 0; c < nrf; c++) {
+        bool has = false;
+        for (int i = 0; i < S; i++) has |= sys->isotope[i] == rf[c].isotope;
+        if (!has) continue; // no spin of that symbol: the channel adds nothing
+        if (std::find(slot_iso.begin(), slot_iso.end(), rf[c].isotope) == slot_iso.end())
+            slot_iso.push_back(rf[c].isotope);
     }
-    refresh_devplan(pl);
-    *out = pl;
-    return BLFID_OK;
+    for (int i = 0; i < S; i++)
+        for (size_t q = 0; q < slot_iso.size(); q++)
+            if (sys->isotope[i] == slot_iso[q]) spin_rf[i] = (int)q;
+    // the phase is applied as exp(-i phi Fz_q); that commutes with every other term unless a
+    // strong J couples a pulsed spin to a spin of another isotope
+    for (int q = 0; q < sys->nj; q++) {
+        const double *p = sys->jc + 4 * q;
+        int i = (int)p[1], j = (int)p[2];
+        if ((int)p[3] != 0 && p[0] != 0.0 && sys->isotope[i] != sys->isotope[j] &&
+            (spin_rf[i] >= 0 || spin_rf[j] >= 0))
+            return fail(BLFID_ERR_UNSUPPORTED,
+                        "strong heteronuclear J coupling on a pulsed isotope is not supported");
+    }
+    const int nq = (int)slot_iso.size();
+    std::vector<double> tab((size_t)steps * nq * 3, 0.0);
+    for (int k = 0; k < steps; k++)
+        for (int q = 0; q < nq; q++) {
+            // channels with amp != 0 on this isotope (pulsedata.cc:114); several channels on one
+            // isotope add as amp e^{i phi}
+            int cnt = 0;
+            double a1 = 0, p1 = 0;
+            cplx c(0.0, 0.0);
+            double off = 0;
+            for (int ch = 0; ch < nrf; ch++) {
+                if (rf[ch].isotope != slot_iso[q]) continue;
+                double a = rf[ch].amp_hz[k], ph = rf[ch].phase_rad[k], o = rf[ch].offset_hz[k];
+                if (!std::isfinite(a) || !std::isfinite(ph) || !std::isfinite(o))
+                    return fail(BLFID_ERR_INVALID, "non-finite rf table entry");
+                if (a == 0.0) continue;
+                cnt++;
+                a1 = a;
+                p1 = ph;
+                c += a * std::exp(cplx(0.0, ph));
+                off += o;
+            }
+            double *t = &tab[(size_t)3 * (k * nq + q)];
+            if (cnt == 1) { t[0] = 0.5 * a1; t[1] = p1; }
+            else if (cnt > 1) { t[0] = 0.5 * std::abs(c); t[1] = std::arg(c); }
+            t[2] = off;
+        }
+    blfid_plan *pl = new_plan(sys, wr_hz, rotor_deg, npts, t0);
+    pl->ptop = true;
+    pl->spinning = false;
+    pl->n = steps;
+    pl->tau = (1.0 / wr_hz) / steps;
+    pl->sw = wr_hz;
+    pl->sw_adj = wr_hz;
+    pl->nrfiso = nq;
+    pl->spin_rf = spin_rf;
+    pl->rf = tab;
+    return finish_plan(pl, out);
 }
 
 void blfid_plan_destroy(blfid_plan *pl) {
@@ -612,6 +739,7 @@ blfid_status blfid_plan_set_operators(blfid_plan *pl, const double *rho_d, const
         for (int j = 0; j < N; j++) tr += D(i, j) * std::conj(D(i, j));
     pl->trdd = tr;
     // rho_eff: det + det^H for the rosym spinning case (compute.h:381-382), rho otherwise
+    // (the static and ptop FIDs propagate rho itself)
     auto RE = [&](int i, int j) {
         if (pl->spinning && rosym) return D(i, j) + std::conj(D(j, i));
         return R(i, j);
@@ -739,6 +867,12 @@ static double pair_flops(const blfid_plan *pl, const std::vector<int> &g) {
 }
 
 
+static double pair_elems(const blfid_plan *pl, const std::vector<int> &g) {
+    double s = 0;
+    for (int p : g) s += (double)pl->blk_dim[pl->pairs[p].I] * pl->blk_dim[pl->pairs[p].J];
+    return s;
+}
+
 static size_t layout_bytes(const blfid_plan *pl, int Kb, size_t off[10]);
 static size_t fixed_bytes(const blfid_plan *pl);
 
@@ -759,7 +893,12 @@ blfid_status blfid_plan_stats(const blfid_plan *pl, double *out) {
         pairsQ += bI * bJ * (bI + bJ);
     }
     double ex = 0;
-    if (pl->spinning) {
+    if (pl->ptop) {
+        ex += 2.0 * 7.0 * s3;        // exp: 7 real GEMMs (no squaring counted)
+        ex += 8.0 * s3;              // Dyson
+        ex += 8.0 * pairsQ / n;      // D and rho into the period eigenbasis
+        ex += 8.0 * P * pairs / n;   // eigenbasis FID (complex MAC per element and point)
+    } else if (pl->spinning) {
         ex += 2.0 * 7.0 * s3;   // exp: 7 real GEMMs (no squaring counted)
         ex += 8.0 * s3;         // Dyson
         ex += 8.0 * s3;         // B = A W
@@ -768,7 +907,8 @@ blfid_status blfid_plan_stats(const blfid_plan *pl, double *out) {
         ex += 8.0 * P / n * pairs; // FID
     }
     out[0] = ex;
-    out[1] = 48.0 * N * N * N + 14.0 * n * N * N + 14.0 * P * N * N / n;
+    if (pl->ptop) out[1] = 8.0 * N * N * N + 16.0 * N * N * N * P / n;
+    else out[1] = 48.0 * N * N * N + 14.0 * n * N * N + 14.0 * P * N * N / n;
     out[2] = (double)pl->blocks.size();
     out[3] = (double)pl->bmax;
     out[4] = bytes_per_crystallite(pl);
@@ -783,7 +923,12 @@ static size_t layout_bytes(const blfid_plan *pl, int Kb, size_t off[10]) {
     const size_t L = (size_t)pl->Lsum, bm2 = (size_t)pl->bmax * pl->bmax;
     const size_t c16 = sizeof(double2), d8 = sizeof(double);
     size_t sz[10];
-    if (pl->spinning) {
+    if (pl->ptop) {
+        sz[0] = (size_t)Kb * pl->n * L * c16;                              // U
+        sz[1] = (size_t)Kb * 2 * L * c16;                                  // A (two slots)
+        sz[2] = 0; sz[3] = 0;
+        sz[4] = (size_t)pl->exp_slots * 9 * bm2 * d8;                     // exp scratch
+    } else if (pl->spinning) {
         sz[0] = (size_t)Kb * pl->n * L * c16;                              // U / B
         sz[1] = (size_t)Kb * pl->n * L * c16;                              // A
         sz[2] = std::max<size_t>(1, (size_t)Kb * pl->n * pl->Etot) * c16;  // Qt
@@ -868,11 +1013,42 @@ blfid_status blfid_plan_run(blfid_plan *pl, int normalize, void *stream) {
     LAUNCHCHK(blfid_dev_launch_zero(pl->comp.p, 2 * pl->npts, st));
     const size_t bm2 = (size_t)pl->bmax * pl->bmax;
     const int ff = pl->rosym == 1 ? 2 : 1;
-    const double scale = pl->spinning ? 1.0 / (double)(pl->n / ff) : 1.0;
+    const double scale = (pl->spinning && !pl->ptop) ? 1.0 / (double)(pl->n / ff) : 1.0;
     for (int c0 = 0; c0 < K; c0 += Kb) {
         int kb = std::min(Kb, K - c0);
         LAUNCHCHK(blfid_dev_launch_zero(pl->fidc, (size_t)kb * 2 * pl->npts, st));
-        if (pl->spinning) {
+        if (pl->ptop) {
+            // U_k = exp(+i theta (H_k + H_rf,k)); A_n = U_n ... U_1; its eigenbasis gives
+            // rho_m = A_n^m rho A_n^-m in closed form
+            for (auto &kv : pl->blk_groups) {
+                PhaseScope ps(pl, 0 * 8 + kv.first, kb * (double)pl->n * group_b3(pl, kv.second) * 14.0, st);
+                LAUNCHCHK(blfid_dev_launch_exp(kv.first, P, kb, c0, pl->d_bgrp[kv.first].p,
+                                               (int)kv.second.size(), pl->scrE, bm2, pl->exp_slots,
+                                               pl->bufU, st));
+            }
+            for (auto &kv : pl->blk_groups) {
+                PhaseScope ps(pl, 1 * 8 + kv.first, kb * (double)(pl->n - 1) * group_b3(pl, kv.second) * 8.0, st);
+                LAUNCHCHK(blfid_dev_launch_dyson(kv.first, P, kb, pl->d_bgrp[kv.first].p,
+                                                 (int)kv.second.size(), pl->bufU, pl->bufA, st));
+            }
+            for (int big = 0; big < 2; big++) {
+                const std::vector<int> &lst = big ? pl->eig_big : pl->eig_small;
+                if (lst.empty()) continue;
+                PhaseScope ps(pl, 2 * 8 + big, 0.0, st);
+                LAUNCHCHK(blfid_dev_launch_eig(P, kb, big ? pl->d_eig_big.p : pl->d_eig_small.p,
+                                               (int)lst.size(),
+                                               pl->bufA + (size_t)((pl->n - 1) % 2) * pl->Lsum,
+                                               (size_t)2 * pl->Lsum, 0, pl->bufW, pl->thetas,
+                                               big, pl->scrEig, bm2, pl->eig_slots, st));
+            }
+            for (auto &kv : pl->pair_groups) {
+                PhaseScope ps(pl, 7 * 8 + kv.first, kb * (pair_flops(pl, kv.second) + 8.0 * pl->npts * pair_elems(pl, kv.second)), st);
+                LAUNCHCHK(blfid_dev_launch_static_fid(kv.first, P, kb, pl->d_pgrp[kv.first].p,
+                                                      (int)kv.second.size(), pl->bufW,
+                                                      pl->thetas, pl->scrQ, bm2, pl->qr_slots,
+                                                      1.0, 0, pl->fidc, st));
+            }
+        } else if (pl->spinning) {
             for (auto &kv : pl->blk_groups) {
                 PhaseScope ps(pl, 0 * 8 + kv.first, kb * (double)pl->n * group_b3(pl, kv.second) * 14.0, st);
                 LAUNCHCHK(blfid_dev_launch_exp(kv.first, P, kb, c0, pl->d_bgrp[kv.first].p,
@@ -929,7 +1105,7 @@ blfid_status blfid_plan_run(blfid_plan *pl, int normalize, void *stream) {
                 LAUNCHCHK(blfid_dev_launch_static_fid(kv.first, P, kb, pl->d_pgrp[kv.first].p,
                                                       (int)kv.second.size(), pl->bufW,
                                                       pl->thetas, pl->scrQ, bm2, pl->qr_slots,
-                                                      1.0 / pl->sw, pl->fidc, st));
+                                                      -PI2 / pl->sw, 1, pl->fidc, st));
             }
         }
         {

@@ -221,8 +221,14 @@ __global__ void k_prep(int K, const double *__restrict__ th, const double *__res
 // ------------------------------------------------------------------- Hamiltonian block build
 // zz of every anisotropic term for lab direction v = R_P^T u, u = row 2 of cartSpin
 // (csa.cc:316-319, dip.cc:196-199): zz_t = v^T C_t v, C_t in the crystal frame.
+// rfk (nullptr without rf): this step's amp/2, phase, offset per pulsed isotope.  The rf
+// enters as (amp/2) on every single flip of a pulsed spin and offset * m_i on the diagonal:
+// PulseData::Pulse (pulsedata.cc:110-125) with its phase taken out as the diagonal similarity
+// exp(-i phi Fz) (amp (cos phi Ix + sin phi Iy) = e^{-i phi Fz} amp Ix e^{i phi Fz}), which
+// k_exp applies to the propagator.
 __device__ void build_h_block(const DevPlan &P, const double *RPc, const double *u, int blk,
-                              double *H, int ld, double *s_zz, double *s_ff, double *s_v) {
+                              double *H, int ld, double *s_zz, double *s_ff, double *s_v,
+                              const double *rfk) {
     const int tid = threadIdx.x;
     if (tid == 0) {
         for (int a = 0; a < 3; a++)
@@ -264,13 +270,23 @@ __device__ void build_h_block(const DevPlan &P, const double *RPc, const double 
                 if (j >= 0) f *= ((ga >> (P.S - 1 - j)) & 1) ? -0.5 : 0.5;
                 h += s_zz[t] * f;
             }
+            if (rfk) {
+                for (int i = 0; i < P.S; i++) {
+                    int q = P.spin_rf[i];
+                    if (q >= 0) h += rfk[3 * q + 2] * (((ga >> (P.S - 1 - i)) & 1) ? -0.5 : 0.5);
+                }
+            }
         } else {
             int x = ga ^ gb;
-            if (__popc(x) == 2 && (ga & x) != 0 && (ga & x) != x) {
+            int pc = __popc(x);
+            if (pc == 2 && (ga & x) != 0 && (ga & x) != x) {
                 int p1 = 31 - __clz(x);
                 int p0 = __ffs(x) - 1;
                 int i = P.S - 1 - p1, j = P.S - 1 - p0;
                 h = s_ff[i * P.S + j];
+            } else if (pc == 1 && rfk) {
+                int q = P.spin_rf[P.S - __ffs(x)];
+                if (q >= 0) h = rfk[3 * q];
             }
         }
         H[ra + (size_t)cb_ * ld] = h;
@@ -286,6 +302,10 @@ __device__ void build_h_block(const DevPlan &P, const double *RPc, const double 
 // for cos (a_j = (-1)^j/(2j)!) and sin(X)/X (a_j = (-1)^j/(2j+1)!): 7 real GEMMs
 // (Y, Y^2, Y^3, Y^4, two Y^4 products, X * sinc), truncation error < 1e-18, then s squarings
 // (C,S) -> (C^2 - S^2, 2 C S).  Nine b x b scratch matrices per workgroup slot.
+// The output is C + i usign S, i.e. exp(usign i theta H): usign = -1 for gamma-COMPUTE
+// (Mexp(H, -i 2 pi tau), compute.h:265), +1 for the pulse-program propagator (Mexp(hh,
+// +i tau 2 pi), propogation.cc:273).  With rf the step's phase similarity R = diag(p_r),
+// p_r = exp(-i sum_q phi_q M_q(r)) (M_q = Fz of pulsed isotope q), gives U = R U0 R^H.
 template <int M>
 __global__ void __launch_bounds__(NThr<M>::v) k_exp(DevPlan P, int Kb, int c0,
                                                     const int *__restrict__ grp, int ngrp,
@@ -297,6 +317,7 @@ __global__ void __launch_bounds__(NThr<M>::v) k_exp(DevPlan P, int Kb, int c0,
     __shared__ double s_ff[DEV_MAX_SPINS * DEV_MAX_SPINS];
     __shared__ double s_v[4];
     __shared__ double s_red[NT];
+    __shared__ cd s_ph[DEV_MAX_BLOCK];
     const int tid = threadIdx.x;
     double *X = scratch + slot * 9 * blockIdx.x;
     double *Y = X + slot, *Y2 = Y + slot, *Y3 = Y2 + slot, *Y4 = Y3 + slot;
@@ -314,7 +335,9 @@ __global__ void __launch_bounds__(NThr<M>::v) k_exp(DevPlan P, int Kb, int c0,
         const int blk = grp[g];
         const int b = P.blk_dim[blk];
         const int bb = b * b;
-        build_h_block(P, P.RP + 9 * (size_t)(c0 + c), P.u3 + 3 * k, blk, X, b, s_zz, s_ff, s_v);
+        const double *rfk = P.nrfiso > 0 ? P.rf + (size_t)3 * P.nrfiso * k : nullptr;
+        build_h_block(P, P.RP + 9 * (size_t)(c0 + c), P.u3 + 3 * k, blk, X, b, s_zz, s_ff, s_v,
+                      rfk);
         double rmax = 0.0;
         for (int r = tid; r < b; r += NT) {
             double sm = 0.0;
@@ -367,13 +390,32 @@ __global__ void __launch_bounds__(NThr<M>::v) k_exp(DevPlan P, int Kb, int c0,
             __syncthreads();
         }
         cd *U = bufU + ((size_t)c * P.n + k) * P.Lsum + P.blk_uoff[blk];
-        for (int e = tid; e < bb; e += NT) U[e] = cmk(Cm[e], -Sm[e]);
+        const double us = (double)P.usign;
+        if (rfk) {
+            const int *st = P.blk_states + P.blk_soff[blk];
+            for (int r = tid; r < b; r += NT) {
+                double phi = 0.0;
+                for (int i = 0; i < P.S; i++) {
+                    int q = P.spin_rf[i];
+                    if (q >= 0) phi += rfk[3 * q + 1] * (((st[r] >> (P.S - 1 - i)) & 1) ? -0.5 : 0.5);
+                }
+                s_ph[r] = cexpi(-phi);
+            }
+            __syncthreads();
+            for (int e = tid; e < bb; e += NT)
+                U[e] = cmul(cmul(s_ph[e % b], cmk(Cm[e], us * Sm[e])), cconj(s_ph[e / b]));
+        } else {
+            for (int e = tid; e < bb; e += NT) U[e] = cmk(Cm[e], us * Sm[e]);
+        }
         __syncthreads();
     }
 }
 
 // ------------------------------------------------------------------------------ k_dyson
-// One task = (crystallite, block) chain: A_1 = U_1, A_{k+1} = U_{k+1} A_k (compute.h:267).
+// One task = (crystallite, block) chain: A_1 = U_1, A_{k+1} = U_{k+1} A_k (compute.h:267;
+// U *= X is U = X U in propogation.cc:273, 288).  Product k goes to slot k % akeep of the
+// crystallite's akeep slots: all of them for gamma-COMPUTE, two alternating ones when only
+// the period propagator A_n is needed.
 template <int M>
 __global__ void __launch_bounds__(NThr<M>::v) k_dyson(DevPlan P, int Kb, const int *__restrict__ grp,
                                                       int ngrp, const cd *__restrict__ bufU,
@@ -385,14 +427,15 @@ __global__ void __launch_bounds__(NThr<M>::v) k_dyson(DevPlan P, int Kb, const i
         const int g = (int)(task % ngrp), c = (int)(task / ngrp);
         const int blk = grp[g], b = P.blk_dim[blk];
         const size_t base = (size_t)c * P.n * P.Lsum + P.blk_uoff[blk];
+        const size_t abase = (size_t)c * P.akeep * P.Lsum + P.blk_uoff[blk];
         const cd *U0 = bufU + base;
-        cd *A0 = bufA + base;
+        cd *A0 = bufA + abase;
         for (int e = threadIdx.x; e < b * b; e += NT) A0[e] = U0[e];
         __syncthreads();
         for (int k = 1; k < P.n; k++) {
             const cd *Uk = bufU + base + (size_t)k * P.Lsum;
-            const cd *Ap = bufA + base + (size_t)(k - 1) * P.Lsum;
-            cd *Ak = bufA + base + (size_t)k * P.Lsum;
+            const cd *Ap = bufA + abase + (size_t)((k - 1) % P.akeep) * P.Lsum;
+            cd *Ak = bufA + abase + (size_t)(k % P.akeep) * P.Lsum;
             wg_gemm<M, NT, cd>(b, b, b, Uk, b, false, Ap, b, false, Ak, b, cmk(0, 0), smem);
         }
     }
@@ -805,22 +848,27 @@ __global__ void __launch_bounds__(256) k_static_h(DevPlan P, int Kb, int c0, con
     for (long task = blockIdx.x; task < ntask; task += gridDim.x) {
         const int g = (int)(task % ngrp), c = (int)(task / ngrp);
         const int blk = grp[g], b = P.blk_dim[blk];
-        build_h_block(P, P.RP + 9 * (size_t)(c0 + c), P.u3, blk, X, b, s_zz, s_ff, s_v);
+        build_h_block(P, P.RP + 9 * (size_t)(c0 + c), P.u3, blk, X, b, s_zz, s_ff, s_v, nullptr);
         cd *H = bufH + (size_t)c * P.Lsum + P.blk_uoff[blk];
         for (int e = threadIdx.x; e < b * b; e += 256) H[e] = cmk(X[e], 0.0);
         __syncthreads();
     }
 }
 
-// Static FID of one (crystallite, block pair): with V the eigenvectors (bufW) and lambda the
-// eigenvalues, A(i,j) = Do(i,j) sig0(j,i), kept when |A|^2 > 1e-10 (oneFID.h:317-325), and
-// fid[t] += A(i,j) exp(-i 2 pi dt t (lambda_j - lambda_i)) (conj(e_i) e_j)^t.
+// Eigenbasis FID of one (crystallite, block pair): with V the eigenvectors (bufW) and lam the
+// eigenvalues or eigenphases, A(i,j) = Do(i,j) sig0(j,i) and
+// fid[t] += A(i,j) exp(i w0 t (lam_j - lam_i)).
+//   static FID:  lam = energies of H, w0 = -2 pi dt, A kept when |A|^2 > 1e-10 (oneFID.h:
+//                317-325): fid[t] += A(i,j) (conj(e_i) e_j)^t;
+//   ptop FID:    lam = eigenphases of the period propagator U, w0 = 1, no pruning: this is
+//                Tr(rho_t det) with rho_{t+1} = U rho_t U^H (sequencerun.cc:164-196).
 template <int M>
 __global__ void __launch_bounds__(NThr<M>::v) k_static_fid(DevPlan P, int Kb, const int *__restrict__ pgrp,
                                                     int ngrp, const cd *__restrict__ bufW,
                                                     const double *__restrict__ lam,
                                                     cd *__restrict__ scratch, size_t slot,
-                                                    double dt, double *__restrict__ fidc) {
+                                                    double w0, int prune,
+                                                    double *__restrict__ fidc) {
     constexpr int NT = NThr<M>::v;
     __shared__ cd smem[2 * 16 * 16 * M];
     const int tid = threadIdx.x;
@@ -864,11 +912,10 @@ __global__ void __launch_bounds__(NThr<M>::v) k_static_fid(DevPlan P, int Kb, co
         for (int e = tid; e < bI * bJ; e += NT) {
             int r = e % bI, s = e / bI;
             cd a = cmul(Gq[e], Gr[s + (size_t)r * bJ]);
-            if (a.x * a.x + a.y * a.y > 1.0e-10) T1[e] = a;
+            if (!prune || a.x * a.x + a.y * a.y > 1.0e-10) T1[e] = a;
             else T1[e] = cmk(0, 0);
         }
         __syncthreads();
-        const double w0 = -2.0 * 3.14159265358979323846 * dt;
         for (int t = tid; t < P.npts; t += NT) {
             cd acc = cmk(0, 0);
             for (int e = 0; e < bI * bJ; e++) {
@@ -1015,17 +1062,17 @@ extern "C" int blfid_dev_launch_static_h(const DevPlan *P, int Kb, int c0, const
 
 extern "C" int blfid_dev_launch_static_fid(int M, const DevPlan *P, int Kb, const int *pgrp,
                                            int ngrp, const cd *bufW, const double *lam,
-                                           cd *scratch, size_t slot, int nslots, double dt,
-                                           double *fidc, hipStream_t st) {
+                                           cd *scratch, size_t slot, int nslots, double w0,
+                                           int prune, double *fidc, hipStream_t st) {
     long ntask = (long)Kb * ngrp;
     int grid = (int)(ntask < nslots ? ntask : nslots);
     if (grid <= 0) return 0;
     switch (M) {
-    case 1: k_static_fid<1><<<grid, 64, 0, st>>>(*P, Kb, pgrp, ngrp, bufW, lam, scratch, slot, dt, fidc); break;
-    case 2: k_static_fid<2><<<grid, 256, 0, st>>>(*P, Kb, pgrp, ngrp, bufW, lam, scratch, slot, dt, fidc); break;
-    case 3: k_static_fid<3><<<grid, 256, 0, st>>>(*P, Kb, pgrp, ngrp, bufW, lam, scratch, slot, dt, fidc); break;
-    case 4: k_static_fid<4><<<grid, 256, 0, st>>>(*P, Kb, pgrp, ngrp, bufW, lam, scratch, slot, dt, fidc); break;
-    default: k_static_fid<5><<<grid, 256, 0, st>>>(*P, Kb, pgrp, ngrp, bufW, lam, scratch, slot, dt, fidc); break;
+    case 1: k_static_fid<1><<<grid, 64, 0, st>>>(*P, Kb, pgrp, ngrp, bufW, lam, scratch, slot, w0, prune, fidc); break;
+    case 2: k_static_fid<2><<<grid, 256, 0, st>>>(*P, Kb, pgrp, ngrp, bufW, lam, scratch, slot, w0, prune, fidc); break;
+    case 3: k_static_fid<3><<<grid, 256, 0, st>>>(*P, Kb, pgrp, ngrp, bufW, lam, scratch, slot, w0, prune, fidc); break;
+    case 4: k_static_fid<4><<<grid, 256, 0, st>>>(*P, Kb, pgrp, ngrp, bufW, lam, scratch, slot, w0, prune, fidc); break;
+    default: k_static_fid<5><<<grid, 256, 0, st>>>(*P, Kb, pgrp, ngrp, bufW, lam, scratch, slot, w0, prune, fidc); break;
     }
     return (int)hipGetLastError();
 }

@@ -9,6 +9,11 @@ Front end (mirrors src/QMspins/driver/oneFID.h:52-218):
     sig = fid.FID(powder, ro, det)             # whole powder on the GPU, oneFID.h:200-208
     sig1 = fid.FID(ro, det, theta, phi, gamma) # one crystallite, oneFID.h:210-218
     fid.sw()                                   # adjusted sweep width n*wr
+
+Pulse programs (solid-2.0's ptop(); propogation.cc:240-304, sequencerun_funcs.cc:624-655):
+
+    sig = ptop_fid(sys, wr, rotor, steps, npts, powder, ro, det,
+                   rf=[("1H", amp[steps], phase_rad[steps], offset[steps])])
 """
 from __future__ import annotations
 
@@ -42,6 +47,11 @@ class _System(ctypes.Structure):
                 ("jc", _d)]
 
 
+class _RF(ctypes.Structure):
+    _fields_ = [("isotope", ctypes.c_int), ("amp_hz", _d), ("phase_rad", _d),
+                ("offset_hz", _d)]
+
+
 _lib = None
 
 
@@ -56,6 +66,10 @@ def lib():
         L.blfid_plan_create.argtypes = [ctypes.POINTER(_System), ctypes.c_double, ctypes.c_double,
                                         ctypes.c_double, ctypes.c_int, ctypes.c_double,
                                         ctypes.c_int, ctypes.POINTER(vp)]
+        L.blfid_plan_create_ptop.argtypes = [ctypes.POINTER(_System), ctypes.c_double,
+                                             ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                             ctypes.c_double, ctypes.c_int, ctypes.POINTER(_RF),
+                                             ctypes.c_int, ctypes.POINTER(vp)]
         L.blfid_plan_destroy.argtypes = [vp]
         L.blfid_plan_destroy.restype = None
         L.blfid_plan_sw.argtypes = [vp]
@@ -124,7 +138,7 @@ class Plan:
     """One spin system + acquisition on the device (blfid_plan_*)."""
 
     def __init__(self, sy: SolidSys, wr: float, rotor_deg: float, sw: float, npts: int,
-                 t0: float = 0.0):
+                 t0: float = 0.0, *, _ptop=None):
         L = lib()
         iso, csa, dip, jc = sy.tables()
         self._keep = [np.ascontiguousarray(iso, np.int32),
@@ -133,11 +147,35 @@ class Plan:
         s = _System(sy.spins, _p(self._keep[0], _i), len(sy.csa), _p(self._keep[1]), len(sy.dip),
                     _p(self._keep[2]), len(sy.jc), _p(self._keep[3]))
         h = ctypes.c_void_p()
-        _check(L.blfid_plan_create(ctypes.byref(s), float(wr), float(rotor_deg), float(sw),
-                                   int(npts), float(t0), 0, ctypes.byref(h)))
+        if _ptop is None:
+            _check(L.blfid_plan_create(ctypes.byref(s), float(wr), float(rotor_deg), float(sw),
+                                       int(npts), float(t0), 0, ctypes.byref(h)))
+        else:
+            steps, rf = _ptop
+            ids = {}
+            for n_ in sy.isotopes:
+                ids.setdefault(n_, len(ids))
+            tabs, chans = [], []
+            for name, amp, ph, off in rf:
+                t = [np.ascontiguousarray(np.broadcast_to(np.asarray(v, np.float64), (steps,)))
+                     for v in (amp, ph, off)]
+                tabs.append(t)
+                chans.append(_RF(ids.get(name, -1), _p(t[0]), _p(t[1]), _p(t[2])))
+            arr = (_RF * max(1, len(chans)))(*chans)
+            self._rf = (tabs, arr)
+            _check(L.blfid_plan_create_ptop(ctypes.byref(s), float(wr), float(rotor_deg),
+                                            int(steps), int(npts), float(t0), len(chans), arr,
+                                            0, ctypes.byref(h)))
         self.h = h
         self.N = sy.N
         self.npts = int(npts)
+
+    @classmethod
+    def ptop(cls, sy: SolidSys, wr: float, rotor_deg: float, steps: int, npts: int, rf=(),
+             t0: float = 0.0) -> "Plan":
+        """Rotor-synchronised pulse-program plan (blfid_plan_create_ptop).  rf: channels
+        (isotope name, amp_hz, phase_rad, offset_hz), each a scalar or one value per step."""
+        return cls(sy, wr, rotor_deg, 0.0, npts, t0, _ptop=(int(steps), list(rf)))
 
     def close(self):
         if getattr(self, "h", None) and self.h.value:
@@ -229,6 +267,16 @@ def powder_fid(sy, wr, rotor_deg, sw, npts, pw, rho, det, t0=0.0, normalize=True
     try:
         f = p.powder_fid(pw, rho, det, normalize)
         return f, p.sw
+    finally:
+        p.close()
+
+
+def ptop_fid(sy, wr, rotor_deg, steps, npts, pw, rho, det, rf=(), t0=0.0, normalize=True):
+    """Powder-summed ptop FID of a one-rotor-period pulse program on the GPU (one point per
+    period): sum_k w_k Tr(rho_m det) [/ Tr(det det^H)], rho_{m+1} = U rho_m U^H."""
+    p = Plan.ptop(sy, wr, rotor_deg, steps, npts, rf, t0)
+    try:
+        return p.powder_fid(pw, rho, det, normalize)
     finally:
         p.close()
 

@@ -109,3 +109,86 @@ def config3() -> Workload:
 
 def small_cluster(nspins: int, seed: int) -> SolidSys:
     return cluster(nspins, np.random.default_rng(seed))
+
+
+@dataclass
+class PulseWorkload:
+    """A rotor-synchronised pulse program: one rotor period of `steps` equal steps with rf
+    channels (isotope, amp_hz[steps], phase_rad[steps], offset_hz[steps]), one FID point per
+    period (solid-2.0's ptop())."""
+    name: str
+    system: SolidSys
+    wr: float
+    rotor: float
+    steps: int
+    npts: int
+    powder_size: int
+    rf: list
+    rho: str = "Ix_0"
+    det: str = "Ip_0"
+    meta: dict = field(default_factory=dict)
+
+    def powder(self) -> Powder:
+        return zcw(self.powder_size)
+
+    def operators(self):
+        return self.system.op(self.rho), self.system.op(self.det)
+
+
+def phase_alternating_rf(isotope: str, steps: int, amp: float, cycles: int) -> list:
+    """rf channel of constant amplitude whose phase alternates 0 / 180 degrees every half
+    cycle, `cycles` full cycles per rotor period (steps must be a multiple of 2 * cycles)."""
+    if steps % (2 * cycles):
+        raise ValueError("steps per period must be a multiple of 2 * cycles")
+    half = steps // (2 * cycles)
+    ph = np.where((np.arange(steps) // half) % 2 == 0, 0.0, np.pi)
+    return [(isotope, np.full(steps, float(amp)), ph, np.zeros(steps))]
+
+
+def ch6_geometry(seed: int = 1000 * 4, r_ch: float = 1.09, dmin_hh: float = 1.0):
+    """Six hydrogen positions at r_ch around a carbon at the origin, directions uniform on the
+    sphere with rejection when two hydrogens are closer than dmin_hh."""
+    rng = np.random.default_rng(seed)
+    pos = []
+    while len(pos) < 6:
+        u = rng.normal(size=3)
+        p = r_ch * u / np.linalg.norm(u)
+        if all(np.linalg.norm(p - q) >= dmin_hh for q in pos):
+            pos.append(p)
+    return np.array(pos)
+
+
+def config4_system(set_index: int, geometry=None) -> SolidSys:
+    """One 13C + six 1H (N=128) parameter set: seeded geometry shared by all sets, J(C-H) =
+    140 Hz, seeded CSA per set (13C delta 2-12 kHz, 1H delta 1-8 kHz, eta uniform [0, 1],
+    Euler angles uniform)."""
+    H = ch6_geometry() if geometry is None else geometry
+    rng = np.random.default_rng(1000 * 4 + 1 + set_index)
+    sy = SolidSys(isotopes=["13C"] + ["1H"] * 6)
+    sy.add_csa(0.0, float(rng.uniform(2000.0, 12000.0)), float(rng.uniform(0.0, 1.0)), 0,
+               *_euler_deg(rng))
+    for i in range(6):
+        sy.add_csa(0.0, float(rng.uniform(1000.0, 8000.0)), float(rng.uniform(0.0, 1.0)), i + 1,
+                   *_euler_deg(rng))
+    for i in range(6):
+        r = np.linalg.norm(H[i])
+        sy.add_dip(B_CH / r ** 3, 0, i + 1, *_dip_angles(H[i]))
+    for i in range(6):
+        for j in range(i + 1, 6):
+            d = H[j] - H[i]
+            sy.add_dip(B_HH / np.linalg.norm(d) ** 3, i + 1, j + 1, *_dip_angles(d))
+    for i in range(6):
+        sy.add_j(140.0, 0, i + 1)
+    return sy
+
+
+def config4(nsets: int = 512) -> list:
+    """13C + 6 1H under MAS 12.5 kHz with 100 kHz phase-alternating 1H rf (8 cycles per
+    period), 256 steps per period, ZCW 1154, 2048 rotor-synchronised points, rho = Ix_0,
+    det = Ip_0; one workload per CSA parameter set."""
+    geo = ch6_geometry()
+    steps = 256
+    rf = phase_alternating_rf("1H", steps, 100000.0, 8)
+    return [PulseWorkload(f"config4-CH6-set{i:03d}", config4_system(i, geo), 12500.0, MAGIC_DEG,
+                          steps, 2048, 1154, rf, meta={"set": i})
+            for i in range(nsets)]

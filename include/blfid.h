@@ -69,6 +69,31 @@ blfid_status blfid_plan_create(const blfid_system *sys, double wr_hz, double rot
                                blfid_plan **out);
 void blfid_plan_destroy(blfid_plan *plan);
 
+/* One rf channel of a pulse program over the steps of one rotor period: on step k every spin
+ * whose isotope label equals `isotope` gets amp_hz[k] (cos(phase_rad[k]) Ix + sin(phase_rad[k])
+ * Iy) + offset_hz[k] Iz, and a step with amp_hz[k] == 0 adds nothing for this channel, its
+ * offset included (PulseData::Pulse, examples/solid-2.0/src/pulsedata.cc:110-125; the script's
+ * phase in degrees is converted to radians at pulsedata.cc:213-215). */
+typedef struct {
+    int isotope;
+    const double *amp_hz;
+    const double *phase_rad;
+    const double *offset_hz;
+} blfid_rf;
+
+/* Plan for a rotor-synchronised point-to-point ("ptop") FID with rf pulses: the sub-sequence is
+ * one rotor period 1/wr cut into `steps` equal steps; step k uses H(t1, t2, wr) at the midpoint
+ * of [t0 + k tau, t0 + (k+1) tau], tau = 1/(wr steps), plus the rf of every channel, and
+ * U <- exp(+i 2 pi tau (H + H_rf)) U.  FID point m records Tr(rho_m det), rho_{m+1} = U rho_m U^H.
+ * Replaces Propogation::propogator (examples/solid-2.0/src/propogation.cc:240-304),
+ * subSequence::propogate (sequencerun.cc:164-196) and the ptop branch of SequenceRun::doFID
+ * (sequencerun_funcs.cc:624-655).  The sampling interval (blfid_plan_sw) is wr.  A strong
+ * (I.S) J coupling between different isotopes where one of them is pulsed is rejected with
+ * BLFID_ERR_UNSUPPORTED. */
+blfid_status blfid_plan_create_ptop(const blfid_system *sys, double wr_hz, double rotor_deg,
+                                    int steps, int npts, double t0, int nrf, const blfid_rf *rf,
+                                    blfid_precision prec, blfid_plan **out);
+
 /* Adjusted sweep width n*wr (compute.h:96, oneFID.h:429), steps per rotor period n, N. */
 double blfid_plan_sw(const blfid_plan *plan);
 int blfid_plan_steps(const blfid_plan *plan);

@@ -15,6 +15,10 @@
 //   oneFID::FID(ro, det, th, ph, ga) oneFID.h:210-218  FID(ro, det, th, ph, ga)
 //   oneFID::sw() (adjusted n*wr) oneFID.h:159, :429    sw()
 //   BL_exception blassert.h:52-79                      blfid::BL_exception
+//   solid-2.0 pulse sub-sequence + ptop() + fid()       blfid::pulseSequence + ptopFID
+//     PulseData::Pulse pulsedata.cc:110-125,             (one rotor period of equal steps,
+//     Propogation::propogator propogation.cc:240-304,    whole powder on GPU)
+//     ptop doFID sequencerun_funcs.cc:624-655
 //
 // Operators are dense complex matrices, column-major, spin 0 the leftmost Kronecker factor.
 #pragma once
@@ -238,6 +242,15 @@ class SolidSys {
         return out;
     }
 
+    // label of an isotope in abi() tables, -1 when no spin has it
+    int isotopeId(const std::string &name) const {
+        std::map<std::string, int> ids;
+        for (int i = 0; i < spins(); i++)
+            if (ids.find(names_[i]) == ids.end()) ids.insert({names_[i], (int)ids.size()});
+        auto it = ids.find(name);
+        return it == ids.end() ? -1 : it->second;
+    }
+
     // flat tables for the C ABI
     blfid_system abi() const {
         iso_ids_.assign(spins(), 0);
@@ -394,6 +407,85 @@ template <class Func_T = SolidSys> class oneFID {
     Func_T *function_;
     int npts_;
     double sw_, wr_, rotb_, startT_;
+};
+
+// rf over one rotor period of `steps` equal steps: one channel per pulsed isotope.  pulse()
+// takes the script's units: amplitude and offset in Hz, phase in degrees (converted to radians
+// as pulsedata.cc:213-215 does).  A step where a channel's amplitude is 0 adds nothing for it.
+class pulseSequence {
+  public:
+    explicit pulseSequence(int steps) : steps_(steps) {
+        if (steps < 1) throw BL_exception("a pulse sequence needs at least one step");
+    }
+    int steps() const { return steps_; }
+    // steps [k0, k1) of isotope `on` get amp (cos(phase) Ix + sin(phase) Iy) + offset Iz
+    void pulse(const std::string &on, int k0, int k1, double amp, double phase_deg,
+               double offset = 0.0) {
+        if (k0 < 0 || k1 > steps_ || k0 > k1) throw BL_exception("pulse steps out of range");
+        channel &c = chan(on);
+        for (int k = k0; k < k1; k++) {
+            c.amp[k] = amp;
+            c.phase[k] = phase_deg * 0.017453292519943295769236907684886;
+            c.offset[k] = offset;
+        }
+    }
+    // list of blfid_rf for a system (channels on isotopes absent from it map to label -1)
+    template <class Func_T> std::vector<blfid_rf> abi(const Func_T &sys) const {
+        std::vector<blfid_rf> out;
+        for (auto &c : chans_)
+            out.push_back({sys.isotopeId(c.on), c.amp.data(), c.phase.data(), c.offset.data()});
+        return out;
+    }
+
+  private:
+    struct channel {
+        std::string on;
+        std::vector<double> amp, phase, offset;
+    };
+    channel &chan(const std::string &on) {
+        for (auto &c : chans_)
+            if (c.on == on) return c;
+        chans_.push_back({on, std::vector<double>(steps_, 0.0), std::vector<double>(steps_, 0.0),
+                          std::vector<double>(steps_, 0.0)});
+        return chans_.back();
+    }
+    int steps_;
+    std::vector<channel> chans_;
+};
+
+// Rotor-synchronised point-to-point acquisition of a one-period pulse program: point m is
+// sum_k w_k Tr(rho_m det) [/ Tr(det det^H) when normalize, as doFID does], rho_{m+1} =
+// U rho_m U^H, U = prod_k exp(+i 2 pi tau (H_k + H_rf,k)).
+template <class Func_T = SolidSys> class ptopFID {
+  public:
+    ptopFID(Func_T &sys, int npts, double wr, double rotb, const pulseSequence &seq,
+            double startT = 0)
+        : function_(&sys), npts_(npts), wr_(wr), rotb_(rotb), startT_(startT), seq_(seq) {}
+
+    std::vector<complex> FID(const powder &pw, const matrix &ro, const matrix &det,
+                             bool normalize = true) {
+        if (npts_ <= 0)
+            throw BL_exception("Fid size was '0' please use 'npts(int)' set set it");
+        blfid_system s = function_->abi();
+        std::vector<blfid_rf> rf = seq_.abi(*function_);
+        blfid_plan *plan = nullptr;
+        check(blfid_plan_create_ptop(&s, wr_, rotb_, seq_.steps(), npts_, startT_, (int)rf.size(),
+                                     rf.data(), BLFID_FP64, &plan));
+        std::vector<complex> out(npts_);
+        blfid_status st = blfid_powder_fid(plan, pw.thetas(), pw.phis(), pw.gammas(),
+                                           pw.weights(), pw.size(), ro.data(), det.data(),
+                                           normalize ? 1 : 0,
+                                           reinterpret_cast<double *>(out.data()));
+        blfid_plan_destroy(plan);
+        check(st);
+        return out;
+    }
+
+  private:
+    Func_T *function_;
+    int npts_;
+    double wr_, rotb_, startT_;
+    pulseSequence seq_;
 };
 
 } // namespace blfid

@@ -12,6 +12,8 @@ from blochlib_amd.spinsys import SolidSys
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 BIN = os.path.join(ROOT, "tests", "cpp", "frontend_check")
+HET = ("numspin 3\nT 13C 0\nT 1H 1\nT 1H 2\nC 200 4000 0.3 0 10 20 30\nC 0 3000 0.6 1 40 50 60\n"
+       "D -20000 0 1 0 30 40\nD -15000 0 2 0 70 10\nD -25000 1 2 0 50 80\nJ 140 0 1\n")
 SPINS = ("numspin 2\nT 1H 0\nT 1H 1\nC 1000 5000 0.5 0 10 20 30\nC -1000 5000 0.5 1 40 50 60\n"
          "D -20000 0 1\n")
 
@@ -51,4 +53,24 @@ def test_frontend_gpu_matches_oracle():
     ref, swr = O.powder_fid(sy, 10000.0, 54.7356, 640000.0, 256, pw, sy.op("Ix"), sy.op("Ip"),
                             normalize=False)
     assert sw == swr
+    assert np.linalg.norm(fid - ref) / np.linalg.norm(ref) < 1e-9
+
+
+@pytest.mark.gpu
+def test_frontend_ptop_matches_oracle():
+    """pulseSequence + ptopFID: pulse() takes degrees; the oracle gets the same table in rad."""
+    from oracle import oracle as O
+    b = build_binary()
+    r = subprocess.run([b, "ptop", P.CRYSTALS], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    fid = np.array([complex(*map(float, l.split())) for l in r.stdout.strip().splitlines()])
+    sy = SolidSys.from_text(HET)
+    n = 32
+    ph = np.where((np.arange(n) // 4) % 2 == 1, np.radians(180.0), 0.0)
+    c_amp = np.where(np.arange(n) >= 16, 7000.0, 0.0)
+    rf = [("1H", np.full(n, 60000.0), ph, np.zeros(n)),
+          ("13C", c_amp, np.where(np.arange(n) >= 16, np.radians(17.0), 0.0),
+           np.where(np.arange(n) >= 16, 300.0, 0.0))]
+    pw = P.zcw(1154).subset(np.arange(7, 1154, 163))
+    ref = O.ptop_fid(sy, 12500.0, 54.7356, n, 40, pw, sy.op("Ix_0"), sy.op("Ip_0"), rf=rf)
     assert np.linalg.norm(fid - ref) / np.linalg.norm(ref) < 1e-9
