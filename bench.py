@@ -37,6 +37,7 @@ sys.path.insert(0, ROOT)
 from blochlib_amd import engine, workloads  # noqa: E402
 
 FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
 METRIC = "crystallite-steps/s + FIDs/s at ≤1e-5 rel L2 vs FP64 oracle; p50/p99 latency"
 
@@ -125,17 +126,18 @@ def cpu_baseline_ptop(w):
     }
 
 
-def accuracy_ptop(wls, ncryst=2, npts=64):
+def accuracy_ptop(wls, precision, ncryst=2, npts=64):
     from oracle import oracle as O
     w = wls[0]
     pw = w.powder().subset(np.arange(ncryst) * 577 % w.powder_size)
     rho, det = w.operators()
-    g = engine.ptop_fid(w.system, w.wr, w.rotor, w.steps, npts, pw, rho, det, rf=w.rf)
+    g = engine.ptop_fid(w.system, w.wr, w.rotor, w.steps, npts, pw, rho, det, rf=w.rf,
+                        precision=precision)
     c = O.ptop_fid(w.system, w.wr, w.rotor, w.steps, npts, pw, rho, det, rf=w.rf)
     return float(np.linalg.norm(g - c) / np.linalg.norm(c))
 
 
-def accuracy(wls, limit_spins=6, ncryst=2):
+def accuracy(wls, precision, limit_spins=6, ncryst=2):
     """rel-L2 of the engine vs the FP64 oracle on a few crystallites of the small clusters."""
     from oracle import oracle as O
     worst = 0.0
@@ -147,7 +149,8 @@ def accuracy(wls, limit_spins=6, ncryst=2):
         seen.add(S)
         pw = w.powder().subset(np.arange(ncryst) * 997 % w.powder_size)
         rho, det = w.operators()
-        g, _ = engine.powder_fid(w.system, w.wr, w.rotor, w.sw, w.npts, pw, rho, det)
+        g, _ = engine.powder_fid(w.system, w.wr, w.rotor, w.sw, w.npts, pw, rho, det,
+                                 precision=precision)
         c, _ = O.powder_fid(w.system, w.wr, w.rotor, w.sw, w.npts, pw, rho, det)
         worst = max(worst, float(np.linalg.norm(g - c) / np.linalg.norm(c)))
     return worst
@@ -164,6 +167,9 @@ def main():
     ap.add_argument("--workspace-gb", type=float, default=120.0)
     ap.add_argument("--workload", choices=["batched", "ptop"], default="batched")
     ap.add_argument("--sets", type=int, default=8, help="parameter sets per step (ptop)")
+    ap.add_argument("--precision", choices=["fp64", "c64"], default="fp64",
+                    help="complex128 throughout, or complex64 propagators/products with FP64 "
+                         "eigen-solver, phases and reductions")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this engine runs on one GPU; multi-GPU runs are not part of this build "
@@ -179,9 +185,10 @@ def main():
     t0 = time.perf_counter()
     for w in wls:
         if ptop:
-            p = engine.Plan.ptop(w.system, w.wr, w.rotor, w.steps, w.npts, w.rf)
+            p = engine.Plan.ptop(w.system, w.wr, w.rotor, w.steps, w.npts, w.rf,
+                                 precision=args.precision)
         else:
-            p = engine.Plan(w.system, w.wr, w.rotor, w.sw, w.npts)
+            p = engine.Plan(w.system, w.wr, w.rotor, w.sw, w.npts, precision=args.precision)
         rho, det = w.operators()
         p.set_operators(rho, det)
         p.set_powder(w.powder())
@@ -234,6 +241,7 @@ def main():
     avg_ms = d["ms"] / d["launches"]
     achieved = d["flops"] / d["launches"] / (avg_ms * 1e-3) / 1e12
     value = cs_per_step * args.steps / elapsed
+    peak = FP64_PEAK_TFLOPS if args.precision == "fp64" else FP32_PEAK_TFLOPS
     out = {
         "metric": METRIC,
         "value": value,
@@ -245,7 +253,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.precision == "fp64" else "c64 (f32 products, f64 eig/phases/sums)",
         "data": "synthetic",
         "config": ({
             "workload": ("heteronuclear rf: 13C + 6 1H (N=128), MAS 12.5 kHz, magic angle, "
@@ -278,17 +286,19 @@ def main():
             "kernel": dom,
             "bound": "mfma",
             "achieved": achieved,
-            "peak": FP64_PEAK_TFLOPS,
+            "peak": peak,
             "unit": "TFLOP/s",
-            "frac": achieved / FP64_PEAK_TFLOPS,
+            "frac": achieved / peak,
             "traffic": None,
             "avg_launch_ms": avg_ms,
             "flops_per_launch": d["flops"] / d["launches"],
-            "note": "FP64 executed algorithmic flops of the block-diagonal algorithm",
+            "note": ("executed algorithmic flops of the block-diagonal algorithm; peak = MI355X "
+                     + ("FP64" if args.precision == "fp64" else "FP32") + " dense"),
         },
     }
     if not args.no_accuracy:
-        out["rel_l2_vs_oracle"] = accuracy_ptop(wls) if ptop else accuracy(wls)
+        out["rel_l2_vs_oracle"] = (accuracy_ptop(wls, args.precision) if ptop
+                                   else accuracy(wls, args.precision))
     if args.cpu_sample != "none":
         cb = cpu_baseline_ptop(wls[0]) if ptop else cpu_baseline(wls, args.cpu_sample)
         out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}

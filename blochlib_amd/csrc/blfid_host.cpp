@@ -32,19 +32,21 @@
 typedef std::complex<double> cplx;
 
 extern "C" {
-int blfid_dev_launch_exp(int M, const DevPlan *P, int Kb, int c0, const int *grp, int ngrp,
-                         double *scratch, size_t slot, int nslots, double2 *bufU, hipStream_t st);
-int blfid_dev_launch_dyson(int M, const DevPlan *P, int Kb, const int *grp, int ngrp,
-                           const double2 *bufU, double2 *bufA, hipStream_t st);
-int blfid_dev_launch_eig(const DevPlan *P, int Kb, const int *grp, int ngrp, const double2 *src,
-                         size_t src_stride, int mode, double2 *bufW, double *outv, int big,
-                         double2 *gscr, size_t gslot, int nslots, hipStream_t st);
-int blfid_dev_launch_bmul(int M, const DevPlan *P, int Kb, const int *grp, int ngrp,
-                          const double2 *bufA, const double2 *bufW, double2 *bufB, hipStream_t st);
-int blfid_dev_launch_qr(int M, const DevPlan *P, int Kb, const int *pgrp, int ngrp,
-                        const double2 *bufB, double2 *scratch, size_t slot, int nslots,
-                        double2 *Qt, double2 *Rt, hipStream_t st);
-int blfid_dev_launch_corr(const DevPlan *P, int Kb, const double2 *Qt, const double2 *Rt,
+int blfid_dev_launch_exp(int prec, int M, const DevPlan *P, int Kb, int c0, const int *grp,
+                         int ngrp, void *scratch, size_t slot, int nslots, void *bufU,
+                         hipStream_t st);
+int blfid_dev_launch_dyson(int prec, int M, const DevPlan *P, int Kb, const int *grp, int ngrp,
+                           const void *bufU, void *bufA, hipStream_t st);
+int blfid_dev_launch_eig(int prec, const DevPlan *P, int Kb, const int *grp, int ngrp,
+                         const void *src, size_t src_stride, int mode, double2 *bufW,
+                         void *bufWr, double *outv, int big, double2 *gscr, size_t gslot,
+                         int nslots, hipStream_t st);
+int blfid_dev_launch_bmul(int prec, int M, const DevPlan *P, int Kb, const int *grp, int ngrp,
+                          const void *bufA, const void *bufW, void *bufB, hipStream_t st);
+int blfid_dev_launch_qr(int prec, int M, const DevPlan *P, int Kb, const int *pgrp, int ngrp,
+                        const void *bufB, void *scratch, size_t slot, int nslots, void *Qt,
+                        void *Rt, hipStream_t st);
+int blfid_dev_launch_corr(int prec, const DevPlan *P, int Kb, const void *Qt, const void *Rt,
                           const double *thetas, const int *er, const int *es, double *fidc,
                           hipStream_t st);
 int blfid_dev_launch_reduce(int Kb, int c0, int npts, const double *w, double scale,
@@ -166,6 +168,7 @@ struct blfid_plan {
     std::vector<double> csa, dip, jc;
     double wr = 0, rotor_deg = 0, sw = 0, t0 = 0;
     int npts = 0;
+    int prec = 0;               // 0: complex128 storage, 1: complex64 storage (BLFID_C64)
     bool spinning = false;
     bool ptop = false;          // point-to-point pulse-program FID (blfid_plan_create_ptop)
     // rf of a ptop plan: slot per pulsed isotope, n x nrfiso x (amp/2, phase, offset)
@@ -208,9 +211,11 @@ struct blfid_plan {
     // working storage (carved from the plan's own arena or a shared workspace)
     DevBuf<char> own_ws;
     blfid_workspace *ws = nullptr;
-    double2 *bufU = nullptr, *bufA = nullptr, *bufW = nullptr, *bufQt = nullptr,
-            *bufRt = nullptr, *scrQ = nullptr;
-    double *thetas = nullptr, *scrE = nullptr, *fidc = nullptr;
+    // U, A, Qt, Rt, Wr and the exp / Q-R scratch are in the storage precision (char* = bytes)
+    char *bufU = nullptr, *bufA = nullptr, *bufQt = nullptr, *bufRt = nullptr, *scrQ = nullptr,
+         *scrE = nullptr, *bufWr = nullptr;
+    double2 *bufW = nullptr;
+    double *thetas = nullptr, *fidc = nullptr;
     double2 *scrEig = nullptr;
     DevBuf<double> acc, comp, fid_out;
     double budget = 0;
@@ -529,7 +534,8 @@ static blfid_status check_system(const blfid_system *sys, int npts, blfid_precis
     if (!out) return fail(BLFID_ERR_INVALID, "out is NULL");
     *out = nullptr;
     if (!sys) return fail(BLFID_ERR_INVALID, "sys is NULL");
-    if (prec != BLFID_FP64) return fail(BLFID_ERR_UNSUPPORTED, "only BLFID_FP64 is implemented");
+    if (prec != BLFID_FP64 && prec != BLFID_C64)
+        return fail(BLFID_ERR_UNSUPPORTED, "precision not implemented (BLFID_FP64, BLFID_C64)");
     if (sys->nspins < 1 || sys->nspins > DEV_MAX_SPINS)
         return fail(BLFID_ERR_INVALID, "nspins must be in [1, 12]");
     if (npts <= 0)
@@ -599,6 +605,7 @@ blfid_status blfid_plan_create(const blfid_system *sys, double wr_hz, double rot
     blfid_status cs = check_system(sys, npts, prec, out);
     if (cs != BLFID_OK) return cs;
     blfid_plan *pl = new_plan(sys, wr_hz, rotor_deg, npts, t0);
+    pl->prec = prec == BLFID_C64 ? 1 : 0;
     pl->sw = sw_hz;
     pl->spinning = !(wr_hz == 0.0 || rotor_deg == 0.0);
     if (pl->spinning) {
@@ -691,6 +698,7 @@ blfid_status blfid_plan_create_ptop(const blfid_system *sys, double wr_hz, doubl
             t[2] = off;
         }
     blfid_plan *pl = new_plan(sys, wr_hz, rotor_deg, npts, t0);
+    pl->prec = prec == BLFID_C64 ? 1 : 0;
     pl->ptop = true;
     pl->spinning = false;
     pl->n = steps;
@@ -873,11 +881,11 @@ static double pair_elems(const blfid_plan *pl, const std::vector<int> &g) {
     return s;
 }
 
-static size_t layout_bytes(const blfid_plan *pl, int Kb, size_t off[10]);
+static size_t layout_bytes(const blfid_plan *pl, int Kb, size_t off[11]);
 static size_t fixed_bytes(const blfid_plan *pl);
 
 static double bytes_per_crystallite(const blfid_plan *pl) {
-    size_t off[10];
+    size_t off[11];
     return (double)(layout_bytes(pl, 1, off) - fixed_bytes(pl));
 }
 
@@ -919,33 +927,36 @@ blfid_status blfid_plan_stats(const blfid_plan *pl, double *out) {
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 // Byte layout of the working storage for a batch of Kb crystallites (fixed scratch first).
-static size_t layout_bytes(const blfid_plan *pl, int Kb, size_t off[10]) {
+static size_t layout_bytes(const blfid_plan *pl, int Kb, size_t off[11]) {
     const size_t L = (size_t)pl->Lsum, bm2 = (size_t)pl->bmax * pl->bmax;
     const size_t c16 = sizeof(double2), d8 = sizeof(double);
-    size_t sz[10];
+    const size_t cs = pl->prec ? 8 : 16, rs = pl->prec ? 4 : 8; // storage complex / real
+    size_t sz[11];
+    sz[10] = 0;
     if (pl->ptop) {
-        sz[0] = (size_t)Kb * pl->n * L * c16;                              // U
-        sz[1] = (size_t)Kb * 2 * L * c16;                                  // A (two slots)
+        sz[0] = (size_t)Kb * pl->n * L * cs;                               // U
+        sz[1] = (size_t)Kb * 2 * L * cs;                                   // A (two slots)
         sz[2] = 0; sz[3] = 0;
-        sz[4] = (size_t)pl->exp_slots * 9 * bm2 * d8;                     // exp scratch
+        sz[4] = (size_t)pl->exp_slots * 9 * bm2 * rs;                     // exp scratch
     } else if (pl->spinning) {
-        sz[0] = (size_t)Kb * pl->n * L * c16;                              // U / B
-        sz[1] = (size_t)Kb * pl->n * L * c16;                              // A
-        sz[2] = std::max<size_t>(1, (size_t)Kb * pl->n * pl->Etot) * c16;  // Qt
-        sz[3] = std::max<size_t>(1, (size_t)Kb * pl->n * pl->Etot) * c16;  // Rt
-        sz[4] = (size_t)pl->exp_slots * 9 * bm2 * d8;                     // exp scratch
+        sz[0] = (size_t)Kb * pl->n * L * cs;                               // U / B
+        sz[1] = (size_t)Kb * pl->n * L * cs;                               // A
+        sz[2] = std::max<size_t>(1, (size_t)Kb * pl->n * pl->Etot) * cs;   // Qt
+        sz[3] = std::max<size_t>(1, (size_t)Kb * pl->n * pl->Etot) * cs;   // Rt
+        sz[4] = (size_t)pl->exp_slots * 9 * bm2 * rs;                     // exp scratch
+        if (pl->prec) sz[10] = (size_t)Kb * L * cs;                        // W in storage precision
     } else {
-        sz[0] = (size_t)Kb * L * c16;
+        sz[0] = (size_t)Kb * L * c16;                                      // H (FP64)
         sz[1] = 0; sz[2] = 0; sz[3] = 0;
         sz[4] = (size_t)pl->exp_slots * bm2 * d8;
     }
-    sz[5] = (size_t)Kb * L * c16;                 // W
+    sz[5] = (size_t)Kb * L * c16;                 // W (FP64)
     sz[6] = (size_t)Kb * pl->N * d8;              // thetas / eigenvalues
-    sz[7] = (size_t)pl->qr_slots * 3 * bm2 * c16; // qr scratch
+    sz[7] = (size_t)pl->qr_slots * 3 * bm2 * c16; // qr / eigenbasis-FID scratch
     sz[8] = (size_t)Kb * 2 * pl->npts * d8;       // per-crystallite FIDs
     sz[9] = (size_t)pl->eig_slots * bm2 * c16;    // eigen-solver scratch for blocks > LDS
     size_t o = 0;
-    for (int i = 0; i < 10; i++) {
+    for (int i = 0; i < 11; i++) {
         off[i] = o;
         o += align256(sz[i]);
     }
@@ -953,21 +964,22 @@ static size_t layout_bytes(const blfid_plan *pl, int Kb, size_t off[10]) {
 }
 
 static size_t fixed_bytes(const blfid_plan *pl) {
-    size_t off[10];
+    size_t off[11];
     return layout_bytes(pl, 0, off);
 }
 
-static void carve(blfid_plan *pl, char *base, const size_t off[10]) {
-    pl->bufU = (double2 *)(base + off[0]);
-    pl->bufA = (double2 *)(base + off[1]);
-    pl->bufQt = (double2 *)(base + off[2]);
-    pl->bufRt = (double2 *)(base + off[3]);
-    pl->scrE = (double *)(base + off[4]);
+static void carve(blfid_plan *pl, char *base, const size_t off[11]) {
+    pl->bufU = base + off[0];
+    pl->bufA = base + off[1];
+    pl->bufQt = base + off[2];
+    pl->bufRt = base + off[3];
+    pl->scrE = base + off[4];
     pl->bufW = (double2 *)(base + off[5]);
     pl->thetas = (double *)(base + off[6]);
-    pl->scrQ = (double2 *)(base + off[7]);
+    pl->scrQ = base + off[7];
     pl->fidc = (double *)(base + off[8]);
     pl->scrEig = (double2 *)(base + off[9]);
+    pl->bufWr = pl->prec && pl->spinning ? base + off[10] : nullptr;
 }
 
 blfid_status blfid_plan_run(blfid_plan *pl, int normalize, void *stream) {
@@ -977,7 +989,7 @@ blfid_status blfid_plan_run(blfid_plan *pl, int normalize, void *stream) {
     pl->last_stream = st;
     const int K = pl->K;
     // batch size from the workspace / memory budget
-    size_t off[10];
+    size_t off[11];
     const double per = (double)(layout_bytes(pl, 1, off) - fixed_bytes(pl));
     const double fixed = (double)fixed_bytes(pl) + 4096.0;
     double avail;
@@ -1012,6 +1024,8 @@ blfid_status blfid_plan_run(blfid_plan *pl, int normalize, void *stream) {
     LAUNCHCHK(blfid_dev_launch_zero(pl->acc.p, 2 * pl->npts, st));
     LAUNCHCHK(blfid_dev_launch_zero(pl->comp.p, 2 * pl->npts, st));
     const size_t bm2 = (size_t)pl->bmax * pl->bmax;
+    const int pr = pl->prec;
+    const size_t cs = pr ? 8 : 16;
     const int ff = pl->rosym == 1 ? 2 : 1;
     const double scale = (pl->spinning && !pl->ptop) ? 1.0 / (double)(pl->n / ff) : 1.0;
     for (int c0 = 0; c0 < K; c0 += Kb) {
@@ -1022,90 +1036,91 @@ blfid_status blfid_plan_run(blfid_plan *pl, int normalize, void *stream) {
             // rho_m = A_n^m rho A_n^-m in closed form
             for (auto &kv : pl->blk_groups) {
                 PhaseScope ps(pl, 0 * 8 + kv.first, kb * (double)pl->n * group_b3(pl, kv.second) * 14.0, st);
-                LAUNCHCHK(blfid_dev_launch_exp(kv.first, P, kb, c0, pl->d_bgrp[kv.first].p,
+                LAUNCHCHK(blfid_dev_launch_exp(pr, kv.first, P, kb, c0, pl->d_bgrp[kv.first].p,
                                                (int)kv.second.size(), pl->scrE, bm2, pl->exp_slots,
                                                pl->bufU, st));
             }
             for (auto &kv : pl->blk_groups) {
                 PhaseScope ps(pl, 1 * 8 + kv.first, kb * (double)(pl->n - 1) * group_b3(pl, kv.second) * 8.0, st);
-                LAUNCHCHK(blfid_dev_launch_dyson(kv.first, P, kb, pl->d_bgrp[kv.first].p,
+                LAUNCHCHK(blfid_dev_launch_dyson(pr, kv.first, P, kb, pl->d_bgrp[kv.first].p,
                                                  (int)kv.second.size(), pl->bufU, pl->bufA, st));
             }
             for (int big = 0; big < 2; big++) {
                 const std::vector<int> &lst = big ? pl->eig_big : pl->eig_small;
                 if (lst.empty()) continue;
                 PhaseScope ps(pl, 2 * 8 + big, 0.0, st);
-                LAUNCHCHK(blfid_dev_launch_eig(P, kb, big ? pl->d_eig_big.p : pl->d_eig_small.p,
+                LAUNCHCHK(blfid_dev_launch_eig(pr, P, kb, big ? pl->d_eig_big.p : pl->d_eig_small.p,
                                                (int)lst.size(),
-                                               pl->bufA + (size_t)((pl->n - 1) % 2) * pl->Lsum,
-                                               (size_t)2 * pl->Lsum, 0, pl->bufW, pl->thetas,
-                                               big, pl->scrEig, bm2, pl->eig_slots, st));
+                                               pl->bufA + (size_t)((pl->n - 1) % 2) * pl->Lsum * cs,
+                                               (size_t)2 * pl->Lsum, 0, pl->bufW, nullptr,
+                                               pl->thetas, big, pl->scrEig, bm2, pl->eig_slots, st));
             }
             for (auto &kv : pl->pair_groups) {
                 PhaseScope ps(pl, 7 * 8 + kv.first, kb * (pair_flops(pl, kv.second) + 8.0 * pl->npts * pair_elems(pl, kv.second)), st);
                 LAUNCHCHK(blfid_dev_launch_static_fid(kv.first, P, kb, pl->d_pgrp[kv.first].p,
                                                       (int)kv.second.size(), pl->bufW,
-                                                      pl->thetas, pl->scrQ, bm2, pl->qr_slots,
-                                                      1.0, 0, pl->fidc, st));
+                                                      pl->thetas, (double2 *)pl->scrQ, bm2,
+                                                      pl->qr_slots, 1.0, 0, pl->fidc, st));
             }
         } else if (pl->spinning) {
             for (auto &kv : pl->blk_groups) {
                 PhaseScope ps(pl, 0 * 8 + kv.first, kb * (double)pl->n * group_b3(pl, kv.second) * 14.0, st);
-                LAUNCHCHK(blfid_dev_launch_exp(kv.first, P, kb, c0, pl->d_bgrp[kv.first].p,
+                LAUNCHCHK(blfid_dev_launch_exp(pr, kv.first, P, kb, c0, pl->d_bgrp[kv.first].p,
                                                (int)kv.second.size(), pl->scrE, bm2, pl->exp_slots,
                                                pl->bufU, st));
             }
             for (auto &kv : pl->blk_groups) {
                 PhaseScope ps(pl, 1 * 8 + kv.first, kb * (double)(pl->n - 1) * group_b3(pl, kv.second) * 8.0, st);
-                LAUNCHCHK(blfid_dev_launch_dyson(kv.first, P, kb, pl->d_bgrp[kv.first].p,
+                LAUNCHCHK(blfid_dev_launch_dyson(pr, kv.first, P, kb, pl->d_bgrp[kv.first].p,
                                                  (int)kv.second.size(), pl->bufU, pl->bufA, st));
             }
             for (int big = 0; big < 2; big++) {
                 const std::vector<int> &lst = big ? pl->eig_big : pl->eig_small;
                 if (lst.empty()) continue;
                 PhaseScope ps(pl, 2 * 8 + big, 0.0, st);
-                LAUNCHCHK(blfid_dev_launch_eig(P, kb, big ? pl->d_eig_big.p : pl->d_eig_small.p,
+                LAUNCHCHK(blfid_dev_launch_eig(pr, P, kb, big ? pl->d_eig_big.p : pl->d_eig_small.p,
                                                (int)lst.size(),
-                                               pl->bufA + (size_t)(pl->n - 1) * pl->Lsum,
-                                               (size_t)pl->n * pl->Lsum, 0, pl->bufW, pl->thetas,
-                                               big, pl->scrEig, bm2, pl->eig_slots, st));
+                                               pl->bufA + (size_t)(pl->n - 1) * pl->Lsum * cs,
+                                               (size_t)pl->n * pl->Lsum, 0, pl->bufW, pl->bufWr,
+                                               pl->thetas, big, pl->scrEig, bm2, pl->eig_slots, st));
             }
             for (auto &kv : pl->blk_groups) {
                 PhaseScope ps(pl, 3 * 8 + kv.first, kb * (double)pl->n * group_b3(pl, kv.second) * 8.0, st);
-                LAUNCHCHK(blfid_dev_launch_bmul(kv.first, P, kb, pl->d_bgrp[kv.first].p,
-                                                (int)kv.second.size(), pl->bufA, pl->bufW,
+                LAUNCHCHK(blfid_dev_launch_bmul(pr, kv.first, P, kb, pl->d_bgrp[kv.first].p,
+                                                (int)kv.second.size(), pl->bufA,
+                                                pr ? (const void *)pl->bufWr : (const void *)pl->bufW,
                                                 pl->bufU, st));
             }
             for (auto &kv : pl->pair_groups) {
                 PhaseScope ps(pl, 4 * 8 + kv.first, kb * (double)pl->n * pair_flops(pl, kv.second), st);
-                LAUNCHCHK(blfid_dev_launch_qr(kv.first, P, kb, pl->d_pgrp[kv.first].p,
+                LAUNCHCHK(blfid_dev_launch_qr(pr, kv.first, P, kb, pl->d_pgrp[kv.first].p,
                                               (int)kv.second.size(), pl->bufU, pl->scrQ, bm2,
                                               pl->qr_slots, pl->bufQt, pl->bufRt, st));
             }
             if (pl->Etot > 0) {
                 PhaseScope ps(pl, 5 * 8, kb * 8.0 * (double)pl->Etot * ((double)pl->n * pl->n + pl->npts), st);
-                LAUNCHCHK(blfid_dev_launch_corr(P, kb, pl->bufQt, pl->bufRt, pl->thetas,
+                LAUNCHCHK(blfid_dev_launch_corr(pr, P, kb, pl->bufQt, pl->bufRt, pl->thetas,
                                                 pl->d_er.p, pl->d_es.p, pl->fidc, st));
             }
         } else {
             for (auto &kv : pl->blk_groups)
                 LAUNCHCHK(blfid_dev_launch_static_h(P, kb, c0, pl->d_bgrp[kv.first].p,
-                                                    (int)kv.second.size(), pl->scrE, bm2,
-                                                    pl->exp_slots, pl->bufU, st));
+                                                    (int)kv.second.size(), (double *)pl->scrE, bm2,
+                                                    pl->exp_slots, (double2 *)pl->bufU, st));
             for (int big = 0; big < 2; big++) {
                 const std::vector<int> &lst = big ? pl->eig_big : pl->eig_small;
                 if (lst.empty()) continue;
-                LAUNCHCHK(blfid_dev_launch_eig(P, kb, big ? pl->d_eig_big.p : pl->d_eig_small.p,
+                LAUNCHCHK(blfid_dev_launch_eig(0, P, kb, big ? pl->d_eig_big.p : pl->d_eig_small.p,
                                                (int)lst.size(), pl->bufU, (size_t)pl->Lsum, 1,
-                                               pl->bufW, pl->thetas, big, pl->scrEig, bm2,
+                                               pl->bufW, nullptr, pl->thetas, big, pl->scrEig, bm2,
                                                pl->eig_slots, st));
             }
             for (auto &kv : pl->pair_groups) {
                 PhaseScope ps(pl, 7 * 8 + kv.first, 0.0, st);
                 LAUNCHCHK(blfid_dev_launch_static_fid(kv.first, P, kb, pl->d_pgrp[kv.first].p,
                                                       (int)kv.second.size(), pl->bufW,
-                                                      pl->thetas, pl->scrQ, bm2, pl->qr_slots,
-                                                      -PI2 / pl->sw, 1, pl->fidc, st));
+                                                      pl->thetas, (double2 *)pl->scrQ, bm2,
+                                                      pl->qr_slots, -PI2 / pl->sw, 1, pl->fidc, st));
             }
         }
         {

@@ -31,17 +31,25 @@
 #include "blfid_dev.h"
 
 typedef double2 cd;
+typedef float2 cf;
 
 // ------------------------------------------------------------------------------ helpers
+// Complex helpers are templates over double2 / float2 (members x = re, y = im).
 __device__ __forceinline__ cd cmk(double a, double b) { return make_double2(a, b); }
-__device__ __forceinline__ cd cadd(cd a, cd b) { return cmk(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ cd csub(cd a, cd b) { return cmk(a.x - b.x, a.y - b.y); }
-__device__ __forceinline__ cd cmul(cd a, cd b) {
-    return cmk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+template <typename V> __device__ __forceinline__ V cadd(V a, V b) {
+    V r; r.x = a.x + b.x; r.y = a.y + b.y; return r;
 }
-__device__ __forceinline__ cd cconj(cd a) { return cmk(a.x, -a.y); }
-__device__ __forceinline__ cd cscale(cd a, double s) { return cmk(a.x * s, a.y * s); }
-__device__ __forceinline__ void cfma(cd &c, cd a, cd b) {
+template <typename V> __device__ __forceinline__ V csub(V a, V b) {
+    V r; r.x = a.x - b.x; r.y = a.y - b.y; return r;
+}
+template <typename V> __device__ __forceinline__ V cmul(V a, V b) {
+    V r; r.x = a.x * b.x - a.y * b.y; r.y = a.x * b.y + a.y * b.x; return r;
+}
+template <typename V> __device__ __forceinline__ V cconj(V a) { V r; r.x = a.x; r.y = -a.y; return r; }
+template <typename V, typename S> __device__ __forceinline__ V cscale(V a, S s) {
+    V r; r.x = a.x * s; r.y = a.y * s; return r;
+}
+template <typename V> __device__ __forceinline__ void cfma(V &c, V a, V b) {
     c.x = fma(a.x, b.x, c.x);
     c.x = fma(-a.y, b.y, c.x);
     c.y = fma(a.x, b.y, c.y);
@@ -53,6 +61,21 @@ __device__ __forceinline__ cd cexpi(double ph) {
     return cmk(c, s);
 }
 
+// Storage precision R of the propagator pipeline: complex type and conversions to / from FP64.
+template <typename R> struct CX;
+template <> struct CX<double> {
+    typedef cd T;
+    static __device__ __forceinline__ cd mk(double a, double b) { return make_double2(a, b); }
+    static __device__ __forceinline__ cd from(cd a) { return a; }
+};
+template <> struct CX<float> {
+    typedef cf T;
+    static __device__ __forceinline__ cf mk(float a, float b) { return make_float2(a, b); }
+    static __device__ __forceinline__ cf from(cd a) { return make_float2((float)a.x, (float)a.y); }
+};
+__device__ __forceinline__ cd to_cd(cd a) { return a; }
+__device__ __forceinline__ cd to_cd(cf a) { return make_double2((double)a.x, (double)a.y); }
+
 template <typename T> struct Arith;
 template <> struct Arith<double> {
     static __device__ __forceinline__ double zero() { return 0.0; }
@@ -60,11 +83,23 @@ template <> struct Arith<double> {
     static __device__ __forceinline__ void mac(double &c, double a, double b) { c = fma(a, b, c); }
     static __device__ __forceinline__ double add(double a, double b) { return a + b; }
 };
+template <> struct Arith<float> {
+    static __device__ __forceinline__ float zero() { return 0.0f; }
+    static __device__ __forceinline__ float cj(float a) { return a; }
+    static __device__ __forceinline__ void mac(float &c, float a, float b) { c = fmaf(a, b, c); }
+    static __device__ __forceinline__ float add(float a, float b) { return a + b; }
+};
 template <> struct Arith<cd> {
     static __device__ __forceinline__ cd zero() { return cmk(0.0, 0.0); }
     static __device__ __forceinline__ cd cj(cd a) { return cconj(a); }
     static __device__ __forceinline__ void mac(cd &c, cd a, cd b) { cfma(c, a, b); }
     static __device__ __forceinline__ cd add(cd a, cd b) { return cadd(a, b); }
+};
+template <> struct Arith<cf> {
+    static __device__ __forceinline__ cf zero() { return make_float2(0.0f, 0.0f); }
+    static __device__ __forceinline__ cf cj(cf a) { return cconj(a); }
+    static __device__ __forceinline__ void mac(cf &c, cf a, cf b) { cfma(c, a, b); }
+    static __device__ __forceinline__ cf add(cf a, cf b) { return cadd(a, b); }
 };
 
 // ---------------------------------------------------------------- workgroup GEMM
@@ -112,7 +147,9 @@ __device__ void wg_gemm(int m, int n, int k, const T *__restrict__ A, int lda, b
                         T diag, T *smem) {
     constexpr int TM = 16 * M, KS = 16, TY = NT / 16, MJ = TM / TY, LPT = TM * KS / NT;
     // register prefetch of the next K slice unless the accumulators already fill the VGPRs
-    constexpr bool PF = (M * MJ * sizeof(T)) <= 256;
+    // (complex64 with 80 x 80 tiles would spill with it)
+    constexpr bool PF = (M * MJ * sizeof(T)) <= 256 && !(sizeof(T) == 8 && alignof(T) == 8 && M >= 5 &&
+                                                         !__is_same(T, double));
     T *As = smem;
     T *Bs = smem + KS * TM;
     const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
@@ -226,8 +263,9 @@ __global__ void k_prep(int K, const double *__restrict__ th, const double *__res
 // PulseData::Pulse (pulsedata.cc:110-125) with its phase taken out as the diagonal similarity
 // exp(-i phi Fz) (amp (cos phi Ix + sin phi Iy) = e^{-i phi Fz} amp Ix e^{i phi Fz}), which
 // k_exp applies to the propagator.
+template <typename R>
 __device__ void build_h_block(const DevPlan &P, const double *RPc, const double *u, int blk,
-                              double *H, int ld, double *s_zz, double *s_ff, double *s_v,
+                              R *H, int ld, double *s_zz, double *s_ff, double *s_v,
                               const double *rfk) {
     const int tid = threadIdx.x;
     if (tid == 0) {
@@ -289,7 +327,7 @@ __device__ void build_h_block(const DevPlan &P, const double *RPc, const double 
                 if (q >= 0) h = rfk[3 * q];
             }
         }
-        H[ra + (size_t)cb_ * ld] = h;
+        H[ra + (size_t)cb_ * ld] = (R)h;
     }
     __syncthreads();
 }
@@ -306,28 +344,29 @@ __device__ void build_h_block(const DevPlan &P, const double *RPc, const double 
 // (Mexp(H, -i 2 pi tau), compute.h:265), +1 for the pulse-program propagator (Mexp(hh,
 // +i tau 2 pi), propogation.cc:273).  With rf the step's phase similarity R = diag(p_r),
 // p_r = exp(-i sum_q phi_q M_q(r)) (M_q = Fz of pulsed isotope q), gives U = R U0 R^H.
-template <int M>
+template <int M, typename R>
 __global__ void __launch_bounds__(NThr<M>::v) k_exp(DevPlan P, int Kb, int c0,
                                                     const int *__restrict__ grp, int ngrp,
-                                                    double *__restrict__ scratch, size_t slot,
-                                                    cd *__restrict__ bufU) {
+                                                    R *__restrict__ scratch, size_t slot,
+                                                    typename CX<R>::T *__restrict__ bufU) {
     constexpr int NT = NThr<M>::v;
-    __shared__ double smem[2 * 16 * 16 * M];
+    typedef typename CX<R>::T CT;
+    __shared__ R smem[2 * 16 * 16 * M];
     __shared__ double s_zz[DEV_MAX_TERMS];
     __shared__ double s_ff[DEV_MAX_SPINS * DEV_MAX_SPINS];
     __shared__ double s_v[4];
     __shared__ double s_red[NT];
     __shared__ cd s_ph[DEV_MAX_BLOCK];
     const int tid = threadIdx.x;
-    double *X = scratch + slot * 9 * blockIdx.x;
-    double *Y = X + slot, *Y2 = Y + slot, *Y3 = Y2 + slot, *Y4 = Y3 + slot;
-    double *Hc = Y4 + slot, *Hs = Hc + slot, *Cm = Hs + slot, *Ps = Cm + slot;
+    R *X = scratch + slot * 9 * blockIdx.x;
+    R *Y = X + slot, *Y2 = Y + slot, *Y3 = Y2 + slot, *Y4 = Y3 + slot;
+    R *Hc = Y4 + slot, *Hs = Hc + slot, *Cm = Hs + slot, *Ps = Cm + slot;
     const long ntask = (long)Kb * P.n * ngrp;
     const double theta = P.theta;
-    const double cc[8] = {1.0, -1.0 / 2, 1.0 / 24, -1.0 / 720, 1.0 / 40320, -1.0 / 3628800,
-                          1.0 / 479001600, -1.0 / 87178291200.0};
-    const double ss[8] = {1.0, -1.0 / 6, 1.0 / 120, -1.0 / 5040, 1.0 / 362880,
-                          -1.0 / 39916800, 1.0 / 6227020800.0, -1.0 / 1307674368000.0};
+    const R cc[8] = {(R)1.0, (R)(-1.0 / 2), (R)(1.0 / 24), (R)(-1.0 / 720), (R)(1.0 / 40320),
+                     (R)(-1.0 / 3628800), (R)(1.0 / 479001600), (R)(-1.0 / 87178291200.0)};
+    const R ss[8] = {(R)1.0, (R)(-1.0 / 6), (R)(1.0 / 120), (R)(-1.0 / 5040), (R)(1.0 / 362880),
+                     (R)(-1.0 / 39916800), (R)(1.0 / 6227020800.0), (R)(-1.0 / 1307674368000.0)};
     for (long task = blockIdx.x; task < ntask; task += gridDim.x) {
         const int g = (int)(task % ngrp);
         const long ck = task / ngrp;
@@ -341,7 +380,7 @@ __global__ void __launch_bounds__(NThr<M>::v) k_exp(DevPlan P, int Kb, int c0,
         double rmax = 0.0;
         for (int r = tid; r < b; r += NT) {
             double sm = 0.0;
-            for (int q = 0; q < b; q++) sm += fabs(X[r + (size_t)q * b]);
+            for (int q = 0; q < b; q++) sm += fabs((double)X[r + (size_t)q * b]);
             rmax = fmax(rmax, sm);
         }
         s_red[tid] = rmax;
@@ -354,42 +393,43 @@ __global__ void __launch_bounds__(NThr<M>::v) k_exp(DevPlan P, int Kb, int c0,
         __syncthreads();
         int sq = 0;
         while (nrm / (double)(1 << sq) > 0.5 && sq < 30) sq++;
-        const double sc = theta / (double)(1 << sq);
+        const R sc = (R)(theta / (double)(1 << sq));
         for (int e = tid; e < bb; e += NT) X[e] *= sc;
         __syncthreads();
-        wg_gemm<M, NT, double>(b, b, b, X, b, false, X, b, false, Y, b, 0.0, smem);
-        wg_gemm<M, NT, double>(b, b, b, Y, b, false, Y, b, false, Y2, b, 0.0, smem);
-        wg_gemm<M, NT, double>(b, b, b, Y2, b, false, Y, b, false, Y3, b, 0.0, smem);
-        wg_gemm<M, NT, double>(b, b, b, Y2, b, false, Y2, b, false, Y4, b, 0.0, smem);
+        const R z0 = (R)0;
+        wg_gemm<M, NT, R>(b, b, b, X, b, false, X, b, false, Y, b, z0, smem);
+        wg_gemm<M, NT, R>(b, b, b, Y, b, false, Y, b, false, Y2, b, z0, smem);
+        wg_gemm<M, NT, R>(b, b, b, Y2, b, false, Y, b, false, Y3, b, z0, smem);
+        wg_gemm<M, NT, R>(b, b, b, Y2, b, false, Y2, b, false, Y4, b, z0, smem);
         for (int e = tid; e < bb; e += NT) {
-            double d = (e % b == e / b) ? 1.0 : 0.0;
-            double y = Y[e], y2 = Y2[e], y3 = Y3[e];
+            R d = (e % b == e / b) ? (R)1 : (R)0;
+            R y = Y[e], y2 = Y2[e], y3 = Y3[e];
             Hc[e] = cc[4] * d + cc[5] * y + cc[6] * y2 + cc[7] * y3;
             Hs[e] = ss[4] * d + ss[5] * y + ss[6] * y2 + ss[7] * y3;
         }
         __syncthreads();
-        wg_gemm<M, NT, double>(b, b, b, Y4, b, false, Hc, b, false, Cm, b, 0.0, smem);
-        wg_gemm<M, NT, double>(b, b, b, Y4, b, false, Hs, b, false, Ps, b, 0.0, smem);
+        wg_gemm<M, NT, R>(b, b, b, Y4, b, false, Hc, b, false, Cm, b, z0, smem);
+        wg_gemm<M, NT, R>(b, b, b, Y4, b, false, Hs, b, false, Ps, b, z0, smem);
         for (int e = tid; e < bb; e += NT) {
-            double d = (e % b == e / b) ? 1.0 : 0.0;
-            double y = Y[e], y2 = Y2[e], y3 = Y3[e];
+            R d = (e % b == e / b) ? (R)1 : (R)0;
+            R y = Y[e], y2 = Y2[e], y3 = Y3[e];
             Cm[e] += cc[0] * d + cc[1] * y + cc[2] * y2 + cc[3] * y3;
             Ps[e] += ss[0] * d + ss[1] * y + ss[2] * y2 + ss[3] * y3;
         }
         __syncthreads();
-        double *Sm = Y4;
-        wg_gemm<M, NT, double>(b, b, b, X, b, false, Ps, b, false, Sm, b, 0.0, smem);
+        R *Sm = Y4;
+        wg_gemm<M, NT, R>(b, b, b, X, b, false, Ps, b, false, Sm, b, z0, smem);
         for (int it = 0; it < sq; it++) {
-            wg_gemm<M, NT, double>(b, b, b, Cm, b, false, Cm, b, false, Y, b, 0.0, smem);
-            wg_gemm<M, NT, double>(b, b, b, Sm, b, false, Sm, b, false, Y2, b, 0.0, smem);
-            wg_gemm<M, NT, double>(b, b, b, Cm, b, false, Sm, b, false, Y3, b, 0.0, smem);
+            wg_gemm<M, NT, R>(b, b, b, Cm, b, false, Cm, b, false, Y, b, z0, smem);
+            wg_gemm<M, NT, R>(b, b, b, Sm, b, false, Sm, b, false, Y2, b, z0, smem);
+            wg_gemm<M, NT, R>(b, b, b, Cm, b, false, Sm, b, false, Y3, b, z0, smem);
             for (int e = tid; e < bb; e += NT) {
                 Cm[e] = Y[e] - Y2[e];
-                Sm[e] = 2.0 * Y3[e];
+                Sm[e] = (R)2 * Y3[e];
             }
             __syncthreads();
         }
-        cd *U = bufU + ((size_t)c * P.n + k) * P.Lsum + P.blk_uoff[blk];
+        CT *U = bufU + ((size_t)c * P.n + k) * P.Lsum + P.blk_uoff[blk];
         const double us = (double)P.usign;
         if (rfk) {
             const int *st = P.blk_states + P.blk_soff[blk];
@@ -403,9 +443,10 @@ __global__ void __launch_bounds__(NThr<M>::v) k_exp(DevPlan P, int Kb, int c0,
             }
             __syncthreads();
             for (int e = tid; e < bb; e += NT)
-                U[e] = cmul(cmul(s_ph[e % b], cmk(Cm[e], us * Sm[e])), cconj(s_ph[e / b]));
+                U[e] = CX<R>::from(cmul(cmul(s_ph[e % b], cmk((double)Cm[e], us * (double)Sm[e])),
+                                        cconj(s_ph[e / b])));
         } else {
-            for (int e = tid; e < bb; e += NT) U[e] = cmk(Cm[e], us * Sm[e]);
+            for (int e = tid; e < bb; e += NT) U[e] = CX<R>::mk(Cm[e], (R)us * Sm[e]);
         }
         __syncthreads();
     }
@@ -416,27 +457,29 @@ __global__ void __launch_bounds__(NThr<M>::v) k_exp(DevPlan P, int Kb, int c0,
 // U *= X is U = X U in propogation.cc:273, 288).  Product k goes to slot k % akeep of the
 // crystallite's akeep slots: all of them for gamma-COMPUTE, two alternating ones when only
 // the period propagator A_n is needed.
-template <int M>
+template <int M, typename R>
 __global__ void __launch_bounds__(NThr<M>::v) k_dyson(DevPlan P, int Kb, const int *__restrict__ grp,
-                                                      int ngrp, const cd *__restrict__ bufU,
-                                                      cd *__restrict__ bufA) {
+                                                      int ngrp,
+                                                      const typename CX<R>::T *__restrict__ bufU,
+                                                      typename CX<R>::T *__restrict__ bufA) {
     constexpr int NT = NThr<M>::v;
-    __shared__ cd smem[2 * 16 * 16 * M];
+    typedef typename CX<R>::T CT;
+    __shared__ CT smem[2 * 16 * 16 * M];
     const long ntask = (long)Kb * ngrp;
     for (long task = blockIdx.x; task < ntask; task += gridDim.x) {
         const int g = (int)(task % ngrp), c = (int)(task / ngrp);
         const int blk = grp[g], b = P.blk_dim[blk];
         const size_t base = (size_t)c * P.n * P.Lsum + P.blk_uoff[blk];
         const size_t abase = (size_t)c * P.akeep * P.Lsum + P.blk_uoff[blk];
-        const cd *U0 = bufU + base;
-        cd *A0 = bufA + abase;
+        const CT *U0 = bufU + base;
+        CT *A0 = bufA + abase;
         for (int e = threadIdx.x; e < b * b; e += NT) A0[e] = U0[e];
         __syncthreads();
         for (int k = 1; k < P.n; k++) {
-            const cd *Uk = bufU + base + (size_t)k * P.Lsum;
-            const cd *Ap = bufA + abase + (size_t)((k - 1) % P.akeep) * P.Lsum;
-            cd *Ak = bufA + abase + (size_t)(k % P.akeep) * P.Lsum;
-            wg_gemm<M, NT, cd>(b, b, b, Uk, b, false, Ap, b, false, Ak, b, cmk(0, 0), smem);
+            const CT *Uk = bufU + base + (size_t)k * P.Lsum;
+            const CT *Ap = bufA + abase + (size_t)((k - 1) % P.akeep) * P.Lsum;
+            CT *Ak = bufA + abase + (size_t)(k % P.akeep) * P.Lsum;
+            wg_gemm<M, NT, CT>(b, b, b, Uk, b, false, Ap, b, false, Ak, b, Arith<CT>::zero(), smem);
         }
     }
 }
@@ -452,12 +495,16 @@ __global__ void __launch_bounds__(NThr<M>::v) k_dyson(DevPlan P, int Kb, const i
 #define EIG_MAXPAIR 128 /* blocks up to 256 */
 // BIG = false: the block lives in LDS (b <= EIG_MAXB); BIG = true: in a global scratch
 // slot of b*b complex per workgroup (any b <= 2*EIG_MAXPAIR), same algorithm.
-template <bool BIG>
+// The source block may be stored in FP32 (complex64 plans); the solver always runs in FP64 and
+// W stays FP64, with a storage-precision copy Wr for the B = A W products when R != double.
+template <bool BIG, typename R>
 __global__ void __launch_bounds__(256) k_eig(DevPlan P, int Kb, const int *__restrict__ grp,
-                                             int ngrp, const cd *__restrict__ src,
+                                             int ngrp, const typename CX<R>::T *__restrict__ src,
                                              size_t src_stride, int mode, cd *__restrict__ bufW,
+                                             typename CX<R>::T *__restrict__ bufWr,
                                              double *__restrict__ outv, cd *__restrict__ gscr,
                                              size_t gslot) {
+    typedef typename CX<R>::T CT;
     __shared__ cd Mlds[BIG ? 1 : EIG_MAXB * (EIG_MAXB + 1)];
     __shared__ double rc[EIG_MAXPAIR];
     __shared__ cd rs[EIG_MAXPAIR];
@@ -470,11 +517,11 @@ __global__ void __launch_bounds__(256) k_eig(DevPlan P, int Kb, const int *__res
         const int g = (int)(task % ngrp), c = (int)(task / ngrp);
         const int blk = grp[g], b = P.blk_dim[blk];
         const int ld = BIG ? b : b + 1;
-        const cd *S0 = src + (size_t)c * src_stride + P.blk_uoff[blk];
+        const CT *S0 = src + (size_t)c * src_stride + P.blk_uoff[blk];
         cd *W = bufW + (size_t)c * P.Lsum + P.blk_uoff[blk];
         for (int e = tid; e < b * b; e += 256) {
             int r = e % b, q = e / b;
-            Ms[r + q * ld] = S0[e];
+            Ms[r + q * ld] = to_cd(S0[e]);
             W[e] = cmk(r == q ? 1.0 : 0.0, 0.0);
         }
         __syncthreads();
@@ -568,19 +615,25 @@ __global__ void __launch_bounds__(256) k_eig(DevPlan P, int Kb, const int *__res
             cd l = Ms[r + r * ld];
             o[r] = (mode == 0) ? atan2(l.y, l.x) : l.x;
         }
+        if (bufWr) {
+            CT *Wr = bufWr + (size_t)c * P.Lsum + P.blk_uoff[blk];
+            for (int e = tid; e < b * b; e += 256) Wr[e] = CX<R>::from(W[e]);
+        }
         __syncthreads();
     }
 }
 
 // ------------------------------------------------------------------------------ k_bmul
 // B_k = A_k W for every (crystallite, step, block) of a group; writes into bufB.
-template <int M>
+template <int M, typename R>
 __global__ void __launch_bounds__(NThr<M>::v) k_bmul(DevPlan P, int Kb, const int *__restrict__ grp,
-                                                     int ngrp, const cd *__restrict__ bufA,
-                                                     const cd *__restrict__ bufW,
-                                                     cd *__restrict__ bufB) {
+                                                     int ngrp,
+                                                     const typename CX<R>::T *__restrict__ bufA,
+                                                     const typename CX<R>::T *__restrict__ bufW,
+                                                     typename CX<R>::T *__restrict__ bufB) {
     constexpr int NT = NThr<M>::v;
-    __shared__ cd smem[2 * 16 * 16 * M];
+    typedef typename CX<R>::T CT;
+    __shared__ CT smem[2 * 16 * 16 * M];
     const long ntask = (long)Kb * P.n * ngrp;
     for (long task = blockIdx.x; task < ntask; task += gridDim.x) {
         const int g = (int)(task % ngrp);
@@ -588,8 +641,9 @@ __global__ void __launch_bounds__(NThr<M>::v) k_bmul(DevPlan P, int Kb, const in
         const int k = (int)(ck % P.n), c = (int)(ck / P.n);
         const int blk = grp[g], b = P.blk_dim[blk];
         const size_t off = ((size_t)c * P.n + k) * P.Lsum + P.blk_uoff[blk];
-        const cd *W = bufW + (size_t)c * P.Lsum + P.blk_uoff[blk];
-        wg_gemm<M, NT, cd>(b, b, b, bufA + off, b, false, W, b, false, bufB + off, b, cmk(0, 0), smem);
+        const CT *W = bufW + (size_t)c * P.Lsum + P.blk_uoff[blk];
+        wg_gemm<M, NT, CT>(b, b, b, bufA + off, b, false, W, b, false, bufB + off, b,
+                           Arith<CT>::zero(), smem);
     }
 }
 
@@ -599,16 +653,19 @@ __global__ void __launch_bounds__(NThr<M>::v) k_bmul(DevPlan P, int Kb, const in
 //   R(s,r) = sum conj(B_J(b,s)) rho_JI(b,a) B_I(a,r)
 // (rho_eff = det + det^H when rho == (det+det^H)/2, compute.h:381-382).  Stored per element
 // e = r + s*b_I at offset pair_eoff, layout [c][k][e]: Qs = Q(r,s), Rs = R(s,r).
-template <int M>
+template <int M, typename R>
 __global__ void __launch_bounds__(NThr<M>::v) k_qr(DevPlan P, int Kb, const int *__restrict__ pgrp,
-                                                   int ngrp, const cd *__restrict__ bufB,
-                                                   cd *__restrict__ scratch, size_t slot,
-                                                   cd *__restrict__ Qt, cd *__restrict__ Rt) {
+                                                   int ngrp,
+                                                   const typename CX<R>::T *__restrict__ bufB,
+                                                   typename CX<R>::T *__restrict__ scratch,
+                                                   size_t slot, typename CX<R>::T *__restrict__ Qt,
+                                                   typename CX<R>::T *__restrict__ Rt) {
     constexpr int NT = NThr<M>::v;
-    __shared__ cd smem[2 * 16 * 16 * M];
+    typedef typename CX<R>::T CT;
+    __shared__ CT smem[2 * 16 * 16 * M];
     const int tid = threadIdx.x;
-    cd *T1 = scratch + slot * 2 * blockIdx.x;
-    cd *G = T1 + slot;
+    CT *T1 = scratch + slot * 2 * blockIdx.x;
+    CT *G = T1 + slot;
     const long ntask = (long)Kb * P.n * ngrp;
     for (long task = blockIdx.x; task < ntask; task += gridDim.x) {
         const int g = (int)(task % ngrp);
@@ -617,8 +674,8 @@ __global__ void __launch_bounds__(NThr<M>::v) k_qr(DevPlan P, int Kb, const int 
         const int pr = pgrp[g];
         const int I = P.pair_I[pr], J = P.pair_J[pr];
         const int bI = P.blk_dim[I], bJ = P.blk_dim[J];
-        const cd *BI = bufB + ((size_t)c * P.n + k) * P.Lsum + P.blk_uoff[I];
-        const cd *BJ = bufB + ((size_t)c * P.n + k) * P.Lsum + P.blk_uoff[J];
+        const CT *BI = bufB + ((size_t)c * P.n + k) * P.Lsum + P.blk_uoff[I];
+        const CT *BJ = bufB + ((size_t)c * P.n + k) * P.Lsum + P.blk_uoff[J];
         const size_t eo = ((size_t)c * P.n + k) * P.Etot + P.pair_eoff[pr];
         {
             const int *rp = P.d_rowptr + P.pair_doff[pr];
@@ -626,28 +683,30 @@ __global__ void __launch_bounds__(NThr<M>::v) k_qr(DevPlan P, int Kb, const int 
             const cd *dv = (const cd *)P.d_val;
             for (int e = tid; e < bI * bJ; e += NT) {
                 int a = e % bI, s = e / bI;
-                cd acc = cmk(0, 0);
-                for (int z = rp[a]; z < rp[a + 1]; z++) cfma(acc, dv[z], BJ[ci[z] + (size_t)s * bJ]);
+                CT acc = Arith<CT>::zero();
+                for (int z = rp[a]; z < rp[a + 1]; z++)
+                    cfma(acc, CX<R>::from(dv[z]), BJ[ci[z] + (size_t)s * bJ]);
                 T1[e] = acc;
             }
         }
         __syncthreads();
-        wg_gemm<M, NT, cd>(bI, bJ, bI, BI, bI, true, T1, bI, false, Qt + eo, bI, cmk(0, 0), smem);
+        wg_gemm<M, NT, CT>(bI, bJ, bI, BI, bI, true, T1, bI, false, Qt + eo, bI, Arith<CT>::zero(), smem);
         {
             const int *rp = P.r_rowptr + P.pair_roff[pr];
             const int *ci = P.r_col;
             const cd *rv = (const cd *)P.r_val;
             for (int e = tid; e < bJ * bI; e += NT) {
                 int bq = e % bJ, r = e / bJ;
-                cd acc = cmk(0, 0);
-                for (int z = rp[bq]; z < rp[bq + 1]; z++) cfma(acc, rv[z], BI[ci[z] + (size_t)r * bI]);
+                CT acc = Arith<CT>::zero();
+                for (int z = rp[bq]; z < rp[bq + 1]; z++)
+                    cfma(acc, CX<R>::from(rv[z]), BI[ci[z] + (size_t)r * bI]);
                 T1[e] = acc;
             }
         }
         __syncthreads();
         // R^T(r,s) = sum_b T1(b,r) conj(B_J(b,s)) = (T1^T conj(B_J))(r,s): computed as
         // G = B_J^H T1 (bJ x bI) then transposed into Rt
-        wg_gemm<M, NT, cd>(bJ, bI, bJ, BJ, bJ, true, T1, bJ, false, G, bJ, cmk(0, 0), smem);
+        wg_gemm<M, NT, CT>(bJ, bI, bJ, BJ, bJ, true, T1, bJ, false, G, bJ, Arith<CT>::zero(), smem);
         for (int e = tid; e < bI * bJ; e += NT) {
             int r = e % bI, s = e / bI;
             Rt[eo + e] = G[s + (size_t)r * bJ];
@@ -674,14 +733,17 @@ __device__ __host__ inline int corr_chunk(int n) {
     int ch = a < b ? a : b;
     return ch < 1 ? 1 : ch;
 }
-__global__ void __launch_bounds__(256) k_corr(DevPlan P, int Kb, const cd *__restrict__ Qt,
-                                              const cd *__restrict__ Rt,
+template <typename R>
+__global__ void __launch_bounds__(256) k_corr(DevPlan P, int Kb,
+                                              const typename CX<R>::T *__restrict__ Qt,
+                                              const typename CX<R>::T *__restrict__ Rt,
                                               const double *__restrict__ thetas,
                                               const int *__restrict__ elem_r,
                                               const int *__restrict__ elem_s,
                                               double *__restrict__ fidc) {
-    __shared__ cd sQ[CORR_SPAN];
-    __shared__ cd sR[CORR_SPAN];
+    typedef typename CX<R>::T CT;
+    __shared__ CT sQ[CORR_SPAN];
+    __shared__ CT sR[CORR_SPAN];
     __shared__ double sphi[256];
     __shared__ cd sstep[256];
     const int n = P.n, tid = threadIdx.x;
@@ -718,33 +780,33 @@ __global__ void __launch_bounds__(256) k_corr(DevPlan P, int Kb, const cd *__res
                     cd q = cmk(0, 0), r = cmk(0, 0);
                     if (e < P.Etot) {
                         size_t o = ((size_t)c * n + k) * P.Etot + e;
-                        q = Qt[o];
-                        r = Rt[o];
+                        q = to_cd(Qt[o]);
+                        r = to_cd(Rt[o]);
                     }
-                    sQ[k * CH + el] = cmul(q, w);
-                    sR[k * CH + el] = cmul(r, cconj(w));
+                    sQ[k * CH + el] = CX<R>::from(cmul(q, w));
+                    sR[k * CH + el] = CX<R>::from(cmul(r, cconj(w)));
                     w = cmul(w, zi);
                 }
             }
         }
         __syncthreads();
-        cd F[CORR_PPT];
+        CT F[CORR_PPT];
         const int el = tid % CH, pg = tid / CH;
         const int p0 = pg * CORR_PPT;
         const bool active = pg < npg;
 #pragma unroll
-        for (int q = 0; q < CORR_PPT; q++) F[q] = cmk(0, 0);
+        for (int q = 0; q < CORR_PPT; q++) F[q] = Arith<CT>::zero();
         if (active) {
             // window slot (i + q) % 8 holds Qt_{(i + p0 + q) % n}; unrolling i by 8 makes the
             // rotation static, so the window never moves between registers
-            cd win[CORR_PPT];
+            CT win[CORR_PPT];
 #pragma unroll
             for (int q = 0; q < CORR_PPT; q++) win[q] = sQ[((p0 + q) % n) * CH + el];
             const int n8 = n & ~(CORR_PPT - 1);
             for (int i = 0; i < n8; i += CORR_PPT) {
 #pragma unroll
                 for (int u = 0; u < CORR_PPT; u++) {
-                    cd r = sR[(i + u) * CH + el];
+                    CT r = sR[(i + u) * CH + el];
 #pragma unroll
                     for (int q = 0; q < CORR_PPT; q++) cfma(F[q], r, win[(u + q) % CORR_PPT]);
                     win[u] = sQ[((i + u + p0 + CORR_PPT) % n) * CH + el];
@@ -753,7 +815,7 @@ __global__ void __launch_bounds__(256) k_corr(DevPlan P, int Kb, const cd *__res
 #pragma unroll
             for (int u = 0; u < CORR_PPT; u++) { // tail when 8 does not divide n
                 if (n8 + u < n) {
-                    cd r = sR[(n8 + u) * CH + el];
+                    CT r = sR[(n8 + u) * CH + el];
 #pragma unroll
                     for (int q = 0; q < CORR_PPT; q++) cfma(F[q], r, win[(u + q) % CORR_PPT]);
                     win[u] = sQ[((n8 + u + p0 + CORR_PPT) % n) * CH + el];
@@ -781,7 +843,7 @@ __global__ void __launch_bounds__(256) k_corr(DevPlan P, int Kb, const cd *__res
 #pragma unroll
                     for (int m = 0; m < 8; m++) {
                         if (m < nt) {
-                            cfma(acc[m], sQ[((t0 + 256 * m) % n) * CH + e], w);
+                            cfma(acc[m], to_cd(sQ[((t0 + 256 * m) % n) * CH + e]), w);
                             w = cmul(w, st);
                         }
                     }
@@ -932,92 +994,149 @@ __global__ void __launch_bounds__(NThr<M>::v) k_static_fid(DevPlan P, int Kb, co
 }
 
 // ------------------------------------------------------------------------------ launchers
+// prec: 0 = FP64 storage (double2), 1 = complex64 storage (float2) of U, A, B, Q, R and the exp
+// scratch; the eigen-solver, phases, FIDs and reductions are FP64 in both.
 #define GRID_CAP 4096
 
-extern "C" int blfid_dev_launch_exp(int M, const DevPlan *P, int Kb, int c0, const int *grp,
-                                    int ngrp, double *scratch, size_t slot, int nslots, cd *bufU,
-                                    hipStream_t st) {
+template <typename R>
+static void launch_exp(int M, const DevPlan *P, int Kb, int c0, const int *grp, int ngrp,
+                       void *scratch, size_t slot, int grid, void *bufU, hipStream_t st) {
+    R *sc = (R *)scratch;
+    typename CX<R>::T *U = (typename CX<R>::T *)bufU;
+    switch (M) {
+    case 1: k_exp<1, R><<<grid, 64, 0, st>>>(*P, Kb, c0, grp, ngrp, sc, slot, U); break;
+    case 2: k_exp<2, R><<<grid, 256, 0, st>>>(*P, Kb, c0, grp, ngrp, sc, slot, U); break;
+    case 3: k_exp<3, R><<<grid, 256, 0, st>>>(*P, Kb, c0, grp, ngrp, sc, slot, U); break;
+    case 4: k_exp<4, R><<<grid, 256, 0, st>>>(*P, Kb, c0, grp, ngrp, sc, slot, U); break;
+    default: k_exp<5, R><<<grid, 256, 0, st>>>(*P, Kb, c0, grp, ngrp, sc, slot, U); break;
+    }
+}
+
+extern "C" int blfid_dev_launch_exp(int prec, int M, const DevPlan *P, int Kb, int c0,
+                                    const int *grp, int ngrp, void *scratch, size_t slot,
+                                    int nslots, void *bufU, hipStream_t st) {
     long ntask = (long)Kb * P->n * ngrp;
     int grid = (int)(ntask < nslots ? ntask : nslots);
     if (grid <= 0) return 0;
-    switch (M) {
-    case 1: k_exp<1><<<grid, 64, 0, st>>>(*P, Kb, c0, grp, ngrp, scratch, slot, bufU); break;
-    case 2: k_exp<2><<<grid, 256, 0, st>>>(*P, Kb, c0, grp, ngrp, scratch, slot, bufU); break;
-    case 3: k_exp<3><<<grid, 256, 0, st>>>(*P, Kb, c0, grp, ngrp, scratch, slot, bufU); break;
-    case 4: k_exp<4><<<grid, 256, 0, st>>>(*P, Kb, c0, grp, ngrp, scratch, slot, bufU); break;
-    default: k_exp<5><<<grid, 256, 0, st>>>(*P, Kb, c0, grp, ngrp, scratch, slot, bufU); break;
-    }
+    if (prec) launch_exp<float>(M, P, Kb, c0, grp, ngrp, scratch, slot, grid, bufU, st);
+    else launch_exp<double>(M, P, Kb, c0, grp, ngrp, scratch, slot, grid, bufU, st);
     return (int)hipGetLastError();
 }
 
-extern "C" int blfid_dev_launch_dyson(int M, const DevPlan *P, int Kb, const int *grp, int ngrp,
-                                      const cd *bufU, cd *bufA, hipStream_t st) {
+template <typename R>
+static void launch_dyson(int M, const DevPlan *P, int Kb, const int *grp, int ngrp, int grid,
+                         const void *bufU, void *bufA, hipStream_t st) {
+    typedef typename CX<R>::T CT;
+    const CT *U = (const CT *)bufU;
+    CT *A = (CT *)bufA;
+    switch (M) {
+    case 1: k_dyson<1, R><<<grid, 64, 0, st>>>(*P, Kb, grp, ngrp, U, A); break;
+    case 2: k_dyson<2, R><<<grid, 256, 0, st>>>(*P, Kb, grp, ngrp, U, A); break;
+    case 3: k_dyson<3, R><<<grid, 256, 0, st>>>(*P, Kb, grp, ngrp, U, A); break;
+    case 4: k_dyson<4, R><<<grid, 256, 0, st>>>(*P, Kb, grp, ngrp, U, A); break;
+    default: k_dyson<5, R><<<grid, 256, 0, st>>>(*P, Kb, grp, ngrp, U, A); break;
+    }
+}
+
+extern "C" int blfid_dev_launch_dyson(int prec, int M, const DevPlan *P, int Kb, const int *grp,
+                                      int ngrp, const void *bufU, void *bufA, hipStream_t st) {
     long ntask = (long)Kb * ngrp;
     int grid = (int)(ntask < GRID_CAP ? ntask : GRID_CAP);
     if (grid <= 0) return 0;
-    switch (M) {
-    case 1: k_dyson<1><<<grid, 64, 0, st>>>(*P, Kb, grp, ngrp, bufU, bufA); break;
-    case 2: k_dyson<2><<<grid, 256, 0, st>>>(*P, Kb, grp, ngrp, bufU, bufA); break;
-    case 3: k_dyson<3><<<grid, 256, 0, st>>>(*P, Kb, grp, ngrp, bufU, bufA); break;
-    case 4: k_dyson<4><<<grid, 256, 0, st>>>(*P, Kb, grp, ngrp, bufU, bufA); break;
-    default: k_dyson<5><<<grid, 256, 0, st>>>(*P, Kb, grp, ngrp, bufU, bufA); break;
-    }
+    if (prec) launch_dyson<float>(M, P, Kb, grp, ngrp, grid, bufU, bufA, st);
+    else launch_dyson<double>(M, P, Kb, grp, ngrp, grid, bufU, bufA, st);
     return (int)hipGetLastError();
 }
 
-extern "C" int blfid_dev_launch_eig(const DevPlan *P, int Kb, const int *grp, int ngrp,
-                                    const cd *src, size_t src_stride, int mode, cd *bufW,
-                                    double *outv, int big, cd *gscr, size_t gslot, int nslots,
-                                    hipStream_t st) {
+template <typename R>
+static void launch_eig(const DevPlan *P, int Kb, const int *grp, int ngrp, int grid,
+                       const void *src, size_t src_stride, int mode, cd *bufW, void *bufWr,
+                       double *outv, int big, cd *gscr, size_t gslot, hipStream_t st) {
+    typedef typename CX<R>::T CT;
+    const CT *S0 = (const CT *)src;
+    CT *Wr = (CT *)bufWr;
+    if (big) k_eig<true, R><<<grid, 256, 0, st>>>(*P, Kb, grp, ngrp, S0, src_stride, mode, bufW, Wr, outv, gscr, gslot);
+    else k_eig<false, R><<<grid, 256, 0, st>>>(*P, Kb, grp, ngrp, S0, src_stride, mode, bufW, Wr, outv, gscr, gslot);
+}
+
+// bufWr: storage-precision copy of W (complex64 plans), or NULL
+extern "C" int blfid_dev_launch_eig(int prec, const DevPlan *P, int Kb, const int *grp, int ngrp,
+                                    const void *src, size_t src_stride, int mode, cd *bufW,
+                                    void *bufWr, double *outv, int big, cd *gscr, size_t gslot,
+                                    int nslots, hipStream_t st) {
     long ntask = (long)Kb * ngrp;
     int cap = big ? nslots : GRID_CAP;
     int grid = (int)(ntask < cap ? ntask : cap);
     if (grid <= 0) return 0;
-    if (big) k_eig<true><<<grid, 256, 0, st>>>(*P, Kb, grp, ngrp, src, src_stride, mode, bufW, outv, gscr, gslot);
-    else k_eig<false><<<grid, 256, 0, st>>>(*P, Kb, grp, ngrp, src, src_stride, mode, bufW, outv, gscr, gslot);
+    if (prec) launch_eig<float>(P, Kb, grp, ngrp, grid, src, src_stride, mode, bufW, bufWr, outv, big, gscr, gslot, st);
+    else launch_eig<double>(P, Kb, grp, ngrp, grid, src, src_stride, mode, bufW, bufWr, outv, big, gscr, gslot, st);
     return (int)hipGetLastError();
 }
 
-extern "C" int blfid_dev_launch_bmul(int M, const DevPlan *P, int Kb, const int *grp, int ngrp,
-                                     const cd *bufA, const cd *bufW, cd *bufB, hipStream_t st) {
+template <typename R>
+static void launch_bmul(int M, const DevPlan *P, int Kb, const int *grp, int ngrp, int grid,
+                        const void *bufA, const void *bufW, void *bufB, hipStream_t st) {
+    typedef typename CX<R>::T CT;
+    const CT *A = (const CT *)bufA, *W = (const CT *)bufW;
+    CT *B = (CT *)bufB;
+    switch (M) {
+    case 1: k_bmul<1, R><<<grid, 64, 0, st>>>(*P, Kb, grp, ngrp, A, W, B); break;
+    case 2: k_bmul<2, R><<<grid, 256, 0, st>>>(*P, Kb, grp, ngrp, A, W, B); break;
+    case 3: k_bmul<3, R><<<grid, 256, 0, st>>>(*P, Kb, grp, ngrp, A, W, B); break;
+    case 4: k_bmul<4, R><<<grid, 256, 0, st>>>(*P, Kb, grp, ngrp, A, W, B); break;
+    default: k_bmul<5, R><<<grid, 256, 0, st>>>(*P, Kb, grp, ngrp, A, W, B); break;
+    }
+}
+
+// bufW: W in the plan's storage precision
+extern "C" int blfid_dev_launch_bmul(int prec, int M, const DevPlan *P, int Kb, const int *grp,
+                                     int ngrp, const void *bufA, const void *bufW, void *bufB,
+                                     hipStream_t st) {
     long ntask = (long)Kb * P->n * ngrp;
     int grid = (int)(ntask < GRID_CAP ? ntask : GRID_CAP);
     if (grid <= 0) return 0;
-    switch (M) {
-    case 1: k_bmul<1><<<grid, 64, 0, st>>>(*P, Kb, grp, ngrp, bufA, bufW, bufB); break;
-    case 2: k_bmul<2><<<grid, 256, 0, st>>>(*P, Kb, grp, ngrp, bufA, bufW, bufB); break;
-    case 3: k_bmul<3><<<grid, 256, 0, st>>>(*P, Kb, grp, ngrp, bufA, bufW, bufB); break;
-    case 4: k_bmul<4><<<grid, 256, 0, st>>>(*P, Kb, grp, ngrp, bufA, bufW, bufB); break;
-    default: k_bmul<5><<<grid, 256, 0, st>>>(*P, Kb, grp, ngrp, bufA, bufW, bufB); break;
-    }
+    if (prec) launch_bmul<float>(M, P, Kb, grp, ngrp, grid, bufA, bufW, bufB, st);
+    else launch_bmul<double>(M, P, Kb, grp, ngrp, grid, bufA, bufW, bufB, st);
     return (int)hipGetLastError();
 }
 
-extern "C" int blfid_dev_launch_qr(int M, const DevPlan *P, int Kb, const int *pgrp, int ngrp,
-                                   const cd *bufB, cd *scratch, size_t slot, int nslots, cd *Qt,
-                                   cd *Rt, hipStream_t st) {
+template <typename R>
+static void launch_qr(int M, const DevPlan *P, int Kb, const int *pgrp, int ngrp, int grid,
+                      const void *bufB, void *scratch, size_t slot, void *Qt, void *Rt,
+                      hipStream_t st) {
+    typedef typename CX<R>::T CT;
+    const CT *B = (const CT *)bufB;
+    CT *sc = (CT *)scratch, *Q = (CT *)Qt, *Rr = (CT *)Rt;
+    switch (M) {
+    case 1: k_qr<1, R><<<grid, 64, 0, st>>>(*P, Kb, pgrp, ngrp, B, sc, slot, Q, Rr); break;
+    case 2: k_qr<2, R><<<grid, 256, 0, st>>>(*P, Kb, pgrp, ngrp, B, sc, slot, Q, Rr); break;
+    case 3: k_qr<3, R><<<grid, 256, 0, st>>>(*P, Kb, pgrp, ngrp, B, sc, slot, Q, Rr); break;
+    case 4: k_qr<4, R><<<grid, 256, 0, st>>>(*P, Kb, pgrp, ngrp, B, sc, slot, Q, Rr); break;
+    default: k_qr<5, R><<<grid, 256, 0, st>>>(*P, Kb, pgrp, ngrp, B, sc, slot, Q, Rr); break;
+    }
+}
+
+extern "C" int blfid_dev_launch_qr(int prec, int M, const DevPlan *P, int Kb, const int *pgrp,
+                                   int ngrp, const void *bufB, void *scratch, size_t slot,
+                                   int nslots, void *Qt, void *Rt, hipStream_t st) {
     long ntask = (long)Kb * P->n * ngrp;
     int grid = (int)(ntask < nslots ? ntask : nslots);
     if (grid <= 0) return 0;
-    switch (M) {
-    case 1: k_qr<1><<<grid, 64, 0, st>>>(*P, Kb, pgrp, ngrp, bufB, scratch, slot, Qt, Rt); break;
-    case 2: k_qr<2><<<grid, 256, 0, st>>>(*P, Kb, pgrp, ngrp, bufB, scratch, slot, Qt, Rt); break;
-    case 3: k_qr<3><<<grid, 256, 0, st>>>(*P, Kb, pgrp, ngrp, bufB, scratch, slot, Qt, Rt); break;
-    case 4: k_qr<4><<<grid, 256, 0, st>>>(*P, Kb, pgrp, ngrp, bufB, scratch, slot, Qt, Rt); break;
-    default: k_qr<5><<<grid, 256, 0, st>>>(*P, Kb, pgrp, ngrp, bufB, scratch, slot, Qt, Rt); break;
-    }
+    if (prec) launch_qr<float>(M, P, Kb, pgrp, ngrp, grid, bufB, scratch, slot, Qt, Rt, st);
+    else launch_qr<double>(M, P, Kb, pgrp, ngrp, grid, bufB, scratch, slot, Qt, Rt, st);
     return (int)hipGetLastError();
 }
 
-extern "C" int blfid_dev_launch_corr(const DevPlan *P, int Kb, const cd *Qt, const cd *Rt,
-                                     const double *thetas, const int *er, const int *es,
-                                     double *fidc, hipStream_t st) {
+extern "C" int blfid_dev_launch_corr(int prec, const DevPlan *P, int Kb, const void *Qt,
+                                     const void *Rt, const double *thetas, const int *er,
+                                     const int *es, double *fidc, hipStream_t st) {
     int CH = corr_chunk(P->n);
     long nchunk = (P->Etot + CH - 1) / CH;
     long ntask = (long)Kb * nchunk;
     int grid = (int)(ntask < GRID_CAP ? ntask : GRID_CAP);
     if (grid <= 0) return 0;
-    k_corr<<<grid, 256, 0, st>>>(*P, Kb, Qt, Rt, thetas, er, es, fidc);
+    if (prec) k_corr<float><<<grid, 256, 0, st>>>(*P, Kb, (const cf *)Qt, (const cf *)Rt, thetas, er, es, fidc);
+    else k_corr<double><<<grid, 256, 0, st>>>(*P, Kb, (const cd *)Qt, (const cd *)Rt, thetas, er, es, fidc);
     return (int)hipGetLastError();
 }
 

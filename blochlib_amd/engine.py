@@ -30,6 +30,9 @@ LIBPATH = os.path.join(HERE, "libblfid.so")
 
 BLFID_OK = 0
 STATUS = {0: "OK", 1: "INVALID", 2: "NOMEM", 3: "DEVICE", 4: "UNSUPPORTED", 5: "NOCONV"}
+# blfid_precision: complex128 throughout, or complex64 propagators / products / correlation sums
+# with the eigen-solver, phases and reductions in FP64
+PRECISION = {"fp64": 0, "c128": 0, "c64": 1}
 
 _d = ctypes.POINTER(ctypes.c_double)
 _i = ctypes.POINTER(ctypes.c_int)
@@ -138,8 +141,11 @@ class Plan:
     """One spin system + acquisition on the device (blfid_plan_*)."""
 
     def __init__(self, sy: SolidSys, wr: float, rotor_deg: float, sw: float, npts: int,
-                 t0: float = 0.0, *, _ptop=None):
+                 t0: float = 0.0, *, precision: str = "fp64", _ptop=None):
         L = lib()
+        if precision not in PRECISION:
+            raise BlochLibError(f"unknown precision {precision!r} (fp64, c64)")
+        prec = PRECISION[precision]
         iso, csa, dip, jc = sy.tables()
         self._keep = [np.ascontiguousarray(iso, np.int32),
                       csa if csa.size else np.zeros(1), dip if dip.size else np.zeros(1),
@@ -149,7 +155,7 @@ class Plan:
         h = ctypes.c_void_p()
         if _ptop is None:
             _check(L.blfid_plan_create(ctypes.byref(s), float(wr), float(rotor_deg), float(sw),
-                                       int(npts), float(t0), 0, ctypes.byref(h)))
+                                       int(npts), float(t0), prec, ctypes.byref(h)))
         else:
             steps, rf = _ptop
             ids = {}
@@ -165,17 +171,18 @@ class Plan:
             self._rf = (tabs, arr)
             _check(L.blfid_plan_create_ptop(ctypes.byref(s), float(wr), float(rotor_deg),
                                             int(steps), int(npts), float(t0), len(chans), arr,
-                                            0, ctypes.byref(h)))
+                                            prec, ctypes.byref(h)))
         self.h = h
         self.N = sy.N
         self.npts = int(npts)
 
     @classmethod
     def ptop(cls, sy: SolidSys, wr: float, rotor_deg: float, steps: int, npts: int, rf=(),
-             t0: float = 0.0) -> "Plan":
+             t0: float = 0.0, precision: str = "fp64") -> "Plan":
         """Rotor-synchronised pulse-program plan (blfid_plan_create_ptop).  rf: channels
         (isotope name, amp_hz, phase_rad, offset_hz), each a scalar or one value per step."""
-        return cls(sy, wr, rotor_deg, 0.0, npts, t0, _ptop=(int(steps), list(rf)))
+        return cls(sy, wr, rotor_deg, 0.0, npts, t0, precision=precision,
+                   _ptop=(int(steps), list(rf)))
 
     def close(self):
         if getattr(self, "h", None) and self.h.value:
@@ -261,9 +268,10 @@ class Plan:
         return self.fetch()
 
 
-def powder_fid(sy, wr, rotor_deg, sw, npts, pw, rho, det, t0=0.0, normalize=True):
+def powder_fid(sy, wr, rotor_deg, sw, npts, pw, rho, det, t0=0.0, normalize=True,
+               precision="fp64"):
     """FID(powder&, ro, det) on the GPU; returns (fid, adjusted sw)."""
-    p = Plan(sy, wr, rotor_deg, sw, npts, t0)
+    p = Plan(sy, wr, rotor_deg, sw, npts, t0, precision=precision)
     try:
         f = p.powder_fid(pw, rho, det, normalize)
         return f, p.sw
@@ -271,10 +279,11 @@ def powder_fid(sy, wr, rotor_deg, sw, npts, pw, rho, det, t0=0.0, normalize=True
         p.close()
 
 
-def ptop_fid(sy, wr, rotor_deg, steps, npts, pw, rho, det, rf=(), t0=0.0, normalize=True):
+def ptop_fid(sy, wr, rotor_deg, steps, npts, pw, rho, det, rf=(), t0=0.0, normalize=True,
+             precision="fp64"):
     """Powder-summed ptop FID of a one-rotor-period pulse program on the GPU (one point per
     period): sum_k w_k Tr(rho_m det) [/ Tr(det det^H)], rho_{m+1} = U rho_m U^H."""
-    p = Plan.ptop(sy, wr, rotor_deg, steps, npts, rf, t0)
+    p = Plan.ptop(sy, wr, rotor_deg, steps, npts, rf, t0, precision)
     try:
         return p.powder_fid(pw, rho, det, normalize)
     finally:

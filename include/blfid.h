@@ -39,7 +39,11 @@ typedef enum {
 } blfid_status;
 
 typedef enum {
-    BLFID_FP64 = 0 /* complex128 everywhere: oneFID<SolidSys, hmatrix> (oneFID.h:52) */
+    BLFID_FP64 = 0, /* complex128 everywhere: oneFID<SolidSys, hmatrix> (oneFID.h:52) */
+    BLFID_C64 = 1   /* complex64 propagators, products and correlation sums, as solid-2.0's
+                     * oneFID<SolidSys, hmatrixs> (oneFID.h:52-60, sequencerun_funcs.cc:602-609);
+                     * the period eigen-solver, phases, FID points and the powder reduction stay
+                     * FP64.  The static path always runs in FP64. */
 } blfid_precision;
 
 /* Tensor-described SolidSys (src/QMspins/sys.h:59-62, read by sys.cc:120-204).

@@ -1,0 +1,102 @@
+"""GPU parity of the complex64 form (BLFID_C64: FP32 propagators, Dyson products, basis changes
+and correlation sums; FP64 eigen-solver, phases, FID points and powder reduction) against the
+FP64 oracle, at the north-star bar: relative L2 <= 1e-5 over the powder-summed FID."""
+import numpy as np
+import pytest
+
+from blochlib_amd import engine, workloads
+from blochlib_amd.powder import Powder, zcw
+from blochlib_amd.spinsys import SolidSys
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL_C64 = 1e-5
+MAGIC = workloads.MAGIC_DEG
+
+
+def rel_l2(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def check(name, sy, wr, rotor, sw, npts, pw, rho, det, normalize=True):
+    print(f"[c64] {name}: S={sy.spins} K={len(pw)} npts={npts}", flush=True)
+    g, swg = engine.powder_fid(sy, wr, rotor, sw, npts, pw, rho, det, normalize=normalize,
+                               precision="c64")
+    c, swc = O.powder_fid(sy, wr, rotor, sw, npts, pw, rho, det, normalize=normalize)
+    assert swg == swc
+    err = rel_l2(g, c)
+    print(f"[c64] {name}: rel-L2 {err:.3e}", flush=True)
+    assert err < TOL_C64, (name, err)
+    return g, c
+
+
+def test_config1_subset_c64():
+    w = workloads.config1()
+    rho, det = w.operators()
+    check("config1", w.system, w.wr, w.rotor, w.sw, w.npts,
+          w.powder().subset(np.arange(0, 1154, 23)), rho, det)
+
+
+@pytest.mark.parametrize("rho_name", ["Ix", "Iz+0.3*Ix_0"])
+def test_heteronuclear_strong_j_c64(rho_name):
+    sy = SolidSys.from_text("""numspin 3
+T 1H 0
+T 1H 1
+T 13C 2
+C 1000 5000 0.5 0 10 20 30
+C -1000 3000 0.2 1 40 50 60
+C 300 7000 0.9 2
+D -20000 0 1 0 30 40
+D 5000 1 2 10 70 20
+J 140 0 2
+J 50 0 1 1
+""")
+    check(f"sys3-{rho_name}", sy, 10000.0, MAGIC, 80000.0, 40,
+          zcw(1154).subset(np.arange(0, 1154, 97)), sy.op(rho_name), sy.op("Ip"), normalize=False)
+
+
+@pytest.mark.parametrize("S", [4, 6, 8])
+def test_config2_clusters_full_points_c64(S):
+    """Config-2 clusters (MAS 15 kHz, 128 steps, 1024 points) on a few ZCW 3722 crystallites."""
+    w = next(x for x in workloads.config2() if x.meta["spins"] == S)
+    rho, det = w.operators()
+    pw = w.powder().subset(np.arange(3) * 1201 % w.powder_size)
+    check(w.name, w.system, w.wr, w.rotor, w.sw, w.npts, pw, rho, det)
+
+
+def test_static_path_is_fp64_in_both_precisions():
+    sy = SolidSys.from_text("numspin 2\nT 1H 0\nT 1H 1\nC 5000  2134 0 0\nC -5000 2789 0.5 1\n")
+    pw = Powder.read("rep2000")
+    a, _ = engine.powder_fid(sy, 0.0, 0.0, 40000.0, 512, pw, sy.op("Ix"), sy.op("Ip"))
+    b, _ = engine.powder_fid(sy, 0.0, 0.0, 40000.0, 512, pw, sy.op("Ix"), sy.op("Ip"),
+                             precision="c64")
+    assert np.array_equal(a, b)
+
+
+def test_config4_ptop_subset_c64():
+    w = workloads.config4(1)[0]
+    rho, det = w.operators()
+    pw = zcw(1154).subset(np.array([3, 500, 1000]))
+    print(f"[c64] config4-set0 K={len(pw)} npts=48", flush=True)
+    g = engine.ptop_fid(w.system, w.wr, w.rotor, w.steps, 48, pw, rho, det, rf=w.rf,
+                        precision="c64")
+    c = O.ptop_fid(w.system, w.wr, w.rotor, w.steps, 48, pw, rho, det, rf=w.rf)
+    err = rel_l2(g, c)
+    print(f"[c64] config4-set0: rel-L2 {err:.3e}", flush=True)
+    assert err < TOL_C64
+
+
+def test_ten_spin_c64_against_fp64_engine():
+    """Config-3 shape (10 spins, N = 1024, 64 steps, 512 points): the complex64 form against the
+    FP64 form of the engine (itself pinned to the oracle) on four crystallites."""
+    w = workloads.config3()
+    rho, det = w.operators()
+    pw = w.powder().subset(np.array([0, 300, 600, 900]))
+    print(f"[c64] config3 K={len(pw)}", flush=True)
+    a, _ = engine.powder_fid(w.system, w.wr, w.rotor, w.sw, w.npts, pw, rho, det)
+    b, _ = engine.powder_fid(w.system, w.wr, w.rotor, w.sw, w.npts, pw, rho, det,
+                             precision="c64")
+    err = rel_l2(b, a)
+    print(f"[c64] config3: rel-L2 vs FP64 engine {err:.3e}", flush=True)
+    assert err < TOL_C64
