@@ -41,4 +41,5 @@ struct DevPlan {
     int nrfiso;            // pulsed isotopes
     const int *spin_rf;    // S: pulsed-isotope slot of each spin or -1
     const double *rf;      // n x nrfiso x 3: amp/2, phase, offset of each step
+    const double *thetak;  // n: 2 pi tau_k of steps of unequal length, or NULL (all theta)
 };

@@ -175,6 +175,7 @@ struct blfid_plan {
     int nrfiso = 0;
     std::vector<int> spin_rf;
     std::vector<double> rf;
+    std::vector<double> step_dt; // unequal step lengths (blfid_plan_create_pulse), else empty
     // derived
     int n = 1;
     double sw_adj = 0, tau = 0;
@@ -207,7 +208,7 @@ struct blfid_plan {
     DevBuf<double> d_dval, d_rval;
     std::map<int, DevBuf<int>> d_bgrp, d_pgrp;
     DevBuf<int> d_eig_small, d_eig_big, d_spin_rf;
-    DevBuf<double> d_rf;
+    DevBuf<double> d_rf, d_thetak;
     // working storage (carved from the plan's own arena or a shared workspace)
     DevBuf<char> own_ws;
     blfid_workspace *ws = nullptr;
@@ -441,15 +442,22 @@ static blfid_status build_tables(blfid_plan *pl) {
     pl->u3.clear();
     if (pl->spinning || pl->ptop) {
         double br = pl->rotor_deg * DEG2RAD;
-        double t1 = pl->t0, t2 = pl->t0 + pl->tau;
+        const bool uneq = !pl->step_dt.empty();
+        double t1 = pl->t0, t2 = pl->t0 + (uneq ? pl->step_dt[0] : pl->tau);
         for (int k = 0; k < pl->n; k++) {
             double tmid = (t2 - t1) / 2.0 + t1;
             double al = tmid * pl->wr * PI2;
             pl->u3.push_back(std::sin(al) * std::sin(br));
             pl->u3.push_back(-std::cos(al) * std::sin(br));
             pl->u3.push_back(std::cos(br));
-            t1 += pl->tau;
-            t2 += pl->tau;
+            if (uneq) {
+                // consecutive segments: the next one starts where this one ends
+                t1 = t2;
+                if (k + 1 < pl->n) t2 = t1 + pl->step_dt[k + 1];
+            } else {
+                t1 += pl->tau;
+                t2 += pl->tau;
+            }
         }
     } else {
         // static: Hamiltonian(0, dt) with wr = 0 -> rotor phase 0 (oneFID.h:297-299)
@@ -477,6 +485,11 @@ static blfid_status upload_tables(blfid_plan *pl) {
     HIPCHK(pl->d_eig_big.upload(pl->eig_big));
     HIPCHK(pl->d_spin_rf.upload(pl->spin_rf));
     HIPCHK(pl->d_rf.upload(pl->rf));
+    if (!pl->step_dt.empty()) {
+        std::vector<double> th(pl->step_dt.size());
+        for (size_t k = 0; k < th.size(); k++) th[k] = PI2 * pl->step_dt[k];
+        HIPCHK(pl->d_thetak.upload(th));
+    }
     return BLFID_OK;
 }
 
@@ -517,6 +530,7 @@ static void refresh_devplan(blfid_plan *pl) {
     d.nrfiso = pl->nrfiso;
     d.spin_rf = pl->d_spin_rf.p;
     d.rf = pl->d_rf.p;
+    d.thetak = pl->step_dt.empty() ? nullptr : pl->d_thetak.p;
 }
 
 // ------------------------------------------------------------------------------ C ABI
@@ -634,14 +648,26 @@ blfid_status blfid_plan_create(const blfid_system *sys, double wr_hz, double rot
     return finish_plan(pl, out);
 }
 
-blfid_status blfid_plan_create_ptop(const blfid_system *sys, double wr_hz, double rotor_deg,
-                                    int steps, int npts, double t0, int nrf, const blfid_rf *rf,
-                                    blfid_precision prec, blfid_plan **out) {
+} // extern "C"
+
+// Shared body of the two pulse-program plan constructors: `steps` steps of length step_dt[k]
+// (or 1/(wr steps) each when step_dt is NULL), consecutive from t0.
+static blfid_status create_pulse_plan(const blfid_system *sys, double wr_hz, double rotor_deg,
+                                      int steps, const double *step_dt, int npts, double t0,
+                                      int nrf, const blfid_rf *rf, blfid_precision prec,
+                                      blfid_plan **out) {
     blfid_status cs = check_system(sys, npts, prec, out);
     if (cs != BLFID_OK) return cs;
-    if (!(wr_hz > 0.0) || !std::isfinite(wr_hz))
+    if (steps < 1) return fail(BLFID_ERR_INVALID, "a pulse program needs at least one step");
+    if (!step_dt && (!(wr_hz > 0.0) || !std::isfinite(wr_hz)))
         return fail(BLFID_ERR_INVALID, "ptop needs a positive rotor speed (one period per point)");
-    if (steps < 1) return fail(BLFID_ERR_INVALID, "steps per rotor period must be >= 1");
+    if (step_dt) {
+        if (!(wr_hz >= 0.0) || !std::isfinite(wr_hz))
+            return fail(BLFID_ERR_INVALID, "rotor speed must be finite and >= 0");
+        for (int k = 0; k < steps; k++)
+            if (!(step_dt[k] > 0.0) || !std::isfinite(step_dt[k]))
+                return fail(BLFID_ERR_INVALID, "step lengths must be positive and finite");
+    }
     if (nrf < 0 || (nrf > 0 && !rf)) return fail(BLFID_ERR_INVALID, "bad rf channel list");
     for (int c = 0; c < nrf; c++)
         if (!rf[c].amp_hz || !rf[c].phase_rad || !rf[c].offset_hz)
@@ -702,13 +728,40 @@ blfid_status blfid_plan_create_ptop(const blfid_system *sys, double wr_hz, doubl
     pl->ptop = true;
     pl->spinning = false;
     pl->n = steps;
-    pl->tau = (1.0 / wr_hz) / steps;
-    pl->sw = wr_hz;
-    pl->sw_adj = wr_hz;
+    if (step_dt) {
+        pl->step_dt.assign(step_dt, step_dt + steps);
+        double T = 0;
+        for (double d : pl->step_dt) T += d;
+        pl->tau = T / steps;
+        pl->sw = pl->sw_adj = 1.0 / T;
+    } else {
+        pl->tau = (1.0 / wr_hz) / steps;
+        pl->sw = wr_hz;
+        pl->sw_adj = wr_hz;
+    }
     pl->nrfiso = nq;
     pl->spin_rf = spin_rf;
     pl->rf = tab;
     return finish_plan(pl, out);
+}
+
+extern "C" {
+
+blfid_status blfid_plan_create_ptop(const blfid_system *sys, double wr_hz, double rotor_deg,
+                                    int steps, int npts, double t0, int nrf, const blfid_rf *rf,
+                                    blfid_precision prec, blfid_plan **out) {
+    return create_pulse_plan(sys, wr_hz, rotor_deg, steps, nullptr, npts, t0, nrf, rf, prec, out);
+}
+
+blfid_status blfid_plan_create_pulse(const blfid_system *sys, double wr_hz, double rotor_deg,
+                                     int nsteps, const double *step_dt, int npts, double t0,
+                                     int nrf, const blfid_rf *rf, blfid_precision prec,
+                                     blfid_plan **out) {
+    if (!step_dt) {
+        if (out) *out = nullptr;
+        return fail(BLFID_ERR_INVALID, "step_dt is NULL");
+    }
+    return create_pulse_plan(sys, wr_hz, rotor_deg, nsteps, step_dt, npts, t0, nrf, rf, prec, out);
 }
 
 void blfid_plan_destroy(blfid_plan *pl) {

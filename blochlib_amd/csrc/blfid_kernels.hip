@@ -142,7 +142,7 @@ __device__ __forceinline__ void gemm_load_slice(int m, int n, int k, int i0, int
 }
 
 template <int M, int NT, typename T>
-__device__ void wg_gemm(int m, int n, int k, const T *__restrict__ A, int lda, bool ca,
+__device__ __forceinline__ void wg_gemm(int m, int n, int k, const T *__restrict__ A, int lda, bool ca,
                         const T *__restrict__ B, int ldb, bool cb, T *__restrict__ C, int ldc,
                         T diag, T *smem) {
     constexpr int TM = 16 * M, KS = 16, TY = NT / 16, MJ = TM / TY, LPT = TM * KS / NT;
@@ -334,16 +334,25 @@ __device__ void build_h_block(const DevPlan &P, const double *RPc, const double 
 
 // ------------------------------------------------------------------------------ k_exp
 // One task = (crystallite c, step k, block) of a block-size group.  U = exp(-i theta H),
-// theta = 2 pi tau, as cos(X) - i sin(X) with X = theta H / 2^s, ||X||_inf <= 1/2 and the
-// degree-15 Taylor pair evaluated by Paterson-Stockmeyer in Y = X^2:
-//   p(Y) = (a0 + a1 Y + a2 Y^2 + a3 Y^3) + Y^4 (a4 + a5 Y + a6 Y^2 + a7 Y^3)
-// for cos (a_j = (-1)^j/(2j)!) and sin(X)/X (a_j = (-1)^j/(2j+1)!): 7 real GEMMs
-// (Y, Y^2, Y^3, Y^4, two Y^4 products, X * sinc), truncation error < 1e-18, then s squarings
-// (C,S) -> (C^2 - S^2, 2 C S).  Nine b x b scratch matrices per workgroup slot.
+// theta = 2 pi tau, as cos(X) - i sin(X) with X = theta H / 2^s, followed by s squarings
+// (C, S) -> (C^2 - S^2, 2 C S).  cos X and sin(X)/X are Taylor polynomials of degree m in
+// Y = X^2 (coefficients (-1)^j/(2j)! and (-1)^j/(2j+1)!), m chosen per task from
+// r = ||X||_inf as the smallest one whose remainder bound r^(2m+2)/(2m+2)! is below the
+// storage precision's tolerance (1e-17 for FP64, 3e-9 for FP32), and evaluated
+// Paterson-Stockmeyer style with the fewest products:
+//   m <= 4      powers Y..Y^m, polynomial formed elementwise         m + 1 GEMMs
+//   m in 5..6   Y, Y^2, Y^3;  p = q0(Y) + Y^3 q1(Y)                      6 GEMMs
+//   m in 7..8   Y..Y^4;       p = q0(Y) + Y^4 q1(Y)                      7 GEMMs
+// (each count includes S = X sinc).  s is the least with r / 2^s <= 0.8 (FP64) or 2 (FP32),
+// where m = 8 still meets the tolerance.  Nine b x b scratch matrices per workgroup slot.
 // The output is C + i usign S, i.e. exp(usign i theta H): usign = -1 for gamma-COMPUTE
 // (Mexp(H, -i 2 pi tau), compute.h:265), +1 for the pulse-program propagator (Mexp(hh,
 // +i tau 2 pi), propogation.cc:273).  With rf the step's phase similarity R = diag(p_r),
 // p_r = exp(-i sum_q phi_q M_q(r)) (M_q = Fz of pulsed isotope q), gives U = R U0 R^H.
+template <typename R> struct ExpTol;
+template <> struct ExpTol<double> { static constexpr double tol = 1e-17, rmax = 0.8; };
+template <> struct ExpTol<float> { static constexpr double tol = 3e-9, rmax = 2.0; };
+
 template <int M, typename R>
 __global__ void __launch_bounds__(NThr<M>::v) k_exp(DevPlan P, int Kb, int c0,
                                                     const int *__restrict__ grp, int ngrp,
@@ -351,22 +360,24 @@ __global__ void __launch_bounds__(NThr<M>::v) k_exp(DevPlan P, int Kb, int c0,
                                                     typename CX<R>::T *__restrict__ bufU) {
     constexpr int NT = NThr<M>::v;
     typedef typename CX<R>::T CT;
-    __shared__ R smem[2 * 16 * 16 * M];
+    __shared__ __align__(16) R smem[2 * 16 * 16 * M];
     __shared__ double s_zz[DEV_MAX_TERMS];
     __shared__ double s_ff[DEV_MAX_SPINS * DEV_MAX_SPINS];
     __shared__ double s_v[4];
     __shared__ double s_red[NT];
-    __shared__ cd s_ph[DEV_MAX_BLOCK];
     const int tid = threadIdx.x;
     R *X = scratch + slot * 9 * blockIdx.x;
     R *Y = X + slot, *Y2 = Y + slot, *Y3 = Y2 + slot, *Y4 = Y3 + slot;
     R *Hc = Y4 + slot, *Hs = Hc + slot, *Cm = Hs + slot, *Ps = Cm + slot;
     const long ntask = (long)Kb * P.n * ngrp;
-    const double theta = P.theta;
-    const R cc[8] = {(R)1.0, (R)(-1.0 / 2), (R)(1.0 / 24), (R)(-1.0 / 720), (R)(1.0 / 40320),
-                     (R)(-1.0 / 3628800), (R)(1.0 / 479001600), (R)(-1.0 / 87178291200.0)};
-    const R ss[8] = {(R)1.0, (R)(-1.0 / 6), (R)(1.0 / 120), (R)(-1.0 / 5040), (R)(1.0 / 362880),
-                     (R)(-1.0 / 39916800), (R)(1.0 / 6227020800.0), (R)(-1.0 / 1307674368000.0)};
+    // 1/(2j)! and 1/(2j+1)! with alternating signs, j = 0..8
+    const R cc[9] = {(R)1.0, (R)(-1.0 / 2), (R)(1.0 / 24), (R)(-1.0 / 720), (R)(1.0 / 40320),
+                     (R)(-1.0 / 3628800), (R)(1.0 / 479001600), (R)(-1.0 / 87178291200.0),
+                     (R)(1.0 / 20922789888000.0)};
+    const R ss[9] = {(R)1.0, (R)(-1.0 / 6), (R)(1.0 / 120), (R)(-1.0 / 5040), (R)(1.0 / 362880),
+                     (R)(-1.0 / 39916800), (R)(1.0 / 6227020800.0), (R)(-1.0 / 1307674368000.0),
+                     (R)(1.0 / 355687428096000.0)};
+    const R z0 = (R)0;
     for (long task = blockIdx.x; task < ntask; task += gridDim.x) {
         const int g = (int)(task % ngrp);
         const long ck = task / ngrp;
@@ -374,6 +385,7 @@ __global__ void __launch_bounds__(NThr<M>::v) k_exp(DevPlan P, int Kb, int c0,
         const int blk = grp[g];
         const int b = P.blk_dim[blk];
         const int bb = b * b;
+        const double theta = P.thetak ? P.thetak[k] : P.theta;
         const double *rfk = P.nrfiso > 0 ? P.rf + (size_t)3 * P.nrfiso * k : nullptr;
         build_h_block(P, P.RP + 9 * (size_t)(c0 + c), P.u3 + 3 * k, blk, X, b, s_zz, s_ff, s_v,
                       rfk);
@@ -392,37 +404,90 @@ __global__ void __launch_bounds__(NThr<M>::v) k_exp(DevPlan P, int Kb, int c0,
         const double nrm = s_red[0] * theta;
         __syncthreads();
         int sq = 0;
-        while (nrm / (double)(1 << sq) > 0.5 && sq < 30) sq++;
+        while (nrm / (double)(1 << sq) > ExpTol<R>::rmax && sq < 30) sq++;
+        const double rx = nrm / (double)(1 << sq);
+        // smallest degree m in Y with rx^(2m+2)/(2m+2)! <= tol (uniform across the workgroup)
+        int m = 1;
+        {
+            double term = rx * rx * rx * rx / 24.0; // m = 1: rx^4/4!
+            while (m < 8 && term > ExpTol<R>::tol) {
+                m++;
+                term *= rx * rx / ((2.0 * m + 1.0) * (2.0 * m + 2.0));
+            }
+            if (m == 5) m = 6;
+            if (m == 7) m = 8;
+        }
         const R sc = (R)(theta / (double)(1 << sq));
         for (int e = tid; e < bb; e += NT) X[e] *= sc;
         __syncthreads();
-        const R z0 = (R)0;
-        wg_gemm<M, NT, R>(b, b, b, X, b, false, X, b, false, Y, b, z0, smem);
-        wg_gemm<M, NT, R>(b, b, b, Y, b, false, Y, b, false, Y2, b, z0, smem);
-        wg_gemm<M, NT, R>(b, b, b, Y2, b, false, Y, b, false, Y3, b, z0, smem);
-        wg_gemm<M, NT, R>(b, b, b, Y2, b, false, Y2, b, false, Y4, b, z0, smem);
-        for (int e = tid; e < bb; e += NT) {
-            R d = (e % b == e / b) ? (R)1 : (R)0;
-            R y = Y[e], y2 = Y2[e], y3 = Y3[e];
-            Hc[e] = cc[4] * d + cc[5] * y + cc[6] * y2 + cc[7] * y3;
-            Hs[e] = ss[4] * d + ss[5] * y + ss[6] * y2 + ss[7] * y3;
+        // powers Y = X X, Y^2 = Y Y, Y^3 = Y^2 Y, Y^4 = Y^2 Y^2 as far as m needs them (one GEMM
+        // call site per loop keeps the kernel small enough for its tiles to stay in registers)
+        const int npow = (m == 6) ? 3 : (m < 4 ? m : 4);
+#pragma unroll 1
+        for (int i = 0; i < npow; i++) {
+            const R *A = (i == 0) ? X : (i == 1) ? Y : Y2;
+            const R *B = (i == 0) ? X : (i == 3) ? Y2 : Y;
+            R *C = (i == 0) ? Y : (i == 1) ? Y2 : (i == 2) ? Y3 : Y4;
+            wg_gemm<M, NT, R>(b, b, b, A, b, false, B, b, false, C, b, z0, smem);
         }
-        __syncthreads();
-        wg_gemm<M, NT, R>(b, b, b, Y4, b, false, Hc, b, false, Cm, b, z0, smem);
-        wg_gemm<M, NT, R>(b, b, b, Y4, b, false, Hs, b, false, Ps, b, z0, smem);
-        for (int e = tid; e < bb; e += NT) {
-            R d = (e % b == e / b) ? (R)1 : (R)0;
-            R y = Y[e], y2 = Y2[e], y3 = Y3[e];
-            Cm[e] += cc[0] * d + cc[1] * y + cc[2] * y2 + cc[3] * y3;
-            Ps[e] += ss[0] * d + ss[1] * y + ss[2] * y2 + ss[3] * y3;
+        if (m <= 4) {
+            for (int e = tid; e < bb; e += NT) {
+                R d = (e % b == e / b) ? (R)1 : (R)0;
+                R y = Y[e];
+                R cv = cc[0] * d + cc[1] * y, sv = ss[0] * d + ss[1] * y;
+                if (m >= 2) { R y2 = Y2[e]; cv += cc[2] * y2; sv += ss[2] * y2; }
+                if (m >= 3) { R y3 = Y3[e]; cv += cc[3] * y3; sv += ss[3] * y3; }
+                if (m >= 4) { R y4 = Y4[e]; cv += cc[4] * y4; sv += ss[4] * y4; }
+                Cm[e] = cv;
+                Ps[e] = sv;
+            }
+            __syncthreads();
+        } else {
+            // p = q0(Y) + Yh q1(Y): Yh = Y^3 (m = 6) or Y^4 (m = 8); the two cases are
+            // spelled out so the coefficient tables are only indexed by constants
+            const bool h4 = (m == 8);
+            const R *Yh = h4 ? Y4 : Y3;
+            for (int e = tid; e < bb; e += NT) {
+                R d = (e % b == e / b) ? (R)1 : (R)0;
+                R y = Y[e], y2 = Y2[e], y3 = Y3[e];
+                R cv, sv;
+                if (h4) {
+                    R y4 = Y4[e];
+                    cv = cc[4] * d + cc[5] * y + cc[6] * y2 + cc[7] * y3 + cc[8] * y4;
+                    sv = ss[4] * d + ss[5] * y + ss[6] * y2 + ss[7] * y3 + ss[8] * y4;
+                } else {
+                    cv = cc[3] * d + cc[4] * y + cc[5] * y2 + cc[6] * y3;
+                    sv = ss[3] * d + ss[4] * y + ss[5] * y2 + ss[6] * y3;
+                }
+                Hc[e] = cv;
+                Hs[e] = sv;
+            }
+            __syncthreads();
+#pragma unroll 1
+            for (int i = 0; i < 2; i++)
+                wg_gemm<M, NT, R>(b, b, b, Yh, b, false, i ? Hs : Hc, b, false, i ? Ps : Cm, b, z0,
+                                  smem);
+            for (int e = tid; e < bb; e += NT) {
+                R d = (e % b == e / b) ? (R)1 : (R)0;
+                R y = Y[e], y2 = Y2[e];
+                R cv = cc[0] * d + cc[1] * y + cc[2] * y2;
+                R sv = ss[0] * d + ss[1] * y + ss[2] * y2;
+                if (h4) { R y3 = Y3[e]; cv += cc[3] * y3; sv += ss[3] * y3; }
+                Cm[e] += cv;
+                Ps[e] += sv;
+            }
+            __syncthreads();
         }
-        __syncthreads();
         R *Sm = Y4;
         wg_gemm<M, NT, R>(b, b, b, X, b, false, Ps, b, false, Sm, b, z0, smem);
         for (int it = 0; it < sq; it++) {
-            wg_gemm<M, NT, R>(b, b, b, Cm, b, false, Cm, b, false, Y, b, z0, smem);
-            wg_gemm<M, NT, R>(b, b, b, Sm, b, false, Sm, b, false, Y2, b, z0, smem);
-            wg_gemm<M, NT, R>(b, b, b, Cm, b, false, Sm, b, false, Y3, b, z0, smem);
+#pragma unroll 1
+            for (int i = 0; i < 3; i++) {
+                const R *A = (i == 1) ? Sm : Cm;
+                const R *B = (i == 0) ? Cm : Sm;
+                R *C = (i == 0) ? Y : (i == 1) ? Y2 : Y3;
+                wg_gemm<M, NT, R>(b, b, b, A, b, false, B, b, false, C, b, z0, smem);
+            }
             for (int e = tid; e < bb; e += NT) {
                 Cm[e] = Y[e] - Y2[e];
                 Sm[e] = (R)2 * Y3[e];
@@ -432,6 +497,8 @@ __global__ void __launch_bounds__(NThr<M>::v) k_exp(DevPlan P, int Kb, int c0,
         CT *U = bufU + ((size_t)c * P.n + k) * P.Lsum + P.blk_uoff[blk];
         const double us = (double)P.usign;
         if (rfk) {
+            // the GEMM staging area is free here: it holds the b phase factors (b <= 16 M)
+            cd *s_ph = (cd *)smem;
             const int *st = P.blk_states + P.blk_soff[blk];
             for (int r = tid; r < b; r += NT) {
                 double phi = 0.0;

@@ -7,7 +7,7 @@ Lines starting with ``#`` or ``%`` are skipped.  Angles are radians.  Weights ar
 renormalised by the reader (only the built-in generators normalise, powder.cc:310).
 
 The ZCW_3_1154, ZCW_3_3722 and rep2000 sets shipped in ``crystals/`` are the reference's data
-files (crystals/README).
+files (crystals/README).  ``Powder.generate`` restates the built-in generators.
 """
 from __future__ import annotations
 
@@ -65,6 +65,74 @@ class Powder:
         f64 = np.float64
         return cls(np.array(th, f64), np.array(ph, f64), np.array(ga, f64), np.array(w, f64),
                    os.path.basename(path))
+
+    @classmethod
+    def generate(cls, method: str, tstep: int, pstep: int, gstep: int = 1) -> "Powder":
+        """Built-in orientation sets of powder::calcPow (powder.cc:179-311): ``zcw`` (also
+        ``zcw2d``), ``alderman``, ``sphere``, ``rect``, ``sophie``, ``liquid``; each set is
+        repeated for max(gstep, 1) gamma values g/gstep * pi and the weights are normalised to
+        sum 1.  Any other name is read as a file (powder.cc:98-111)."""
+        m = {"zcw2d": "zcw", "zcw2D": "zcw"}.get(method, method)
+        if m not in ("zcw", "alderman", "sphere", "rect", "sophie", "liquid"):
+            return cls.read(method)
+        th, ph, w = [], [], []
+        pi = np.pi
+        if m == "zcw":
+            ts, ps = (21, 34, 55, 89, 144, 233, 377, 616, 987, 4184), (8, 13, 21, 34, 55, 89, 144,
+                                                                         233, 377, 2584)
+            if tstep not in ts or pstep not in ps or tstep <= pstep:
+                raise BlochLibError("For using the ZCW method 'theta_step' and 'phi_step' must be "
+                                    "fibonacci numbers where 'theta_step' > 'phi_step'")
+            for k in range(1, tstep):
+                fac = k / tstep
+                f0 = fac - np.floor(fac)
+                f1 = fac * pstep
+                f1 = f1 - np.floor(f1)
+                ph.append(2.0 * pi * f0)
+                th.append(pi * f1)
+                w.append(np.sin(pi * f1) / tstep)
+        elif m == "sphere":
+            for k in range(1, tstep + 1):
+                for q in range(1, pstep + 1):
+                    t = (pi / 2.0) * (k / tstep)
+                    th.append(t)
+                    ph.append(2 * pi * (q / (pstep + 1)))
+                    w.append(np.sin(t))
+        elif m == "rect":
+            t = pi / (2.0 * tstep)
+            while t <= pi / 2.0:
+                nmax = 2.0 * np.sin(t) * pstep
+                q = 0
+                while q < nmax:
+                    ph.append(2 * pi * q / nmax)
+                    th.append(t)
+                    w.append(np.sin(t))
+                    q += 1
+                t += pi / (2.0 * tstep)
+        elif m == "sophie":
+            for k in range(1, tstep):
+                for q in range(1, k + 1):
+                    th.append((pi / 2) * (k / (tstep - 1)))
+                    ph.append((pi / 2) * ((k - q + 1) / k))
+                    w.append(1.0)
+        elif m == "alderman":
+            for kk in range(2):
+                for i in range(1, tstep + 1):
+                    for j in range(1, tstep - i + 1):
+                        R = np.sqrt(float(i * i + j * j + (tstep - i - j) ** 2))
+                        w.append(1.0 / R ** 3)
+                        th.append(np.arccos((tstep - i - j) / R))
+                        ph.append(np.arctan(j / i) + kk * pi / 2)
+        else:  # liquid
+            th, ph, w = [0.0], [0.0], [1.0]
+        ng = max(gstep, 1)
+        gam = [0.0 if gstep == 0 else g / gstep * pi for g in range(ng)]
+        f64 = np.float64
+        T = np.tile(np.array(th, f64), ng)
+        P = np.tile(np.array(ph, f64), ng)
+        G = np.repeat(np.array(gam, f64), len(th)) if m != "liquid" else np.zeros(ng)
+        W = np.tile(np.array(w, f64), ng)
+        return cls(T, P, G, W / W.sum(), f"{method}({tstep},{pstep},{gstep})")
 
     @classmethod
     def single(cls, theta=0.0, phi=0.0, gamma=0.0, weight=1.0) -> "Powder":

@@ -772,12 +772,17 @@ static void add_rf(const orc_sys *sy, const double *rfk, int nrf, cplx *H) {
     }
 }
 
-int orc_ptop_fid(int S, const int *isotopes, int ncsa, const double *csa, int ndip,
-                 const double *dip, int nj, const double *jc, double wr, double rotor_deg,
-                 int nsteps, int npts, double t0, int nrf, const double *rf, int K,
-                 const double *theta, const double *phi, const double *gamma, const double *w,
-                 const double *rho, const double *det, int normalize, double *fid_out) {
-    if (S < 1 || S > 12 || nsteps < 1 || npts < 0 || K < 0 || wr <= 0.0) return ORC_EINVAL;
+/* step_dt: length of each step (consecutive from t0), or NULL for nsteps equal steps of one
+ * rotor period; a pulse segment of length T split into m substeps is m entries T/m
+ * (propogation.cc:264-300: tau /= tmpp; t1 += tau; t2 += tau). */
+int orc_pulse_fid(int S, const int *isotopes, int ncsa, const double *csa, int ndip,
+                  const double *dip, int nj, const double *jc, double wr, double rotor_deg,
+                  int nsteps, const double *step_dt, int npts, double t0, int nrf,
+                  const double *rf, int K, const double *theta, const double *phi,
+                  const double *gamma, const double *w, const double *rho, const double *det,
+                  int normalize, double *fid_out) {
+    if (S < 1 || S > 12 || nsteps < 1 || npts < 0 || K < 0) return ORC_EINVAL;
+    if (!step_dt && wr <= 0.0) return ORC_EINVAL;
     orc_sys sy = {S, 1 << S, isotopes, ncsa, csa, ndip, dip, nj, jc};
     const int N = sy.N;
     const size_t NN = (size_t)N * N;
@@ -785,24 +790,30 @@ int orc_ptop_fid(int S, const int *isotopes, int ncsa, const double *csa, int nd
     cplx *H = malloc(sizeof(cplx) * NN), *X = malloc(sizeof(cplx) * NN), *U = malloc(sizeof(cplx) * NN);
     cplx *T = malloc(sizeof(cplx) * NN), *r = malloc(sizeof(cplx) * NN), *acc = calloc(npts > 0 ? npts : 1, sizeof(cplx));
     if (!H || !X || !U || !T || !r || !acc) { free(H); free(X); free(U); free(T); free(r); free(acc); return ORC_ENOMEM; }
-    double tau = (1.0 / wr) / nsteps;
+    double tau = step_dt ? 0.0 : (1.0 / wr) / nsteps;
     int st = ORC_OK;
     for (int c = 0; c < K && st == ORC_OK; c++) {
         double cp[9];
         rot3(phi[c], theta[c], gamma[c], cp);
         for (size_t q = 0; q < NN; q++) U[q] = 0;
         for (int i = 0; i < N; i++) U[i + (size_t)i * N] = 1.0;
-        double t1 = t0, t2 = t0 + tau;
+        double t1 = t0, t2 = t0 + (step_dt ? step_dt[0] : tau);
         for (int k = 0; k < nsteps; k++) {
+            double dt = step_dt ? step_dt[k] : tau;
             double tmid = (t2 - t1) / 2.0 + t1;
             build_hamiltonian(&sy, cp, rotor_deg * DEG2RAD, tmid * wr * PI2, H);
             add_rf(&sy, rf + (size_t)k * 4 * nrf, nrf, H);
-            st = orc_expm_herm(N, H, CMPLX(0.0, 1.0) * tau * PI2, X);
+            st = orc_expm_herm(N, H, CMPLX(0.0, 1.0) * dt * PI2, X);
             if (st) break;
             zgemm(N, 'N', 'N', X, U, T);
             memcpy(U, T, sizeof(cplx) * NN);
-            t1 += tau;
-            t2 += tau;
+            if (step_dt) {
+                t1 = t2;
+                if (k + 1 < nsteps) t2 = t1 + step_dt[k + 1];
+            } else {
+                t1 += tau;
+                t2 += tau;
+            }
         }
         if (st) break;
         memcpy(r, R0, sizeof(cplx) * NN);
@@ -827,6 +838,15 @@ int orc_ptop_fid(int S, const int *isotopes, int ncsa, const double *csa, int nd
     }
     free(H); free(X); free(U); free(T); free(r); free(acc);
     return st;
+}
+
+int orc_ptop_fid(int S, const int *isotopes, int ncsa, const double *csa, int ndip,
+                 const double *dip, int nj, const double *jc, double wr, double rotor_deg,
+                 int nsteps, int npts, double t0, int nrf, const double *rf, int K,
+                 const double *theta, const double *phi, const double *gamma, const double *w,
+                 const double *rho, const double *det, int normalize, double *fid_out) {
+    return orc_pulse_fid(S, isotopes, ncsa, csa, ndip, dip, nj, jc, wr, rotor_deg, nsteps, NULL,
+                         npts, t0, nrf, rf, K, theta, phi, gamma, w, rho, det, normalize, fid_out);
 }
 
 /* Test hooks (exported): the pieces above on their own. */

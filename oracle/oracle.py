@@ -40,6 +40,12 @@ def lib():
                                    ctypes.c_int, _d, ctypes.c_double, ctypes.c_double,
                                    ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, _d,
                                    ctypes.c_int, _d, _d, _d, _d, _d, _d, ctypes.c_int, _d]
+        L.orc_pulse_fid.restype = ctypes.c_int
+        L.orc_pulse_fid.argtypes = [ctypes.c_int, _i, ctypes.c_int, _d, ctypes.c_int, _d,
+                                    ctypes.c_int, _d, ctypes.c_double, ctypes.c_double,
+                                    ctypes.c_int, _d, ctypes.c_int, ctypes.c_double,
+                                    ctypes.c_int, _d, ctypes.c_int, _d, _d, _d, _d, _d, _d,
+                                    ctypes.c_int, _d]
         L.orc_hamiltonian.argtypes = [ctypes.c_int, _i, ctypes.c_int, _d, ctypes.c_int, _d,
                                       ctypes.c_int, _d] + [ctypes.c_double] * 5 + [_d]
         L.orc_eig_herm_x.argtypes = [ctypes.c_int, _d, _d, _d]
@@ -145,11 +151,16 @@ def rf_table(sy, rf, nsteps):
     return np.ascontiguousarray(tab.ravel()), max(len(rf), 1)
 
 
-def ptop_fid(sy, wr, rotor_deg, nsteps, npts, powder, rho, det, rf=(), t0=0.0, normalize=True):
-    """Rotor-synchronised point-to-point FID with rf over one rotor period per point."""
+def ptop_fid(sy, wr, rotor_deg, nsteps, npts, powder, rho, det, rf=(), t0=0.0, normalize=True,
+             step_dt=None):
+    """Point-to-point FID with rf: one application of the sub-sequence per point.  The
+    sub-sequence is one rotor period of nsteps equal steps, or the consecutive steps of lengths
+    step_dt."""
     L = lib()
     iso, nc, csa, nd, dip, nj, jc = _sys(sy)
     tab, nrf = rf_table(sy, list(rf), nsteps)
+    if step_dt is not None:
+        dts = np.ascontiguousarray(np.broadcast_to(np.asarray(step_dt, np.float64), (nsteps,)))
     r = _cm(rho)
     d = _cm(det)
     out = np.zeros(2 * max(npts, 1))
@@ -157,10 +168,11 @@ def ptop_fid(sy, wr, rotor_deg, nsteps, npts, powder, rho, det, rf=(), t0=0.0, n
     ph = np.ascontiguousarray(powder.phi, np.float64)
     ga = np.ascontiguousarray(powder.gamma, np.float64)
     w = np.ascontiguousarray(powder.weight, np.float64)
-    st = L.orc_ptop_fid(sy.spins, _p(iso, _i), nc, _p(csa), nd, _p(dip), nj, _p(jc), float(wr),
-                        float(rotor_deg), int(nsteps), int(npts), float(t0), nrf, _p(tab),
-                        len(th), _p(th), _p(ph), _p(ga), _p(w), _p(r), _p(d),
-                        int(bool(normalize)), _p(out))
+    dptr = _p(dts) if step_dt is not None else None
+    st = L.orc_pulse_fid(sy.spins, _p(iso, _i), nc, _p(csa), nd, _p(dip), nj, _p(jc), float(wr),
+                         float(rotor_deg), int(nsteps), dptr, int(npts), float(t0), nrf, _p(tab),
+                         len(th), _p(th), _p(ph), _p(ga), _p(w), _p(r), _p(d),
+                         int(bool(normalize)), _p(out))
     if st != 0:
         raise RuntimeError(f"oracle failed with status {st}")
     return out[: 2 * npts].view(np.complex128).copy()

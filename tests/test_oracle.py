@@ -283,3 +283,23 @@ def test_ptop_offset_closed_forms(oracle):
     f0 = oracle.ptop_fid(sy, wr, MAGIC, n, 11, Powder.single(), sy.op("Ix"), sy.op("Ix"),
                          rf=[("1H", np.zeros(n), np.zeros(n), np.full(n, o))], normalize=False)
     assert np.allclose(f0, 0.5, atol=1e-14)
+
+
+@pytest.mark.parametrize("wr", [12500.0, 0.0])
+def test_unequal_steps_match_explicit_numpy(oracle, wr):
+    """A general sub-sequence: consecutive steps of different lengths (pulses and delays split
+    into substeps), also without spinning."""
+    sy = SolidSys.from_text(HET)
+    dts = np.array([2e-6] * 5 + [7.5e-6] * 3 + [1e-6] * 4)
+    n = len(dts)
+    amp = np.r_[np.full(5, 60000.0), np.zeros(3), np.full(4, 25000.0)]
+    ph = np.r_[np.zeros(5), np.zeros(3), np.full(4, 1.1)]
+    rf = [("1H", amp, ph, np.zeros(n)), ("13C", np.r_[np.zeros(8), np.full(4, 9000.0)],
+                                        np.full(n, 0.4), np.full(n, 150.0))]
+    rho, det = sy.op("Ix_0"), sy.op("Ip_0")
+    for th, ph_, ga in [(0.4, 1.2, 2.0), (2.2, 5.0, 0.3)]:
+        f1 = oracle.ptop_fid(sy, wr, MAGIC if wr else 0.0, n, 10, Powder.single(th, ph_, ga), rho,
+                             det, rf=rf, t0=3e-6, normalize=False, step_dt=dts)
+        f2 = B.ptop_equivalent(sy, th, ph_, ga, MAGIC if wr else 0.0, wr, n, 10, rho, det, rf=rf,
+                               t0=3e-6, step_dt=dts)
+        assert np.abs(f1 - f2).max() < 1e-11

@@ -157,3 +157,23 @@ def test_config4_full_points_invariants():
     bound = sw * np.linalg.norm(sy.op("Ix_0")) * np.linalg.norm(det) / np.linalg.norm(det) ** 2
     assert np.abs(fx).max() <= bound * (1 + 1e-12)
     assert np.all(np.isfinite(fx))
+
+
+@pytest.mark.parametrize("wr,rotor", [(12500.0, MAGIC), (0.0, 0.0)])
+@pytest.mark.parametrize("precision,tol", [("fp64", TOL), ("c64", 1e-5)])
+def test_general_subsequence_unequal_steps(wr, rotor, precision, tol):
+    """Pulses and delays of different lengths (blfid_plan_create_pulse), spinning or static,
+    both precisions."""
+    sy = SolidSys.from_text(HET)
+    dts = np.array([2e-6] * 5 + [7.5e-6] * 3 + [1e-6] * 4)
+    n = len(dts)
+    rf = [("1H", np.r_[np.full(5, 60000.0), np.zeros(3), np.full(4, 25000.0)],
+           np.r_[np.zeros(8), np.full(4, 1.1)], np.zeros(n)),
+          ("13C", np.r_[np.zeros(8), np.full(4, 9000.0)], np.full(n, 0.4), np.full(n, 150.0))]
+    rho, det = sy.op("Ix_0"), sy.op("Ip_0")
+    print(f"[ptop] unequal steps wr={wr} {precision}", flush=True)
+    g = engine.pulse_fid(sy, wr, rotor, dts, 30, PW, rho, det, rf=rf, t0=3e-6, precision=precision)
+    c = O.ptop_fid(sy, wr, rotor, n, 30, PW, rho, det, rf=rf, t0=3e-6, step_dt=dts)
+    err = rel_l2(g, c)
+    print(f"[ptop] unequal steps: rel-L2 {err:.3e}", flush=True)
+    assert err < tol
