@@ -176,6 +176,7 @@ struct blfid_plan {
     std::vector<int> spin_rf;
     std::vector<double> rf;
     std::vector<double> step_dt; // unequal step lengths (blfid_plan_create_pulse), else empty
+    std::vector<double> step_t1; // explicit step start times, else empty (consecutive)
     // derived
     int n = 1;
     double sw_adj = 0, tau = 0;
@@ -443,17 +444,18 @@ static blfid_status build_tables(blfid_plan *pl) {
     if (pl->spinning || pl->ptop) {
         double br = pl->rotor_deg * DEG2RAD;
         const bool uneq = !pl->step_dt.empty();
-        double t1 = pl->t0, t2 = pl->t0 + (uneq ? pl->step_dt[0] : pl->tau);
+        double t1 = (uneq && !pl->step_t1.empty()) ? pl->step_t1[0] : pl->t0;
+        double t2 = t1 + (uneq ? pl->step_dt[0] : pl->tau);
         for (int k = 0; k < pl->n; k++) {
             double tmid = (t2 - t1) / 2.0 + t1;
             double al = tmid * pl->wr * PI2;
             pl->u3.push_back(std::sin(al) * std::sin(br));
             pl->u3.push_back(-std::cos(al) * std::sin(br));
             pl->u3.push_back(std::cos(br));
-            if (uneq) {
-                // consecutive segments: the next one starts where this one ends
-                t1 = t2;
-                if (k + 1 < pl->n) t2 = t1 + pl->step_dt[k + 1];
+            if (uneq && k + 1 < pl->n) {
+                // the next step starts at its own time, or where this one ends
+                t1 = pl->step_t1.empty() ? t2 : pl->step_t1[k + 1];
+                t2 = t1 + pl->step_dt[k + 1];
             } else {
                 t1 += pl->tau;
                 t2 += pl->tau;
@@ -653,9 +655,9 @@ blfid_status blfid_plan_create(const blfid_system *sys, double wr_hz, double rot
 // Shared body of the two pulse-program plan constructors: `steps` steps of length step_dt[k]
 // (or 1/(wr steps) each when step_dt is NULL), consecutive from t0.
 static blfid_status create_pulse_plan(const blfid_system *sys, double wr_hz, double rotor_deg,
-                                      int steps, const double *step_dt, int npts, double t0,
-                                      int nrf, const blfid_rf *rf, blfid_precision prec,
-                                      blfid_plan **out) {
+                                      int steps, const double *step_t1, const double *step_dt,
+                                      int npts, double t0, int nrf, const blfid_rf *rf,
+                                      blfid_precision prec, blfid_plan **out) {
     blfid_status cs = check_system(sys, npts, prec, out);
     if (cs != BLFID_OK) return cs;
     if (steps < 1) return fail(BLFID_ERR_INVALID, "a pulse program needs at least one step");
@@ -665,7 +667,8 @@ static blfid_status create_pulse_plan(const blfid_system *sys, double wr_hz, dou
         if (!(wr_hz >= 0.0) || !std::isfinite(wr_hz))
             return fail(BLFID_ERR_INVALID, "rotor speed must be finite and >= 0");
         for (int k = 0; k < steps; k++)
-            if (!(step_dt[k] > 0.0) || !std::isfinite(step_dt[k]))
+            if (!(step_dt[k] > 0.0) || !std::isfinite(step_dt[k]) ||
+                (step_t1 && !std::isfinite(step_t1[k])))
                 return fail(BLFID_ERR_INVALID, "step lengths must be positive and finite");
     }
     if (nrf < 0 || (nrf > 0 && !rf)) return fail(BLFID_ERR_INVALID, "bad rf channel list");
@@ -730,6 +733,7 @@ static blfid_status create_pulse_plan(const blfid_system *sys, double wr_hz, dou
     pl->n = steps;
     if (step_dt) {
         pl->step_dt.assign(step_dt, step_dt + steps);
+        if (step_t1) pl->step_t1.assign(step_t1, step_t1 + steps);
         double T = 0;
         for (double d : pl->step_dt) T += d;
         pl->tau = T / steps;
@@ -750,18 +754,20 @@ extern "C" {
 blfid_status blfid_plan_create_ptop(const blfid_system *sys, double wr_hz, double rotor_deg,
                                     int steps, int npts, double t0, int nrf, const blfid_rf *rf,
                                     blfid_precision prec, blfid_plan **out) {
-    return create_pulse_plan(sys, wr_hz, rotor_deg, steps, nullptr, npts, t0, nrf, rf, prec, out);
+    return create_pulse_plan(sys, wr_hz, rotor_deg, steps, nullptr, nullptr, npts, t0, nrf, rf,
+                             prec, out);
 }
 
 blfid_status blfid_plan_create_pulse(const blfid_system *sys, double wr_hz, double rotor_deg,
-                                     int nsteps, const double *step_dt, int npts, double t0,
-                                     int nrf, const blfid_rf *rf, blfid_precision prec,
-                                     blfid_plan **out) {
+                                     int nsteps, const double *step_t1, const double *step_dt,
+                                     int npts, double t0, int nrf, const blfid_rf *rf,
+                                     blfid_precision prec, blfid_plan **out) {
     if (!step_dt) {
         if (out) *out = nullptr;
         return fail(BLFID_ERR_INVALID, "step_dt is NULL");
     }
-    return create_pulse_plan(sys, wr_hz, rotor_deg, nsteps, step_dt, npts, t0, nrf, rf, prec, out);
+    return create_pulse_plan(sys, wr_hz, rotor_deg, nsteps, step_t1, step_dt, npts, t0, nrf, rf,
+                             prec, out);
 }
 
 void blfid_plan_destroy(blfid_plan *pl) {

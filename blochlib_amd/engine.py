@@ -74,9 +74,10 @@ def lib():
                                              ctypes.c_double, ctypes.c_int, ctypes.POINTER(_RF),
                                              ctypes.c_int, ctypes.POINTER(vp)]
         L.blfid_plan_create_pulse.argtypes = [ctypes.POINTER(_System), ctypes.c_double,
-                                              ctypes.c_double, ctypes.c_int, _d, ctypes.c_int,
-                                              ctypes.c_double, ctypes.c_int, ctypes.POINTER(_RF),
-                                              ctypes.c_int, ctypes.POINTER(vp)]
+                                              ctypes.c_double, ctypes.c_int, _d, _d,
+                                              ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                              ctypes.POINTER(_RF), ctypes.c_int,
+                                              ctypes.POINTER(vp)]
         L.blfid_plan_destroy.argtypes = [vp]
         L.blfid_plan_destroy.restype = None
         L.blfid_plan_sw.argtypes = [vp]
@@ -161,7 +162,7 @@ class Plan:
             _check(L.blfid_plan_create(ctypes.byref(s), float(wr), float(rotor_deg), float(sw),
                                        int(npts), float(t0), prec, ctypes.byref(h)))
         else:
-            steps, rf, step_dt = _ptop
+            steps, rf, step_dt, step_t1 = _ptop
             ids = {}
             for n_ in sy.isotopes:
                 ids.setdefault(n_, len(ids))
@@ -180,9 +181,13 @@ class Plan:
             else:
                 self._dt = np.ascontiguousarray(
                     np.broadcast_to(np.asarray(step_dt, np.float64), (steps,)))
+                self._t1 = (None if step_t1 is None else np.ascontiguousarray(
+                    np.broadcast_to(np.asarray(step_t1, np.float64), (steps,))))
                 _check(L.blfid_plan_create_pulse(ctypes.byref(s), float(wr), float(rotor_deg),
-                                                 int(steps), _p(self._dt), int(npts), float(t0),
-                                                 len(chans), arr, prec, ctypes.byref(h)))
+                                                 int(steps),
+                                                 None if self._t1 is None else _p(self._t1),
+                                                 _p(self._dt), int(npts), float(t0), len(chans),
+                                                 arr, prec, ctypes.byref(h)))
         self.h = h
         self.N = sy.N
         self.npts = int(npts)
@@ -193,16 +198,17 @@ class Plan:
         """Rotor-synchronised pulse-program plan (blfid_plan_create_ptop).  rf: channels
         (isotope name, amp_hz, phase_rad, offset_hz), each a scalar or one value per step."""
         return cls(sy, wr, rotor_deg, 0.0, npts, t0, precision=precision,
-                   _ptop=(int(steps), list(rf), None))
+                   _ptop=(int(steps), list(rf), None, None))
 
     @classmethod
     def pulse(cls, sy: SolidSys, wr: float, rotor_deg: float, step_dt, npts: int, rf=(),
-              t0: float = 0.0, precision: str = "fp64") -> "Plan":
-        """Point-to-point plan of a general sub-sequence (blfid_plan_create_pulse): consecutive
-        steps of lengths step_dt (s), rf channels with one entry per step."""
+              t0: float = 0.0, precision: str = "fp64", step_t1=None) -> "Plan":
+        """Point-to-point plan of a general sub-sequence (blfid_plan_create_pulse): steps of
+        lengths step_dt (s) starting at step_t1 (or consecutive from t0), rf channels with one
+        entry per step."""
         dt = np.atleast_1d(np.asarray(step_dt, np.float64))
         return cls(sy, wr, rotor_deg, 0.0, npts, t0, precision=precision,
-                   _ptop=(len(dt), list(rf), dt))
+                   _ptop=(len(dt), list(rf), dt, step_t1))
 
     def close(self):
         if getattr(self, "h", None) and self.h.value:
@@ -300,10 +306,11 @@ def powder_fid(sy, wr, rotor_deg, sw, npts, pw, rho, det, t0=0.0, normalize=True
 
 
 def pulse_fid(sy, wr, rotor_deg, step_dt, npts, pw, rho, det, rf=(), t0=0.0, normalize=True,
-              precision="fp64"):
-    """Powder-summed point-to-point FID of a general sub-sequence (steps of lengths step_dt),
-    one FID point per application of the whole sub-sequence."""
-    p = Plan.pulse(sy, wr, rotor_deg, step_dt, npts, rf, t0, precision)
+              precision="fp64", step_t1=None):
+    """Powder-summed point-to-point FID of a general sub-sequence (steps of lengths step_dt,
+    starting at step_t1 or consecutive from t0), one FID point per application of the whole
+    sub-sequence."""
+    p = Plan.pulse(sy, wr, rotor_deg, step_dt, npts, rf, t0, precision, step_t1)
     try:
         return p.powder_fid(pw, rho, det, normalize)
     finally:

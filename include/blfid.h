@@ -98,15 +98,16 @@ blfid_status blfid_plan_create_ptop(const blfid_system *sys, double wr_hz, doubl
                                     int steps, int npts, double t0, int nrf, const blfid_rf *rf,
                                     blfid_precision prec, blfid_plan **out);
 
-/* The same for a general sub-sequence: `nsteps` consecutive steps of lengths step_dt[k] (s)
- * starting at t0, each with the rf tables' entry k, e.g. a list of pulses and delays each split
- * into ceil(length / maxtstep) equal steps as Propogation::propogator does
- * (propogation.cc:257-300).  wr may be 0 (no rotor phase).  One FID point per application of
- * the whole sub-sequence (blfid_plan_sw = 1 / total length). */
+/* The same for a general sub-sequence: `nsteps` steps of lengths step_dt[k] (s), each with the
+ * rf tables' entry k, e.g. a list of pulses and delays each split into ceil(length / maxtstep)
+ * equal steps as Propogation::propogator does (propogation.cc:257-300).  Step k spans
+ * [step_t1[k], step_t1[k] + step_dt[k]] (the rotor phase is taken at its midpoint); with
+ * step_t1 == NULL the steps are consecutive from t0.  wr may be 0 (no rotor phase).  One FID
+ * point per application of the whole sub-sequence (blfid_plan_sw = 1 / total length). */
 blfid_status blfid_plan_create_pulse(const blfid_system *sys, double wr_hz, double rotor_deg,
-                                     int nsteps, const double *step_dt, int npts, double t0,
-                                     int nrf, const blfid_rf *rf, blfid_precision prec,
-                                     blfid_plan **out);
+                                     int nsteps, const double *step_t1, const double *step_dt,
+                                     int npts, double t0, int nrf, const blfid_rf *rf,
+                                     blfid_precision prec, blfid_plan **out);
 
 /* Adjusted sweep width n*wr (compute.h:96, oneFID.h:429), steps per rotor period n, N. */
 double blfid_plan_sw(const blfid_plan *plan);

@@ -772,12 +772,14 @@ static void add_rf(const orc_sys *sy, const double *rfk, int nrf, cplx *H) {
     }
 }
 
-/* step_dt: length of each step (consecutive from t0), or NULL for nsteps equal steps of one
- * rotor period; a pulse segment of length T split into m substeps is m entries T/m
- * (propogation.cc:264-300: tau /= tmpp; t1 += tau; t2 += tau). */
+/* step_dt: length of each step, or NULL for nsteps equal steps of one rotor period; step_t1:
+ * start time of each step, or NULL for consecutive steps from t0.  A pulse segment of length T
+ * split into m substeps is m entries T/m (propogation.cc:264-300: tau /= tmpp; t1 += tau;
+ * t2 += tau). */
 int orc_pulse_fid(int S, const int *isotopes, int ncsa, const double *csa, int ndip,
                   const double *dip, int nj, const double *jc, double wr, double rotor_deg,
-                  int nsteps, const double *step_dt, int npts, double t0, int nrf,
+                  int nsteps, const double *step_t1, const double *step_dt, int npts,
+                  double t0, int nrf,
                   const double *rf, int K, const double *theta, const double *phi,
                   const double *gamma, const double *w, const double *rho, const double *det,
                   int normalize, double *fid_out) {
@@ -797,7 +799,8 @@ int orc_pulse_fid(int S, const int *isotopes, int ncsa, const double *csa, int n
         rot3(phi[c], theta[c], gamma[c], cp);
         for (size_t q = 0; q < NN; q++) U[q] = 0;
         for (int i = 0; i < N; i++) U[i + (size_t)i * N] = 1.0;
-        double t1 = t0, t2 = t0 + (step_dt ? step_dt[0] : tau);
+        double t1 = (step_dt && step_t1) ? step_t1[0] : t0;
+        double t2 = t1 + (step_dt ? step_dt[0] : tau);
         for (int k = 0; k < nsteps; k++) {
             double dt = step_dt ? step_dt[k] : tau;
             double tmid = (t2 - t1) / 2.0 + t1;
@@ -808,8 +811,10 @@ int orc_pulse_fid(int S, const int *isotopes, int ncsa, const double *csa, int n
             zgemm(N, 'N', 'N', X, U, T);
             memcpy(U, T, sizeof(cplx) * NN);
             if (step_dt) {
-                t1 = t2;
-                if (k + 1 < nsteps) t2 = t1 + step_dt[k + 1];
+                if (k + 1 < nsteps) {
+                    t1 = step_t1 ? step_t1[k + 1] : t2;
+                    t2 = t1 + step_dt[k + 1];
+                }
             } else {
                 t1 += tau;
                 t2 += tau;
@@ -846,7 +851,8 @@ int orc_ptop_fid(int S, const int *isotopes, int ncsa, const double *csa, int nd
                  const double *theta, const double *phi, const double *gamma, const double *w,
                  const double *rho, const double *det, int normalize, double *fid_out) {
     return orc_pulse_fid(S, isotopes, ncsa, csa, ndip, dip, nj, jc, wr, rotor_deg, nsteps, NULL,
-                         npts, t0, nrf, rf, K, theta, phi, gamma, w, rho, det, normalize, fid_out);
+                         NULL, npts, t0, nrf, rf, K, theta, phi, gamma, w, rho, det, normalize,
+                         fid_out);
 }
 
 /* Test hooks (exported): the pieces above on their own. */

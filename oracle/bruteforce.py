@@ -164,13 +164,14 @@ def rf_hamiltonian(sy, rf_k):
 
 
 def ptop_equivalent(sy, theta, phi, gamma, rotor_deg, wr, nsteps, npts, rho, det, rf=(), t0=0.0,
-                    step_dt=None):
+                    step_dt=None, step_t1=None):
     """One crystallite: U = prod_k exp(+i 2 pi tau_k (H_k + H_rf,k)) (propogation.cc:240-304,
     note the + sign), fid[m] = Tr(rho_m det), rho_{m+1} = U rho_m U^H (sequencerun.cc:164-196).
     Steps are one rotor period of equal length, or consecutive steps of lengths step_dt."""
     dts = (np.full(nsteps, 1.0 / wr / nsteps) if step_dt is None
            else np.broadcast_to(np.asarray(step_dt, float), (nsteps,)))
-    starts = t0 + np.concatenate([[0.0], np.cumsum(dts)[:-1]])
+    starts = (t0 + np.concatenate([[0.0], np.cumsum(dts)[:-1]]) if step_t1 is None
+              else np.broadcast_to(np.asarray(step_t1, float), (nsteps,)))
     U = np.eye(sy.N, dtype=np.complex128)
     for k in range(nsteps):
         tau = dts[k]

@@ -43,7 +43,7 @@ def lib():
         L.orc_pulse_fid.restype = ctypes.c_int
         L.orc_pulse_fid.argtypes = [ctypes.c_int, _i, ctypes.c_int, _d, ctypes.c_int, _d,
                                     ctypes.c_int, _d, ctypes.c_double, ctypes.c_double,
-                                    ctypes.c_int, _d, ctypes.c_int, ctypes.c_double,
+                                    ctypes.c_int, _d, _d, ctypes.c_int, ctypes.c_double,
                                     ctypes.c_int, _d, ctypes.c_int, _d, _d, _d, _d, _d, _d,
                                     ctypes.c_int, _d]
         L.orc_hamiltonian.argtypes = [ctypes.c_int, _i, ctypes.c_int, _d, ctypes.c_int, _d,
@@ -152,7 +152,7 @@ def rf_table(sy, rf, nsteps):
 
 
 def ptop_fid(sy, wr, rotor_deg, nsteps, npts, powder, rho, det, rf=(), t0=0.0, normalize=True,
-             step_dt=None):
+             step_dt=None, step_t1=None):
     """Point-to-point FID with rf: one application of the sub-sequence per point.  The
     sub-sequence is one rotor period of nsteps equal steps, or the consecutive steps of lengths
     step_dt."""
@@ -169,8 +169,12 @@ def ptop_fid(sy, wr, rotor_deg, nsteps, npts, powder, rho, det, rf=(), t0=0.0, n
     ga = np.ascontiguousarray(powder.gamma, np.float64)
     w = np.ascontiguousarray(powder.weight, np.float64)
     dptr = _p(dts) if step_dt is not None else None
+    if step_t1 is not None:
+        t1s = np.ascontiguousarray(np.broadcast_to(np.asarray(step_t1, np.float64), (nsteps,)))
+    tptr = _p(t1s) if step_t1 is not None else None
     st = L.orc_pulse_fid(sy.spins, _p(iso, _i), nc, _p(csa), nd, _p(dip), nj, _p(jc), float(wr),
-                         float(rotor_deg), int(nsteps), dptr, int(npts), float(t0), nrf, _p(tab),
+                         float(rotor_deg), int(nsteps), tptr, dptr, int(npts), float(t0), nrf,
+                         _p(tab),
                          len(th), _p(th), _p(ph), _p(ga), _p(w), _p(r), _p(d),
                          int(bool(normalize)), _p(out))
     if st != 0:

@@ -303,3 +303,22 @@ def test_unequal_steps_match_explicit_numpy(oracle, wr):
         f2 = B.ptop_equivalent(sy, th, ph_, ga, MAGIC if wr else 0.0, wr, n, 10, rho, det, rf=rf,
                                t0=3e-6, step_dt=dts)
         assert np.abs(f1 - f2).max() < 1e-11
+
+
+def test_explicit_step_start_times_match_explicit_numpy(oracle):
+    """Two sub-sequences each timed from 0 (non-consecutive step start times)."""
+    sy = SolidSys.from_text(HET)
+    dts = np.array([3e-6, 3e-6, 5e-6, 2e-6, 2e-6, 2e-6])
+    t1s = np.array([0.0, 3e-6, 6e-6, 0.0, 2e-6, 4e-6])
+    n = len(dts)
+    rf = [("1H", np.r_[np.full(3, 40000.0), np.full(3, 70000.0)], np.r_[np.zeros(3), np.full(3, 0.5)],
+           np.zeros(n))]
+    rho, det = sy.op("Ix_0"), sy.op("Ip_0")
+    f1 = oracle.ptop_fid(sy, 9000.0, MAGIC, n, 12, Powder.single(0.9, 2.1, 0.4), rho, det, rf=rf,
+                         normalize=False, step_dt=dts, step_t1=t1s)
+    f2 = B.ptop_equivalent(sy, 0.9, 2.1, 0.4, MAGIC, 9000.0, n, 12, rho, det, rf=rf, step_dt=dts,
+                           step_t1=t1s)
+    f3 = oracle.ptop_fid(sy, 9000.0, MAGIC, n, 12, Powder.single(0.9, 2.1, 0.4), rho, det, rf=rf,
+                         normalize=False, step_dt=dts)
+    assert np.abs(f1 - f2).max() < 1e-11
+    assert np.abs(f1 - f3).max() > 1e-6   # the start times matter under spinning
