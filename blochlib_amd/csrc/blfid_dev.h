@@ -42,4 +42,5 @@ struct DevPlan {
     const int *spin_rf;    // S: pulsed-isotope slot of each spin or -1
     const double *rf;      // n x nrfiso x 3: amp/2, phase, offset of each step
     const double *thetak;  // n: 2 pi tau_k of steps of unequal length, or NULL (all theta)
+    int rt_from_q;         // 1: R(s,r) = conj(Q(r,s)), R is neither formed nor stored
 };

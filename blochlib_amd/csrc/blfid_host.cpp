@@ -198,9 +198,11 @@ struct blfid_plan {
     std::vector<int> pair_doff, pair_roff, d_rowptr, d_col, r_rowptr, r_col, elem_r, elem_s;
     std::vector<double> d_val, r_val;
     long Etot = 0;
+    bool rt_from_q = false;    // R = Q^H elementwise (rosym with det_JI = 0 on every pair)
     std::map<int, std::vector<int>> pair_groups; // M -> pairs
     // powder
     int K = 0;
+    bool rp_dirty = false;      // cartPowder of the new powder still to be formed (in run)
     // device
     DevBuf<double> d_terms, d_dconst, d_ffconst, d_u3, d_RP, d_th, d_ph, d_ga, d_w;
     DevBuf<int> d_blk_dim, d_blk_soff, d_blk_states, d_pair_I, d_pair_J, d_pair_doff,
@@ -506,6 +508,7 @@ static void refresh_devplan(blfid_plan *pl) {
     d.npairs = (int)pl->pairs.size();
     d.Lsum = pl->Lsum;
     d.Etot = pl->Etot;
+    d.rt_from_q = pl->rt_from_q ? 1 : 0;
     d.theta = PI2 * pl->tau;
     d.terms = pl->d_terms.p;
     d.dconst = pl->d_dconst.p;
@@ -876,6 +879,15 @@ blfid_status blfid_plan_set_operators(blfid_plan *pl, const double *rho_d, const
         pl->pair_groups[M].push_back((int)p);
     }
     pl->Etot = eo;
+    // rosym spinning case: R_(I,J) = B_J^H (det_JI + det_IJ^H) B_I = Q_(J,I) + Q_(I,J)^H; when no
+    // pair has det_JI != 0 (det raises or lowers the block, e.g. I+), R is Q^H elementwise and
+    // neither its products nor its storage are needed
+    pl->rt_from_q = false;
+    if (pl->spinning && rosym) {
+        bool any_rev = false;
+        for (auto &pp : pl->pairs) any_rev |= dnz[pp.J * nb + pp.I] != 0;
+        pl->rt_from_q = !any_rev;
+    }
     std::vector<int> pI, pJ;
     for (auto &pp : pl->pairs) {
         pI.push_back(pp.I);
@@ -913,8 +925,9 @@ blfid_status blfid_plan_set_powder(blfid_plan *pl, const double *theta, const do
     HIPCHK(pl->d_ga.upload(ga));
     HIPCHK(pl->d_w.upload(w));
     HIPCHK(pl->d_RP.alloc((size_t)9 * K));
-    LAUNCHCHK(blfid_dev_launch_prep(K, pl->d_th.p, pl->d_ph.p, pl->d_ga.p, pl->d_RP.p, nullptr));
-    HIPCHK(hipDeviceSynchronize());
+    // the rotations are formed by the next run, on its stream, so that setting a powder never
+    // waits for work of other plans
+    pl->rp_dirty = true;
     refresh_devplan(pl);
     return BLFID_OK;
 }
@@ -928,7 +941,7 @@ static double pair_flops(const blfid_plan *pl, const std::vector<int> &g) {
     double s = 0;
     for (int p : g) {
         double bI = pl->blk_dim[pl->pairs[p].I], bJ = pl->blk_dim[pl->pairs[p].J];
-        s += 8.0 * (bI * bI * bJ + bJ * bJ * bI);
+        s += 8.0 * (bI * bI * bJ + (pl->rt_from_q ? 0.0 : bJ * bJ * bI));
     }
     return s;
 }
@@ -957,7 +970,7 @@ blfid_status blfid_plan_stats(const blfid_plan *pl, double *out) {
     for (auto &pp : pl->pairs) {
         double bI = pl->blk_dim[pp.I], bJ = pl->blk_dim[pp.J];
         pairs += bI * bJ;
-        pairsQ += bI * bJ * (bI + bJ);
+        pairsQ += bI * bJ * (bI + (pl->rt_from_q ? 0.0 : bJ));
     }
     double ex = 0;
     if (pl->ptop) {
@@ -1001,7 +1014,8 @@ static size_t layout_bytes(const blfid_plan *pl, int Kb, size_t off[11]) {
         sz[0] = (size_t)Kb * pl->n * L * cs;                               // U / B
         sz[1] = (size_t)Kb * pl->n * L * cs;                               // A
         sz[2] = std::max<size_t>(1, (size_t)Kb * pl->n * pl->Etot) * cs;   // Qt
-        sz[3] = std::max<size_t>(1, (size_t)Kb * pl->n * pl->Etot) * cs;   // Rt
+        sz[3] = pl->rt_from_q ? 0                                           // Rt
+                              : std::max<size_t>(1, (size_t)Kb * pl->n * pl->Etot) * cs;
         sz[4] = (size_t)pl->exp_slots * 9 * bm2 * rs;                     // exp scratch
         if (pl->prec) sz[10] = (size_t)Kb * L * cs;                        // W in storage precision
     } else {
@@ -1080,6 +1094,10 @@ blfid_status blfid_plan_run(blfid_plan *pl, int normalize, void *stream) {
     HIPCHK(pl->fid_out.alloc((size_t)2 * pl->npts));
     refresh_devplan(pl);
     const DevPlan *P = &pl->dp;
+    if (pl->rp_dirty && K > 0) {
+        LAUNCHCHK(blfid_dev_launch_prep(K, pl->d_th.p, pl->d_ph.p, pl->d_ga.p, pl->d_RP.p, st));
+        pl->rp_dirty = false;
+    }
     LAUNCHCHK(blfid_dev_launch_zero(pl->acc.p, 2 * pl->npts, st));
     LAUNCHCHK(blfid_dev_launch_zero(pl->comp.p, 2 * pl->npts, st));
     const size_t bm2 = (size_t)pl->bmax * pl->bmax;

@@ -572,6 +572,10 @@ __global__ void __launch_bounds__(256) k_eig(DevPlan P, int Kb, const int *__res
                                              double *__restrict__ outv, cd *__restrict__ gscr,
                                              size_t gslot) {
     typedef typename CX<R>::T CT;
+    // convergence: rotations below tol_rot end the sweeps; pairs below tol_off are skipped.  A
+    // complex64 period propagator carries ~1e-7 relative error, so its solve stops earlier.
+    constexpr double tol_rot = __is_same(R, double) ? 1e-15 : 1e-10;
+    constexpr double tol_off = __is_same(R, double) ? 1e-17 : 1e-12;
     __shared__ cd Mlds[BIG ? 1 : EIG_MAXB * (EIG_MAXB + 1)];
     __shared__ double rc[EIG_MAXPAIR];
     __shared__ cd rs[EIG_MAXPAIR];
@@ -615,7 +619,7 @@ __global__ void __launch_bounds__(256) k_eig(DevPlan P, int Kb, const int *__res
                         cd h2pq = cmk(0.5 * d2.y, -0.5 * d2.x); // d2 / (2i)
                         double off = fabs(h1pq.x) + fabs(h1pq.y) + fabs(h2pq.x) + fabs(h2pq.y);
                         double scale = fabs(mpp.x) + fabs(mpp.y) + fabs(mqq.x) + fabs(mqq.y) + 1e-300;
-                        if (off > 1e-17 * scale) {
+                        if (off > tol_off * scale) {
                             double h1[3] = {mpp.x - mqq.x, 2.0 * h1pq.x, 2.0 * h1pq.y};
                             double h2[3] = {mpp.y - mqq.y, 2.0 * h2pq.x, 2.0 * h2pq.y};
                             double ka = h1[0] * h1[0] + h1[1] * h1[1] + h1[2] * h1[2];
@@ -637,7 +641,7 @@ __global__ void __launch_bounds__(256) k_eig(DevPlan P, int Kb, const int *__res
                                 cth = sqrt((ux + r) / (2.0 * r));
                                 double den = sqrt(2.0 * r * (ux + r));
                                 sth = cmk(uy / den, -uz / den);
-                                if (fabs(sth.x) + fabs(sth.y) > 1e-15) s_any = 1;
+                                if (fabs(sth.x) + fabs(sth.y) > tol_rot) s_any = 1;
                             }
                         }
                     }
@@ -758,6 +762,7 @@ __global__ void __launch_bounds__(NThr<M>::v) k_qr(DevPlan P, int Kb, const int 
         }
         __syncthreads();
         wg_gemm<M, NT, CT>(bI, bJ, bI, BI, bI, true, T1, bI, false, Qt + eo, bI, Arith<CT>::zero(), smem);
+        if (P.rt_from_q) continue; // R = Q^H elementwise, formed in k_corr
         {
             const int *rp = P.r_rowptr + P.pair_roff[pr];
             const int *ci = P.r_col;
@@ -848,7 +853,7 @@ __global__ void __launch_bounds__(256) k_corr(DevPlan P, int Kb,
                     if (e < P.Etot) {
                         size_t o = ((size_t)c * n + k) * P.Etot + e;
                         q = to_cd(Qt[o]);
-                        r = to_cd(Rt[o]);
+                        r = P.rt_from_q ? cconj(q) : to_cd(Rt[o]);
                     }
                     sQ[k * CH + el] = CX<R>::from(cmul(q, w));
                     sR[k * CH + el] = CX<R>::from(cmul(r, cconj(w)));

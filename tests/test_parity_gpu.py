@@ -173,3 +173,15 @@ def test_ten_spin_dense_shape_invariants():
     b = p.fetch()
     assert rel_l2(b, a) < 1e-13
     p.close()
+
+
+@pytest.mark.parametrize("det_name", ["Ix", "Im"])
+def test_rosym_detector_variants(det_name):
+    """rho = (det + det^H)/2 with a Hermitian detector (both block orders of det non-zero: R is
+    formed by its own products) and with a lowering detector (R = Q^H elementwise)."""
+    sy = workloads.small_cluster(5, 91)
+    det = sy.op(det_name)
+    rho = 0.5 * (det + det.conj().T)
+    pw = zcw(1154).subset(np.arange(0, 1154, 151))
+    compare(f"rosym-{det_name}", sy, 15000.0, MAGIC, 15000.0 * 16, 48, pw, rho, det,
+            normalize=False)
