@@ -42,5 +42,6 @@ struct DevPlan {
     const int *spin_rf;    // S: pulsed-isotope slot of each spin or -1
     const double *rf;      // n x nrfiso x 3: amp/2, phase, offset of each step
     const double *thetak;  // n: 2 pi tau_k of steps of unequal length, or NULL (all theta)
-    int rt_from_q;         // 1: R(s,r) = conj(Q(r,s)), R is neither formed nor stored
+    int rt_from_q;         // 1: R(s,r) = Q_rev + conj(Q(r,s)), R is neither formed nor stored
+    const int *elem_rev;   // Etot: element (s,r) of the reverse pair (J,I), or -1
 };

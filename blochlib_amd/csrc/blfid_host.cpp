@@ -198,7 +198,8 @@ struct blfid_plan {
     std::vector<int> pair_doff, pair_roff, d_rowptr, d_col, r_rowptr, r_col, elem_r, elem_s;
     std::vector<double> d_val, r_val;
     long Etot = 0;
-    bool rt_from_q = false;    // R = Q^H elementwise (rosym with det_JI = 0 on every pair)
+    bool rt_from_q = false;    // rosym spinning case: R formed from Q in k_corr
+    std::vector<int> elem_rev;  // per element (r,s) of pair (I,J): element (s,r) of (J,I) or -1
     std::map<int, std::vector<int>> pair_groups; // M -> pairs
     // powder
     int K = 0;
@@ -206,7 +207,7 @@ struct blfid_plan {
     // device
     DevBuf<double> d_terms, d_dconst, d_ffconst, d_u3, d_RP, d_th, d_ph, d_ga, d_w;
     DevBuf<int> d_blk_dim, d_blk_soff, d_blk_states, d_pair_I, d_pair_J, d_pair_doff,
-        d_pair_roff, d_drow, d_dcol, d_rrow, d_rcol, d_er, d_es;
+        d_pair_roff, d_drow, d_dcol, d_rrow, d_rcol, d_er, d_es, d_erev;
     DevBuf<long> d_blk_uoff, d_pair_eoff;
     DevBuf<double> d_dval, d_rval;
     std::map<int, DevBuf<int>> d_bgrp, d_pgrp;
@@ -509,6 +510,7 @@ static void refresh_devplan(blfid_plan *pl) {
     d.Lsum = pl->Lsum;
     d.Etot = pl->Etot;
     d.rt_from_q = pl->rt_from_q ? 1 : 0;
+    d.elem_rev = pl->d_erev.p;
     d.theta = PI2 * pl->tau;
     d.terms = pl->d_terms.p;
     d.dconst = pl->d_dconst.p;
@@ -879,14 +881,25 @@ blfid_status blfid_plan_set_operators(blfid_plan *pl, const double *rho_d, const
         pl->pair_groups[M].push_back((int)p);
     }
     pl->Etot = eo;
-    // rosym spinning case: R_(I,J) = B_J^H (det_JI + det_IJ^H) B_I = Q_(J,I) + Q_(I,J)^H; when no
-    // pair has det_JI != 0 (det raises or lowers the block, e.g. I+), R is Q^H elementwise and
-    // neither its products nor its storage are needed
-    pl->rt_from_q = false;
-    if (pl->spinning && rosym) {
-        bool any_rev = false;
-        for (auto &pp : pl->pairs) any_rev |= dnz[pp.J * nb + pp.I] != 0;
-        pl->rt_from_q = !any_rev;
+    // rosym spinning case: R_(I,J) = B_J^H (det_JI + det_IJ^H) B_I, i.e. R_(I,J)(s,r) =
+    // Q_(J,I)(s,r) + conj(Q_(I,J)(r,s)); the pair (J,I) exists exactly when det_JI != 0, so R
+    // needs neither its own products nor storage: k_corr forms it from Q through elem_rev
+    pl->rt_from_q = pl->spinning && rosym;
+    pl->elem_rev.clear();
+    if (pl->rt_from_q) {
+        std::map<std::pair<int, int>, int> pidx;
+        for (size_t p = 0; p < pl->pairs.size(); p++) pidx[{pl->pairs[p].I, pl->pairs[p].J}] = (int)p;
+        pl->elem_rev.assign(eo, -1);
+        for (size_t p = 0; p < pl->pairs.size(); p++) {
+            int I = pl->pairs[p].I, J = pl->pairs[p].J;
+            auto it = pidx.find({J, I});
+            if (it == pidx.end()) continue;
+            const int bI = pl->blk_dim[I], bJ = pl->blk_dim[J];
+            const long ro = pl->pair_eoff[it->second];
+            for (int s2 = 0; s2 < bJ; s2++)
+                for (int r = 0; r < bI; r++)
+                    pl->elem_rev[pl->pair_eoff[p] + r + (long)s2 * bI] = (int)(ro + s2 + (long)r * bJ);
+        }
     }
     std::vector<int> pI, pJ;
     for (auto &pp : pl->pairs) {
@@ -906,6 +919,7 @@ blfid_status blfid_plan_set_operators(blfid_plan *pl, const double *rho_d, const
     HIPCHK(pl->d_rval.upload(pl->r_val));
     HIPCHK(pl->d_er.upload(pl->elem_r));
     HIPCHK(pl->d_es.upload(pl->elem_s));
+    HIPCHK(pl->d_erev.upload(pl->elem_rev));
     for (auto &kv : pl->pair_groups) HIPCHK(pl->d_pgrp[kv.first].upload(kv.second));
     pl->have_ops = true;
     refresh_devplan(pl);
@@ -941,7 +955,7 @@ static double pair_flops(const blfid_plan *pl, const std::vector<int> &g) {
     double s = 0;
     for (int p : g) {
         double bI = pl->blk_dim[pl->pairs[p].I], bJ = pl->blk_dim[pl->pairs[p].J];
-        s += 8.0 * (bI * bI * bJ + (pl->rt_from_q ? 0.0 : bJ * bJ * bI));
+        s += 8.0 * (bI * bI * bJ + (pl->rt_from_q ? 0.0 : bJ * bJ * bI));  // Q (and R)
     }
     return s;
 }

@@ -762,7 +762,7 @@ __global__ void __launch_bounds__(NThr<M>::v) k_qr(DevPlan P, int Kb, const int 
         }
         __syncthreads();
         wg_gemm<M, NT, CT>(bI, bJ, bI, BI, bI, true, T1, bI, false, Qt + eo, bI, Arith<CT>::zero(), smem);
-        if (P.rt_from_q) continue; // R = Q^H elementwise, formed in k_corr
+        if (P.rt_from_q) continue; // R is formed from Q in k_corr
         {
             const int *rp = P.r_rowptr + P.pair_roff[pr];
             const int *ci = P.r_col;
@@ -853,7 +853,14 @@ __global__ void __launch_bounds__(256) k_corr(DevPlan P, int Kb,
                     if (e < P.Etot) {
                         size_t o = ((size_t)c * n + k) * P.Etot + e;
                         q = to_cd(Qt[o]);
-                        r = P.rt_from_q ? cconj(q) : to_cd(Rt[o]);
+                        if (P.rt_from_q) {
+                            // R(s,r) = Q_(J,I)(s,r) + conj(Q_(I,J)(r,s))
+                            r = cconj(q);
+                            const int rv = P.elem_rev[e];
+                            if (rv >= 0) r = cadd(r, to_cd(Qt[((size_t)c * n + k) * P.Etot + rv]));
+                        } else {
+                            r = to_cd(Rt[o]);
+                        }
                     }
                     sQ[k * CH + el] = CX<R>::from(cmul(q, w));
                     sR[k * CH + el] = CX<R>::from(cmul(r, cconj(w)));
