@@ -50,7 +50,27 @@ def hip():
     L.hipEventSynchronize.argtypes = [ctypes.c_void_p]
     L.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p,
                                       ctypes.c_void_p]
+    L.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    L.hipFree.argtypes = [ctypes.c_void_p]
+    L.hipMemset.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t]
+    L.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
     return L
+
+
+def copy_calibration(H, mib: int):
+    """One device-to-device copy of `mib` MiB (the runtime's copy kernel): under a counter
+    profile its FETCH_SIZE / WRITE_SIZE per dispatch calibrate the kernels' byte counts."""
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    n = mib << 20
+    H.hipMalloc(ctypes.byref(a), n)
+    H.hipMalloc(ctypes.byref(b), n)
+    H.hipMemset(a, 1, n)
+    H.hipDeviceSynchronize()
+    H.hipMemcpy(b, a, n, 3)  # hipMemcpyDeviceToDevice
+    H.hipDeviceSynchronize()
+    H.hipFree(a)
+    H.hipFree(b)
+    print(f"[bench] calibration copy of {mib} MiB done", flush=True)
 
 
 def cpu_baseline(wls, sample_mode: str):
@@ -210,6 +230,8 @@ def main():
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--workspace-gb", type=float, default=120.0)
     ap.add_argument("--workload", choices=["batched", "ptop", "serve"], default="batched")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="one device copy of MIB MiB before the warm-up (counter calibration)")
     ap.add_argument("--requests", type=int, default=48, help="requests of the serving mix")
     ap.add_argument("--max-spins", type=int, default=10, help="largest cluster of the serving mix")
     ap.add_argument("--sets", type=int, default=8, help="parameter sets per step (ptop)")
@@ -251,6 +273,8 @@ def main():
             if events is not None:
                 H.hipEventRecord(events[i], None)
 
+    if args.copy_calibration:
+        copy_calibration(H, args.copy_calibration)
     for i in range(args.warmup):
         step()
         H.hipDeviceSynchronize()

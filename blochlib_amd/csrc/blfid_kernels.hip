@@ -263,73 +263,104 @@ __global__ void k_prep(int K, const double *__restrict__ th, const double *__res
 // PulseData::Pulse (pulsedata.cc:110-125) with its phase taken out as the diagonal similarity
 // exp(-i phi Fz) (amp (cos phi Ix + sin phi Iy) = e^{-i phi Fz} amp Ix e^{i phi Fz}), which
 // k_exp applies to the propagator.
-template <typename R>
-__device__ void build_h_block(const DevPlan &P, const double *RPc, const double *u, int blk,
-                              R *H, int ld, double *s_zz, double *s_ff, double *s_v,
-                              const double *rfk) {
+// The block is written into H (leading dimension ld) and its infinity norm is returned to every
+// thread.  Diagonal entries are formed once per row (s_diag); a row's off-diagonal entries are
+// its flip-flop partners (bits i, j differ: s_ff[i][j]) and, with rf, its single flips of
+// pulsed spins, all inside the block, so the row sums need no pass over H.
+template <typename R, int NT>
+__device__ double build_h_block(const DevPlan &P, const double *RPc, const double *u, int blk,
+                                R *H, int ld, double *s_zz, double *s_ff, double *s_v,
+                                double *s_diag, double *s_red, const double *rfk) {
     const int tid = threadIdx.x;
+    const int S = P.S;
     if (tid == 0) {
         for (int a = 0; a < 3; a++)
             s_v[a] = RPc[0 * 3 + a] * u[0] + RPc[1 * 3 + a] * u[1] + RPc[2 * 3 + a] * u[2];
     }
     __syncthreads();
     const double vx = s_v[0], vy = s_v[1], vz = s_v[2];
-    for (int t = tid; t < P.nterms; t += blockDim.x) {
+    for (int t = tid; t < P.nterms; t += NT) {
         const double *c = P.terms + DEV_TERM_STRIDE * t;
         s_zz[t] = c[0] * vx * vx + c[1] * vy * vy + c[2] * vz * vz +
                   2.0 * (c[3] * vx * vy + c[4] * vx * vz + c[5] * vy * vz);
     }
-    for (int q = tid; q < P.S * P.S; q += blockDim.x) s_ff[q] = P.ffconst[q];
     __syncthreads();
-    if (tid == 0) {
+    // flip-flop coefficient of each spin pair: constant part + anisotropic terms in term order
+    for (int q = tid; q < S * S; q += NT) {
+        const int i = q / S, j = q % S;
+        double f = P.ffconst[q];
         for (int t = 0; t < P.nterms; t++) {
             const double *c = P.terms + DEV_TERM_STRIDE * t;
-            if (c[7] != 0.0) {
-                int i = (int)c[8], j = (int)c[9];
-                s_ff[i * P.S + j] += s_zz[t] * c[7];
-                s_ff[j * P.S + i] += s_zz[t] * c[7];
-            }
+            const int ti = (int)c[8], tj = (int)c[9];
+            if (c[7] != 0.0 && ((ti == i && tj == j) || (ti == j && tj == i))) f += s_zz[t] * c[7];
         }
+        s_ff[q] = f;
     }
-    __syncthreads();
     const int b = P.blk_dim[blk];
     const int *st = P.blk_states + P.blk_soff[blk];
-    for (int e = tid; e < b * b; e += blockDim.x) {
+    for (int r = tid; r < b; r += NT) {
+        const int ga = st[r];
+        double h = P.dconst[ga];
+        for (int t = 0; t < P.nterms; t++) {
+            const double *c = P.terms + DEV_TERM_STRIDE * t;
+            int i = (int)c[8], j = (int)c[9];
+            double mi = ((ga >> (S - 1 - i)) & 1) ? -0.5 : 0.5;
+            double f = c[6] * mi;
+            if (j >= 0) f *= ((ga >> (S - 1 - j)) & 1) ? -0.5 : 0.5;
+            h += s_zz[t] * f;
+        }
+        if (rfk) {
+            for (int i = 0; i < S; i++) {
+                int q = P.spin_rf[i];
+                if (q >= 0) h += rfk[3 * q + 2] * (((ga >> (S - 1 - i)) & 1) ? -0.5 : 0.5);
+            }
+        }
+        s_diag[r] = h;
+    }
+    __syncthreads();
+    double rmax = 0.0;
+    for (int r = tid; r < b; r += NT) {
+        const int ga = st[r];
+        double sm = fabs(s_diag[r]);
+        for (int i = 0; i < S; i++) {
+            const int bi = (ga >> (S - 1 - i)) & 1;
+            for (int j = i + 1; j < S; j++)
+                if (bi != ((ga >> (S - 1 - j)) & 1)) sm += fabs(s_ff[i * S + j]);
+            if (rfk) {
+                int q = P.spin_rf[i];
+                if (q >= 0) sm += fabs(rfk[3 * q]);
+            }
+        }
+        rmax = fmax(rmax, sm);
+    }
+    for (int e = tid; e < b * b; e += NT) {
         int ra = e % b, cb_ = e / b;
-        int ga = st[ra], gb = st[cb_];
         double h = 0.0;
         if (ra == cb_) {
-            h = P.dconst[ga];
-            for (int t = 0; t < P.nterms; t++) {
-                const double *c = P.terms + DEV_TERM_STRIDE * t;
-                int i = (int)c[8], j = (int)c[9];
-                double mi = ((ga >> (P.S - 1 - i)) & 1) ? -0.5 : 0.5;
-                double f = c[6] * mi;
-                if (j >= 0) f *= ((ga >> (P.S - 1 - j)) & 1) ? -0.5 : 0.5;
-                h += s_zz[t] * f;
-            }
-            if (rfk) {
-                for (int i = 0; i < P.S; i++) {
-                    int q = P.spin_rf[i];
-                    if (q >= 0) h += rfk[3 * q + 2] * (((ga >> (P.S - 1 - i)) & 1) ? -0.5 : 0.5);
-                }
-            }
+            h = s_diag[ra];
         } else {
+            int ga = st[ra], gb = st[cb_];
             int x = ga ^ gb;
             int pc = __popc(x);
             if (pc == 2 && (ga & x) != 0 && (ga & x) != x) {
                 int p1 = 31 - __clz(x);
                 int p0 = __ffs(x) - 1;
-                int i = P.S - 1 - p1, j = P.S - 1 - p0;
-                h = s_ff[i * P.S + j];
+                int i = S - 1 - p1, j = S - 1 - p0;
+                h = s_ff[i * S + j];
             } else if (pc == 1 && rfk) {
-                int q = P.spin_rf[P.S - __ffs(x)];
+                int q = P.spin_rf[S - __ffs(x)];
                 if (q >= 0) h = rfk[3 * q];
             }
         }
         H[ra + (size_t)cb_ * ld] = (R)h;
     }
+    // infinity norm: wavefront maxima, then across the workgroup's waves
+    for (int off = 32; off > 0; off >>= 1) rmax = fmax(rmax, __shfl_xor(rmax, off));
+    if ((tid & 63) == 0) s_red[tid >> 6] = rmax;
     __syncthreads();
+    double nrm = s_red[0];
+    for (int w = 1; w < NT / 64; w++) nrm = fmax(nrm, s_red[w]);
+    return nrm;
 }
 
 // ------------------------------------------------------------------------------ k_exp
@@ -364,7 +395,8 @@ __global__ void __launch_bounds__(NThr<M>::v) k_exp(DevPlan P, int Kb, int c0,
     __shared__ double s_zz[DEV_MAX_TERMS];
     __shared__ double s_ff[DEV_MAX_SPINS * DEV_MAX_SPINS];
     __shared__ double s_v[4];
-    __shared__ double s_red[NT];
+    __shared__ double s_red[NT / 64];
+    __shared__ double s_diag[DEV_MAX_BLOCK];
     const int tid = threadIdx.x;
     R *X = scratch + slot * 9 * blockIdx.x;
     R *Y = X + slot, *Y2 = Y + slot, *Y3 = Y2 + slot, *Y4 = Y3 + slot;
@@ -387,22 +419,9 @@ __global__ void __launch_bounds__(NThr<M>::v) k_exp(DevPlan P, int Kb, int c0,
         const int bb = b * b;
         const double theta = P.thetak ? P.thetak[k] : P.theta;
         const double *rfk = P.nrfiso > 0 ? P.rf + (size_t)3 * P.nrfiso * k : nullptr;
-        build_h_block(P, P.RP + 9 * (size_t)(c0 + c), P.u3 + 3 * k, blk, X, b, s_zz, s_ff, s_v,
-                      rfk);
-        double rmax = 0.0;
-        for (int r = tid; r < b; r += NT) {
-            double sm = 0.0;
-            for (int q = 0; q < b; q++) sm += fabs((double)X[r + (size_t)q * b]);
-            rmax = fmax(rmax, sm);
-        }
-        s_red[tid] = rmax;
-        __syncthreads();
-        for (int w = NT / 2; w > 0; w >>= 1) {
-            if (tid < w) s_red[tid] = fmax(s_red[tid], s_red[tid + w]);
-            __syncthreads();
-        }
-        const double nrm = s_red[0] * theta;
-        __syncthreads();
+        const double nrm =
+            build_h_block<R, NT>(P, P.RP + 9 * (size_t)(c0 + c), P.u3 + 3 * k, blk, X, b, s_zz,
+                                 s_ff, s_v, s_diag, s_red, rfk) * theta;
         int sq = 0;
         while (nrm / (double)(1 << sq) > ExpTol<R>::rmax && sq < 30) sq++;
         const double rx = nrm / (double)(1 << sq);
@@ -651,28 +670,37 @@ __global__ void __launch_bounds__(256) k_eig(DevPlan P, int Kb, const int *__res
                     rq[pi] = q;
                 }
                 __syncthreads();
-                // rows: M <- R^H M, R = [[c, -conj(s)], [s, c]] on (p, q)
-                for (int e = tid; e < npair * b; e += 256) {
-                    int pi = e % npair, col = e / npair;
-                    int p = rp[pi], q = rq[pi];
-                    if (q >= b) continue;
-                    double cth = rc[pi];
-                    cd sth = rs[pi];
-                    cd xp = Ms[p + col * ld], xq = Ms[q + col * ld];
-                    Ms[p + col * ld] = cadd(cscale(xp, cth), cmul(cconj(sth), xq));
-                    Ms[q + col * ld] = csub(cscale(xq, cth), cmul(sth, xp));
+                // M <- R^H M R and W <- W R in one pass, R = [[c, -conj(s)], [s, c]] on each
+                // pair (p, q): every 2 x 2 block (rows of pair i, columns of pair j) is rotated
+                // by both of its pairs' rotations; a dummy index (odd b) rotates by identity.
+                for (int e = tid; e < npair * npair; e += 256) {
+                    const int pi = e % npair, pj = e / npair;
+                    const int p = rp[pi], q = rq[pi], p2 = rp[pj], q2 = rq[pj];
+                    const bool vq = q < b, vq2 = q2 < b;
+                    const double ci = rc[pi], cj = rc[pj];
+                    const cd si = rs[pi], sj = rs[pj];
+                    const cd z = cmk(0, 0);
+                    cd a = Ms[p + p2 * ld];
+                    cd bb2 = vq2 ? Ms[p + q2 * ld] : z;
+                    cd c2 = vq ? Ms[q + p2 * ld] : z;
+                    cd d = (vq && vq2) ? Ms[q + q2 * ld] : z;
+                    // rows (pair i)
+                    cd a1 = cadd(cscale(a, ci), cmul(cconj(si), c2));
+                    cd c1 = csub(cscale(c2, ci), cmul(si, a));
+                    cd b1 = cadd(cscale(bb2, ci), cmul(cconj(si), d));
+                    cd d1 = csub(cscale(d, ci), cmul(si, bb2));
+                    // columns (pair j)
+                    Ms[p + p2 * ld] = cadd(cscale(a1, cj), cmul(sj, b1));
+                    if (vq2) Ms[p + q2 * ld] = csub(cscale(b1, cj), cmul(cconj(sj), a1));
+                    if (vq) Ms[q + p2 * ld] = cadd(cscale(c1, cj), cmul(sj, d1));
+                    if (vq && vq2) Ms[q + q2 * ld] = csub(cscale(d1, cj), cmul(cconj(sj), c1));
                 }
-                __syncthreads();
-                // columns: M <- M R, W <- W R
                 for (int e = tid; e < npair * b; e += 256) {
                     int pi = e % npair, row = e / npair;
                     int p = rp[pi], q = rq[pi];
                     if (q >= b) continue;
                     double cth = rc[pi];
                     cd sth = rs[pi];
-                    cd xp = Ms[row + p * ld], xq = Ms[row + q * ld];
-                    Ms[row + p * ld] = cadd(cscale(xp, cth), cmul(sth, xq));
-                    Ms[row + q * ld] = csub(cscale(xq, cth), cmul(cconj(sth), xp));
                     cd wp = W[row + (size_t)p * b], wq = W[row + (size_t)q * b];
                     W[row + (size_t)p * b] = cadd(cscale(wp, cth), cmul(sth, wq));
                     W[row + (size_t)q * b] = csub(cscale(wq, cth), cmul(cconj(sth), wp));
@@ -984,12 +1012,15 @@ __global__ void __launch_bounds__(256) k_static_h(DevPlan P, int Kb, int c0, con
     __shared__ double s_zz[DEV_MAX_TERMS];
     __shared__ double s_ff[DEV_MAX_SPINS * DEV_MAX_SPINS];
     __shared__ double s_v[4];
+    __shared__ double s_diag[DEV_MAX_BLOCK];
+    __shared__ double s_red[4];
     double *X = scratch + slot * blockIdx.x;
     const long ntask = (long)Kb * ngrp;
     for (long task = blockIdx.x; task < ntask; task += gridDim.x) {
         const int g = (int)(task % ngrp), c = (int)(task / ngrp);
         const int blk = grp[g], b = P.blk_dim[blk];
-        build_h_block(P, P.RP + 9 * (size_t)(c0 + c), P.u3, blk, X, b, s_zz, s_ff, s_v, nullptr);
+        build_h_block<double, 256>(P, P.RP + 9 * (size_t)(c0 + c), P.u3, blk, X, b, s_zz, s_ff, s_v,
+                                   s_diag, s_red, nullptr);
         cd *H = bufH + (size_t)c * P.Lsum + P.blk_uoff[blk];
         for (int e = threadIdx.x; e < b * b; e += 256) H[e] = cmk(X[e], 0.0);
         __syncthreads();
