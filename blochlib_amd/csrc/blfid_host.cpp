@@ -39,7 +39,7 @@ int blfid_dev_launch_dyson(int prec, int M, const DevPlan *P, int Kb, const int 
                            const void *bufU, void *bufA, hipStream_t st);
 int blfid_dev_launch_eig(int prec, const DevPlan *P, int Kb, const int *grp, int ngrp,
                          const void *src, size_t src_stride, int mode, double2 *bufW,
-                         void *bufWr, double *outv, int big, double2 *gscr, size_t gslot,
+                         void *bufWr, double *outv, int cls, double2 *gscr, size_t gslot,
                          int nslots, hipStream_t st);
 int blfid_dev_launch_bmul(int prec, int M, const DevPlan *P, int Kb, const int *grp, int ngrp,
                           const void *bufA, const void *bufW, void *bufB, hipStream_t st);
@@ -187,7 +187,7 @@ struct blfid_plan {
     long Lsum = 0;
     int bmax = 0;
     std::map<int, std::vector<int>> blk_groups; // M -> blocks
-    std::vector<int> eig_small, eig_big;        // blocks whose eigen-solve fits LDS / not
+    std::map<int, std::vector<int>> eig_cls;    // eigen-solver size class -> blocks
     int exp_slots = 0, qr_slots = 0, eig_slots = 0;
     // operators
     bool have_ops = false;
@@ -211,7 +211,8 @@ struct blfid_plan {
     DevBuf<long> d_blk_uoff, d_pair_eoff;
     DevBuf<double> d_dval, d_rval;
     std::map<int, DevBuf<int>> d_bgrp, d_pgrp;
-    DevBuf<int> d_eig_small, d_eig_big, d_spin_rf;
+    std::map<int, DevBuf<int>> d_eig_cls;
+    DevBuf<int> d_spin_rf;
     DevBuf<double> d_rf, d_thetak;
     // working storage (carved from the plan's own arena or a shared workspace)
     DevBuf<char> own_ws;
@@ -272,6 +273,17 @@ static void collect_profile(blfid_plan *pl) {
 }
 
 // ------------------------------------------------------------------------------ planning
+// Eigen-solver size class of a block: 0..4 keep the block in LDS sized for b <= 16, 32, 48, 64,
+// 80; 5 uses global scratch.
+static int eig_class(int b) {
+    if (b <= 16) return 0;
+    if (b <= 32) return 1;
+    if (b <= 48) return 2;
+    if (b <= 64) return 3;
+    if (b <= DEV_LDS_BLOCK) return 4;
+    return 5;
+}
+
 static blfid_status build_tables(blfid_plan *pl) {
     const int S = pl->S, N = pl->N;
     pl->terms.clear();
@@ -426,17 +438,16 @@ static blfid_status build_tables(blfid_plan *pl) {
         pl->blk_groups[M].push_back((int)b);
     }
     pl->Lsum = uo;
-    pl->eig_small.clear();
-    pl->eig_big.clear();
+    pl->eig_cls.clear();
     for (size_t b = 0; b < pl->blocks.size(); b++)
-        (pl->blk_dim[b] <= DEV_LDS_BLOCK ? pl->eig_small : pl->eig_big).push_back((int)b);
+        pl->eig_cls[eig_class(pl->blk_dim[b])].push_back((int)b);
     {
         // concurrent workgroup slots of scratch, sized so each scratch region stays ~2.5 GB
         const double bm2 = (double)pl->bmax * pl->bmax;
         auto clampi = [](double v, int lo, int hi) { return (int)std::max<double>(lo, std::min<double>(hi, v)); };
         pl->exp_slots = clampi(std::floor(2.5e9 / (9.0 * bm2 * 8.0)), 64, 8192);
         pl->qr_slots = clampi(std::floor(2.5e9 / (3.0 * bm2 * 16.0)), 64, 8192);
-        pl->eig_slots = pl->eig_big.empty() ? 0 : clampi(std::floor(1.0e9 / (bm2 * 16.0)), 64, 2048);
+        pl->eig_slots = pl->eig_cls.count(5) ? clampi(std::floor(1.0e9 / (bm2 * 16.0)), 64, 2048) : 0;
     }
     if (pl->bmax > DEV_MAX_BLOCK)
         return fail(BLFID_ERR_UNSUPPORTED,
@@ -486,8 +497,7 @@ static blfid_status upload_tables(blfid_plan *pl) {
     HIPCHK(pl->d_blk_states.upload(pl->blk_states));
     HIPCHK(pl->d_blk_uoff.upload(pl->blk_uoff));
     for (auto &kv : pl->blk_groups) HIPCHK(pl->d_bgrp[kv.first].upload(kv.second));
-    HIPCHK(pl->d_eig_small.upload(pl->eig_small));
-    HIPCHK(pl->d_eig_big.upload(pl->eig_big));
+    for (auto &kv : pl->eig_cls) HIPCHK(pl->d_eig_cls[kv.first].upload(kv.second));
     HIPCHK(pl->d_spin_rf.upload(pl->spin_rf));
     HIPCHK(pl->d_rf.upload(pl->rf));
     if (!pl->step_dt.empty()) {
@@ -1136,15 +1146,14 @@ blfid_status blfid_plan_run(blfid_plan *pl, int normalize, void *stream) {
                 LAUNCHCHK(blfid_dev_launch_dyson(pr, kv.first, P, kb, pl->d_bgrp[kv.first].p,
                                                  (int)kv.second.size(), pl->bufU, pl->bufA, st));
             }
-            for (int big = 0; big < 2; big++) {
-                const std::vector<int> &lst = big ? pl->eig_big : pl->eig_small;
-                if (lst.empty()) continue;
-                PhaseScope ps(pl, 2 * 8 + big, 0.0, st);
-                LAUNCHCHK(blfid_dev_launch_eig(pr, P, kb, big ? pl->d_eig_big.p : pl->d_eig_small.p,
-                                               (int)lst.size(),
+            for (auto &kv : pl->eig_cls) {
+                PhaseScope ps(pl, 2 * 8 + kv.first + 1, 0.0, st);
+                LAUNCHCHK(blfid_dev_launch_eig(pr, P, kb, pl->d_eig_cls[kv.first].p,
+                                               (int)kv.second.size(),
                                                pl->bufA + (size_t)((pl->n - 1) % 2) * pl->Lsum * cs,
                                                (size_t)2 * pl->Lsum, 0, pl->bufW, nullptr,
-                                               pl->thetas, big, pl->scrEig, bm2, pl->eig_slots, st));
+                                               pl->thetas, kv.first, pl->scrEig, bm2,
+                                               pl->eig_slots, st));
             }
             for (auto &kv : pl->pair_groups) {
                 PhaseScope ps(pl, 7 * 8 + kv.first, kb * (pair_flops(pl, kv.second) + 8.0 * pl->npts * pair_elems(pl, kv.second)), st);
@@ -1165,15 +1174,14 @@ blfid_status blfid_plan_run(blfid_plan *pl, int normalize, void *stream) {
                 LAUNCHCHK(blfid_dev_launch_dyson(pr, kv.first, P, kb, pl->d_bgrp[kv.first].p,
                                                  (int)kv.second.size(), pl->bufU, pl->bufA, st));
             }
-            for (int big = 0; big < 2; big++) {
-                const std::vector<int> &lst = big ? pl->eig_big : pl->eig_small;
-                if (lst.empty()) continue;
-                PhaseScope ps(pl, 2 * 8 + big, 0.0, st);
-                LAUNCHCHK(blfid_dev_launch_eig(pr, P, kb, big ? pl->d_eig_big.p : pl->d_eig_small.p,
-                                               (int)lst.size(),
+            for (auto &kv : pl->eig_cls) {
+                PhaseScope ps(pl, 2 * 8 + kv.first + 1, 0.0, st);
+                LAUNCHCHK(blfid_dev_launch_eig(pr, P, kb, pl->d_eig_cls[kv.first].p,
+                                               (int)kv.second.size(),
                                                pl->bufA + (size_t)(pl->n - 1) * pl->Lsum * cs,
                                                (size_t)pl->n * pl->Lsum, 0, pl->bufW, pl->bufWr,
-                                               pl->thetas, big, pl->scrEig, bm2, pl->eig_slots, st));
+                                               pl->thetas, kv.first, pl->scrEig, bm2,
+                                               pl->eig_slots, st));
             }
             for (auto &kv : pl->blk_groups) {
                 PhaseScope ps(pl, 3 * 8 + kv.first, kb * (double)pl->n * group_b3(pl, kv.second) * 8.0, st);
@@ -1198,14 +1206,11 @@ blfid_status blfid_plan_run(blfid_plan *pl, int normalize, void *stream) {
                 LAUNCHCHK(blfid_dev_launch_static_h(P, kb, c0, pl->d_bgrp[kv.first].p,
                                                     (int)kv.second.size(), (double *)pl->scrE, bm2,
                                                     pl->exp_slots, (double2 *)pl->bufU, st));
-            for (int big = 0; big < 2; big++) {
-                const std::vector<int> &lst = big ? pl->eig_big : pl->eig_small;
-                if (lst.empty()) continue;
-                LAUNCHCHK(blfid_dev_launch_eig(0, P, kb, big ? pl->d_eig_big.p : pl->d_eig_small.p,
-                                               (int)lst.size(), pl->bufU, (size_t)pl->Lsum, 1,
-                                               pl->bufW, nullptr, pl->thetas, big, pl->scrEig, bm2,
-                                               pl->eig_slots, st));
-            }
+            for (auto &kv : pl->eig_cls)
+                LAUNCHCHK(blfid_dev_launch_eig(0, P, kb, pl->d_eig_cls[kv.first].p,
+                                               (int)kv.second.size(), pl->bufU, (size_t)pl->Lsum, 1,
+                                               pl->bufW, nullptr, pl->thetas, kv.first,
+                                               pl->scrEig, bm2, pl->eig_slots, st));
             for (auto &kv : pl->pair_groups) {
                 PhaseScope ps(pl, 7 * 8 + kv.first, 0.0, st);
                 LAUNCHCHK(blfid_dev_launch_static_fid(kv.first, P, kb, pl->d_pgrp[kv.first].p,

@@ -579,12 +579,13 @@ __global__ void __launch_bounds__(NThr<M>::v) k_dyson(DevPlan P, int Kb, const i
 // mode 1: Mx = H block (Hermitian, static path) -> out value = Re(Mx_rr)
 #define EIG_MAXB 80
 #define EIG_MAXPAIR 128 /* blocks up to 256 */
-// BIG = false: the block lives in LDS (b <= EIG_MAXB); BIG = true: in a global scratch
-// slot of b*b complex per workgroup (any b <= 2*EIG_MAXPAIR), same algorithm.
+// MB > 0: the block lives in LDS (b <= MB), with LDS and threads sized for that class so that
+// workgroups of small blocks share a CU; MB == 0: in a global scratch slot of b*b complex per
+// workgroup (any b <= 2*EIG_MAXPAIR), same algorithm.
 // The source block may be stored in FP32 (complex64 plans); the solver always runs in FP64 and
 // W stays FP64, with a storage-precision copy Wr for the B = A W products when R != double.
-template <bool BIG, typename R>
-__global__ void __launch_bounds__(256) k_eig(DevPlan P, int Kb, const int *__restrict__ grp,
+template <int MB, int NT, typename R>
+__global__ void __launch_bounds__(NT) k_eig(DevPlan P, int Kb, const int *__restrict__ grp,
                                              int ngrp, const typename CX<R>::T *__restrict__ src,
                                              size_t src_stride, int mode, cd *__restrict__ bufW,
                                              typename CX<R>::T *__restrict__ bufWr,
@@ -595,10 +596,12 @@ __global__ void __launch_bounds__(256) k_eig(DevPlan P, int Kb, const int *__res
     // complex64 period propagator carries ~1e-7 relative error, so its solve stops earlier.
     constexpr double tol_rot = __is_same(R, double) ? 1e-15 : 1e-10;
     constexpr double tol_off = __is_same(R, double) ? 1e-17 : 1e-12;
-    __shared__ cd Mlds[BIG ? 1 : EIG_MAXB * (EIG_MAXB + 1)];
-    __shared__ double rc[EIG_MAXPAIR];
-    __shared__ cd rs[EIG_MAXPAIR];
-    __shared__ int rp[EIG_MAXPAIR], rq[EIG_MAXPAIR];
+    constexpr bool BIG = MB == 0;
+    constexpr int MP = BIG ? EIG_MAXPAIR : (MB + 1) / 2;
+    __shared__ cd Mlds[BIG ? 1 : MB * (MB + 1)];
+    __shared__ double rc[MP];
+    __shared__ cd rs[MP];
+    __shared__ int rp[MP], rq[MP];
     __shared__ int s_any;
     const int tid = threadIdx.x;
     cd *Ms = BIG ? gscr + gslot * blockIdx.x : Mlds;
@@ -609,7 +612,7 @@ __global__ void __launch_bounds__(256) k_eig(DevPlan P, int Kb, const int *__res
         const int ld = BIG ? b : b + 1;
         const CT *S0 = src + (size_t)c * src_stride + P.blk_uoff[blk];
         cd *W = bufW + (size_t)c * P.Lsum + P.blk_uoff[blk];
-        for (int e = tid; e < b * b; e += 256) {
+        for (int e = tid; e < b * b; e += NT) {
             int r = e % b, q = e / b;
             Ms[r + q * ld] = to_cd(S0[e]);
             W[e] = cmk(r == q ? 1.0 : 0.0, 0.0);
@@ -621,7 +624,7 @@ __global__ void __launch_bounds__(256) k_eig(DevPlan P, int Kb, const int *__res
             if (tid == 0) s_any = 0;
             __syncthreads();
             for (int round = 0; round < bb - 1; round++) {
-                for (int pi = tid; pi < npair; pi += 256) {
+                for (int pi = tid; pi < npair; pi += NT) {
                     // circle method: position 0 fixed, positions 1..bb-1 rotate
                     int pa = pi, pb = bb - 1 - pi;
                     int ia = (pa == 0) ? 0 : ((pa - 1 + round) % (bb - 1)) + 1;
@@ -673,17 +676,16 @@ __global__ void __launch_bounds__(256) k_eig(DevPlan P, int Kb, const int *__res
                 // M <- R^H M R and W <- W R in one pass, R = [[c, -conj(s)], [s, c]] on each
                 // pair (p, q): every 2 x 2 block (rows of pair i, columns of pair j) is rotated
                 // by both of its pairs' rotations; a dummy index (odd b) rotates by identity.
-                for (int e = tid; e < npair * npair; e += 256) {
+                for (int e = tid; e < npair * npair; e += NT) {
                     const int pi = e % npair, pj = e / npair;
                     const int p = rp[pi], q = rq[pi], p2 = rp[pj], q2 = rq[pj];
                     const bool vq = q < b, vq2 = q2 < b;
                     const double ci = rc[pi], cj = rc[pj];
                     const cd si = rs[pi], sj = rs[pj];
-                    const cd z = cmk(0, 0);
-                    cd a = Ms[p + p2 * ld];
-                    cd bb2 = vq2 ? Ms[p + q2 * ld] : z;
-                    cd c2 = vq ? Ms[q + p2 * ld] : z;
-                    cd d = (vq && vq2) ? Ms[q + q2 * ld] : z;
+                    cd a = Ms[p + p2 * ld], bb2 = cmk(0, 0), c2 = cmk(0, 0), d = cmk(0, 0);
+                    if (vq2) bb2 = Ms[p + q2 * ld];
+                    if (vq) c2 = Ms[q + p2 * ld];
+                    if (vq && vq2) d = Ms[q + q2 * ld];
                     // rows (pair i)
                     cd a1 = cadd(cscale(a, ci), cmul(cconj(si), c2));
                     cd c1 = csub(cscale(c2, ci), cmul(si, a));
@@ -695,7 +697,7 @@ __global__ void __launch_bounds__(256) k_eig(DevPlan P, int Kb, const int *__res
                     if (vq) Ms[q + p2 * ld] = cadd(cscale(c1, cj), cmul(sj, d1));
                     if (vq && vq2) Ms[q + q2 * ld] = csub(cscale(d1, cj), cmul(cconj(sj), c1));
                 }
-                for (int e = tid; e < npair * b; e += 256) {
+                for (int e = tid; e < npair * b; e += NT) {
                     int pi = e % npair, row = e / npair;
                     int p = rp[pi], q = rq[pi];
                     if (q >= b) continue;
@@ -710,13 +712,13 @@ __global__ void __launch_bounds__(256) k_eig(DevPlan P, int Kb, const int *__res
             if (!s_any) break;
         }
         double *o = outv + (size_t)c * P.N + P.blk_soff[blk];
-        for (int r = tid; r < b; r += 256) {
+        for (int r = tid; r < b; r += NT) {
             cd l = Ms[r + r * ld];
             o[r] = (mode == 0) ? atan2(l.y, l.x) : l.x;
         }
         if (bufWr) {
             CT *Wr = bufWr + (size_t)c * P.Lsum + P.blk_uoff[blk];
-            for (int e = tid; e < b * b; e += 256) Wr[e] = CX<R>::from(W[e]);
+            for (int e = tid; e < b * b; e += NT) Wr[e] = CX<R>::from(W[e]);
         }
         __syncthreads();
     }
@@ -822,9 +824,15 @@ __global__ void __launch_bounds__(NThr<M>::v) k_qr(DevPlan P, int Kb, const int 
 // each thread walking a run of k with one sincos and complex multiplies.  Thread (el, pg)
 // then computes F_p(el) for p in [8 pg, 8 pg + 8):
 //   F_p(e) = sum_i Rt_i(e) Qt_{(i+p)%n}(e)
+// For n a power of two (n <= 2 CORR_TW) the correlation is formed with FFTs instead:
+//   F = IDFT( Rtilde . DFT(Qt) ),  Rtilde_k = sum_i Rt_i e^{+2 pi i ik/n},
+// both forward transforms decimation-in-frequency (natural order in, bit-reversed out), the
+// product taken in that order and the inverse decimation-in-time (bit-reversed in, natural
+// out), so no permutation pass is needed: 3 (n/2) log2 n butterflies instead of n^2 products.
 // F replaces Qt in LDS and thread t forms fid[t] = sum_e F_{t%n}(e) z_e^t (compute.h:432-457),
 // z_e^t advanced by z_e^256, added atomically to fid_c[c][t].
 // CH = min(CORR_SPAN / n, 256 / ceil(n/8)) is computed identically on the host.
+#define CORR_TW 256
 #define CORR_SPAN 2048
 #define CORR_PPT 8
 __device__ __host__ inline int corr_chunk(int n) {
@@ -846,8 +854,16 @@ __global__ void __launch_bounds__(256) k_corr(DevPlan P, int Kb,
     __shared__ CT sR[CORR_SPAN];
     __shared__ double sphi[256];
     __shared__ cd sstep[256];
+    __shared__ CT s_tw[CORR_TW];
     const int n = P.n, tid = threadIdx.x;
     const int CH = corr_chunk(n);
+    const bool use_fft = n >= 2 && (n & (n - 1)) == 0 && n <= 2 * CORR_TW;
+    if (use_fft) {
+        // e^{-2 pi i j / n}, j < n/2
+        for (int j = tid; j < n / 2; j += 256)
+            s_tw[j] = CX<R>::from(cexpi(-2.0 * 3.14159265358979323846 * (double)j / (double)n));
+        __syncthreads();
+    }
     const int npg = (n + CORR_PPT - 1) / CORR_PPT;
     const int nseg = 256 / CH;
     const int kpt = (n + nseg - 1) / nseg;
@@ -897,45 +913,82 @@ __global__ void __launch_bounds__(256) k_corr(DevPlan P, int Kb,
             }
         }
         __syncthreads();
-        CT F[CORR_PPT];
-        const int el = tid % CH, pg = tid / CH;
-        const int p0 = pg * CORR_PPT;
-        const bool active = pg < npg;
+        if (use_fft) {
+            const int nbf = (n >> 1) * CH;
+            // forward transforms of Qt (e^-) and Rt (e^+), decimation in frequency
+            for (int half = n >> 1, step = 1; half >= 1; half >>= 1, step <<= 1) {
+                for (int idx = tid; idx < 2 * nbf; idx += 256) {
+                    const bool isR = idx >= nbf;
+                    const int id = isR ? idx - nbf : idx;
+                    const int el = id % CH, bf = id / CH;
+                    const int j = bf % half, g = bf / half;
+                    const int i0 = (g * 2 * half + j) * CH + el, i1 = i0 + half * CH;
+                    CT *x = isR ? sR : sQ;
+                    CT w = s_tw[j * step];
+                    if (isR) w.y = -w.y;
+                    const CT a = x[i0], bq = x[i1];
+                    x[i0] = cadd(a, bq);
+                    x[i1] = cmul(csub(a, bq), w);
+                }
+                __syncthreads();
+            }
+            const R inv_n = (R)(1.0 / (double)n);
+            for (int idx = tid; idx < n * CH; idx += 256) sQ[idx] = cscale(cmul(sR[idx], sQ[idx]), inv_n);
+            __syncthreads();
+            // inverse transform (e^+), decimation in time
+            for (int half = 1, step = n >> 1; half < n; half <<= 1, step >>= 1) {
+                for (int idx = tid; idx < nbf; idx += 256) {
+                    const int el = idx % CH, bf = idx / CH;
+                    const int j = bf % half, g = bf / half;
+                    const int i0 = (g * 2 * half + j) * CH + el, i1 = i0 + half * CH;
+                    const CT w = cconj(s_tw[j * step]);
+                    const CT a = sQ[i0], t = cmul(sQ[i1], w);
+                    sQ[i0] = cadd(a, t);
+                    sQ[i1] = csub(a, t);
+                }
+                __syncthreads();
+            }
+        } else {
+            CT F[CORR_PPT];
+            const int el = tid % CH, pg = tid / CH;
+            const int p0 = pg * CORR_PPT;
+            const bool active = pg < npg;
 #pragma unroll
-        for (int q = 0; q < CORR_PPT; q++) F[q] = Arith<CT>::zero();
-        if (active) {
-            // window slot (i + q) % 8 holds Qt_{(i + p0 + q) % n}; unrolling i by 8 makes the
-            // rotation static, so the window never moves between registers
-            CT win[CORR_PPT];
+            for (int q = 0; q < CORR_PPT; q++) F[q] = Arith<CT>::zero();
+            if (active) {
+                // window slot (i + q) % 8 holds Qt_{(i + p0 + q) % n}; unrolling i by 8 makes the
+                // rotation static, so the window never moves between registers
+                CT win[CORR_PPT];
 #pragma unroll
-            for (int q = 0; q < CORR_PPT; q++) win[q] = sQ[((p0 + q) % n) * CH + el];
-            const int n8 = n & ~(CORR_PPT - 1);
-            for (int i = 0; i < n8; i += CORR_PPT) {
+                for (int q = 0; q < CORR_PPT; q++) win[q] = sQ[((p0 + q) % n) * CH + el];
+                const int n8 = n & ~(CORR_PPT - 1);
+                for (int i = 0; i < n8; i += CORR_PPT) {
 #pragma unroll
-                for (int u = 0; u < CORR_PPT; u++) {
-                    CT r = sR[(i + u) * CH + el];
+                    for (int u = 0; u < CORR_PPT; u++) {
+                        CT r = sR[(i + u) * CH + el];
 #pragma unroll
-                    for (int q = 0; q < CORR_PPT; q++) cfma(F[q], r, win[(u + q) % CORR_PPT]);
-                    win[u] = sQ[((i + u + p0 + CORR_PPT) % n) * CH + el];
+                        for (int q = 0; q < CORR_PPT; q++) cfma(F[q], r, win[(u + q) % CORR_PPT]);
+                        win[u] = sQ[((i + u + p0 + CORR_PPT) % n) * CH + el];
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < CORR_PPT; u++) { // tail when 8 does not divide n
+                    if (n8 + u < n) {
+                        CT r = sR[(n8 + u) * CH + el];
+#pragma unroll
+                        for (int q = 0; q < CORR_PPT; q++) cfma(F[q], r, win[(u + q) % CORR_PPT]);
+                        win[u] = sQ[((n8 + u + p0 + CORR_PPT) % n) * CH + el];
+                    }
                 }
             }
+            __syncthreads();
+            if (active) {
 #pragma unroll
-            for (int u = 0; u < CORR_PPT; u++) { // tail when 8 does not divide n
-                if (n8 + u < n) {
-                    CT r = sR[(n8 + u) * CH + el];
-#pragma unroll
-                    for (int q = 0; q < CORR_PPT; q++) cfma(F[q], r, win[(u + q) % CORR_PPT]);
-                    win[u] = sQ[((n8 + u + p0 + CORR_PPT) % n) * CH + el];
-                }
+                for (int q = 0; q < CORR_PPT; q++)
+                    if (p0 + q < n) sQ[(p0 + q) * CH + el] = F[q];
             }
+            __syncthreads();
         }
-        __syncthreads();
-        if (active) {
-#pragma unroll
-            for (int q = 0; q < CORR_PPT; q++)
-                if (p0 + q < n) sQ[(p0 + q) * CH + el] = F[q];
-        }
-        __syncthreads();
         for (int tb = 0; tb < P.npts; tb += 2048) {
             const int t0 = tb + tid;
             if (t0 < P.npts) {
@@ -1161,25 +1214,34 @@ extern "C" int blfid_dev_launch_dyson(int prec, int M, const DevPlan *P, int Kb,
 template <typename R>
 static void launch_eig(const DevPlan *P, int Kb, const int *grp, int ngrp, int grid,
                        const void *src, size_t src_stride, int mode, cd *bufW, void *bufWr,
-                       double *outv, int big, cd *gscr, size_t gslot, hipStream_t st) {
+                       double *outv, int cls, cd *gscr, size_t gslot, hipStream_t st) {
     typedef typename CX<R>::T CT;
     const CT *S0 = (const CT *)src;
     CT *Wr = (CT *)bufWr;
-    if (big) k_eig<true, R><<<grid, 256, 0, st>>>(*P, Kb, grp, ngrp, S0, src_stride, mode, bufW, Wr, outv, gscr, gslot);
-    else k_eig<false, R><<<grid, 256, 0, st>>>(*P, Kb, grp, ngrp, S0, src_stride, mode, bufW, Wr, outv, gscr, gslot);
+#define EIG_ARGS *P, Kb, grp, ngrp, S0, src_stride, mode, bufW, Wr, outv, gscr, gslot
+    switch (cls) {
+    case 0: k_eig<16, 64, R><<<grid, 64, 0, st>>>(EIG_ARGS); break;
+    case 1: k_eig<32, 128, R><<<grid, 128, 0, st>>>(EIG_ARGS); break;
+    case 2: k_eig<48, 256, R><<<grid, 256, 0, st>>>(EIG_ARGS); break;
+    case 3: k_eig<64, 256, R><<<grid, 256, 0, st>>>(EIG_ARGS); break;
+    case 4: k_eig<80, 256, R><<<grid, 256, 0, st>>>(EIG_ARGS); break;
+    default: k_eig<0, 256, R><<<grid, 256, 0, st>>>(EIG_ARGS); break;
+    }
+#undef EIG_ARGS
 }
 
-// bufWr: storage-precision copy of W (complex64 plans), or NULL
+// bufWr: storage-precision copy of W (complex64 plans), or NULL.  cls: block-size class
+// 0..4 (b <= 16, 32, 48, 64, 80, LDS-resident) or 5 (larger, global scratch slots).
 extern "C" int blfid_dev_launch_eig(int prec, const DevPlan *P, int Kb, const int *grp, int ngrp,
                                     const void *src, size_t src_stride, int mode, cd *bufW,
-                                    void *bufWr, double *outv, int big, cd *gscr, size_t gslot,
+                                    void *bufWr, double *outv, int cls, cd *gscr, size_t gslot,
                                     int nslots, hipStream_t st) {
     long ntask = (long)Kb * ngrp;
-    int cap = big ? nslots : GRID_CAP;
+    int cap = cls >= 5 ? nslots : GRID_CAP;
     int grid = (int)(ntask < cap ? ntask : cap);
     if (grid <= 0) return 0;
-    if (prec) launch_eig<float>(P, Kb, grp, ngrp, grid, src, src_stride, mode, bufW, bufWr, outv, big, gscr, gslot, st);
-    else launch_eig<double>(P, Kb, grp, ngrp, grid, src, src_stride, mode, bufW, bufWr, outv, big, gscr, gslot, st);
+    if (prec) launch_eig<float>(P, Kb, grp, ngrp, grid, src, src_stride, mode, bufW, bufWr, outv, cls, gscr, gslot, st);
+    else launch_eig<double>(P, Kb, grp, ngrp, grid, src, src_stride, mode, bufW, bufWr, outv, cls, gscr, gslot, st);
     return (int)hipGetLastError();
 }
 

@@ -282,7 +282,10 @@ class Plan:
         for slot in range(64):
             if n[slot]:
                 ph, M = divmod(slot, 8)
-                name = self.KERNELS[ph] + (f"<{M}>" if M else "")
+                if ph == 2:   # eigen-solver size classes
+                    name = "k_eig<" + ("16", "32", "48", "64", "80", "global")[M - 1] + ">"
+                else:
+                    name = self.KERNELS[ph] + (f"<{M}>" if M else "")
                 out[name] = {"ms": float(ms[slot]), "launches": int(n[slot]),
                              "flops": float(fl[slot])}
         return out

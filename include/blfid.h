@@ -163,7 +163,8 @@ blfid_status blfid_plan_set_workspace(blfid_plan *plan, blfid_workspace *ws);
  * (enable before blfid_plan_run; costs a few microseconds per launch).  Phases:
  *   0 exp (H build + propagator)   1 dyson   2 period eig   3 B = A W   4 Q/R similarity
  *   5 correlation + FID            6 powder reduce            7 static FID
- * Templated kernels come in tile variants M = 1..5 (16M x 16M output tiles).
+ * Templated kernels come in tile variants M = 1..5 (16M x 16M output tiles); for the period
+ * eigen-solver (phase 2) M - 1 is its block-size class (b <= 16, 32, 48, 64, 80, larger).
  * blfid_plan_phase_times waits for the last run and returns, per slot phase*8 + M (M = 0 for
  * untemplated kernels), the summed kernel time in ms, the number of launches and the executed
  * algorithmic FP64 flops, accumulated since profiling was enabled.  Arrays hold BLFID_NSLOT
