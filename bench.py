@@ -3,21 +3,27 @@
 Workload (the batched configuration the headline metric is quoted on): 64 seeded homonuclear
 1H clusters of 4-8 spins (Hilbert dimension 16-256), MAS 15 kHz at the magic angle, 128 steps
 per rotor period (sw = 1.92 MHz), ZCW 3722 orientations, 1024-point FID per cluster,
-rho = Ix, detect = I+ (gamma-COMPUTE path, FP64).
+rho = Ix, detect = I+ (gamma-COMPUTE path).
 
 One step = one pass of the engine over the whole workload: 64 powder FIDs, i.e.
 64 * 3722 * 128 = 30.5 M crystallite-steps.  Inputs (spin-system tables, operators, powder
 sets) are resident in HBM before the timed region; the timed region is launch-to-completion
-of every plan's run on one stream.
+of every plan's run (plans dealt round robin to --streams HIP streams, default 1).
 
-Prints one JSON line (rank 0): metric, value (crystallite-steps/s), FIDs/s, per-FID latency
-p50/p99 inside a step, accuracy against the FP64 oracle, the roofline of the dominant kernel
-(HIP events around its launches during the timed steps) and a CPU baseline (the oracle,
-i.e. the reference algorithm restated in C, on a bounded sample, one thread per sample).
+The default precision is c64 (complex64 propagators and products on matrix cores, FP64
+eigen-solver, phases, FID points and powder sum: solid-2.0's own form); the plain FP64 form is
+then timed on the same workload too, and every c64 FID is compared with its FP64 counterpart.
+
+Prints one JSON line: metric, value (crystallite-steps/s), FIDs/s, per-FID latency p50/p99
+inside a step, accuracy against the FP64 oracle, the roofline of the dominant kernel (HIP
+events around its launches during the timed steps; HBM traffic from the committed counter
+summary of this command) and a CPU baseline (the oracle, i.e. the reference algorithm
+restated in C, on a bounded sample, one thread per sample).
 
 --workload ptop measures the pulse-program path instead: 13C + 6 1H (N=128) under MAS
 12.5 kHz with 100 kHz phase-alternating 1H rf, 256 steps per period, ZCW 1154, 2048
 rotor-synchronised points, one FID per CSA parameter set (--sets of them per step).
+--workload serve runs the serving mix (Poisson arrivals admitted by device-memory footprint).
 """
 from __future__ import annotations
 
@@ -40,6 +46,54 @@ FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
 METRIC = "crystallite-steps/s + FIDs/s at ≤1e-5 rel L2 vs FP64 oracle; p50/p99 latency"
+# counter summaries of this benchmark's default command (FETCH_SIZE and WRITE_SIZE passes)
+COUNTER_SUMMARY = os.path.join(ROOT, "profiles", "config2_batched_{prec}_hbm_counters.txt")
+
+
+def corr_flops_per_elem(n: int, P: int) -> float:
+    """Executed flops of k_corr per correlation element and crystallite (as the engine counts
+    them): FFT correlation for n a power of two up to 512, else the direct sum, plus twiddle
+    staging and the FID recurrence."""
+    fft = n >= 2 and (n & (n - 1)) == 0 and n <= 512
+    c = 15.0 * n * np.log2(n) + 8.0 * n if fft else 8.0 * n * n
+    return c + 18.0 * n + 14.0 * P
+
+
+def profiled_traffic(kernel: str, precision: str):
+    """HBM bytes per launch of `kernel` from the committed counter summary of this workload:
+    FETCH_SIZE + WRITE_SIZE (KiB per dispatch, averaged over the kernel's dispatches; all
+    template variants whose name starts with `kernel`), each scaled by the known size of the
+    calibration copy over what the same counter read for it (the largest runtime copy
+    dispatch).  None when the summary is absent."""
+    path = COUNTER_SUMMARY.format(prec=precision)
+    if not os.path.exists(path):
+        return None
+    copy_bytes = None
+    vals = {}  # counter -> [dispatches, sum of means * dispatches]
+    cal = {}
+    for line in open(path):
+        if line.startswith("# calibration copy bytes:"):
+            copy_bytes = float(line.split(":")[1])
+            continue
+        f = line.split()
+        if len(f) < 5 or f[-4] not in ("FETCH_SIZE", "WRITE_SIZE"):
+            continue
+        name, cname = " ".join(f[:-4]), f[-4]
+        n, mean, mx = int(f[-3]), float(f[-2]), float(f[-1])
+        if name.startswith("__amd_rocclr_copyBuffer"):
+            cal[cname] = max(cal.get(cname, 0.0), mx)
+        elif name == kernel or name.startswith(kernel + "<"):
+            v = vals.setdefault(cname, [0, 0.0])
+            v[0] += n
+            v[1] += mean * n
+    if copy_bytes is None or set(vals) != {"FETCH_SIZE", "WRITE_SIZE"} or set(cal) != set(vals):
+        return None
+    factor = {c: copy_bytes / (cal[c] * 1024.0) for c in cal}
+    fetch = vals["FETCH_SIZE"][1] / vals["FETCH_SIZE"][0] * 1024.0 * factor["FETCH_SIZE"]
+    write = vals["WRITE_SIZE"][1] / vals["WRITE_SIZE"][0] * 1024.0 * factor["WRITE_SIZE"]
+    return {"bytes": fetch + write, "fetch_bytes": fetch, "write_bytes": write,
+            "fetch_factor": factor["FETCH_SIZE"], "write_factor": factor["WRITE_SIZE"],
+            "source": os.path.relpath(path, ROOT)}
 
 
 def hip():
@@ -54,6 +108,9 @@ def hip():
     L.hipFree.argtypes = [ctypes.c_void_p]
     L.hipMemset.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t]
     L.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    L.hipStreamCreateWithFlags.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_uint]
+    L.hipStreamDestroy.argtypes = [ctypes.c_void_p]
+    L.hipEventDestroy.argtypes = [ctypes.c_void_p]
     return L
 
 
@@ -220,6 +277,89 @@ def bench_serve(args):
     print(json.dumps(out), flush=True)
 
 
+def measure(H, wls, ptop, precision, steps, warmup, args, calibrate=False):
+    """Plans for every workload item resident on the device, `warmup` untimed passes, then
+    `steps` timed passes (launch to completion of every plan, HIP events per plan on its
+    stream, per-kernel HIP-event times from the engine).  Plans are dealt round robin to
+    `--streams` non-blocking HIP streams (default priority), each with its own workspace."""
+    nst = max(1, args.streams)
+    streams = []
+    for _ in range(nst):
+        st = ctypes.c_void_p()
+        H.hipStreamCreateWithFlags(ctypes.byref(st), 1)
+        streams.append(st)
+    wss = [engine.Workspace(args.workspace_gb * 1e9 / nst) for _ in range(nst)]
+    plans = []
+    t0 = time.perf_counter()
+    for w in wls:
+        if ptop:
+            p = engine.Plan.ptop(w.system, w.wr, w.rotor, w.steps, w.npts, w.rf,
+                                 precision=precision)
+        else:
+            p = engine.Plan(w.system, w.wr, w.rotor, w.sw, w.npts, precision=precision)
+        rho, det = w.operators()
+        p.set_operators(rho, det)
+        p.set_powder(w.powder())
+        p.set_workspace(wss[len(plans) % nst])
+        plans.append(p)
+    print(f"[bench] {precision}: {len(plans)} plans resident in "
+          f"{time.perf_counter() - t0:.1f}s", flush=True)
+
+    def step(events=None):
+        for i, p in enumerate(plans):
+            st = streams[i % nst]
+            if events is not None:
+                H.hipEventRecord(events[2 * i], st)
+            p.run(True, stream=st.value)
+            if events is not None:
+                H.hipEventRecord(events[2 * i + 1], st)
+
+    if calibrate and args.copy_calibration:
+        copy_calibration(H, args.copy_calibration)
+    for i in range(warmup):
+        step()
+        H.hipDeviceSynchronize()
+        print(f"[bench] {precision}: warmup {i + 1}/{warmup} done", flush=True)
+    for p in plans:
+        p.set_profiling(True)
+    evs = []
+    for _ in range(2 * len(plans)):
+        e = ctypes.c_void_p()
+        H.hipEventCreate(ctypes.byref(e))
+        evs.append(e)
+    lat = []
+    H.hipDeviceSynchronize()
+    t_start = time.perf_counter()
+    for s in range(steps):
+        step(evs)
+        H.hipDeviceSynchronize()
+        for i in range(len(plans)):
+            ms = ctypes.c_float()
+            H.hipEventElapsedTime(ctypes.byref(ms), evs[2 * i], evs[2 * i + 1])
+            lat.append(ms.value)
+        print(f"[bench] {precision}: step {s + 1}/{steps} done", flush=True)
+    H.hipDeviceSynchronize()
+    elapsed = time.perf_counter() - t_start
+    # per-kernel times over the timed steps
+    agg = {}
+    for p in plans:
+        for k, v in p.phase_times().items():
+            a = agg.setdefault(k, {"ms": 0.0, "launches": 0, "flops": 0.0})
+            a["ms"] += v["ms"]
+            a["launches"] += v["launches"]
+            a["flops"] += v["flops"]
+    fids = [p.fetch() for p in plans]
+    for p in plans:
+        p.close()
+    for ws in wss:
+        ws.close()
+    for e in evs:
+        H.hipEventDestroy(e)
+    for st in streams:
+        H.hipStreamDestroy(st)
+    return {"elapsed": elapsed, "lat": lat, "agg": agg, "fids": fids, "streams": nst}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -228,16 +368,21 @@ def main():
     ap.add_argument("--clusters", type=int, default=64)
     ap.add_argument("--cpu-sample", choices=["full", "quick", "none"], default="full")
     ap.add_argument("--no-accuracy", action="store_true")
-    ap.add_argument("--workspace-gb", type=float, default=120.0)
+    ap.add_argument("--workspace-gb", type=float, default=120.0,
+                    help="device working memory shared out over the streams (GB)")
+    ap.add_argument("--streams", type=int, default=1,
+                    help="HIP streams the plans are dealt to, round robin")
     ap.add_argument("--workload", choices=["batched", "ptop", "serve"], default="batched")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="one device copy of MIB MiB before the warm-up (counter calibration)")
     ap.add_argument("--requests", type=int, default=48, help="requests of the serving mix")
     ap.add_argument("--max-spins", type=int, default=10, help="largest cluster of the serving mix")
     ap.add_argument("--sets", type=int, default=8, help="parameter sets per step (ptop)")
-    ap.add_argument("--precision", choices=["fp64", "c64"], default="fp64",
-                    help="complex128 throughout, or complex64 propagators/products with FP64 "
-                         "eigen-solver, phases and reductions")
+    ap.add_argument("--precision", choices=["fp64", "c64"], default="c64",
+                    help="complex64 propagators/products with FP64 eigen-solver, phases and "
+                         "reductions (solid-2.0's form), or complex128 throughout")
+    ap.add_argument("--no-fp64-form", action="store_true",
+                    help="skip timing the FP64 form beside the c64 headline")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this engine runs on one GPU; multi-GPU runs are not part of this build "
@@ -250,65 +395,12 @@ def main():
     H = hip()
     ptop = args.workload == "ptop"
     wls = workloads.config4(args.sets) if ptop else workloads.config2(args.clusters)
-    ws = engine.Workspace(args.workspace_gb * 1e9)
-    plans = []
-    t0 = time.perf_counter()
-    for w in wls:
-        if ptop:
-            p = engine.Plan.ptop(w.system, w.wr, w.rotor, w.steps, w.npts, w.rf,
-                                 precision=args.precision)
-        else:
-            p = engine.Plan(w.system, w.wr, w.rotor, w.sw, w.npts, precision=args.precision)
-        rho, det = w.operators()
-        p.set_operators(rho, det)
-        p.set_powder(w.powder())
-        p.set_workspace(ws)
-        plans.append(p)
-    print(f"[bench] {len(plans)} plans resident in {time.perf_counter() - t0:.1f}s", flush=True)
     cs_per_step = sum(w.powder_size * w.steps for w in wls)
-
-    def step(events=None):
-        for i, p in enumerate(plans):
-            p.run(True)
-            if events is not None:
-                H.hipEventRecord(events[i], None)
-
-    if args.copy_calibration:
-        copy_calibration(H, args.copy_calibration)
-    for i in range(args.warmup):
-        step()
-        H.hipDeviceSynchronize()
-        print(f"[bench] warmup {i + 1}/{args.warmup} done", flush=True)
-    for p in plans:
-        p.set_profiling(True)
-    evs = []
-    for _ in range(len(plans) + 1):
-        e = ctypes.c_void_p()
-        H.hipEventCreate(ctypes.byref(e))
-        evs.append(e)
-    lat = []
-    H.hipDeviceSynchronize()
-    t_start = time.perf_counter()
-    for s in range(args.steps):
-        H.hipEventRecord(evs[0], None)
-        step(evs[1:])
-        H.hipEventSynchronize(evs[-1])
-        for i in range(len(plans)):
-            ms = ctypes.c_float()
-            H.hipEventElapsedTime(ctypes.byref(ms), evs[i], evs[i + 1])
-            lat.append(ms.value)
-        print(f"[bench] step {s + 1}/{args.steps} done", flush=True)
-    H.hipDeviceSynchronize()
-    elapsed = time.perf_counter() - t_start
-    # per-kernel times over the timed steps
-    agg = {}
-    for p in plans:
-        for k, v in p.phase_times().items():
-            a = agg.setdefault(k, {"ms": 0.0, "launches": 0, "flops": 0.0})
-            a["ms"] += v["ms"]
-            a["launches"] += v["launches"]
-            a["flops"] += v["flops"]
+    m = measure(H, wls, ptop, args.precision, args.steps, args.warmup, args, calibrate=True)
+    elapsed, lat, agg = m["elapsed"], m["lat"], m["agg"]
+    nst = m["streams"]
     dom = max(agg, key=lambda k: agg[k]["ms"])
+    traffic = None if ptop else profiled_traffic(dom, args.precision)
     d = agg[dom]
     avg_ms = d["ms"] / d["launches"]
     achieved = d["flops"] / d["launches"] / (avg_ms * 1e-3) / 1e12
@@ -331,8 +423,10 @@ def main():
             "workload": ("heteronuclear rf: 13C + 6 1H (N=128), MAS 12.5 kHz, magic angle, "
                          "256 steps/period, 100 kHz 0/180 phase-alternating 1H rf (8 cycles per "
                          "period), ZCW 1154, 2048 rotor-synchronised points, rho=Ix_0, "
-                         "detect=Ip_0, complex128, one FID per seeded CSA set"),
+                         "detect=Ip_0, one FID per seeded CSA set"),
+            "precision": args.precision,
             "sets": len(wls),
+            "streams": nst,
             "crystallites": wls[0].powder_size,
             "steps_per_period": wls[0].steps,
             "npts": wls[0].npts,
@@ -340,8 +434,10 @@ def main():
         } if ptop else {
             "workload": ("batched uneven sizes: 64 seeded 1H clusters of 4-8 spins (N=16-256), "
                          "MAS 15 kHz, magic angle, 128 steps/period, ZCW 3722, 1024-point FID, "
-                         "rho=Ix, detect=I+, complex128"),
+                         "rho=Ix, detect=I+"),
+            "precision": args.precision,
             "clusters": len(wls),
+            "streams": nst,
             "crystallites": wls[0].powder_size,
             "steps_per_period": wls[0].steps,
             "npts": wls[0].npts,
@@ -350,7 +446,8 @@ def main():
         "fids_per_s": len(wls) * args.steps / elapsed,
         "latency_ms_per_fid": {"p50": float(np.percentile(lat, 50)),
                                "p99": float(np.percentile(lat, 99)),
-                               "note": "per-plan (cluster or parameter set) powder FID service time inside a step"},
+                               "note": ("per-plan (cluster or parameter set) powder FID time on its stream "
+                                        "inside a step (plans on other streams run alongside)")},
         "kernels": {k: {"ms_total": v["ms"], "launches": v["launches"],
                         "tflops": v["flops"] / max(v["ms"], 1e-9) / 1e9}
                     for k, v in sorted(agg.items(), key=lambda kv: -kv[1]["ms"])},
@@ -361,13 +458,39 @@ def main():
             "peak": peak,
             "unit": "TFLOP/s",
             "frac": achieved / peak,
-            "traffic": None,
+            "traffic": traffic["bytes"] if traffic else None,
             "avg_launch_ms": avg_ms,
             "flops_per_launch": d["flops"] / d["launches"],
-            "note": ("executed algorithmic flops of the block-diagonal algorithm; peak = MI355X "
+            "note": ("achieved = executed algorithmic flops of the kernel (block-diagonal "
+                     "algorithm, unpadded sizes) per launch / its average launch time (HIP events "
+                     "on the launch stream); peak = MI355X "
                      + ("FP64" if args.precision == "fp64" else "FP32") + " dense"),
         },
     }
+    if traffic:
+        out["roofline"]["traffic_detail"] = traffic
+        out["roofline"]["achieved_hbm_gbs"] = traffic["bytes"] / (avg_ms * 1e-3) / 1e9
+    if dom == "k_corr":
+        n, P = wls[0].steps, wls[0].npts
+        ratio = (14.0 * n * n + 14.0 * P) / corr_flops_per_elem(n, P)
+        out["roofline"]["canonical_flops_per_launch"] = ratio * d["flops"] / d["launches"]
+        out["roofline"]["canonical_tflops"] = ratio * achieved
+        out["roofline"]["canonical_note"] = (
+            "reference algorithm's count for this kernel (14 n^2 + 14 P flops per correlation "
+            "element and crystallite: direct cyclic sum with a complex exp-multiply per term and "
+            "the FID recurrence); the kernel executes it with FFTs")
+    if args.precision == "c64" and not args.no_fp64_form:
+        # the plain FP64 form on the same workload (one timed step) and the c64 FIDs against it
+        m64 = measure(H, wls, ptop, "fp64", 1, 1, args)
+        errs = [float(np.linalg.norm(a - b) / np.linalg.norm(b))
+                for a, b in zip(m["fids"], m64["fids"])]
+        out["fp64_form"] = {
+            "value": cs_per_step / m64["elapsed"],
+            "ms_per_step": m64["elapsed"] * 1e3,
+            "rel_l2_c64_vs_fp64_gpu": {"max": max(errs), "median": float(np.median(errs)),
+                                       "fids": len(errs),
+                                       "note": "every FID of the step at full size"},
+        }
     if not args.no_accuracy:
         out["rel_l2_vs_oracle"] = (accuracy_ptop(wls, args.precision) if ptop
                                    else accuracy(wls, args.precision))

@@ -44,4 +44,5 @@ struct DevPlan {
     const double *thetak;  // n: 2 pi tau_k of steps of unequal length, or NULL (all theta)
     int rt_from_q;         // 1: R(s,r) = Q_rev + conj(Q(r,s)), R is neither formed nor stored
     const int *elem_rev;   // Etot: element (s,r) of the reverse pair (J,I), or -1
+    double *expflops;      // 8: executed propagator flops per tile variant (profiling), or NULL
 };

@@ -224,6 +224,7 @@ struct blfid_plan {
     double *thetas = nullptr, *fidc = nullptr;
     double2 *scrEig = nullptr;
     DevBuf<double> acc, comp, fid_out;
+    DevBuf<double> d_expflops; // propagator flop counters while profiling
     double budget = 0;
     int Kb = 0;
     hipStream_t last_stream = nullptr;
@@ -521,6 +522,7 @@ static void refresh_devplan(blfid_plan *pl) {
     d.Etot = pl->Etot;
     d.rt_from_q = pl->rt_from_q ? 1 : 0;
     d.elem_rev = pl->d_erev.p;
+    d.expflops = pl->prof ? pl->d_expflops.p : nullptr;
     d.theta = PI2 * pl->tau;
     d.terms = pl->d_terms.p;
     d.dconst = pl->d_dconst.p;
@@ -971,6 +973,17 @@ static double pair_flops(const blfid_plan *pl, const std::vector<int> &g) {
 }
 
 
+// Executed flops per correlation element and crystallite in k_corr: for n a power of two up to
+// 512, three radix-2 transforms of n/2 log2 n butterflies at 10 flops and the pointwise product
+// with the 1/n scale (8 per step); otherwise the direct cyclic sum, 8 n^2.  Plus the twiddle
+// staging (18 per step) and the FID recurrence (14 per point: complex multiply-add and the
+// z^t update).
+static double corr_flops_per_elem(int n, int P) {
+    const bool fft = n >= 2 && (n & (n - 1)) == 0 && n <= 512;
+    const double c = fft ? 15.0 * n * std::log2((double)n) + 8.0 * n : 8.0 * (double)n * n;
+    return c + 18.0 * n + 14.0 * P;
+}
+
 static double pair_elems(const blfid_plan *pl, const std::vector<int> &g) {
     double s = 0;
     for (int p : g) s += (double)pl->blk_dim[pl->pairs[p].I] * pl->blk_dim[pl->pairs[p].J];
@@ -998,17 +1011,16 @@ blfid_status blfid_plan_stats(const blfid_plan *pl, double *out) {
     }
     double ex = 0;
     if (pl->ptop) {
-        ex += 2.0 * 7.0 * s3;        // exp: 7 real GEMMs (no squaring counted)
+        ex += 2.0 * 7.0 * s3;        // exp: nominal degree-8 Taylor, 7 real GEMMs
         ex += 8.0 * s3;              // Dyson
         ex += 8.0 * pairsQ / n;      // D and rho into the period eigenbasis
         ex += 8.0 * P * pairs / n;   // eigenbasis FID (complex MAC per element and point)
     } else if (pl->spinning) {
-        ex += 2.0 * 7.0 * s3;   // exp: 7 real GEMMs (no squaring counted)
+        ex += 2.0 * 7.0 * s3;   // exp: nominal degree-8 Taylor, 7 real GEMMs
         ex += 8.0 * s3;         // Dyson
         ex += 8.0 * s3;         // B = A W
         ex += 8.0 * pairsQ;     // Q and R
-        ex += 8.0 * n * pairs;  // correlation
-        ex += 8.0 * P / n * pairs; // FID
+        ex += pairs * corr_flops_per_elem(pl->n, pl->npts) / n; // correlation + FID
     }
     out[0] = ex;
     if (pl->ptop) out[1] = 8.0 * N * N * N + 16.0 * N * N * N * P / n;
@@ -1136,7 +1148,7 @@ blfid_status blfid_plan_run(blfid_plan *pl, int normalize, void *stream) {
             // U_k = exp(+i theta (H_k + H_rf,k)); A_n = U_n ... U_1; its eigenbasis gives
             // rho_m = A_n^m rho A_n^-m in closed form
             for (auto &kv : pl->blk_groups) {
-                PhaseScope ps(pl, 0 * 8 + kv.first, kb * (double)pl->n * group_b3(pl, kv.second) * 14.0, st);
+                PhaseScope ps(pl, 0 * 8 + kv.first, 0.0, st); // flops counted by k_exp
                 LAUNCHCHK(blfid_dev_launch_exp(pr, kv.first, P, kb, c0, pl->d_bgrp[kv.first].p,
                                                (int)kv.second.size(), pl->scrE, bm2, pl->exp_slots,
                                                pl->bufU, st));
@@ -1164,7 +1176,7 @@ blfid_status blfid_plan_run(blfid_plan *pl, int normalize, void *stream) {
             }
         } else if (pl->spinning) {
             for (auto &kv : pl->blk_groups) {
-                PhaseScope ps(pl, 0 * 8 + kv.first, kb * (double)pl->n * group_b3(pl, kv.second) * 14.0, st);
+                PhaseScope ps(pl, 0 * 8 + kv.first, 0.0, st); // flops counted by k_exp
                 LAUNCHCHK(blfid_dev_launch_exp(pr, kv.first, P, kb, c0, pl->d_bgrp[kv.first].p,
                                                (int)kv.second.size(), pl->scrE, bm2, pl->exp_slots,
                                                pl->bufU, st));
@@ -1197,7 +1209,7 @@ blfid_status blfid_plan_run(blfid_plan *pl, int normalize, void *stream) {
                                               pl->qr_slots, pl->bufQt, pl->bufRt, st));
             }
             if (pl->Etot > 0) {
-                PhaseScope ps(pl, 5 * 8, kb * 8.0 * (double)pl->Etot * ((double)pl->n * pl->n + pl->npts), st);
+                PhaseScope ps(pl, 5 * 8, kb * (double)pl->Etot * corr_flops_per_elem(pl->n, pl->npts), st);
                 LAUNCHCHK(blfid_dev_launch_corr(pr, P, kb, pl->bufQt, pl->bufRt, pl->thetas,
                                                 pl->d_er.p, pl->d_es.p, pl->fidc, st));
             }
@@ -1276,18 +1288,29 @@ blfid_status blfid_plan_set_workspace(blfid_plan *pl, blfid_workspace *w) {
 
 blfid_status blfid_plan_set_profiling(blfid_plan *pl, int on) {
     if (!pl) return fail(BLFID_ERR_INVALID, "plan is NULL");
+    collect_profile(pl);
     pl->prof = on != 0;
     for (int i = 0; i < BLFID_NSLOT; i++) { pl->ph_ms[i] = 0; pl->ph_n[i] = 0; pl->ph_flops[i] = 0; }
+    if (pl->prof) {
+        // the propagator's degree is chosen per task on the device, so its executed flops are
+        // counted there (one atomic add per workgroup)
+        if (!pl->d_expflops.p) HIPCHK(pl->d_expflops.alloc(8));
+        HIPCHK(hipDeviceSynchronize());
+        HIPCHK(hipMemset(pl->d_expflops.p, 0, 8 * sizeof(double)));
+    }
     return BLFID_OK;
 }
 
 blfid_status blfid_plan_phase_times(blfid_plan *pl, double *ms, int *launches, double *flops) {
     if (!pl || !ms || !launches || !flops) return fail(BLFID_ERR_INVALID, "NULL argument");
     collect_profile(pl);
+    double ef[8] = {0};
+    if (pl->prof && pl->d_expflops.p)
+        HIPCHK(hipMemcpy(ef, pl->d_expflops.p, sizeof(ef), hipMemcpyDeviceToHost));
     for (int i = 0; i < BLFID_NSLOT; i++) {
         ms[i] = pl->ph_ms[i];
         launches[i] = pl->ph_n[i];
-        flops[i] = pl->ph_flops[i];
+        flops[i] = pl->ph_flops[i] + (i < 8 ? ef[i] : 0.0);
     }
     return BLFID_OK;
 }

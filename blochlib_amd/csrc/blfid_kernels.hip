@@ -147,8 +147,9 @@ __device__ __forceinline__ void wg_gemm(int m, int n, int k, const T *__restrict
                         T diag, T *smem) {
     constexpr int TM = 16 * M, KS = 16, TY = NT / 16, MJ = TM / TY, LPT = TM * KS / NT;
     // register prefetch of the next K slice unless the accumulators already fill the VGPRs
-    // (complex64 with 80 x 80 tiles would spill with it)
-    constexpr bool PF = (M * MJ * sizeof(T)) <= 256 && !(sizeof(T) == 8 && alignof(T) == 8 && M >= 5 &&
+    // (complex64 with 80 x 80 tiles would spill with it; complex128 with 64 x 64 tiles would
+    // drop to one wave per SIMD)
+    constexpr bool PF = (M * MJ * sizeof(T)) < 256 && !(sizeof(T) == 8 && alignof(T) == 8 && M >= 5 &&
                                                          !__is_same(T, double));
     T *As = smem;
     T *Bs = smem + KS * TM;
@@ -236,6 +237,143 @@ __device__ __forceinline__ void wg_gemm(int m, int n, int k, const T *__restrict
 
 // Threads per workgroup for a tile variant: 64 for the 16 x 16 tiles of small blocks.
 template <int M> struct NThr { static constexpr int v = (M == 1) ? 64 : 256; };
+
+// --------------------------------------------------------- complex64 matrix-core GEMM
+// C = op(A) B with op(A) = A or A^H, all column-major, complex64.  Output tiles are 16M x 16M,
+// split into 16 x 16 sub-tiles dealt round robin to the NT / 64 waves.  k is staged in chunks
+// of KC into LDS with k contiguous (re and im planes: At[i][q] = op(A)(i, q), Bt[j][q] = B(q, j)),
+// rows padded to KC + 4 floats so that the float4 fragment reads of 8 consecutive rows fall in
+// distinct banks, zero-filled past m, n, k.  Per 16-wide k slice a lane of group
+// g = lane / 16 reads At[row][4g .. 4g + 3] and Bt[col][4g .. 4g + 3] (re and im) as float4
+// and issues v_mfma_f32_16x16x4f32 16 times, step u supplying k = 4g + u (A and B share that
+// k order, so the sum over u and g is the whole slice): Cr += Ar Br - Ai Bi,
+// Ci += Ar Bi + Ai Br.  The accumulators of lane l hold rows 4 (l / 16) + j, column l % 16 of
+// its sub-tile.  `smem` holds 4 * 16 M * (KC + 4) floats.
+typedef float v4f __attribute__((ext_vector_type(4)));
+template <int NSUB, int M, int NW>
+__device__ __forceinline__ void mfma_store(int i0, int j0, int m, int n, int w, int lane,
+                                           const v4f (&cr)[NSUB], const v4f (&ci)[NSUB],
+                                           cf *C, int ldc) {
+    const int g = lane >> 4, r = lane & 15;
+#pragma unroll
+    for (int s = 0; s < NSUB; s++) {
+        const int t = w + NW * s;
+        if (t < M * M) {
+            const int col = j0 + 16 * (t % M) + r;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int row = i0 + 16 * (t / M) + 4 * g + j;
+                if (row < m && col < n) C[row + (size_t)col * ldc] = make_float2(cr[s][j], ci[s][j]);
+            }
+        }
+    }
+}
+
+template <int M, int NT, int KC>
+__device__ __forceinline__ void wg_cgemm_mfma(int m, int n, int k, const cf *__restrict__ A,
+                                              int lda, bool ca, const cf *__restrict__ B,
+                                              int ldb, cf *__restrict__ C, int ldc,
+                                              float *smem) {
+    constexpr int TM = 16 * M, LDK = KC + 4, NW = NT / 64, NSUB = (M * M + NW - 1) / NW;
+    float *Ar = smem, *Ai = Ar + TM * LDK, *Br = Ai + TM * LDK, *Bi = Br + TM * LDK;
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63, g = lane >> 4, r = lane & 15;
+#pragma unroll 1
+    for (int i0 = 0; i0 < m; i0 += TM) {
+#pragma unroll 1
+        for (int j0 = 0; j0 < n; j0 += TM) {
+            v4f cr[NSUB], ci[NSUB];
+#pragma unroll
+            for (int s = 0; s < NSUB; s++) {
+                cr[s] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+                ci[s] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+            }
+#pragma unroll 1
+            for (int q0 = 0; q0 < k; q0 += KC) {
+                const int kc = min(KC, k - q0), kp = (kc + 15) & ~15;
+                __syncthreads();
+                if (ca) {
+                    for (int idx = tid; idx < TM * kp; idx += NT) {
+                        const int q = idx % kp, i = idx / kp;
+                        cf v = make_float2(0.0f, 0.0f);
+                        if (i0 + i < m && q < kc) {
+                            v = A[(q0 + q) + (size_t)(i0 + i) * lda];
+                            v.y = -v.y;
+                        }
+                        Ar[i * LDK + q] = v.x;
+                        Ai[i * LDK + q] = v.y;
+                    }
+                } else {
+                    for (int idx = tid; idx < TM * kp; idx += NT) {
+                        const int i = idx % TM, q = idx / TM;
+                        cf v = make_float2(0.0f, 0.0f);
+                        if (i0 + i < m && q < kc) v = A[(i0 + i) + (size_t)(q0 + q) * lda];
+                        Ar[i * LDK + q] = v.x;
+                        Ai[i * LDK + q] = v.y;
+                    }
+                }
+                for (int idx = tid; idx < TM * kp; idx += NT) {
+                    const int q = idx % kp, j = idx / kp;
+                    cf v = make_float2(0.0f, 0.0f);
+                    if (j0 + j < n && q < kc) v = B[(q0 + q) + (size_t)(j0 + j) * ldb];
+                    Br[j * LDK + q] = v.x;
+                    Bi[j * LDK + q] = v.y;
+                }
+                __syncthreads();
+#pragma unroll 1
+                for (int qq = 0; qq < kp; qq += 16) {
+#pragma unroll
+                    for (int s = 0; s < NSUB; s++) {
+                        const int t = w + NW * s;
+                        if (t < M * M) {
+                            const int oa = (16 * (t / M) + r) * LDK + qq + 4 * g;
+                            const int ob = (16 * (t % M) + r) * LDK + qq + 4 * g;
+                            const v4f ar = *(const v4f *)&Ar[oa], ai = *(const v4f *)&Ai[oa];
+                            const v4f br = *(const v4f *)&Br[ob], bi = *(const v4f *)&Bi[ob];
+                            const v4f na = -ai;
+#pragma unroll
+                            for (int u = 0; u < 4; u++) {
+                                cr[s] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[u], br[u], cr[s], 0, 0, 0);
+                                ci[s] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[u], bi[u], ci[s], 0, 0, 0);
+                                cr[s] = __builtin_amdgcn_mfma_f32_16x16x4f32(na[u], bi[u], cr[s], 0, 0, 0);
+                                ci[s] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai[u], br[u], ci[s], 0, 0, 0);
+                            }
+                        }
+                    }
+                }
+            }
+            mfma_store<NSUB, M, NW>(i0, j0, m, n, w, lane, cr, ci, C, ldc);
+        }
+    }
+    __syncthreads();
+}
+
+// Complex products: matrix cores for complex64, the VALU tile GEMM for complex128.
+template <int M, int NT, typename R> struct CGemm {
+    typedef typename CX<R>::T CT;
+    static constexpr int BYTES = 2 * 16 * 16 * M * (int)sizeof(CT);
+    __device__ __forceinline__ static void run(int m, int n, int k, const CT *A, int lda, bool ca, const CT *B,
+                               int ldb, CT *C, int ldc, void *smem) {
+        wg_gemm<M, NT, CT>(m, n, k, A, lda, ca, B, ldb, false, C, ldc, Arith<CT>::zero(),
+                           (CT *)smem);
+    }
+};
+template <int M, int NT> struct CGemm<M, NT, float> {
+    static constexpr int KC = (M == 1) ? 16 : 32;
+    static constexpr int BYTES = 4 * 16 * M * (KC + 4) * 4;
+    __device__ __forceinline__ static void run(int m, int n, int k, const cf *A, int lda, bool ca, const cf *B,
+                               int ldb, cf *C, int ldc, void *smem) {
+        wg_cgemm_mfma<M, NT, KC>(m, n, k, A, lda, ca, B, ldb, C, ldc, (float *)smem);
+    }
+};
+
+// Real products of the propagator (both precisions on the VALU tile GEMM: measured faster here
+// than matrix cores, the propagator being bound by its staging and synchronisation).
+template <int M, int NT, typename R> struct ExpGemm {
+    static constexpr int SMEM = 2 * 16 * 16 * M;
+    __device__ __forceinline__ static void run(int b, const R *A, const R *B, R *C, R *smem) {
+        wg_gemm<M, NT, R>(b, b, b, A, b, false, B, b, false, C, b, (R)0, smem);
+    }
+};
 
 // ------------------------------------------------------------------------------ k_prep
 // cartPowder = rotationMatrix3D(phi, theta, gamma) (space_ten.cc:578-618, set by
@@ -391,7 +529,7 @@ __global__ void __launch_bounds__(NThr<M>::v) k_exp(DevPlan P, int Kb, int c0,
                                                     typename CX<R>::T *__restrict__ bufU) {
     constexpr int NT = NThr<M>::v;
     typedef typename CX<R>::T CT;
-    __shared__ __align__(16) R smem[2 * 16 * 16 * M];
+    __shared__ __align__(16) R smem[ExpGemm<M, NT, R>::SMEM];
     __shared__ double s_zz[DEV_MAX_TERMS];
     __shared__ double s_ff[DEV_MAX_SPINS * DEV_MAX_SPINS];
     __shared__ double s_v[4];
@@ -409,7 +547,7 @@ __global__ void __launch_bounds__(NThr<M>::v) k_exp(DevPlan P, int Kb, int c0,
     const R ss[9] = {(R)1.0, (R)(-1.0 / 6), (R)(1.0 / 120), (R)(-1.0 / 5040), (R)(1.0 / 362880),
                      (R)(-1.0 / 39916800), (R)(1.0 / 6227020800.0), (R)(-1.0 / 1307674368000.0),
                      (R)(1.0 / 355687428096000.0)};
-    const R z0 = (R)0;
+    double nflops = 0.0; // executed GEMM flops of this workgroup (counted when profiling)
     for (long task = blockIdx.x; task < ntask; task += gridDim.x) {
         const int g = (int)(task % ngrp);
         const long ck = task / ngrp;
@@ -436,6 +574,8 @@ __global__ void __launch_bounds__(NThr<M>::v) k_exp(DevPlan P, int Kb, int c0,
             if (m == 5) m = 6;
             if (m == 7) m = 8;
         }
+        // GEMMs of the Taylor part (S = X sinc included) and three per squaring
+        nflops += 2.0 * b * b * b * ((m == 6 ? 6 : m == 8 ? 7 : m + 1) + 3 * sq);
         const R sc = (R)(theta / (double)(1 << sq));
         for (int e = tid; e < bb; e += NT) X[e] *= sc;
         __syncthreads();
@@ -447,7 +587,7 @@ __global__ void __launch_bounds__(NThr<M>::v) k_exp(DevPlan P, int Kb, int c0,
             const R *A = (i == 0) ? X : (i == 1) ? Y : Y2;
             const R *B = (i == 0) ? X : (i == 3) ? Y2 : Y;
             R *C = (i == 0) ? Y : (i == 1) ? Y2 : (i == 2) ? Y3 : Y4;
-            wg_gemm<M, NT, R>(b, b, b, A, b, false, B, b, false, C, b, z0, smem);
+            ExpGemm<M, NT, R>::run(b, A, B, C, smem);
         }
         if (m <= 4) {
             for (int e = tid; e < bb; e += NT) {
@@ -484,8 +624,7 @@ __global__ void __launch_bounds__(NThr<M>::v) k_exp(DevPlan P, int Kb, int c0,
             __syncthreads();
 #pragma unroll 1
             for (int i = 0; i < 2; i++)
-                wg_gemm<M, NT, R>(b, b, b, Yh, b, false, i ? Hs : Hc, b, false, i ? Ps : Cm, b, z0,
-                                  smem);
+                ExpGemm<M, NT, R>::run(b, Yh, i ? Hs : Hc, i ? Ps : Cm, smem);
             for (int e = tid; e < bb; e += NT) {
                 R d = (e % b == e / b) ? (R)1 : (R)0;
                 R y = Y[e], y2 = Y2[e];
@@ -498,14 +637,14 @@ __global__ void __launch_bounds__(NThr<M>::v) k_exp(DevPlan P, int Kb, int c0,
             __syncthreads();
         }
         R *Sm = Y4;
-        wg_gemm<M, NT, R>(b, b, b, X, b, false, Ps, b, false, Sm, b, z0, smem);
+        ExpGemm<M, NT, R>::run(b, X, Ps, Sm, smem);
         for (int it = 0; it < sq; it++) {
 #pragma unroll 1
             for (int i = 0; i < 3; i++) {
                 const R *A = (i == 1) ? Sm : Cm;
                 const R *B = (i == 0) ? Cm : Sm;
                 R *C = (i == 0) ? Y : (i == 1) ? Y2 : Y3;
-                wg_gemm<M, NT, R>(b, b, b, A, b, false, B, b, false, C, b, z0, smem);
+                ExpGemm<M, NT, R>::run(b, A, B, C, smem);
             }
             for (int e = tid; e < bb; e += NT) {
                 Cm[e] = Y[e] - Y2[e];
@@ -536,6 +675,7 @@ __global__ void __launch_bounds__(NThr<M>::v) k_exp(DevPlan P, int Kb, int c0,
         }
         __syncthreads();
     }
+    if (P.expflops && tid == 0 && nflops > 0.0) atomicAdd(&P.expflops[M], nflops);
 }
 
 // ------------------------------------------------------------------------------ k_dyson
@@ -550,7 +690,7 @@ __global__ void __launch_bounds__(NThr<M>::v) k_dyson(DevPlan P, int Kb, const i
                                                       typename CX<R>::T *__restrict__ bufA) {
     constexpr int NT = NThr<M>::v;
     typedef typename CX<R>::T CT;
-    __shared__ CT smem[2 * 16 * 16 * M];
+    __shared__ __align__(16) char smem[CGemm<M, NT, R>::BYTES];
     const long ntask = (long)Kb * ngrp;
     for (long task = blockIdx.x; task < ntask; task += gridDim.x) {
         const int g = (int)(task % ngrp), c = (int)(task / ngrp);
@@ -565,7 +705,7 @@ __global__ void __launch_bounds__(NThr<M>::v) k_dyson(DevPlan P, int Kb, const i
             const CT *Uk = bufU + base + (size_t)k * P.Lsum;
             const CT *Ap = bufA + abase + (size_t)((k - 1) % P.akeep) * P.Lsum;
             CT *Ak = bufA + abase + (size_t)(k % P.akeep) * P.Lsum;
-            wg_gemm<M, NT, CT>(b, b, b, Uk, b, false, Ap, b, false, Ak, b, Arith<CT>::zero(), smem);
+            CGemm<M, NT, R>::run(b, b, b, Uk, b, false, Ap, b, Ak, b, smem);
         }
     }
 }
@@ -709,7 +849,10 @@ __global__ void __launch_bounds__(NT) k_eig(DevPlan P, int Kb, const int *__rest
                 }
                 __syncthreads();
             }
-            if (!s_any) break;
+            // every wave reads the flag before the next sweep's reset by thread 0
+            const int any = s_any;
+            __syncthreads();
+            if (!any) break;
         }
         double *o = outv + (size_t)c * P.N + P.blk_soff[blk];
         for (int r = tid; r < b; r += NT) {
@@ -734,7 +877,7 @@ __global__ void __launch_bounds__(NThr<M>::v) k_bmul(DevPlan P, int Kb, const in
                                                      typename CX<R>::T *__restrict__ bufB) {
     constexpr int NT = NThr<M>::v;
     typedef typename CX<R>::T CT;
-    __shared__ CT smem[2 * 16 * 16 * M];
+    __shared__ __align__(16) char smem[CGemm<M, NT, R>::BYTES];
     const long ntask = (long)Kb * P.n * ngrp;
     for (long task = blockIdx.x; task < ntask; task += gridDim.x) {
         const int g = (int)(task % ngrp);
@@ -743,8 +886,7 @@ __global__ void __launch_bounds__(NThr<M>::v) k_bmul(DevPlan P, int Kb, const in
         const int blk = grp[g], b = P.blk_dim[blk];
         const size_t off = ((size_t)c * P.n + k) * P.Lsum + P.blk_uoff[blk];
         const CT *W = bufW + (size_t)c * P.Lsum + P.blk_uoff[blk];
-        wg_gemm<M, NT, CT>(b, b, b, bufA + off, b, false, W, b, false, bufB + off, b,
-                           Arith<CT>::zero(), smem);
+        CGemm<M, NT, R>::run(b, b, b, bufA + off, b, false, W, b, bufB + off, b, smem);
     }
 }
 
@@ -763,7 +905,7 @@ __global__ void __launch_bounds__(NThr<M>::v) k_qr(DevPlan P, int Kb, const int 
                                                    typename CX<R>::T *__restrict__ Rt) {
     constexpr int NT = NThr<M>::v;
     typedef typename CX<R>::T CT;
-    __shared__ CT smem[2 * 16 * 16 * M];
+    __shared__ __align__(16) char smem[CGemm<M, NT, R>::BYTES];
     const int tid = threadIdx.x;
     CT *T1 = scratch + slot * 2 * blockIdx.x;
     CT *G = T1 + slot;
@@ -791,7 +933,7 @@ __global__ void __launch_bounds__(NThr<M>::v) k_qr(DevPlan P, int Kb, const int 
             }
         }
         __syncthreads();
-        wg_gemm<M, NT, CT>(bI, bJ, bI, BI, bI, true, T1, bI, false, Qt + eo, bI, Arith<CT>::zero(), smem);
+        CGemm<M, NT, R>::run(bI, bJ, bI, BI, bI, true, T1, bI, Qt + eo, bI, smem);
         if (P.rt_from_q) continue; // R is formed from Q in k_corr
         {
             const int *rp = P.r_rowptr + P.pair_roff[pr];
@@ -808,7 +950,7 @@ __global__ void __launch_bounds__(NThr<M>::v) k_qr(DevPlan P, int Kb, const int 
         __syncthreads();
         // R^T(r,s) = sum_b T1(b,r) conj(B_J(b,s)) = (T1^T conj(B_J))(r,s): computed as
         // G = B_J^H T1 (bJ x bI) then transposed into Rt
-        wg_gemm<M, NT, CT>(bJ, bI, bJ, BJ, bJ, true, T1, bJ, false, G, bJ, Arith<CT>::zero(), smem);
+        CGemm<M, NT, R>::run(bJ, bI, bJ, BJ, bJ, true, T1, bJ, G, bJ, smem);
         for (int e = tid; e < bI * bJ; e += NT) {
             int r = e % bI, s = e / bI;
             Rt[eo + e] = G[s + (size_t)r * bJ];
@@ -818,7 +960,8 @@ __global__ void __launch_bounds__(NThr<M>::v) k_qr(DevPlan P, int Kb, const int 
 }
 
 // ------------------------------------------------------------------------------ k_corr
-// One workgroup = (crystallite, chunk of CH elements).  Staging applies the gamma-COMPUTE
+// One workgroup task = (crystallite, segment of consecutive chunks of CH elements); per chunk
+// the steps below run in LDS.  Staging applies the gamma-COMPUTE
 // twiddles, Qt_k = Q_k z^-k and Rt_k = R_k(s,r) z^k with z = exp(i phi_e),
 // phi_e = (th_r - th_s)/n (compute.h:406-427 written as exp(-i ind2 tau w) = z^-(i+p)%n z^i),
 // each thread walking a run of k with one sincos and complex multiplies.  Thread (el, pg)
@@ -830,19 +973,31 @@ __global__ void __launch_bounds__(NThr<M>::v) k_qr(DevPlan P, int Kb, const int 
 // product taken in that order and the inverse decimation-in-time (bit-reversed in, natural
 // out), so no permutation pass is needed: 3 (n/2) log2 n butterflies instead of n^2 products.
 // F replaces Qt in LDS and thread t forms fid[t] = sum_e F_{t%n}(e) z_e^t (compute.h:432-457),
-// z_e^t advanced by z_e^256, added atomically to fid_c[c][t].
+// z_e^t advanced by z_e^256, summed over the segment's chunks and added atomically to
+// fid_c[c][t] (per chunk when npts > 2048).
 // CH = min(CORR_SPAN / n, 256 / ceil(n/8)) is computed identically on the host.
 #define CORR_TW 256
 #define CORR_SPAN 2048
 #define CORR_PPT 8
+// Chunks per segment: a crystallite's chunks in runs of up to CORR_SEG_CHUNKS (each run ends
+// with one atomic add per FID point, so runs are long enough for that to be negligible).
+#define CORR_SEG_CHUNKS 64
+__device__ __host__ inline int corr_seg_chunks(int nchunk) {
+    return nchunk < CORR_SEG_CHUNKS ? (nchunk < 1 ? 1 : nchunk) : CORR_SEG_CHUNKS;
+}
 __device__ __host__ inline int corr_chunk(int n) {
     int npg = (n + CORR_PPT - 1) / CORR_PPT;
     int a = CORR_SPAN / n, b = 256 / npg;
     int ch = a < b ? a : b;
     return ch < 1 ? 1 : ch;
 }
-template <typename R>
-__global__ void __launch_bounds__(256) k_corr(DevPlan P, int Kb,
+__host__ __device__ inline bool corr_use_fft(int n) {
+    return n >= 2 && (n & (n - 1)) == 0 && n <= 2 * CORR_TW;
+}
+// The complex64 FFT variant is held to three waves per SIMD (168 VGPRs, three workgroups per CU
+// with its 40 KB of LDS); the others are bound by LDS or registers at two.
+template <typename R, bool FFT>
+__global__ void __launch_bounds__(256, (FFT && sizeof(R) == 4) ? 3 : 2) k_corr(DevPlan P, int Kb,
                                               const typename CX<R>::T *__restrict__ Qt,
                                               const typename CX<R>::T *__restrict__ Rt,
                                               const double *__restrict__ thetas,
@@ -857,7 +1012,7 @@ __global__ void __launch_bounds__(256) k_corr(DevPlan P, int Kb,
     __shared__ CT s_tw[CORR_TW];
     const int n = P.n, tid = threadIdx.x;
     const int CH = corr_chunk(n);
-    const bool use_fft = n >= 2 && (n & (n - 1)) == 0 && n <= 2 * CORR_TW;
+    constexpr bool use_fft = FFT; // launched only where corr_use_fft(n)
     if (use_fft) {
         // e^{-2 pi i j / n}, j < n/2
         for (int j = tid; j < n / 2; j += 256)
@@ -868,157 +1023,202 @@ __global__ void __launch_bounds__(256) k_corr(DevPlan P, int Kb,
     const int nseg = 256 / CH;
     const int kpt = (n + nseg - 1) / nseg;
     const int nchunk = (int)((P.Etot + CH - 1) / CH);
-    const long ntask = (long)Kb * nchunk;
+    // Task = (crystallite, segment of consecutive chunks); the segmentation depends on the plan
+    // only (not on the batch), so a crystallite's FID is summed in the same grouping whatever
+    // the batch size.  For npts <= 2048 the segment's FID partial sums (t = tid + 256 m) stay
+    // in registers and go to fid_c with one atomic add per point at the segment's end.
+    const int spc = corr_seg_chunks(nchunk);
+    const int nsg = (nchunk + spc - 1) / spc;
+    const long ntask = (long)Kb * nsg;
+    const bool hold = P.npts <= 2048;
+    const int nth = tid < P.npts ? min(8, (P.npts - tid + 255) / 256) : 0;
+    int hoff[8];
+#pragma unroll
+    for (int m = 0; m < 8; m++) hoff[m] = ((tid + 256 * m) % n) * CH;
     for (long task = blockIdx.x; task < ntask; task += gridDim.x) {
-        const int ch = (int)(task % nchunk), c = (int)(task / nchunk);
-        const long e0 = (long)ch * CH;
-        for (int el = tid; el < CH; el += 256) {
-            long e = e0 + el;
-            double ph = 0.0;
-            if (e < P.Etot) {
-                const double *th = thetas + (size_t)c * P.N;
-                ph = (th[elem_r[e]] - th[elem_s[e]]) / (double)n;
+        const int c = (int)(task / nsg), sg = (int)(task % nsg);
+        const int ch_end = min(nchunk, (sg + 1) * spc);
+        cd hacc[8];
+#pragma unroll
+        for (int m = 0; m < 8; m++) hacc[m] = cmk(0, 0);
+        for (int ch = sg * spc; ch < ch_end; ch++) {
+            const long e0 = (long)ch * CH;
+            for (int el = tid; el < CH; el += 256) {
+                long e = e0 + el;
+                double ph = 0.0;
+                if (e < P.Etot) {
+                    const double *th = thetas + (size_t)c * P.N;
+                    ph = (th[elem_r[e]] - th[elem_s[e]]) / (double)n;
+                }
+                sphi[el] = ph;
+                sstep[el] = cexpi(256.0 * ph);
             }
-            sphi[el] = ph;
-            sstep[el] = cexpi(256.0 * ph);
-        }
-        __syncthreads();
-        {
-            const int el = tid % CH, seg = tid / CH;
-            const long e = e0 + el;
-            const int ka = seg * kpt, kb = min(n, ka + kpt);
-            if (seg < nseg && ka < kb) {
-                const double ph = sphi[el];
-                cd w = cexpi(-(double)ka * ph);   // z^-k
-                const cd z = cexpi(ph);
-                const cd zi = cconj(z);
-                for (int k = ka; k < kb; k++) {
-                    cd q = cmk(0, 0), r = cmk(0, 0);
-                    if (e < P.Etot) {
-                        size_t o = ((size_t)c * n + k) * P.Etot + e;
-                        q = to_cd(Qt[o]);
-                        if (P.rt_from_q) {
-                            // R(s,r) = Q_(J,I)(s,r) + conj(Q_(I,J)(r,s))
-                            r = cconj(q);
-                            const int rv = P.elem_rev[e];
-                            if (rv >= 0) r = cadd(r, to_cd(Qt[((size_t)c * n + k) * P.Etot + rv]));
-                        } else {
-                            r = to_cd(Rt[o]);
+            __syncthreads();
+            {
+                const int el = tid % CH, seg = tid / CH;
+                const long e = e0 + el;
+                const int ka = seg * kpt, kb = min(n, ka + kpt);
+                if (seg < nseg && ka < kb) {
+                    const double ph = sphi[el];
+                    cd w = cexpi(-(double)ka * ph);   // z^-k
+                    const cd z = cexpi(ph);
+                    const cd zi = cconj(z);
+                    for (int k = ka; k < kb; k++) {
+                        cd q = cmk(0, 0), r = cmk(0, 0);
+                        if (e < P.Etot) {
+                            size_t o = ((size_t)c * n + k) * P.Etot + e;
+                            q = to_cd(Qt[o]);
+                            if (P.rt_from_q) {
+                                // R(s,r) = Q_(J,I)(s,r) + conj(Q_(I,J)(r,s))
+                                r = cconj(q);
+                                const int rv = P.elem_rev[e];
+                                if (rv >= 0) r = cadd(r, to_cd(Qt[((size_t)c * n + k) * P.Etot + rv]));
+                            } else {
+                                r = to_cd(Rt[o]);
+                            }
+                        }
+                        sQ[k * CH + el] = CX<R>::from(cmul(q, w));
+                        sR[k * CH + el] = CX<R>::from(cmul(r, cconj(w)));
+                        w = cmul(w, zi);
+                    }
+                }
+            }
+            __syncthreads();
+            if (use_fft) {
+                // n and CH are powers of two here: butterfly indexes by shifts and masks
+                const int nbf = (n >> 1) * CH;
+                const int lc = __ffs(CH) - 1, ln = __ffs(n) - 1;
+                // forward transforms of Qt (e^-) and Rt (e^+), decimation in frequency
+                for (int lh = ln - 1; lh >= 0; lh--) {
+                    const int half = 1 << lh, ls = ln - 1 - lh; // twiddle stride n / (2 half)
+                    for (int idx = tid; idx < 2 * nbf; idx += 256) {
+                        const bool isR = idx >= nbf;
+                        const int id = isR ? idx - nbf : idx;
+                        const int el = id & (CH - 1), bf = id >> lc;
+                        const int j = bf & (half - 1), g = bf >> lh;
+                        const int i0 = (((g << (lh + 1)) + j) << lc) + el, i1 = i0 + (half << lc);
+                        CT *x = isR ? sR : sQ;
+                        CT w = s_tw[j << ls];
+                        if (isR) w.y = -w.y;
+                        const CT a = x[i0], bq = x[i1];
+                        x[i0] = cadd(a, bq);
+                        x[i1] = cmul(csub(a, bq), w);
+                    }
+                    __syncthreads();
+                }
+                const R inv_n = (R)(1.0 / (double)n);
+                for (int idx = tid; idx < n * CH; idx += 256) sQ[idx] = cscale(cmul(sR[idx], sQ[idx]), inv_n);
+                __syncthreads();
+                // inverse transform (e^+), decimation in time
+                for (int lh = 0; lh < ln; lh++) {
+                    const int half = 1 << lh, ls = ln - 1 - lh;
+                    for (int idx = tid; idx < nbf; idx += 256) {
+                        const int el = idx & (CH - 1), bf = idx >> lc;
+                        const int j = bf & (half - 1), g = bf >> lh;
+                        const int i0 = (((g << (lh + 1)) + j) << lc) + el, i1 = i0 + (half << lc);
+                        const CT w = cconj(s_tw[j << ls]);
+                        const CT a = sQ[i0], t = cmul(sQ[i1], w);
+                        sQ[i0] = cadd(a, t);
+                        sQ[i1] = csub(a, t);
+                    }
+                    __syncthreads();
+                }
+            } else {
+                CT F[CORR_PPT];
+                const int el = tid % CH, pg = tid / CH;
+                const int p0 = pg * CORR_PPT;
+                const bool active = pg < npg;
+#pragma unroll
+                for (int q = 0; q < CORR_PPT; q++) F[q] = Arith<CT>::zero();
+                if (active) {
+                    // window slot (i + q) % 8 holds Qt_{(i + p0 + q) % n}; unrolling i by 8 makes the
+                    // rotation static, so the window never moves between registers
+                    CT win[CORR_PPT];
+#pragma unroll
+                    for (int q = 0; q < CORR_PPT; q++) win[q] = sQ[((p0 + q) % n) * CH + el];
+                    const int n8 = n & ~(CORR_PPT - 1);
+                    for (int i = 0; i < n8; i += CORR_PPT) {
+#pragma unroll
+                        for (int u = 0; u < CORR_PPT; u++) {
+                            CT r = sR[(i + u) * CH + el];
+#pragma unroll
+                            for (int q = 0; q < CORR_PPT; q++) cfma(F[q], r, win[(u + q) % CORR_PPT]);
+                            win[u] = sQ[((i + u + p0 + CORR_PPT) % n) * CH + el];
                         }
                     }
-                    sQ[k * CH + el] = CX<R>::from(cmul(q, w));
-                    sR[k * CH + el] = CX<R>::from(cmul(r, cconj(w)));
-                    w = cmul(w, zi);
+#pragma unroll
+                    for (int u = 0; u < CORR_PPT; u++) { // tail when 8 does not divide n
+                        if (n8 + u < n) {
+                            CT r = sR[(n8 + u) * CH + el];
+#pragma unroll
+                            for (int q = 0; q < CORR_PPT; q++) cfma(F[q], r, win[(u + q) % CORR_PPT]);
+                            win[u] = sQ[((n8 + u + p0 + CORR_PPT) % n) * CH + el];
+                        }
+                    }
                 }
-            }
-        }
-        __syncthreads();
-        if (use_fft) {
-            const int nbf = (n >> 1) * CH;
-            // forward transforms of Qt (e^-) and Rt (e^+), decimation in frequency
-            for (int half = n >> 1, step = 1; half >= 1; half >>= 1, step <<= 1) {
-                for (int idx = tid; idx < 2 * nbf; idx += 256) {
-                    const bool isR = idx >= nbf;
-                    const int id = isR ? idx - nbf : idx;
-                    const int el = id % CH, bf = id / CH;
-                    const int j = bf % half, g = bf / half;
-                    const int i0 = (g * 2 * half + j) * CH + el, i1 = i0 + half * CH;
-                    CT *x = isR ? sR : sQ;
-                    CT w = s_tw[j * step];
-                    if (isR) w.y = -w.y;
-                    const CT a = x[i0], bq = x[i1];
-                    x[i0] = cadd(a, bq);
-                    x[i1] = cmul(csub(a, bq), w);
+                __syncthreads();
+                if (active) {
+#pragma unroll
+                    for (int q = 0; q < CORR_PPT; q++)
+                        if (p0 + q < n) sQ[(p0 + q) * CH + el] = F[q];
                 }
                 __syncthreads();
             }
-            const R inv_n = (R)(1.0 / (double)n);
-            for (int idx = tid; idx < n * CH; idx += 256) sQ[idx] = cscale(cmul(sR[idx], sQ[idx]), inv_n);
-            __syncthreads();
-            // inverse transform (e^+), decimation in time
-            for (int half = 1, step = n >> 1; half < n; half <<= 1, step >>= 1) {
-                for (int idx = tid; idx < nbf; idx += 256) {
-                    const int el = idx % CH, bf = idx / CH;
-                    const int j = bf % half, g = bf / half;
-                    const int i0 = (g * 2 * half + j) * CH + el, i1 = i0 + half * CH;
-                    const CT w = cconj(s_tw[j * step]);
-                    const CT a = sQ[i0], t = cmul(sQ[i1], w);
-                    sQ[i0] = cadd(a, t);
-                    sQ[i1] = csub(a, t);
-                }
-                __syncthreads();
-            }
-        } else {
-            CT F[CORR_PPT];
-            const int el = tid % CH, pg = tid / CH;
-            const int p0 = pg * CORR_PPT;
-            const bool active = pg < npg;
-#pragma unroll
-            for (int q = 0; q < CORR_PPT; q++) F[q] = Arith<CT>::zero();
-            if (active) {
-                // window slot (i + q) % 8 holds Qt_{(i + p0 + q) % n}; unrolling i by 8 makes the
-                // rotation static, so the window never moves between registers
-                CT win[CORR_PPT];
-#pragma unroll
-                for (int q = 0; q < CORR_PPT; q++) win[q] = sQ[((p0 + q) % n) * CH + el];
-                const int n8 = n & ~(CORR_PPT - 1);
-                for (int i = 0; i < n8; i += CORR_PPT) {
-#pragma unroll
-                    for (int u = 0; u < CORR_PPT; u++) {
-                        CT r = sR[(i + u) * CH + el];
-#pragma unroll
-                        for (int q = 0; q < CORR_PPT; q++) cfma(F[q], r, win[(u + q) % CORR_PPT]);
-                        win[u] = sQ[((i + u + p0 + CORR_PPT) % n) * CH + el];
-                    }
-                }
-#pragma unroll
-                for (int u = 0; u < CORR_PPT; u++) { // tail when 8 does not divide n
-                    if (n8 + u < n) {
-                        CT r = sR[(n8 + u) * CH + el];
-#pragma unroll
-                        for (int q = 0; q < CORR_PPT; q++) cfma(F[q], r, win[(u + q) % CORR_PPT]);
-                        win[u] = sQ[((n8 + u + p0 + CORR_PPT) % n) * CH + el];
-                    }
-                }
-            }
-            __syncthreads();
-            if (active) {
-#pragma unroll
-                for (int q = 0; q < CORR_PPT; q++)
-                    if (p0 + q < n) sQ[(p0 + q) * CH + el] = F[q];
-            }
-            __syncthreads();
-        }
-        for (int tb = 0; tb < P.npts; tb += 2048) {
-            const int t0 = tb + tid;
-            if (t0 < P.npts) {
-                const int nt = min(8, (P.npts - t0 + 255) / 256); // t = t0 + 256 m
-                cd acc[8];
-#pragma unroll
-                for (int m = 0; m < 8; m++) acc[m] = cmk(0, 0);
-                for (int e = 0; e < CH; e++) {
-                    if (e0 + e >= P.Etot) break;
-                    cd w = cexpi((double)t0 * sphi[e]);
+            if (hold) {
+                const int ne = (int)min((long)CH, P.Etot - e0);
+                for (int e = 0; e < ne; e++) {
+                    cd w = cexpi((double)tid * sphi[e]);
                     const cd st = sstep[e];
 #pragma unroll
                     for (int m = 0; m < 8; m++) {
-                        if (m < nt) {
-                            cfma(acc[m], to_cd(sQ[((t0 + 256 * m) % n) * CH + e]), w);
+                        if (m < nth) {
+                            cfma(hacc[m], to_cd(sQ[hoff[m] + e]), w);
                             w = cmul(w, st);
                         }
                     }
                 }
+            }
+            for (int tb = 0; !hold && tb < P.npts; tb += 2048) {
+                const int t0 = tb + tid;
+                if (t0 < P.npts) {
+                    const int nt = min(8, (P.npts - t0 + 255) / 256); // t = t0 + 256 m
+                    cd acc[8];
 #pragma unroll
-                for (int m = 0; m < 8; m++) {
-                    if (m < nt) {
-                        const int t = t0 + 256 * m;
-                        atomicAdd(&fidc[2 * ((size_t)c * P.npts + t)], acc[m].x);
-                        atomicAdd(&fidc[2 * ((size_t)c * P.npts + t) + 1], acc[m].y);
+                    for (int m = 0; m < 8; m++) acc[m] = cmk(0, 0);
+                    for (int e = 0; e < CH; e++) {
+                        if (e0 + e >= P.Etot) break;
+                        cd w = cexpi((double)t0 * sphi[e]);
+                        const cd st = sstep[e];
+#pragma unroll
+                        for (int m = 0; m < 8; m++) {
+                            if (m < nt) {
+                                cfma(acc[m], to_cd(sQ[((t0 + 256 * m) % n) * CH + e]), w);
+                                w = cmul(w, st);
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int m = 0; m < 8; m++) {
+                        if (m < nt) {
+                            const int t = t0 + 256 * m;
+                            atomicAdd(&fidc[2 * ((size_t)c * P.npts + t)], acc[m].x);
+                            atomicAdd(&fidc[2 * ((size_t)c * P.npts + t) + 1], acc[m].y);
+                        }
                     }
                 }
             }
+            __syncthreads();
         }
-        __syncthreads();
+        if (hold) {
+#pragma unroll
+            for (int m = 0; m < 8; m++) {
+                if (m < nth) {
+                    const size_t o = 2 * ((size_t)c * P.npts + tid + 256 * m);
+                    atomicAdd(&fidc[o], hacc[m].x);
+                    atomicAdd(&fidc[o + 1], hacc[m].y);
+                }
+            }
+        }
     }
 }
 
@@ -1303,12 +1503,19 @@ extern "C" int blfid_dev_launch_corr(int prec, const DevPlan *P, int Kb, const v
                                      const void *Rt, const double *thetas, const int *er,
                                      const int *es, double *fidc, hipStream_t st) {
     int CH = corr_chunk(P->n);
-    long nchunk = (P->Etot + CH - 1) / CH;
-    long ntask = (long)Kb * nchunk;
+    int nchunk = (int)((P->Etot + CH - 1) / CH);
+    int spc = corr_seg_chunks(nchunk);
+    long ntask = (long)Kb * ((nchunk + spc - 1) / spc);
+    const bool fft = corr_use_fft(P->n);
     int grid = (int)(ntask < GRID_CAP ? ntask : GRID_CAP);
     if (grid <= 0) return 0;
-    if (prec) k_corr<float><<<grid, 256, 0, st>>>(*P, Kb, (const cf *)Qt, (const cf *)Rt, thetas, er, es, fidc);
-    else k_corr<double><<<grid, 256, 0, st>>>(*P, Kb, (const cd *)Qt, (const cd *)Rt, thetas, er, es, fidc);
+    if (prec) {
+        if (fft) k_corr<float, true><<<grid, 256, 0, st>>>(*P, Kb, (const cf *)Qt, (const cf *)Rt, thetas, er, es, fidc);
+        else k_corr<float, false><<<grid, 256, 0, st>>>(*P, Kb, (const cf *)Qt, (const cf *)Rt, thetas, er, es, fidc);
+    } else {
+        if (fft) k_corr<double, true><<<grid, 256, 0, st>>>(*P, Kb, (const cd *)Qt, (const cd *)Rt, thetas, er, es, fidc);
+        else k_corr<double, false><<<grid, 256, 0, st>>>(*P, Kb, (const cd *)Qt, (const cd *)Rt, thetas, er, es, fidc);
+    }
     return (int)hipGetLastError();
 }
 

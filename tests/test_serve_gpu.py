@@ -55,7 +55,11 @@ def test_mix_equals_solo_runs(budget_factor):
     for r in done:
         solo, sw = engine.powder_fid(r.system, r.wr, r.rotor, r.sw, r.npts, r.powder, r.rho, r.det)
         assert r.sw_out == sw
-        assert rel(r.fid, solo) < 1e-12, r.rid
+        # equal to rounding: each crystallite's FID is summed in a plan-fixed grouping, but the
+        # group partial sums reach it by atomic adds in run-dependent order
+        err = rel(r.fid, solo)
+        print(f"[serve] request {r.rid}: mix vs solo rel-L2 {err:.2e}", flush=True)
+        assert err < 1e-11, r.rid
         assert r.t_done >= r.t_admit >= r.arrival - 1e-9
     # and two of them against the oracle
     for r in done[:2]:
