@@ -40,10 +40,12 @@ typedef enum {
 
 typedef enum {
     BLFID_FP64 = 0, /* complex128 everywhere: oneFID<SolidSys, hmatrix> (oneFID.h:52) */
-    BLFID_C64 = 1   /* complex64 propagators, products and correlation sums, as solid-2.0's
+    BLFID_C64 = 1,  /* complex64 propagators, products and correlation sums, as solid-2.0's
                      * oneFID<SolidSys, hmatrixs> (oneFID.h:52-60, sequencerun_funcs.cc:602-609);
-                     * the period eigen-solver, phases, FID points and the powder reduction stay
+                     * the complex products run on FP32 matrix cores with FP32 accumulation; the
+                     * period eigen-solver, phases, FID points and the powder reduction stay
                      * FP64.  The static path always runs in FP64. */
+    BLFID_C64_MFMA = BLFID_C64
 } blfid_precision;
 
 /* Tensor-described SolidSys (src/QMspins/sys.h:59-62, read by sys.cc:120-204).
