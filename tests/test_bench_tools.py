@@ -1,0 +1,77 @@
+"""Measurement plumbing on the CPU: the rocpd summariser, the counter-summary parser that turns
+FETCH_SIZE / WRITE_SIZE into calibrated bytes per launch, and the k_corr flop count."""
+import sqlite3
+
+import numpy as np
+
+import bench
+from blochlib_amd import profsum
+
+
+def make_db(path):
+    con = sqlite3.connect(path)
+    con.execute("create table kernels (name text, start integer, end integer)")
+    con.executemany("insert into kernels values (?, ?, ?)", [
+        ("void k_exp<4>(DevPlan, int)", 0, 2_000_000),
+        ("void k_exp<4>(DevPlan, int)", 5_000_000, 9_000_000),
+        ("k_corr<float, true>(DevPlan, int)", 0, 1_000_000),
+    ])
+    con.execute("create table counters_collection (kernel_name text, dispatch_id integer, "
+                "counter_name text, value real)")
+    # two instances per dispatch (e.g. per XCD) are summed, then averaged over dispatches
+    con.executemany("insert into counters_collection values (?, ?, ?, ?)", [
+        ("k_corr<float, true>(DevPlan, int)", 1, "FETCH_SIZE", 100.0),
+        ("k_corr<float, true>(DevPlan, int)", 1, "FETCH_SIZE", 50.0),
+        ("k_corr<float, true>(DevPlan, int)", 2, "FETCH_SIZE", 250.0),
+    ])
+    con.commit()
+    con.close()
+
+
+def test_profsum_kernel_and_counter_tables(tmp_path):
+    db = tmp_path / "run_results.db"
+    make_db(db)
+    kt = profsum.kernel_table(sqlite3.connect(db))
+    assert kt[0][0] == "k_exp<4>" and kt[0][1] == 2
+    assert abs(kt[0][2] - 6.0) < 1e-12 and abs(kt[0][3] - 3.0) < 1e-12
+    ct = profsum.counter_table(sqlite3.connect(db))
+    assert ct == [("k_corr<float, true>", "FETCH_SIZE", 2, 200.0, 250.0)]
+    text = profsum.summarize(str(tmp_path), "python3 bench.py")
+    assert "k_corr<float, true>" in text and "# command" in text
+
+
+def test_profiled_traffic_calibration(tmp_path, monkeypatch):
+    f = tmp_path / "summary.txt"
+    f.write_text(
+        "# calibration copy bytes: 1073741824\n"
+        "__amd_rocclr_copyBuffer            FETCH_SIZE   10   2000.0   2097152.0\n"
+        "__amd_rocclr_copyBuffer            WRITE_SIZE   10   2000.0   1048576.0\n"
+        "k_corr<float, true>                FETCH_SIZE   74   1000.0   3000.0\n"
+        "k_corr<float, true>                WRITE_SIZE   74    100.0    300.0\n"
+        "k_corr<float, false>               FETCH_SIZE   26   2000.0   3000.0\n"
+        "k_corr<float, false>               WRITE_SIZE   26    200.0    300.0\n")
+    monkeypatch.setattr(bench, "COUNTER_SUMMARY", str(f))
+    t = bench.profiled_traffic("k_corr", "c64")
+    # the copy read 2 GiB for 1 GiB moved: fetch counts are halved, writes taken as they are
+    assert abs(t["fetch_factor"] - 0.5) < 1e-12 and abs(t["write_factor"] - 1.0) < 1e-12
+    fetch = (74 * 1000.0 + 26 * 2000.0) / 100 * 1024 * 0.5
+    write = (74 * 100.0 + 26 * 200.0) / 100 * 1024
+    assert abs(t["bytes"] - (fetch + write)) < 1e-6
+    assert bench.profiled_traffic("k_exp", "c64") is None
+
+
+def test_corr_flop_count():
+    n, P = 128, 1024
+    assert bench.corr_flops_per_elem(n, P) == 15 * n * 7 + 8 * n + 18 * n + 14 * P
+    # not a power of two: the direct cyclic sum
+    assert bench.corr_flops_per_elem(96, P) == 8 * 96 * 96 + 18 * 96 + 14 * P
+    assert np.isclose(bench.corr_flops_per_elem(1024, P), 8 * 1024.0 ** 2 + 18 * 1024 + 14 * P)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, bench.__file__, "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
