@@ -21,6 +21,8 @@ struct DevPlan {
     const double *terms;   // nterms x DEV_TERM_STRIDE
     const double *dconst;  // N: isotropic diagonal of H (-iso_csa m_i + J m_i m_j)
     const double *ffconst; // S x S: constant flip-flop coefficients (strong J / 2)
+    const int *ff_off;     // S x S + 1: per ordered spin pair, its run in ff_term
+    const int *ff_term;    // terms with a flip-flop part, listed under both orders of their pair
     const double *u3;      // n x 3: row 2 of cartSpin at each step midpoint
     const double *RP;      // K x 9: cartPowder of each crystallite (row-major)
     const int *blk_dim;    // nblocks

@@ -181,6 +181,7 @@ struct blfid_plan {
     int n = 1;
     double sw_adj = 0, tau = 0;
     std::vector<double> terms, dconst, ffconst, u3;
+    std::vector<int> ff_off, ff_term; // flip-flop terms of each ordered spin pair (CSR)
     std::vector<Block> blocks;
     std::vector<int> blk_dim, blk_soff, blk_states;
     std::vector<long> blk_uoff;
@@ -206,6 +207,7 @@ struct blfid_plan {
     bool rp_dirty = false;      // cartPowder of the new powder still to be formed (in run)
     // device
     DevBuf<double> d_terms, d_dconst, d_ffconst, d_u3, d_RP, d_th, d_ph, d_ga, d_w;
+    DevBuf<int> d_ff_off, d_ff_term;
     DevBuf<int> d_blk_dim, d_blk_soff, d_blk_states, d_pair_I, d_pair_J, d_pair_doff,
         d_pair_roff, d_drow, d_dcol, d_rrow, d_rcol, d_er, d_es, d_erev;
     DevBuf<long> d_blk_uoff, d_pair_eoff;
@@ -347,6 +349,23 @@ static blfid_status build_tables(blfid_plan *pl) {
     }
     if ((int)(pl->terms.size() / DEV_TERM_STRIDE) > DEV_MAX_TERMS)
         return fail(BLFID_ERR_UNSUPPORTED, "too many interactions (limit 128 anisotropic terms)");
+    {
+        std::vector<std::vector<int>> lists((size_t)S * S);
+        for (size_t t = 0; t < pl->terms.size() / DEV_TERM_STRIDE; t++) {
+            const double *c = &pl->terms[DEV_TERM_STRIDE * t];
+            const int i = (int)c[8], j = (int)c[9];
+            if (c[7] != 0.0 && j >= 0) {
+                lists[(size_t)i * S + j].push_back((int)t);
+                lists[(size_t)j * S + i].push_back((int)t);
+            }
+        }
+        pl->ff_off.assign((size_t)S * S + 1, 0);
+        pl->ff_term.clear();
+        for (size_t q = 0; q < lists.size(); q++) {
+            pl->ff_term.insert(pl->ff_term.end(), lists[q].begin(), lists[q].end());
+            pl->ff_off[q + 1] = (int)pl->ff_term.size();
+        }
+    }
     // blocks: union-find over states, edges = flip-flops of coupled pairs
     std::vector<int> par(N);
     std::iota(par.begin(), par.end(), 0);
@@ -492,6 +511,8 @@ static blfid_status upload_tables(blfid_plan *pl) {
     HIPCHK(pl->d_terms.upload(pl->terms));
     HIPCHK(pl->d_dconst.upload(pl->dconst));
     HIPCHK(pl->d_ffconst.upload(pl->ffconst));
+    HIPCHK(pl->d_ff_off.upload(pl->ff_off));
+    HIPCHK(pl->d_ff_term.upload(pl->ff_term));
     HIPCHK(pl->d_u3.upload(pl->u3));
     HIPCHK(pl->d_blk_dim.upload(pl->blk_dim));
     HIPCHK(pl->d_blk_soff.upload(pl->blk_soff));
@@ -525,6 +546,8 @@ static void refresh_devplan(blfid_plan *pl) {
     d.expflops = pl->prof ? pl->d_expflops.p : nullptr;
     d.theta = PI2 * pl->tau;
     d.terms = pl->d_terms.p;
+    d.ff_off = pl->d_ff_off.p;
+    d.ff_term = pl->d_ff_term.p;
     d.dconst = pl->d_dconst.p;
     d.ffconst = pl->d_ffconst.p;
     d.u3 = pl->d_u3.p;

@@ -405,10 +405,22 @@ __global__ void k_prep(int K, const double *__restrict__ th, const double *__res
 // thread.  Diagonal entries are formed once per row (s_diag); a row's off-diagonal entries are
 // its flip-flop partners (bits i, j differ: s_ff[i][j]) and, with rf, its single flips of
 // pulsed spins, all inside the block, so the row sums need no pass over H.
+// The plan's interaction-term table (nterms x DEV_TERM_STRIDE doubles) copied into the
+// workgroup's dynamic LDS once per launch: the Hamiltonian build reads it for every row and
+// flip-flop pair of every task.
+extern __shared__ double dyn_terms[];
+template <int NT>
+__device__ __forceinline__ const double *stage_terms(const DevPlan &P) {
+    for (int i = threadIdx.x; i < P.nterms * DEV_TERM_STRIDE; i += NT) dyn_terms[i] = P.terms[i];
+    __syncthreads();
+    return dyn_terms;
+}
+
 template <typename R, int NT>
-__device__ double build_h_block(const DevPlan &P, const double *RPc, const double *u, int blk,
-                                R *H, int ld, double *s_zz, double *s_ff, double *s_v,
-                                double *s_diag, double *s_red, const double *rfk) {
+__device__ double build_h_block(const DevPlan &P, const double *terms, const double *RPc,
+                                const double *u, int blk, R *H, int ld, double *s_zz,
+                                double *s_ff, double *s_v, double *s_diag, double *s_red,
+                                const double *rfk) {
     const int tid = threadIdx.x;
     const int S = P.S;
     if (tid == 0) {
@@ -418,42 +430,52 @@ __device__ double build_h_block(const DevPlan &P, const double *RPc, const doubl
     __syncthreads();
     const double vx = s_v[0], vy = s_v[1], vz = s_v[2];
     for (int t = tid; t < P.nterms; t += NT) {
-        const double *c = P.terms + DEV_TERM_STRIDE * t;
+        const double *c = terms + DEV_TERM_STRIDE * t;
         s_zz[t] = c[0] * vx * vx + c[1] * vy * vy + c[2] * vz * vz +
                   2.0 * (c[3] * vx * vy + c[4] * vx * vz + c[5] * vy * vz);
     }
     __syncthreads();
-    // flip-flop coefficient of each spin pair: constant part + anisotropic terms in term order
+    // flip-flop coefficient of each ordered spin pair: constant part + its anisotropic terms (the
+    // pair's list, in term order)
     for (int q = tid; q < S * S; q += NT) {
-        const int i = q / S, j = q % S;
         double f = P.ffconst[q];
-        for (int t = 0; t < P.nterms; t++) {
-            const double *c = P.terms + DEV_TERM_STRIDE * t;
-            const int ti = (int)c[8], tj = (int)c[9];
-            if (c[7] != 0.0 && ((ti == i && tj == j) || (ti == j && tj == i))) f += s_zz[t] * c[7];
+        for (int z = P.ff_off[q]; z < P.ff_off[q + 1]; z++) {
+            const int t = P.ff_term[z];
+            f += s_zz[t] * terms[DEV_TERM_STRIDE * t + 7];
         }
         s_ff[q] = f;
     }
     const int b = P.blk_dim[blk];
     const int *st = P.blk_states + P.blk_soff[blk];
-    for (int r = tid; r < b; r += NT) {
-        const int ga = st[r];
-        double h = P.dconst[ga];
-        for (int t = 0; t < P.nterms; t++) {
-            const double *c = P.terms + DEV_TERM_STRIDE * t;
-            int i = (int)c[8], j = (int)c[9];
-            double mi = ((ga >> (S - 1 - i)) & 1) ? -0.5 : 0.5;
-            double f = c[6] * mi;
-            if (j >= 0) f *= ((ga >> (S - 1 - j)) & 1) ? -0.5 : 0.5;
-            h += s_zz[t] * f;
-        }
-        if (rfk) {
-            for (int i = 0; i < S; i++) {
-                int q = P.spin_rf[i];
-                if (q >= 0) h += rfk[3 * q + 2] * (((ga >> (S - 1 - i)) & 1) ? -0.5 : 0.5);
+    // diagonal: each row's term sum split over tpr consecutive lanes (terms dealt round
+    // robin), combined by lane shuffles; every thread runs the same number of passes
+    int tpr = 1;
+    while (tpr < 16 && tpr * 2 * b <= NT) tpr *= 2;
+    for (int r0 = 0; r0 < b; r0 += NT / tpr) {
+        const int r = r0 + tid / tpr, part = tid % tpr;
+        double h = 0.0;
+        if (r < b) {
+            const int ga = st[r];
+            for (int t = part; t < P.nterms; t += tpr) {
+                const double *c = terms + DEV_TERM_STRIDE * t;
+                int i = (int)c[8], j = (int)c[9];
+                double mi = ((ga >> (S - 1 - i)) & 1) ? -0.5 : 0.5;
+                double f = c[6] * mi;
+                if (j >= 0) f *= ((ga >> (S - 1 - j)) & 1) ? -0.5 : 0.5;
+                h += s_zz[t] * f;
+            }
+            if (part == 0) {
+                h += P.dconst[ga];
+                if (rfk) {
+                    for (int i = 0; i < S; i++) {
+                        int q = P.spin_rf[i];
+                        if (q >= 0) h += rfk[3 * q + 2] * (((ga >> (S - 1 - i)) & 1) ? -0.5 : 0.5);
+                    }
+                }
             }
         }
-        s_diag[r] = h;
+        for (int off = tpr / 2; off > 0; off >>= 1) h += __shfl_xor(h, off);
+        if (r < b && part == 0) s_diag[r] = h;
     }
     __syncthreads();
     double rmax = 0.0;
@@ -547,6 +569,7 @@ __global__ void __launch_bounds__(NThr<M>::v) k_exp(DevPlan P, int Kb, int c0,
     const R ss[9] = {(R)1.0, (R)(-1.0 / 6), (R)(1.0 / 120), (R)(-1.0 / 5040), (R)(1.0 / 362880),
                      (R)(-1.0 / 39916800), (R)(1.0 / 6227020800.0), (R)(-1.0 / 1307674368000.0),
                      (R)(1.0 / 355687428096000.0)};
+    const double *terms = stage_terms<NT>(P);
     double nflops = 0.0; // executed GEMM flops of this workgroup (counted when profiling)
     for (long task = blockIdx.x; task < ntask; task += gridDim.x) {
         const int g = (int)(task % ngrp);
@@ -558,7 +581,7 @@ __global__ void __launch_bounds__(NThr<M>::v) k_exp(DevPlan P, int Kb, int c0,
         const double theta = P.thetak ? P.thetak[k] : P.theta;
         const double *rfk = P.nrfiso > 0 ? P.rf + (size_t)3 * P.nrfiso * k : nullptr;
         const double nrm =
-            build_h_block<R, NT>(P, P.RP + 9 * (size_t)(c0 + c), P.u3 + 3 * k, blk, X, b, s_zz,
+            build_h_block<R, NT>(P, terms, P.RP + 9 * (size_t)(c0 + c), P.u3 + 3 * k, blk, X, b, s_zz,
                                  s_ff, s_v, s_diag, s_red, rfk) * theta;
         int sq = 0;
         while (nrm / (double)(1 << sq) > ExpTol<R>::rmax && sq < 30) sq++;
@@ -890,6 +913,51 @@ __global__ void __launch_bounds__(NThr<M>::v) k_bmul(DevPlan P, int Kb, const in
     }
 }
 
+// T[a + s rows] = sum_z val[z] B[col[z] + s ldb] for a < rows, s < ncols: the CSR rows
+// rp[0..rows] (absolute offsets into col / val) of one block pair of a sparse operator times
+// a dense block.  The rows' columns and values are first staged in `smem` (free until the
+// GEMM that follows) when they fit its `cap` bytes, so the gathers of B are the only global
+// loads of the product; larger pairs read them from global memory.  Ends with a barrier.
+template <int NT, typename R>
+__device__ __forceinline__ void sparse_rows_times(const int *__restrict__ rp,
+                                                  const int *__restrict__ col,
+                                                  const cd *__restrict__ val, int rows, int ncols,
+                                                  const typename CX<R>::T *__restrict__ B, int ldb,
+                                                  typename CX<R>::T *__restrict__ T, char *smem,
+                                                  int cap) {
+    typedef typename CX<R>::T CT;
+    const int tid = threadIdx.x;
+    const int z0 = rp[0], nnz = rp[rows] - z0;
+    const int off = ((rows + 1) * 4 + 15) & ~15;
+    if (off + (size_t)nnz * (sizeof(CT) + 4) <= (size_t)cap) {
+        int *s_rp = (int *)smem;
+        CT *s_v = (CT *)(smem + off);
+        int *s_c = (int *)(s_v + nnz);
+        for (int i = tid; i <= rows; i += NT) s_rp[i] = rp[i] - z0;
+        for (int i = tid; i < nnz; i += NT) {
+            s_c[i] = col[z0 + i];
+            s_v[i] = CX<R>::from(val[z0 + i]);
+        }
+        __syncthreads();
+        for (int e = tid; e < rows * ncols; e += NT) {
+            const int a = e % rows, s = e / rows;
+            CT acc = Arith<CT>::zero();
+            for (int z = s_rp[a]; z < s_rp[a + 1]; z++)
+                cfma(acc, s_v[z], B[s_c[z] + (size_t)s * ldb]);
+            T[e] = acc;
+        }
+    } else {
+        for (int e = tid; e < rows * ncols; e += NT) {
+            const int a = e % rows, s = e / rows;
+            CT acc = Arith<CT>::zero();
+            for (int z = rp[a]; z < rp[a + 1]; z++)
+                cfma(acc, CX<R>::from(val[z]), B[col[z] + (size_t)s * ldb]);
+            T[e] = acc;
+        }
+    }
+    __syncthreads();
+}
+
 // ------------------------------------------------------------------------------ k_qr
 // One task = (crystallite, step, block pair (I,J)).  With B = A_{k+1} W (per block):
 //   Q(r,s) = sum conj(B_I(a,r)) D_IJ(a,b) B_J(b,s)        r in I, s in J
@@ -920,34 +988,12 @@ __global__ void __launch_bounds__(NThr<M>::v) k_qr(DevPlan P, int Kb, const int 
         const CT *BI = bufB + ((size_t)c * P.n + k) * P.Lsum + P.blk_uoff[I];
         const CT *BJ = bufB + ((size_t)c * P.n + k) * P.Lsum + P.blk_uoff[J];
         const size_t eo = ((size_t)c * P.n + k) * P.Etot + P.pair_eoff[pr];
-        {
-            const int *rp = P.d_rowptr + P.pair_doff[pr];
-            const int *ci = P.d_col;
-            const cd *dv = (const cd *)P.d_val;
-            for (int e = tid; e < bI * bJ; e += NT) {
-                int a = e % bI, s = e / bI;
-                CT acc = Arith<CT>::zero();
-                for (int z = rp[a]; z < rp[a + 1]; z++)
-                    cfma(acc, CX<R>::from(dv[z]), BJ[ci[z] + (size_t)s * bJ]);
-                T1[e] = acc;
-            }
-        }
-        __syncthreads();
+        sparse_rows_times<NT, R>(P.d_rowptr + P.pair_doff[pr], P.d_col, (const cd *)P.d_val, bI,
+                                 bJ, BJ, bJ, T1, smem, CGemm<M, NT, R>::BYTES);
         CGemm<M, NT, R>::run(bI, bJ, bI, BI, bI, true, T1, bI, Qt + eo, bI, smem);
         if (P.rt_from_q) continue; // R is formed from Q in k_corr
-        {
-            const int *rp = P.r_rowptr + P.pair_roff[pr];
-            const int *ci = P.r_col;
-            const cd *rv = (const cd *)P.r_val;
-            for (int e = tid; e < bJ * bI; e += NT) {
-                int bq = e % bJ, r = e / bJ;
-                CT acc = Arith<CT>::zero();
-                for (int z = rp[bq]; z < rp[bq + 1]; z++)
-                    cfma(acc, CX<R>::from(rv[z]), BI[ci[z] + (size_t)r * bI]);
-                T1[e] = acc;
-            }
-        }
-        __syncthreads();
+        sparse_rows_times<NT, R>(P.r_rowptr + P.pair_roff[pr], P.r_col, (const cd *)P.r_val, bJ,
+                                 bI, BI, bI, T1, smem, CGemm<M, NT, R>::BYTES);
         // R^T(r,s) = sum_b T1(b,r) conj(B_J(b,s)) = (T1^T conj(B_J))(r,s): computed as
         // G = B_J^H T1 (bJ x bI) then transposed into Rt
         CGemm<M, NT, R>::run(bJ, bI, bJ, BJ, bJ, true, T1, bJ, G, bJ, smem);
@@ -994,10 +1040,11 @@ __device__ __host__ inline int corr_chunk(int n) {
 __host__ __device__ inline bool corr_use_fft(int n) {
     return n >= 2 && (n & (n - 1)) == 0 && n <= 2 * CORR_TW;
 }
-// The complex64 FFT variant is held to three waves per SIMD (168 VGPRs, three workgroups per CU
-// with its 40 KB of LDS); the others are bound by LDS or registers at two.
-template <typename R, bool FFT>
-__global__ void __launch_bounds__(256, (FFT && sizeof(R) == 4) ? 3 : 2) k_corr(DevPlan P, int Kb,
+// NPT FID points per thread are held over a segment (npts <= 256 NPT).  The complex64 FFT
+// variant with four points is held to four waves per SIMD (four workgroups per CU with its 40 KB
+// of LDS); the others are bound by LDS or registers at two.
+template <typename R, bool FFT, int NPT>
+__global__ void __launch_bounds__(256, (FFT && sizeof(R) == 4 && NPT == 4) ? 4 : 2) k_corr(DevPlan P, int Kb,
                                               const typename CX<R>::T *__restrict__ Qt,
                                               const typename CX<R>::T *__restrict__ Rt,
                                               const double *__restrict__ thetas,
@@ -1012,6 +1059,12 @@ __global__ void __launch_bounds__(256, (FFT && sizeof(R) == 4) ? 3 : 2) k_corr(D
     __shared__ CT s_tw[CORR_TW];
     const int n = P.n, tid = threadIdx.x;
     const int CH = corr_chunk(n);
+    // LDS slot of (step k, element el): rows of CH elements, the element index XOR-ed with a
+    // function of the row when CH is a power of two, so that one element read across
+    // consecutive rows (the FID sums) falls in distinct banks
+    const int swz = (CH & (CH - 1)) == 0 ? CH - 1 : 0;
+    constexpr int ssh = sizeof(CT) == 8 ? 1 : 0;
+    auto at = [=](int k, int el) { return k * CH + (el ^ ((k >> ssh) & swz)); };
     constexpr bool use_fft = FFT; // launched only where corr_use_fft(n)
     if (use_fft) {
         // e^{-2 pi i j / n}, j < n/2
@@ -1030,17 +1083,17 @@ __global__ void __launch_bounds__(256, (FFT && sizeof(R) == 4) ? 3 : 2) k_corr(D
     const int spc = corr_seg_chunks(nchunk);
     const int nsg = (nchunk + spc - 1) / spc;
     const long ntask = (long)Kb * nsg;
-    const bool hold = P.npts <= 2048;
-    const int nth = tid < P.npts ? min(8, (P.npts - tid + 255) / 256) : 0;
-    int hoff[8];
+    const bool hold = P.npts <= 256 * NPT;
+    const int nth = tid < P.npts ? min(NPT, (P.npts - tid + 255) / 256) : 0;
+    int hoff[NPT];
 #pragma unroll
-    for (int m = 0; m < 8; m++) hoff[m] = ((tid + 256 * m) % n) * CH;
+    for (int m = 0; m < NPT; m++) hoff[m] = FFT ? 0 : (tid + 256 * m) % n; // step of point tid + 256 m
     for (long task = blockIdx.x; task < ntask; task += gridDim.x) {
         const int c = (int)(task / nsg), sg = (int)(task % nsg);
         const int ch_end = min(nchunk, (sg + 1) * spc);
-        cd hacc[8];
+        cd hacc[NPT];
 #pragma unroll
-        for (int m = 0; m < 8; m++) hacc[m] = cmk(0, 0);
+        for (int m = 0; m < NPT; m++) hacc[m] = cmk(0, 0);
         for (int ch = sg * spc; ch < ch_end; ch++) {
             const long e0 = (long)ch * CH;
             for (int el = tid; el < CH; el += 256) {
@@ -1077,8 +1130,8 @@ __global__ void __launch_bounds__(256, (FFT && sizeof(R) == 4) ? 3 : 2) k_corr(D
                                 r = to_cd(Rt[o]);
                             }
                         }
-                        sQ[k * CH + el] = CX<R>::from(cmul(q, w));
-                        sR[k * CH + el] = CX<R>::from(cmul(r, cconj(w)));
+                        sQ[at(k, el)] = CX<R>::from(cmul(q, w));
+                        sR[at(k, el)] = CX<R>::from(cmul(r, cconj(w)));
                         w = cmul(w, zi);
                     }
                 }
@@ -1096,7 +1149,8 @@ __global__ void __launch_bounds__(256, (FFT && sizeof(R) == 4) ? 3 : 2) k_corr(D
                         const int id = isR ? idx - nbf : idx;
                         const int el = id & (CH - 1), bf = id >> lc;
                         const int j = bf & (half - 1), g = bf >> lh;
-                        const int i0 = (((g << (lh + 1)) + j) << lc) + el, i1 = i0 + (half << lc);
+                        const int r0 = (g << (lh + 1)) + j;
+                        const int i0 = at(r0, el), i1 = at(r0 + half, el);
                         CT *x = isR ? sR : sQ;
                         CT w = s_tw[j << ls];
                         if (isR) w.y = -w.y;
@@ -1115,7 +1169,8 @@ __global__ void __launch_bounds__(256, (FFT && sizeof(R) == 4) ? 3 : 2) k_corr(D
                     for (int idx = tid; idx < nbf; idx += 256) {
                         const int el = idx & (CH - 1), bf = idx >> lc;
                         const int j = bf & (half - 1), g = bf >> lh;
-                        const int i0 = (((g << (lh + 1)) + j) << lc) + el, i1 = i0 + (half << lc);
+                        const int r0 = (g << (lh + 1)) + j;
+                        const int i0 = at(r0, el), i1 = at(r0 + half, el);
                         const CT w = cconj(s_tw[j << ls]);
                         const CT a = sQ[i0], t = cmul(sQ[i1], w);
                         sQ[i0] = cadd(a, t);
@@ -1135,24 +1190,24 @@ __global__ void __launch_bounds__(256, (FFT && sizeof(R) == 4) ? 3 : 2) k_corr(D
                     // rotation static, so the window never moves between registers
                     CT win[CORR_PPT];
 #pragma unroll
-                    for (int q = 0; q < CORR_PPT; q++) win[q] = sQ[((p0 + q) % n) * CH + el];
+                    for (int q = 0; q < CORR_PPT; q++) win[q] = sQ[at((p0 + q) % n, el)];
                     const int n8 = n & ~(CORR_PPT - 1);
                     for (int i = 0; i < n8; i += CORR_PPT) {
 #pragma unroll
                         for (int u = 0; u < CORR_PPT; u++) {
-                            CT r = sR[(i + u) * CH + el];
+                            CT r = sR[at(i + u, el)];
 #pragma unroll
                             for (int q = 0; q < CORR_PPT; q++) cfma(F[q], r, win[(u + q) % CORR_PPT]);
-                            win[u] = sQ[((i + u + p0 + CORR_PPT) % n) * CH + el];
+                            win[u] = sQ[at((i + u + p0 + CORR_PPT) % n, el)];
                         }
                     }
 #pragma unroll
                     for (int u = 0; u < CORR_PPT; u++) { // tail when 8 does not divide n
                         if (n8 + u < n) {
-                            CT r = sR[(n8 + u) * CH + el];
+                            CT r = sR[at(n8 + u, el)];
 #pragma unroll
                             for (int q = 0; q < CORR_PPT; q++) cfma(F[q], r, win[(u + q) % CORR_PPT]);
-                            win[u] = sQ[((n8 + u + p0 + CORR_PPT) % n) * CH + el];
+                            win[u] = sQ[at((n8 + u + p0 + CORR_PPT) % n, el)];
                         }
                     }
                 }
@@ -1160,7 +1215,7 @@ __global__ void __launch_bounds__(256, (FFT && sizeof(R) == 4) ? 3 : 2) k_corr(D
                 if (active) {
 #pragma unroll
                     for (int q = 0; q < CORR_PPT; q++)
-                        if (p0 + q < n) sQ[(p0 + q) * CH + el] = F[q];
+                        if (p0 + q < n) sQ[at(p0 + q, el)] = F[q];
                 }
                 __syncthreads();
             }
@@ -1170,35 +1225,35 @@ __global__ void __launch_bounds__(256, (FFT && sizeof(R) == 4) ? 3 : 2) k_corr(D
                     cd w = cexpi((double)tid * sphi[e]);
                     const cd st = sstep[e];
 #pragma unroll
-                    for (int m = 0; m < 8; m++) {
+                    for (int m = 0; m < NPT; m++) {
                         if (m < nth) {
-                            cfma(hacc[m], to_cd(sQ[hoff[m] + e]), w);
+                            cfma(hacc[m], to_cd(sQ[at(FFT ? (tid + 256 * m) & (n - 1) : hoff[m], e)]), w);
                             w = cmul(w, st);
                         }
                     }
                 }
             }
-            for (int tb = 0; !hold && tb < P.npts; tb += 2048) {
+            for (int tb = 0; !hold && tb < P.npts; tb += 256 * NPT) {
                 const int t0 = tb + tid;
                 if (t0 < P.npts) {
-                    const int nt = min(8, (P.npts - t0 + 255) / 256); // t = t0 + 256 m
-                    cd acc[8];
+                    const int nt = min(NPT, (P.npts - t0 + 255) / 256); // t = t0 + 256 m
+                    cd acc[NPT];
 #pragma unroll
-                    for (int m = 0; m < 8; m++) acc[m] = cmk(0, 0);
+                    for (int m = 0; m < NPT; m++) acc[m] = cmk(0, 0);
                     for (int e = 0; e < CH; e++) {
                         if (e0 + e >= P.Etot) break;
                         cd w = cexpi((double)t0 * sphi[e]);
                         const cd st = sstep[e];
 #pragma unroll
-                        for (int m = 0; m < 8; m++) {
+                        for (int m = 0; m < NPT; m++) {
                             if (m < nt) {
-                                cfma(acc[m], to_cd(sQ[((t0 + 256 * m) % n) * CH + e]), w);
+                                cfma(acc[m], to_cd(sQ[at((t0 + 256 * m) % n, e)]), w);
                                 w = cmul(w, st);
                             }
                         }
                     }
 #pragma unroll
-                    for (int m = 0; m < 8; m++) {
+                    for (int m = 0; m < NPT; m++) {
                         if (m < nt) {
                             const int t = t0 + 256 * m;
                             atomicAdd(&fidc[2 * ((size_t)c * P.npts + t)], acc[m].x);
@@ -1211,7 +1266,7 @@ __global__ void __launch_bounds__(256, (FFT && sizeof(R) == 4) ? 3 : 2) k_corr(D
         }
         if (hold) {
 #pragma unroll
-            for (int m = 0; m < 8; m++) {
+            for (int m = 0; m < NPT; m++) {
                 if (m < nth) {
                     const size_t o = 2 * ((size_t)c * P.npts + tid + 256 * m);
                     atomicAdd(&fidc[o], hacc[m].x);
@@ -1268,12 +1323,13 @@ __global__ void __launch_bounds__(256) k_static_h(DevPlan P, int Kb, int c0, con
     __shared__ double s_diag[DEV_MAX_BLOCK];
     __shared__ double s_red[4];
     double *X = scratch + slot * blockIdx.x;
+    const double *terms = stage_terms<256>(P);
     const long ntask = (long)Kb * ngrp;
     for (long task = blockIdx.x; task < ntask; task += gridDim.x) {
         const int g = (int)(task % ngrp), c = (int)(task / ngrp);
         const int blk = grp[g], b = P.blk_dim[blk];
-        build_h_block<double, 256>(P, P.RP + 9 * (size_t)(c0 + c), P.u3, blk, X, b, s_zz, s_ff, s_v,
-                                   s_diag, s_red, nullptr);
+        build_h_block<double, 256>(P, terms, P.RP + 9 * (size_t)(c0 + c), P.u3, blk, X, b, s_zz,
+                                   s_ff, s_v, s_diag, s_red, nullptr);
         cd *H = bufH + (size_t)c * P.Lsum + P.blk_uoff[blk];
         for (int e = threadIdx.x; e < b * b; e += 256) H[e] = cmk(X[e], 0.0);
         __syncthreads();
@@ -1295,7 +1351,7 @@ __global__ void __launch_bounds__(NThr<M>::v) k_static_fid(DevPlan P, int Kb, co
                                                     double w0, int prune,
                                                     double *__restrict__ fidc) {
     constexpr int NT = NThr<M>::v;
-    __shared__ cd smem[2 * 16 * 16 * M];
+    __shared__ __align__(16) cd smem[2 * 16 * 16 * M];
     const int tid = threadIdx.x;
     cd *T1 = scratch + slot * 3 * blockIdx.x;
     cd *Gq = T1 + slot, *Gr = Gq + slot;
@@ -1309,29 +1365,11 @@ __global__ void __launch_bounds__(NThr<M>::v) k_static_fid(DevPlan P, int Kb, co
         const cd *VJ = bufW + (size_t)c * P.Lsum + P.blk_uoff[J];
         const double *lI = lam + (size_t)c * P.N + P.blk_soff[I];
         const double *lJ = lam + (size_t)c * P.N + P.blk_soff[J];
-        {
-            const int *rp = P.d_rowptr + P.pair_doff[pr];
-            const cd *dv = (const cd *)P.d_val;
-            for (int e = tid; e < bI * bJ; e += NT) {
-                int a = e % bI, s = e / bI;
-                cd acc = cmk(0, 0);
-                for (int z = rp[a]; z < rp[a + 1]; z++) cfma(acc, dv[z], VJ[P.d_col[z] + (size_t)s * bJ]);
-                T1[e] = acc;
-            }
-        }
-        __syncthreads();
+        sparse_rows_times<NT, double>(P.d_rowptr + P.pair_doff[pr], P.d_col, (const cd *)P.d_val,
+                                      bI, bJ, VJ, bJ, T1, (char *)smem, (int)sizeof(smem));
         wg_gemm<M, NT, cd>(bI, bJ, bI, VI, bI, true, T1, bI, false, Gq, bI, cmk(0, 0), smem);
-        {
-            const int *rp = P.r_rowptr + P.pair_roff[pr];
-            const cd *rv = (const cd *)P.r_val;
-            for (int e = tid; e < bJ * bI; e += NT) {
-                int bq = e % bJ, r = e / bJ;
-                cd acc = cmk(0, 0);
-                for (int z = rp[bq]; z < rp[bq + 1]; z++) cfma(acc, rv[z], VI[P.r_col[z] + (size_t)r * bI]);
-                T1[e] = acc;
-            }
-        }
-        __syncthreads();
+        sparse_rows_times<NT, double>(P.r_rowptr + P.pair_roff[pr], P.r_col, (const cd *)P.r_val,
+                                      bJ, bI, VI, bI, T1, (char *)smem, (int)sizeof(smem));
         wg_gemm<M, NT, cd>(bJ, bI, bJ, VJ, bJ, true, T1, bJ, false, Gr, bJ, cmk(0, 0), smem);
         // A(i,j) = Do(i,j) sig0(j,i): i in I, j in J
         for (int e = tid; e < bI * bJ; e += NT) {
@@ -1366,12 +1404,13 @@ static void launch_exp(int M, const DevPlan *P, int Kb, int c0, const int *grp, 
                        void *scratch, size_t slot, int grid, void *bufU, hipStream_t st) {
     R *sc = (R *)scratch;
     typename CX<R>::T *U = (typename CX<R>::T *)bufU;
+    const size_t tb = sizeof(double) * DEV_TERM_STRIDE * (size_t)P->nterms; // staged terms
     switch (M) {
-    case 1: k_exp<1, R><<<grid, 64, 0, st>>>(*P, Kb, c0, grp, ngrp, sc, slot, U); break;
-    case 2: k_exp<2, R><<<grid, 256, 0, st>>>(*P, Kb, c0, grp, ngrp, sc, slot, U); break;
-    case 3: k_exp<3, R><<<grid, 256, 0, st>>>(*P, Kb, c0, grp, ngrp, sc, slot, U); break;
-    case 4: k_exp<4, R><<<grid, 256, 0, st>>>(*P, Kb, c0, grp, ngrp, sc, slot, U); break;
-    default: k_exp<5, R><<<grid, 256, 0, st>>>(*P, Kb, c0, grp, ngrp, sc, slot, U); break;
+    case 1: k_exp<1, R><<<grid, 64, tb, st>>>(*P, Kb, c0, grp, ngrp, sc, slot, U); break;
+    case 2: k_exp<2, R><<<grid, 256, tb, st>>>(*P, Kb, c0, grp, ngrp, sc, slot, U); break;
+    case 3: k_exp<3, R><<<grid, 256, tb, st>>>(*P, Kb, c0, grp, ngrp, sc, slot, U); break;
+    case 4: k_exp<4, R><<<grid, 256, tb, st>>>(*P, Kb, c0, grp, ngrp, sc, slot, U); break;
+    default: k_exp<5, R><<<grid, 256, tb, st>>>(*P, Kb, c0, grp, ngrp, sc, slot, U); break;
     }
 }
 
@@ -1499,6 +1538,19 @@ extern "C" int blfid_dev_launch_qr(int prec, int M, const DevPlan *P, int Kb, co
     return (int)hipGetLastError();
 }
 
+template <int NPT>
+static void launch_corr(int prec, bool fft, int grid, const DevPlan *P, int Kb, const void *Qt,
+                        const void *Rt, const double *thetas, const int *er, const int *es,
+                        double *fidc, hipStream_t st) {
+    if (prec) {
+        if (fft) k_corr<float, true, NPT><<<grid, 256, 0, st>>>(*P, Kb, (const cf *)Qt, (const cf *)Rt, thetas, er, es, fidc);
+        else k_corr<float, false, NPT><<<grid, 256, 0, st>>>(*P, Kb, (const cf *)Qt, (const cf *)Rt, thetas, er, es, fidc);
+    } else {
+        if (fft) k_corr<double, true, NPT><<<grid, 256, 0, st>>>(*P, Kb, (const cd *)Qt, (const cd *)Rt, thetas, er, es, fidc);
+        else k_corr<double, false, NPT><<<grid, 256, 0, st>>>(*P, Kb, (const cd *)Qt, (const cd *)Rt, thetas, er, es, fidc);
+    }
+}
+
 extern "C" int blfid_dev_launch_corr(int prec, const DevPlan *P, int Kb, const void *Qt,
                                      const void *Rt, const double *thetas, const int *er,
                                      const int *es, double *fidc, hipStream_t st) {
@@ -1509,13 +1561,9 @@ extern "C" int blfid_dev_launch_corr(int prec, const DevPlan *P, int Kb, const v
     const bool fft = corr_use_fft(P->n);
     int grid = (int)(ntask < GRID_CAP ? ntask : GRID_CAP);
     if (grid <= 0) return 0;
-    if (prec) {
-        if (fft) k_corr<float, true><<<grid, 256, 0, st>>>(*P, Kb, (const cf *)Qt, (const cf *)Rt, thetas, er, es, fidc);
-        else k_corr<float, false><<<grid, 256, 0, st>>>(*P, Kb, (const cf *)Qt, (const cf *)Rt, thetas, er, es, fidc);
-    } else {
-        if (fft) k_corr<double, true><<<grid, 256, 0, st>>>(*P, Kb, (const cd *)Qt, (const cd *)Rt, thetas, er, es, fidc);
-        else k_corr<double, false><<<grid, 256, 0, st>>>(*P, Kb, (const cd *)Qt, (const cd *)Rt, thetas, er, es, fidc);
-    }
+    // four points per thread up to 1024 FID points, eight above
+    if (P->npts <= 1024) launch_corr<4>(prec, fft, grid, P, Kb, Qt, Rt, thetas, er, es, fidc, st);
+    else launch_corr<8>(prec, fft, grid, P, Kb, Qt, Rt, thetas, er, es, fidc, st);
     return (int)hipGetLastError();
 }
 
@@ -1554,7 +1602,8 @@ extern "C" int blfid_dev_launch_static_h(const DevPlan *P, int Kb, int c0, const
     long ntask = (long)Kb * ngrp;
     int grid = (int)(ntask < nslots ? ntask : nslots);
     if (grid <= 0) return 0;
-    k_static_h<<<grid, 256, 0, st>>>(*P, Kb, c0, grp, ngrp, scratch, slot, bufH);
+    k_static_h<<<grid, 256, sizeof(double) * DEV_TERM_STRIDE * (size_t)P->nterms, st>>>(
+        *P, Kb, c0, grp, ngrp, scratch, slot, bufH);
     return (int)hipGetLastError();
 }
 
