@@ -860,8 +860,10 @@ __global__ void __launch_bounds__(NT) k_eig(DevPlan P, int Kb, const int *__rest
                     if (vq) Ms[q + p2 * ld] = cadd(cscale(c1, cj), cmul(sj, d1));
                     if (vq && vq2) Ms[q + q2 * ld] = csub(cscale(d1, cj), cmul(cconj(sj), c1));
                 }
+                // W <- W R: consecutive threads take consecutive rows of one column pair, so
+                // the global accesses of a wave are contiguous
                 for (int e = tid; e < npair * b; e += NT) {
-                    int pi = e % npair, row = e / npair;
+                    int row = e % b, pi = e / b;
                     int p = rp[pi], q = rq[pi];
                     if (q >= b) continue;
                     double cth = rc[pi];
