@@ -142,7 +142,8 @@ blfid_status blfid_powder_fid(blfid_plan *plan, const double *theta, const doubl
                               double *fid_out);
 
 /* Work accounting of the plan for the current operators (per crystallite):
- * out[0] executed flops per crystallite-step, out[1] reference dense flops per
+ * out[0] executed flops per crystallite-step (the propagator counted at its nominal degree-8
+ * Taylor form, 7 real GEMMs per block), out[1] reference dense flops per
  * crystallite-step (48N^3 + 14nN^2 + 14PN^2/n), out[2] number of Hilbert-space blocks,
  * out[3] largest block, out[4] device bytes per crystallite of working storage,
  * out[5] fixed device bytes of working storage independent of the batch size. */
@@ -169,8 +170,9 @@ blfid_status blfid_plan_set_workspace(blfid_plan *plan, blfid_workspace *ws);
  * eigen-solver (phase 2) M - 1 is its block-size class (b <= 16, 32, 48, 64, 80, larger).
  * blfid_plan_phase_times waits for the last run and returns, per slot phase*8 + M (M = 0 for
  * untemplated kernels), the summed kernel time in ms, the number of launches and the executed
- * algorithmic FP64 flops, accumulated since profiling was enabled.  Arrays hold BLFID_NSLOT
- * entries. */
+ * algorithmic flops (unpadded block sizes; the propagator's as executed, its Taylor degree being
+ * chosen per task on the device; the correlation's as FFT or direct sum, as run), accumulated
+ * since profiling was enabled.  Arrays hold BLFID_NSLOT entries. */
 #define BLFID_NPHASE 8
 #define BLFID_NSLOT 64
 blfid_status blfid_plan_set_profiling(blfid_plan *plan, int on);
