@@ -1026,6 +1026,7 @@ __global__ void __launch_bounds__(NThr<M>::v) k_qr(DevPlan P, int Kb, const int 
 // CH = min(CORR_SPAN / n, 256 / ceil(n/8)) is computed identically on the host.
 #define CORR_TW 256
 #define CORR_SPAN 2048
+#define CORR_VTAB 512 /* complex128 entries of the period table, within the second buffer */
 #define CORR_PPT 8
 // Chunks per segment: a crystallite's chunks in runs of up to CORR_SEG_CHUNKS (each run ends
 // with one atomic add per FID point, so runs are long enough for that to be negligible).
@@ -1055,7 +1056,7 @@ __global__ void __launch_bounds__(256, (FFT && sizeof(R) == 4 && NPT == 4) ? 4 :
                                               double *__restrict__ fidc) {
     typedef typename CX<R>::T CT;
     __shared__ CT sQ[CORR_SPAN];
-    __shared__ CT sR[CORR_SPAN];
+    __shared__ __align__(16) CT sR[CORR_SPAN];
     __shared__ double sphi[256];
     __shared__ cd sstep[256];
     __shared__ CT s_tw[CORR_TW];
@@ -1085,7 +1086,13 @@ __global__ void __launch_bounds__(256, (FFT && sizeof(R) == 4 && NPT == 4) ? 4 :
     const int spc = corr_seg_chunks(nchunk);
     const int nsg = (nchunk + spc - 1) / spc;
     const long ntask = (long)Kb * nsg;
-    const bool hold = P.npts <= 256 * NPT;
+    // FFT variant: with J = ceil(npts / n) periods, z_e^t = z_e^p (z_e^n)^j for t = p + n j, so
+    // the FID sums are sum_e G_p(e) V_j(e) with G = F z^p formed in place and the table
+    // V_j(e) = z_e^(n j) (in the correlation's second buffer, free by then): no sine or cosine
+    // per point.  Used when CH J fits CORR_VTAB entries; longer FIDs go out per chunk.
+    const int J = (P.npts + n - 1) / n;
+    const bool hold = P.npts <= 256 * NPT && (!FFT || CH * J <= CORR_VTAB);
+    cd *sV = (cd *)sR;
     const int nth = tid < P.npts ? min(NPT, (P.npts - tid + 255) / 256) : 0;
     int hoff[NPT];
 #pragma unroll
@@ -1221,7 +1228,35 @@ __global__ void __launch_bounds__(256, (FFT && sizeof(R) == 4 && NPT == 4) ? 4 :
                 }
                 __syncthreads();
             }
-            if (hold) {
+            if (FFT && hold) {
+                {
+                    const int el = tid % CH, seg = tid / CH;
+                    const int ka = seg * kpt, kb = min(n, ka + kpt);
+                    if (seg < nseg && ka < kb) {
+                        const double ph = sphi[el];
+                        cd w = cexpi((double)ka * ph); // z^p
+                        const cd z = cexpi(ph);
+                        for (int k = ka; k < kb; k++) {
+                            sQ[at(k, el)] = CX<R>::from(cmul(to_cd(sQ[at(k, el)]), w));
+                            w = cmul(w, z);
+                        }
+                    }
+                }
+                for (int q = tid; q < CH * J; q += 256)
+                    sV[q] = cexpi((double)n * (double)(q % J) * sphi[q / J]);
+                __syncthreads();
+                const int ne = (int)min((long)CH, P.Etot - e0);
+                const int ln = __ffs(n) - 1;
+                for (int e = 0; e < ne; e++) {
+#pragma unroll
+                    for (int m = 0; m < NPT; m++) {
+                        if (m < nth) {
+                            const int t = tid + 256 * m;
+                            cfma(hacc[m], to_cd(sQ[at(t & (n - 1), e)]), sV[e * J + (t >> ln)]);
+                        }
+                    }
+                }
+            } else if (hold) {
                 const int ne = (int)min((long)CH, P.Etot - e0);
                 for (int e = 0; e < ne; e++) {
                     cd w = cexpi((double)tid * sphi[e]);
