@@ -45,6 +45,8 @@ from blochlib_amd import engine, workloads  # noqa: E402
 FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
+DTYPE = {"fp64": "f64", "c64": "c64 (f32 products, f64 eig/phases/sums)",
+         "bf16x3": "c64 (bf16x3 split products, f32 exp, f64 eig/phases/sums)"}
 METRIC = "crystallite-steps/s + FIDs/s at ≤1e-5 rel L2 vs FP64 oracle; p50/p99 latency"
 # counter summaries of this benchmark's default command (FETCH_SIZE and WRITE_SIZE passes)
 COUNTER_SUMMARY = os.path.join(ROOT, "profiles", "config2_batched_{prec}_hbm_counters.txt")
@@ -261,7 +263,7 @@ def bench_serve(args):
         "metric": METRIC, "value": cs / span, "unit": "crystallite-steps/s", "n_gpus": 1,
         "steps": 1, "warmup": 1, "ms_per_step": span * 1e3, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "c64 (f32 products, f64 eig/phases/sums)",
+        "dtype": DTYPE[args.precision],
         "data": "synthetic",
         "config": {"workload": ("serving mix: Poisson 200 req/s, seeded 1H clusters of 2-"
                                 f"{args.max_spins} spins (log-uniform), MAS 5-25 kHz, ZCW 1154/3722, "
@@ -378,9 +380,10 @@ def main():
     ap.add_argument("--requests", type=int, default=48, help="requests of the serving mix")
     ap.add_argument("--max-spins", type=int, default=10, help="largest cluster of the serving mix")
     ap.add_argument("--sets", type=int, default=8, help="parameter sets per step (ptop)")
-    ap.add_argument("--precision", choices=["fp64", "c64"], default="c64",
+    ap.add_argument("--precision", choices=["fp64", "c64", "bf16x3"], default="c64",
                     help="complex64 propagators/products with FP64 eigen-solver, phases and "
-                         "reductions (solid-2.0's form), or complex128 throughout")
+                         "reductions (solid-2.0's form), complex128 throughout, or the complex64 "
+                         "form with its complex products as three bf16 products each")
     ap.add_argument("--no-fp64-form", action="store_true",
                     help="skip timing the FP64 form beside the c64 headline")
     args = ap.parse_args()
@@ -417,7 +420,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "c64 (f32 products, f64 eig/phases/sums)",
+        "dtype": DTYPE[args.precision],
         "data": "synthetic",
         "config": ({
             "workload": ("heteronuclear rf: 13C + 6 1H (N=128), MAS 12.5 kHz, magic angle, "
@@ -479,7 +482,7 @@ def main():
             "reference algorithm's count for this kernel (14 n^2 + 14 P flops per correlation "
             "element and crystallite: direct cyclic sum with a complex exp-multiply per term and "
             "the FID recurrence); the kernel executes it with FFTs")
-    if args.precision == "c64" and not args.no_fp64_form:
+    if args.precision != "fp64" and not args.no_fp64_form:
         # the plain FP64 form on the same workload (one timed step) and the c64 FIDs against it
         m64 = measure(H, wls, ptop, "fp64", 1, 1, args)
         errs = [float(np.linalg.norm(a - b) / np.linalg.norm(b))

@@ -47,4 +47,5 @@ struct DevPlan {
     int rt_from_q;         // 1: R(s,r) = Q_rev + conj(Q(r,s)), R is neither formed nor stored
     const int *elem_rev;   // Etot: element (s,r) of the reverse pair (J,I), or -1
     double *expflops;      // 8: executed propagator flops per tile variant (profiling), or NULL
+    int bf16x3;            // complex64 products as three bf16 matrix-core products (BLFID_C64_BF16X3)
 };

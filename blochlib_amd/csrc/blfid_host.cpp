@@ -169,6 +169,7 @@ struct blfid_plan {
     double wr = 0, rotor_deg = 0, sw = 0, t0 = 0;
     int npts = 0;
     int prec = 0;               // 0: complex128 storage, 1: complex64 storage (BLFID_C64)
+    int bf16x3 = 0;             // complex64 products as three bf16 products (BLFID_C64_BF16X3)
     bool spinning = false;
     bool ptop = false;          // point-to-point pulse-program FID (blfid_plan_create_ptop)
     // rf of a ptop plan: slot per pulsed isotope, n x nrfiso x (amp/2, phase, offset)
@@ -544,6 +545,7 @@ static void refresh_devplan(blfid_plan *pl) {
     d.rt_from_q = pl->rt_from_q ? 1 : 0;
     d.elem_rev = pl->d_erev.p;
     d.expflops = pl->prof ? pl->d_expflops.p : nullptr;
+    d.bf16x3 = pl->bf16x3;
     d.theta = PI2 * pl->tau;
     d.terms = pl->d_terms.p;
     d.ff_off = pl->d_ff_off.p;
@@ -590,8 +592,9 @@ static blfid_status check_system(const blfid_system *sys, int npts, blfid_precis
     if (!out) return fail(BLFID_ERR_INVALID, "out is NULL");
     *out = nullptr;
     if (!sys) return fail(BLFID_ERR_INVALID, "sys is NULL");
-    if (prec != BLFID_FP64 && prec != BLFID_C64)
-        return fail(BLFID_ERR_UNSUPPORTED, "precision not implemented (BLFID_FP64, BLFID_C64)");
+    if (prec != BLFID_FP64 && prec != BLFID_C64 && prec != BLFID_C64_BF16X3)
+        return fail(BLFID_ERR_UNSUPPORTED,
+                    "precision not implemented (BLFID_FP64, BLFID_C64, BLFID_C64_BF16X3)");
     if (sys->nspins < 1 || sys->nspins > DEV_MAX_SPINS)
         return fail(BLFID_ERR_INVALID, "nspins must be in [1, 12]");
     if (npts <= 0)
@@ -661,7 +664,8 @@ blfid_status blfid_plan_create(const blfid_system *sys, double wr_hz, double rot
     blfid_status cs = check_system(sys, npts, prec, out);
     if (cs != BLFID_OK) return cs;
     blfid_plan *pl = new_plan(sys, wr_hz, rotor_deg, npts, t0);
-    pl->prec = prec == BLFID_C64 ? 1 : 0;
+    pl->prec = prec == BLFID_FP64 ? 0 : 1;
+    pl->bf16x3 = prec == BLFID_C64_BF16X3;
     pl->sw = sw_hz;
     pl->spinning = !(wr_hz == 0.0 || rotor_deg == 0.0);
     if (pl->spinning) {
@@ -767,7 +771,8 @@ static blfid_status create_pulse_plan(const blfid_system *sys, double wr_hz, dou
             t[2] = off;
         }
     blfid_plan *pl = new_plan(sys, wr_hz, rotor_deg, npts, t0);
-    pl->prec = prec == BLFID_C64 ? 1 : 0;
+    pl->prec = prec == BLFID_FP64 ? 0 : 1;
+    pl->bf16x3 = prec == BLFID_C64_BF16X3;
     pl->ptop = true;
     pl->spinning = false;
     pl->n = steps;

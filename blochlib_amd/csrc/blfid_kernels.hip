@@ -27,6 +27,7 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <type_traits>
 
 #include "blfid_dev.h"
 
@@ -107,7 +108,13 @@ template <> struct Arith<cf> {
 // NT threads (256 or 64) form a 16 x (NT/16) grid; output tiles are 16M x 16M, thread (tx,ty)
 // owns rows tx + 16 i and columns ty + (NT/16) j.  K is walked in slices of 16 staged through
 // LDS with zero fill (unpadded sizes work); the next slice is prefetched into registers while
-// the current one is consumed.  C must not alias A or B.  `smem` holds 2*16*16M elements.
+// the current one is consumed.  C must not alias A or B.  `smem` holds GemmLds<M, T>::ELEMS
+// elements: two slices of 16 rows, each row padded from 16M to S elements so that the transposing
+// stores (16 consecutive lanes on 16 consecutive k, one column each) fall in distinct banks.
+template <int M, typename T> struct GemmLds {
+    static constexpr int S = 16 * M + 16 / (int)sizeof(T);
+    static constexpr int ELEMS = 2 * 16 * S;
+};
 template <int M, int NT, typename T>
 __device__ __forceinline__ void gemm_load_slice(int m, int n, int k, int i0, int j0, int k0,
                                                 const T *__restrict__ A, int lda, bool ca,
@@ -146,13 +153,14 @@ __device__ __forceinline__ void wg_gemm(int m, int n, int k, const T *__restrict
                         const T *__restrict__ B, int ldb, bool cb, T *__restrict__ C, int ldc,
                         T diag, T *smem) {
     constexpr int TM = 16 * M, KS = 16, TY = NT / 16, MJ = TM / TY, LPT = TM * KS / NT;
+    constexpr int S = GemmLds<M, T>::S;
     // register prefetch of the next K slice unless the accumulators already fill the VGPRs
     // (complex64 with 80 x 80 tiles would spill with it; complex128 with 64 x 64 tiles would
     // drop to one wave per SIMD)
     constexpr bool PF = (M * MJ * sizeof(T)) < 256 && !(sizeof(T) == 8 && alignof(T) == 8 && M >= 5 &&
                                                          !__is_same(T, double));
     T *As = smem;
-    T *Bs = smem + KS * TM;
+    T *Bs = smem + KS * S;
     const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
     for (int i0 = 0; i0 < m; i0 += TM) {
         for (int j0 = 0; j0 < n; j0 += TM) {
@@ -173,11 +181,11 @@ __device__ __forceinline__ void wg_gemm(int m, int n, int k, const T *__restrict
                         int r, q;
                         if (!ca) { r = idx % TM; q = idx / TM; }
                         else { q = idx % KS; r = idx / KS; }
-                        As[q * TM + r] = ra[l];
+                        As[q * S + r] = ra[l];
                         int c2, q2;
                         if (!cb) { q2 = idx % KS; c2 = idx / KS; }
                         else { c2 = idx % TM; q2 = idx / TM; }
-                        Bs[q2 * TM + c2] = rb[l];
+                        Bs[q2 * S + c2] = rb[l];
                     }
                 } else {
                     for (int idx = tid; idx < TM * KS; idx += NT) {
@@ -188,7 +196,7 @@ __device__ __forceinline__ void wg_gemm(int m, int n, int k, const T *__restrict
                         T v = Arith<T>::zero();
                         if (gr < m && gq < k)
                             v = ca ? Arith<T>::cj(A[gq + (size_t)gr * lda]) : A[gr + (size_t)gq * lda];
-                        As[q * TM + r] = v;
+                        As[q * S + r] = v;
                     }
                     for (int idx = tid; idx < TM * KS; idx += NT) {
                         int c, q;
@@ -198,7 +206,7 @@ __device__ __forceinline__ void wg_gemm(int m, int n, int k, const T *__restrict
                         T v = Arith<T>::zero();
                         if (gc < n && gq < k)
                             v = cb ? Arith<T>::cj(B[gc + (size_t)gq * ldb]) : B[gq + (size_t)gc * ldb];
-                        Bs[q * TM + c] = v;
+                        Bs[q * S + c] = v;
                     }
                 }
                 __syncthreads();
@@ -210,9 +218,9 @@ __device__ __forceinline__ void wg_gemm(int m, int n, int k, const T *__restrict
                 for (int q = 0; q < KS; q++) {
                     T a[M], b[MJ];
 #pragma unroll
-                    for (int i = 0; i < M; i++) a[i] = As[q * TM + tx + 16 * i];
+                    for (int i = 0; i < M; i++) a[i] = As[q * S + tx + 16 * i];
 #pragma unroll
-                    for (int j = 0; j < MJ; j++) b[j] = Bs[q * TM + ty + TY * j];
+                    for (int j = 0; j < MJ; j++) b[j] = Bs[q * S + ty + TY * j];
 #pragma unroll
                     for (int i = 0; i < M; i++)
 #pragma unroll
@@ -250,6 +258,22 @@ template <int M> struct NThr { static constexpr int v = (M == 1) ? 64 : 256; };
 // Ci += Ar Bi + Ai Br.  The accumulators of lane l hold rows 4 (l / 16) + j, column l % 16 of
 // its sub-tile.  `smem` holds 4 * 16 M * (KC + 4) floats.
 typedef float v4f __attribute__((ext_vector_type(4)));
+typedef short v4s __attribute__((ext_vector_type(4)));
+// bfloat16 split of four floats: hi = bf16(x) (round to nearest even), lo = bf16(x - hi), so
+// x = hi + lo to 16 significand bits.
+__device__ __forceinline__ unsigned short f2bf(float x) {
+    unsigned int u = __float_as_uint(x);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ void bf_split(v4f x, v4s &hi, v4s &lo) {
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        const unsigned short h = f2bf(x[u]);
+        hi[u] = (short)h;
+        lo[u] = (short)f2bf(x[u] - __uint_as_float((unsigned int)h << 16));
+    }
+}
 template <int NSUB, int M, int NW>
 __device__ __forceinline__ void mfma_store(int i0, int j0, int m, int n, int w, int lane,
                                            const v4f (&cr)[NSUB], const v4f (&ci)[NSUB],
@@ -269,7 +293,7 @@ __device__ __forceinline__ void mfma_store(int i0, int j0, int m, int n, int w, 
     }
 }
 
-template <int M, int NT, int KC>
+template <int M, int NT, int KC, bool SPLIT>
 __device__ __forceinline__ void wg_cgemm_mfma(int m, int n, int k, const cf *__restrict__ A,
                                               int lda, bool ca, const cf *__restrict__ B,
                                               int ldb, cf *__restrict__ C, int ldc,
@@ -303,8 +327,13 @@ __device__ __forceinline__ void wg_cgemm_mfma(int m, int n, int k, const cf *__r
                         Ai[i * LDK + q] = v.y;
                     }
                 } else {
+                    // A(i, q) is contiguous in i.  Each 16 lanes load 16 consecutive rows of one
+                    // column (a 128-byte segment) and a wave covers 4 columns, so the 64 stores
+                    // i * (KC + 4) + q of a wave spread over the banks (consecutive lanes on
+                    // consecutive rows of one column would share a quarter of them).
                     for (int idx = tid; idx < TM * kp; idx += NT) {
-                        const int i = idx % TM, q = idx / TM;
+                        const int rest = idx >> 6;
+                        const int i = (idx & 15) + 16 * (rest % M), q = ((idx >> 4) & 3) + 4 * (rest / M);
                         cf v = make_float2(0.0f, 0.0f);
                         if (i0 + i < m && q < kc) v = A[(i0 + i) + (size_t)(q0 + q) * lda];
                         Ar[i * LDK + q] = v.x;
@@ -329,13 +358,37 @@ __device__ __forceinline__ void wg_cgemm_mfma(int m, int n, int k, const cf *__r
                             const int ob = (16 * (t % M) + r) * LDK + qq + 4 * g;
                             const v4f ar = *(const v4f *)&Ar[oa], ai = *(const v4f *)&Ai[oa];
                             const v4f br = *(const v4f *)&Br[ob], bi = *(const v4f *)&Bi[ob];
-                            const v4f na = -ai;
+                            if constexpr (SPLIT) {
+                                // BF16x3: one 16x16x16 bf16 product covers the slice; each real
+                                // product is hi*hi + hi*lo + lo*hi of the split operands
+                                v4s arh, arl, aih, ail, brh, brl, bih, bil;
+                                bf_split(ar, arh, arl);
+                                bf_split(ai, aih, ail);
+                                bf_split(br, brh, brl);
+                                bf_split(bi, bih, bil);
+                                const v4s nsg = v4s{(short)0x8000, (short)0x8000, (short)0x8000, (short)0x8000};
+                                const v4s naih = aih ^ nsg, nail = ail ^ nsg;
+                                cr[s] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(arh, brh, cr[s], 0, 0, 0);
+                                cr[s] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(arh, brl, cr[s], 0, 0, 0);
+                                cr[s] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(arl, brh, cr[s], 0, 0, 0);
+                                cr[s] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(naih, bih, cr[s], 0, 0, 0);
+                                cr[s] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(naih, bil, cr[s], 0, 0, 0);
+                                cr[s] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(nail, bih, cr[s], 0, 0, 0);
+                                ci[s] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(arh, bih, ci[s], 0, 0, 0);
+                                ci[s] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(arh, bil, ci[s], 0, 0, 0);
+                                ci[s] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(arl, bih, ci[s], 0, 0, 0);
+                                ci[s] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(aih, brh, ci[s], 0, 0, 0);
+                                ci[s] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(aih, brl, ci[s], 0, 0, 0);
+                                ci[s] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ail, brh, ci[s], 0, 0, 0);
+                            } else {
+                                const v4f na = -ai;
 #pragma unroll
-                            for (int u = 0; u < 4; u++) {
-                                cr[s] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[u], br[u], cr[s], 0, 0, 0);
-                                ci[s] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[u], bi[u], ci[s], 0, 0, 0);
-                                cr[s] = __builtin_amdgcn_mfma_f32_16x16x4f32(na[u], bi[u], cr[s], 0, 0, 0);
-                                ci[s] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai[u], br[u], ci[s], 0, 0, 0);
+                                for (int u = 0; u < 4; u++) {
+                                    cr[s] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[u], br[u], cr[s], 0, 0, 0);
+                                    ci[s] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[u], bi[u], ci[s], 0, 0, 0);
+                                    cr[s] = __builtin_amdgcn_mfma_f32_16x16x4f32(na[u], bi[u], cr[s], 0, 0, 0);
+                                    ci[s] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai[u], br[u], ci[s], 0, 0, 0);
+                                }
                             }
                         }
                     }
@@ -348,28 +401,28 @@ __device__ __forceinline__ void wg_cgemm_mfma(int m, int n, int k, const cf *__r
 }
 
 // Complex products: matrix cores for complex64, the VALU tile GEMM for complex128.
-template <int M, int NT, typename R> struct CGemm {
+template <int M, int NT, typename R, bool SPLIT = false> struct CGemm {
     typedef typename CX<R>::T CT;
-    static constexpr int BYTES = 2 * 16 * 16 * M * (int)sizeof(CT);
+    static constexpr int BYTES = GemmLds<M, CT>::ELEMS * (int)sizeof(CT);
     __device__ __forceinline__ static void run(int m, int n, int k, const CT *A, int lda, bool ca, const CT *B,
                                int ldb, CT *C, int ldc, void *smem) {
         wg_gemm<M, NT, CT>(m, n, k, A, lda, ca, B, ldb, false, C, ldc, Arith<CT>::zero(),
                            (CT *)smem);
     }
 };
-template <int M, int NT> struct CGemm<M, NT, float> {
+template <int M, int NT, bool SPLIT> struct CGemm<M, NT, float, SPLIT> {
     static constexpr int KC = (M == 1) ? 16 : 32;
     static constexpr int BYTES = 4 * 16 * M * (KC + 4) * 4;
     __device__ __forceinline__ static void run(int m, int n, int k, const cf *A, int lda, bool ca, const cf *B,
                                int ldb, cf *C, int ldc, void *smem) {
-        wg_cgemm_mfma<M, NT, KC>(m, n, k, A, lda, ca, B, ldb, C, ldc, (float *)smem);
+        wg_cgemm_mfma<M, NT, KC, SPLIT>(m, n, k, A, lda, ca, B, ldb, C, ldc, (float *)smem);
     }
 };
 
 // Real products of the propagator (both precisions on the VALU tile GEMM: measured faster here
 // than matrix cores, the propagator being bound by its staging and synchronisation).
 template <int M, int NT, typename R> struct ExpGemm {
-    static constexpr int SMEM = 2 * 16 * 16 * M;
+    static constexpr int SMEM = GemmLds<M, R>::ELEMS;
     __device__ __forceinline__ static void run(int b, const R *A, const R *B, R *C, R *smem) {
         wg_gemm<M, NT, R>(b, b, b, A, b, false, B, b, false, C, b, (R)0, smem);
     }
@@ -706,14 +759,14 @@ __global__ void __launch_bounds__(NThr<M>::v) k_exp(DevPlan P, int Kb, int c0,
 // U *= X is U = X U in propogation.cc:273, 288).  Product k goes to slot k % akeep of the
 // crystallite's akeep slots: all of them for gamma-COMPUTE, two alternating ones when only
 // the period propagator A_n is needed.
-template <int M, typename R>
+template <int M, typename R, bool SPLIT = false>
 __global__ void __launch_bounds__(NThr<M>::v) k_dyson(DevPlan P, int Kb, const int *__restrict__ grp,
                                                       int ngrp,
                                                       const typename CX<R>::T *__restrict__ bufU,
                                                       typename CX<R>::T *__restrict__ bufA) {
     constexpr int NT = NThr<M>::v;
     typedef typename CX<R>::T CT;
-    __shared__ __align__(16) char smem[CGemm<M, NT, R>::BYTES];
+    __shared__ __align__(16) char smem[CGemm<M, NT, R, SPLIT>::BYTES];
     const long ntask = (long)Kb * ngrp;
     for (long task = blockIdx.x; task < ntask; task += gridDim.x) {
         const int g = (int)(task % ngrp), c = (int)(task / ngrp);
@@ -728,7 +781,7 @@ __global__ void __launch_bounds__(NThr<M>::v) k_dyson(DevPlan P, int Kb, const i
             const CT *Uk = bufU + base + (size_t)k * P.Lsum;
             const CT *Ap = bufA + abase + (size_t)((k - 1) % P.akeep) * P.Lsum;
             CT *Ak = bufA + abase + (size_t)(k % P.akeep) * P.Lsum;
-            CGemm<M, NT, R>::run(b, b, b, Uk, b, false, Ap, b, Ak, b, smem);
+            CGemm<M, NT, R, SPLIT>::run(b, b, b, Uk, b, false, Ap, b, Ak, b, smem);
         }
     }
 }
@@ -894,7 +947,7 @@ __global__ void __launch_bounds__(NT) k_eig(DevPlan P, int Kb, const int *__rest
 
 // ------------------------------------------------------------------------------ k_bmul
 // B_k = A_k W for every (crystallite, step, block) of a group; writes into bufB.
-template <int M, typename R>
+template <int M, typename R, bool SPLIT = false>
 __global__ void __launch_bounds__(NThr<M>::v) k_bmul(DevPlan P, int Kb, const int *__restrict__ grp,
                                                      int ngrp,
                                                      const typename CX<R>::T *__restrict__ bufA,
@@ -902,7 +955,7 @@ __global__ void __launch_bounds__(NThr<M>::v) k_bmul(DevPlan P, int Kb, const in
                                                      typename CX<R>::T *__restrict__ bufB) {
     constexpr int NT = NThr<M>::v;
     typedef typename CX<R>::T CT;
-    __shared__ __align__(16) char smem[CGemm<M, NT, R>::BYTES];
+    __shared__ __align__(16) char smem[CGemm<M, NT, R, SPLIT>::BYTES];
     const long ntask = (long)Kb * P.n * ngrp;
     for (long task = blockIdx.x; task < ntask; task += gridDim.x) {
         const int g = (int)(task % ngrp);
@@ -911,7 +964,7 @@ __global__ void __launch_bounds__(NThr<M>::v) k_bmul(DevPlan P, int Kb, const in
         const int blk = grp[g], b = P.blk_dim[blk];
         const size_t off = ((size_t)c * P.n + k) * P.Lsum + P.blk_uoff[blk];
         const CT *W = bufW + (size_t)c * P.Lsum + P.blk_uoff[blk];
-        CGemm<M, NT, R>::run(b, b, b, bufA + off, b, false, W, b, bufB + off, b, smem);
+        CGemm<M, NT, R, SPLIT>::run(b, b, b, bufA + off, b, false, W, b, bufB + off, b, smem);
     }
 }
 
@@ -966,7 +1019,7 @@ __device__ __forceinline__ void sparse_rows_times(const int *__restrict__ rp,
 //   R(s,r) = sum conj(B_J(b,s)) rho_JI(b,a) B_I(a,r)
 // (rho_eff = det + det^H when rho == (det+det^H)/2, compute.h:381-382).  Stored per element
 // e = r + s*b_I at offset pair_eoff, layout [c][k][e]: Qs = Q(r,s), Rs = R(s,r).
-template <int M, typename R>
+template <int M, typename R, bool SPLIT = false>
 __global__ void __launch_bounds__(NThr<M>::v) k_qr(DevPlan P, int Kb, const int *__restrict__ pgrp,
                                                    int ngrp,
                                                    const typename CX<R>::T *__restrict__ bufB,
@@ -975,7 +1028,7 @@ __global__ void __launch_bounds__(NThr<M>::v) k_qr(DevPlan P, int Kb, const int 
                                                    typename CX<R>::T *__restrict__ Rt) {
     constexpr int NT = NThr<M>::v;
     typedef typename CX<R>::T CT;
-    __shared__ __align__(16) char smem[CGemm<M, NT, R>::BYTES];
+    __shared__ __align__(16) char smem[CGemm<M, NT, R, SPLIT>::BYTES];
     const int tid = threadIdx.x;
     CT *T1 = scratch + slot * 2 * blockIdx.x;
     CT *G = T1 + slot;
@@ -991,14 +1044,14 @@ __global__ void __launch_bounds__(NThr<M>::v) k_qr(DevPlan P, int Kb, const int 
         const CT *BJ = bufB + ((size_t)c * P.n + k) * P.Lsum + P.blk_uoff[J];
         const size_t eo = ((size_t)c * P.n + k) * P.Etot + P.pair_eoff[pr];
         sparse_rows_times<NT, R>(P.d_rowptr + P.pair_doff[pr], P.d_col, (const cd *)P.d_val, bI,
-                                 bJ, BJ, bJ, T1, smem, CGemm<M, NT, R>::BYTES);
-        CGemm<M, NT, R>::run(bI, bJ, bI, BI, bI, true, T1, bI, Qt + eo, bI, smem);
+                                 bJ, BJ, bJ, T1, smem, CGemm<M, NT, R, SPLIT>::BYTES);
+        CGemm<M, NT, R, SPLIT>::run(bI, bJ, bI, BI, bI, true, T1, bI, Qt + eo, bI, smem);
         if (P.rt_from_q) continue; // R is formed from Q in k_corr
         sparse_rows_times<NT, R>(P.r_rowptr + P.pair_roff[pr], P.r_col, (const cd *)P.r_val, bJ,
-                                 bI, BI, bI, T1, smem, CGemm<M, NT, R>::BYTES);
+                                 bI, BI, bI, T1, smem, CGemm<M, NT, R, SPLIT>::BYTES);
         // R^T(r,s) = sum_b T1(b,r) conj(B_J(b,s)) = (T1^T conj(B_J))(r,s): computed as
         // G = B_J^H T1 (bJ x bI) then transposed into Rt
-        CGemm<M, NT, R>::run(bJ, bI, bJ, BJ, bJ, true, T1, bJ, G, bJ, smem);
+        CGemm<M, NT, R, SPLIT>::run(bJ, bI, bJ, BJ, bJ, true, T1, bJ, G, bJ, smem);
         for (int e = tid; e < bI * bJ; e += NT) {
             int r = e % bI, s = e / bI;
             Rt[eo + e] = G[s + (size_t)r * bJ];
@@ -1432,6 +1485,14 @@ __global__ void __launch_bounds__(NThr<M>::v) k_static_fid(DevPlan P, int Kb, co
 }
 
 // ------------------------------------------------------------------------------ launchers
+// Complex64 plans with BF16x3 products launch the SPLIT instance of the product kernels.
+#define LAUNCH_SPLIT(kern, M_, R_, grid_, nt_, st_, ...)                                         \
+    do {                                                                                         \
+        if (std::is_same<R_, float>::value && P->bf16x3)                                         \
+            kern<M_, R_, std::is_same<R_, float>::value><<<grid_, nt_, 0, st_>>>(__VA_ARGS__);   \
+        else                                                                                     \
+            kern<M_, R_, false><<<grid_, nt_, 0, st_>>>(__VA_ARGS__);                            \
+    } while (0)
 // prec: 0 = FP64 storage (double2), 1 = complex64 storage (float2) of U, A, B, Q, R and the exp
 // scratch; the eigen-solver, phases, FIDs and reductions are FP64 in both.
 #define GRID_CAP 4096
@@ -1469,11 +1530,11 @@ static void launch_dyson(int M, const DevPlan *P, int Kb, const int *grp, int ng
     const CT *U = (const CT *)bufU;
     CT *A = (CT *)bufA;
     switch (M) {
-    case 1: k_dyson<1, R><<<grid, 64, 0, st>>>(*P, Kb, grp, ngrp, U, A); break;
-    case 2: k_dyson<2, R><<<grid, 256, 0, st>>>(*P, Kb, grp, ngrp, U, A); break;
-    case 3: k_dyson<3, R><<<grid, 256, 0, st>>>(*P, Kb, grp, ngrp, U, A); break;
-    case 4: k_dyson<4, R><<<grid, 256, 0, st>>>(*P, Kb, grp, ngrp, U, A); break;
-    default: k_dyson<5, R><<<grid, 256, 0, st>>>(*P, Kb, grp, ngrp, U, A); break;
+    case 1: LAUNCH_SPLIT(k_dyson, 1, R, grid, 64, st, *P, Kb, grp, ngrp, U, A); break;
+    case 2: LAUNCH_SPLIT(k_dyson, 2, R, grid, 256, st, *P, Kb, grp, ngrp, U, A); break;
+    case 3: LAUNCH_SPLIT(k_dyson, 3, R, grid, 256, st, *P, Kb, grp, ngrp, U, A); break;
+    case 4: LAUNCH_SPLIT(k_dyson, 4, R, grid, 256, st, *P, Kb, grp, ngrp, U, A); break;
+    default: LAUNCH_SPLIT(k_dyson, 5, R, grid, 256, st, *P, Kb, grp, ngrp, U, A); break;
     }
 }
 
@@ -1528,11 +1589,11 @@ static void launch_bmul(int M, const DevPlan *P, int Kb, const int *grp, int ngr
     const CT *A = (const CT *)bufA, *W = (const CT *)bufW;
     CT *B = (CT *)bufB;
     switch (M) {
-    case 1: k_bmul<1, R><<<grid, 64, 0, st>>>(*P, Kb, grp, ngrp, A, W, B); break;
-    case 2: k_bmul<2, R><<<grid, 256, 0, st>>>(*P, Kb, grp, ngrp, A, W, B); break;
-    case 3: k_bmul<3, R><<<grid, 256, 0, st>>>(*P, Kb, grp, ngrp, A, W, B); break;
-    case 4: k_bmul<4, R><<<grid, 256, 0, st>>>(*P, Kb, grp, ngrp, A, W, B); break;
-    default: k_bmul<5, R><<<grid, 256, 0, st>>>(*P, Kb, grp, ngrp, A, W, B); break;
+    case 1: LAUNCH_SPLIT(k_bmul, 1, R, grid, 64, st, *P, Kb, grp, ngrp, A, W, B); break;
+    case 2: LAUNCH_SPLIT(k_bmul, 2, R, grid, 256, st, *P, Kb, grp, ngrp, A, W, B); break;
+    case 3: LAUNCH_SPLIT(k_bmul, 3, R, grid, 256, st, *P, Kb, grp, ngrp, A, W, B); break;
+    case 4: LAUNCH_SPLIT(k_bmul, 4, R, grid, 256, st, *P, Kb, grp, ngrp, A, W, B); break;
+    default: LAUNCH_SPLIT(k_bmul, 5, R, grid, 256, st, *P, Kb, grp, ngrp, A, W, B); break;
     }
 }
 
@@ -1556,11 +1617,11 @@ static void launch_qr(int M, const DevPlan *P, int Kb, const int *pgrp, int ngrp
     const CT *B = (const CT *)bufB;
     CT *sc = (CT *)scratch, *Q = (CT *)Qt, *Rr = (CT *)Rt;
     switch (M) {
-    case 1: k_qr<1, R><<<grid, 64, 0, st>>>(*P, Kb, pgrp, ngrp, B, sc, slot, Q, Rr); break;
-    case 2: k_qr<2, R><<<grid, 256, 0, st>>>(*P, Kb, pgrp, ngrp, B, sc, slot, Q, Rr); break;
-    case 3: k_qr<3, R><<<grid, 256, 0, st>>>(*P, Kb, pgrp, ngrp, B, sc, slot, Q, Rr); break;
-    case 4: k_qr<4, R><<<grid, 256, 0, st>>>(*P, Kb, pgrp, ngrp, B, sc, slot, Q, Rr); break;
-    default: k_qr<5, R><<<grid, 256, 0, st>>>(*P, Kb, pgrp, ngrp, B, sc, slot, Q, Rr); break;
+    case 1: LAUNCH_SPLIT(k_qr, 1, R, grid, 64, st, *P, Kb, pgrp, ngrp, B, sc, slot, Q, Rr); break;
+    case 2: LAUNCH_SPLIT(k_qr, 2, R, grid, 256, st, *P, Kb, pgrp, ngrp, B, sc, slot, Q, Rr); break;
+    case 3: LAUNCH_SPLIT(k_qr, 3, R, grid, 256, st, *P, Kb, pgrp, ngrp, B, sc, slot, Q, Rr); break;
+    case 4: LAUNCH_SPLIT(k_qr, 4, R, grid, 256, st, *P, Kb, pgrp, ngrp, B, sc, slot, Q, Rr); break;
+    default: LAUNCH_SPLIT(k_qr, 5, R, grid, 256, st, *P, Kb, pgrp, ngrp, B, sc, slot, Q, Rr); break;
     }
 }
 

@@ -31,8 +31,9 @@ LIBPATH = os.path.join(HERE, "libblfid.so")
 BLFID_OK = 0
 STATUS = {0: "OK", 1: "INVALID", 2: "NOMEM", 3: "DEVICE", 4: "UNSUPPORTED", 5: "NOCONV"}
 # blfid_precision: complex128 throughout, or complex64 propagators / products / correlation sums
-# with the eigen-solver, phases and reductions in FP64
-PRECISION = {"fp64": 0, "c128": 0, "c64": 1}
+# with the eigen-solver, phases and reductions in FP64; "bf16x3" is the complex64 form with its
+# complex products as three bf16 matrix-core products each (hi*hi + hi*lo + lo*hi)
+PRECISION = {"fp64": 0, "c128": 0, "c64": 1, "bf16x3": 2}
 
 _d = ctypes.POINTER(ctypes.c_double)
 _i = ctypes.POINTER(ctypes.c_int)
@@ -149,7 +150,7 @@ class Plan:
                  t0: float = 0.0, *, precision: str = "fp64", _ptop=None):
         L = lib()
         if precision not in PRECISION:
-            raise BlochLibError(f"unknown precision {precision!r} (fp64, c64)")
+            raise BlochLibError(f"unknown precision {precision!r} (fp64, c64, bf16x3)")
         prec = PRECISION[precision]
         iso, csa, dip, jc = sy.tables()
         self._keep = [np.ascontiguousarray(iso, np.int32),

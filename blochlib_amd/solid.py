@@ -19,7 +19,7 @@ from .spinsys import BlochLibError
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(prog="python -m blochlib_amd.solid")
     ap.add_argument("simfile")
-    ap.add_argument("--precision", choices=["fp64", "c64"], default="fp64")
+    ap.add_argument("--precision", choices=["fp64", "c64", "bf16x3"], default="fp64")
     ap.add_argument("--outdir", default=".")
     args = ap.parse_args(argv)
     try:

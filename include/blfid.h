@@ -45,7 +45,12 @@ typedef enum {
                      * the complex products run on FP32 matrix cores with FP32 accumulation; the
                      * period eigen-solver, phases, FID points and the powder reduction stay
                      * FP64.  The static path always runs in FP64. */
-    BLFID_C64_MFMA = BLFID_C64
+    BLFID_C64_MFMA = BLFID_C64,
+    BLFID_C64_BF16X3 = 2 /* as BLFID_C64, but each FP32 operand of the complex products (Dyson,
+                          * B = A W, Q) is split into bf16 hi + lo parts and every real product
+                          * is hi*hi + hi*lo + lo*hi on bf16 matrix cores with FP32 accumulation
+                          * (about 16 mantissa bits per product).  No reference counterpart: an
+                          * opt-in speed/accuracy trade with a looser error than BLFID_C64. */
 } blfid_precision;
 
 /* Tensor-described SolidSys (src/QMspins/sys.h:59-62, read by sys.cc:120-204).
