@@ -54,10 +54,17 @@ COUNTER_SUMMARY = os.path.join(ROOT, "profiles", "config2_batched_{prec}_hbm_cou
 
 def corr_flops_per_elem(n: int, P: int) -> float:
     """Executed flops of k_corr per correlation element and crystallite (as the engine counts
-    them): FFT correlation for n a power of two up to 512, else the direct sum, plus twiddle
-    staging and the FID recurrence."""
-    fft = n >= 2 and (n & (n - 1)) == 0 and n <= 512
-    c = 15.0 * n * np.log2(n) + 8.0 * n if fft else 8.0 * n * n
+    them): for n a power of two from 16 to 512 three mixed-radix FFTs (radix-8 passes of a
+    56-flop DFT and 7 twiddle products per butterfly, the last pass of radix 2, 4 or 8 without
+    twiddles) and the scaled product, else the direct sum; plus twiddle staging and the FID
+    sums."""
+    c = 8.0 * n * n
+    if 16 <= n <= 512 and (n & (n - 1)) == 0:
+        ln = int(n).bit_length() - 1
+        npass = (ln + 2) // 3
+        rl = 1 << (ln - 3 * (npass - 1))
+        dft = {2: 4.0, 4: 16.0, 8: 56.0}[rl]
+        c = 3.0 * (npass - 1) * (n / 8.0) * 98.0 + 3.0 * (n / rl) * dft + 8.0 * n
     return c + 18.0 * n + 14.0 * P
 
 

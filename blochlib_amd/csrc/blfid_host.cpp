@@ -1006,9 +1006,18 @@ static double pair_flops(const blfid_plan *pl, const std::vector<int> &g) {
 // with the 1/n scale (8 per step); otherwise the direct cyclic sum, 8 n^2.  Plus the twiddle
 // staging (18 per step) and the FID recurrence (14 per point: complex multiply-add and the
 // z^t update).
+// Executed flops of the correlation per element: FFT variant (n a power of two, 16..512) as
+// 3 transforms of np = ceil(log2(n) / 3) passes, radix 8 (a 56-flop DFT and 7 twiddle products
+// per butterfly) but the last, of radix rl without twiddles (DFT of 4, 16 or 56 flops), plus the
+// scaled product (8 per point); direct variant 8 n^2; plus twiddle staging (18 n) and the FID
+// sums (14 per point).
 static double corr_flops_per_elem(int n, int P) {
-    const bool fft = n >= 2 && (n & (n - 1)) == 0 && n <= 512;
-    const double c = fft ? 15.0 * n * std::log2((double)n) + 8.0 * n : 8.0 * (double)n * n;
+    double c = 8.0 * (double)n * n;
+    if (n >= 16 && (n & (n - 1)) == 0 && n <= 512) {
+        const int ln = __builtin_ctz((unsigned)n), np = (ln + 2) / 3, rl = 1 << (ln - 3 * (np - 1));
+        const double dft = rl == 2 ? 4.0 : rl == 4 ? 16.0 : 56.0;
+        c = 3.0 * (np - 1) * (n / 8.0) * 98.0 + 3.0 * ((double)n / rl) * dft + 8.0 * n;
+    }
     return c + 18.0 * n + 14.0 * P;
 }
 

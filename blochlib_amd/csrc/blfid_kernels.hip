@@ -1068,14 +1068,14 @@ __global__ void __launch_bounds__(NThr<M>::v) k_qr(DevPlan P, int Kb, const int 
 // each thread walking a run of k with one sincos and complex multiplies.  Thread (el, pg)
 // then computes F_p(el) for p in [8 pg, 8 pg + 8):
 //   F_p(e) = sum_i Rt_i(e) Qt_{(i+p)%n}(e)
-// For n a power of two (n <= 2 CORR_TW) the correlation is formed with FFTs instead:
+// For n a power of two (16 <= n <= 2 CORR_TW) the correlation is formed with FFTs instead:
 //   F = IDFT( Rtilde . DFT(Qt) ),  Rtilde_k = sum_i Rt_i e^{+2 pi i ik/n},
-// both forward transforms decimation-in-frequency (natural order in, bit-reversed out), the
-// product taken in that order and the inverse decimation-in-time (bit-reversed in, natural
-// out), so no permutation pass is needed: 3 (n/2) log2 n butterflies instead of n^2 products.
-// F replaces Qt in LDS and thread t forms fid[t] = sum_e F_{t%n}(e) z_e^t (compute.h:432-457),
-// z_e^t advanced by z_e^256, summed over the segment's chunks and added atomically to
-// fid_c[c][t] (per chunk when npts > 2048).
+// as in-place mixed-radix passes in LDS (radix 8, the last pass radix 2, 4 or 8): staging and
+// the twiddles fuse into the first forward pass, the last forward pass, the product and the first
+// inverse pass into one, and G_p = F_p z^p into the last inverse pass (corr_fwd_pass and the
+// passes after it).  Thread t then forms fid[t] = sum_e F_{t%n}(e) z_e^t (compute.h:432-457)
+// from G and a per-chunk table of z_e^{n j}, summed over the segment's chunks in registers and
+// added atomically to fid_c[c][t] (per chunk when the FID is too long to hold).
 // CH = min(CORR_SPAN / n, 256 / ceil(n/8)) is computed identically on the host.
 #define CORR_TW 256
 #define CORR_SPAN 2048
@@ -1094,13 +1094,173 @@ __device__ __host__ inline int corr_chunk(int n) {
     return ch < 1 ? 1 : ch;
 }
 __host__ __device__ inline bool corr_use_fft(int n) {
-    return n >= 2 && (n & (n - 1)) == 0 && n <= 2 * CORR_TW;
+    return n >= 16 && (n & (n - 1)) == 0 && n <= 2 * CORR_TW;
 }
-// NPT FID points per thread are held over a segment (npts <= 256 NPT).  The complex64 FFT
-// variant with four points is held to four waves per SIMD (four workgroups per CU with its 40 KB
-// of LDS); the others are bound by LDS or registers at two.
-template <typename R, bool FFT, int NPT>
-__global__ void __launch_bounds__(256, (FFT && sizeof(R) == 4 && NPT == 4) ? 4 : 2) k_corr(DevPlan P, int Kb,
+
+// In-register DFT of RAD = 2, 4 or 8 points, X_k = sum_m x_m e^{SG 2 pi i m k / RAD}, natural
+// order in and out (radix 8 as two radix-4 halves joined by e^{SG i pi k / 4}).
+template <int SG, typename V> __device__ __forceinline__ V mul_si(V a) { // a * (SG i)
+    V r;
+    r.x = SG > 0 ? -a.y : a.y;
+    r.y = SG > 0 ? a.x : -a.x;
+    return r;
+}
+template <int RAD, int SG, typename R> struct RegDFT;
+template <int SG, typename R> struct RegDFT<2, SG, R> {
+    typedef typename CX<R>::T V;
+    static __device__ __forceinline__ void run(V (&x)[2]) {
+        const V a = x[0];
+        x[0] = cadd(a, x[1]);
+        x[1] = csub(a, x[1]);
+    }
+};
+template <int SG, typename R> struct RegDFT<4, SG, R> {
+    typedef typename CX<R>::T V;
+    static __device__ __forceinline__ void run(V (&x)[4]) {
+        const V a = cadd(x[0], x[2]), b = csub(x[0], x[2]);
+        const V c = cadd(x[1], x[3]), d = mul_si<SG>(csub(x[1], x[3]));
+        x[0] = cadd(a, c);
+        x[2] = csub(a, c);
+        x[1] = cadd(b, d);
+        x[3] = csub(b, d);
+    }
+};
+template <int SG, typename R> struct RegDFT<8, SG, R> {
+    typedef typename CX<R>::T V;
+    static __device__ __forceinline__ void run(V (&x)[8]) {
+        V e[4] = {x[0], x[2], x[4], x[6]}, o[4] = {x[1], x[3], x[5], x[7]};
+        RegDFT<4, SG, R>::run(e);
+        RegDFT<4, SG, R>::run(o);
+        const R h = (R)0.70710678118654752440;
+        V t1, t3; // o_1 (1 + SG i) h and o_3 (-1 + SG i) h
+        t1.x = h * (o[1].x - SG * o[1].y);
+        t1.y = h * (o[1].y + SG * o[1].x);
+        t3.x = h * (-o[3].x - SG * o[3].y);
+        t3.y = h * (-o[3].y + SG * o[3].x);
+        const V t2 = mul_si<SG>(o[2]);
+        x[0] = cadd(e[0], o[0]);
+        x[4] = csub(e[0], o[0]);
+        x[1] = cadd(e[1], t1);
+        x[5] = csub(e[1], t1);
+        x[2] = cadd(e[2], t2);
+        x[6] = csub(e[2], t2);
+        x[3] = cadd(e[3], t3);
+        x[7] = csub(e[3], t3);
+    }
+};
+
+// Compile-time layout of the FFT variant for n = 2^LN (16 <= n <= 512): chunks of CH = 2048 / n
+// elements, LDS slot of (step k, element el) = k (CH + 1) + el (rows padded by one element so that
+// one element read across consecutive rows, as the FID sums do, falls in distinct banks).
+// tw holds e^{-2 pi i j / n}, j < n / 2.
+template <int LN> struct CorrFft {
+    static constexpr int N = 1 << LN, CH = CORR_SPAN >> LN, LC = 11 - LN, ROW = CH + 1;
+    static constexpr int NP = (LN + 2) / 3;         // passes per transform: radix 8 but the last
+    static constexpr int LRL = LN - 3 * (NP - 1);   // log2 of the last pass's radix
+    static __device__ __forceinline__ int at(int k, int el) { return k * ROW + el; }
+    template <int SG, typename CT> static __device__ __forceinline__ CT twid(const CT *tw, int idx) {
+        CT w = tw[idx & (N / 2 - 1)]; // e^{SG 2 pi i idx / n}, 0 <= idx < n
+        if (idx & (N / 2)) { w.x = -w.x; w.y = -w.y; }
+        if (SG > 0) w.y = -w.y;
+        return w;
+    }
+};
+
+// Mixed-radix in-place FFTs of the correlation F = IDFT(DFT+(Rt) . DFT-(Qt)): the forward
+// transforms are decimation-in-frequency passes of radix 8 (the last of radix 2^LRL), output in
+// digit-reversed order; the inverse runs the exact inverse passes in reverse order, so it returns
+// natural order with no permutation.  A pass over sub-transforms of length L = 2^LL and radix RAD
+// takes, for butterfly j < L / RAD of block blk, the RAD points at blk L + j + m L / RAD, applies
+// the RAD-point DFT, multiplies output k by e^{SG 2 pi i j k / L} and stores it at
+// blk L + j + k L / RAD; the inverse pass multiplies by the conjugate twiddle first and applies
+// the opposite-sign DFT.  One work item = one butterfly of one element (2048 / RAD items).
+template <int RAD> struct Log2R { static constexpr int v = RAD == 8 ? 3 : RAD == 4 ? 2 : 1; };
+template <int RAD, int LN, int LL, typename R>
+__device__ __forceinline__ void corr_fwd_pass(typename CX<R>::T *sQ, typename CX<R>::T *sR,
+                                              const typename CX<R>::T *tw) {
+    typedef typename CX<R>::T CT;
+    typedef CorrFft<LN> F;
+    constexpr int LR = Log2R<RAD>::v, LS = LL - LR, TSH = LN - LL;
+#pragma unroll
+    for (int it = 0; it < (CORR_SPAN >> LR) / 256; it++) {
+        const int idx = threadIdx.x + 256 * it;
+        const int el = idx & (F::CH - 1), b = idx >> F::LC;
+        const int j = b & ((1 << LS) - 1), base = F::at(((b >> LS) << LL) + j, el);
+        CT q[RAD], r[RAD];
+#pragma unroll
+        for (int m = 0; m < RAD; m++) {
+            q[m] = sQ[base + ((m << LS) * F::ROW)];
+            r[m] = sR[base + ((m << LS) * F::ROW)];
+        }
+        RegDFT<RAD, -1, R>::run(q);
+        RegDFT<RAD, +1, R>::run(r);
+#pragma unroll
+        for (int k = 0; k < RAD; k++) {
+            if (k) {
+                const CT w = F::template twid<-1>(tw, (j * k) << TSH);
+                q[k] = cmul(q[k], w);
+                r[k] = cmul(r[k], cconj(w));
+            }
+            sQ[base + ((k << LS) * F::ROW)] = q[k];
+            sR[base + ((k << LS) * F::ROW)] = r[k];
+        }
+    }
+}
+template <int RAD, int LN, int LL, typename R>
+__device__ __forceinline__ void corr_inv_pass(typename CX<R>::T *sQ, const typename CX<R>::T *tw) {
+    typedef typename CX<R>::T CT;
+    typedef CorrFft<LN> F;
+    constexpr int LR = Log2R<RAD>::v, LS = LL - LR, TSH = LN - LL;
+#pragma unroll
+    for (int it = 0; it < (CORR_SPAN >> LR) / 256; it++) {
+        const int idx = threadIdx.x + 256 * it;
+        const int el = idx & (F::CH - 1), b = idx >> F::LC;
+        const int j = b & ((1 << LS) - 1), base = F::at(((b >> LS) << LL) + j, el);
+        CT x[RAD];
+#pragma unroll
+        for (int k = 0; k < RAD; k++) {
+            x[k] = sQ[base + ((k << LS) * F::ROW)];
+            if (k) x[k] = cmul(x[k], F::template twid<+1>(tw, (j * k) << TSH));
+        }
+        RegDFT<RAD, +1, R>::run(x);
+#pragma unroll
+        for (int m = 0; m < RAD; m++) sQ[base + ((m << LS) * F::ROW)] = x[m];
+    }
+}
+// The last forward pass (sub-transforms of length RAD: no twiddles), the product scaled by 1/n
+// and the first inverse pass, on consecutive groups of RAD points, into sQ.
+template <int LN, typename R>
+__device__ __forceinline__ void corr_mid_pass(typename CX<R>::T *sQ, const typename CX<R>::T *sR) {
+    typedef typename CX<R>::T CT;
+    typedef CorrFft<LN> F;
+    constexpr int LR = F::LRL, RAD = 1 << LR;
+    const R inv_n = (R)(1.0 / (double)F::N);
+#pragma unroll
+    for (int it = 0; it < (CORR_SPAN >> LR) / 256; it++) {
+        const int idx = threadIdx.x + 256 * it;
+        const int el = idx & (F::CH - 1), base = F::at((idx >> F::LC) << LR, el);
+        CT q[RAD], r[RAD];
+#pragma unroll
+        for (int m = 0; m < RAD; m++) {
+            q[m] = sQ[base + m * F::ROW];
+            r[m] = sR[base + m * F::ROW];
+        }
+        RegDFT<RAD, -1, R>::run(q);
+        RegDFT<RAD, +1, R>::run(r);
+#pragma unroll
+        for (int m = 0; m < RAD; m++) q[m] = cscale(cmul(q[m], r[m]), inv_n);
+        RegDFT<RAD, +1, R>::run(q);
+#pragma unroll
+        for (int m = 0; m < RAD; m++) sQ[base + m * F::ROW] = q[m];
+    }
+}
+
+// NPT FID points per thread are held over a segment (npts <= 256 NPT).  LN > 0 selects the FFT
+// variant for n = 2^LN (CorrFft<LN>), LN = 0 the direct sum for any n.  The complex64 FFT variant
+// with four points is held to four waves per SIMD (four workgroups per CU with its <= 40 KB of
+// LDS); the others are bound by LDS or registers at two.
+template <typename R, int LN, int NPT>
+__global__ void __launch_bounds__(256, (LN > 0 && sizeof(R) == 4 && NPT == 4) ? 4 : 2) k_corr(DevPlan P, int Kb,
                                               const typename CX<R>::T *__restrict__ Qt,
                                               const typename CX<R>::T *__restrict__ Rt,
                                               const double *__restrict__ thetas,
@@ -1108,24 +1268,28 @@ __global__ void __launch_bounds__(256, (FFT && sizeof(R) == 4 && NPT == 4) ? 4 :
                                               const int *__restrict__ elem_s,
                                               double *__restrict__ fidc) {
     typedef typename CX<R>::T CT;
-    __shared__ CT sQ[CORR_SPAN];
-    __shared__ __align__(16) CT sR[CORR_SPAN];
-    __shared__ double sphi[256];
-    __shared__ cd sstep[256];
-    __shared__ CT s_tw[CORR_TW];
-    const int n = P.n, tid = threadIdx.x;
-    const int CH = corr_chunk(n);
-    // LDS slot of (step k, element el): rows of CH elements, the element index XOR-ed with a
-    // function of the row when CH is a power of two, so that one element read across
-    // consecutive rows (the FID sums) falls in distinct banks
+    constexpr bool FFT = LN > 0;
+    typedef CorrFft<FFT ? LN : 4> FF;
+    constexpr int SPAN = FFT ? FF::N * FF::ROW : CORR_SPAN, NE = FFT ? FF::CH : 256;
+    __shared__ CT sQ[SPAN];
+    __shared__ __align__(16) CT sR[SPAN];
+    __shared__ double sphi[NE];
+    __shared__ cd sstep[NE];
+    __shared__ CT s_tw[FFT ? FF::N / 2 : 1];
+    const int n = FFT ? FF::N : P.n, tid = threadIdx.x;
+    const int CH = FFT ? FF::CH : corr_chunk(n);
+    // LDS slot of (step k, element el).  Direct variant: rows of CH elements, the element index
+    // XOR-ed with a function of the row when CH is a power of two, so that one element read
+    // across consecutive rows (the FID sums) falls in distinct banks; FFT variant: padded rows.
     const int swz = (CH & (CH - 1)) == 0 ? CH - 1 : 0;
     constexpr int ssh = sizeof(CT) == 8 ? 1 : 0;
-    auto at = [=](int k, int el) { return k * CH + (el ^ ((k >> ssh) & swz)); };
-    constexpr bool use_fft = FFT; // launched only where corr_use_fft(n)
-    if (use_fft) {
+    auto at = [=](int k, int el) {
+        return FFT ? FF::at(k, el) : k * CH + (el ^ ((k >> ssh) & swz));
+    };
+    if constexpr (FFT) {
         // e^{-2 pi i j / n}, j < n/2
-        for (int j = tid; j < n / 2; j += 256)
-            s_tw[j] = CX<R>::from(cexpi(-2.0 * 3.14159265358979323846 * (double)j / (double)n));
+        for (int j = tid; j < FF::N / 2; j += 256)
+            s_tw[j] = CX<R>::from(cexpi(-2.0 * 3.14159265358979323846 * (double)j / (double)FF::N));
         __syncthreads();
     }
     const int npg = (n + CORR_PPT - 1) / CORR_PPT;
@@ -1158,89 +1322,152 @@ __global__ void __launch_bounds__(256, (FFT && sizeof(R) == 4 && NPT == 4) ? 4 :
         for (int m = 0; m < NPT; m++) hacc[m] = cmk(0, 0);
         for (int ch = sg * spc; ch < ch_end; ch++) {
             const long e0 = (long)ch * CH;
-            for (int el = tid; el < CH; el += 256) {
-                long e = e0 + el;
-                double ph = 0.0;
-                if (e < P.Etot) {
-                    const double *th = thetas + (size_t)c * P.N;
-                    ph = (th[elem_r[e]] - th[elem_s[e]]) / (double)n;
-                }
-                sphi[el] = ph;
-                sstep[el] = cexpi(256.0 * ph);
-            }
-            __syncthreads();
-            {
-                const int el = tid % CH, seg = tid / CH;
-                const long e = e0 + el;
-                const int ka = seg * kpt, kb = min(n, ka + kpt);
-                if (seg < nseg && ka < kb) {
-                    const double ph = sphi[el];
-                    cd w = cexpi(-(double)ka * ph);   // z^-k
-                    const cd z = cexpi(ph);
-                    const cd zi = cconj(z);
-                    for (int k = ka; k < kb; k++) {
-                        cd q = cmk(0, 0), r = cmk(0, 0);
-                        if (e < P.Etot) {
-                            size_t o = ((size_t)c * n + k) * P.Etot + e;
-                            q = to_cd(Qt[o]);
+            if constexpr (FFT) {
+                // staging fused with the first forward pass (radix 8, L = n): item (el, j) loads
+                // steps k = j + m n/8 of element e0 + el, applies the gamma-COMPUTE twiddles z^-k
+                // (Q) and z^k (R) in FP64 and transforms both
+                constexpr int LS = LN - 3;
+                const int el = tid & (FF::CH - 1), j = tid >> FF::LC;
+                const int base = FF::at(j, el);
+                {
+                    const long e = e0 + el;
+                    const bool ok = e < P.Etot;
+                    double ph = 0.0;
+                    int rv = -1;
+                    if (ok) {
+                        const double *th = thetas + (size_t)c * P.N;
+                        ph = (th[elem_r[e]] - th[elem_s[e]]) / (double)FF::N;
+                        if (P.rt_from_q) rv = P.elem_rev[e];
+                    }
+                    if (j == 0) {
+                        sphi[el] = ph;
+                        sstep[el] = cexpi(256.0 * ph);
+                    }
+                    const cd w0 = cexpi(-(double)j * ph);                // z^-k
+                    const cd st = cexpi(-(double)(1 << LS) * ph);
+                    const size_t str = (size_t)(1 << LS) * P.Etot;
+                    const CT *qb = Qt + ((size_t)c * FF::N + j) * P.Etot;
+                    CT x[8];
+                    cd w = w0;
+#pragma unroll
+                    for (int m = 0; m < 8; m++) {
+                        const CT qv = ok ? qb[(size_t)m * str + e] : Arith<CT>::zero();
+                        x[m] = CX<R>::from(cmul(to_cd(qv), w));
+                        w = cmul(w, st);
+                    }
+                    RegDFT<8, -1, R>::run(x);
+#pragma unroll
+                    for (int k = 0; k < 8; k++) {
+                        if (k) x[k] = cmul(x[k], FF::template twid<-1>(s_tw, j * k));
+                        sQ[base + (k << LS) * FF::ROW] = x[k];
+                    }
+                    w = cconj(w0);
+                    const cd sti = cconj(st);
+                    const CT *rb = Rt + ((size_t)c * FF::N + j) * P.Etot;
+#pragma unroll
+                    for (int m = 0; m < 8; m++) {
+                        cd rr = cmk(0, 0);
+                        if (ok) {
                             if (P.rt_from_q) {
                                 // R(s,r) = Q_(J,I)(s,r) + conj(Q_(I,J)(r,s))
-                                r = cconj(q);
-                                const int rv = P.elem_rev[e];
-                                if (rv >= 0) r = cadd(r, to_cd(Qt[((size_t)c * n + k) * P.Etot + rv]));
+                                rr = cconj(to_cd(qb[(size_t)m * str + e]));
+                                if (rv >= 0) rr = cadd(rr, to_cd(qb[(size_t)m * str + rv]));
                             } else {
-                                r = to_cd(Rt[o]);
+                                rr = to_cd(rb[(size_t)m * str + e]);
                             }
                         }
-                        sQ[at(k, el)] = CX<R>::from(cmul(q, w));
-                        sR[at(k, el)] = CX<R>::from(cmul(r, cconj(w)));
-                        w = cmul(w, zi);
+                        x[m] = CX<R>::from(cmul(rr, w));
+                        w = cmul(w, sti);
+                    }
+                    RegDFT<8, +1, R>::run(x);
+#pragma unroll
+                    for (int k = 0; k < 8; k++) {
+                        if (k) x[k] = cmul(x[k], FF::template twid<+1>(s_tw, j * k));
+                        sR[base + (k << LS) * FF::ROW] = x[k];
                     }
                 }
-            }
-            __syncthreads();
-            if (use_fft) {
-                // n and CH are powers of two here: butterfly indexes by shifts and masks
-                const int nbf = (n >> 1) * CH;
-                const int lc = __ffs(CH) - 1, ln = __ffs(n) - 1;
-                // forward transforms of Qt (e^-) and Rt (e^+), decimation in frequency
-                for (int lh = ln - 1; lh >= 0; lh--) {
-                    const int half = 1 << lh, ls = ln - 1 - lh; // twiddle stride n / (2 half)
-                    for (int idx = tid; idx < 2 * nbf; idx += 256) {
-                        const bool isR = idx >= nbf;
-                        const int id = isR ? idx - nbf : idx;
-                        const int el = id & (CH - 1), bf = id >> lc;
-                        const int j = bf & (half - 1), g = bf >> lh;
-                        const int r0 = (g << (lh + 1)) + j;
-                        const int i0 = at(r0, el), i1 = at(r0 + half, el);
-                        CT *x = isR ? sR : sQ;
-                        CT w = s_tw[j << ls];
-                        if (isR) w.y = -w.y;
-                        const CT a = x[i0], bq = x[i1];
-                        x[i0] = cadd(a, bq);
-                        x[i1] = cmul(csub(a, bq), w);
-                    }
-                    __syncthreads();
-                }
-                const R inv_n = (R)(1.0 / (double)n);
-                for (int idx = tid; idx < n * CH; idx += 256) sQ[idx] = cscale(cmul(sR[idx], sQ[idx]), inv_n);
                 __syncthreads();
-                // inverse transform (e^+), decimation in time
-                for (int lh = 0; lh < ln; lh++) {
-                    const int half = 1 << lh, ls = ln - 1 - lh;
-                    for (int idx = tid; idx < nbf; idx += 256) {
-                        const int el = idx & (CH - 1), bf = idx >> lc;
-                        const int j = bf & (half - 1), g = bf >> lh;
-                        const int r0 = (g << (lh + 1)) + j;
-                        const int i0 = at(r0, el), i1 = at(r0 + half, el);
-                        const CT w = cconj(s_tw[j << ls]);
-                        const CT a = sQ[i0], t = cmul(sQ[i1], w);
-                        sQ[i0] = cadd(a, t);
-                        sQ[i1] = csub(a, t);
-                    }
+                if constexpr (FF::NP == 3) {
+                    corr_fwd_pass<8, LN, LN - 3, R>(sQ, sR, s_tw);
                     __syncthreads();
                 }
+                corr_mid_pass<LN, R>(sQ, sR);
+                __syncthreads();
+                if constexpr (FF::NP == 3) {
+                    corr_inv_pass<8, LN, LN - 3, R>(sQ, s_tw);
+                    __syncthreads();
+                }
+                // last inverse pass (radix 8, L = n) giving F_p in natural order; for the held FID
+                // sums G_p = F_p z^p is stored instead, and the period table V_j(e) = z_e^(n j)
+                // goes to the second buffer (free since the product)
+                {
+                    CT x[8];
+#pragma unroll
+                    for (int k = 0; k < 8; k++) {
+                        x[k] = sQ[base + (k << LS) * FF::ROW];
+                        if (k) x[k] = cmul(x[k], FF::template twid<+1>(s_tw, j * k));
+                    }
+                    RegDFT<8, +1, R>::run(x);
+                    if (hold) {
+                        const double ph = sphi[el];
+                        cd w = cexpi((double)j * ph);
+                        const cd st = cexpi((double)(1 << LS) * ph);
+#pragma unroll
+                        for (int m = 0; m < 8; m++) {
+                            sQ[base + (m << LS) * FF::ROW] = CX<R>::from(cmul(to_cd(x[m]), w));
+                            w = cmul(w, st);
+                        }
+                    } else {
+#pragma unroll
+                        for (int m = 0; m < 8; m++) sQ[base + (m << LS) * FF::ROW] = x[m];
+                    }
+                }
+                if (hold)
+                    for (int q = tid; q < FF::CH * J; q += 256)
+                        sV[q] = cexpi((double)FF::N * (double)(q % J) * sphi[q / J]);
+                __syncthreads();
             } else {
+                for (int el = tid; el < CH; el += 256) {
+                    long e = e0 + el;
+                    double ph = 0.0;
+                    if (e < P.Etot) {
+                        const double *th = thetas + (size_t)c * P.N;
+                        ph = (th[elem_r[e]] - th[elem_s[e]]) / (double)n;
+                    }
+                    sphi[el] = ph;
+                    sstep[el] = cexpi(256.0 * ph);
+                }
+                __syncthreads();
+                {
+                    const int el = tid % CH, seg = tid / CH;
+                    const long e = e0 + el;
+                    const int ka = seg * kpt, kb = min(n, ka + kpt);
+                    if (seg < nseg && ka < kb) {
+                        const double ph = sphi[el];
+                        cd w = cexpi(-(double)ka * ph);   // z^-k
+                        const cd z = cexpi(ph);
+                        const cd zi = cconj(z);
+                        for (int k = ka; k < kb; k++) {
+                            cd q = cmk(0, 0), r = cmk(0, 0);
+                            if (e < P.Etot) {
+                                size_t o = ((size_t)c * n + k) * P.Etot + e;
+                                q = to_cd(Qt[o]);
+                                if (P.rt_from_q) {
+                                    // R(s,r) = Q_(J,I)(s,r) + conj(Q_(I,J)(r,s))
+                                    r = cconj(q);
+                                    const int rv = P.elem_rev[e];
+                                    if (rv >= 0) r = cadd(r, to_cd(Qt[((size_t)c * n + k) * P.Etot + rv]));
+                                } else {
+                                    r = to_cd(Rt[o]);
+                                }
+                            }
+                            sQ[at(k, el)] = CX<R>::from(cmul(q, w));
+                            sR[at(k, el)] = CX<R>::from(cmul(r, cconj(w)));
+                            w = cmul(w, zi);
+                        }
+                    }
+                }
+                __syncthreads();
                 CT F[CORR_PPT];
                 const int el = tid % CH, pg = tid / CH;
                 const int p0 = pg * CORR_PPT;
@@ -1282,30 +1509,13 @@ __global__ void __launch_bounds__(256, (FFT && sizeof(R) == 4 && NPT == 4) ? 4 :
                 __syncthreads();
             }
             if (FFT && hold) {
-                {
-                    const int el = tid % CH, seg = tid / CH;
-                    const int ka = seg * kpt, kb = min(n, ka + kpt);
-                    if (seg < nseg && ka < kb) {
-                        const double ph = sphi[el];
-                        cd w = cexpi((double)ka * ph); // z^p
-                        const cd z = cexpi(ph);
-                        for (int k = ka; k < kb; k++) {
-                            sQ[at(k, el)] = CX<R>::from(cmul(to_cd(sQ[at(k, el)]), w));
-                            w = cmul(w, z);
-                        }
-                    }
-                }
-                for (int q = tid; q < CH * J; q += 256)
-                    sV[q] = cexpi((double)n * (double)(q % J) * sphi[q / J]);
-                __syncthreads();
                 const int ne = (int)min((long)CH, P.Etot - e0);
-                const int ln = __ffs(n) - 1;
                 for (int e = 0; e < ne; e++) {
 #pragma unroll
                     for (int m = 0; m < NPT; m++) {
                         if (m < nth) {
                             const int t = tid + 256 * m;
-                            cfma(hacc[m], to_cd(sQ[at(t & (n - 1), e)]), sV[e * J + (t >> ln)]);
+                            cfma(hacc[m], to_cd(sQ[at(t & (n - 1), e)]), sV[e * J + (t >> LN)]);
                         }
                     }
                 }
@@ -1636,16 +1846,28 @@ extern "C" int blfid_dev_launch_qr(int prec, int M, const DevPlan *P, int Kb, co
     return (int)hipGetLastError();
 }
 
-template <int NPT>
-static void launch_corr(int prec, bool fft, int grid, const DevPlan *P, int Kb, const void *Qt,
-                        const void *Rt, const double *thetas, const int *er, const int *es,
-                        double *fidc, hipStream_t st) {
-    if (prec) {
-        if (fft) k_corr<float, true, NPT><<<grid, 256, 0, st>>>(*P, Kb, (const cf *)Qt, (const cf *)Rt, thetas, er, es, fidc);
-        else k_corr<float, false, NPT><<<grid, 256, 0, st>>>(*P, Kb, (const cf *)Qt, (const cf *)Rt, thetas, er, es, fidc);
-    } else {
-        if (fft) k_corr<double, true, NPT><<<grid, 256, 0, st>>>(*P, Kb, (const cd *)Qt, (const cd *)Rt, thetas, er, es, fidc);
-        else k_corr<double, false, NPT><<<grid, 256, 0, st>>>(*P, Kb, (const cd *)Qt, (const cd *)Rt, thetas, er, es, fidc);
+template <typename R, int LN, int NPT>
+static void launch_corr_t(int grid, const DevPlan *P, int Kb, const void *Qt, const void *Rt,
+                          const double *thetas, const int *er, const int *es, double *fidc,
+                          hipStream_t st) {
+    typedef typename CX<R>::T CT;
+    k_corr<R, LN, NPT><<<grid, 256, 0, st>>>(*P, Kb, (const CT *)Qt, (const CT *)Rt, thetas, er,
+                                              es, fidc);
+}
+// FFT variants by log2 n (corr_use_fft(n): 16 <= n <= 512), LN = 0 the direct sum
+template <typename R, int NPT>
+static void launch_corr(int grid, const DevPlan *P, int Kb, const void *Qt, const void *Rt,
+                        const double *thetas, const int *er, const int *es, double *fidc,
+                        hipStream_t st) {
+    const int ln = corr_use_fft(P->n) ? __builtin_ctz((unsigned)P->n) : 0;
+    switch (ln) {
+    case 4: launch_corr_t<R, 4, NPT>(grid, P, Kb, Qt, Rt, thetas, er, es, fidc, st); break;
+    case 5: launch_corr_t<R, 5, NPT>(grid, P, Kb, Qt, Rt, thetas, er, es, fidc, st); break;
+    case 6: launch_corr_t<R, 6, NPT>(grid, P, Kb, Qt, Rt, thetas, er, es, fidc, st); break;
+    case 7: launch_corr_t<R, 7, NPT>(grid, P, Kb, Qt, Rt, thetas, er, es, fidc, st); break;
+    case 8: launch_corr_t<R, 8, NPT>(grid, P, Kb, Qt, Rt, thetas, er, es, fidc, st); break;
+    case 9: launch_corr_t<R, 9, NPT>(grid, P, Kb, Qt, Rt, thetas, er, es, fidc, st); break;
+    default: launch_corr_t<R, 0, NPT>(grid, P, Kb, Qt, Rt, thetas, er, es, fidc, st); break;
     }
 }
 
@@ -1656,12 +1878,16 @@ extern "C" int blfid_dev_launch_corr(int prec, const DevPlan *P, int Kb, const v
     int nchunk = (int)((P->Etot + CH - 1) / CH);
     int spc = corr_seg_chunks(nchunk);
     long ntask = (long)Kb * ((nchunk + spc - 1) / spc);
-    const bool fft = corr_use_fft(P->n);
     int grid = (int)(ntask < GRID_CAP ? ntask : GRID_CAP);
     if (grid <= 0) return 0;
     // four points per thread up to 1024 FID points, eight above
-    if (P->npts <= 1024) launch_corr<4>(prec, fft, grid, P, Kb, Qt, Rt, thetas, er, es, fidc, st);
-    else launch_corr<8>(prec, fft, grid, P, Kb, Qt, Rt, thetas, er, es, fidc, st);
+    if (P->npts <= 1024) {
+        if (prec) launch_corr<float, 4>(grid, P, Kb, Qt, Rt, thetas, er, es, fidc, st);
+        else launch_corr<double, 4>(grid, P, Kb, Qt, Rt, thetas, er, es, fidc, st);
+    } else {
+        if (prec) launch_corr<float, 8>(grid, P, Kb, Qt, Rt, thetas, er, es, fidc, st);
+        else launch_corr<double, 8>(grid, P, Kb, Qt, Rt, thetas, er, es, fidc, st);
+    }
     return (int)hipGetLastError();
 }
 

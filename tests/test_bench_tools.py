@@ -62,7 +62,14 @@ def test_profiled_traffic_calibration(tmp_path, monkeypatch):
 
 def test_corr_flop_count():
     n, P = 128, 1024
-    assert bench.corr_flops_per_elem(n, P) == 15 * n * 7 + 8 * n + 18 * n + 14 * P
+    # passes 8, 8, 2: per transform 2 x 16 radix-8 butterflies (56 + 7 x 6 flops) and 64
+    # radix-2 ones (4 flops), three transforms, the scaled product 8 flops per point
+    fft = 3 * (2 * 16 * (56 + 42) + 64 * 4) + 8 * n
+    assert bench.corr_flops_per_elem(n, P) == fft + 18 * n + 14 * P
+    # n = 64: passes 8, 8 (the last of radix 8 without twiddles)
+    assert bench.corr_flops_per_elem(64, P) == 3 * (8 * 98 + 8 * 56) + 8 * 64 + 18 * 64 + 14 * P
+    # below 16 points the direct sum
+    assert bench.corr_flops_per_elem(8, P) == 8 * 64 + 18 * 8 + 14 * P
     # not a power of two: the direct cyclic sum
     assert bench.corr_flops_per_elem(96, P) == 8 * 96 * 96 + 18 * 96 + 14 * P
     assert np.isclose(bench.corr_flops_per_elem(1024, P), 8 * 1024.0 ** 2 + 18 * 1024 + 14 * P)

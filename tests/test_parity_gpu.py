@@ -185,3 +185,26 @@ def test_rosym_detector_variants(det_name):
     pw = zcw(1154).subset(np.arange(0, 1154, 151))
     compare(f"rosym-{det_name}", sy, 15000.0, MAGIC, 15000.0 * 16, 48, pw, rho, det,
             normalize=False)
+
+
+@pytest.mark.parametrize("n", [16, 32, 64, 128, 256, 512])
+@pytest.mark.parametrize("npts", [40, 300])
+def test_correlation_fft_sizes(n, npts):
+    """Every FFT size of the correlation (mixed-radix passes 8+2, 8+4, 8+8, 8+8+2, 8+8+4,
+    8+8+8) with the FID sums held in registers (40 points) and, where the period table does not
+    fit, accumulated per chunk (300 points); general rho (R from its own products) and rosym rho
+    (R formed from Q), FP64 and complex64."""
+    sy = SolidSys.from_text(SYS3)
+    pw = zcw(1154).subset(np.array([5, 400, 777, 1100]))
+    wr = 10000.0
+    for rho_name in ("Iz+0.3*Ix_0", "Ix"):
+        compare(f"fft-n{n}-p{npts}-{rho_name}", sy, wr, MAGIC, wr * n, npts, pw,
+                sy.op(rho_name), sy.op("Ip"), normalize=False)
+        print(f"[parity] fft-n{n}-p{npts}-{rho_name}: complex64", flush=True)
+        g, _ = engine.powder_fid(sy, wr, MAGIC, wr * n, npts, pw, sy.op(rho_name), sy.op("Ip"),
+                                 normalize=False, precision="c64")
+        c, _ = O.powder_fid(sy, wr, MAGIC, wr * n, npts, pw, sy.op(rho_name), sy.op("Ip"),
+                            normalize=False)
+        err = rel_l2(g, c)
+        print(f"[parity] fft-n{n}-p{npts}-{rho_name}: complex64 rel-L2 {err:.3e}", flush=True)
+        assert err < 1e-5
