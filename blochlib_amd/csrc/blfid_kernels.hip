@@ -1257,10 +1257,11 @@ __device__ __forceinline__ void corr_mid_pass(typename CX<R>::T *sQ, const typen
 
 // NPT FID points per thread are held over a segment (npts <= 256 NPT).  LN > 0 selects the FFT
 // variant for n = 2^LN (CorrFft<LN>), LN = 0 the direct sum for any n.  The complex64 FFT variant
-// with four points is held to four waves per SIMD (four workgroups per CU with its <= 40 KB of
-// LDS); the others are bound by LDS or registers at two.
+// with four points is held to three waves per SIMD (168 registers: at four it spilled, and its
+// sixteen loads per chunk then waited on the spill reloads); the others are bound by LDS or
+// registers at two.
 template <typename R, int LN, int NPT>
-__global__ void __launch_bounds__(256, (LN > 0 && sizeof(R) == 4 && NPT == 4) ? 4 : 2) k_corr(DevPlan P, int Kb,
+__global__ void __launch_bounds__(256, (LN > 0 && sizeof(R) == 4 && NPT == 4) ? 3 : 2) k_corr(DevPlan P, int Kb,
                                               const typename CX<R>::T *__restrict__ Qt,
                                               const typename CX<R>::T *__restrict__ Rt,
                                               const double *__restrict__ thetas,
@@ -1330,29 +1331,40 @@ __global__ void __launch_bounds__(256, (LN > 0 && sizeof(R) == 4 && NPT == 4) ? 
                 const int el = tid & (FF::CH - 1), j = tid >> FF::LC;
                 const int base = FF::at(j, el);
                 {
+                    // all loads are unconditional (indexes clamped in range, masked afterwards)
+                    // so that the sixteen loads of Q and R issue back to back
                     const long e = e0 + el;
                     const bool ok = e < P.Etot;
-                    double ph = 0.0;
-                    int rv = -1;
-                    if (ok) {
-                        const double *th = thetas + (size_t)c * P.N;
-                        ph = (th[elem_r[e]] - th[elem_s[e]]) / (double)FF::N;
-                        if (P.rt_from_q) rv = P.elem_rev[e];
-                    }
+                    const long ec = ok ? e : 0;
+                    const int er_ = elem_r[ec], es_ = elem_s[ec];
+                    const int rv = P.rt_from_q ? P.elem_rev[ec] : -1;
+                    const double *th = thetas + (size_t)c * P.N;
+                    const size_t str = (size_t)(1 << LS) * P.Etot;
+                    const CT *qb = Qt + ((size_t)c * FF::N + j) * P.Etot;
+                    const CT *rb = P.rt_from_q ? qb : Rt + ((size_t)c * FF::N + j) * P.Etot;
+                    const long rc = P.rt_from_q ? (rv >= 0 ? (long)rv : ec) : ec;
+                    const double th_r = th[er_], th_s = th[es_];
+                    CT qv[8], rw[8];
+#pragma unroll
+                    for (int m = 0; m < 8; m++) qv[m] = qb[(size_t)m * str + ec];
+#pragma unroll
+                    for (int m = 0; m < 8; m++) rw[m] = rb[(size_t)m * str + rc];
+                    const double ph = ok ? (th_r - th_s) / (double)FF::N : 0.0;
                     if (j == 0) {
                         sphi[el] = ph;
                         sstep[el] = cexpi(256.0 * ph);
                     }
+                    // Q: okf masks a padded element; R = conj(Q) + reverse Q (rosym, reverse
+                    // masked by rvf when the element has none) or the stored R
+                    const double okf = ok ? 1.0 : 0.0;
+                    const double rvf = (ok && P.rt_from_q && rv >= 0) ? 1.0 : 0.0;
                     const cd w0 = cexpi(-(double)j * ph);                // z^-k
                     const cd st = cexpi(-(double)(1 << LS) * ph);
-                    const size_t str = (size_t)(1 << LS) * P.Etot;
-                    const CT *qb = Qt + ((size_t)c * FF::N + j) * P.Etot;
                     CT x[8];
                     cd w = w0;
 #pragma unroll
                     for (int m = 0; m < 8; m++) {
-                        const CT qv = ok ? qb[(size_t)m * str + e] : Arith<CT>::zero();
-                        x[m] = CX<R>::from(cmul(to_cd(qv), w));
+                        x[m] = CX<R>::from(cmul(cscale(to_cd(qv[m]), okf), w));
                         w = cmul(w, st);
                     }
                     RegDFT<8, -1, R>::run(x);
@@ -1363,19 +1375,13 @@ __global__ void __launch_bounds__(256, (LN > 0 && sizeof(R) == 4 && NPT == 4) ? 
                     }
                     w = cconj(w0);
                     const cd sti = cconj(st);
-                    const CT *rb = Rt + ((size_t)c * FF::N + j) * P.Etot;
 #pragma unroll
                     for (int m = 0; m < 8; m++) {
-                        cd rr = cmk(0, 0);
-                        if (ok) {
-                            if (P.rt_from_q) {
-                                // R(s,r) = Q_(J,I)(s,r) + conj(Q_(I,J)(r,s))
-                                rr = cconj(to_cd(qb[(size_t)m * str + e]));
-                                if (rv >= 0) rr = cadd(rr, to_cd(qb[(size_t)m * str + rv]));
-                            } else {
-                                rr = to_cd(rb[(size_t)m * str + e]);
-                            }
-                        }
+                        cd rr;
+                        if (P.rt_from_q) // R(s,r) = Q_(J,I)(s,r) + conj(Q_(I,J)(r,s))
+                            rr = cadd(cscale(cconj(to_cd(qv[m])), okf), cscale(to_cd(rw[m]), rvf));
+                        else
+                            rr = cscale(to_cd(rw[m]), okf);
                         x[m] = CX<R>::from(cmul(rr, w));
                         w = cmul(w, sti);
                     }
