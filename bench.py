@@ -482,6 +482,26 @@ def main():
         out["roofline"]["achieved_hbm_gbs"] = traffic["bytes"] / (avg_ms * 1e-3) / 1e9
     if dom == "k_corr":
         n, P = wls[0].steps, wls[0].npts
+        # algorithmic HBM bytes: every step of every correlation element is read twice (Q, and
+        # the reverse element of Q or the stored R), in the storage precision
+        elems = d["flops"] / d["launches"] / corr_flops_per_elem(n, P)
+        alg_bytes = elems * n * (32.0 if args.precision == "fp64" else 16.0)
+        if d["flops"] / d["launches"] / alg_bytes < peak * 1e12 / (HBM_PEAK_GBS * 1e9):
+            # below the ridge point: HBM bound
+            out["roofline"].update({
+                "bound": "hbm", "achieved": alg_bytes / (avg_ms * 1e-3) / 1e9,
+                "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": alg_bytes / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                "bytes_per_launch": alg_bytes, "achieved_tflops": achieved,
+                "flop_peak_tflops": peak,
+                "note": ("achieved = algorithmic HBM bytes per launch (each step of each "
+                         "correlation element read twice: Q and the reverse element or R, "
+                         + ("32" if args.precision == "fp64" else "16") + " B per element-step) "
+                         "/ the average launch time (HIP events on the launch stream); the "
+                         "kernel's executed flops per algorithmic byte are below MI355X's ridge "
+                         "point, so HBM (8 TB/s) is its roof; achieved_tflops = executed flops "
+                         "rate"),
+            })
         ratio = (14.0 * n * n + 14.0 * P) / corr_flops_per_elem(n, P)
         out["roofline"]["canonical_flops_per_launch"] = ratio * d["flops"] / d["launches"]
         out["roofline"]["canonical_tflops"] = ratio * achieved
