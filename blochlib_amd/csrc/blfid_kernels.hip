@@ -597,8 +597,10 @@ template <typename R> struct ExpTol;
 template <> struct ExpTol<double> { static constexpr double tol = 1e-17, rmax = 0.8; };
 template <> struct ExpTol<float> { static constexpr double tol = 3e-9, rmax = 2.0; };
 
+// The 16 x 16 complex64 variant is held to four waves per SIMD (128 registers, no spills; three
+// without the bound).
 template <int M, typename R>
-__global__ void __launch_bounds__(NThr<M>::v) k_exp(DevPlan P, int Kb, int c0,
+__global__ void __launch_bounds__(NThr<M>::v, (M == 1 && sizeof(R) == 4) ? 4 : 1) k_exp(DevPlan P, int Kb, int c0,
                                                     const int *__restrict__ grp, int ngrp,
                                                     R *__restrict__ scratch, size_t slot,
                                                     typename CX<R>::T *__restrict__ bufU) {
@@ -814,7 +816,13 @@ __global__ void __launch_bounds__(NT) k_eig(DevPlan P, int Kb, const int *__rest
     constexpr double tol_off = __is_same(R, double) ? 1e-17 : 1e-12;
     constexpr bool BIG = MB == 0;
     constexpr int MP = BIG ? EIG_MAXPAIR : (MB + 1) / 2;
+    // complex64 eigenvectors of the largest LDS class accumulate in LDS (FP32) when only the
+    // storage-precision copy is wanted (the spinning path): the rotations then never wait on
+    // global memory.  The FP64 copy W is not written in that case.
+    constexpr bool WLDS = MB == EIG_MAXB && __is_same(R, float);
     __shared__ cd Mlds[BIG ? 1 : MB * (MB + 1)];
+    __shared__ cf Wl[WLDS ? MB * MB : 1];
+    const bool wl = WLDS && bufWr != nullptr;
     __shared__ double rc[MP];
     __shared__ cd rs[MP];
     __shared__ int rp[MP], rq[MP];
@@ -831,7 +839,8 @@ __global__ void __launch_bounds__(NT) k_eig(DevPlan P, int Kb, const int *__rest
         for (int e = tid; e < b * b; e += NT) {
             int r = e % b, q = e / b;
             Ms[r + q * ld] = to_cd(S0[e]);
-            W[e] = cmk(r == q ? 1.0 : 0.0, 0.0);
+            if (wl) Wl[e] = make_float2(r == q ? 1.0f : 0.0f, 0.0f);
+            else W[e] = cmk(r == q ? 1.0 : 0.0, 0.0);
         }
         __syncthreads();
         const int bb = (b + 1) & ~1; // even count, index b is a dummy when b is odd
@@ -921,9 +930,17 @@ __global__ void __launch_bounds__(NT) k_eig(DevPlan P, int Kb, const int *__rest
                     if (q >= b) continue;
                     double cth = rc[pi];
                     cd sth = rs[pi];
-                    cd wp = W[row + (size_t)p * b], wq = W[row + (size_t)q * b];
-                    W[row + (size_t)p * b] = cadd(cscale(wp, cth), cmul(sth, wq));
-                    W[row + (size_t)q * b] = csub(cscale(wq, cth), cmul(cconj(sth), wp));
+                    if (wl) {
+                        const float cf_ = (float)cth;
+                        const cf sf = make_float2((float)sth.x, (float)sth.y);
+                        const cf wp = Wl[row + p * b], wq = Wl[row + q * b];
+                        Wl[row + p * b] = cadd(cscale(wp, cf_), cmul(sf, wq));
+                        Wl[row + q * b] = csub(cscale(wq, cf_), cmul(cconj(sf), wp));
+                    } else {
+                        cd wp = W[row + (size_t)p * b], wq = W[row + (size_t)q * b];
+                        W[row + (size_t)p * b] = cadd(cscale(wp, cth), cmul(sth, wq));
+                        W[row + (size_t)q * b] = csub(cscale(wq, cth), cmul(cconj(sth), wp));
+                    }
                 }
                 __syncthreads();
             }
@@ -939,7 +956,14 @@ __global__ void __launch_bounds__(NT) k_eig(DevPlan P, int Kb, const int *__rest
         }
         if (bufWr) {
             CT *Wr = bufWr + (size_t)c * P.Lsum + P.blk_uoff[blk];
-            for (int e = tid; e < b * b; e += NT) Wr[e] = CX<R>::from(W[e]);
+            if constexpr (WLDS) {
+                if (wl)
+                    for (int e = tid; e < b * b; e += NT) Wr[e] = Wl[e];
+                else
+                    for (int e = tid; e < b * b; e += NT) Wr[e] = CX<R>::from(W[e]);
+            } else {
+                for (int e = tid; e < b * b; e += NT) Wr[e] = CX<R>::from(W[e]);
+            }
         }
         __syncthreads();
     }
@@ -970,9 +994,11 @@ __global__ void __launch_bounds__(NThr<M>::v) k_bmul(DevPlan P, int Kb, const in
 
 // T[a + s rows] = sum_z val[z] B[col[z] + s ldb] for a < rows, s < ncols: the CSR rows
 // rp[0..rows] (absolute offsets into col / val) of one block pair of a sparse operator times
-// a dense block.  The rows' columns and values are first staged in `smem` (free until the
-// GEMM that follows) when they fit its `cap` bytes, so the gathers of B are the only global
-// loads of the product; larger pairs read them from global memory.  Ends with a barrier.
+// a dense block B (ldb x ncols, contiguous).  The rows' columns and values, and B itself when
+// it fits too, are first staged in `smem` (free until the GEMM that follows, `cap` bytes): the
+// gathers of B then read LDS instead of scattered global addresses.  When B does not fit it is
+// gathered from global memory; when the rows do not fit either, they are read from global
+// memory as well.  Ends with a barrier.
 template <int NT, typename R>
 __device__ __forceinline__ void sparse_rows_times(const int *__restrict__ rp,
                                                   const int *__restrict__ col,
@@ -984,22 +1010,37 @@ __device__ __forceinline__ void sparse_rows_times(const int *__restrict__ rp,
     const int tid = threadIdx.x;
     const int z0 = rp[0], nnz = rp[rows] - z0;
     const int off = ((rows + 1) * 4 + 15) & ~15;
+    const size_t boff = ((size_t)off + (size_t)nnz * (sizeof(CT) + 4) + 15) & ~(size_t)15;
+    const size_t nb = (size_t)ldb * ncols;
     if (off + (size_t)nnz * (sizeof(CT) + 4) <= (size_t)cap) {
         int *s_rp = (int *)smem;
         CT *s_v = (CT *)(smem + off);
         int *s_c = (int *)(s_v + nnz);
+        const bool bl = boff + nb * sizeof(CT) <= (size_t)cap;
+        CT *s_b = (CT *)(smem + boff);
         for (int i = tid; i <= rows; i += NT) s_rp[i] = rp[i] - z0;
         for (int i = tid; i < nnz; i += NT) {
             s_c[i] = col[z0 + i];
             s_v[i] = CX<R>::from(val[z0 + i]);
         }
+        if (bl)
+            for (int i = tid; i < (int)nb; i += NT) s_b[i] = B[i];
         __syncthreads();
-        for (int e = tid; e < rows * ncols; e += NT) {
-            const int a = e % rows, s = e / rows;
-            CT acc = Arith<CT>::zero();
-            for (int z = s_rp[a]; z < s_rp[a + 1]; z++)
-                cfma(acc, s_v[z], B[s_c[z] + (size_t)s * ldb]);
-            T[e] = acc;
+        if (bl) {
+            for (int e = tid; e < rows * ncols; e += NT) {
+                const int a = e % rows, s = e / rows;
+                CT acc = Arith<CT>::zero();
+                for (int z = s_rp[a]; z < s_rp[a + 1]; z++) cfma(acc, s_v[z], s_b[s_c[z] + s * ldb]);
+                T[e] = acc;
+            }
+        } else {
+            for (int e = tid; e < rows * ncols; e += NT) {
+                const int a = e % rows, s = e / rows;
+                CT acc = Arith<CT>::zero();
+                for (int z = s_rp[a]; z < s_rp[a + 1]; z++)
+                    cfma(acc, s_v[z], B[s_c[z] + (size_t)s * ldb]);
+                T[e] = acc;
+            }
         }
     } else {
         for (int e = tid; e < rows * ncols; e += NT) {
@@ -1020,7 +1061,7 @@ __device__ __forceinline__ void sparse_rows_times(const int *__restrict__ rp,
 // (rho_eff = det + det^H when rho == (det+det^H)/2, compute.h:381-382).  Stored per element
 // e = r + s*b_I at offset pair_eoff, layout [c][k][e]: Qs = Q(r,s), Rs = R(s,r).
 template <int M, typename R, bool SPLIT = false>
-__global__ void __launch_bounds__(NThr<M>::v) k_qr(DevPlan P, int Kb, const int *__restrict__ pgrp,
+__global__ void __launch_bounds__(NThr<M>::v, (M == 5 && sizeof(R) == 4) ? 3 : 1) k_qr(DevPlan P, int Kb, const int *__restrict__ pgrp,
                                                    int ngrp,
                                                    const typename CX<R>::T *__restrict__ bufB,
                                                    typename CX<R>::T *__restrict__ scratch,
