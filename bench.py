@@ -68,6 +68,20 @@ def corr_flops_per_elem(n: int, P: int) -> float:
     return c + 18.0 * n + 14.0 * P
 
 
+def corr_bytes_per_elem_step(wls, precision: str) -> float:
+    """Bytes k_corr must read per correlation element and step: Q in the storage precision,
+    plus the same again when R is not conj(Q) alone -- rho is not (det + det^H) / 2, or det has
+    entries in both (a, b) and (b, a) so that some elements have a reverse element."""
+    cs = 16.0 if precision == "fp64" else 8.0
+    for w in wls:
+        rho, det = w.operators()
+        scale = max(np.abs(det).max(), 1.0)
+        rosym = np.abs(rho - 0.5 * (det + det.conj().T)).max() <= 1e-12 * scale
+        if not rosym or np.any((det != 0) & (det.T != 0)):
+            return 2.0 * cs
+    return cs
+
+
 def profiled_traffic(kernel: str, precision: str):
     """HBM bytes per launch of `kernel` from the committed counter summary of this workload:
     FETCH_SIZE + WRITE_SIZE (KiB per dispatch, averaged over the kernel's dispatches; all
@@ -482,10 +496,11 @@ def main():
         out["roofline"]["achieved_hbm_gbs"] = traffic["bytes"] / (avg_ms * 1e-3) / 1e9
     if dom == "k_corr":
         n, P = wls[0].steps, wls[0].npts
-        # algorithmic HBM bytes: every step of every correlation element is read twice (Q, and
-        # the reverse element of Q or the stored R), in the storage precision
+        # algorithmic HBM bytes: every step of every correlation element is read once as Q, and
+        # a second time (the reverse element of Q, or the stored R) unless R = conj(Q) there
         elems = d["flops"] / d["launches"] / corr_flops_per_elem(n, P)
-        alg_bytes = elems * n * (32.0 if args.precision == "fp64" else 16.0)
+        per_step = corr_bytes_per_elem_step(wls, args.precision)
+        alg_bytes = elems * n * per_step
         if d["flops"] / d["launches"] / alg_bytes < peak * 1e12 / (HBM_PEAK_GBS * 1e9):
             # below the ridge point: HBM bound
             out["roofline"].update({
@@ -494,14 +509,17 @@ def main():
                 "frac": alg_bytes / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
                 "bytes_per_launch": alg_bytes, "achieved_tflops": achieved,
                 "flop_peak_tflops": peak,
-                "note": ("achieved = algorithmic HBM bytes per launch (each step of each "
-                         "correlation element read twice: Q and the reverse element or R, "
-                         + ("32" if args.precision == "fp64" else "16") + " B per element-step) "
-                         "/ the average launch time (HIP events on the launch stream); the "
-                         "kernel's executed flops per algorithmic byte are below MI355X's ridge "
-                         "point, so HBM (8 TB/s) is its roof; achieved_tflops = executed flops "
-                         "rate"),
+                "note": ("achieved = algorithmic HBM bytes per launch (%g B per correlation "
+                         "element-step) / the average launch time (HIP events on the launch "
+                         "stream); the kernel's executed flops per algorithmic byte are below "
+                         "MI355X's ridge point, so HBM (8 TB/s) is its roof; achieved_tflops = "
+                         "executed flops rate" % per_step),
             })
+        else:
+            out["roofline"]["bytes_per_launch"] = alg_bytes
+            out["roofline"]["note"] += (
+                "; algorithmic HBM bytes %g per correlation element-step (%.3g GB per launch): "
+                "executed flops per byte above the ridge point" % (per_step, alg_bytes / 1e9))
         ratio = (14.0 * n * n + 14.0 * P) / corr_flops_per_elem(n, P)
         out["roofline"]["canonical_flops_per_launch"] = ratio * d["flops"] / d["launches"]
         out["roofline"]["canonical_tflops"] = ratio * achieved

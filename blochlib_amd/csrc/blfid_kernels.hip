@@ -595,7 +595,8 @@ __device__ double build_h_block(const DevPlan &P, const double *terms, const dou
 // p_r = exp(-i sum_q phi_q M_q(r)) (M_q = Fz of pulsed isotope q), gives U = R U0 R^H.
 template <typename R> struct ExpTol;
 template <> struct ExpTol<double> { static constexpr double tol = 1e-17, rmax = 0.8; };
-template <> struct ExpTol<float> { static constexpr double tol = 3e-9, rmax = 2.0; };
+// complex64: a Taylor remainder below 2e-8 relative is under half an FP32 ulp of the result
+template <> struct ExpTol<float> { static constexpr double tol = 2e-8, rmax = 2.0; };
 
 // The 16 x 16 complex64 variant is held to four waves per SIMD (128 registers, no spills; three
 // without the bound).
@@ -1817,9 +1818,9 @@ static void launch_eig(const DevPlan *P, int Kb, const int *grp, int ngrp, int g
     case 0: k_eig<16, 64, R><<<grid, 64, 0, st>>>(EIG_ARGS); break;
     case 1: k_eig<32, 128, R><<<grid, 128, 0, st>>>(EIG_ARGS); break;
     case 2: k_eig<48, 256, R><<<grid, 256, 0, st>>>(EIG_ARGS); break;
-    case 3: k_eig<64, 256, R><<<grid, 256, 0, st>>>(EIG_ARGS); break;
-    case 4: k_eig<80, 256, R><<<grid, 256, 0, st>>>(EIG_ARGS); break;
-    default: k_eig<0, 256, R><<<grid, 256, 0, st>>>(EIG_ARGS); break;
+    case 3: k_eig<64, 1024, R><<<grid, 1024, 0, st>>>(EIG_ARGS); break;
+    case 4: k_eig<80, 1024, R><<<grid, 1024, 0, st>>>(EIG_ARGS); break;
+    default: k_eig<0, 1024, R><<<grid, 1024, 0, st>>>(EIG_ARGS); break;
     }
 #undef EIG_ARGS
 }

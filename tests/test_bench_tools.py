@@ -82,3 +82,23 @@ def test_bench_refuses_more_than_one_gpu():
                        text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU" in r.stderr
+
+
+def test_corr_algorithmic_bytes():
+    """k_corr reads Q once per element-step, and a second array only where R is not conj(Q)."""
+    class W:
+        def __init__(self, rho, det):
+            self._ops = (rho, det)
+
+        def operators(self):
+            return self._ops
+
+    ip = np.zeros((4, 4), complex)
+    ip[0, 1] = ip[2, 3] = 1.0                      # a raising operator: no reverse entries
+    ix = 0.5 * (ip + ip.T)
+    assert bench.corr_bytes_per_elem_step([W(ix, ip)], "c64") == 8.0
+    assert bench.corr_bytes_per_elem_step([W(ix, ip)], "fp64") == 16.0
+    # rho not (det + det^H)/2: R is stored and read
+    assert bench.corr_bytes_per_elem_step([W(ix + np.eye(4), ip)], "c64") == 16.0
+    # det with entries in both (a, b) and (b, a): reverse elements are read
+    assert bench.corr_bytes_per_elem_step([W(ix, ix)], "c64") == 16.0
