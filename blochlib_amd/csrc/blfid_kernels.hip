@@ -454,8 +454,9 @@ __global__ void k_prep(int K, const double *__restrict__ th, const double *__res
 // PulseData::Pulse (pulsedata.cc:110-125) with its phase taken out as the diagonal similarity
 // exp(-i phi Fz) (amp (cos phi Ix + sin phi Iy) = e^{-i phi Fz} amp Ix e^{i phi Fz}), which
 // k_exp applies to the propagator.
-// The block is written into H (leading dimension ld) and its infinity norm is returned to every
-// thread.  Diagonal entries are formed once per row (s_diag); a row's off-diagonal entries are
+// The infinity norm of the block is returned to every thread, and the block times
+// scale_of(norm) is written into H (leading dimension ld), so the propagator's scaling needs no
+// separate pass.  Diagonal entries are formed once per row (s_diag); a row's off-diagonal entries are
 // its flip-flop partners (bits i, j differ: s_ff[i][j]) and, with rf, its single flips of
 // pulsed spins, all inside the block, so the row sums need no pass over H.
 // The plan's interaction-term table (nterms x DEV_TERM_STRIDE doubles) copied into the
@@ -469,11 +470,11 @@ __device__ __forceinline__ const double *stage_terms(const DevPlan &P) {
     return dyn_terms;
 }
 
-template <typename R, int NT>
+template <typename R, int NT, typename SF>
 __device__ double build_h_block(const DevPlan &P, const double *terms, const double *RPc,
                                 const double *u, int blk, R *H, int ld, double *s_zz,
                                 double *s_ff, double *s_v, double *s_diag, double *s_red,
-                                const double *rfk) {
+                                const double *rfk, SF scale_of) {
     const int tid = threadIdx.x;
     const int S = P.S;
     if (tid == 0) {
@@ -546,6 +547,14 @@ __device__ double build_h_block(const DevPlan &P, const double *terms, const dou
         }
         rmax = fmax(rmax, sm);
     }
+    // infinity norm: wavefront maxima, then across the workgroup's waves
+    for (int off = 32; off > 0; off >>= 1) rmax = fmax(rmax, __shfl_xor(rmax, off));
+    if ((tid & 63) == 0) s_red[tid >> 6] = rmax;
+    __syncthreads();
+    double nrm = s_red[0];
+    for (int w = 1; w < NT / 64; w++) nrm = fmax(nrm, s_red[w]);
+    // H is written already multiplied by the caller's factor for this norm
+    const double hs = scale_of(nrm);
     for (int e = tid; e < b * b; e += NT) {
         int ra = e % b, cb_ = e / b;
         double h = 0.0;
@@ -565,14 +574,9 @@ __device__ double build_h_block(const DevPlan &P, const double *terms, const dou
                 if (q >= 0) h = rfk[3 * q];
             }
         }
-        H[ra + (size_t)cb_ * ld] = (R)h;
+        H[ra + (size_t)cb_ * ld] = (R)(h * hs);
     }
-    // infinity norm: wavefront maxima, then across the workgroup's waves
-    for (int off = 32; off > 0; off >>= 1) rmax = fmax(rmax, __shfl_xor(rmax, off));
-    if ((tid & 63) == 0) s_red[tid >> 6] = rmax;
     __syncthreads();
-    double nrm = s_red[0];
-    for (int w = 1; w < NT / 64; w++) nrm = fmax(nrm, s_red[w]);
     return nrm;
 }
 
@@ -636,11 +640,14 @@ __global__ void __launch_bounds__(NThr<M>::v, (M == 1 && sizeof(R) == 4) ? 4 : 1
         const int bb = b * b;
         const double theta = P.thetak ? P.thetak[k] : P.theta;
         const double *rfk = P.nrfiso > 0 ? P.rf + (size_t)3 * P.nrfiso * k : nullptr;
+        // X = theta H / 2^sq, the squaring count chosen from the norm bound of theta H
+        int sq = 0;
         const double nrm =
             build_h_block<R, NT>(P, terms, P.RP + 9 * (size_t)(c0 + c), P.u3 + 3 * k, blk, X, b, s_zz,
-                                 s_ff, s_v, s_diag, s_red, rfk) * theta;
-        int sq = 0;
-        while (nrm / (double)(1 << sq) > ExpTol<R>::rmax && sq < 30) sq++;
+                                 s_ff, s_v, s_diag, s_red, rfk, [&](double hn) {
+                                     while (hn * theta / (double)(1 << sq) > ExpTol<R>::rmax && sq < 30) sq++;
+                                     return theta / (double)(1 << sq);
+                                 }) * theta;
         const double rx = nrm / (double)(1 << sq);
         // smallest degree m in Y with rx^(2m+2)/(2m+2)! <= tol (uniform across the workgroup)
         int m = 1;
@@ -655,9 +662,6 @@ __global__ void __launch_bounds__(NThr<M>::v, (M == 1 && sizeof(R) == 4) ? 4 : 1
         }
         // GEMMs of the Taylor part (S = X sinc included) and three per squaring
         nflops += 2.0 * b * b * b * ((m == 6 ? 6 : m == 8 ? 7 : m + 1) + 3 * sq);
-        const R sc = (R)(theta / (double)(1 << sq));
-        for (int e = tid; e < bb; e += NT) X[e] *= sc;
-        __syncthreads();
         // powers Y = X X, Y^2 = Y Y, Y^3 = Y^2 Y, Y^4 = Y^2 Y^2 as far as m needs them (one GEMM
         // call site per loop keeps the kernel small enough for its tiles to stay in registers)
         const int npow = (m == 6) ? 3 : (m < 4 ? m : 4);
@@ -1677,7 +1681,7 @@ __global__ void __launch_bounds__(256) k_static_h(DevPlan P, int Kb, int c0, con
         const int g = (int)(task % ngrp), c = (int)(task / ngrp);
         const int blk = grp[g], b = P.blk_dim[blk];
         build_h_block<double, 256>(P, terms, P.RP + 9 * (size_t)(c0 + c), P.u3, blk, X, b, s_zz,
-                                   s_ff, s_v, s_diag, s_red, nullptr);
+                                   s_ff, s_v, s_diag, s_red, nullptr, [](double) { return 1.0; });
         cd *H = bufH + (size_t)c * P.Lsum + P.blk_uoff[blk];
         for (int e = threadIdx.x; e < b * b; e += 256) H[e] = cmk(X[e], 0.0);
         __syncthreads();
