@@ -466,8 +466,8 @@ static blfid_status build_tables(blfid_plan *pl) {
         // concurrent workgroup slots of scratch, sized so each scratch region stays ~2.5 GB
         const double bm2 = (double)pl->bmax * pl->bmax;
         auto clampi = [](double v, int lo, int hi) { return (int)std::max<double>(lo, std::min<double>(hi, v)); };
-        pl->exp_slots = clampi(std::floor(2.5e9 / (9.0 * bm2 * 8.0)), 64, 8192);
-        pl->qr_slots = clampi(std::floor(2.5e9 / (3.0 * bm2 * 16.0)), 64, 8192);
+        pl->exp_slots = clampi(std::floor(2.5e9 / (9.0 * bm2 * 8.0)), 64, 16384);
+        pl->qr_slots = clampi(std::floor(2.5e9 / (3.0 * bm2 * 16.0)), 64, 16384);
         pl->eig_slots = pl->eig_cls.count(5) ? clampi(std::floor(1.0e9 / (bm2 * 16.0)), 64, 2048) : 0;
     }
     if (pl->bmax > DEV_MAX_BLOCK)
@@ -986,6 +986,26 @@ blfid_status blfid_plan_set_powder(blfid_plan *pl, const double *theta, const do
     return BLFID_OK;
 }
 
+// Scratch slots of a launch over a block group (or pair group) whose largest matrix has `e`
+// entries: the plan's scratch holds `slots` slots of bm2 = bmax^2 entries, so a group of smaller
+// blocks gets proportionally more slots of e entries (more workgroups in flight), up to 16384
+// for the 64-thread single-wave variant (M = 1) and 8192 otherwise.
+static void group_slots(size_t bm2, int slots, size_t e, int M, size_t *slot, int *n) {
+    *slot = std::max<size_t>(e, 1);
+    const double k = (double)slots * (double)bm2 / (double)*slot;
+    *n = (int)std::min<double>(k, M == 1 ? 16384.0 : 8192.0);
+}
+static size_t group_e(const blfid_plan *pl, const std::vector<int> &g) {
+    size_t e = 0;
+    for (int b : g) e = std::max<size_t>(e, (size_t)pl->blk_dim[b] * pl->blk_dim[b]);
+    return e;
+}
+static size_t pair_e(const blfid_plan *pl, const std::vector<int> &g) {
+    size_t e = 0;
+    for (int p : g)
+        e = std::max<size_t>(e, (size_t)pl->blk_dim[pl->pairs[p].I] * pl->blk_dim[pl->pairs[p].J]);
+    return e;
+}
 static double group_b3(const blfid_plan *pl, const std::vector<int> &g) {
     double s = 0;
     for (int b : g) s += (double)pl->blk_dim[b] * pl->blk_dim[b] * pl->blk_dim[b];
@@ -1214,8 +1234,11 @@ blfid_status blfid_plan_run(blfid_plan *pl, int normalize, void *stream) {
         } else if (pl->spinning) {
             for (auto &kv : pl->blk_groups) {
                 PhaseScope ps(pl, 0 * 8 + kv.first, 0.0, st); // flops counted by k_exp
+                size_t slot;
+                int ns;
+                group_slots(bm2, pl->exp_slots, group_e(pl, kv.second), kv.first, &slot, &ns);
                 LAUNCHCHK(blfid_dev_launch_exp(pr, kv.first, P, kb, c0, pl->d_bgrp[kv.first].p,
-                                               (int)kv.second.size(), pl->scrE, bm2, pl->exp_slots,
+                                               (int)kv.second.size(), pl->scrE, slot, ns,
                                                pl->bufU, st));
             }
             for (auto &kv : pl->blk_groups) {
@@ -1241,9 +1264,12 @@ blfid_status blfid_plan_run(blfid_plan *pl, int normalize, void *stream) {
             }
             for (auto &kv : pl->pair_groups) {
                 PhaseScope ps(pl, 4 * 8 + kv.first, kb * (double)pl->n * pair_flops(pl, kv.second), st);
+                size_t slot;
+                int ns;
+                group_slots(bm2, pl->qr_slots, pair_e(pl, kv.second), kv.first, &slot, &ns);
                 LAUNCHCHK(blfid_dev_launch_qr(pr, kv.first, P, kb, pl->d_pgrp[kv.first].p,
-                                              (int)kv.second.size(), pl->bufU, pl->scrQ, bm2,
-                                              pl->qr_slots, pl->bufQt, pl->bufRt, st));
+                                              (int)kv.second.size(), pl->bufU, pl->scrQ, slot,
+                                              ns, pl->bufQt, pl->bufRt, st));
             }
             if (pl->Etot > 0) {
                 PhaseScope ps(pl, 5 * 8, kb * (double)pl->Etot * corr_flops_per_elem(pl->n, pl->npts), st);

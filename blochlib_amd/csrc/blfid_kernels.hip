@@ -1803,7 +1803,8 @@ static void launch_dyson(int M, const DevPlan *P, int Kb, const int *grp, int ng
 extern "C" int blfid_dev_launch_dyson(int prec, int M, const DevPlan *P, int Kb, const int *grp,
                                       int ngrp, const void *bufU, void *bufA, hipStream_t st) {
     long ntask = (long)Kb * ngrp;
-    int grid = (int)(ntask < GRID_CAP ? ntask : GRID_CAP);
+    const long cap = M == 1 ? 4 * GRID_CAP : GRID_CAP; // single-wave workgroups: four times as many
+    int grid = (int)(ntask < cap ? ntask : cap);
     if (grid <= 0) return 0;
     if (prec) launch_dyson<float>(M, P, Kb, grp, ngrp, grid, bufU, bufA, st);
     else launch_dyson<double>(M, P, Kb, grp, ngrp, grid, bufU, bufA, st);
@@ -1864,7 +1865,8 @@ extern "C" int blfid_dev_launch_bmul(int prec, int M, const DevPlan *P, int Kb, 
                                      int ngrp, const void *bufA, const void *bufW, void *bufB,
                                      hipStream_t st) {
     long ntask = (long)Kb * P->n * ngrp;
-    int grid = (int)(ntask < GRID_CAP ? ntask : GRID_CAP);
+    const long cap = M == 1 ? 4 * GRID_CAP : GRID_CAP;
+    int grid = (int)(ntask < cap ? ntask : cap);
     if (grid <= 0) return 0;
     if (prec) launch_bmul<float>(M, P, Kb, grp, ngrp, grid, bufA, bufW, bufB, st);
     else launch_bmul<double>(M, P, Kb, grp, ngrp, grid, bufA, bufW, bufB, st);
