@@ -762,6 +762,99 @@ __global__ void __launch_bounds__(NThr<M>::v, (M == 1 && sizeof(R) == 4) ? 4 : 1
 }
 
 // ------------------------------------------------------------------------------ k_dyson
+// Dyson chain of one block of at most 16 M <= 32 states on matrix cores (complex64): one LDS
+// slice holds the whole K dimension, so the running product stays in LDS as the next product's
+// B operand (Bt[j][q] = A_{k-1}(q, j), written there straight from the accumulators) and each
+// step loads only U_k, prefetched into registers one step ahead.  Operand layout, fragments and
+// products as in wg_cgemm_mfma.  `smem` holds 4 * 16 M * (16 M + 4) floats.
+template <int M, int NT>
+__device__ __forceinline__ void dyson_chain_mfma(int b, int n, const cf *__restrict__ U0,
+                                                 size_t ustride, cf *__restrict__ A0,
+                                                 size_t astride, int akeep, float *smem) {
+    constexpr int TM = 16 * M, LDK = TM + 4, NW = NT / 64, NSUB = (M * M + NW - 1) / NW;
+    constexpr int PER = TM * TM / NT; // operand elements staged per thread
+    float *Ar = smem, *Ai = Ar + TM * LDK, *Br = Ai + TM * LDK, *Bi = Br + TM * LDK;
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63, g = lane >> 4, r = lane & 15;
+    const int kp = (b + 15) & ~15;
+    // A_0 = U_0: to global and, zero-filled past b, into the B operand
+    for (int idx = tid; idx < TM * TM; idx += NT) {
+        const int q = idx % TM, j = idx / TM;
+        cf v = make_float2(0.0f, 0.0f);
+        if (q < b && j < b) {
+            v = U0[q + j * b];
+            A0[q + j * b] = v;
+        }
+        Br[j * LDK + q] = v.x;
+        Bi[j * LDK + q] = v.y;
+    }
+    // element u of this thread's share of the A operand: row i, column q (16 lanes on 16
+    // consecutive rows of one column, as the transposing staging of wg_cgemm_mfma)
+    auto at_i = [&](int u) { const int idx = tid + u * NT; return (idx & 15) + 16 * ((idx >> 6) % M); };
+    auto at_q = [&](int u) { const int idx = tid + u * NT; return ((idx >> 4) & 3) + 4 * ((idx >> 6) / M); };
+    cf pre[PER];
+    auto load_u = [&](int k) {
+        const cf *Uk = U0 + (size_t)k * ustride;
+#pragma unroll
+        for (int u = 0; u < PER; u++) {
+            const int i = at_i(u), q = at_q(u);
+            pre[u] = (i < b && q < b) ? Uk[i + q * b] : make_float2(0.0f, 0.0f);
+        }
+    };
+    if (n > 1) load_u(1);
+    for (int k = 1; k < n; k++) {
+#pragma unroll
+        for (int u = 0; u < PER; u++) {
+            const int i = at_i(u), q = at_q(u);
+            Ar[i * LDK + q] = pre[u].x;
+            Ai[i * LDK + q] = pre[u].y;
+        }
+        __syncthreads(); // U_k staged; the previous product is in the B operand
+        if (k + 1 < n) load_u(k + 1);
+        v4f cr[NSUB], ci[NSUB];
+#pragma unroll
+        for (int s = 0; s < NSUB; s++) {
+            cr[s] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+            ci[s] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+        }
+        for (int qq = 0; qq < kp; qq += 16) {
+#pragma unroll
+            for (int s = 0; s < NSUB; s++) {
+                const int t = w + NW * s;
+                if (t < M * M) {
+                    const int oa = (16 * (t / M) + r) * LDK + qq + 4 * g;
+                    const int ob = (16 * (t % M) + r) * LDK + qq + 4 * g;
+                    const v4f ar = *(const v4f *)&Ar[oa], ai = *(const v4f *)&Ai[oa];
+                    const v4f br = *(const v4f *)&Br[ob], bi = *(const v4f *)&Bi[ob];
+                    const v4f na = -ai;
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                        cr[s] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[u], br[u], cr[s], 0, 0, 0);
+                        ci[s] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[u], bi[u], ci[s], 0, 0, 0);
+                        cr[s] = __builtin_amdgcn_mfma_f32_16x16x4f32(na[u], bi[u], cr[s], 0, 0, 0);
+                        ci[s] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai[u], br[u], ci[s], 0, 0, 0);
+                    }
+                }
+            }
+        }
+        __syncthreads(); // every read of both operands done
+        cf *Ak = A0 + (size_t)(k % akeep) * astride;
+#pragma unroll
+        for (int s = 0; s < NSUB; s++) {
+            const int t = w + NW * s;
+            if (t < M * M) {
+                const int col = 16 * (t % M) + r, row0 = 16 * (t / M) + 4 * g;
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    if (row0 + j < b && col < b)
+                        Ak[row0 + j + col * b] = make_float2(cr[s][j], ci[s][j]);
+                *(v4f *)&Br[col * LDK + row0] = cr[s];
+                *(v4f *)&Bi[col * LDK + row0] = ci[s];
+            }
+        }
+    }
+    __syncthreads();
+}
+
 // One task = (crystallite, block) chain: A_1 = U_1, A_{k+1} = U_{k+1} A_k (compute.h:267;
 // U *= X is U = X U in propogation.cc:273, 288).  Product k goes to slot k % akeep of the
 // crystallite's akeep slots: all of them for gamma-COMPUTE, two alternating ones when only
@@ -782,6 +875,10 @@ __global__ void __launch_bounds__(NThr<M>::v) k_dyson(DevPlan P, int Kb, const i
         const size_t abase = (size_t)c * P.akeep * P.Lsum + P.blk_uoff[blk];
         const CT *U0 = bufU + base;
         CT *A0 = bufA + abase;
+        if constexpr (M <= 2 && std::is_same<R, float>::value && !SPLIT) {
+            dyson_chain_mfma<M, NT>(b, P.n, U0, P.Lsum, A0, P.Lsum, P.akeep, (float *)smem);
+            continue;
+        }
         for (int e = threadIdx.x; e < b * b; e += NT) A0[e] = U0[e];
         __syncthreads();
         for (int k = 1; k < P.n; k++) {
