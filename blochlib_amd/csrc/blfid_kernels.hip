@@ -762,27 +762,31 @@ __global__ void __launch_bounds__(NThr<M>::v, (M == 1 && sizeof(R) == 4) ? 4 : 1
 }
 
 // ------------------------------------------------------------------------------ k_dyson
-// Dyson chain of one block of at most 16 M <= 32 states on matrix cores (complex64): one LDS
-// slice holds the whole K dimension, so the running product stays in LDS as the next product's
-// B operand (Bt[j][q] = A_{k-1}(q, j), written there straight from the accumulators) and each
-// step loads only U_k, prefetched into registers one step ahead.  Operand layout, fragments and
-// products as in wg_cgemm_mfma.  `smem` holds 4 * 16 M * (16 M + 4) floats.
-template <int M, int NT>
-__device__ __forceinline__ void dyson_chain_mfma(int b, int n, const cf *__restrict__ U0,
-                                                 size_t ustride, cf *__restrict__ A0,
-                                                 size_t astride, int akeep, float *smem) {
+// Chains of products of one block of at most 16 M <= 80 states on matrix cores (complex64), with
+// the whole K dimension in one LDS slice and the B operand resident in LDS (Bt[j][q] = B(q, j)):
+//   DYSON:  A_0 = U_0, A_k = U_k A_{k-1}, k = 1..n-1 -- the running product is written back into
+//           the B operand straight from the accumulators;
+//   !DYSON: C_k = X_k W, k = 0..n-1 -- W stays the B operand for every k.
+// Each step then loads only its A operand (U_k or X_k), prefetched into registers one step
+// ahead.  Operand layout, fragments and products as in wg_cgemm_mfma.  X_k = X0 + k xstride,
+// output k at C0 + (k % ckeep) cstride.  `smem` holds 4 * 16 M * (16 M + 4) floats.
+template <int M, int NT, bool DYSON>
+__device__ __forceinline__ void chain_mfma(int b, int n, const cf *__restrict__ X0, size_t xstride,
+                                           const cf *__restrict__ W, cf *__restrict__ C0,
+                                           size_t cstride, int ckeep, float *smem) {
     constexpr int TM = 16 * M, LDK = TM + 4, NW = NT / 64, NSUB = (M * M + NW - 1) / NW;
     constexpr int PER = TM * TM / NT; // operand elements staged per thread
     float *Ar = smem, *Ai = Ar + TM * LDK, *Br = Ai + TM * LDK, *Bi = Br + TM * LDK;
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63, g = lane >> 4, r = lane & 15;
     const int kp = (b + 15) & ~15;
-    // A_0 = U_0: to global and, zero-filled past b, into the B operand
+    // B operand (zero-filled past b): U_0, also written out as A_0, or W
+    const cf *B0 = DYSON ? X0 : W;
     for (int idx = tid; idx < TM * TM; idx += NT) {
         const int q = idx % TM, j = idx / TM;
         cf v = make_float2(0.0f, 0.0f);
         if (q < b && j < b) {
-            v = U0[q + j * b];
-            A0[q + j * b] = v;
+            v = B0[q + j * b];
+            if (DYSON) C0[q + j * b] = v;
         }
         Br[j * LDK + q] = v.x;
         Bi[j * LDK + q] = v.y;
@@ -792,24 +796,25 @@ __device__ __forceinline__ void dyson_chain_mfma(int b, int n, const cf *__restr
     auto at_i = [&](int u) { const int idx = tid + u * NT; return (idx & 15) + 16 * ((idx >> 6) % M); };
     auto at_q = [&](int u) { const int idx = tid + u * NT; return ((idx >> 4) & 3) + 4 * ((idx >> 6) / M); };
     cf pre[PER];
-    auto load_u = [&](int k) {
-        const cf *Uk = U0 + (size_t)k * ustride;
+    auto load_x = [&](int k) {
+        const cf *Xk = X0 + (size_t)k * xstride;
 #pragma unroll
         for (int u = 0; u < PER; u++) {
             const int i = at_i(u), q = at_q(u);
-            pre[u] = (i < b && q < b) ? Uk[i + q * b] : make_float2(0.0f, 0.0f);
+            pre[u] = (i < b && q < b) ? Xk[i + q * b] : make_float2(0.0f, 0.0f);
         }
     };
-    if (n > 1) load_u(1);
-    for (int k = 1; k < n; k++) {
+    const int k0 = DYSON ? 1 : 0;
+    if (k0 < n) load_x(k0);
+    for (int k = k0; k < n; k++) {
 #pragma unroll
         for (int u = 0; u < PER; u++) {
             const int i = at_i(u), q = at_q(u);
             Ar[i * LDK + q] = pre[u].x;
             Ai[i * LDK + q] = pre[u].y;
         }
-        __syncthreads(); // U_k staged; the previous product is in the B operand
-        if (k + 1 < n) load_u(k + 1);
+        __syncthreads(); // A operand staged; B operand complete
+        if (k + 1 < n) load_x(k + 1);
         v4f cr[NSUB], ci[NSUB];
 #pragma unroll
         for (int s = 0; s < NSUB; s++) {
@@ -836,8 +841,8 @@ __device__ __forceinline__ void dyson_chain_mfma(int b, int n, const cf *__restr
                 }
             }
         }
-        __syncthreads(); // every read of both operands done
-        cf *Ak = A0 + (size_t)(k % akeep) * astride;
+        __syncthreads(); // every operand read done
+        cf *Ck = C0 + (size_t)(k % ckeep) * cstride;
 #pragma unroll
         for (int s = 0; s < NSUB; s++) {
             const int t = w + NW * s;
@@ -846,9 +851,11 @@ __device__ __forceinline__ void dyson_chain_mfma(int b, int n, const cf *__restr
 #pragma unroll
                 for (int j = 0; j < 4; j++)
                     if (row0 + j < b && col < b)
-                        Ak[row0 + j + col * b] = make_float2(cr[s][j], ci[s][j]);
-                *(v4f *)&Br[col * LDK + row0] = cr[s];
-                *(v4f *)&Bi[col * LDK + row0] = ci[s];
+                        Ck[row0 + j + col * b] = make_float2(cr[s][j], ci[s][j]);
+                if (DYSON) {
+                    *(v4f *)&Br[col * LDK + row0] = cr[s];
+                    *(v4f *)&Bi[col * LDK + row0] = ci[s];
+                }
             }
         }
     }
@@ -860,13 +867,15 @@ __device__ __forceinline__ void dyson_chain_mfma(int b, int n, const cf *__restr
 // crystallite's akeep slots: all of them for gamma-COMPUTE, two alternating ones when only
 // the period propagator A_n is needed.
 template <int M, typename R, bool SPLIT = false>
-__global__ void __launch_bounds__(NThr<M>::v) k_dyson(DevPlan P, int Kb, const int *__restrict__ grp,
+__global__ void __launch_bounds__(NThr<M>::v, (M == 3 && sizeof(R) == 4) ? 3 : (M == 4 && sizeof(R) == 4) ? 2 : 1) k_dyson(DevPlan P, int Kb, const int *__restrict__ grp,
                                                       int ngrp,
                                                       const typename CX<R>::T *__restrict__ bufU,
                                                       typename CX<R>::T *__restrict__ bufA) {
     constexpr int NT = NThr<M>::v;
     typedef typename CX<R>::T CT;
-    __shared__ __align__(16) char smem[CGemm<M, NT, R, SPLIT>::BYTES];
+    constexpr bool CHAIN = std::is_same<R, float>::value && !SPLIT;
+    constexpr int CHB = 4 * 16 * M * (16 * M + 4) * 4, GB = CGemm<M, NT, R, SPLIT>::BYTES;
+    __shared__ __align__(16) char smem[CHAIN && CHB > GB ? CHB : GB];
     const long ntask = (long)Kb * ngrp;
     for (long task = blockIdx.x; task < ntask; task += gridDim.x) {
         const int g = (int)(task % ngrp), c = (int)(task / ngrp);
@@ -875,17 +884,20 @@ __global__ void __launch_bounds__(NThr<M>::v) k_dyson(DevPlan P, int Kb, const i
         const size_t abase = (size_t)c * P.akeep * P.Lsum + P.blk_uoff[blk];
         const CT *U0 = bufU + base;
         CT *A0 = bufA + abase;
-        if constexpr (M <= 2 && std::is_same<R, float>::value && !SPLIT) {
-            dyson_chain_mfma<M, NT>(b, P.n, U0, P.Lsum, A0, P.Lsum, P.akeep, (float *)smem);
-            continue;
-        }
-        for (int e = threadIdx.x; e < b * b; e += NT) A0[e] = U0[e];
-        __syncthreads();
-        for (int k = 1; k < P.n; k++) {
-            const CT *Uk = bufU + base + (size_t)k * P.Lsum;
-            const CT *Ap = bufA + abase + (size_t)((k - 1) % P.akeep) * P.Lsum;
-            CT *Ak = bufA + abase + (size_t)(k % P.akeep) * P.Lsum;
-            CGemm<M, NT, R, SPLIT>::run(b, b, b, Uk, b, false, Ap, b, Ak, b, smem);
+        // the chain holds one 16 M x 16 M tile: blocks above 80 states (M = 5) take the tiled path
+        if (CHAIN && (M < 5 || b <= 16 * M)) {
+            if constexpr (CHAIN)
+                chain_mfma<M, NT, true>(b, P.n, U0, P.Lsum, nullptr, A0, P.Lsum, P.akeep,
+                                        (float *)smem);
+        } else if constexpr (!CHAIN || M == 5) {
+            for (int e = threadIdx.x; e < b * b; e += NT) A0[e] = U0[e];
+            __syncthreads();
+            for (int k = 1; k < P.n; k++) {
+                const CT *Uk = bufU + base + (size_t)k * P.Lsum;
+                const CT *Ap = bufA + abase + (size_t)((k - 1) % P.akeep) * P.Lsum;
+                CT *Ak = bufA + abase + (size_t)(k % P.akeep) * P.Lsum;
+                CGemm<M, NT, R, SPLIT>::run(b, b, b, Uk, b, false, Ap, b, Ak, b, smem);
+            }
         }
     }
 }
@@ -1074,23 +1086,47 @@ __global__ void __launch_bounds__(NT) k_eig(DevPlan P, int Kb, const int *__rest
 // ------------------------------------------------------------------------------ k_bmul
 // B_k = A_k W for every (crystallite, step, block) of a group; writes into bufB.
 template <int M, typename R, bool SPLIT = false>
-__global__ void __launch_bounds__(NThr<M>::v) k_bmul(DevPlan P, int Kb, const int *__restrict__ grp,
+__global__ void __launch_bounds__(NThr<M>::v, (M == 3 && sizeof(R) == 4) ? 3 : (M == 4 && sizeof(R) == 4) ? 2 : 1) k_bmul(DevPlan P, int Kb, const int *__restrict__ grp,
                                                      int ngrp,
                                                      const typename CX<R>::T *__restrict__ bufA,
                                                      const typename CX<R>::T *__restrict__ bufW,
                                                      typename CX<R>::T *__restrict__ bufB) {
     constexpr int NT = NThr<M>::v;
     typedef typename CX<R>::T CT;
-    __shared__ __align__(16) char smem[CGemm<M, NT, R, SPLIT>::BYTES];
-    const long ntask = (long)Kb * P.n * ngrp;
-    for (long task = blockIdx.x; task < ntask; task += gridDim.x) {
-        const int g = (int)(task % ngrp);
-        const long ck = task / ngrp;
-        const int k = (int)(ck % P.n), c = (int)(ck / P.n);
-        const int blk = grp[g], b = P.blk_dim[blk];
-        const size_t off = ((size_t)c * P.n + k) * P.Lsum + P.blk_uoff[blk];
-        const CT *W = bufW + (size_t)c * P.Lsum + P.blk_uoff[blk];
-        CGemm<M, NT, R, SPLIT>::run(b, b, b, bufA + off, b, false, W, b, bufB + off, b, smem);
+    // complex64: one task per (crystallite, block) walks all n steps with W resident in LDS;
+    // complex128 (and the BF16x3 form): one task per (crystallite, step, block)
+    constexpr bool CHAIN = std::is_same<R, float>::value && !SPLIT;
+    constexpr int CHB = 4 * 16 * M * (16 * M + 4) * 4, GB = CGemm<M, NT, R, SPLIT>::BYTES;
+    __shared__ __align__(16) char smem[CHAIN && CHB > GB ? CHB : GB];
+    if constexpr (CHAIN) {
+        // blocks above 80 states (M = 5) exceed the chain's tile: one task per step for them
+        const long ntask = (long)Kb * ngrp;
+        for (long task = blockIdx.x; task < ntask; task += gridDim.x) {
+            const int g = (int)(task % ngrp), c = (int)(task / ngrp);
+            const int blk = grp[g], b = P.blk_dim[blk];
+            const size_t base = (size_t)c * P.n * P.Lsum + P.blk_uoff[blk];
+            const CT *W = bufW + (size_t)c * P.Lsum + P.blk_uoff[blk];
+            if (M < 5 || b <= 16 * M) {
+                chain_mfma<M, NT, false>(b, P.n, bufA + base, P.Lsum, W, bufB + base, P.Lsum,
+                                         P.n, (float *)smem);
+            } else if constexpr (M == 5) {
+                for (int k = 0; k < P.n; k++)
+                    CGemm<M, NT, R, SPLIT>::run(b, b, b, bufA + base + (size_t)k * P.Lsum, b,
+                                                false, W, b, bufB + base + (size_t)k * P.Lsum, b,
+                                                smem);
+            }
+        }
+    } else {
+        const long ntask = (long)Kb * P.n * ngrp;
+        for (long task = blockIdx.x; task < ntask; task += gridDim.x) {
+            const int g = (int)(task % ngrp);
+            const long ck = task / ngrp;
+            const int k = (int)(ck % P.n), c = (int)(ck / P.n);
+            const int blk = grp[g], b = P.blk_dim[blk];
+            const size_t off = ((size_t)c * P.n + k) * P.Lsum + P.blk_uoff[blk];
+            const CT *W = bufW + (size_t)c * P.Lsum + P.blk_uoff[blk];
+            CGemm<M, NT, R, SPLIT>::run(b, b, b, bufA + off, b, false, W, b, bufB + off, b, smem);
+        }
     }
 }
 
