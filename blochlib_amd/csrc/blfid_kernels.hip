@@ -930,10 +930,11 @@ __global__ void __launch_bounds__(NT) k_eig(DevPlan P, int Kb, const int *__rest
     constexpr double tol_off = __is_same(R, double) ? 1e-17 : 1e-12;
     constexpr bool BIG = MB == 0;
     constexpr int MP = BIG ? EIG_MAXPAIR : (MB + 1) / 2;
-    // complex64 eigenvectors of the largest LDS class accumulate in LDS (FP32) when only the
-    // storage-precision copy is wanted (the spinning path): the rotations then never wait on
-    // global memory.  The FP64 copy W is not written in that case.
-    constexpr bool WLDS = MB == EIG_MAXB && __is_same(R, float);
+    // complex64 eigenvectors of the two largest LDS classes accumulate in LDS (FP32) when only
+    // the storage-precision copy is wanted (the spinning path): the rotations then never wait on
+    // global memory.  The FP64 copy W is not written in that case.  (Both classes already run
+    // one 1024-thread workgroup per CU, so the extra LDS costs no occupancy.)
+    constexpr bool WLDS = (MB == EIG_MAXB || MB == 64) && __is_same(R, float);
     __shared__ cd Mlds[BIG ? 1 : MB * (MB + 1)];
     __shared__ cf Wl[WLDS ? MB * MB : 1];
     const bool wl = WLDS && bufWr != nullptr;
