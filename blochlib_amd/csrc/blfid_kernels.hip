@@ -1194,6 +1194,106 @@ __device__ __forceinline__ void sparse_rows_times(const int *__restrict__ rp,
 }
 
 // ------------------------------------------------------------------------------ k_qr
+// Q products of one block pair (both blocks of at most 16 M states, complex64) for all n steps
+// of a crystallite: the pair's sparse rows of D are staged in LDS once; per step B_J is copied
+// to LDS for the gathers, T1 = D_IJ B_J is formed straight into the B operand planes
+// (Bt[s][a] = T1(a, s)) and Q = B_I^H T1 taken on matrix cores, so T1 never goes to global
+// memory.  Returns false (nothing done) when the pair's rows do not fit the LDS left for them.
+// Layout after the operand planes: B_J copy (16 M x 16 M), row pointers, values, columns.
+template <int M> struct QrChainLds {
+    static constexpr int TM = 16 * M, PLANES = 4 * TM * (TM + 4) * 4, SB = TM * TM * 8;
+    static constexpr int CSR = M == 1 ? 2304 : 4096, BYTES = PLANES + SB + CSR;
+};
+template <int M, int NT>
+__device__ __forceinline__ bool qr_chain_mfma(int bI, int bJ, int n, const int *__restrict__ rp,
+                                              const int *__restrict__ col,
+                                              const cd *__restrict__ val,
+                                              const cf *__restrict__ BI0,
+                                              const cf *__restrict__ BJ0, size_t bstride,
+                                              cf *__restrict__ Q0, size_t qstride, char *smem) {
+    typedef QrChainLds<M> L;
+    constexpr int TM = L::TM, LDK = TM + 4, NW = NT / 64, NSUB = (M * M + NW - 1) / NW;
+    const int z0 = rp[0], nnz = rp[bI] - z0;
+    const int roff = ((bI + 1) * 4 + 15) & ~15;
+    if (roff + nnz * 12 > L::CSR) return false;
+    float *Ar = (float *)smem, *Ai = Ar + TM * LDK, *Br = Ai + TM * LDK, *Bi = Br + TM * LDK;
+    cf *sB = (cf *)(smem + L::PLANES);
+    int *s_rp = (int *)(smem + L::PLANES + L::SB);
+    cf *s_v = (cf *)((char *)s_rp + roff);
+    int *s_c = (int *)(s_v + nnz);
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63, g = lane >> 4, r = lane & 15;
+    for (int i = tid; i <= bI; i += NT) s_rp[i] = rp[i] - z0;
+    for (int i = tid; i < nnz; i += NT) {
+        s_c[i] = col[z0 + i];
+        s_v[i] = CX<float>::from(val[z0 + i]);
+    }
+    const int kp = (bI + 15) & ~15;
+    for (int k = 0; k < n; k++) {
+        const cf *BI = BI0 + (size_t)k * bstride, *BJ = BJ0 + (size_t)k * bstride;
+        for (int idx = tid; idx < bJ * bJ; idx += NT) sB[idx] = BJ[idx];
+        // A operand At[i][q] = conj(B_I(q, i)), zero past bI
+        for (int idx = tid; idx < TM * TM; idx += NT) {
+            const int q = idx % TM, i = idx / TM;
+            cf v = make_float2(0.0f, 0.0f);
+            if (i < bI && q < bI) {
+                v = BI[q + i * bI];
+                v.y = -v.y;
+            }
+            Ar[i * LDK + q] = v.x;
+            Ai[i * LDK + q] = v.y;
+        }
+        __syncthreads(); // B_J copy, rows and A operand in place
+        for (int idx = tid; idx < TM * TM; idx += NT) {
+            const int a = idx % TM, sc = idx / TM;
+            cf acc = make_float2(0.0f, 0.0f);
+            if (a < bI && sc < bJ)
+                for (int z = s_rp[a]; z < s_rp[a + 1]; z++) cfma(acc, s_v[z], sB[s_c[z] + sc * bJ]);
+            Br[sc * LDK + a] = acc.x;
+            Bi[sc * LDK + a] = acc.y;
+        }
+        __syncthreads(); // T1 in the B operand
+        v4f cr[NSUB], ci[NSUB];
+#pragma unroll
+        for (int s = 0; s < NSUB; s++) {
+            cr[s] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+            ci[s] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+        }
+        for (int qq = 0; qq < kp; qq += 16) {
+#pragma unroll
+            for (int s = 0; s < NSUB; s++) {
+                const int t = w + NW * s;
+                if (t < M * M) {
+                    const int oa = (16 * (t / M) + r) * LDK + qq + 4 * g;
+                    const int ob = (16 * (t % M) + r) * LDK + qq + 4 * g;
+                    const v4f ar = *(const v4f *)&Ar[oa], ai = *(const v4f *)&Ai[oa];
+                    const v4f br = *(const v4f *)&Br[ob], bi = *(const v4f *)&Bi[ob];
+                    const v4f na = -ai;
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                        cr[s] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[u], br[u], cr[s], 0, 0, 0);
+                        ci[s] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[u], bi[u], ci[s], 0, 0, 0);
+                        cr[s] = __builtin_amdgcn_mfma_f32_16x16x4f32(na[u], bi[u], cr[s], 0, 0, 0);
+                        ci[s] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai[u], br[u], ci[s], 0, 0, 0);
+                    }
+                }
+            }
+        }
+        cf *Qk = Q0 + (size_t)k * qstride;
+#pragma unroll
+        for (int s = 0; s < NSUB; s++) {
+            const int t = w + NW * s;
+            if (t < M * M) {
+                const int cc = 16 * (t % M) + r, row0 = 16 * (t / M) + 4 * g;
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    if (row0 + j < bI && cc < bJ) Qk[row0 + j + cc * bI] = make_float2(cr[s][j], ci[s][j]);
+            }
+        }
+        __syncthreads(); // every operand read done before the next step's copies
+    }
+    return true;
+}
+
 // One task = (crystallite, step, block pair (I,J)).  With B = A_{k+1} W (per block):
 //   Q(r,s) = sum conj(B_I(a,r)) D_IJ(a,b) B_J(b,s)        r in I, s in J
 //   R(s,r) = sum conj(B_J(b,s)) rho_JI(b,a) B_I(a,r)
@@ -1208,10 +1308,40 @@ __global__ void __launch_bounds__(NThr<M>::v, (M == 5 && sizeof(R) == 4) ? 3 : 1
                                                    typename CX<R>::T *__restrict__ Rt) {
     constexpr int NT = NThr<M>::v;
     typedef typename CX<R>::T CT;
-    __shared__ __align__(16) char smem[CGemm<M, NT, R, SPLIT>::BYTES];
+    // complex64 pairs of small blocks with R formed from Q: one task per (crystallite, pair)
+    // walks all steps (qr_chain_mfma); otherwise one task per (crystallite, step, pair)
+    constexpr bool CHAIN = M <= 2 && std::is_same<R, float>::value && !SPLIT;
+    constexpr int GB = CGemm<M, NT, R, SPLIT>::BYTES, CB = QrChainLds<M>::BYTES;
+    __shared__ __align__(16) char smem[CHAIN && CB > GB ? CB : GB];
     const int tid = threadIdx.x;
     CT *T1 = scratch + slot * 2 * blockIdx.x;
     CT *G = T1 + slot;
+    if constexpr (CHAIN) {
+        if (P.rt_from_q) {
+            const long ntc = (long)Kb * ngrp;
+            for (long task = blockIdx.x; task < ntc; task += gridDim.x) {
+                const int g = (int)(task % ngrp), c = (int)(task / ngrp);
+                const int pr = pgrp[g];
+                const int I = P.pair_I[pr], J = P.pair_J[pr];
+                const int bI = P.blk_dim[I], bJ = P.blk_dim[J];
+                const size_t b0 = (size_t)c * P.n * P.Lsum;
+                const size_t e0 = (size_t)c * P.n * P.Etot + P.pair_eoff[pr];
+                if (qr_chain_mfma<M, NT>(bI, bJ, P.n, P.d_rowptr + P.pair_doff[pr], P.d_col,
+                                         (const cd *)P.d_val, bufB + b0 + P.blk_uoff[I],
+                                         bufB + b0 + P.blk_uoff[J], P.Lsum, Qt + e0, P.Etot, smem))
+                    continue;
+                for (int k = 0; k < P.n; k++) { // rows too many for the LDS left: per step
+                    const CT *BI = bufB + b0 + (size_t)k * P.Lsum + P.blk_uoff[I];
+                    const CT *BJ = bufB + b0 + (size_t)k * P.Lsum + P.blk_uoff[J];
+                    sparse_rows_times<NT, R>(P.d_rowptr + P.pair_doff[pr], P.d_col,
+                                             (const cd *)P.d_val, bI, bJ, BJ, bJ, T1, smem, GB);
+                    CGemm<M, NT, R, SPLIT>::run(bI, bJ, bI, BI, bI, true, T1, bI,
+                                                Qt + e0 + (size_t)k * P.Etot, bI, smem);
+                }
+            }
+            return;
+        }
+    }
     const long ntask = (long)Kb * P.n * ngrp;
     for (long task = blockIdx.x; task < ntask; task += gridDim.x) {
         const int g = (int)(task % ngrp);
