@@ -409,7 +409,7 @@ def main():
                     help="skip timing the FP64 form beside the c64 headline")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this engine runs on one GPU; multi-GPU runs are not part of this build "
+        print("bench.py: this engine runs on one GPU; multi-GPU runs are not implemented "
               "(crystallites would shard across GPUs with one P-point all-reduce).",
               file=sys.stderr)
         sys.exit(2)
