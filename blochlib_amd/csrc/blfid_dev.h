@@ -48,4 +48,34 @@ struct DevPlan {
     const int *elem_rev;   // Etot: element (s,r) of the reverse pair (J,I), or -1
     double *expflops;      // 8: executed propagator flops per tile variant (profiling), or NULL
     int bf16x3;            // complex64 products as three bf16 matrix-core products (BLFID_C64_BF16X3)
+    long Estr;             // Q / R row stride per step: Etot padded to a multiple of 16 elements
+    int nparts;            // FID partial sums per crystallite (correlation segments or block pairs)
+    int rt_rev;            // rt_from_q and some element has a reverse element (elem_rev >= 0)
 };
+
+// Correlation work split (k_corr), shared by the host planner and the kernel: a crystallite's
+// Etot elements are cut into chunks of CH elements (CH = min(CORR_SPAN / n, 256 / ceil(n / 8)))
+// and the chunks into segments of up to CORR_SEG_CHUNKS; each segment writes one FID partial sum.
+#define CORR_TW 256
+#define CORR_SPAN 2048
+#define CORR_VTAB 512 /* complex128 entries of the period table, within the second buffer */
+#define CORR_PPT 8
+#define CORR_SEG_CHUNKS 64
+__device__ __host__ inline int corr_seg_chunks(int nchunk) {
+    return nchunk < CORR_SEG_CHUNKS ? (nchunk < 1 ? 1 : nchunk) : CORR_SEG_CHUNKS;
+}
+__device__ __host__ inline int corr_chunk(int n) {
+    int npg = (n + CORR_PPT - 1) / CORR_PPT;
+    int a = CORR_SPAN / n, b = 256 / npg;
+    int ch = a < b ? a : b;
+    return ch < 1 ? 1 : ch;
+}
+__device__ __host__ inline int corr_segments(int n, long Etot) {
+    const int CH = corr_chunk(n);
+    const int nchunk = (int)((Etot + CH - 1) / CH);
+    const int spc = corr_seg_chunks(nchunk);
+    return (nchunk + spc - 1) / spc;
+}
+__host__ __device__ inline bool corr_use_fft(int n) {
+    return n >= 16 && (n & (n - 1)) == 0 && n <= 2 * CORR_TW;
+}

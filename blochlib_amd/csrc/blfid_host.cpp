@@ -28,6 +28,7 @@
 
 #include "../../include/blfid.h"
 #include "blfid_dev.h"
+#include "blfid_layout.h"
 
 typedef std::complex<double> cplx;
 
@@ -47,12 +48,12 @@ int blfid_dev_launch_qr(int prec, int M, const DevPlan *P, int Kb, const int *pg
                         const void *bufB, void *scratch, size_t slot, int nslots, void *Qt,
                         void *Rt, hipStream_t st);
 int blfid_dev_launch_corr(int prec, const DevPlan *P, int Kb, const void *Qt, const void *Rt,
-                          const double *thetas, const int *er, const int *es, double *fidc,
+                          const double *thetas, const int *er, const int *es, double *part,
                           hipStream_t st);
-int blfid_dev_launch_reduce(int Kb, int c0, int npts, const double *w, double scale,
-                            const double *fidc, double *acc, double *comp, hipStream_t st);
-int blfid_dev_launch_finish(int npts, const double *acc, double nre, double nim, double *out,
-                            hipStream_t st);
+int blfid_dev_launch_reduce(int Kb, int c0, int npts, int nparts, int G, const double *w,
+                            double scale, const double *part, double *grp, hipStream_t st);
+int blfid_dev_launch_finish(int npts, int ngroups, const double *grp, double nre, double nim,
+                            double *out, hipStream_t st);
 int blfid_dev_launch_zero(double *p, size_t n, hipStream_t st);
 int blfid_dev_launch_prep(int K, const double *th, const double *ph, const double *ga,
                           double *RP, hipStream_t st);
@@ -61,7 +62,7 @@ int blfid_dev_launch_static_h(const DevPlan *P, int Kb, int c0, const int *grp, 
                               hipStream_t st);
 int blfid_dev_launch_static_fid(int M, const DevPlan *P, int Kb, const int *pgrp, int ngrp,
                                 const double2 *bufW, const double *lam, double2 *scratch,
-                                size_t slot, int nslots, double w0, int prune, double *fidc,
+                                size_t slot, int nslots, double w0, int prune, double *part,
                                 hipStream_t st);
 }
 
@@ -200,7 +201,10 @@ struct blfid_plan {
     std::vector<int> pair_doff, pair_roff, d_rowptr, d_col, r_rowptr, r_col, elem_r, elem_s;
     std::vector<double> d_val, r_val;
     long Etot = 0;
+    long Estr = 0;             // Q row stride per step: Etot padded to 16 elements
+    int nparts = 1;            // FID partial sums per crystallite
     bool rt_from_q = false;    // rosym spinning case: R formed from Q in k_corr
+    bool rt_rev = false;       // rt_from_q and some element has a reverse element
     std::vector<int> elem_rev;  // per element (r,s) of pair (I,J): element (s,r) of (J,I) or -1
     std::map<int, std::vector<int>> pair_groups; // M -> pairs
     // powder
@@ -224,9 +228,9 @@ struct blfid_plan {
     char *bufU = nullptr, *bufA = nullptr, *bufQt = nullptr, *bufRt = nullptr, *scrQ = nullptr,
          *scrE = nullptr, *bufWr = nullptr;
     double2 *bufW = nullptr;
-    double *thetas = nullptr, *fidc = nullptr;
+    double *thetas = nullptr, *part = nullptr, *grp = nullptr;
     double2 *scrEig = nullptr;
-    DevBuf<double> acc, comp, fid_out;
+    DevBuf<double> fid_out;
     DevBuf<double> d_expflops; // propagator flop counters while profiling
     double budget = 0;
     int Kb = 0;
@@ -546,6 +550,9 @@ static void refresh_devplan(blfid_plan *pl) {
     d.elem_rev = pl->d_erev.p;
     d.expflops = pl->prof ? pl->d_expflops.p : nullptr;
     d.bf16x3 = pl->bf16x3;
+    d.Estr = pl->Estr;
+    d.nparts = pl->nparts;
+    d.rt_rev = pl->rt_rev ? 1 : 0;
     d.theta = PI2 * pl->tau;
     d.terms = pl->d_terms.p;
     d.ff_off = pl->d_ff_off.p;
@@ -771,8 +778,15 @@ static blfid_status create_pulse_plan(const blfid_system *sys, double wr_hz, dou
             t[2] = off;
         }
     blfid_plan *pl = new_plan(sys, wr_hz, rotor_deg, npts, t0);
-    pl->prec = prec == BLFID_FP64 ? 0 : 1;
-    pl->bf16x3 = prec == BLFID_C64_BF16X3;
+    // The period propagator of a pulse program is raised to the power m for FID point m, so its
+    // error grows linearly along the FID: with complex64 storage of the step propagators alone
+    // (exact FP64 values rounded, products in FP64) one crystallite of the 13C + 6 1H rf case
+    // already differs from FP64 by 1.5e-5 relative L2 over 2048 points, and 2.5e-4 with the
+    // FP32 Taylor series and complex64 products.  Every precision therefore forms the period
+    // propagator in FP64; the rest of this path (eigenbasis FID, powder sum) is FP64 anyway.
+    (void)prec;
+    pl->prec = 0;
+    pl->bf16x3 = 0;
     pl->ptop = true;
     pl->spinning = false;
     pl->n = steps;
@@ -921,6 +935,11 @@ blfid_status blfid_plan_set_operators(blfid_plan *pl, const double *rho_d, const
         pl->pair_groups[M].push_back((int)p);
     }
     pl->Etot = eo;
+    pl->Estr = (eo + 15) & ~15L;
+    if (pl->spinning && !pl->ptop)
+        pl->nparts = eo > 0 ? corr_segments(pl->n, eo) : 1;
+    else
+        pl->nparts = std::max<int>(1, (int)pl->pairs.size());
     // rosym spinning case: R_(I,J) = B_J^H (det_JI + det_IJ^H) B_I, i.e. R_(I,J)(s,r) =
     // Q_(J,I)(s,r) + conj(Q_(I,J)(r,s)); the pair (J,I) exists exactly when det_JI != 0, so R
     // needs neither its own products nor storage: k_corr forms it from Q through elem_rev
@@ -941,6 +960,8 @@ blfid_status blfid_plan_set_operators(blfid_plan *pl, const double *rho_d, const
                     pl->elem_rev[pl->pair_eoff[p] + r + (long)s2 * bI] = (int)(ro + s2 + (long)r * bJ);
         }
     }
+    pl->rt_rev = false;
+    for (int v : pl->elem_rev) pl->rt_rev |= v >= 0;
     std::vector<int> pI, pJ;
     for (auto &pp : pl->pairs) {
         pI.push_back(pp.I);
@@ -1047,12 +1068,28 @@ static double pair_elems(const blfid_plan *pl, const std::vector<int> &g) {
     return s;
 }
 
-static size_t layout_bytes(const blfid_plan *pl, int Kb, size_t off[11]);
-static size_t fixed_bytes(const blfid_plan *pl);
+static blfid_layout::Shape shape_of(const blfid_plan *pl) {
+    blfid_layout::Shape s;
+    s.prec = pl->prec;
+    s.kind = pl->ptop ? 2 : (pl->spinning ? 0 : 1);
+    s.n = pl->n;
+    s.npts = pl->npts;
+    s.N = pl->N;
+    s.bmax = pl->bmax;
+    s.rt_from_q = pl->rt_from_q ? 1 : 0;
+    s.nparts = pl->nparts;
+    s.exp_slots = pl->exp_slots;
+    s.qr_slots = pl->qr_slots;
+    s.eig_slots = pl->eig_slots;
+    s.Lsum = pl->Lsum;
+    s.Estr = pl->Estr;
+    s.K = pl->K;
+    return s;
+}
 
 static double bytes_per_crystallite(const blfid_plan *pl) {
-    size_t off[11];
-    return (double)(layout_bytes(pl, 1, off) - fixed_bytes(pl));
+    const blfid_layout::Shape sh = shape_of(pl);
+    return (double)(blfid_layout::layout_bytes(sh, 1) - blfid_layout::layout_bytes(sh, 0));
 }
 
 blfid_status blfid_plan_stats(const blfid_plan *pl, double *out) {
@@ -1085,122 +1122,54 @@ blfid_status blfid_plan_stats(const blfid_plan *pl, double *out) {
     out[2] = (double)pl->blocks.size();
     out[3] = (double)pl->bmax;
     out[4] = bytes_per_crystallite(pl);
-    out[5] = (double)fixed_bytes(pl) + 4096.0;
+    out[5] = (double)blfid_layout::layout_bytes(shape_of(pl), 0);
     return BLFID_OK;
 }
 
-static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
-
-// Byte layout of the working storage for a batch of Kb crystallites (fixed scratch first).
-static size_t layout_bytes(const blfid_plan *pl, int Kb, size_t off[11]) {
-    const size_t L = (size_t)pl->Lsum, bm2 = (size_t)pl->bmax * pl->bmax;
-    const size_t c16 = sizeof(double2), d8 = sizeof(double);
-    const size_t cs = pl->prec ? 8 : 16, rs = pl->prec ? 4 : 8; // storage complex / real
-    size_t sz[11];
-    sz[10] = 0;
-    if (pl->ptop) {
-        sz[0] = (size_t)Kb * pl->n * L * cs;                               // U
-        sz[1] = (size_t)Kb * 2 * L * cs;                                   // A (two slots)
-        sz[2] = 0; sz[3] = 0;
-        sz[4] = (size_t)pl->exp_slots * 9 * bm2 * rs;                     // exp scratch
-    } else if (pl->spinning) {
-        sz[0] = (size_t)Kb * pl->n * L * cs;                               // U / B
-        sz[1] = (size_t)Kb * pl->n * L * cs;                               // A
-        sz[2] = std::max<size_t>(1, (size_t)Kb * pl->n * pl->Etot) * cs;   // Qt
-        sz[3] = pl->rt_from_q ? 0                                           // Rt
-                              : std::max<size_t>(1, (size_t)Kb * pl->n * pl->Etot) * cs;
-        sz[4] = (size_t)pl->exp_slots * 9 * bm2 * rs;                     // exp scratch
-        if (pl->prec) sz[10] = (size_t)Kb * L * cs;                        // W in storage precision
-    } else {
-        sz[0] = (size_t)Kb * L * c16;                                      // H (FP64)
-        sz[1] = 0; sz[2] = 0; sz[3] = 0;
-        sz[4] = (size_t)pl->exp_slots * bm2 * d8;
-    }
-    sz[5] = (size_t)Kb * L * c16;                 // W (FP64)
-    sz[6] = (size_t)Kb * pl->N * d8;              // thetas / eigenvalues
-    sz[7] = (size_t)pl->qr_slots * 3 * bm2 * c16; // qr / eigenbasis-FID scratch
-    sz[8] = (size_t)Kb * 2 * pl->npts * d8;       // per-crystallite FIDs
-    sz[9] = (size_t)pl->eig_slots * bm2 * c16;    // eigen-solver scratch for blocks > LDS
-    size_t o = 0;
-    for (int i = 0; i < 11; i++) {
-        off[i] = o;
-        o += align256(sz[i]);
-    }
-    return o;
+static void carve(blfid_plan *pl, char *base, const size_t *off) {
+    using namespace blfid_layout;
+    pl->bufU = base + off[R_U];
+    pl->bufA = base + off[R_A];
+    pl->bufQt = base + off[R_QT];
+    pl->bufRt = base + off[R_RT];
+    pl->scrE = base + off[R_EXP];
+    pl->bufW = (double2 *)(base + off[R_W]);
+    pl->thetas = (double *)(base + off[R_THETA]);
+    pl->scrQ = base + off[R_QR];
+    pl->part = (double *)(base + off[R_PART]);
+    pl->scrEig = (double2 *)(base + off[R_EIG]);
+    pl->bufWr = pl->prec && pl->spinning ? base + off[R_WR] : nullptr;
+    pl->grp = (double *)(base + off[R_GROUP]);
 }
 
-static size_t fixed_bytes(const blfid_plan *pl) {
-    size_t off[11];
-    return layout_bytes(pl, 0, off);
-}
-
-static void carve(blfid_plan *pl, char *base, const size_t off[11]) {
-    pl->bufU = base + off[0];
-    pl->bufA = base + off[1];
-    pl->bufQt = base + off[2];
-    pl->bufRt = base + off[3];
-    pl->scrE = base + off[4];
-    pl->bufW = (double2 *)(base + off[5]);
-    pl->thetas = (double *)(base + off[6]);
-    pl->scrQ = base + off[7];
-    pl->fidc = (double *)(base + off[8]);
-    pl->scrEig = (double2 *)(base + off[9]);
-    pl->bufWr = pl->prec && pl->spinning ? base + off[10] : nullptr;
-}
-
-blfid_status blfid_plan_run(blfid_plan *pl, int normalize, void *stream) {
-    if (!pl) return fail(BLFID_ERR_INVALID, "plan is NULL");
-    if (!pl->have_ops) return fail(BLFID_ERR_INVALID, "operators not set");
-    hipStream_t st = (hipStream_t)stream;
-    pl->last_stream = st;
+// Enqueues the whole powder on `st` with the working storage at `base` (regions at off[],
+// batches of Kb crystallites) and the normalised FID written to `out` (2 npts doubles, device).
+// Every error return after the first launch leaves the launched work in the stream.
+static blfid_status enqueue_run(blfid_plan *pl, char *base, const size_t *off, long Kb,
+                                int normalize, double *out, hipStream_t st) {
     const int K = pl->K;
-    // batch size from the workspace / memory budget
-    size_t off[11];
-    const double per = (double)(layout_bytes(pl, 1, off) - fixed_bytes(pl));
-    const double fixed = (double)fixed_bytes(pl) + 4096.0;
-    double avail;
-    if (pl->ws) {
-        avail = (double)pl->ws->bytes;
-    } else {
-        avail = pl->budget;
-        if (avail <= 0) {
-            size_t fre = 0, tot = 0;
-            HIPCHK(hipMemGetInfo(&fre, &tot));
-            avail = 0.4 * (double)(fre + pl->own_ws.n);
-        }
-    }
-    int Kb = (int)std::min<double>(std::max(1, K), std::floor((avail - fixed) / per));
-    if (Kb < 1) return fail(BLFID_ERR_NOMEM, "one crystallite does not fit the memory budget");
-    pl->Kb = Kb;
-    size_t need = layout_bytes(pl, Kb, off);
-    char *base;
-    if (pl->ws) {
-        base = pl->ws->base;
-        if (pl->ws->used && pl->ws->last != st) HIPCHK(hipStreamWaitEvent(st, pl->ws->ev, 0));
-    } else {
-        HIPCHK(pl->own_ws.alloc(need));
-        base = pl->own_ws.p;
-    }
+    pl->Kb = (int)Kb;
     carve(pl, base, off);
-    HIPCHK(pl->acc.alloc((size_t)2 * pl->npts));
-    HIPCHK(pl->comp.alloc((size_t)2 * pl->npts));
-    HIPCHK(pl->fid_out.alloc((size_t)2 * pl->npts));
     refresh_devplan(pl);
     const DevPlan *P = &pl->dp;
     if (pl->rp_dirty && K > 0) {
         LAUNCHCHK(blfid_dev_launch_prep(K, pl->d_th.p, pl->d_ph.p, pl->d_ga.p, pl->d_RP.p, st));
         pl->rp_dirty = false;
     }
-    LAUNCHCHK(blfid_dev_launch_zero(pl->acc.p, 2 * pl->npts, st));
-    LAUNCHCHK(blfid_dev_launch_zero(pl->comp.p, 2 * pl->npts, st));
+    const int G = blfid_layout::CRYST_GROUP;
+    const int ng = (int)blfid_layout::ngroups(K);
+    LAUNCHCHK(blfid_dev_launch_zero(pl->grp, (size_t)ng * 2 * 2 * pl->npts, st));
     const size_t bm2 = (size_t)pl->bmax * pl->bmax;
     const int pr = pl->prec;
     const size_t cs = pr ? 8 : 16;
     const int ff = pl->rosym == 1 ? 2 : 1;
     const double scale = (pl->spinning && !pl->ptop) ? 1.0 / (double)(pl->n / ff) : 1.0;
-    for (int c0 = 0; c0 < K; c0 += Kb) {
-        int kb = std::min(Kb, K - c0);
-        LAUNCHCHK(blfid_dev_launch_zero(pl->fidc, (size_t)kb * 2 * pl->npts, st));
+    for (int c0 = 0; c0 < K; c0 += (int)Kb) {
+        int kb = (int)std::min<long>(Kb, K - c0);
+        // the FID parts are all written by the correlation / eigenbasis-FID kernels, unless
+        // the operators couple no block pair at all
+        if (pl->pairs.empty())
+            LAUNCHCHK(blfid_dev_launch_zero(pl->part, (size_t)kb * pl->nparts * 2 * pl->npts, st));
         if (pl->ptop) {
             // U_k = exp(+i theta (H_k + H_rf,k)); A_n = U_n ... U_1; its eigenbasis gives
             // rho_m = A_n^m rho A_n^-m in closed form
@@ -1229,7 +1198,7 @@ blfid_status blfid_plan_run(blfid_plan *pl, int normalize, void *stream) {
                 LAUNCHCHK(blfid_dev_launch_static_fid(kv.first, P, kb, pl->d_pgrp[kv.first].p,
                                                       (int)kv.second.size(), pl->bufW,
                                                       pl->thetas, (double2 *)pl->scrQ, bm2,
-                                                      pl->qr_slots, 1.0, 0, pl->fidc, st));
+                                                      pl->qr_slots, 1.0, 0, pl->part, st));
             }
         } else if (pl->spinning) {
             for (auto &kv : pl->blk_groups) {
@@ -1274,7 +1243,7 @@ blfid_status blfid_plan_run(blfid_plan *pl, int normalize, void *stream) {
             if (pl->Etot > 0) {
                 PhaseScope ps(pl, 5 * 8, kb * (double)pl->Etot * corr_flops_per_elem(pl->n, pl->npts), st);
                 LAUNCHCHK(blfid_dev_launch_corr(pr, P, kb, pl->bufQt, pl->bufRt, pl->thetas,
-                                                pl->d_er.p, pl->d_es.p, pl->fidc, st));
+                                                pl->d_er.p, pl->d_es.p, pl->part, st));
             }
         } else {
             for (auto &kv : pl->blk_groups)
@@ -1291,26 +1260,65 @@ blfid_status blfid_plan_run(blfid_plan *pl, int normalize, void *stream) {
                 LAUNCHCHK(blfid_dev_launch_static_fid(kv.first, P, kb, pl->d_pgrp[kv.first].p,
                                                       (int)kv.second.size(), pl->bufW,
                                                       pl->thetas, (double2 *)pl->scrQ, bm2,
-                                                      pl->qr_slots, -PI2 / pl->sw, 1, pl->fidc, st));
+                                                      pl->qr_slots, -PI2 / pl->sw, 1, pl->part, st));
             }
         }
         {
-            PhaseScope ps(pl, 6 * 8, 4.0 * kb * 2.0 * pl->npts, st);
-            LAUNCHCHK(blfid_dev_launch_reduce(kb, c0, pl->npts, pl->d_w.p, scale, pl->fidc,
-                                              pl->acc.p, pl->comp.p, st));
+            PhaseScope ps(pl, 6 * 8, 4.0 * kb * 2.0 * pl->npts * pl->nparts, st);
+            LAUNCHCHK(blfid_dev_launch_reduce(kb, c0, pl->npts, pl->nparts, G, pl->d_w.p, scale,
+                                              pl->part, pl->grp, st));
         }
     }
     cplx norm = normalize ? pl->trdd : cplx(1.0, 0.0);
-    if (normalize && norm == 0.0) return fail(BLFID_ERR_INVALID, "Tr(det det^H) is zero");
-    LAUNCHCHK(blfid_dev_launch_finish(pl->npts, pl->acc.p, norm.real(), norm.imag(),
-                                      pl->fid_out.p, st));
-    if (pl->ws) {
-        HIPCHK(hipEventRecord(pl->ws->ev, st));
-        pl->ws->last = st;
-        pl->ws->used = true;
-    }
+    LAUNCHCHK(blfid_dev_launch_finish(pl->npts, ng, pl->grp, norm.real(), norm.imag(), out, st));
     pl->ran = true;
     return BLFID_OK;
+}
+
+blfid_status blfid_plan_run(blfid_plan *pl, int normalize, void *stream) {
+    if (!pl) return fail(BLFID_ERR_INVALID, "plan is NULL");
+    if (!pl->have_ops) return fail(BLFID_ERR_INVALID, "operators not set");
+    if (normalize && pl->trdd == 0.0) return fail(BLFID_ERR_INVALID, "Tr(det det^H) is zero");
+    hipStream_t st = (hipStream_t)stream;
+    pl->last_stream = st;
+    const blfid_layout::Shape sh = shape_of(pl);
+    // batch size: the largest whose exact layout fits the workspace / memory budget
+    double avail;
+    if (pl->ws) {
+        avail = (double)pl->ws->bytes;
+    } else {
+        avail = pl->budget;
+        if (avail <= 0) {
+            size_t fre = 0, tot = 0;
+            HIPCHK(hipMemGetInfo(&fre, &tot));
+            avail = 0.4 * (double)(fre + pl->own_ws.n);
+        }
+    }
+    const long Kb = blfid_layout::choose_batch(sh, pl->K, avail);
+    if (Kb < 1) return fail(BLFID_ERR_NOMEM, "one crystallite does not fit the memory budget");
+    size_t off[blfid_layout::R_COUNT];
+    const size_t need = blfid_layout::layout_bytes(sh, Kb, off);
+    char *base;
+    if (pl->ws) {
+        if (need > pl->ws->bytes)
+            return fail(BLFID_ERR_NOMEM, "working storage exceeds the workspace");
+        base = pl->ws->base;
+        if (pl->ws->used && pl->ws->last != st) HIPCHK(hipStreamWaitEvent(st, pl->ws->ev, 0));
+    } else {
+        HIPCHK(pl->own_ws.alloc(need));
+        base = pl->own_ws.p;
+    }
+    HIPCHK(pl->fid_out.alloc((size_t)2 * pl->npts));
+    blfid_status rs = enqueue_run(pl, base, off, Kb, normalize, pl->fid_out.p, st);
+    if (pl->ws) {
+        // the next user of the workspace waits for whatever was enqueued, on every exit
+        hipError_t e = hipEventRecord(pl->ws->ev, st);
+        pl->ws->last = st;
+        pl->ws->used = true;
+        if (rs == BLFID_OK && e != hipSuccess)
+            return fail(BLFID_ERR_DEVICE, std::string("hipEventRecord: ") + hipGetErrorString(e));
+    }
+    return rs;
 }
 
 blfid_status blfid_workspace_create(double bytes, blfid_workspace **out) {

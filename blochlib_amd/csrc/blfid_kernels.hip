@@ -1,4 +1,4 @@
-// blfid_kernels.hip -- gfx950 kernels of the powder-averaged MAS FID engine (FP64).
+// blfid_kernels.hip -- gfx950 kernels of the powder-averaged MAS FID engine.
 //
 // The work of one spin system is organised by the block structure of its Hamiltonian: every
 // SolidSys term (CSA, secular dipolar, J; src/QMspins/sys.cc:749-770) commutes with the
@@ -8,8 +8,10 @@
 // Pipeline for a batch of crystallites (host: blfid_host.cpp):
 //   k_prep      rotation of each crystallite (cartPowder, space_ten.cc:578-618)
 //   k_exp       H_k block at the step midpoint (sys.cc:749-770) and U_k = exp(-i 2 pi tau H_k)
-//               by scaling-and-squaring of the degree-15 Taylor cos/sin pair, real FP64
-//               (replaces Mexp's eigen route, matexp.h:239-252; H is real symmetric)
+//               as cos X - i sin X by Taylor polynomials in X^2 of adaptive degree (at most 8,
+//               Paterson-Stockmeyer) with scaling and squaring, in real arithmetic of the
+//               storage precision (FP64, or FP32 for complex64 plans); replaces Mexp's eigen
+//               route, matexp.h:239-252 (H is real symmetric)
 //   k_dyson     A_k = U_k A_{k-1} along the rotor period (compute.h:257-274)
 //   k_eig       U_T = A_n = W diag(lambda) W^H by joint Jacobi on (U+U^H)/2, (U-U^H)/2i
 //               (replaces diag(Uf) -> corth/comqr3, compute.h:317)
@@ -19,7 +21,8 @@
 //   k_corr      F_p = sum_i Qt_{(i+p)%n} Rt_i (cyclic correlation) and
 //               fid[t] += sum_rs F_{t%n}(r,s) z_rs^t (compute.h:402-457)
 //   k_reduce    FID += w_c * fid_c / (n/ff) [/ Tr(D D^H)], Kahan-compensated, in crystallite
-//               order (oneFID.h:396-407, sequencerun_funcs.cc:698-702)
+//               order within groups of 64, then the groups in order (k_finish)
+//               (oneFID.h:396-407, sequencerun_funcs.cc:698-702)
 // and, for wr == 0 or rotor == 0, the static FID (oneFID.h:292-336):
 //   k_static_h  H(0, dt) block;  k_eig (Hermitian case);  k_static_fid.
 //
@@ -1298,7 +1301,8 @@ __device__ __forceinline__ bool qr_chain_mfma(int bI, int bJ, int n, const int *
 //   Q(r,s) = sum conj(B_I(a,r)) D_IJ(a,b) B_J(b,s)        r in I, s in J
 //   R(s,r) = sum conj(B_J(b,s)) rho_JI(b,a) B_I(a,r)
 // (rho_eff = det + det^H when rho == (det+det^H)/2, compute.h:381-382).  Stored per element
-// e = r + s*b_I at offset pair_eoff, layout [c][k][e]: Qs = Q(r,s), Rs = R(s,r).
+// e = r + s*b_I at offset pair_eoff, layout [c][k][e] with rows of Estr elements:
+// Qs = Q(r,s), Rs = R(s,r).
 template <int M, typename R, bool SPLIT = false>
 __global__ void __launch_bounds__(NThr<M>::v, (M == 5 && sizeof(R) == 4) ? 3 : 1) k_qr(DevPlan P, int Kb, const int *__restrict__ pgrp,
                                                    int ngrp,
@@ -1325,10 +1329,10 @@ __global__ void __launch_bounds__(NThr<M>::v, (M == 5 && sizeof(R) == 4) ? 3 : 1
                 const int I = P.pair_I[pr], J = P.pair_J[pr];
                 const int bI = P.blk_dim[I], bJ = P.blk_dim[J];
                 const size_t b0 = (size_t)c * P.n * P.Lsum;
-                const size_t e0 = (size_t)c * P.n * P.Etot + P.pair_eoff[pr];
+                const size_t e0 = (size_t)c * P.n * P.Estr + P.pair_eoff[pr];
                 if (qr_chain_mfma<M, NT>(bI, bJ, P.n, P.d_rowptr + P.pair_doff[pr], P.d_col,
                                          (const cd *)P.d_val, bufB + b0 + P.blk_uoff[I],
-                                         bufB + b0 + P.blk_uoff[J], P.Lsum, Qt + e0, P.Etot, smem))
+                                         bufB + b0 + P.blk_uoff[J], P.Lsum, Qt + e0, P.Estr, smem))
                     continue;
                 for (int k = 0; k < P.n; k++) { // rows too many for the LDS left: per step
                     const CT *BI = bufB + b0 + (size_t)k * P.Lsum + P.blk_uoff[I];
@@ -1336,7 +1340,7 @@ __global__ void __launch_bounds__(NThr<M>::v, (M == 5 && sizeof(R) == 4) ? 3 : 1
                     sparse_rows_times<NT, R>(P.d_rowptr + P.pair_doff[pr], P.d_col,
                                              (const cd *)P.d_val, bI, bJ, BJ, bJ, T1, smem, GB);
                     CGemm<M, NT, R, SPLIT>::run(bI, bJ, bI, BI, bI, true, T1, bI,
-                                                Qt + e0 + (size_t)k * P.Etot, bI, smem);
+                                                Qt + e0 + (size_t)k * P.Estr, bI, smem);
                 }
             }
             return;
@@ -1352,7 +1356,7 @@ __global__ void __launch_bounds__(NThr<M>::v, (M == 5 && sizeof(R) == 4) ? 3 : 1
         const int bI = P.blk_dim[I], bJ = P.blk_dim[J];
         const CT *BI = bufB + ((size_t)c * P.n + k) * P.Lsum + P.blk_uoff[I];
         const CT *BJ = bufB + ((size_t)c * P.n + k) * P.Lsum + P.blk_uoff[J];
-        const size_t eo = ((size_t)c * P.n + k) * P.Etot + P.pair_eoff[pr];
+        const size_t eo = ((size_t)c * P.n + k) * P.Estr + P.pair_eoff[pr];
         sparse_rows_times<NT, R>(P.d_rowptr + P.pair_doff[pr], P.d_col, (const cd *)P.d_val, bI,
                                  bJ, BJ, bJ, T1, smem, CGemm<M, NT, R, SPLIT>::BYTES);
         CGemm<M, NT, R, SPLIT>::run(bI, bJ, bI, BI, bI, true, T1, bI, Qt + eo, bI, smem);
@@ -1385,28 +1389,12 @@ __global__ void __launch_bounds__(NThr<M>::v, (M == 5 && sizeof(R) == 4) ? 3 : 1
 // inverse pass into one, and G_p = F_p z^p into the last inverse pass (corr_fwd_pass and the
 // passes after it).  Thread t then forms fid[t] = sum_e F_{t%n}(e) z_e^t (compute.h:432-457)
 // from G and a per-chunk table of z_e^{n j}, summed over the segment's chunks in registers and
-// added atomically to fid_c[c][t] (per chunk when the FID is too long to hold).
-// CH = min(CORR_SPAN / n, 256 / ceil(n/8)) is computed identically on the host.
-#define CORR_TW 256
-#define CORR_SPAN 2048
-#define CORR_VTAB 512 /* complex128 entries of the period table, within the second buffer */
-#define CORR_PPT 8
-// Chunks per segment: a crystallite's chunks in runs of up to CORR_SEG_CHUNKS (each run ends
-// with one atomic add per FID point, so runs are long enough for that to be negligible).
-#define CORR_SEG_CHUNKS 64
-__device__ __host__ inline int corr_seg_chunks(int nchunk) {
-    return nchunk < CORR_SEG_CHUNKS ? (nchunk < 1 ? 1 : nchunk) : CORR_SEG_CHUNKS;
-}
-__device__ __host__ inline int corr_chunk(int n) {
-    int npg = (n + CORR_PPT - 1) / CORR_PPT;
-    int a = CORR_SPAN / n, b = 256 / npg;
-    int ch = a < b ? a : b;
-    return ch < 1 ? 1 : ch;
-}
-__host__ __device__ inline bool corr_use_fft(int n) {
-    return n >= 16 && (n & (n - 1)) == 0 && n <= 2 * CORR_TW;
-}
-
+// stored as the segment's partial FID part[c][sg][t] (accumulated chunk by chunk by the thread
+// owning t when the FID is too long to hold).  No atomics: each (crystallite, segment, point) has
+// one writer, and k_reduce adds the segments in order, so the FID does not depend on scheduling.
+// CH = min(CORR_SPAN / n, 256 / ceil(n/8)) is computed identically on the host (blfid_dev.h).
+// Q rows are Estr elements apart (Etot padded to 16 elements), so every chunk row starts on a
+// 128-byte line.
 // In-register DFT of RAD = 2, 4 or 8 points, X_k = sum_m x_m e^{SG 2 pi i m k / RAD}, natural
 // order in and out (radix 8 as two radix-4 halves joined by e^{SG i pi k / 4}).
 template <int SG, typename V> __device__ __forceinline__ V mul_si(V a) { // a * (SG i)
@@ -1577,7 +1565,7 @@ __global__ void __launch_bounds__(256, (LN > 0 && sizeof(R) == 4 && NPT == 4) ? 
                                               const double *__restrict__ thetas,
                                               const int *__restrict__ elem_r,
                                               const int *__restrict__ elem_s,
-                                              double *__restrict__ fidc) {
+                                              double *__restrict__ part) {
     typedef typename CX<R>::T CT;
     constexpr bool FFT = LN > 0;
     typedef CorrFft<FFT ? LN : 4> FF;
@@ -1609,8 +1597,8 @@ __global__ void __launch_bounds__(256, (LN > 0 && sizeof(R) == 4 && NPT == 4) ? 
     const int nchunk = (int)((P.Etot + CH - 1) / CH);
     // Task = (crystallite, segment of consecutive chunks); the segmentation depends on the plan
     // only (not on the batch), so a crystallite's FID is summed in the same grouping whatever
-    // the batch size.  For npts <= 2048 the segment's FID partial sums (t = tid + 256 m) stay
-    // in registers and go to fid_c with one atomic add per point at the segment's end.
+    // the batch size.  The segment's FID partial sums (t = tid + 256 m) stay in registers and
+    // are stored once at the segment's end (hold), or accumulated in part[] per chunk.
     const int spc = corr_seg_chunks(nchunk);
     const int nsg = (nchunk + spc - 1) / spc;
     const long ntask = (long)Kb * nsg;
@@ -1649,16 +1637,23 @@ __global__ void __launch_bounds__(256, (LN > 0 && sizeof(R) == 4 && NPT == 4) ? 
                     const int er_ = elem_r[ec], es_ = elem_s[ec];
                     const int rv = P.rt_from_q ? P.elem_rev[ec] : -1;
                     const double *th = thetas + (size_t)c * P.N;
-                    const size_t str = (size_t)(1 << LS) * P.Etot;
-                    const CT *qb = Qt + ((size_t)c * FF::N + j) * P.Etot;
-                    const CT *rb = P.rt_from_q ? qb : Rt + ((size_t)c * FF::N + j) * P.Etot;
+                    const size_t str = (size_t)(1 << LS) * P.Estr;
+                    const CT *qb = Qt + ((size_t)c * FF::N + j) * P.Estr;
+                    const CT *rb = P.rt_from_q ? qb : Rt + ((size_t)c * FF::N + j) * P.Estr;
                     const long rc = P.rt_from_q ? (rv >= 0 ? (long)rv : ec) : ec;
                     const double th_r = th[er_], th_s = th[es_];
                     CT qv[8], rw[8];
 #pragma unroll
                     for (int m = 0; m < 8; m++) qv[m] = qb[(size_t)m * str + ec];
+                    // R = conj(Q) alone when no element of the plan has a reverse element: its
+                    // loads are skipped (a uniform branch on the plan)
+                    if (!P.rt_from_q || P.rt_rev) {
 #pragma unroll
-                    for (int m = 0; m < 8; m++) rw[m] = rb[(size_t)m * str + rc];
+                        for (int m = 0; m < 8; m++) rw[m] = rb[(size_t)m * str + rc];
+                    } else {
+#pragma unroll
+                        for (int m = 0; m < 8; m++) rw[m] = Arith<CT>::zero();
+                    }
                     const double ph = ok ? (th_r - th_s) / (double)FF::N : 0.0;
                     if (j == 0) {
                         sphi[el] = ph;
@@ -1766,13 +1761,13 @@ __global__ void __launch_bounds__(256, (LN > 0 && sizeof(R) == 4 && NPT == 4) ? 
                         for (int k = ka; k < kb; k++) {
                             cd q = cmk(0, 0), r = cmk(0, 0);
                             if (e < P.Etot) {
-                                size_t o = ((size_t)c * n + k) * P.Etot + e;
+                                size_t o = ((size_t)c * n + k) * P.Estr + e;
                                 q = to_cd(Qt[o]);
                                 if (P.rt_from_q) {
                                     // R(s,r) = Q_(J,I)(s,r) + conj(Q_(I,J)(r,s))
                                     r = cconj(q);
                                     const int rv = P.elem_rev[e];
-                                    if (rv >= 0) r = cadd(r, to_cd(Qt[((size_t)c * n + k) * P.Etot + rv]));
+                                    if (rv >= 0) r = cadd(r, to_cd(Qt[((size_t)c * n + k) * P.Estr + rv]));
                                 } else {
                                     r = to_cd(Rt[o]);
                                 }
@@ -1868,12 +1863,19 @@ __global__ void __launch_bounds__(256, (LN > 0 && sizeof(R) == 4 && NPT == 4) ? 
                             }
                         }
                     }
+                    // the first chunk of the segment stores, the later ones add (same thread)
+                    double *dst = part + 2 * (((size_t)c * nsg + sg) * P.npts);
 #pragma unroll
                     for (int m = 0; m < NPT; m++) {
                         if (m < nt) {
                             const int t = t0 + 256 * m;
-                            atomicAdd(&fidc[2 * ((size_t)c * P.npts + t)], acc[m].x);
-                            atomicAdd(&fidc[2 * ((size_t)c * P.npts + t) + 1], acc[m].y);
+                            if (ch == sg * spc) {
+                                dst[2 * t] = acc[m].x;
+                                dst[2 * t + 1] = acc[m].y;
+                            } else {
+                                dst[2 * t] += acc[m].x;
+                                dst[2 * t + 1] += acc[m].y;
+                            }
                         }
                     }
                 }
@@ -1881,12 +1883,13 @@ __global__ void __launch_bounds__(256, (LN > 0 && sizeof(R) == 4 && NPT == 4) ? 
             __syncthreads();
         }
         if (hold) {
+            double *dst = part + 2 * (((size_t)c * nsg + sg) * P.npts);
 #pragma unroll
             for (int m = 0; m < NPT; m++) {
                 if (m < nth) {
-                    const size_t o = 2 * ((size_t)c * P.npts + tid + 256 * m);
-                    atomicAdd(&fidc[o], hacc[m].x);
-                    atomicAdd(&fidc[o + 1], hacc[m].y);
+                    const int t = tid + 256 * m;
+                    dst[2 * t] = hacc[m].x;
+                    dst[2 * t + 1] = hacc[m].y;
                 }
             }
         }
@@ -1894,28 +1897,57 @@ __global__ void __launch_bounds__(256, (LN > 0 && sizeof(R) == 4 && NPT == 4) ? 
 }
 
 // ------------------------------------------------------------------------------ k_reduce
-// acc[t] += sum_c w_c * scale * fid_c[t], Kahan-compensated, crystallites in order.
-__global__ void k_reduce(int Kb, int c0, int npts, const double *__restrict__ w, double scale,
-                         const double *__restrict__ fidc, double *__restrict__ acc,
-                         double *__restrict__ comp) {
-    int q = blockIdx.x * blockDim.x + threadIdx.x; // re/im component index
-    if (q >= 2 * npts) return;
-    double s = acc[q], cmp = comp[q];
-    for (int c = 0; c < Kb; c++) {
-        double y = w[c0 + c] * scale * fidc[(size_t)c * 2 * npts + q] - cmp;
+// Powder sum in a fixed order (oneFID.h:396-407, sequencerun_funcs.cc:698-702): crystallites
+// fall into groups of G consecutive (absolute) indices; thread (component q, group g of the
+// batch) adds fid_c = sum of the crystallite's nparts partial FIDs (in part order) times
+// w_c * scale to the group's Kahan sum (gsum, gcomp), crystallites in order.  A group split
+// across two batches carries its (sum, compensation) pair, so the result is the same bit for bit
+// whatever the batch size.  Layout of grp: [group][0: sums, 1: compensations][2 npts].
+__global__ void k_reduce(int Kb, int c0, int npts, int nparts, int G, const double *__restrict__ w,
+                         double scale, const double *__restrict__ part, double *__restrict__ grp) {
+    const int n2 = 2 * npts;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x; // re/im component index
+    const int g = c0 / G + blockIdx.y;                   // absolute group
+    if (q >= n2) return;
+    const int ca = max(c0, g * G), cb = min(c0 + Kb, (g + 1) * G);
+    if (ca >= cb) return;
+    double *gs = grp + (size_t)g * 2 * n2;
+    double s = gs[q], cmp = gs[n2 + q];
+    for (int c = ca; c < cb; c++) {
+        const double *pc = part + (size_t)(c - c0) * nparts * n2 + q;
+        double f = 0.0;
+        for (int p = 0; p < nparts; p++) f += pc[(size_t)p * n2];
+        double y = w[c] * scale * f - cmp;
         double t = s + y;
         cmp = (t - s) - y;
         s = t;
     }
-    acc[q] = s;
-    comp[q] = cmp;
+    gs[q] = s;
+    gs[n2 + q] = cmp;
 }
 
-__global__ void k_finish(int npts, const double *__restrict__ acc, double2 norm,
+// FID point t: the groups' sums added in group order (Kahan), their compensations subtracted,
+// divided by norm = Tr(D D^H) (or 1).
+__global__ void k_finish(int npts, int ngroups, const double *__restrict__ grp, double2 norm,
                          double *__restrict__ out) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= npts) return;
-    cd v = cmk(acc[2 * t], acc[2 * t + 1]);
+    const int n2 = 2 * npts;
+    double acc[2];
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        double s = 0.0, cmp = 0.0, lost = 0.0;
+        for (int g = 0; g < ngroups; g++) {
+            const double *gs = grp + (size_t)g * 2 * n2;
+            double y = gs[2 * t + h] - cmp;
+            double u = s + y;
+            cmp = (u - s) - y;
+            s = u;
+            lost += gs[n2 + 2 * t + h];
+        }
+        acc[h] = s - (cmp + lost);
+    }
+    cd v = cmk(acc[0], acc[1]);
     // v / norm (complex division; norm = Tr(D D^H) is real positive in practice)
     double d = norm.x * norm.x + norm.y * norm.y;
     cd r = cmk((v.x * norm.x + v.y * norm.y) / d, (v.y * norm.x - v.x * norm.y) / d);
@@ -1959,13 +1991,14 @@ __global__ void __launch_bounds__(256) k_static_h(DevPlan P, int Kb, int c0, con
 //                317-325): fid[t] += A(i,j) (conj(e_i) e_j)^t;
 //   ptop FID:    lam = eigenphases of the period propagator U, w0 = 1, no pruning: this is
 //                Tr(rho_t det) with rho_{t+1} = U rho_t U^H (sequencerun.cc:164-196).
+// The pair's FID goes to its own partial sum part[c][pair][t] (one writer, no atomics).
 template <int M>
 __global__ void __launch_bounds__(NThr<M>::v) k_static_fid(DevPlan P, int Kb, const int *__restrict__ pgrp,
                                                     int ngrp, const cd *__restrict__ bufW,
                                                     const double *__restrict__ lam,
                                                     cd *__restrict__ scratch, size_t slot,
                                                     double w0, int prune,
-                                                    double *__restrict__ fidc) {
+                                                    double *__restrict__ part) {
     constexpr int NT = NThr<M>::v;
     __shared__ __align__(16) cd smem[2 * 16 * 16 * M];
     const int tid = threadIdx.x;
@@ -2003,8 +2036,9 @@ __global__ void __launch_bounds__(NThr<M>::v) k_static_fid(DevPlan P, int Kb, co
                 int r = e % bI, s = e / bI;
                 cfma(acc, a, cexpi(w0 * (double)t * (lJ[s] - lI[r])));
             }
-            atomicAdd(&fidc[2 * ((size_t)c * P.npts + t)], acc.x);
-            atomicAdd(&fidc[2 * ((size_t)c * P.npts + t) + 1], acc.y);
+            double *dst = part + 2 * (((size_t)c * P.nparts + pr) * P.npts + t);
+            dst[0] = acc.x;
+            dst[1] = acc.y;
         }
         __syncthreads();
     }
@@ -2166,32 +2200,32 @@ extern "C" int blfid_dev_launch_qr(int prec, int M, const DevPlan *P, int Kb, co
 
 template <typename R, int LN, int NPT>
 static void launch_corr_t(int grid, const DevPlan *P, int Kb, const void *Qt, const void *Rt,
-                          const double *thetas, const int *er, const int *es, double *fidc,
+                          const double *thetas, const int *er, const int *es, double *part,
                           hipStream_t st) {
     typedef typename CX<R>::T CT;
     k_corr<R, LN, NPT><<<grid, 256, 0, st>>>(*P, Kb, (const CT *)Qt, (const CT *)Rt, thetas, er,
-                                              es, fidc);
+                                              es, part);
 }
 // FFT variants by log2 n (corr_use_fft(n): 16 <= n <= 512), LN = 0 the direct sum
 template <typename R, int NPT>
 static void launch_corr(int grid, const DevPlan *P, int Kb, const void *Qt, const void *Rt,
-                        const double *thetas, const int *er, const int *es, double *fidc,
+                        const double *thetas, const int *er, const int *es, double *part,
                         hipStream_t st) {
     const int ln = corr_use_fft(P->n) ? __builtin_ctz((unsigned)P->n) : 0;
     switch (ln) {
-    case 4: launch_corr_t<R, 4, NPT>(grid, P, Kb, Qt, Rt, thetas, er, es, fidc, st); break;
-    case 5: launch_corr_t<R, 5, NPT>(grid, P, Kb, Qt, Rt, thetas, er, es, fidc, st); break;
-    case 6: launch_corr_t<R, 6, NPT>(grid, P, Kb, Qt, Rt, thetas, er, es, fidc, st); break;
-    case 7: launch_corr_t<R, 7, NPT>(grid, P, Kb, Qt, Rt, thetas, er, es, fidc, st); break;
-    case 8: launch_corr_t<R, 8, NPT>(grid, P, Kb, Qt, Rt, thetas, er, es, fidc, st); break;
-    case 9: launch_corr_t<R, 9, NPT>(grid, P, Kb, Qt, Rt, thetas, er, es, fidc, st); break;
-    default: launch_corr_t<R, 0, NPT>(grid, P, Kb, Qt, Rt, thetas, er, es, fidc, st); break;
+    case 4: launch_corr_t<R, 4, NPT>(grid, P, Kb, Qt, Rt, thetas, er, es, part, st); break;
+    case 5: launch_corr_t<R, 5, NPT>(grid, P, Kb, Qt, Rt, thetas, er, es, part, st); break;
+    case 6: launch_corr_t<R, 6, NPT>(grid, P, Kb, Qt, Rt, thetas, er, es, part, st); break;
+    case 7: launch_corr_t<R, 7, NPT>(grid, P, Kb, Qt, Rt, thetas, er, es, part, st); break;
+    case 8: launch_corr_t<R, 8, NPT>(grid, P, Kb, Qt, Rt, thetas, er, es, part, st); break;
+    case 9: launch_corr_t<R, 9, NPT>(grid, P, Kb, Qt, Rt, thetas, er, es, part, st); break;
+    default: launch_corr_t<R, 0, NPT>(grid, P, Kb, Qt, Rt, thetas, er, es, part, st); break;
     }
 }
 
 extern "C" int blfid_dev_launch_corr(int prec, const DevPlan *P, int Kb, const void *Qt,
                                      const void *Rt, const double *thetas, const int *er,
-                                     const int *es, double *fidc, hipStream_t st) {
+                                     const int *es, double *part, hipStream_t st) {
     int CH = corr_chunk(P->n);
     int nchunk = (int)((P->Etot + CH - 1) / CH);
     int spc = corr_seg_chunks(nchunk);
@@ -2200,26 +2234,29 @@ extern "C" int blfid_dev_launch_corr(int prec, const DevPlan *P, int Kb, const v
     if (grid <= 0) return 0;
     // four points per thread up to 1024 FID points, eight above
     if (P->npts <= 1024) {
-        if (prec) launch_corr<float, 4>(grid, P, Kb, Qt, Rt, thetas, er, es, fidc, st);
-        else launch_corr<double, 4>(grid, P, Kb, Qt, Rt, thetas, er, es, fidc, st);
+        if (prec) launch_corr<float, 4>(grid, P, Kb, Qt, Rt, thetas, er, es, part, st);
+        else launch_corr<double, 4>(grid, P, Kb, Qt, Rt, thetas, er, es, part, st);
     } else {
-        if (prec) launch_corr<float, 8>(grid, P, Kb, Qt, Rt, thetas, er, es, fidc, st);
-        else launch_corr<double, 8>(grid, P, Kb, Qt, Rt, thetas, er, es, fidc, st);
+        if (prec) launch_corr<float, 8>(grid, P, Kb, Qt, Rt, thetas, er, es, part, st);
+        else launch_corr<double, 8>(grid, P, Kb, Qt, Rt, thetas, er, es, part, st);
     }
     return (int)hipGetLastError();
 }
 
-extern "C" int blfid_dev_launch_reduce(int Kb, int c0, int npts, const double *w, double scale,
-                                       const double *fidc, double *acc, double *comp,
-                                       hipStream_t st) {
-    int n2 = 2 * npts;
-    k_reduce<<<(n2 + 255) / 256, 256, 0, st>>>(Kb, c0, npts, w, scale, fidc, acc, comp);
+extern "C" int blfid_dev_launch_reduce(int Kb, int c0, int npts, int nparts, int G,
+                                       const double *w, double scale, const double *part,
+                                       double *grp, hipStream_t st) {
+    if (Kb <= 0) return 0;
+    const int n2 = 2 * npts;
+    const int g0 = c0 / G, g1 = (c0 + Kb - 1) / G;
+    dim3 grid((n2 + 255) / 256, g1 - g0 + 1);
+    k_reduce<<<grid, 256, 0, st>>>(Kb, c0, npts, nparts, G, w, scale, part, grp);
     return (int)hipGetLastError();
 }
 
-extern "C" int blfid_dev_launch_finish(int npts, const double *acc, double nre, double nim,
-                                       double *out, hipStream_t st) {
-    k_finish<<<(npts + 255) / 256, 256, 0, st>>>(npts, acc, make_double2(nre, nim), out);
+extern "C" int blfid_dev_launch_finish(int npts, int ngroups, const double *grp, double nre,
+                                       double nim, double *out, hipStream_t st) {
+    k_finish<<<(npts + 255) / 256, 256, 0, st>>>(npts, ngroups, grp, make_double2(nre, nim), out);
     return (int)hipGetLastError();
 }
 
@@ -2252,16 +2289,16 @@ extern "C" int blfid_dev_launch_static_h(const DevPlan *P, int Kb, int c0, const
 extern "C" int blfid_dev_launch_static_fid(int M, const DevPlan *P, int Kb, const int *pgrp,
                                            int ngrp, const cd *bufW, const double *lam,
                                            cd *scratch, size_t slot, int nslots, double w0,
-                                           int prune, double *fidc, hipStream_t st) {
+                                           int prune, double *part, hipStream_t st) {
     long ntask = (long)Kb * ngrp;
     int grid = (int)(ntask < nslots ? ntask : nslots);
     if (grid <= 0) return 0;
     switch (M) {
-    case 1: k_static_fid<1><<<grid, 64, 0, st>>>(*P, Kb, pgrp, ngrp, bufW, lam, scratch, slot, w0, prune, fidc); break;
-    case 2: k_static_fid<2><<<grid, 256, 0, st>>>(*P, Kb, pgrp, ngrp, bufW, lam, scratch, slot, w0, prune, fidc); break;
-    case 3: k_static_fid<3><<<grid, 256, 0, st>>>(*P, Kb, pgrp, ngrp, bufW, lam, scratch, slot, w0, prune, fidc); break;
-    case 4: k_static_fid<4><<<grid, 256, 0, st>>>(*P, Kb, pgrp, ngrp, bufW, lam, scratch, slot, w0, prune, fidc); break;
-    default: k_static_fid<5><<<grid, 256, 0, st>>>(*P, Kb, pgrp, ngrp, bufW, lam, scratch, slot, w0, prune, fidc); break;
+    case 1: k_static_fid<1><<<grid, 64, 0, st>>>(*P, Kb, pgrp, ngrp, bufW, lam, scratch, slot, w0, prune, part); break;
+    case 2: k_static_fid<2><<<grid, 256, 0, st>>>(*P, Kb, pgrp, ngrp, bufW, lam, scratch, slot, w0, prune, part); break;
+    case 3: k_static_fid<3><<<grid, 256, 0, st>>>(*P, Kb, pgrp, ngrp, bufW, lam, scratch, slot, w0, prune, part); break;
+    case 4: k_static_fid<4><<<grid, 256, 0, st>>>(*P, Kb, pgrp, ngrp, bufW, lam, scratch, slot, w0, prune, part); break;
+    default: k_static_fid<5><<<grid, 256, 0, st>>>(*P, Kb, pgrp, ngrp, bufW, lam, scratch, slot, w0, prune, part); break;
     }
     return (int)hipGetLastError();
 }

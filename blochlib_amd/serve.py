@@ -44,6 +44,7 @@ class Request:
     t_admit: float = 0.0
     t_done: float = 0.0
     footprint: float = 0.0
+    error: Exception | None = None  # a failed request carries its error; the others go on
 
     @property
     def latency(self) -> float:
@@ -120,14 +121,15 @@ class FIDServer:
                 r.rotor, r.sw, r.npts, precision)
 
     def _plan(self, r: Request):
-        """A plan for the request: a cached one (its arena bytes leave the cache account) or a
-        new one."""
+        """A plan for the request and the bytes it held in the cache: a cached one (its arena
+        bytes leave the cache account, and go back with it if it is not admitted after all) or
+        a new one (0 bytes)."""
         k = self.key(r, self.precision)
         hit = self.cache.pop(k, None)
         if hit is not None:
             self.used -= hit[1]
-            return k, hit[0]
-        return k, engine.Plan(r.system, r.wr, r.rotor, r.sw, r.npts, precision=self.precision)
+            return k, hit[0], hit[1]
+        return k, engine.Plan(r.system, r.wr, r.rotor, r.sw, r.npts, precision=self.precision), 0.0
 
     def _release(self, k, p, nbytes):
         """Keep a finished plan (its device arena stays allocated and counted) or drop it."""
@@ -155,8 +157,19 @@ class FIDServer:
         st = p.stats()
         return st["fixed_bytes"] + st["bytes_per_crystallite"] * min(K, self.min_batch)
 
+    def _fail(self, r: Request, err: Exception, t: float, done: list, on_done):
+        """Completes a request with its error (bad system, unsupported input, out of memory):
+        the server and the other requests go on."""
+        r.error = err
+        r.fid = None
+        r.t_done = t
+        done.append(r)
+        if on_done:
+            on_done(r)
+
     def run(self, requests, clock=time.perf_counter, on_done=None) -> list:
-        """Serve `requests` (arrival times in seconds from now); returns them completed."""
+        """Serve `requests` (arrival times in seconds from now); returns them completed (a
+        request that failed has `error` set and no FID)."""
         todo = deque(sorted(requests, key=lambda r: r.arrival))
         queue: deque = deque()
         done = []
@@ -172,23 +185,37 @@ class FIDServer:
                 if not free:
                     break
                 r = queue[0]
-                k, p = self._plan(r)
-                need = self.footprint(p, len(r.powder))
+                try:
+                    k, p, held = self._plan(r)
+                    need = self.footprint(p, len(r.powder))
+                except engine.BlochLibError as e:
+                    queue.popleft()
+                    self._fail(r, e, clock() - t0, done, on_done)
+                    progressed = True
+                    continue
                 while need > self.budget - self.used and self._evict():
                     pass
                 if need > self.budget - self.used and not any(x.req for x in self.slots):
                     self._bury()
                 room = self.budget - self.used
                 if need > room and self.used > 0:
-                    self._release(k, p, 0.0)
+                    self._release(k, p, held)
                     break
                 share = max(need, min(room, need * 4))
                 queue.popleft()
-                p.set_memory_budget(share)
-                p.set_operators(r.rho, r.det)
-                p.set_powder(r.powder)
                 sl = free[0]
-                p.run(True, stream=sl.stream.value)
+                try:
+                    p.set_memory_budget(share)
+                    p.set_operators(r.rho, r.det)
+                    p.set_powder(r.powder)
+                    p.run(True, stream=sl.stream.value)
+                except engine.BlochLibError as e:
+                    # the plan may hold work enqueued before the failure: it is retired, not
+                    # cached, and destroyed once nothing is in flight
+                    self.graveyard.append((p, 0.0))
+                    self._fail(r, e, clock() - t0, done, on_done)
+                    progressed = True
+                    continue
                 self.hip.hipEventRecord(sl.event, sl.stream)
                 sl.req, sl.plan, sl.key = r, p, k
                 r.footprint = share
@@ -198,15 +225,19 @@ class FIDServer:
             for sl in self.slots:
                 if sl.req is not None and self.hip.hipEventQuery(sl.event) == 0:
                     r = sl.req
-                    r.fid = sl.plan.fetch()
-                    r.sw_out = sl.plan.sw
-                    r.t_done = clock() - t0
                     self.used -= r.footprint
-                    self._release(sl.key, sl.plan, r.footprint)
+                    try:
+                        r.fid = sl.plan.fetch()
+                        r.sw_out = sl.plan.sw
+                        r.t_done = clock() - t0
+                        self._release(sl.key, sl.plan, r.footprint)
+                        done.append(r)
+                        if on_done:
+                            on_done(r)
+                    except engine.BlochLibError as e:
+                        self.graveyard.append((sl.plan, 0.0))
+                        self._fail(r, e, clock() - t0, done, on_done)
                     sl.req, sl.plan, sl.key = None, None, None
-                    done.append(r)
-                    if on_done:
-                        on_done(r)
                     progressed = True
             if self.graveyard and not any(sl.req is not None for sl in self.slots):
                 self._bury()

@@ -121,7 +121,9 @@ def test_repeat_runs_are_consistent_and_batches_agree():
     p.set_memory_budget(st["fixed_bytes"] + 3.5 * st["bytes_per_crystallite"])  # batches of 3
     p.run(True)
     b = p.fetch()
-    assert rel_l2(b, a) < 1e-13
+    # the powder sum runs in a fixed order (crystallite groups of 64 carried across batches),
+    # so the batch size does not change a single bit
+    assert np.array_equal(b, a)
 
 
 def test_shared_workspace_two_plans_small_batches():
@@ -171,7 +173,7 @@ def test_ten_spin_dense_shape_invariants():
     p.set_memory_budget(st["fixed_bytes"] + 2.5 * st["bytes_per_crystallite"])
     p.run(True)
     b = p.fetch()
-    assert rel_l2(b, a) < 1e-13
+    assert np.array_equal(b, a)
     p.close()
 
 

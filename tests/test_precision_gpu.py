@@ -74,17 +74,29 @@ def test_static_path_is_fp64_in_both_precisions():
     assert np.array_equal(a, b)
 
 
-def test_config4_ptop_subset_c64():
+def test_config4_ptop_full_length_c64():
+    """Config-4 pulse program at its full 2048 rotor-synchronised points: the complex64 plan
+    against the FP64 plan on three crystallites and against the oracle on one.  The period
+    propagator's error is multiplied by the point index here, so every precision forms it in
+    FP64 (a complex64 chain differs from FP64 by ~2.5e-4 per crystallite at 2048 points)."""
     w = workloads.config4(1)[0]
     rho, det = w.operators()
     pw = zcw(1154).subset(np.array([3, 500, 1000]))
-    print(f"[c64] config4-set0 K={len(pw)} npts=48", flush=True)
-    g = engine.ptop_fid(w.system, w.wr, w.rotor, w.steps, 48, pw, rho, det, rf=w.rf,
+    print(f"[c64] config4-set0 K={len(pw)} npts=2048", flush=True)
+    g = engine.ptop_fid(w.system, w.wr, w.rotor, w.steps, 2048, pw, rho, det, rf=w.rf,
                         precision="c64")
-    c = O.ptop_fid(w.system, w.wr, w.rotor, w.steps, 48, pw, rho, det, rf=w.rf)
-    err = rel_l2(g, c)
-    print(f"[c64] config4-set0: rel-L2 {err:.3e}", flush=True)
+    f = engine.ptop_fid(w.system, w.wr, w.rotor, w.steps, 2048, pw, rho, det, rf=w.rf,
+                        precision="fp64")
+    err = rel_l2(g, f)
+    print(f"[c64] config4-set0: rel-L2 vs FP64 form {err:.3e}", flush=True)
     assert err < TOL_C64
+    one = pw.subset(np.array([0]))
+    g1 = engine.ptop_fid(w.system, w.wr, w.rotor, w.steps, 2048, one, rho, det, rf=w.rf,
+                         precision="c64")
+    c1 = O.ptop_fid(w.system, w.wr, w.rotor, w.steps, 2048, one, rho, det, rf=w.rf)
+    err1 = rel_l2(g1, c1)
+    print(f"[c64] config4-set0 crystallite 3: rel-L2 vs oracle {err1:.3e}", flush=True)
+    assert err1 < TOL_C64
 
 
 def test_ten_spin_c64_against_fp64_engine():
