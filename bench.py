@@ -68,18 +68,33 @@ def corr_flops_per_elem(n: int, P: int) -> float:
     return c + 18.0 * n + 14.0 * P
 
 
-def corr_bytes_per_elem_step(wls, precision: str) -> float:
-    """Bytes k_corr must read per correlation element and step: Q in the storage precision,
-    plus the same again when R is not conj(Q) alone -- rho is not (det + det^H) / 2, or det has
-    entries in both (a, b) and (b, a) so that some elements have a reverse element."""
-    cs = 16.0 if precision == "fp64" else 8.0
-    for w in wls:
-        rho, det = w.operators()
-        scale = max(np.abs(det).max(), 1.0)
-        rosym = np.abs(rho - 0.5 * (det + det.conj().T)).max() <= 1e-12 * scale
-        if not rosym or np.any((det != 0) & (det.T != 0)):
-            return 2.0 * cs
-    return cs
+# What each kernel family issues on MI355X, per precision: "mfma" where its products run on the
+# matrix cores (complex64 Dyson chain, B = A W and Q products), "valu" where they run on the
+# vector ALUs (the propagator's real Taylor GEMMs, the FFT correlation, every FP64 product).
+ISSUES = {"k_exp": "valu", "k_dyson": "mfma", "k_bmul": "mfma", "k_qr": "mfma",
+          "k_corr": "valu", "k_eig": "valu", "k_reduce": "valu", "k_static_fid": "valu"}
+
+
+def family(kernel: str) -> str:
+    return kernel.split("<")[0]
+
+
+def cpu_info():
+    """Host cores this process may use (capped at the 16-core share of one GPU on the box) and
+    the CPU model name."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return min(avail, 16), model
 
 
 def profiled_traffic(kernel: str, precision: str):
@@ -119,6 +134,43 @@ def profiled_traffic(kernel: str, precision: str):
             "source": os.path.relpath(path, ROOT)}
 
 
+def family_roofline(fam: str, d: dict, precision: str) -> dict:
+    """Roofline of one kernel family over the timed steps (all its tile variants): executed
+    algorithmic flops and algorithmic HBM bytes (as the engine counts them per launch) over the
+    family's summed launch time.  The bound follows what the family issues (ISSUES): matrix-
+    core ("mfma") or vector ("valu") FP32 for complex64, vector FP64 for the FP64 form; when its
+    flops per algorithmic byte fall below MI355X's ridge point it is "hbm" with GB/s."""
+    fp64 = precision == "fp64"
+    bound = "valu" if fp64 else ISSUES.get(fam, "valu")
+    peak = FP64_PEAK_TFLOPS if fp64 else FP32_PEAK_TFLOPS
+    sec = max(d["ms"], 1e-9) * 1e-3
+    launches = max(d["launches"], 1)
+    tflops = d["flops"] / sec / 1e12
+    gbs = d["bytes"] / sec / 1e9
+    out = {"kernel": fam, "bound": bound, "achieved": tflops, "peak": peak, "unit": "TFLOP/s",
+           "frac": tflops / peak, "traffic": None, "avg_launch_ms": d["ms"] / launches,
+           "launches": d["launches"], "ms_total": d["ms"],
+           "flops_per_launch": d["flops"] / launches,
+           "alg_bytes_per_launch": d["bytes"] / launches, "alg_gbs": gbs}
+    if d["flops"] > 0 and d["bytes"] > 0 and \
+            d["flops"] / d["bytes"] < peak * 1e12 / (HBM_PEAK_GBS * 1e9):
+        out.update({"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": gbs / HBM_PEAK_GBS, "achieved_tflops": tflops,
+                    "flop_peak_tflops": peak})
+    elif d["flops"] == 0 and d["bytes"] > 0:
+        # no flops counted (the iterative eigen-solver): its bytes against HBM
+        out.update({"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": gbs / HBM_PEAK_GBS})
+    out["note"] = ("achieved = executed algorithmic flops (block-diagonal algorithm, unpadded "
+                   "sizes) or algorithmic HBM bytes of the family per launch / its average "
+                   "launch time (HIP events on the launch stream); peak = MI355X published "
+                   + ("FP64" if fp64 else "FP32 (matrix for mfma, vector for valu; the vector "
+                      "figure assumes packed FP32, which this build does not emit, so its "
+                      "practical vector ceiling is half)")
+                   + "; bound from what the family issues")
+    return out
+
+
 def hip():
     L = ctypes.CDLL("libamdhip64.so")
     L.hipDeviceSynchronize.restype = ctypes.c_int
@@ -154,75 +206,96 @@ def copy_calibration(H, mib: int):
 
 
 def cpu_baseline(wls, sample_mode: str):
-    """Time the oracle on one crystallite of a representative cluster per spin count,
-    each sample on its own thread (single-threaded oracle), and extrapolate the config's
-    crystallite-steps/s per core from the cluster mix."""
+    """The oracle (the reference gamma-COMPUTE algorithm restated in C, FP64, dense N x N, its
+    GEMMs on a register-blocked 4x4 complex AVX2 kernel) threaded over crystallites on all the
+    host cores of this process (capped at 16, one GPU's share of the box).  For each spin count
+    of the cluster mix, C crystallites of one cluster (one per thread) are timed as a whole;
+    median of 3 repetitions (one for 8 spins, whose single pass already takes ~30-60 s).  The
+    crystallite-steps/s of the mix follow from those per-spin-count rates and the mix's
+    composition.  sample_mode "quick" skips 8 spins (scaled from 7 by N^3, and said so)."""
     from oracle import oracle as O
+    cores, model = cpu_info()
     reps = {}
-    for w in wls:
-        reps.setdefault(w.meta["spins"], w)
     counts = {}
     for w in wls:
+        reps.setdefault(w.meta["spins"], w)
         counts[w.meta["spins"]] = counts.get(w.meta["spins"], 0) + 1
     spins = sorted(reps)
-    measure = spins if sample_mode == "full" else [s for s in spins if s <= 7]
-
-    def one(S):
+    measure = spins if sample_mode == "full" else [x for x in spins if x <= 7]
+    sec = {}  # wall seconds per crystallite with `cores` threads busy
+    detail = {}
+    t_all = time.perf_counter()
+    for S in measure:
         w = reps[S]
-        pw = w.powder().subset([0])
         rho, det = w.operators()
-        t = time.perf_counter()
-        O.powder_fid(w.system, w.wr, w.rotor, w.sw, w.npts, pw, rho, det, nthreads=1)
-        return S, time.perf_counter() - t
-
-    with cf.ThreadPoolExecutor(max_workers=len(measure)) as ex:
-        t_s = dict(ex.map(one, measure))
-    if sample_mode != "full":  # extrapolate the largest spin counts by N^3
-        for S in spins:
-            if S not in t_s:
-                t_s[S] = t_s[max(t_s)] * 8.0 ** (S - max(t_s))
+        pw = w.powder().subset(np.arange(cores) * 37 % w.powder_size)
+        runs = []
+        for _ in range(3 if S <= 7 else 1):
+            t = time.perf_counter()
+            O.powder_fid(w.system, w.wr, w.rotor, w.sw, w.npts, pw, rho, det, nthreads=cores)
+            runs.append(time.perf_counter() - t)
+        sec[S] = float(np.median(runs)) / cores
+        detail[S] = [round(x, 2) for x in runs]
+        print(f"[bench] cpu baseline: {S} spins, {cores} crystallites on {cores} threads: "
+              f"{detail[S]} s", flush=True)
+    scaled = [S for S in spins if S not in sec]
+    for S in scaled:
+        sec[S] = sec[max(measure)] * 8.0 ** (S - max(measure))
     n = wls[0].steps
     K = wls[0].powder_size
     cs_total = sum(counts[S] * K * n for S in spins)
-    sec_total = sum(counts[S] * K * t_s[S] for S in spins)
+    sec_total = sum(counts[S] * K * sec[S] for S in spins)
     return {
         "value": cs_total / sec_total,
         "unit": "crystallite-steps/s",
-        "cores": 1,
+        "cores": cores,
         "kind": "port",
-        "sample": ("oracle (reference gamma-COMPUTE algorithm in C, FP64, dense N x N) on 1 "
-                   f"crystallite of the first cluster with S spins for S in {measure}, each on "
-                   "its own thread; rate extrapolated over the 64-cluster mix by spin count "
-                   f"(seconds per crystallite: { {s: round(v, 3) for s, v in t_s.items()} })"),
-        "seconds_per_crystallite": {str(k): v for k, v in t_s.items()},
+        "cpu_model": model,
+        "sample": ("oracle (reference gamma-COMPUTE algorithm in C, FP64, dense N x N, GEMMs on a "
+                   "register-blocked 4x4 complex AVX2/FMA kernel, the Fave sum with a complex exp "
+                   f"per term as the reference) on {cores} threads, {cores} crystallites of one "
+                   f"cluster per spin count {measure} timed as a whole (wall s per repetition "
+                   f"{detail}; median of 3, one pass for 8 spins), weighted by the 64-cluster mix"
+                   + (f"; spin counts {scaled} scaled by N^3" if scaled else "")
+                   + f"; {time.perf_counter() - t_all:.0f} s of sampling on {model}"),
+        "seconds_per_crystallite_all_cores": {str(k): v for k, v in sec.items()},
         "fids_per_s": len(wls) / sec_total,
     }
 
 
 def cpu_baseline_ptop(w):
-    """Oracle (explicit Dyson product with rf, then rho <- U rho U^H per point, FP64, one
-    thread) on one crystallite of one parameter set: the period propagator and the per-point
-    cost are timed separately (16 and 48 points) and combined for the full point count."""
+    """Oracle (explicit Dyson product with rf, then rho <- U rho U^H per point, FP64) threaded
+    over crystallites on all host cores of this process (capped at 16): C crystallites of one
+    parameter set at 16 and at 48 points (median of 3 each) separate the period propagator from
+    the per-point cost, combined for the full point count."""
     from oracle import oracle as O
-    pw = w.powder().subset([0])
+    cores, model = cpu_info()
+    pw = w.powder().subset(np.arange(cores) * 37 % w.powder_size)
     rho, det = w.operators()
     t = []
     for npts in (16, 48):
-        t0 = time.perf_counter()
-        O.ptop_fid(w.system, w.wr, w.rotor, w.steps, npts, pw, rho, det, rf=w.rf)
-        t.append(time.perf_counter() - t0)
+        runs = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            O.ptop_fid(w.system, w.wr, w.rotor, w.steps, npts, pw, rho, det, rf=w.rf,
+                       nthreads=cores)
+            runs.append(time.perf_counter() - t0)
+        t.append(float(np.median(runs)) / cores)
     per_point = max((t[1] - t[0]) / 32.0, 0.0)
     fixed = max(t[0] - 16 * per_point, 0.0)
     sec = fixed + w.npts * per_point
     return {
         "value": w.steps / sec,
         "unit": "crystallite-steps/s",
-        "cores": 1,
+        "cores": cores,
         "kind": "port",
+        "cpu_model": model,
         "sample": ("oracle (explicit rf Dyson product + point-by-point U rho U^H, FP64, dense "
-                   f"N=128) on 1 crystallite of parameter set 0, timed at 16 and 48 points: "
-                   f"{fixed:.3f} s for the period propagator + {per_point * 1e3:.2f} ms per "
-                   f"point, combined for {w.npts} points = {sec:.2f} s per crystallite"),
+                   f"N=128, blocked AVX2 GEMMs) on {cores} threads, {cores} crystallites of "
+                   f"parameter set 0 at 16 and 48 points (median of 3): {fixed:.3f} s for the "
+                   f"period propagator + {per_point * 1e3:.2f} ms per point per crystallite with "
+                   f"all threads busy, combined for {w.npts} points = {sec:.3f} s per "
+                   f"crystallite; {model}"),
     }
 
 
@@ -367,10 +440,9 @@ def measure(H, wls, ptop, precision, steps, warmup, args, calibrate=False):
     agg = {}
     for p in plans:
         for k, v in p.phase_times().items():
-            a = agg.setdefault(k, {"ms": 0.0, "launches": 0, "flops": 0.0})
-            a["ms"] += v["ms"]
-            a["launches"] += v["launches"]
-            a["flops"] += v["flops"]
+            a = agg.setdefault(k, {"ms": 0.0, "launches": 0, "flops": 0.0, "bytes": 0.0})
+            for q in a:
+                a[q] += v[q]
     fids = [p.fetch() for p in plans]
     for p in plans:
         p.close()
@@ -423,11 +495,15 @@ def main():
     m = measure(H, wls, ptop, args.precision, args.steps, args.warmup, args, calibrate=True)
     elapsed, lat, agg = m["elapsed"], m["lat"], m["agg"]
     nst = m["streams"]
-    dom = max(agg, key=lambda k: agg[k]["ms"])
+    fams = {}
+    for k, v in agg.items():
+        f = fams.setdefault(family(k), {"ms": 0.0, "launches": 0, "flops": 0.0, "bytes": 0.0})
+        for q in f:
+            f[q] += v[q]
+    # the dominant kernel family by summed kernel time over the timed steps
+    dom = max(fams, key=lambda k: fams[k]["ms"])
+    roof = family_roofline(dom, fams[dom], args.precision)
     traffic = None if ptop else profiled_traffic(dom, args.precision)
-    d = agg[dom]
-    avg_ms = d["ms"] / d["launches"]
-    achieved = d["flops"] / d["launches"] / (avg_ms * 1e-3) / 1e12
     value = cs_per_step * args.steps / elapsed
     peak = FP64_PEAK_TFLOPS if args.precision == "fp64" else FP32_PEAK_TFLOPS
     out = {
@@ -475,55 +551,23 @@ def main():
         "kernels": {k: {"ms_total": v["ms"], "launches": v["launches"],
                         "tflops": v["flops"] / max(v["ms"], 1e-9) / 1e9}
                     for k, v in sorted(agg.items(), key=lambda kv: -kv[1]["ms"])},
-        "roofline": {
-            "kernel": dom,
-            "bound": "mfma",
-            "achieved": achieved,
-            "peak": peak,
-            "unit": "TFLOP/s",
-            "frac": achieved / peak,
-            "traffic": traffic["bytes"] if traffic else None,
-            "avg_launch_ms": avg_ms,
-            "flops_per_launch": d["flops"] / d["launches"],
-            "note": ("achieved = executed algorithmic flops of the kernel (block-diagonal "
-                     "algorithm, unpadded sizes) per launch / its average launch time (HIP events "
-                     "on the launch stream); peak = MI355X "
-                     + ("FP64" if args.precision == "fp64" else "FP32") + " dense"),
-        },
+        "roofline": roof,
+        "families": {k: family_roofline(k, v, args.precision)
+                     for k, v in sorted(fams.items(), key=lambda kv: -kv[1]["ms"])},
     }
     if traffic:
-        out["roofline"]["traffic_detail"] = traffic
-        out["roofline"]["achieved_hbm_gbs"] = traffic["bytes"] / (avg_ms * 1e-3) / 1e9
+        # counter-measured HBM bytes per launch of the family (committed profile of this
+        # command) against its algorithmic bytes per launch: re-reads and spills show above 1
+        roof["traffic"] = traffic["bytes"]
+        roof["traffic_ratio"] = traffic["bytes"] / max(roof["alg_bytes_per_launch"], 1.0)
+        roof["traffic_detail"] = traffic
     if dom == "k_corr":
         n, P = wls[0].steps, wls[0].npts
-        # algorithmic HBM bytes: every step of every correlation element is read once as Q, and
-        # a second time (the reverse element of Q, or the stored R) unless R = conj(Q) there
-        elems = d["flops"] / d["launches"] / corr_flops_per_elem(n, P)
-        per_step = corr_bytes_per_elem_step(wls, args.precision)
-        alg_bytes = elems * n * per_step
-        if d["flops"] / d["launches"] / alg_bytes < peak * 1e12 / (HBM_PEAK_GBS * 1e9):
-            # below the ridge point: HBM bound
-            out["roofline"].update({
-                "bound": "hbm", "achieved": alg_bytes / (avg_ms * 1e-3) / 1e9,
-                "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": alg_bytes / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                "bytes_per_launch": alg_bytes, "achieved_tflops": achieved,
-                "flop_peak_tflops": peak,
-                "note": ("achieved = algorithmic HBM bytes per launch (%g B per correlation "
-                         "element-step) / the average launch time (HIP events on the launch "
-                         "stream); the kernel's executed flops per algorithmic byte are below "
-                         "MI355X's ridge point, so HBM (8 TB/s) is its roof; achieved_tflops = "
-                         "executed flops rate" % per_step),
-            })
-        else:
-            out["roofline"]["bytes_per_launch"] = alg_bytes
-            out["roofline"]["note"] += (
-                "; algorithmic HBM bytes %g per correlation element-step (%.3g GB per launch): "
-                "executed flops per byte above the ridge point" % (per_step, alg_bytes / 1e9))
+        d = fams[dom]
         ratio = (14.0 * n * n + 14.0 * P) / corr_flops_per_elem(n, P)
-        out["roofline"]["canonical_flops_per_launch"] = ratio * d["flops"] / d["launches"]
-        out["roofline"]["canonical_tflops"] = ratio * achieved
-        out["roofline"]["canonical_note"] = (
+        roof["canonical_flops_per_launch"] = ratio * d["flops"] / d["launches"]
+        roof["canonical_tflops"] = ratio * d["flops"] / (d["ms"] * 1e-3) / 1e12
+        roof["canonical_note"] = (
             "reference algorithm's count for this kernel (14 n^2 + 14 P flops per correlation "
             "element and crystallite: direct cyclic sum with a complex exp-multiply per term and "
             "the FID recurrence); the kernel executes it with FFTs")
@@ -544,7 +588,8 @@ def main():
                                    else accuracy(wls, args.precision))
     if args.cpu_sample != "none":
         cb = cpu_baseline_ptop(wls[0]) if ptop else cpu_baseline(wls, args.cpu_sample)
-        out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
+        out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample",
+                                                  "cpu_model")}
         out["speedup_vs_cpu_core"] = value / cb["value"]
     print(json.dumps(out), flush=True)
 

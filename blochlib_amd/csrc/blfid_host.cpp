@@ -239,19 +239,23 @@ struct blfid_plan {
     DevPlan dp{};
     // profiling
     bool prof = false;
-    struct Ev { int phase; hipEvent_t a, b; double flops; };
+    struct Ev { int phase; hipEvent_t a, b; double flops, bytes; };
     std::vector<Ev> pending;
-    double ph_ms[BLFID_NSLOT] = {0}, ph_flops[BLFID_NSLOT] = {0};
+    double ph_ms[BLFID_NSLOT] = {0}, ph_flops[BLFID_NSLOT] = {0}, ph_bytes[BLFID_NSLOT] = {0};
     int ph_n[BLFID_NSLOT] = {0};
     ~blfid_plan() {
         for (auto &e : pending) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
     }
 };
 
-// Records HIP events around one launch on `st` when profiling is on.
+// Records HIP events around one launch on `st` when profiling is on, with the launch's
+// executed algorithmic flops and its algorithmic HBM bytes (what the kernel must read and
+// write at the least: its inputs once, its outputs once).
 struct PhaseScope {
-    blfid_plan *pl; int phase; double flops; hipStream_t st; hipEvent_t a = nullptr, b = nullptr;
-    PhaseScope(blfid_plan *p, int ph, double fl, hipStream_t s) : pl(p), phase(ph), flops(fl), st(s) {
+    blfid_plan *pl; int phase; double flops, bytes; hipStream_t st;
+    hipEvent_t a = nullptr, b = nullptr;
+    PhaseScope(blfid_plan *p, int ph, double fl, double by, hipStream_t s)
+        : pl(p), phase(ph), flops(fl), bytes(by), st(s) {
         if (pl->prof) {
             (void)hipEventCreate(&a);
             (void)hipEventCreate(&b);
@@ -261,7 +265,7 @@ struct PhaseScope {
     ~PhaseScope() {
         if (pl->prof) {
             (void)hipEventRecord(b, st);
-            pl->pending.push_back({phase, a, b, flops});
+            pl->pending.push_back({phase, a, b, flops, bytes});
         }
     }
 };
@@ -274,6 +278,7 @@ static void collect_profile(blfid_plan *pl) {
         pl->ph_ms[e.phase] += ms;
         pl->ph_n[e.phase] += 1;
         pl->ph_flops[e.phase] += e.flops;
+        pl->ph_bytes[e.phase] += e.bytes;
         (void)hipEventDestroy(e.a);
         (void)hipEventDestroy(e.b);
     }
@@ -1032,6 +1037,25 @@ static double group_b3(const blfid_plan *pl, const std::vector<int> &g) {
     for (int b : g) s += (double)pl->blk_dim[b] * pl->blk_dim[b] * pl->blk_dim[b];
     return s;
 }
+static double group_b2(const blfid_plan *pl, const std::vector<int> &g) {
+    double s = 0;
+    for (int b : g) s += (double)pl->blk_dim[b] * pl->blk_dim[b];
+    return s;
+}
+static double group_b1(const blfid_plan *pl, const std::vector<int> &g) {
+    double s = 0;
+    for (int b : g) s += (double)pl->blk_dim[b];
+    return s;
+}
+// sum over the group's pairs of b_I^2 + b_J^2 (the two dense blocks a pair's product reads)
+static double pair_reads(const blfid_plan *pl, const std::vector<int> &g) {
+    double s = 0;
+    for (int p : g) {
+        double bI = pl->blk_dim[pl->pairs[p].I], bJ = pl->blk_dim[pl->pairs[p].J];
+        s += bI * bI + bJ * bJ;
+    }
+    return s;
+}
 static double pair_flops(const blfid_plan *pl, const std::vector<int> &g) {
     double s = 0;
     for (int p : g) {
@@ -1145,8 +1169,9 @@ static void carve(blfid_plan *pl, char *base, const size_t *off) {
 // Enqueues the whole powder on `st` with the working storage at `base` (regions at off[],
 // batches of Kb crystallites) and the normalised FID written to `out` (2 npts doubles, device).
 // Every error return after the first launch leaves the launched work in the stream.
-static blfid_status enqueue_run(blfid_plan *pl, char *base, const size_t *off, long Kb,
-                                int normalize, double *out, hipStream_t st) {
+static blfid_status enqueue_run(blfid_plan *pl, const blfid_layout::Shape &sh, char *base,
+                                const size_t *off, long Kb, int normalize, double *out,
+                                hipStream_t st) {
     const int K = pl->K;
     pl->Kb = (int)Kb;
     carve(pl, base, off);
@@ -1174,74 +1199,87 @@ static blfid_status enqueue_run(blfid_plan *pl, char *base, const size_t *off, l
             // U_k = exp(+i theta (H_k + H_rf,k)); A_n = U_n ... U_1; its eigenbasis gives
             // rho_m = A_n^m rho A_n^-m in closed form
             for (auto &kv : pl->blk_groups) {
-                PhaseScope ps(pl, 0 * 8 + kv.first, 0.0, st); // flops counted by k_exp
+                PhaseScope ps(pl, 0 * 8 + kv.first, 0.0,
+                              kb * (double)pl->n * group_b2(pl, kv.second) * cs, st); // flops: k_exp
                 LAUNCHCHK(blfid_dev_launch_exp(pr, kv.first, P, kb, c0, pl->d_bgrp[kv.first].p,
-                                               (int)kv.second.size(), pl->scrE, bm2, pl->exp_slots,
+                                               (int)kv.second.size(), pl->scrE, bm2, sh.exp_slots,
                                                pl->bufU, st));
             }
             for (auto &kv : pl->blk_groups) {
-                PhaseScope ps(pl, 1 * 8 + kv.first, kb * (double)(pl->n - 1) * group_b3(pl, kv.second) * 8.0, st);
+                PhaseScope ps(pl, 1 * 8 + kv.first, kb * (double)(pl->n - 1) * group_b3(pl, kv.second) * 8.0,
+                              kb * (double)(pl->n + 1) * group_b2(pl, kv.second) * cs, st);
                 LAUNCHCHK(blfid_dev_launch_dyson(pr, kv.first, P, kb, pl->d_bgrp[kv.first].p,
                                                  (int)kv.second.size(), pl->bufU, pl->bufA, st));
             }
             for (auto &kv : pl->eig_cls) {
-                PhaseScope ps(pl, 2 * 8 + kv.first + 1, 0.0, st);
+                PhaseScope ps(pl, 2 * 8 + kv.first + 1, 0.0,
+                              kb * (group_b2(pl, kv.second) * (cs + 16.0) + group_b1(pl, kv.second) * 8.0), st);
                 LAUNCHCHK(blfid_dev_launch_eig(pr, P, kb, pl->d_eig_cls[kv.first].p,
                                                (int)kv.second.size(),
                                                pl->bufA + (size_t)((pl->n - 1) % 2) * pl->Lsum * cs,
                                                (size_t)2 * pl->Lsum, 0, pl->bufW, nullptr,
                                                pl->thetas, kv.first, pl->scrEig, bm2,
-                                               pl->eig_slots, st));
+                                               sh.eig_slots, st));
             }
             for (auto &kv : pl->pair_groups) {
-                PhaseScope ps(pl, 7 * 8 + kv.first, kb * (pair_flops(pl, kv.second) + 8.0 * pl->npts * pair_elems(pl, kv.second)), st);
+                PhaseScope ps(pl, 7 * 8 + kv.first, kb * (pair_flops(pl, kv.second) + 8.0 * pl->npts * pair_elems(pl, kv.second)),
+                              kb * (pair_reads(pl, kv.second) * 16.0 + (double)kv.second.size() * pl->npts * 16.0), st);
                 LAUNCHCHK(blfid_dev_launch_static_fid(kv.first, P, kb, pl->d_pgrp[kv.first].p,
                                                       (int)kv.second.size(), pl->bufW,
                                                       pl->thetas, (double2 *)pl->scrQ, bm2,
-                                                      pl->qr_slots, 1.0, 0, pl->part, st));
+                                                      sh.qr_slots, 1.0, 0, pl->part, st));
             }
         } else if (pl->spinning) {
             for (auto &kv : pl->blk_groups) {
-                PhaseScope ps(pl, 0 * 8 + kv.first, 0.0, st); // flops counted by k_exp
+                PhaseScope ps(pl, 0 * 8 + kv.first, 0.0,
+                              kb * (double)pl->n * group_b2(pl, kv.second) * cs, st); // flops: k_exp
                 size_t slot;
                 int ns;
-                group_slots(bm2, pl->exp_slots, group_e(pl, kv.second), kv.first, &slot, &ns);
+                group_slots(bm2, sh.exp_slots, group_e(pl, kv.second), kv.first, &slot, &ns);
                 LAUNCHCHK(blfid_dev_launch_exp(pr, kv.first, P, kb, c0, pl->d_bgrp[kv.first].p,
                                                (int)kv.second.size(), pl->scrE, slot, ns,
                                                pl->bufU, st));
             }
             for (auto &kv : pl->blk_groups) {
-                PhaseScope ps(pl, 1 * 8 + kv.first, kb * (double)(pl->n - 1) * group_b3(pl, kv.second) * 8.0, st);
+                PhaseScope ps(pl, 1 * 8 + kv.first, kb * (double)(pl->n - 1) * group_b3(pl, kv.second) * 8.0,
+                              kb * 2.0 * pl->n * group_b2(pl, kv.second) * cs, st);
                 LAUNCHCHK(blfid_dev_launch_dyson(pr, kv.first, P, kb, pl->d_bgrp[kv.first].p,
                                                  (int)kv.second.size(), pl->bufU, pl->bufA, st));
             }
             for (auto &kv : pl->eig_cls) {
-                PhaseScope ps(pl, 2 * 8 + kv.first + 1, 0.0, st);
+                PhaseScope ps(pl, 2 * 8 + kv.first + 1, 0.0,
+                              kb * (group_b2(pl, kv.second) * (2.0 * cs) + group_b1(pl, kv.second) * 8.0), st);
                 LAUNCHCHK(blfid_dev_launch_eig(pr, P, kb, pl->d_eig_cls[kv.first].p,
                                                (int)kv.second.size(),
                                                pl->bufA + (size_t)(pl->n - 1) * pl->Lsum * cs,
                                                (size_t)pl->n * pl->Lsum, 0, pl->bufW, pl->bufWr,
                                                pl->thetas, kv.first, pl->scrEig, bm2,
-                                               pl->eig_slots, st));
+                                               sh.eig_slots, st));
             }
             for (auto &kv : pl->blk_groups) {
-                PhaseScope ps(pl, 3 * 8 + kv.first, kb * (double)pl->n * group_b3(pl, kv.second) * 8.0, st);
+                PhaseScope ps(pl, 3 * 8 + kv.first, kb * (double)pl->n * group_b3(pl, kv.second) * 8.0,
+                              kb * (2.0 * pl->n + 1.0) * group_b2(pl, kv.second) * cs, st);
                 LAUNCHCHK(blfid_dev_launch_bmul(pr, kv.first, P, kb, pl->d_bgrp[kv.first].p,
                                                 (int)kv.second.size(), pl->bufA,
                                                 pr ? (const void *)pl->bufWr : (const void *)pl->bufW,
                                                 pl->bufU, st));
             }
             for (auto &kv : pl->pair_groups) {
-                PhaseScope ps(pl, 4 * 8 + kv.first, kb * (double)pl->n * pair_flops(pl, kv.second), st);
+                PhaseScope ps(pl, 4 * 8 + kv.first, kb * (double)pl->n * pair_flops(pl, kv.second),
+                              kb * (double)pl->n * cs *
+                                  (pair_reads(pl, kv.second) + (pl->rt_from_q ? 1.0 : 2.0) * pair_elems(pl, kv.second)),
+                              st);
                 size_t slot;
                 int ns;
-                group_slots(bm2, pl->qr_slots, pair_e(pl, kv.second), kv.first, &slot, &ns);
+                group_slots(bm2, sh.qr_slots, pair_e(pl, kv.second), kv.first, &slot, &ns);
                 LAUNCHCHK(blfid_dev_launch_qr(pr, kv.first, P, kb, pl->d_pgrp[kv.first].p,
                                               (int)kv.second.size(), pl->bufU, pl->scrQ, slot,
                                               ns, pl->bufQt, pl->bufRt, st));
             }
             if (pl->Etot > 0) {
-                PhaseScope ps(pl, 5 * 8, kb * (double)pl->Etot * corr_flops_per_elem(pl->n, pl->npts), st);
+                const double rq = (!pl->rt_from_q || pl->rt_rev) ? 2.0 : 1.0; // Q, and R or reverse Q
+                PhaseScope ps(pl, 5 * 8, kb * (double)pl->Etot * corr_flops_per_elem(pl->n, pl->npts),
+                              kb * ((double)pl->n * pl->Etot * cs * rq + (double)pl->nparts * pl->npts * 16.0), st);
                 LAUNCHCHK(blfid_dev_launch_corr(pr, P, kb, pl->bufQt, pl->bufRt, pl->thetas,
                                                 pl->d_er.p, pl->d_es.p, pl->part, st));
             }
@@ -1249,29 +1287,30 @@ static blfid_status enqueue_run(blfid_plan *pl, char *base, const size_t *off, l
             for (auto &kv : pl->blk_groups)
                 LAUNCHCHK(blfid_dev_launch_static_h(P, kb, c0, pl->d_bgrp[kv.first].p,
                                                     (int)kv.second.size(), (double *)pl->scrE, bm2,
-                                                    pl->exp_slots, (double2 *)pl->bufU, st));
+                                                    sh.exp_slots, (double2 *)pl->bufU, st));
             for (auto &kv : pl->eig_cls)
                 LAUNCHCHK(blfid_dev_launch_eig(0, P, kb, pl->d_eig_cls[kv.first].p,
                                                (int)kv.second.size(), pl->bufU, (size_t)pl->Lsum, 1,
                                                pl->bufW, nullptr, pl->thetas, kv.first,
-                                               pl->scrEig, bm2, pl->eig_slots, st));
+                                               pl->scrEig, bm2, sh.eig_slots, st));
             for (auto &kv : pl->pair_groups) {
-                PhaseScope ps(pl, 7 * 8 + kv.first, 0.0, st);
+                PhaseScope ps(pl, 7 * 8 + kv.first, 0.0,
+                              kb * (pair_reads(pl, kv.second) * 16.0 + (double)kv.second.size() * pl->npts * 16.0), st);
                 LAUNCHCHK(blfid_dev_launch_static_fid(kv.first, P, kb, pl->d_pgrp[kv.first].p,
                                                       (int)kv.second.size(), pl->bufW,
                                                       pl->thetas, (double2 *)pl->scrQ, bm2,
-                                                      pl->qr_slots, -PI2 / pl->sw, 1, pl->part, st));
+                                                      sh.qr_slots, -PI2 / pl->sw, 1, pl->part, st));
             }
         }
         {
-            PhaseScope ps(pl, 6 * 8, 4.0 * kb * 2.0 * pl->npts * pl->nparts, st);
+            PhaseScope ps(pl, 6 * 8, 4.0 * kb * 2.0 * pl->npts * pl->nparts,
+                          (double)kb * pl->nparts * pl->npts * 16.0, st);
             LAUNCHCHK(blfid_dev_launch_reduce(kb, c0, pl->npts, pl->nparts, G, pl->d_w.p, scale,
                                               pl->part, pl->grp, st));
         }
     }
     cplx norm = normalize ? pl->trdd : cplx(1.0, 0.0);
     LAUNCHCHK(blfid_dev_launch_finish(pl->npts, ng, pl->grp, norm.real(), norm.imag(), out, st));
-    pl->ran = true;
     return BLFID_OK;
 }
 
@@ -1309,7 +1348,8 @@ blfid_status blfid_plan_run(blfid_plan *pl, int normalize, void *stream) {
         base = pl->own_ws.p;
     }
     HIPCHK(pl->fid_out.alloc((size_t)2 * pl->npts));
-    blfid_status rs = enqueue_run(pl, base, off, Kb, normalize, pl->fid_out.p, st);
+    blfid_status rs = enqueue_run(pl, sh, base, off, Kb, normalize, pl->fid_out.p, st);
+    if (rs == BLFID_OK) pl->ran = true;
     if (pl->ws) {
         // the next user of the workspace waits for whatever was enqueued, on every exit
         hipError_t e = hipEventRecord(pl->ws->ev, st);
@@ -1361,7 +1401,12 @@ blfid_status blfid_plan_set_profiling(blfid_plan *pl, int on) {
     if (!pl) return fail(BLFID_ERR_INVALID, "plan is NULL");
     collect_profile(pl);
     pl->prof = on != 0;
-    for (int i = 0; i < BLFID_NSLOT; i++) { pl->ph_ms[i] = 0; pl->ph_n[i] = 0; pl->ph_flops[i] = 0; }
+    for (int i = 0; i < BLFID_NSLOT; i++) {
+        pl->ph_ms[i] = 0;
+        pl->ph_n[i] = 0;
+        pl->ph_flops[i] = 0;
+        pl->ph_bytes[i] = 0;
+    }
     if (pl->prof) {
         // the propagator's degree is chosen per task on the device, so its executed flops are
         // counted there (one atomic add per workgroup)
@@ -1383,6 +1428,13 @@ blfid_status blfid_plan_phase_times(blfid_plan *pl, double *ms, int *launches, d
         launches[i] = pl->ph_n[i];
         flops[i] = pl->ph_flops[i] + (i < 8 ? ef[i] : 0.0);
     }
+    return BLFID_OK;
+}
+
+blfid_status blfid_plan_phase_bytes(blfid_plan *pl, double *bytes) {
+    if (!pl || !bytes) return fail(BLFID_ERR_INVALID, "NULL argument");
+    collect_profile(pl);
+    for (int i = 0; i < BLFID_NSLOT; i++) bytes[i] = pl->ph_bytes[i];
     return BLFID_OK;
 }
 

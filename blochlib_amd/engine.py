@@ -100,6 +100,7 @@ def lib():
         L.blfid_plan_set_workspace.argtypes = [vp, vp]
         L.blfid_plan_set_profiling.argtypes = [vp, ctypes.c_int]
         L.blfid_plan_phase_times.argtypes = [vp, _d, _i, _d]
+        L.blfid_plan_phase_bytes.argtypes = [vp, _d]
         L.blfid_last_error.restype = ctypes.c_char_p
         L.blfid_version.restype = ctypes.c_char_p
         _lib = L
@@ -274,11 +275,14 @@ class Plan:
         _check(lib().blfid_plan_set_profiling(self.h, int(bool(on))))
 
     def phase_times(self) -> dict:
-        """{kernel name: {ms, launches, flops}} for every slot that ran, e.g. 'k_exp<5>'."""
+        """{kernel name: {ms, launches, flops, bytes}} for every slot that ran, e.g.
+        'k_exp<5>' (bytes: algorithmic HBM bytes)."""
         ms = np.zeros(64)
         n = np.zeros(64, dtype=np.int32)
         fl = np.zeros(64)
         _check(lib().blfid_plan_phase_times(self.h, _p(ms), _p(n, _i), _p(fl)))
+        by = np.zeros(64)
+        _check(lib().blfid_plan_phase_bytes(self.h, _p(by)))
         out = {}
         for slot in range(64):
             if n[slot]:
@@ -288,7 +292,7 @@ class Plan:
                 else:
                     name = self.KERNELS[ph] + (f"<{M}>" if M else "")
                 out[name] = {"ms": float(ms[slot]), "launches": int(n[slot]),
-                             "flops": float(fl[slot])}
+                             "flops": float(fl[slot]), "bytes": float(by[slot])}
         return out
 
     def powder_fid(self, pw: Powder, rho, det, normalize=True) -> np.ndarray:

@@ -182,6 +182,13 @@ blfid_status blfid_plan_set_workspace(blfid_plan *plan, blfid_workspace *ws);
 #define BLFID_NSLOT 64
 blfid_status blfid_plan_set_profiling(blfid_plan *plan, int on);
 blfid_status blfid_plan_phase_times(blfid_plan *plan, double *ms, int *launches, double *flops);
+/* Algorithmic HBM bytes per slot (same slots, accumulated likewise): what each kernel must read
+ * and write at the least -- its inputs once and its outputs once (U_k written by the propagator;
+ * U_k read and A_k written by the Dyson chain; A_n read and W, eigenphases written by the eigen-
+ * solver; A_k, W read and B_k written; both blocks of a pair read and Q (R) written; Q (and R or
+ * the reverse Q) read and the partial FIDs written by the correlation; partial FIDs read by the
+ * powder sum).  Beside counter-measured traffic it shows re-reads and spills. */
+blfid_status blfid_plan_phase_bytes(blfid_plan *plan, double *bytes);
 
 /* Thread-local description of the last error. */
 const char *blfid_last_error(void);

@@ -199,36 +199,83 @@ static void build_hamiltonian(const orc_sys *sy, const double cartPowder[9], dou
 
 #define EL(A, i, j, N) (A)[(size_t)(i) + (size_t)(j) * (N)]
 
-/* C = op(A) * op(B); op = 'N' or 'C' (conjugate transpose).  C must not alias A or B. */
-static void zgemm(int N, char ta, char tb, const cplx *A, const cplx *B, cplx *C) {
-    memset(C, 0, sizeof(cplx) * (size_t)N * N);
-    if (ta == 'C') { /* C(i,j) = <A(:,i), op(B)(:,j)> : contiguous column dot products */
-        cplx *bj = (tb == 'N') ? NULL : malloc(sizeof(cplx) * N);
-        for (int j = 0; j < N; j++) {
-            const cplx *b = B + (size_t)j * N;
-            if (bj) {
-                for (int k = 0; k < N; k++) bj[k] = conj(EL(B, j, k, N));
-                b = bj;
+/* C += A B, all N x N column-major: cache-blocked over k (panels of 128) with a 4 x 4 complex
+ * register tile; each complex product a*b is a*Re(b) + swap(a)*(-Im b, +Im b) on AVX2 vectors
+ * of two complex values.  Rows and columns beyond a multiple of 4 take a scalar loop. */
+typedef double v4d __attribute__((vector_size(32)));
+typedef long long v4l __attribute__((vector_size(32)));
+
+static inline v4d ld4(const cplx *p) {
+    v4d v;
+    memcpy(&v, p, sizeof(v));
+    return v;
+}
+static inline void st4(cplx *p, v4d v) { memcpy(p, &v, sizeof(v)); }
+
+static void gemm_nn(int N, const cplx *A, const cplx *B, cplx *C) {
+    const int KB = 128, N4 = N & ~3;
+    const v4l sw = {1, 0, 3, 2};
+    for (int k0 = 0; k0 < N; k0 += KB) {
+        const int k1 = k0 + KB < N ? k0 + KB : N;
+        for (int j = 0; j < N4; j += 4) {
+            for (int i = 0; i < N4; i += 4) {
+                v4d c[4][2];
+                memset(c, 0, sizeof(c));
+                for (int k = k0; k < k1; k++) {
+                    const cplx *a = A + i + (size_t)k * N;
+                    const v4d a0 = ld4(a), a1 = ld4(a + 2);
+                    const v4d s0 = __builtin_shuffle(a0, sw), s1 = __builtin_shuffle(a1, sw);
+                    for (int q = 0; q < 4; q++) {
+                        const cplx b = B[k + (size_t)(j + q) * N];
+                        const double br = creal(b), bi = cimag(b);
+                        const v4d vr = {br, br, br, br}, vi = {-bi, bi, -bi, bi};
+                        c[q][0] += a0 * vr + s0 * vi;
+                        c[q][1] += a1 * vr + s1 * vi;
+                    }
+                }
+                for (int q = 0; q < 4; q++) {
+                    cplx *cc = C + i + (size_t)(j + q) * N;
+                    st4(cc, ld4(cc) + c[q][0]);
+                    st4(cc + 2, ld4(cc + 2) + c[q][1]);
+                }
             }
+            for (int i = N4; i < N; i++)
+                for (int q = 0; q < 4; q++) {
+                    cplx s = 0;
+                    for (int k = k0; k < k1; k++) s += EL(A, i, k, N) * EL(B, k, j + q, N);
+                    EL(C, i, j + q, N) += s;
+                }
+        }
+        for (int j = N4; j < N; j++)
             for (int i = 0; i < N; i++) {
-                const cplx *a = A + (size_t)i * N;
                 cplx s = 0;
-                for (int k = 0; k < N; k++) s += conj(a[k]) * b[k];
-                EL(C, i, j, N) = s;
+                for (int k = k0; k < k1; k++) s += EL(A, i, k, N) * EL(B, k, j, N);
+                EL(C, i, j, N) += s;
             }
-        }
-        free(bj);
-        return;
     }
-    for (int j = 0; j < N; j++) { /* C(:,j) += A(:,k) op(B)(k,j) : contiguous axpy */
-        cplx *c = C + (size_t)j * N;
-        for (int k = 0; k < N; k++) {
-            cplx b = (tb == 'N') ? EL(B, k, j, N) : conj(EL(B, j, k, N));
-            if (b == 0) continue;
-            const cplx *a = A + (size_t)k * N;
-            for (int i = 0; i < N; i++) c[i] += a[i] * b;
-        }
+}
+
+/* C = op(A) * op(B); op = 'N' or 'C' (conjugate transpose).  C must not alias A or B.
+ * A conjugate-transposed operand is formed explicitly (O(N^2)) and the product taken by
+ * gemm_nn. */
+static void zgemm(int N, char ta, char tb, const cplx *A, const cplx *B, cplx *C) {
+    cplx *at = NULL, *bt = NULL;
+    if (ta == 'C') {
+        at = malloc(sizeof(cplx) * (size_t)N * N);
+        for (int j = 0; j < N; j++)
+            for (int i = 0; i < N; i++) EL(at, i, j, N) = conj(EL(A, j, i, N));
+        A = at;
     }
+    if (tb == 'C') {
+        bt = malloc(sizeof(cplx) * (size_t)N * N);
+        for (int j = 0; j < N; j++)
+            for (int i = 0; i < N; i++) EL(bt, i, j, N) = conj(EL(B, j, i, N));
+        B = bt;
+    }
+    memset(C, 0, sizeof(cplx) * (size_t)N * N);
+    gemm_nn(N, A, B, C);
+    free(at);
+    free(bt);
 }
 
 /* Hermitian eigen-decomposition: A = V diag(w) V^H.  Householder tridiagonalisation,
@@ -776,72 +823,125 @@ static void add_rf(const orc_sys *sy, const double *rfk, int nrf, cplx *H) {
  * start time of each step, or NULL for consecutive steps from t0.  A pulse segment of length T
  * split into m substeps is m entries T/m (propogation.cc:264-300: tau /= tmpp; t1 += tau;
  * t2 += tau). */
+typedef struct {
+    const orc_sys *sy;
+    double wr, rotb_rad, t0, tau;
+    int nsteps, npts, nrf;
+    const double *step_t1, *step_dt, *rf;
+    const cplx *rho, *det;
+    const double *theta, *phi, *gamma, *wt;
+    int k0, k1, status;
+    cplx *acc;
+} orc_pulse_job;
+
+/* crystallites k0..k1-1 of a pulse-program powder: acc[m] += w_c Tr(rho_m det) */
+static void *pulse_job_run(void *arg) {
+    orc_pulse_job *jb = (orc_pulse_job *)arg;
+    const orc_sys *sy = jb->sy;
+    const int N = sy->N;
+    const size_t NN = (size_t)N * N;
+    const cplx *D = jb->det;
+    cplx *H = malloc(sizeof(cplx) * NN), *X = malloc(sizeof(cplx) * NN), *U = malloc(sizeof(cplx) * NN);
+    cplx *T = malloc(sizeof(cplx) * NN), *r = malloc(sizeof(cplx) * NN);
+    for (int m = 0; m < jb->npts; m++) jb->acc[m] = 0;
+    if (!H || !X || !U || !T || !r) {
+        jb->status = ORC_ENOMEM;
+        goto done;
+    }
+    for (int c = jb->k0; c < jb->k1; c++) {
+        double cp[9];
+        rot3(jb->phi[c], jb->theta[c], jb->gamma[c], cp);
+        for (size_t q = 0; q < NN; q++) U[q] = 0;
+        for (int i = 0; i < N; i++) U[i + (size_t)i * N] = 1.0;
+        const double *sdt = jb->step_dt, *st1 = jb->step_t1;
+        double t1 = (sdt && st1) ? st1[0] : jb->t0;
+        double t2 = t1 + (sdt ? sdt[0] : jb->tau);
+        for (int k = 0; k < jb->nsteps; k++) {
+            double dt = sdt ? sdt[k] : jb->tau;
+            double tmid = (t2 - t1) / 2.0 + t1;
+            build_hamiltonian(sy, cp, jb->rotb_rad, tmid * jb->wr * PI2, H);
+            add_rf(sy, jb->rf + (size_t)k * 4 * jb->nrf, jb->nrf, H);
+            int st = orc_expm_herm(N, H, CMPLX(0.0, 1.0) * dt * PI2, X);
+            if (st) {
+                jb->status = st;
+                goto done;
+            }
+            zgemm(N, 'N', 'N', X, U, T);
+            memcpy(U, T, sizeof(cplx) * NN);
+            if (sdt) {
+                if (k + 1 < jb->nsteps) {
+                    t1 = st1 ? st1[k + 1] : t2;
+                    t2 = t1 + sdt[k + 1];
+                }
+            } else {
+                t1 += jb->tau;
+                t2 += jb->tau;
+            }
+        }
+        memcpy(r, jb->rho, sizeof(cplx) * NN);
+        for (int m = 0; m < jb->npts; m++) {
+            cplx tr = 0;
+            for (int i = 0; i < N; i++)
+                for (int j = 0; j < N; j++) tr += EL(r, i, j, N) * EL(D, j, i, N);
+            jb->acc[m] += jb->wt[c] * tr;
+            if (m + 1 < jb->npts) {
+                zgemm(N, 'N', 'C', r, U, T);   /* r U^H */
+                zgemm(N, 'N', 'N', U, T, r);   /* U r U^H */
+            }
+        }
+    }
+done:
+    free(H); free(X); free(U); free(T); free(r);
+    return NULL;
+}
+
 int orc_pulse_fid(int S, const int *isotopes, int ncsa, const double *csa, int ndip,
                   const double *dip, int nj, const double *jc, double wr, double rotor_deg,
                   int nsteps, const double *step_t1, const double *step_dt, int npts,
                   double t0, int nrf,
                   const double *rf, int K, const double *theta, const double *phi,
                   const double *gamma, const double *w, const double *rho, const double *det,
-                  int normalize, double *fid_out) {
+                  int normalize, int nthreads, double *fid_out) {
     if (S < 1 || S > 12 || nsteps < 1 || npts < 0 || K < 0) return ORC_EINVAL;
     if (!step_dt && wr <= 0.0) return ORC_EINVAL;
     orc_sys sy = {S, 1 << S, isotopes, ncsa, csa, ndip, dip, nj, jc};
     const int N = sy.N;
-    const size_t NN = (size_t)N * N;
-    const cplx *R0 = (const cplx *)rho, *D = (const cplx *)det;
-    cplx *H = malloc(sizeof(cplx) * NN), *X = malloc(sizeof(cplx) * NN), *U = malloc(sizeof(cplx) * NN);
-    cplx *T = malloc(sizeof(cplx) * NN), *r = malloc(sizeof(cplx) * NN), *acc = calloc(npts > 0 ? npts : 1, sizeof(cplx));
-    if (!H || !X || !U || !T || !r || !acc) { free(H); free(X); free(U); free(T); free(r); free(acc); return ORC_ENOMEM; }
-    double tau = step_dt ? 0.0 : (1.0 / wr) / nsteps;
-    int st = ORC_OK;
-    for (int c = 0; c < K && st == ORC_OK; c++) {
-        double cp[9];
-        rot3(phi[c], theta[c], gamma[c], cp);
-        for (size_t q = 0; q < NN; q++) U[q] = 0;
-        for (int i = 0; i < N; i++) U[i + (size_t)i * N] = 1.0;
-        double t1 = (step_dt && step_t1) ? step_t1[0] : t0;
-        double t2 = t1 + (step_dt ? step_dt[0] : tau);
-        for (int k = 0; k < nsteps; k++) {
-            double dt = step_dt ? step_dt[k] : tau;
-            double tmid = (t2 - t1) / 2.0 + t1;
-            build_hamiltonian(&sy, cp, rotor_deg * DEG2RAD, tmid * wr * PI2, H);
-            add_rf(&sy, rf + (size_t)k * 4 * nrf, nrf, H);
-            st = orc_expm_herm(N, H, CMPLX(0.0, 1.0) * dt * PI2, X);
-            if (st) break;
-            zgemm(N, 'N', 'N', X, U, T);
-            memcpy(U, T, sizeof(cplx) * NN);
-            if (step_dt) {
-                if (k + 1 < nsteps) {
-                    t1 = step_t1 ? step_t1[k + 1] : t2;
-                    t2 = t1 + step_dt[k + 1];
-                }
-            } else {
-                t1 += tau;
-                t2 += tau;
-            }
-        }
-        if (st) break;
-        memcpy(r, R0, sizeof(cplx) * NN);
-        for (int m = 0; m < npts; m++) {
-            cplx tr = 0;
-            for (int i = 0; i < N; i++)
-                for (int j = 0; j < N; j++) tr += EL(r, i, j, N) * EL(D, j, i, N);
-            acc[m] += w[c] * tr;
-            if (m + 1 < npts) {
-                zgemm(N, 'N', 'C', r, U, T);   /* r U^H */
-                zgemm(N, 'N', 'N', U, T, r);   /* U r U^H */
-            }
-        }
+    const cplx *D = (const cplx *)det;
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > K && K > 0) nthreads = K;
+    if (K == 0) nthreads = 1;
+    const int P1 = npts > 0 ? npts : 1;
+    orc_pulse_job *jobs = calloc(nthreads, sizeof(orc_pulse_job));
+    pthread_t *th = calloc(nthreads, sizeof(pthread_t));
+    cplx *accs = calloc((size_t)nthreads * P1, sizeof(cplx));
+    if (!jobs || !th || !accs) { free(jobs); free(th); free(accs); return ORC_ENOMEM; }
+    for (int t = 0; t < nthreads; t++) {
+        orc_pulse_job *jb = &jobs[t];
+        jb->sy = &sy; jb->wr = wr; jb->rotb_rad = rotor_deg * DEG2RAD; jb->t0 = t0;
+        jb->tau = step_dt ? 0.0 : (1.0 / wr) / nsteps;
+        jb->nsteps = nsteps; jb->npts = npts; jb->nrf = nrf;
+        jb->step_t1 = step_t1; jb->step_dt = step_dt; jb->rf = rf;
+        jb->rho = (const cplx *)rho; jb->det = D;
+        jb->theta = theta; jb->phi = phi; jb->gamma = gamma; jb->wt = w;
+        jb->k0 = (int)((long)K * t / nthreads); jb->k1 = (int)((long)K * (t + 1) / nthreads);
+        jb->acc = accs + (size_t)t * P1;
     }
+    for (int t = 1; t < nthreads; t++) pthread_create(&th[t], NULL, pulse_job_run, &jobs[t]);
+    pulse_job_run(&jobs[0]);
+    for (int t = 1; t < nthreads; t++) pthread_join(th[t], NULL);
+    int st = ORC_OK;
+    for (int t = 0; t < nthreads; t++) if (jobs[t].status) st = jobs[t].status;
     cplx trdd = 0;
     for (int i = 0; i < N; i++)
         for (int j = 0; j < N; j++) trdd += EL(D, i, j, N) * conj(EL(D, i, j, N));
     for (int m = 0; m < npts; m++) {
-        cplx v = normalize ? acc[m] / trdd : acc[m];
+        cplx v = 0;
+        for (int t = 0; t < nthreads; t++) v += jobs[t].acc[m];
+        if (normalize) v /= trdd;
         fid_out[2 * m] = creal(v);
         fid_out[2 * m + 1] = cimag(v);
     }
-    free(H); free(X); free(U); free(T); free(r); free(acc);
+    free(jobs); free(th); free(accs);
     return st;
 }
 
@@ -849,9 +949,11 @@ int orc_ptop_fid(int S, const int *isotopes, int ncsa, const double *csa, int nd
                  const double *dip, int nj, const double *jc, double wr, double rotor_deg,
                  int nsteps, int npts, double t0, int nrf, const double *rf, int K,
                  const double *theta, const double *phi, const double *gamma, const double *w,
-                 const double *rho, const double *det, int normalize, double *fid_out) {
+                 const double *rho, const double *det, int normalize, int nthreads,
+                 double *fid_out) {
     return orc_pulse_fid(S, isotopes, ncsa, csa, ndip, dip, nj, jc, wr, rotor_deg, nsteps, NULL,
                          NULL, npts, t0, nrf, rf, K, theta, phi, gamma, w, rho, det, normalize,
+                         nthreads,
                          fid_out);
 }
 

@@ -39,13 +39,14 @@ def lib():
         L.orc_ptop_fid.argtypes = [ctypes.c_int, _i, ctypes.c_int, _d, ctypes.c_int, _d,
                                    ctypes.c_int, _d, ctypes.c_double, ctypes.c_double,
                                    ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, _d,
-                                   ctypes.c_int, _d, _d, _d, _d, _d, _d, ctypes.c_int, _d]
+                                   ctypes.c_int, _d, _d, _d, _d, _d, _d, ctypes.c_int,
+                                   ctypes.c_int, _d]
         L.orc_pulse_fid.restype = ctypes.c_int
         L.orc_pulse_fid.argtypes = [ctypes.c_int, _i, ctypes.c_int, _d, ctypes.c_int, _d,
                                     ctypes.c_int, _d, ctypes.c_double, ctypes.c_double,
                                     ctypes.c_int, _d, _d, ctypes.c_int, ctypes.c_double,
                                     ctypes.c_int, _d, ctypes.c_int, _d, _d, _d, _d, _d, _d,
-                                    ctypes.c_int, _d]
+                                    ctypes.c_int, ctypes.c_int, _d]
         L.orc_hamiltonian.argtypes = [ctypes.c_int, _i, ctypes.c_int, _d, ctypes.c_int, _d,
                                       ctypes.c_int, _d] + [ctypes.c_double] * 5 + [_d]
         L.orc_eig_herm_x.argtypes = [ctypes.c_int, _d, _d, _d]
@@ -152,7 +153,7 @@ def rf_table(sy, rf, nsteps):
 
 
 def ptop_fid(sy, wr, rotor_deg, nsteps, npts, powder, rho, det, rf=(), t0=0.0, normalize=True,
-             step_dt=None, step_t1=None):
+             step_dt=None, step_t1=None, nthreads=None):
     """Point-to-point FID with rf: one application of the sub-sequence per point.  The
     sub-sequence is one rotor period of nsteps equal steps, or the consecutive steps of lengths
     step_dt."""
@@ -176,7 +177,7 @@ def ptop_fid(sy, wr, rotor_deg, nsteps, npts, powder, rho, det, rf=(), t0=0.0, n
                          float(rotor_deg), int(nsteps), tptr, dptr, int(npts), float(t0), nrf,
                          _p(tab),
                          len(th), _p(th), _p(ph), _p(ga), _p(w), _p(r), _p(d),
-                         int(bool(normalize)), _p(out))
+                         int(bool(normalize)), int(nthreads or os.cpu_count() or 1), _p(out))
     if st != 0:
         raise RuntimeError(f"oracle failed with status {st}")
     return out[: 2 * npts].view(np.complex128).copy()

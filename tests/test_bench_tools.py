@@ -84,21 +84,30 @@ def test_bench_refuses_more_than_one_gpu():
     assert "one GPU" in r.stderr
 
 
-def test_corr_algorithmic_bytes():
-    """k_corr reads Q once per element-step, and a second array only where R is not conj(Q)."""
-    class W:
-        def __init__(self, rho, det):
-            self._ops = (rho, det)
+def test_family_roofline_bound_and_units():
+    """The bound follows what a family issues (matrix cores for the complex64 chains and Q
+    products, vector ALUs for the propagator, the correlation and every FP64 kernel), and a
+    family whose flops per algorithmic byte sit below the ridge point is rated against HBM."""
+    assert bench.family("k_exp<5>") == "k_exp" and bench.family("k_corr") == "k_corr"
+    d = {"ms": 1000.0, "launches": 10, "flops": 50e12, "bytes": 1e9}   # 50 TFLOP/s, 50 kflop/B
+    r = bench.family_roofline("k_dyson", d, "c64")
+    assert r["bound"] == "mfma" and r["unit"] == "TFLOP/s" and abs(r["achieved"] - 50.0) < 1e-9
+    assert abs(r["frac"] - 50.0 / bench.FP32_PEAK_TFLOPS) < 1e-12
+    assert abs(r["alg_bytes_per_launch"] - 1e8) < 1e-3
+    assert bench.family_roofline("k_exp", d, "c64")["bound"] == "valu"
+    assert bench.family_roofline("k_corr", d, "c64")["bound"] == "valu"
+    r64 = bench.family_roofline("k_dyson", d, "fp64")
+    assert r64["bound"] == "valu" and r64["peak"] == bench.FP64_PEAK_TFLOPS
+    # 1 flop per byte, far below the ridge (~20 flop/B): HBM bound, achieved in GB/s
+    low = {"ms": 1000.0, "launches": 1, "flops": 4e12, "bytes": 4e12}
+    h = bench.family_roofline("k_corr", low, "c64")
+    assert h["bound"] == "hbm" and h["unit"] == "GB/s" and abs(h["achieved"] - 4000.0) < 1e-6
+    assert abs(h["frac"] - 0.5) < 1e-12
+    # no flops counted (eigen-solver): bytes against HBM
+    e = bench.family_roofline("k_eig", {"ms": 1.0, "launches": 1, "flops": 0.0, "bytes": 8e9}, "c64")
+    assert e["bound"] == "hbm" and abs(e["achieved"] - 8000.0) < 1e-6
 
-        def operators(self):
-            return self._ops
 
-    ip = np.zeros((4, 4), complex)
-    ip[0, 1] = ip[2, 3] = 1.0                      # a raising operator: no reverse entries
-    ix = 0.5 * (ip + ip.T)
-    assert bench.corr_bytes_per_elem_step([W(ix, ip)], "c64") == 8.0
-    assert bench.corr_bytes_per_elem_step([W(ix, ip)], "fp64") == 16.0
-    # rho not (det + det^H)/2: R is stored and read
-    assert bench.corr_bytes_per_elem_step([W(ix + np.eye(4), ip)], "c64") == 16.0
-    # det with entries in both (a, b) and (b, a): reverse elements are read
-    assert bench.corr_bytes_per_elem_step([W(ix, ix)], "c64") == 16.0
+def test_cpu_info_caps_at_one_gpu_share():
+    cores, model = bench.cpu_info()
+    assert 1 <= cores <= 16 and isinstance(model, str)
