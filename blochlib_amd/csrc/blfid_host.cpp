@@ -22,6 +22,7 @@
 #include <cstring>
 #include <functional>
 #include <map>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -1359,6 +1360,81 @@ blfid_status blfid_plan_run(blfid_plan *pl, int normalize, void *stream) {
             return fail(BLFID_ERR_DEVICE, std::string("hipEventRecord: ") + hipGetErrorString(e));
     }
     return rs;
+}
+
+// ------------------------------------------------------------------------------ no-alloc run
+// Working storage of blfid_plan_run_noalloc: one statically sized device array per GPU (the
+// code object's global, allocated when the library's device code is loaded on that GPU and
+// held until the process ends).  Host code never passes `scratch` itself to a kernel (on the
+// host it only names the symbol): the device address comes from hipGetSymbolAddress, once per
+// device, with that device current.
+static __device__ __align__(256) char scratch[BLFID_NOALLOC_SCRATCH_BYTES];
+#define NOALLOC_MAX_DEVICES 64
+static std::mutex g_noalloc_mu;
+static char *g_noalloc_ptr[NOALLOC_MAX_DEVICES];
+
+static blfid_status noalloc_scratch(char **out) {
+    int dev = -1;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES)
+        return fail(BLFID_ERR_DEVICE, "device index beyond the no-alloc scratch table");
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (!g_noalloc_ptr[dev]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(BLFID_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") +
+                                              hipGetErrorString(e));
+        g_noalloc_ptr[dev] = (char *)p;
+    }
+    *out = g_noalloc_ptr[dev];
+    return BLFID_OK;
+}
+
+// The plan's layout with its scratch-slot counts cut so that each slot region takes at most an
+// eighth of the static scratch (fewer workgroups in flight for the propagator, Q/R and large-
+// block eigen kernels; results are unchanged, every task still runs).
+static blfid_layout::Shape noalloc_shape(const blfid_plan *pl) {
+    blfid_layout::Shape sh = shape_of(pl);
+    const double part = (double)BLFID_NOALLOC_SCRATCH_BYTES / 8.0;
+    const double bm2 = (double)pl->bmax * pl->bmax;
+    const double rs = pl->prec ? 4.0 : 8.0;
+    const double e1 = sh.kind == 1 ? bm2 * 8.0 : 9.0 * bm2 * rs; // one exp / static-H slot
+    auto cap = [&](int slots, double per) {
+        return std::max(1, std::min(slots, (int)std::floor(part / per)));
+    };
+    sh.exp_slots = cap(sh.exp_slots, e1);
+    sh.qr_slots = cap(sh.qr_slots, 3.0 * bm2 * 16.0);
+    if (sh.eig_slots > 0) sh.eig_slots = cap(sh.eig_slots, bm2 * 16.0);
+    return sh;
+}
+
+blfid_status blfid_plan_run_noalloc(blfid_plan *pl, int normalize, double *fid_out_dev,
+                                    void *stream) {
+    if (!pl || !fid_out_dev) return fail(BLFID_ERR_INVALID, "plan or output is NULL");
+    if (!pl->have_ops) return fail(BLFID_ERR_INVALID, "operators not set");
+    if (normalize && pl->trdd == 0.0) return fail(BLFID_ERR_INVALID, "Tr(det det^H) is zero");
+    char *base = nullptr;
+    blfid_status s = noalloc_scratch(&base);
+    if (s != BLFID_OK) return s;
+    const blfid_layout::Shape sh = noalloc_shape(pl);
+    const long Kb = blfid_layout::choose_batch(sh, pl->K, (double)BLFID_NOALLOC_SCRATCH_BYTES);
+    if (Kb < 1)
+        return fail(BLFID_ERR_NOMEM,
+                    "one crystallite of this plan needs " +
+                        std::to_string(blfid_layout::layout_bytes(sh, 1)) +
+                        " bytes of working storage, beyond the no-alloc scratch of " +
+                        std::to_string((size_t)BLFID_NOALLOC_SCRATCH_BYTES) + " bytes");
+    size_t off[blfid_layout::R_COUNT];
+    (void)blfid_layout::layout_bytes(sh, Kb, off);
+    hipStream_t st = (hipStream_t)stream;
+    pl->last_stream = st;
+    // profiling records HIP events per launch: not done here
+    const bool prof = pl->prof;
+    pl->prof = false;
+    s = enqueue_run(pl, sh, base, off, Kb, normalize, fid_out_dev, st);
+    pl->prof = prof;
+    return s;
 }
 
 blfid_status blfid_workspace_create(double bytes, blfid_workspace **out) {

@@ -88,6 +88,7 @@ def lib():
         L.blfid_plan_set_operators.argtypes = [vp, _d, _d]
         L.blfid_plan_set_powder.argtypes = [vp, _d, _d, _d, _d, ctypes.c_int]
         L.blfid_plan_run.argtypes = [vp, ctypes.c_int, vp]
+        L.blfid_plan_run_noalloc.argtypes = [vp, ctypes.c_int, vp, vp]
         L.blfid_plan_fetch.argtypes = [vp, _d]
         L.blfid_plan_device_fid.argtypes = [vp]
         L.blfid_plan_device_fid.restype = vp
@@ -251,6 +252,14 @@ class Plan:
     def run(self, normalize: bool = True, stream: int | None = None):
         _check(lib().blfid_plan_run(self.h, int(bool(normalize)),
                                     ctypes.c_void_p(stream) if stream else None))
+
+    def run_noalloc(self, fid_out_dev: int, normalize: bool = True, stream: int | None = None):
+        """blfid_plan_run_noalloc: the run without device allocation, working storage in the
+        library's static per-GPU scratch, FID (2 npts doubles) written to the device address
+        fid_out_dev."""
+        _check(lib().blfid_plan_run_noalloc(self.h, int(bool(normalize)),
+                                            ctypes.c_void_p(fid_out_dev),
+                                            ctypes.c_void_p(stream) if stream else None))
 
     def fetch(self) -> np.ndarray:
         out = np.zeros(2 * self.npts)

@@ -134,6 +134,21 @@ blfid_status blfid_plan_set_powder(blfid_plan *plan, const double *theta, const 
  * The result stays on the device until blfid_plan_fetch. */
 blfid_status blfid_plan_run(blfid_plan *plan, int normalize, void *stream);
 
+/* Variant of blfid_plan_run that allocates nothing: no reference counterpart.  It is for callers
+ * that cannot allocate device memory while they run (HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The working storage is a static device array of
+ * BLFID_NOALLOC_SCRATCH_BYTES per GPU (held until the process ends); the crystallites run in
+ * batches that fit it, with fewer concurrent scratch slots than blfid_plan_run, and the FID is
+ * bit for bit the one blfid_plan_run gives.  The normalised FID (2*npts doubles, re/im) is
+ * written to the caller's device buffer fid_out_dev, asynchronously on `stream`.  Plan tables,
+ * operators and powder must have been set (those calls allocate, at set-up).  A plan whose single
+ * crystallite needs more than the scratch returns BLFID_ERR_NOMEM before any launch.  Every call
+ * on one GPU shares that GPU's array: calls that overlap in time on one GPU (two streams, two
+ * threads) race and must be serialised by the caller.  Profiling is not recorded here. */
+#define BLFID_NOALLOC_SCRATCH_BYTES (256u << 20)
+blfid_status blfid_plan_run_noalloc(blfid_plan *plan, int normalize, double *fid_out_dev,
+                                    void *stream);
+
 /* Wait for the last run and copy its FID (2*npts doubles, re/im interleaved) to host. */
 blfid_status blfid_plan_fetch(blfid_plan *plan, double *fid_out);
 
