@@ -52,26 +52,43 @@ METRIC = "crystallite-steps/s + FIDs/s at ≤1e-5 rel L2 vs FP64 oracle; p50/p99
 COUNTER_SUMMARY = os.path.join(ROOT, "profiles", "config2_batched_{prec}_hbm_counters.txt")
 
 
-def corr_flops_per_elem(n: int, P: int) -> float:
+def corr_flops_per_elem(n: int, P: int, autoc: bool = False) -> float:
     """Executed flops of k_corr per correlation element and crystallite (as the engine counts
     them): for n a power of two from 16 to 512 three mixed-radix FFTs (radix-8 passes of a
     56-flop DFT and 7 twiddle products per butterfly, the last pass of radix 2, 4 or 8 without
     twiddles) and the scaled product, else the direct sum; plus twiddle staging and the FID
-    sums."""
+    sums.  autoc: the autocorrelation variant (R = conj(Q)) with two transforms, a |q|^2 / n
+    product (4 flops per point) and the staging of Q alone (9 n)."""
     c = 8.0 * n * n
+    stage = 18.0 * n
     if 16 <= n <= 512 and (n & (n - 1)) == 0:
         ln = int(n).bit_length() - 1
         npass = (ln + 2) // 3
         rl = 1 << (ln - 3 * (npass - 1))
         dft = {2: 4.0, 4: 16.0, 8: 56.0}[rl]
-        c = 3.0 * (npass - 1) * (n / 8.0) * 98.0 + 3.0 * (n / rl) * dft + 8.0 * n
-    return c + 18.0 * n + 14.0 * P
+        t = 2.0 if autoc else 3.0
+        c = t * (npass - 1) * (n / 8.0) * 98.0 + t * (n / rl) * dft + (4.0 if autoc else 8.0) * n
+        if autoc:
+            stage = 9.0 * n
+    return c + stage + 14.0 * P
+
+
+def corr_is_autocorrelation(wls) -> bool:
+    """R = conj(Q) for every workload: rho == (det + det^H) / 2 exactly and det has no entry
+    pair (a, b), (b, a) (so no correlation element has a reverse element)."""
+    for w in wls:
+        rho, det = w.operators()
+        if not np.array_equal(rho, 0.5 * (det + det.conj().T)):
+            return False
+        if np.any((det != 0) & (det.T != 0)):
+            return False
+    return True
 
 
 # What each kernel family issues on MI355X, per precision: "mfma" where its products run on the
 # matrix cores (complex64 Dyson chain, B = A W and Q products), "valu" where they run on the
 # vector ALUs (the propagator's real Taylor GEMMs, the FFT correlation, every FP64 product).
-ISSUES = {"k_exp": "valu", "k_dyson": "mfma", "k_bmul": "mfma", "k_qr": "mfma",
+ISSUES = {"k_exp": "valu", "k_dyson": "mfma", "k_expdyson": "mfma", "k_bmul": "mfma", "k_qr": "mfma",
           "k_corr": "valu", "k_eig": "valu", "k_reduce": "valu", "k_static_fid": "valu"}
 
 
@@ -564,7 +581,7 @@ def main():
     if dom == "k_corr":
         n, P = wls[0].steps, wls[0].npts
         d = fams[dom]
-        ratio = (14.0 * n * n + 14.0 * P) / corr_flops_per_elem(n, P)
+        ratio = (14.0 * n * n + 14.0 * P) / corr_flops_per_elem(n, P, corr_is_autocorrelation(wls))
         roof["canonical_flops_per_launch"] = ratio * d["flops"] / d["launches"]
         roof["canonical_tflops"] = ratio * d["flops"] / (d["ms"] * 1e-3) / 1e12
         roof["canonical_note"] = (

@@ -46,7 +46,8 @@ struct DevPlan {
     const double *thetak;  // n: 2 pi tau_k of steps of unequal length, or NULL (all theta)
     int rt_from_q;         // 1: R(s,r) = Q_rev + conj(Q(r,s)), R is neither formed nor stored
     const int *elem_rev;   // Etot: element (s,r) of the reverse pair (J,I), or -1
-    double *expflops;      // 8: executed propagator flops per tile variant (profiling), or NULL
+    double *expflops;      // 16: executed flops per tile variant of the propagator [0..7] and
+                           // the fused propagator + Dyson chain [8..15] (profiling), or NULL
     int bf16x3;            // complex64 products as three bf16 matrix-core products (BLFID_C64_BF16X3)
     long Estr;             // Q / R row stride per step: Etot padded to a multiple of 16 elements
     int nparts;            // FID partial sums per crystallite (correlation segments or block pairs)

@@ -39,6 +39,8 @@ int blfid_dev_launch_exp(int prec, int M, const DevPlan *P, int Kb, int c0, cons
                          hipStream_t st);
 int blfid_dev_launch_dyson(int prec, int M, const DevPlan *P, int Kb, const int *grp, int ngrp,
                            const void *bufU, void *bufA, hipStream_t st);
+int blfid_dev_launch_expdyson(int M, const DevPlan *P, int Kb, int c0, const int *grp, int ngrp,
+                              void *bufA, hipStream_t st);
 int blfid_dev_launch_eig(int prec, const DevPlan *P, int Kb, const int *grp, int ngrp,
                          const void *src, size_t src_stride, int mode, double2 *bufW,
                          void *bufWr, double *outv, int cls, double2 *gscr, size_t gslot,
@@ -191,6 +193,8 @@ struct blfid_plan {
     long Lsum = 0;
     int bmax = 0;
     std::map<int, std::vector<int>> blk_groups; // M -> blocks
+    std::map<int, std::vector<int>> fuse_groups; // M -> blocks of <= DEV_LDS_BLOCK states
+    std::vector<int> big_blocks;                 // blocks beyond DEV_LDS_BLOCK states
     std::map<int, std::vector<int>> eig_cls;    // eigen-solver size class -> blocks
     int exp_slots = 0, qr_slots = 0, eig_slots = 0;
     // operators
@@ -218,7 +222,8 @@ struct blfid_plan {
         d_pair_roff, d_drow, d_dcol, d_rrow, d_rcol, d_er, d_es, d_erev;
     DevBuf<long> d_blk_uoff, d_pair_eoff;
     DevBuf<double> d_dval, d_rval;
-    std::map<int, DevBuf<int>> d_bgrp, d_pgrp;
+    std::map<int, DevBuf<int>> d_bgrp, d_pgrp, d_fgrp;
+    DevBuf<int> d_bigb;
     std::map<int, DevBuf<int>> d_eig_cls;
     DevBuf<int> d_spin_rf;
     DevBuf<double> d_rf, d_thetak;
@@ -453,6 +458,8 @@ static blfid_status build_tables(blfid_plan *pl) {
     }
     pl->blk_dim.clear(); pl->blk_soff.clear(); pl->blk_uoff.clear(); pl->blk_states.clear();
     pl->blk_groups.clear();
+    pl->fuse_groups.clear();
+    pl->big_blocks.clear();
     long uo = 0;
     int so = 0;
     pl->bmax = 0;
@@ -467,6 +474,8 @@ static blfid_status build_tables(blfid_plan *pl) {
         pl->bmax = std::max(pl->bmax, d);
         int M = std::min(5, (d + 15) / 16);
         pl->blk_groups[M].push_back((int)b);
+        if (d <= DEV_LDS_BLOCK) pl->fuse_groups[M].push_back((int)b);
+        else pl->big_blocks.push_back((int)b);
     }
     pl->Lsum = uo;
     pl->eig_cls.clear();
@@ -530,6 +539,8 @@ static blfid_status upload_tables(blfid_plan *pl) {
     HIPCHK(pl->d_blk_states.upload(pl->blk_states));
     HIPCHK(pl->d_blk_uoff.upload(pl->blk_uoff));
     for (auto &kv : pl->blk_groups) HIPCHK(pl->d_bgrp[kv.first].upload(kv.second));
+    for (auto &kv : pl->fuse_groups) HIPCHK(pl->d_fgrp[kv.first].upload(kv.second));
+    HIPCHK(pl->d_bigb.upload(pl->big_blocks));
     for (auto &kv : pl->eig_cls) HIPCHK(pl->d_eig_cls[kv.first].upload(kv.second));
     HIPCHK(pl->d_spin_rf.upload(pl->spin_rf));
     HIPCHK(pl->d_rf.upload(pl->rf));
@@ -1067,24 +1078,23 @@ static double pair_flops(const blfid_plan *pl, const std::vector<int> &g) {
 }
 
 
-// Executed flops per correlation element and crystallite in k_corr: for n a power of two up to
-// 512, three radix-2 transforms of n/2 log2 n butterflies at 10 flops and the pointwise product
-// with the 1/n scale (8 per step); otherwise the direct cyclic sum, 8 n^2.  Plus the twiddle
-// staging (18 per step) and the FID recurrence (14 per point: complex multiply-add and the
-// z^t update).
-// Executed flops of the correlation per element: FFT variant (n a power of two, 16..512) as
-// 3 transforms of np = ceil(log2(n) / 3) passes, radix 8 (a 56-flop DFT and 7 twiddle products
-// per butterfly) but the last, of radix rl without twiddles (DFT of 4, 16 or 56 flops), plus the
-// scaled product (8 per point); direct variant 8 n^2; plus twiddle staging (18 n) and the FID
-// sums (14 per point).
-static double corr_flops_per_elem(int n, int P) {
+// Executed flops of the correlation per element and crystallite: FFT variant (n a power of two,
+// 16..512) as T transforms of np = ceil(log2(n) / 3) passes, radix 8 (a 56-flop DFT and 7
+// twiddle products per butterfly) but the last, of radix rl without twiddles (DFT of 4, 16 or
+// 56 flops), plus the scaled product (8 per point); direct variant 8 n^2; plus twiddle staging
+// (18 n) and the FID sums (14 per point).  T = 3 in general; the autocorrelation variant
+// (R = conj(Q)) forms two transforms, a 4-flop |q|^2 / n product and stages Q alone (9 n).
+static double corr_flops_per_elem(int n, int P, bool autoc = false) {
     double c = 8.0 * (double)n * n;
+    double stage = 18.0 * n;
     if (n >= 16 && (n & (n - 1)) == 0 && n <= 512) {
         const int ln = __builtin_ctz((unsigned)n), np = (ln + 2) / 3, rl = 1 << (ln - 3 * (np - 1));
         const double dft = rl == 2 ? 4.0 : rl == 4 ? 16.0 : 56.0;
-        c = 3.0 * (np - 1) * (n / 8.0) * 98.0 + 3.0 * ((double)n / rl) * dft + 8.0 * n;
+        const double T = autoc ? 2.0 : 3.0;
+        c = T * (np - 1) * (n / 8.0) * 98.0 + T * ((double)n / rl) * dft + (autoc ? 4.0 : 8.0) * n;
+        if (autoc) stage = 9.0 * n;
     }
-    return c + 18.0 * n + 14.0 * P;
+    return c + stage + 14.0 * P;
 }
 
 static double pair_elems(const blfid_plan *pl, const std::vector<int> &g) {
@@ -1139,7 +1149,7 @@ blfid_status blfid_plan_stats(const blfid_plan *pl, double *out) {
         ex += 8.0 * s3;         // Dyson
         ex += 8.0 * s3;         // B = A W
         ex += 8.0 * pairsQ;     // Q and R
-        ex += pairs * corr_flops_per_elem(pl->n, pl->npts) / n; // correlation + FID
+        ex += pairs * corr_flops_per_elem(pl->n, pl->npts, pl->rt_from_q && !pl->rt_rev) / n;
     }
     out[0] = ex;
     if (pl->ptop) out[1] = 8.0 * N * N * N + 16.0 * N * N * N * P / n;
@@ -1231,21 +1241,42 @@ static blfid_status enqueue_run(blfid_plan *pl, const blfid_layout::Shape &sh, c
                                                       sh.qr_slots, 1.0, 0, pl->part, st));
             }
         } else if (pl->spinning) {
-            for (auto &kv : pl->blk_groups) {
-                PhaseScope ps(pl, 0 * 8 + kv.first, 0.0,
-                              kb * (double)pl->n * group_b2(pl, kv.second) * cs, st); // flops: k_exp
+            // complex64 matrix-core plans: blocks of <= 80 states take the fused propagator +
+            // Dyson chain (U_k never leaves LDS); larger blocks, the FP64 form and the BF16x3
+            // form take the propagator and the Dyson chain as two kernels
+            const bool fused = pr == 1 && !pl->bf16x3;
+            if (fused) {
+                for (auto &kv : pl->fuse_groups) {
+                    PhaseScope ps(pl, 8 * 8 + kv.first, 0.0,
+                                  kb * (double)pl->n * group_b2(pl, kv.second) * cs, st); // flops: device
+                    LAUNCHCHK(blfid_dev_launch_expdyson(kv.first, P, kb, c0, pl->d_fgrp[kv.first].p,
+                                                        (int)kv.second.size(), pl->bufA, st));
+                }
+            }
+            // (tile variant, device group list, host group list) of the two-kernel path
+            std::vector<std::pair<int, std::pair<const int *, const std::vector<int> *>>> two;
+            if (fused) {
+                if (!pl->big_blocks.empty()) two.push_back({5, {pl->d_bigb.p, &pl->big_blocks}});
+            } else {
+                for (auto &kv : pl->blk_groups) two.push_back({kv.first, {pl->d_bgrp[kv.first].p, &kv.second}});
+            }
+            for (auto &gq : two) {
+                const int M = gq.first;
+                const std::vector<int> &gv = *gq.second.second;
+                PhaseScope ps(pl, 0 * 8 + M, 0.0, kb * (double)pl->n * group_b2(pl, gv) * cs, st); // flops: k_exp
                 size_t slot;
                 int ns;
-                group_slots(bm2, sh.exp_slots, group_e(pl, kv.second), kv.first, &slot, &ns);
-                LAUNCHCHK(blfid_dev_launch_exp(pr, kv.first, P, kb, c0, pl->d_bgrp[kv.first].p,
-                                               (int)kv.second.size(), pl->scrE, slot, ns,
-                                               pl->bufU, st));
+                group_slots(bm2, sh.exp_slots, group_e(pl, gv), M, &slot, &ns);
+                LAUNCHCHK(blfid_dev_launch_exp(pr, M, P, kb, c0, gq.second.first, (int)gv.size(),
+                                               pl->scrE, slot, ns, pl->bufU, st));
             }
-            for (auto &kv : pl->blk_groups) {
-                PhaseScope ps(pl, 1 * 8 + kv.first, kb * (double)(pl->n - 1) * group_b3(pl, kv.second) * 8.0,
-                              kb * 2.0 * pl->n * group_b2(pl, kv.second) * cs, st);
-                LAUNCHCHK(blfid_dev_launch_dyson(pr, kv.first, P, kb, pl->d_bgrp[kv.first].p,
-                                                 (int)kv.second.size(), pl->bufU, pl->bufA, st));
+            for (auto &gq : two) {
+                const int M = gq.first;
+                const std::vector<int> &gv = *gq.second.second;
+                PhaseScope ps(pl, 1 * 8 + M, kb * (double)(pl->n - 1) * group_b3(pl, gv) * 8.0,
+                              kb * 2.0 * pl->n * group_b2(pl, gv) * cs, st);
+                LAUNCHCHK(blfid_dev_launch_dyson(pr, M, P, kb, gq.second.first, (int)gv.size(),
+                                                 pl->bufU, pl->bufA, st));
             }
             for (auto &kv : pl->eig_cls) {
                 PhaseScope ps(pl, 2 * 8 + kv.first + 1, 0.0,
@@ -1279,7 +1310,8 @@ static blfid_status enqueue_run(blfid_plan *pl, const blfid_layout::Shape &sh, c
             }
             if (pl->Etot > 0) {
                 const double rq = (!pl->rt_from_q || pl->rt_rev) ? 2.0 : 1.0; // Q, and R or reverse Q
-                PhaseScope ps(pl, 5 * 8, kb * (double)pl->Etot * corr_flops_per_elem(pl->n, pl->npts),
+                const bool autoc = pl->rt_from_q && !pl->rt_rev;
+                PhaseScope ps(pl, 5 * 8, kb * (double)pl->Etot * corr_flops_per_elem(pl->n, pl->npts, autoc),
                               kb * ((double)pl->n * pl->Etot * cs * rq + (double)pl->nparts * pl->npts * 16.0), st);
                 LAUNCHCHK(blfid_dev_launch_corr(pr, P, kb, pl->bufQt, pl->bufRt, pl->thetas,
                                                 pl->d_er.p, pl->d_es.p, pl->part, st));
@@ -1486,9 +1518,9 @@ blfid_status blfid_plan_set_profiling(blfid_plan *pl, int on) {
     if (pl->prof) {
         // the propagator's degree is chosen per task on the device, so its executed flops are
         // counted there (one atomic add per workgroup)
-        if (!pl->d_expflops.p) HIPCHK(pl->d_expflops.alloc(8));
+        if (!pl->d_expflops.p) HIPCHK(pl->d_expflops.alloc(16));
         HIPCHK(hipDeviceSynchronize());
-        HIPCHK(hipMemset(pl->d_expflops.p, 0, 8 * sizeof(double)));
+        HIPCHK(hipMemset(pl->d_expflops.p, 0, 16 * sizeof(double)));
     }
     return BLFID_OK;
 }
@@ -1496,13 +1528,15 @@ blfid_status blfid_plan_set_profiling(blfid_plan *pl, int on) {
 blfid_status blfid_plan_phase_times(blfid_plan *pl, double *ms, int *launches, double *flops) {
     if (!pl || !ms || !launches || !flops) return fail(BLFID_ERR_INVALID, "NULL argument");
     collect_profile(pl);
-    double ef[8] = {0};
+    // device-counted flops: [0..7] the propagator per tile variant (phase 0), [8..15] the fused
+    // propagator + Dyson chain per tile variant (phase 8)
+    double ef[16] = {0};
     if (pl->prof && pl->d_expflops.p)
         HIPCHK(hipMemcpy(ef, pl->d_expflops.p, sizeof(ef), hipMemcpyDeviceToHost));
     for (int i = 0; i < BLFID_NSLOT; i++) {
         ms[i] = pl->ph_ms[i];
         launches[i] = pl->ph_n[i];
-        flops[i] = pl->ph_flops[i] + (i < 8 ? ef[i] : 0.0);
+        flops[i] = pl->ph_flops[i] + (i < 8 ? ef[i] : (i >= 64 ? ef[8 + (i - 64)] : 0.0));
     }
     return BLFID_OK;
 }

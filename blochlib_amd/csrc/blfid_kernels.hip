@@ -905,6 +905,274 @@ __global__ void __launch_bounds__(NThr<M>::v, (M == 3 && sizeof(R) == 4) ? 3 : (
     }
 }
 
+// ------------------------------------------------------------------------------ k_expdyson
+// LDS-resident real symmetric products on FP32 matrix cores.  A plane holds a 16M x 16M symmetric
+// matrix (zero past the block size b) in rows of LDK = 16M + 4 floats: plane[r LDK + q] =
+// X(r, q) = X(q, r), so one layout is both the k-contiguous A operand (rows) and B operand
+// (columns) of v_mfma_f32_16x16x4f32 (lane 16 g + r reads the float4 at row r, k = 4g .. 4g + 3
+// of each 16-wide k slice, as wg_cgemm_mfma).  Every polynomial in X, and cos X, sin X, is
+// symmetric, so the accumulators (rows 4g + j, column r of a sub-tile) are stored transposed as
+// float4 runs, plane[col][row .. row + 3], with no exchange.  Sub-tiles t = w + NW s of the
+// output are dealt round robin to the NW waves.
+template <int M, int NT> struct SymTile {
+    static constexpr int TM = 16 * M, LDK = TM + 4, NW = NT / 64, NSUB = (M * M + NW - 1) / NW;
+    static constexpr int PLANE = TM * LDK; // floats
+};
+
+template <int NS> __device__ __forceinline__ void sym_zero(v4f (&acc)[NS]) {
+#pragma unroll
+    for (int s = 0; s < NS; s++) acc[s] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+}
+
+// acc += sa PA PB over k < kp (the wave's sub-tiles).  VIRT: the B operand is formed on the fly
+// from two planes, ca I + cb PY + cc PZ (the identity masked to k < b), instead of read from PB.
+template <int M, int NT, bool VIRT, int NS>
+__device__ __forceinline__ void sym_mfma(const float *PA, const float *PB, const float *PY,
+                                         const float *PZ, float sa, float ca, float cb, float cc,
+                                         int b, int kp, v4f (&acc)[NS]) {
+    typedef SymTile<M, NT> T;
+    static_assert(NS == T::NSUB, "sub-tile count");
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63, g = lane >> 4, r = lane & 15;
+    for (int qq = 0; qq < kp; qq += 16) {
+#pragma unroll
+        for (int s = 0; s < T::NSUB; s++) {
+            const int t = w + T::NW * s;
+            if (t < M * M) {
+                const int col = 16 * (t % M) + r;
+                const int oa = (16 * (t / M) + r) * T::LDK + qq + 4 * g;
+                const int ob = col * T::LDK + qq + 4 * g;
+                const v4f av = *(const v4f *)&PA[oa];
+                v4f bv;
+                if constexpr (VIRT) {
+                    const v4f y = *(const v4f *)&PY[ob], z = *(const v4f *)&PZ[ob];
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                        const int k = qq + 4 * g + u;
+                        bv[u] = cb * y[u] + cc * z[u] + ((k == col && k < b) ? ca : 0.0f);
+                    }
+                } else {
+                    bv = *(const v4f *)&PB[ob];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; u++)
+                    acc[s] = __builtin_amdgcn_mfma_f32_16x16x4f32(sa * av[u], bv[u], acc[s], 0, 0, 0);
+            }
+        }
+    }
+}
+
+// plane = sc * acc (symmetric result, full 16M x 16M tile including the zero padding)
+template <int M, int NT, int NS>
+__device__ __forceinline__ void sym_store(float *PL, const v4f (&acc)[NS], float sc) {
+    typedef SymTile<M, NT> T;
+    static_assert(NS == T::NSUB, "sub-tile count");
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63, g = lane >> 4, r = lane & 15;
+#pragma unroll
+    for (int s = 0; s < T::NSUB; s++) {
+        const int t = w + T::NW * s;
+        if (t < M * M) {
+            v4f v;
+#pragma unroll
+            for (int j = 0; j < 4; j++) v[j] = sc * acc[s][j];
+            *(v4f *)&PL[(16 * (t % M) + r) * T::LDK + 16 * (t / M) + 4 * g] = v;
+        }
+    }
+}
+
+// Fused propagator and Dyson chain of the complex64 gamma-COMPUTE path for blocks of at most
+// 16 M <= 80 states: one task = (crystallite c, block) walks the n steps of the rotor period with
+// every matrix in LDS (five planes: X, Y, Z and the running product's re / im parts):
+//   X  = theta H_k / 2^s                      (build_h_block, H at the step midpoint, sys.cc:749-770)
+//   cos X, sin X by Taylor polynomials in Y = X^2 of degree m (Paterson-Stockmeyer):
+//     m = 1:  C = I - Y/2,  P = I - Y/6                       Y                     (2 products)
+//     m = 2:  C, P elementwise in Y, Y^2                      Y, Y^2                (3 products)
+//     m = 4:  C = c0 I + c1 Y + Y^2 (c2 I + c3 Y + c4 Y^2), P likewise, the bracket formed on the
+//             fly as the B operand                            Y, Y^2, C, P          (5 products)
+//     S = X P                                                                        (+1 product)
+//   s squarings (C, S) <- (C^2 - S^2, 2 C S), s the least with ||X||_inf <= 0.75 (where m = 4
+//   meets the FP32 tolerance of k_exp), then
+//   A_k = (C + i usign S) A_{k-1}   (compute.h:267; usign = -1: Mexp(H, -i 2 pi tau), compute.h:265)
+// on v_mfma_f32_16x16x4f32 with A resident in LDS.  Only A_k goes to global memory (the layout
+// k_dyson writes: product k at slot k of the crystallite's n slots).  m is chosen per step from the
+// norm bound, as the smallest whose remainder bound r^(2m+2)/(2m+2)! is below 2e-8.  Replaces
+// k_exp + k_dyson for these blocks (matexp.h:239-252, compute.h:257-274).
+template <int M>
+__global__ void __launch_bounds__(NThr<M>::v) k_expdyson(DevPlan P, int Kb, int c0,
+                                                         const int *__restrict__ grp, int ngrp,
+                                                         cf *__restrict__ bufA) {
+    constexpr int NT = NThr<M>::v;
+    typedef SymTile<M, NT> T;
+    constexpr int TM = T::TM, LDK = T::LDK, NSUB = T::NSUB, NW = T::NW;
+    __shared__ __align__(16) float pl[5 * T::PLANE];
+    float *PX = pl, *PY = PX + T::PLANE, *PZ = PY + T::PLANE, *PBr = PZ + T::PLANE,
+          *PBi = PBr + T::PLANE;
+    __shared__ double s_zz[DEV_MAX_TERMS];
+    __shared__ double s_ff[DEV_MAX_SPINS * DEV_MAX_SPINS];
+    __shared__ double s_v[4];
+    __shared__ double s_red[NT / 64];
+    __shared__ double s_diag[DEV_MAX_BLOCK];
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63, g = lane >> 4, r = lane & 15;
+    const double *terms = stage_terms<NT>(P);
+    const float us = (float)P.usign;
+    constexpr double RMAX = 0.75, TOL = 2e-8;
+    double nflops = 0.0;
+    const long ntask = (long)Kb * ngrp;
+    for (long task = blockIdx.x; task < ntask; task += gridDim.x) {
+        const int gi = (int)(task % ngrp), c = (int)(task / ngrp);
+        const int blk = grp[gi], b = P.blk_dim[blk];
+        const int kp = (b + 15) & ~15;
+        cf *A0 = bufA + (size_t)c * P.n * P.Lsum + P.blk_uoff[blk];
+        // X's padding (rows and columns b .. 16M - 1) stays zero: the H build writes the b x b
+        // part only, and every product of zero-padded symmetric planes is zero-padded
+        for (int e = tid; e < T::PLANE; e += NT) PX[e] = 0.0f;
+        __syncthreads();
+        for (int k = 0; k < P.n; k++) {
+            const double theta = P.theta;
+            int sq = 0;
+            const double nrm =
+                build_h_block<float, NT>(P, terms, P.RP + 9 * (size_t)(c0 + c), P.u3 + 3 * k, blk,
+                                         PX, LDK, s_zz, s_ff, s_v, s_diag, s_red, nullptr,
+                                         [&](double hn) {
+                                             while (hn * theta / (double)(1 << sq) > RMAX && sq < 30) sq++;
+                                             return theta / (double)(1 << sq);
+                                         }) * theta;
+            const double rx = nrm / (double)(1 << sq);
+            const double r2 = rx * rx;
+            const int m = (r2 * r2 / 24.0 <= TOL) ? 1 : (r2 * r2 * r2 / 720.0 <= TOL) ? 2 : 4;
+            v4f acc[NSUB], acc2[NSUB];
+            // Y = X X
+            sym_zero(acc);
+            sym_mfma<M, NT, false>(PX, PX, nullptr, nullptr, 1.0f, 0.f, 0.f, 0.f, b, kp, acc);
+            sym_store<M, NT>(PY, acc, 1.0f);
+            __syncthreads();
+            int ng = 2 + 3 * sq;
+            if (m >= 2) {
+                // Z = Y Y
+                sym_zero(acc);
+                sym_mfma<M, NT, false>(PY, PY, nullptr, nullptr, 1.0f, 0.f, 0.f, 0.f, b, kp, acc);
+                sym_store<M, NT>(PZ, acc, 1.0f);
+                __syncthreads();
+                ng += 1;
+            }
+            if (m <= 2) {
+                // C = I - Y/2 + Y^2/24, P = I - Y/6 + Y^2/120 (no Y^2 term for m = 1), into Y, Z
+                const float z2c = m == 2 ? (float)(1.0 / 24.0) : 0.0f;
+                const float z2s = m == 2 ? (float)(1.0 / 120.0) : 0.0f;
+                for (int e = tid; e < TM * TM; e += NT) {
+                    const int rr = e / TM, qq = e - rr * TM, o = rr * LDK + qq;
+                    const float d = (rr == qq && rr < b) ? 1.0f : 0.0f;
+                    const float y = PY[o], z = m == 2 ? PZ[o] : 0.0f;
+                    PY[o] = d - 0.5f * y + z2c * z;
+                    PZ[o] = d - (float)(1.0 / 6.0) * y + z2s * z;
+                }
+                __syncthreads();
+            } else {
+                // C = I - Y/2 + Y^2 (1/24 I - Y/720 + Y^2/40320), P with 1/120, -1/5040, 1/362880
+                sym_zero(acc);
+                sym_zero(acc2);
+                sym_mfma<M, NT, true>(PZ, nullptr, PY, PZ, 1.0f, (float)(1.0 / 24.0),
+                                      (float)(-1.0 / 720.0), (float)(1.0 / 40320.0), b, kp, acc);
+                sym_mfma<M, NT, true>(PZ, nullptr, PY, PZ, 1.0f, (float)(1.0 / 120.0),
+                                      (float)(-1.0 / 5040.0), (float)(1.0 / 362880.0), b, kp, acc2);
+                __syncthreads(); // every read of Y and Z done
+#pragma unroll
+                for (int s = 0; s < NSUB; s++) {
+                    const int t = w + NW * s;
+                    if (t < M * M) {
+                        const int col = 16 * (t % M) + r, row0 = 16 * (t / M) + 4 * g;
+                        const int o = col * LDK + row0;
+                        const v4f y = *(const v4f *)&PY[o];
+                        v4f cv, pv;
+#pragma unroll
+                        for (int j = 0; j < 4; j++) {
+                            const float d = (row0 + j == col && col < b) ? 1.0f : 0.0f;
+                            cv[j] = acc[s][j] + d - 0.5f * y[j];
+                            pv[j] = acc2[s][j] + d - (float)(1.0 / 6.0) * y[j];
+                        }
+                        *(v4f *)&PY[o] = cv;
+                        *(v4f *)&PZ[o] = pv;
+                    }
+                }
+                __syncthreads();
+                ng += 2;
+            }
+            // S = X P into X
+            sym_zero(acc);
+            sym_mfma<M, NT, false>(PX, PZ, nullptr, nullptr, 1.0f, 0.f, 0.f, 0.f, b, kp, acc);
+            __syncthreads();
+            sym_store<M, NT>(PX, acc, 1.0f);
+            __syncthreads();
+            // squarings: (C, S) <- (C^2 - S^2, 2 C S)
+            for (int it = 0; it < sq; it++) {
+                sym_zero(acc);
+                sym_zero(acc2);
+                sym_mfma<M, NT, false>(PY, PY, nullptr, nullptr, 1.0f, 0.f, 0.f, 0.f, b, kp, acc);
+                sym_mfma<M, NT, false>(PX, PX, nullptr, nullptr, -1.0f, 0.f, 0.f, 0.f, b, kp, acc);
+                sym_mfma<M, NT, false>(PY, PX, nullptr, nullptr, 1.0f, 0.f, 0.f, 0.f, b, kp, acc2);
+                __syncthreads();
+                sym_store<M, NT>(PY, acc, 1.0f);
+                sym_store<M, NT>(PX, acc2, 2.0f);
+                __syncthreads();
+            }
+            nflops += 2.0 * b * b * b * ng;
+            // A_k = U_k A_{k-1}, U = C + i us S (C in Y, S in X)
+            cf *Ak = A0 + (size_t)k * P.Lsum;
+            if (k == 0) {
+                for (int e = tid; e < TM * TM; e += NT) {
+                    const int jj = e / TM, qq = e - jj * TM, o = jj * LDK + qq;
+                    const float cr = PY[o], ci = us * PX[o];
+                    PBr[o] = cr; // column jj of A_0 (U_0 is symmetric)
+                    PBi[o] = ci;
+                    if (jj < b && qq < b) Ak[qq + jj * b] = make_float2(cr, ci);
+                }
+                __syncthreads();
+                continue;
+            }
+            v4f cr[NSUB], ci[NSUB];
+            sym_zero(cr);
+            sym_zero(ci);
+            for (int qq = 0; qq < kp; qq += 16) {
+#pragma unroll
+                for (int s = 0; s < NSUB; s++) {
+                    const int t = w + NW * s;
+                    if (t < M * M) {
+                        const int oa = (16 * (t / M) + r) * LDK + qq + 4 * g;
+                        const int ob = (16 * (t % M) + r) * LDK + qq + 4 * g;
+                        const v4f uc = *(const v4f *)&PY[oa], usn = *(const v4f *)&PX[oa];
+                        const v4f br = *(const v4f *)&PBr[ob], bi = *(const v4f *)&PBi[ob];
+#pragma unroll
+                        for (int u = 0; u < 4; u++) {
+                            const float si = us * usn[u];
+                            cr[s] = __builtin_amdgcn_mfma_f32_16x16x4f32(uc[u], br[u], cr[s], 0, 0, 0);
+                            cr[s] = __builtin_amdgcn_mfma_f32_16x16x4f32(-si, bi[u], cr[s], 0, 0, 0);
+                            ci[s] = __builtin_amdgcn_mfma_f32_16x16x4f32(uc[u], bi[u], ci[s], 0, 0, 0);
+                            ci[s] = __builtin_amdgcn_mfma_f32_16x16x4f32(si, br[u], ci[s], 0, 0, 0);
+                        }
+                    }
+                }
+            }
+            __syncthreads(); // every read of the running product done
+#pragma unroll
+            for (int s = 0; s < NSUB; s++) {
+                const int t = w + NW * s;
+                if (t < M * M) {
+                    const int col = 16 * (t % M) + r, row0 = 16 * (t / M) + 4 * g;
+                    *(v4f *)&PBr[col * LDK + row0] = cr[s];
+                    *(v4f *)&PBi[col * LDK + row0] = ci[s];
+                    if (col < b) {
+#pragma unroll
+                        for (int j = 0; j < 4; j++)
+                            if (row0 + j < b) Ak[row0 + j + col * b] = make_float2(cr[s][j], ci[s][j]);
+                    }
+                }
+            }
+            nflops += 8.0 * b * b * b;
+            __syncthreads();
+        }
+    }
+    if (P.expflops && tid == 0 && nflops > 0.0) atomicAdd(&P.expflops[8 + M], nflops);
+}
+
 // ------------------------------------------------------------------------------ k_eig
 // Joint Jacobi diagonalisation of a normal matrix Mx = H1 + i H2 (H1, H2 Hermitian,
 // commuting): each 2x2 rotation is the Cardoso-Souloumiac angle for the pair (H1, H2)
@@ -1473,7 +1741,8 @@ template <int LN> struct CorrFft {
 // blk L + j + k L / RAD; the inverse pass multiplies by the conjugate twiddle first and applies
 // the opposite-sign DFT.  One work item = one butterfly of one element (2048 / RAD items).
 template <int RAD> struct Log2R { static constexpr int v = RAD == 8 ? 3 : RAD == 4 ? 2 : 1; };
-template <int RAD, int LN, int LL, typename R>
+// AUTO: R~ = conj(Q~) (the autocorrelation case), only the Q transform is formed.
+template <int RAD, int LN, int LL, typename R, bool AUTO>
 __device__ __forceinline__ void corr_fwd_pass(typename CX<R>::T *sQ, typename CX<R>::T *sR,
                                               const typename CX<R>::T *tw) {
     typedef typename CX<R>::T CT;
@@ -1484,23 +1753,23 @@ __device__ __forceinline__ void corr_fwd_pass(typename CX<R>::T *sQ, typename CX
         const int idx = threadIdx.x + 256 * it;
         const int el = idx & (F::CH - 1), b = idx >> F::LC;
         const int j = b & ((1 << LS) - 1), base = F::at(((b >> LS) << LL) + j, el);
-        CT q[RAD], r[RAD];
+        CT q[RAD], r[AUTO ? 1 : RAD];
 #pragma unroll
         for (int m = 0; m < RAD; m++) {
             q[m] = sQ[base + ((m << LS) * F::ROW)];
-            r[m] = sR[base + ((m << LS) * F::ROW)];
+            if constexpr (!AUTO) r[m] = sR[base + ((m << LS) * F::ROW)];
         }
         RegDFT<RAD, -1, R>::run(q);
-        RegDFT<RAD, +1, R>::run(r);
+        if constexpr (!AUTO) RegDFT<RAD, +1, R>::run(r);
 #pragma unroll
         for (int k = 0; k < RAD; k++) {
             if (k) {
                 const CT w = F::template twid<-1>(tw, (j * k) << TSH);
                 q[k] = cmul(q[k], w);
-                r[k] = cmul(r[k], cconj(w));
+                if constexpr (!AUTO) r[k] = cmul(r[k], cconj(w));
             }
             sQ[base + ((k << LS) * F::ROW)] = q[k];
-            sR[base + ((k << LS) * F::ROW)] = r[k];
+            if constexpr (!AUTO) sR[base + ((k << LS) * F::ROW)] = r[k];
         }
     }
 }
@@ -1526,8 +1795,9 @@ __device__ __forceinline__ void corr_inv_pass(typename CX<R>::T *sQ, const typen
     }
 }
 // The last forward pass (sub-transforms of length RAD: no twiddles), the product scaled by 1/n
-// and the first inverse pass, on consecutive groups of RAD points, into sQ.
-template <int LN, typename R>
+// and the first inverse pass, on consecutive groups of RAD points, into sQ.  AUTO: the product
+// is |Q~|^2 / n (R~ = conj(Q~)).
+template <int LN, typename R, bool AUTO>
 __device__ __forceinline__ void corr_mid_pass(typename CX<R>::T *sQ, const typename CX<R>::T *sR) {
     typedef typename CX<R>::T CT;
     typedef CorrFft<LN> F;
@@ -1537,16 +1807,24 @@ __device__ __forceinline__ void corr_mid_pass(typename CX<R>::T *sQ, const typen
     for (int it = 0; it < (CORR_SPAN >> LR) / 256; it++) {
         const int idx = threadIdx.x + 256 * it;
         const int el = idx & (F::CH - 1), base = F::at((idx >> F::LC) << LR, el);
-        CT q[RAD], r[RAD];
+        CT q[RAD], r[AUTO ? 1 : RAD];
 #pragma unroll
         for (int m = 0; m < RAD; m++) {
             q[m] = sQ[base + m * F::ROW];
-            r[m] = sR[base + m * F::ROW];
+            if constexpr (!AUTO) r[m] = sR[base + m * F::ROW];
         }
         RegDFT<RAD, -1, R>::run(q);
-        RegDFT<RAD, +1, R>::run(r);
+        if constexpr (AUTO) {
 #pragma unroll
-        for (int m = 0; m < RAD; m++) q[m] = cscale(cmul(q[m], r[m]), inv_n);
+            for (int m = 0; m < RAD; m++) {
+                q[m].x = (q[m].x * q[m].x + q[m].y * q[m].y) * inv_n;
+                q[m].y = (R)0;
+            }
+        } else {
+            RegDFT<RAD, +1, R>::run(r);
+#pragma unroll
+            for (int m = 0; m < RAD; m++) q[m] = cscale(cmul(q[m], r[m]), inv_n);
+        }
         RegDFT<RAD, +1, R>::run(q);
 #pragma unroll
         for (int m = 0; m < RAD; m++) sQ[base + m * F::ROW] = q[m];
@@ -1558,8 +1836,8 @@ __device__ __forceinline__ void corr_mid_pass(typename CX<R>::T *sQ, const typen
 // with four points is held to three waves per SIMD (168 registers: at four it spilled, and its
 // sixteen loads per chunk then waited on the spill reloads); the others are bound by LDS or
 // registers at two.
-template <typename R, int LN, int NPT>
-__global__ void __launch_bounds__(256, (LN > 0 && sizeof(R) == 4 && NPT == 4) ? 3 : 2) k_corr(DevPlan P, int Kb,
+template <typename R, int LN, int NPT, bool AUTO>
+__global__ void __launch_bounds__(256, (LN > 0 && sizeof(R) == 4 && NPT == 4) ? (AUTO ? 4 : 3) : 2) k_corr(DevPlan P, int Kb,
                                               const typename CX<R>::T *__restrict__ Qt,
                                               const typename CX<R>::T *__restrict__ Rt,
                                               const double *__restrict__ thetas,
@@ -1635,24 +1913,21 @@ __global__ void __launch_bounds__(256, (LN > 0 && sizeof(R) == 4 && NPT == 4) ? 
                     const bool ok = e < P.Etot;
                     const long ec = ok ? e : 0;
                     const int er_ = elem_r[ec], es_ = elem_s[ec];
-                    const int rv = P.rt_from_q ? P.elem_rev[ec] : -1;
+                    const int rv = (!AUTO && P.rt_from_q) ? P.elem_rev[ec] : -1;
                     const double *th = thetas + (size_t)c * P.N;
                     const size_t str = (size_t)(1 << LS) * P.Estr;
                     const CT *qb = Qt + ((size_t)c * FF::N + j) * P.Estr;
                     const CT *rb = P.rt_from_q ? qb : Rt + ((size_t)c * FF::N + j) * P.Estr;
                     const long rc = P.rt_from_q ? (rv >= 0 ? (long)rv : ec) : ec;
                     const double th_r = th[er_], th_s = th[es_];
-                    CT qv[8], rw[8];
+                    // AUTO (R = conj(Q): rho = (D + D^H) / 2 and no element has a reverse): the
+                    // R sequence is conj of the twiddled Q sequence, so only Q is loaded
+                    CT qv[8], rw[AUTO ? 1 : 8];
 #pragma unroll
                     for (int m = 0; m < 8; m++) qv[m] = qb[(size_t)m * str + ec];
-                    // R = conj(Q) alone when no element of the plan has a reverse element: its
-                    // loads are skipped (a uniform branch on the plan)
-                    if (!P.rt_from_q || P.rt_rev) {
+                    if constexpr (!AUTO) {
 #pragma unroll
                         for (int m = 0; m < 8; m++) rw[m] = rb[(size_t)m * str + rc];
-                    } else {
-#pragma unroll
-                        for (int m = 0; m < 8; m++) rw[m] = Arith<CT>::zero();
                     }
                     const double ph = ok ? (th_r - th_s) / (double)FF::N : 0.0;
                     if (j == 0) {
@@ -1678,31 +1953,33 @@ __global__ void __launch_bounds__(256, (LN > 0 && sizeof(R) == 4 && NPT == 4) ? 
                         if (k) x[k] = cmul(x[k], FF::template twid<-1>(s_tw, j * k));
                         sQ[base + (k << LS) * FF::ROW] = x[k];
                     }
-                    w = cconj(w0);
-                    const cd sti = cconj(st);
+                    if constexpr (!AUTO) {
+                        w = cconj(w0);
+                        const cd sti = cconj(st);
 #pragma unroll
-                    for (int m = 0; m < 8; m++) {
-                        cd rr;
-                        if (P.rt_from_q) // R(s,r) = Q_(J,I)(s,r) + conj(Q_(I,J)(r,s))
-                            rr = cadd(cscale(cconj(to_cd(qv[m])), okf), cscale(to_cd(rw[m]), rvf));
-                        else
-                            rr = cscale(to_cd(rw[m]), okf);
-                        x[m] = CX<R>::from(cmul(rr, w));
-                        w = cmul(w, sti);
-                    }
-                    RegDFT<8, +1, R>::run(x);
+                        for (int m = 0; m < 8; m++) {
+                            cd rr;
+                            if (P.rt_from_q) // R(s,r) = Q_(J,I)(s,r) + conj(Q_(I,J)(r,s))
+                                rr = cadd(cscale(cconj(to_cd(qv[m])), okf), cscale(to_cd(rw[m]), rvf));
+                            else
+                                rr = cscale(to_cd(rw[m]), okf);
+                            x[m] = CX<R>::from(cmul(rr, w));
+                            w = cmul(w, sti);
+                        }
+                        RegDFT<8, +1, R>::run(x);
 #pragma unroll
-                    for (int k = 0; k < 8; k++) {
-                        if (k) x[k] = cmul(x[k], FF::template twid<+1>(s_tw, j * k));
-                        sR[base + (k << LS) * FF::ROW] = x[k];
+                        for (int k = 0; k < 8; k++) {
+                            if (k) x[k] = cmul(x[k], FF::template twid<+1>(s_tw, j * k));
+                            sR[base + (k << LS) * FF::ROW] = x[k];
+                        }
                     }
                 }
                 __syncthreads();
                 if constexpr (FF::NP == 3) {
-                    corr_fwd_pass<8, LN, LN - 3, R>(sQ, sR, s_tw);
+                    corr_fwd_pass<8, LN, LN - 3, R, AUTO>(sQ, sR, s_tw);
                     __syncthreads();
                 }
-                corr_mid_pass<LN, R>(sQ, sR);
+                corr_mid_pass<LN, R, AUTO>(sQ, sR);
                 __syncthreads();
                 if constexpr (FF::NP == 3) {
                     corr_inv_pass<8, LN, LN - 3, R>(sQ, s_tw);
@@ -2083,6 +2360,25 @@ extern "C" int blfid_dev_launch_exp(int prec, int M, const DevPlan *P, int Kb, i
     return (int)hipGetLastError();
 }
 
+// Fused propagator + Dyson chain (complex64 gamma-COMPUTE, blocks of at most 80 states).
+extern "C" int blfid_dev_launch_expdyson(int M, const DevPlan *P, int Kb, int c0, const int *grp,
+                                         int ngrp, void *bufA, hipStream_t st) {
+    const long ntask = (long)Kb * ngrp;
+    const long cap = M == 1 ? 4 * GRID_CAP : GRID_CAP;
+    const int grid = (int)(ntask < cap ? ntask : cap);
+    if (grid <= 0) return 0;
+    cf *A = (cf *)bufA;
+    const size_t tb = sizeof(double) * DEV_TERM_STRIDE * (size_t)P->nterms; // staged terms
+    switch (M) {
+    case 1: k_expdyson<1><<<grid, 64, tb, st>>>(*P, Kb, c0, grp, ngrp, A); break;
+    case 2: k_expdyson<2><<<grid, 256, tb, st>>>(*P, Kb, c0, grp, ngrp, A); break;
+    case 3: k_expdyson<3><<<grid, 256, tb, st>>>(*P, Kb, c0, grp, ngrp, A); break;
+    case 4: k_expdyson<4><<<grid, 256, tb, st>>>(*P, Kb, c0, grp, ngrp, A); break;
+    default: k_expdyson<5><<<grid, 256, tb, st>>>(*P, Kb, c0, grp, ngrp, A); break;
+    }
+    return (int)hipGetLastError();
+}
+
 template <typename R>
 static void launch_dyson(int M, const DevPlan *P, int Kb, const int *grp, int ngrp, int grid,
                          const void *bufU, void *bufA, hipStream_t st) {
@@ -2198,13 +2494,18 @@ extern "C" int blfid_dev_launch_qr(int prec, int M, const DevPlan *P, int Kb, co
     return (int)hipGetLastError();
 }
 
+// AUTO variant (FFT sizes only) when R = conj(Q): rosym with no reverse elements
 template <typename R, int LN, int NPT>
 static void launch_corr_t(int grid, const DevPlan *P, int Kb, const void *Qt, const void *Rt,
                           const double *thetas, const int *er, const int *es, double *part,
                           hipStream_t st) {
     typedef typename CX<R>::T CT;
-    k_corr<R, LN, NPT><<<grid, 256, 0, st>>>(*P, Kb, (const CT *)Qt, (const CT *)Rt, thetas, er,
-                                              es, part);
+    if (LN > 0 && P->rt_from_q && !P->rt_rev)
+        k_corr<R, LN, NPT, LN != 0><<<grid, 256, 0, st>>>(*P, Kb, (const CT *)Qt, (const CT *)Rt,
+                                                          thetas, er, es, part);
+    else
+        k_corr<R, LN, NPT, false><<<grid, 256, 0, st>>>(*P, Kb, (const CT *)Qt, (const CT *)Rt,
+                                                        thetas, er, es, part);
 }
 // FFT variants by log2 n (corr_use_fft(n): 16 <= n <= 512), LN = 0 the direct sum
 template <typename R, int NPT>

@@ -276,9 +276,10 @@ class Plan:
                 "max_block": int(out[3]), "bytes_per_crystallite": out[4],
                 "fixed_bytes": out[5]}
 
-    PHASES = ("exp", "dyson", "eig", "bmul", "qr", "corr", "reduce", "static")
+    PHASES = ("exp", "dyson", "eig", "bmul", "qr", "corr", "reduce", "static", "expdyson")
     KERNELS = ("k_exp", "k_dyson", "k_eig", "k_bmul", "k_qr", "k_corr", "k_reduce",
-               "k_static_fid")
+               "k_static_fid", "k_expdyson")
+    NSLOT = 72  # BLFID_NSLOT
 
     def set_profiling(self, on: bool = True):
         _check(lib().blfid_plan_set_profiling(self.h, int(bool(on))))
@@ -286,14 +287,14 @@ class Plan:
     def phase_times(self) -> dict:
         """{kernel name: {ms, launches, flops, bytes}} for every slot that ran, e.g.
         'k_exp<5>' (bytes: algorithmic HBM bytes)."""
-        ms = np.zeros(64)
-        n = np.zeros(64, dtype=np.int32)
-        fl = np.zeros(64)
+        ms = np.zeros(self.NSLOT)
+        n = np.zeros(self.NSLOT, dtype=np.int32)
+        fl = np.zeros(self.NSLOT)
         _check(lib().blfid_plan_phase_times(self.h, _p(ms), _p(n, _i), _p(fl)))
-        by = np.zeros(64)
+        by = np.zeros(self.NSLOT)
         _check(lib().blfid_plan_phase_bytes(self.h, _p(by)))
         out = {}
-        for slot in range(64):
+        for slot in range(self.NSLOT):
             if n[slot]:
                 ph, M = divmod(slot, 8)
                 if ph == 2:   # eigen-solver size classes

@@ -186,6 +186,7 @@ blfid_status blfid_plan_set_workspace(blfid_plan *plan, blfid_workspace *ws);
  * (enable before blfid_plan_run; costs a few microseconds per launch).  Phases:
  *   0 exp (H build + propagator)   1 dyson   2 period eig   3 B = A W   4 Q/R similarity
  *   5 correlation + FID            6 powder reduce            7 static FID
+ *   8 fused propagator + Dyson chain (complex64 gamma-COMPUTE plans, blocks of <= 80 states)
  * Templated kernels come in tile variants M = 1..5 (16M x 16M output tiles); for the period
  * eigen-solver (phase 2) M - 1 is its block-size class (b <= 16, 32, 48, 64, 80, larger).
  * blfid_plan_phase_times waits for the last run and returns, per slot phase*8 + M (M = 0 for
@@ -193,8 +194,8 @@ blfid_status blfid_plan_set_workspace(blfid_plan *plan, blfid_workspace *ws);
  * algorithmic flops (unpadded block sizes; the propagator's as executed, its Taylor degree being
  * chosen per task on the device; the correlation's as FFT or direct sum, as run), accumulated
  * since profiling was enabled.  Arrays hold BLFID_NSLOT entries. */
-#define BLFID_NPHASE 8
-#define BLFID_NSLOT 64
+#define BLFID_NPHASE 9
+#define BLFID_NSLOT 72
 blfid_status blfid_plan_set_profiling(blfid_plan *plan, int on);
 blfid_status blfid_plan_phase_times(blfid_plan *plan, double *ms, int *launches, double *flops);
 /* Algorithmic HBM bytes per slot (same slots, accumulated likewise): what each kernel must read

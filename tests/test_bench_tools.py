@@ -73,6 +73,11 @@ def test_corr_flop_count():
     # not a power of two: the direct cyclic sum
     assert bench.corr_flops_per_elem(96, P) == 8 * 96 * 96 + 18 * 96 + 14 * P
     assert np.isclose(bench.corr_flops_per_elem(1024, P), 8 * 1024.0 ** 2 + 18 * 1024 + 14 * P)
+    # autocorrelation (R = conj(Q)): two transforms, |q|^2 / n, Q staging alone
+    auto = 2 * (2 * 16 * (56 + 42) + 64 * 4) + 4 * n
+    assert bench.corr_flops_per_elem(n, P, True) == auto + 9 * n + 14 * P
+    # the direct sum is the same either way
+    assert bench.corr_flops_per_elem(96, P, True) == bench.corr_flops_per_elem(96, P)
 
 
 def test_bench_refuses_more_than_one_gpu():
