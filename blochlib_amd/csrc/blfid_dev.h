@@ -52,6 +52,12 @@ struct DevPlan {
     long Estr;             // Q / R row stride per step: Etot padded to a multiple of 16 elements
     int nparts;            // FID partial sums per crystallite (correlation segments or block pairs)
     int rt_rev;            // rt_from_q and some element has a reverse element (elem_rev >= 0)
+    const int *xd_off;     // nblocks + 1: start of each block's flip-flop positions in xd_list
+    const int *xd_list;    // per block of <= DEV_LDS_BLOCK states: (r LDK + q) << 8 | (i S + j)
+    const int *elem_r;     // Etot: state-list position of each pair element's column (block I)
+    const int *elem_s;     // Etot: state-list position of each pair element's row (block J)
+    const void *rho_c;     // prepared rho per crystallite of the batch on the block pairs (storage
+                           // precision, conj(rho_JI) at element r + s b_I), or NULL
 };
 
 // Correlation work split (k_corr), shared by the host planner and the kernel: a crystallite's

@@ -41,6 +41,10 @@ int blfid_dev_launch_dyson(int prec, int M, const DevPlan *P, int Kb, const int 
                            const void *bufU, void *bufA, hipStream_t st);
 int blfid_dev_launch_expdyson(int M, const DevPlan *P, int Kb, int c0, const int *grp, int ngrp,
                               void *bufA, hipStream_t st);
+int blfid_dev_launch_rho_prep(int prec, const DevPlan *P, const DevPlan *PP, int Kb,
+                              const double2 *Aprep, const int *sblk, const int *sloc,
+                              const int *nzi, const double *nzv, int nnz, void *rho_out,
+                              hipStream_t st);
 int blfid_dev_launch_eig(int prec, const DevPlan *P, int Kb, const int *grp, int ngrp,
                          const void *src, size_t src_stride, int mode, double2 *bufW,
                          void *bufWr, double *outv, int cls, double2 *gscr, size_t gslot,
@@ -195,6 +199,9 @@ struct blfid_plan {
     std::map<int, std::vector<int>> blk_groups; // M -> blocks
     std::map<int, std::vector<int>> fuse_groups; // M -> blocks of <= DEV_LDS_BLOCK states
     std::vector<int> big_blocks;                 // blocks beyond DEV_LDS_BLOCK states
+    // fused kernel: per block of <= DEV_LDS_BLOCK states, its flip-flop positions in the LDS plane
+    // of its tile variant M, (r LDK + q) << 8 | (i S + j), LDK = 16 M + 4 (CSR over blocks)
+    std::vector<int> xd_off, xd_list;
     std::map<int, std::vector<int>> eig_cls;    // eigen-solver size class -> blocks
     int exp_slots = 0, qr_slots = 0, eig_slots = 0;
     // operators
@@ -223,10 +230,19 @@ struct blfid_plan {
     DevBuf<long> d_blk_uoff, d_pair_eoff;
     DevBuf<double> d_dval, d_rval;
     std::map<int, DevBuf<int>> d_bgrp, d_pgrp, d_fgrp;
-    DevBuf<int> d_bigb;
+    DevBuf<int> d_bigb, d_xd_off, d_xd_list;
     std::map<int, DevBuf<int>> d_eig_cls;
     DevBuf<int> d_spin_rf;
     DevBuf<double> d_rf, d_thetak;
+    // pulses applied to rho before the gamma-COMPUTE FID (blfid_plan_set_prep): a pulse plan for
+    // the preparation propagator of each crystallite, its state -> (block, local index) maps, and
+    // the nonzero entries (a, b, re, im) of rho, from which the prepared blocks are formed
+    blfid_plan *prep = nullptr;
+    double prep_rotor = 0;
+    DevBuf<int> d_prep_sblk, d_prep_sloc, d_rho_nzi;
+    DevBuf<double> d_rho_nzv;
+    int rho_nnz = 0;
+    char *bufRho = nullptr;
     // working storage (carved from the plan's own arena or a shared workspace)
     DevBuf<char> own_ws;
     blfid_workspace *ws = nullptr;
@@ -251,6 +267,7 @@ struct blfid_plan {
     int ph_n[BLFID_NSLOT] = {0};
     ~blfid_plan() {
         for (auto &e : pending) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
+        delete prep;
     }
 };
 
@@ -478,6 +495,27 @@ static blfid_status build_tables(blfid_plan *pl) {
         else pl->big_blocks.push_back((int)b);
     }
     pl->Lsum = uo;
+    // flip-flop positions of the fused kernel's H planes
+    pl->xd_off.assign(1, 0);
+    pl->xd_list.clear();
+    for (size_t b = 0; b < pl->blocks.size(); b++) {
+        const auto &st = pl->blocks[b].states;
+        const int d = (int)st.size();
+        if (d <= DEV_LDS_BLOCK) {
+            const int LDK = 16 * ((d + 15) / 16) + 4;
+            for (int r = 0; r < d; r++)
+                for (int q = 0; q < d; q++) {
+                    const int x = st[r] ^ st[q];
+                    if (__builtin_popcount((unsigned)x) != 2 || (st[r] & x) == 0 || (st[r] & x) == x)
+                        continue;
+                    const int p1 = 31 - __builtin_clz((unsigned)x), p0 = __builtin_ctz((unsigned)x);
+                    const int i = S - 1 - p1, j = S - 1 - p0;
+                    if (!ffpair[i][j]) continue;
+                    pl->xd_list.push_back(((r * LDK + q) << 8) | (i * S + j));
+                }
+        }
+        pl->xd_off.push_back((int)pl->xd_list.size());
+    }
     pl->eig_cls.clear();
     for (size_t b = 0; b < pl->blocks.size(); b++)
         pl->eig_cls[eig_class(pl->blk_dim[b])].push_back((int)b);
@@ -541,6 +579,8 @@ static blfid_status upload_tables(blfid_plan *pl) {
     for (auto &kv : pl->blk_groups) HIPCHK(pl->d_bgrp[kv.first].upload(kv.second));
     for (auto &kv : pl->fuse_groups) HIPCHK(pl->d_fgrp[kv.first].upload(kv.second));
     HIPCHK(pl->d_bigb.upload(pl->big_blocks));
+    HIPCHK(pl->d_xd_off.upload(pl->xd_off));
+    HIPCHK(pl->d_xd_list.upload(pl->xd_list));
     for (auto &kv : pl->eig_cls) HIPCHK(pl->d_eig_cls[kv.first].upload(kv.second));
     HIPCHK(pl->d_spin_rf.upload(pl->spin_rf));
     HIPCHK(pl->d_rf.upload(pl->rf));
@@ -599,6 +639,11 @@ static void refresh_devplan(blfid_plan *pl) {
     d.spin_rf = pl->d_spin_rf.p;
     d.rf = pl->d_rf.p;
     d.thetak = pl->step_dt.empty() ? nullptr : pl->d_thetak.p;
+    d.xd_off = pl->d_xd_off.p;
+    d.xd_list = pl->d_xd_list.p;
+    d.rho_c = pl->bufRho;
+    d.elem_r = pl->d_er.p;
+    d.elem_s = pl->d_es.p;
 }
 
 // ------------------------------------------------------------------------------ C ABI
@@ -846,6 +891,44 @@ blfid_status blfid_plan_create_pulse(const blfid_system *sys, double wr_hz, doub
                              prec, out);
 }
 
+blfid_status blfid_plan_set_prep(blfid_plan *pl, double rotor_deg, int nsteps,
+                                 const double *step_t1, const double *step_dt, int nrf,
+                                 const blfid_rf *rf) {
+    if (!pl) return fail(BLFID_ERR_INVALID, "plan is NULL");
+    if (nsteps < 0) return fail(BLFID_ERR_INVALID, "negative step count");
+    if (nsteps > 0 && !step_dt) return fail(BLFID_ERR_INVALID, "step_dt is NULL");
+    if (nsteps > 0 && (!pl->spinning || pl->ptop))
+        return fail(BLFID_ERR_UNSUPPORTED,
+                    "pulses before the FID are implemented for the spinning gamma-COMPUTE FID");
+    if (pl->last_stream) (void)hipStreamSynchronize(pl->last_stream);
+    blfid_plan *pp = nullptr;
+    if (nsteps > 0) {
+        const blfid_system sys{pl->S, pl->iso.data(), (int)(pl->csa.size() / 7), pl->csa.data(),
+                               (int)(pl->dip.size() / 6), pl->dip.data(), (int)(pl->jc.size() / 4),
+                               pl->jc.data()};
+        blfid_status s = create_pulse_plan(&sys, pl->wr, rotor_deg, nsteps, step_t1, step_dt, 1,
+                                           0.0, nrf, rf, BLFID_FP64, &pp);
+        if (s != BLFID_OK) return s;
+        std::vector<int> sblk(pl->N), sloc(pl->N);
+        for (size_t b = 0; b < pp->blocks.size(); b++)
+            for (size_t a = 0; a < pp->blocks[b].states.size(); a++) {
+                sblk[pp->blocks[b].states[a]] = (int)b;
+                sloc[pp->blocks[b].states[a]] = (int)a;
+            }
+        hipError_t e = pl->d_prep_sblk.upload(sblk);
+        if (e == hipSuccess) e = pl->d_prep_sloc.upload(sloc);
+        if (e != hipSuccess) {
+            delete pp;
+            return fail(BLFID_ERR_DEVICE, std::string("prep tables: ") + hipGetErrorString(e));
+        }
+    }
+    delete pl->prep;
+    pl->prep = pp;
+    pl->prep_rotor = rotor_deg;
+    pl->have_ops = false; // the block-pair tables depend on it: operators are set again
+    return BLFID_OK;
+}
+
 void blfid_plan_destroy(blfid_plan *pl) {
     if (!pl) return;
     if (pl->last_stream) (void)hipStreamSynchronize(pl->last_stream);
@@ -876,6 +959,10 @@ blfid_status blfid_plan_set_operators(blfid_plan *pl, const double *rho_d, const
                 rosym = 0;
                 break;
             }
+    // with pulses before the FID every crystallite has its own prepared rho, which couples any
+    // pair of blocks: the general (non-rosym) form, on every block pair that D couples
+    const bool prepared = pl->prep != nullptr;
+    if (prepared) rosym = 0;
     pl->rosym = rosym;
     cplx tr = 0;
     for (int i = 0; i < N; i++)
@@ -899,7 +986,7 @@ blfid_status blfid_plan_set_operators(blfid_plan *pl, const double *rho_d, const
     for (int i = 0; i < N; i++)
         for (int j = 0; j < N; j++) {
             if (D(i, j) != 0.0) dnz[sblk[i] * nb + sblk[j]] = 1;
-            if (RE(i, j) != 0.0) rnz[sblk[i] * nb + sblk[j]] = 1;
+            if (RE(i, j) != 0.0 || prepared) rnz[sblk[i] * nb + sblk[j]] = 1;
         }
     pl->pairs.clear();
     for (int I = 0; I < nb; I++)
@@ -979,6 +1066,22 @@ blfid_status blfid_plan_set_operators(blfid_plan *pl, const double *rho_d, const
     }
     pl->rt_rev = false;
     for (int v : pl->elem_rev) pl->rt_rev |= v >= 0;
+    if (prepared) {
+        // rho's nonzero entries, from which each crystallite's prepared blocks are formed
+        std::vector<int> nzi;
+        std::vector<double> nzv;
+        for (int j = 0; j < N; j++)
+            for (int i = 0; i < N; i++)
+                if (R(i, j) != 0.0) {
+                    nzi.push_back(i);
+                    nzi.push_back(j);
+                    nzv.push_back(R(i, j).real());
+                    nzv.push_back(R(i, j).imag());
+                }
+        pl->rho_nnz = (int)(nzv.size() / 2);
+        HIPCHK(pl->d_rho_nzi.upload(nzi));
+        HIPCHK(pl->d_rho_nzv.upload(nzv));
+    }
     std::vector<int> pI, pJ;
     for (auto &pp : pl->pairs) {
         pI.push_back(pp.I);
@@ -1073,6 +1176,7 @@ static double pair_flops(const blfid_plan *pl, const std::vector<int> &g) {
     for (int p : g) {
         double bI = pl->blk_dim[pl->pairs[p].I], bJ = pl->blk_dim[pl->pairs[p].J];
         s += 8.0 * (bI * bI * bJ + (pl->rt_from_q ? 0.0 : bJ * bJ * bI));  // Q (and R)
+        if (pl->prep) s += 8.0 * bJ * bI * bI;                              // rho_c B_I
     }
     return s;
 }
@@ -1119,11 +1223,32 @@ static blfid_layout::Shape shape_of(const blfid_plan *pl) {
     s.Lsum = pl->Lsum;
     s.Estr = pl->Estr;
     s.K = pl->K;
+    s.rho_elems = pl->prep ? pl->Etot : 0;
     return s;
 }
 
+// Layout of a preparation plan inside its FID plan's arena: only its propagators run there.
+static blfid_layout::Shape prep_shape_of(const blfid_plan *pp) {
+    blfid_layout::Shape s = shape_of(pp);
+    s.qr_slots = 0;
+    s.eig_slots = 0;
+    s.nparts = 0;
+    s.K = 0;
+    return s;
+}
+
+// The plan's layout with its preparation's (ps, referenced by s.prep) when it has one.
+static void full_shape(const blfid_plan *pl, blfid_layout::Shape &s, blfid_layout::Shape &ps) {
+    s = shape_of(pl);
+    if (pl->prep) {
+        ps = prep_shape_of(pl->prep);
+        s.prep = &ps;
+    }
+}
+
 static double bytes_per_crystallite(const blfid_plan *pl) {
-    const blfid_layout::Shape sh = shape_of(pl);
+    blfid_layout::Shape sh, ps;
+    full_shape(pl, sh, ps);
     return (double)(blfid_layout::layout_bytes(sh, 1) - blfid_layout::layout_bytes(sh, 0));
 }
 
@@ -1157,7 +1282,9 @@ blfid_status blfid_plan_stats(const blfid_plan *pl, double *out) {
     out[2] = (double)pl->blocks.size();
     out[3] = (double)pl->bmax;
     out[4] = bytes_per_crystallite(pl);
-    out[5] = (double)blfid_layout::layout_bytes(shape_of(pl), 0);
+    blfid_layout::Shape sh, ps;
+    full_shape(pl, sh, ps);
+    out[5] = (double)blfid_layout::layout_bytes(sh, 0);
     return BLFID_OK;
 }
 
@@ -1175,6 +1302,7 @@ static void carve(blfid_plan *pl, char *base, const size_t *off) {
     pl->scrEig = (double2 *)(base + off[R_EIG]);
     pl->bufWr = pl->prec && pl->spinning ? base + off[R_WR] : nullptr;
     pl->grp = (double *)(base + off[R_GROUP]);
+    pl->bufRho = pl->prep ? base + off[R_RHO] : nullptr;
 }
 
 // Enqueues the whole powder on `st` with the working storage at `base` (regions at off[],
@@ -1241,6 +1369,38 @@ static blfid_status enqueue_run(blfid_plan *pl, const blfid_layout::Shape &sh, c
                                                       sh.qr_slots, 1.0, 0, pl->part, st));
             }
         } else if (pl->spinning) {
+            if (pl->prep) {
+                // pulses before the FID: the preparation propagator of each crystallite (FP64,
+                // in the preparation plan's blocks, its storage inside this arena), then the
+                // prepared rho on the FID plan's block pairs
+                blfid_plan *pp = pl->prep;
+                size_t poff[blfid_layout::R_COUNT];
+                (void)blfid_layout::layout_bytes(*sh.prep, Kb, poff);
+                carve(pp, base + off[blfid_layout::R_PREP], poff);
+                refresh_devplan(pp);
+                pp->dp.RP = pl->d_RP.p;
+                const DevPlan *PP = &pp->dp;
+                const size_t pbm2 = (size_t)pp->bmax * pp->bmax;
+                for (auto &kv : pp->blk_groups) {
+                    PhaseScope ps(pl, 0 * 8 + kv.first, 0.0,
+                                  kb * (double)pp->n * group_b2(pp, kv.second) * 16.0, st);
+                    LAUNCHCHK(blfid_dev_launch_exp(0, kv.first, PP, kb, c0, pp->d_bgrp[kv.first].p,
+                                                   (int)kv.second.size(), pp->scrE, pbm2,
+                                                   sh.prep->exp_slots, pp->bufU, st));
+                }
+                for (auto &kv : pp->blk_groups) {
+                    PhaseScope ps(pl, 1 * 8 + kv.first,
+                                  kb * (double)(pp->n - 1) * group_b3(pp, kv.second) * 8.0,
+                                  kb * (double)(pp->n + 1) * group_b2(pp, kv.second) * 16.0, st);
+                    LAUNCHCHK(blfid_dev_launch_dyson(0, kv.first, PP, kb, pp->d_bgrp[kv.first].p,
+                                                     (int)kv.second.size(), pp->bufU, pp->bufA, st));
+                }
+                LAUNCHCHK(blfid_dev_launch_rho_prep(
+                    pr, P, PP, kb,
+                    (const double2 *)(pp->bufA + (size_t)((pp->n - 1) % 2) * pp->Lsum * 16),
+                    pl->d_prep_sblk.p, pl->d_prep_sloc.p, pl->d_rho_nzi.p, pl->d_rho_nzv.p,
+                    pl->rho_nnz, pl->bufRho, st));
+            }
             // complex64 matrix-core plans: blocks of <= 80 states take the fused propagator +
             // Dyson chain (U_k never leaves LDS); larger blocks, the FP64 form and the BF16x3
             // form take the propagator and the Dyson chain as two kernels
@@ -1353,7 +1513,8 @@ blfid_status blfid_plan_run(blfid_plan *pl, int normalize, void *stream) {
     if (normalize && pl->trdd == 0.0) return fail(BLFID_ERR_INVALID, "Tr(det det^H) is zero");
     hipStream_t st = (hipStream_t)stream;
     pl->last_stream = st;
-    const blfid_layout::Shape sh = shape_of(pl);
+    blfid_layout::Shape sh, psh;
+    full_shape(pl, sh, psh);
     // batch size: the largest whose exact layout fits the workspace / memory budget
     double avail;
     if (pl->ws) {
@@ -1423,11 +1584,8 @@ static blfid_status noalloc_scratch(char **out) {
     return BLFID_OK;
 }
 
-// The plan's layout with its scratch-slot counts cut so that each slot region takes at most an
-// eighth of the static scratch (fewer workgroups in flight for the propagator, Q/R and large-
-// block eigen kernels; results are unchanged, every task still runs).
-static blfid_layout::Shape noalloc_shape(const blfid_plan *pl) {
-    blfid_layout::Shape sh = shape_of(pl);
+// Scratch-slot counts of one layout cut to an eighth of the static scratch each.
+static void noalloc_cap(blfid_layout::Shape &sh, const blfid_plan *pl) {
     const double part = (double)BLFID_NOALLOC_SCRATCH_BYTES / 8.0;
     const double bm2 = (double)pl->bmax * pl->bmax;
     const double rs = pl->prec ? 4.0 : 8.0;
@@ -1436,9 +1594,18 @@ static blfid_layout::Shape noalloc_shape(const blfid_plan *pl) {
         return std::max(1, std::min(slots, (int)std::floor(part / per)));
     };
     sh.exp_slots = cap(sh.exp_slots, e1);
-    sh.qr_slots = cap(sh.qr_slots, 3.0 * bm2 * 16.0);
+    if (sh.qr_slots > 0) sh.qr_slots = cap(sh.qr_slots, 3.0 * bm2 * 16.0);
     if (sh.eig_slots > 0) sh.eig_slots = cap(sh.eig_slots, bm2 * 16.0);
-    return sh;
+}
+
+// The plan's layout (and its preparation's, in ps) with the scratch-slot counts cut so that
+// each slot region takes at most an eighth of the static scratch (fewer workgroups in flight
+// for the propagator, Q/R and large-block eigen kernels; results are unchanged, every task
+// still runs).
+static void noalloc_shape(const blfid_plan *pl, blfid_layout::Shape &sh, blfid_layout::Shape &ps) {
+    full_shape(pl, sh, ps);
+    noalloc_cap(sh, pl);
+    if (pl->prep) noalloc_cap(ps, pl->prep);
 }
 
 blfid_status blfid_plan_run_noalloc(blfid_plan *pl, int normalize, double *fid_out_dev,
@@ -1449,7 +1616,8 @@ blfid_status blfid_plan_run_noalloc(blfid_plan *pl, int normalize, double *fid_o
     char *base = nullptr;
     blfid_status s = noalloc_scratch(&base);
     if (s != BLFID_OK) return s;
-    const blfid_layout::Shape sh = noalloc_shape(pl);
+    blfid_layout::Shape sh, psh;
+    noalloc_shape(pl, sh, psh);
     const long Kb = blfid_layout::choose_batch(sh, pl->K, (double)BLFID_NOALLOC_SCRATCH_BYTES);
     if (Kb < 1)
         return fail(BLFID_ERR_NOMEM,

@@ -979,28 +979,119 @@ __device__ __forceinline__ void sym_store(float *PL, const v4f (&acc)[NS], float
     }
 }
 
+// Hamiltonian of one step for the fused kernel, in three phases separated by barriers:
+//   A: v = R_P^T u (every thread), zz_t = v^T C_t v for every anisotropic term and the flip-flop
+//      coefficient of every ordered spin pair (its terms' zz recomputed in place, so A has no
+//      internal dependency);
+//   B: diagonal of each row (terms split over lanes, combined by shuffles) and its row sum of
+//      absolute values (diagonal plus every flip-flop partner), the block's infinity norm per wave;
+//   C: X = theta H / 2^s rounded once from FP64 -- its diagonal and the block's fixed flip-flop
+//      positions (xd_list) -- scattered into the plane, s the least with ||X||_inf <= rmax.
+// Every other entry of the plane is zero and stays zero.  Returns ||H||_inf and s.  Same
+// values as build_h_block (sys.cc:749-770; csa.cc:289-329, dip.cc:171-207).
+template <int NT>
+__device__ double xd_build_h(const DevPlan &P, const double *terms, const double *RPc,
+                             const double *u, int blk, float *PX, int LDK, double *s_zz,
+                             double *s_ff, double *s_diag, double *s_red, double theta,
+                             double rmax_x, int &sq) {
+    const int tid = threadIdx.x, S = P.S, nt = P.nterms;
+    const double vx = RPc[0] * u[0] + RPc[3] * u[1] + RPc[6] * u[2];
+    const double vy = RPc[1] * u[0] + RPc[4] * u[1] + RPc[7] * u[2];
+    const double vz = RPc[2] * u[0] + RPc[5] * u[1] + RPc[8] * u[2];
+    auto zzf = [&](int t) {
+        const double *c = terms + DEV_TERM_STRIDE * t;
+        return c[0] * vx * vx + c[1] * vy * vy + c[2] * vz * vz +
+               2.0 * (c[3] * vx * vy + c[4] * vx * vz + c[5] * vy * vz);
+    };
+    for (int x = tid; x < nt + S * S; x += NT) {
+        if (x < nt) {
+            s_zz[x] = zzf(x);
+        } else {
+            const int q = x - nt;
+            double f = P.ffconst[q];
+            for (int z = P.ff_off[q]; z < P.ff_off[q + 1]; z++) {
+                const int t = P.ff_term[z];
+                f += zzf(t) * terms[DEV_TERM_STRIDE * t + 7];
+            }
+            s_ff[q] = f;
+        }
+    }
+    __syncthreads();
+    const int b = P.blk_dim[blk];
+    const int *st = P.blk_states + P.blk_soff[blk];
+    int tpr = 1;
+    while (tpr < 16 && tpr * 2 * b <= NT) tpr *= 2;
+    double rmax = 0.0;
+    for (int r0 = 0; r0 < b; r0 += NT / tpr) {
+        const int r = r0 + tid / tpr, part = tid % tpr;
+        double h = 0.0;
+        int ga = 0;
+        if (r < b) {
+            ga = st[r];
+            for (int t = part; t < nt; t += tpr) {
+                const double *c = terms + DEV_TERM_STRIDE * t;
+                const int i = (int)c[8], j = (int)c[9];
+                double f = c[6] * (((ga >> (S - 1 - i)) & 1) ? -0.5 : 0.5);
+                if (j >= 0) f *= ((ga >> (S - 1 - j)) & 1) ? -0.5 : 0.5;
+                h += s_zz[t] * f;
+            }
+        }
+        for (int off = tpr / 2; off > 0; off >>= 1) h += __shfl_xor(h, off);
+        if (r < b && part == 0) {
+            h += P.dconst[ga];
+            s_diag[r] = h;
+            double sm = fabs(h);
+            for (int i = 0; i < S; i++) {
+                const int bi = (ga >> (S - 1 - i)) & 1;
+                for (int j = i + 1; j < S; j++)
+                    if (bi != ((ga >> (S - 1 - j)) & 1)) sm += fabs(s_ff[i * S + j]);
+            }
+            rmax = fmax(rmax, sm);
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) rmax = fmax(rmax, __shfl_xor(rmax, off));
+    if ((tid & 63) == 0) s_red[tid >> 6] = rmax;
+    __syncthreads();
+    double nrm = s_red[0];
+    for (int w = 1; w < NT / 64; w++) nrm = fmax(nrm, s_red[w]);
+    sq = 0;
+    while (nrm * theta / (double)(1 << sq) > rmax_x && sq < 30) sq++;
+    const double sc = theta / (double)(1 << sq);
+    for (int r = tid; r < b; r += NT) PX[r * LDK + r] = (float)(sc * s_diag[r]);
+    const int z0 = P.xd_off[blk], z1 = P.xd_off[blk + 1];
+    for (int z = z0 + tid; z < z1; z += NT) {
+        const int e = P.xd_list[z];
+        PX[e >> 8] = (float)(sc * s_ff[e & 255]);
+    }
+    __syncthreads();
+    return nrm;
+}
+
 // Fused propagator and Dyson chain of the complex64 gamma-COMPUTE path for blocks of at most
 // 16 M <= 80 states: one task = (crystallite c, block) walks the n steps of the rotor period with
-// every matrix in LDS (five planes: X, Y, Z and the running product's re / im parts):
-//   X  = theta H_k / 2^s                      (build_h_block, H at the step midpoint, sys.cc:749-770)
+// every matrix in LDS (five planes: H, C, S and the running product's re / im parts):
+//   X = theta H_k / 2^s, H at the step midpoint (xd_build_h; sys.cc:749-770)
 //   cos X, sin X by Taylor polynomials in Y = X^2 of degree m (Paterson-Stockmeyer):
-//     m = 1:  C = I - Y/2,  P = I - Y/6                       Y                     (2 products)
-//     m = 2:  C, P elementwise in Y, Y^2                      Y, Y^2                (3 products)
-//     m = 4:  C = c0 I + c1 Y + Y^2 (c2 I + c3 Y + c4 Y^2), P likewise, the bracket formed on the
-//             fly as the B operand                            Y, Y^2, C, P          (5 products)
-//     S = X P                                                                        (+1 product)
+//     m = 1:  C = I - Y/2, P = I - Y/6 from the Y product's accumulators         (1 product)
+//     m = 2:  C, P from Y and the Y^2 product's accumulators                    (2 products)
+//     m = 4:  C = I - Y/2 + Y^2 (I/24 - Y/720 + Y^2/40320), P likewise with 1/120, -1/5040,
+//             1/362880, the bracket formed on the fly as the B operand         (4 products)
+//     S = X P                                                                  (+1 product)
 //   s squarings (C, S) <- (C^2 - S^2, 2 C S), s the least with ||X||_inf <= 0.75 (where m = 4
 //   meets the FP32 tolerance of k_exp), then
 //   A_k = (C + i usign S) A_{k-1}   (compute.h:267; usign = -1: Mexp(H, -i 2 pi tau), compute.h:265)
 // on v_mfma_f32_16x16x4f32 with A resident in LDS.  Only A_k goes to global memory (the layout
-// k_dyson writes: product k at slot k of the crystallite's n slots).  m is chosen per step from the
-// norm bound, as the smallest whose remainder bound r^(2m+2)/(2m+2)! is below 2e-8.  Replaces
-// k_exp + k_dyson for these blocks (matexp.h:239-252, compute.h:257-274).
+// k_dyson writes: product k at slot k of the crystallite's n slots).  m is chosen per step from
+// the norm bound as the smallest whose remainder bound r^(2m+2)/(2m+2)! is below 2e-8.
+// Replaces k_exp + k_dyson
+// for these blocks (matexp.h:239-252, compute.h:257-274).  Blocks of 49-80 states run eight
+// waves (two per SIMD; the LDS allows one workgroup per CU).
+template <int M> struct XdThreads { static constexpr int v = M >= 4 ? 512 : NThr<M>::v; };
 template <int M>
-__global__ void __launch_bounds__(NThr<M>::v) k_expdyson(DevPlan P, int Kb, int c0,
-                                                         const int *__restrict__ grp, int ngrp,
-                                                         cf *__restrict__ bufA) {
-    constexpr int NT = NThr<M>::v;
+__global__ void __launch_bounds__(XdThreads<M>::v) k_expdyson(DevPlan P, int Kb, int c0,
+                                                              const int *__restrict__ grp,
+                                                              int ngrp, cf *__restrict__ bufA) {
+    constexpr int NT = XdThreads<M>::v;
     typedef SymTile<M, NT> T;
     constexpr int TM = T::TM, LDK = T::LDK, NSUB = T::NSUB, NW = T::NW;
     __shared__ __align__(16) float pl[5 * T::PLANE];
@@ -1008,9 +1099,8 @@ __global__ void __launch_bounds__(NThr<M>::v) k_expdyson(DevPlan P, int Kb, int 
           *PBi = PBr + T::PLANE;
     __shared__ double s_zz[DEV_MAX_TERMS];
     __shared__ double s_ff[DEV_MAX_SPINS * DEV_MAX_SPINS];
-    __shared__ double s_v[4];
     __shared__ double s_red[NT / 64];
-    __shared__ double s_diag[DEV_MAX_BLOCK];
+    __shared__ double s_diag[DEV_LDS_BLOCK];
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63, g = lane >> 4, r = lane & 15;
     const double *terms = stage_terms<NT>(P);
     const float us = (float)P.usign;
@@ -1022,59 +1112,68 @@ __global__ void __launch_bounds__(NThr<M>::v) k_expdyson(DevPlan P, int Kb, int 
         const int blk = grp[gi], b = P.blk_dim[blk];
         const int kp = (b + 15) & ~15;
         cf *A0 = bufA + (size_t)c * P.n * P.Lsum + P.blk_uoff[blk];
-        // X's padding (rows and columns b .. 16M - 1) stays zero: the H build writes the b x b
-        // part only, and every product of zero-padded symmetric planes is zero-padded
+        // the H plane's zero entries (everything but the diagonal and the flip-flop positions,
+        // padding included) are set once per task
         for (int e = tid; e < T::PLANE; e += NT) PX[e] = 0.0f;
         __syncthreads();
         for (int k = 0; k < P.n; k++) {
             const double theta = P.theta;
             int sq = 0;
-            const double nrm =
-                build_h_block<float, NT>(P, terms, P.RP + 9 * (size_t)(c0 + c), P.u3 + 3 * k, blk,
-                                         PX, LDK, s_zz, s_ff, s_v, s_diag, s_red, nullptr,
-                                         [&](double hn) {
-                                             while (hn * theta / (double)(1 << sq) > RMAX && sq < 30) sq++;
-                                             return theta / (double)(1 << sq);
-                                         }) * theta;
-            const double rx = nrm / (double)(1 << sq);
-            const double r2 = rx * rx;
+            const double hn = xd_build_h<NT>(P, terms, P.RP + 9 * (size_t)(c0 + c), P.u3 + 3 * k,
+                                             blk, PX, LDK, s_zz, s_ff, s_diag, s_red, theta, RMAX,
+                                             sq);
+            const double rx = hn * theta / (double)(1 << sq), r2 = rx * rx;
             const int m = (r2 * r2 / 24.0 <= TOL) ? 1 : (r2 * r2 * r2 / 720.0 <= TOL) ? 2 : 4;
+            int ng = 1 + 3 * sq; // products of this step (Y, the squarings; more below)
             v4f acc[NSUB], acc2[NSUB];
             // Y = X X
             sym_zero(acc);
             sym_mfma<M, NT, false>(PX, PX, nullptr, nullptr, 1.0f, 0.f, 0.f, 0.f, b, kp, acc);
-            sym_store<M, NT>(PY, acc, 1.0f);
-            __syncthreads();
-            int ng = 2 + 3 * sq;
-            if (m >= 2) {
-                // Z = Y Y
-                sym_zero(acc);
-                sym_mfma<M, NT, false>(PY, PY, nullptr, nullptr, 1.0f, 0.f, 0.f, 0.f, b, kp, acc);
-                sym_store<M, NT>(PZ, acc, 1.0f);
-                __syncthreads();
-                ng += 1;
-            }
-            if (m <= 2) {
-                // C = I - Y/2 + Y^2/24, P = I - Y/6 + Y^2/120 (no Y^2 term for m = 1), into Y, Z
-                const float z2c = m == 2 ? (float)(1.0 / 24.0) : 0.0f;
-                const float z2s = m == 2 ? (float)(1.0 / 120.0) : 0.0f;
-                for (int e = tid; e < TM * TM; e += NT) {
-                    const int rr = e / TM, qq = e - rr * TM, o = rr * LDK + qq;
-                    const float d = (rr == qq && rr < b) ? 1.0f : 0.0f;
-                    const float y = PY[o], z = m == 2 ? PZ[o] : 0.0f;
-                    PY[o] = d - 0.5f * y + z2c * z;
-                    PZ[o] = d - (float)(1.0 / 6.0) * y + z2s * z;
+            if (m == 1) {
+                // C = I - Y/2 into Y's plane, P = I - Y/6 into Z's (no operand of this product)
+#pragma unroll
+                for (int s = 0; s < NSUB; s++) {
+                    const int t = w + NW * s;
+                    if (t < M * M) {
+                        const int col = 16 * (t % M) + r, row0 = 16 * (t / M) + 4 * g;
+                        v4f cv, pv;
+#pragma unroll
+                        for (int j = 0; j < 4; j++) {
+                            const float d = (row0 + j == col && col < b) ? 1.0f : 0.0f;
+                            cv[j] = d - 0.5f * acc[s][j];
+                            pv[j] = d - (float)(1.0 / 6.0) * acc[s][j];
+                        }
+                        *(v4f *)&PY[col * LDK + row0] = cv;
+                        *(v4f *)&PZ[col * LDK + row0] = pv;
+                    }
                 }
                 __syncthreads();
             } else {
-                // C = I - Y/2 + Y^2 (1/24 I - Y/720 + Y^2/40320), P with 1/120, -1/5040, 1/362880
-                sym_zero(acc);
-                sym_zero(acc2);
-                sym_mfma<M, NT, true>(PZ, nullptr, PY, PZ, 1.0f, (float)(1.0 / 24.0),
-                                      (float)(-1.0 / 720.0), (float)(1.0 / 40320.0), b, kp, acc);
-                sym_mfma<M, NT, true>(PZ, nullptr, PY, PZ, 1.0f, (float)(1.0 / 120.0),
-                                      (float)(-1.0 / 5040.0), (float)(1.0 / 362880.0), b, kp, acc2);
-                __syncthreads(); // every read of Y and Z done
+                sym_store<M, NT>(PY, acc, 1.0f);
+                __syncthreads();
+                if (m == 2) {
+                    // Y^2; then C = I - Y/2 + Y^2/24 into Y's plane, P = I - Y/6 + Y^2/120 into Z's
+                    sym_zero(acc);
+                    sym_mfma<M, NT, false>(PY, PY, nullptr, nullptr, 1.0f, 0.f, 0.f, 0.f, b, kp, acc);
+                    __syncthreads(); // every read of Y done
+                    ng += 1;
+                } else {
+                    // Z = Y Y, then C and P with the bracket formed from Y and Z on the fly
+                    sym_zero(acc);
+                    sym_mfma<M, NT, false>(PY, PY, nullptr, nullptr, 1.0f, 0.f, 0.f, 0.f, b, kp, acc);
+                    sym_store<M, NT>(PZ, acc, 1.0f);
+                    __syncthreads();
+                    sym_zero(acc);
+                    sym_zero(acc2);
+                    sym_mfma<M, NT, true>(PZ, nullptr, PY, PZ, 1.0f, (float)(1.0 / 24.0),
+                                          (float)(-1.0 / 720.0), (float)(1.0 / 40320.0), b, kp, acc);
+                    sym_mfma<M, NT, true>(PZ, nullptr, PY, PZ, 1.0f, (float)(1.0 / 120.0),
+                                          (float)(-1.0 / 5040.0), (float)(1.0 / 362880.0), b, kp, acc2);
+                    __syncthreads(); // every read of Y and Z done
+                    ng += 3;
+                }
+                const float ac = m == 2 ? (float)(1.0 / 24.0) : 1.0f;
+                const float as = m == 2 ? (float)(1.0 / 120.0) : 1.0f;
 #pragma unroll
                 for (int s = 0; s < NSUB; s++) {
                     const int t = w + NW * s;
@@ -1085,42 +1184,45 @@ __global__ void __launch_bounds__(NThr<M>::v) k_expdyson(DevPlan P, int Kb, int 
                         v4f cv, pv;
 #pragma unroll
                         for (int j = 0; j < 4; j++) {
+                            // (I - Y/2) first: its diagonal stays below 1, where FP32 rounds
+                            // twice as finely as for 1 + Z/24 (half the error on the chain)
                             const float d = (row0 + j == col && col < b) ? 1.0f : 0.0f;
-                            cv[j] = acc[s][j] + d - 0.5f * y[j];
-                            pv[j] = acc2[s][j] + d - (float)(1.0 / 6.0) * y[j];
+                            const float p2 = m == 2 ? acc[s][j] : acc2[s][j];
+                            cv[j] = (d - 0.5f * y[j]) + ac * acc[s][j];
+                            pv[j] = (d - (float)(1.0 / 6.0) * y[j]) + as * p2;
                         }
                         *(v4f *)&PY[o] = cv;
                         *(v4f *)&PZ[o] = pv;
                     }
                 }
                 __syncthreads();
-                ng += 2;
             }
-            // S = X P into X
+            // S = X P, into P's plane
             sym_zero(acc);
             sym_mfma<M, NT, false>(PX, PZ, nullptr, nullptr, 1.0f, 0.f, 0.f, 0.f, b, kp, acc);
             __syncthreads();
-            sym_store<M, NT>(PX, acc, 1.0f);
+            sym_store<M, NT>(PZ, acc, 1.0f);
             __syncthreads();
+            ng += 1;
             // squarings: (C, S) <- (C^2 - S^2, 2 C S)
             for (int it = 0; it < sq; it++) {
                 sym_zero(acc);
                 sym_zero(acc2);
                 sym_mfma<M, NT, false>(PY, PY, nullptr, nullptr, 1.0f, 0.f, 0.f, 0.f, b, kp, acc);
-                sym_mfma<M, NT, false>(PX, PX, nullptr, nullptr, -1.0f, 0.f, 0.f, 0.f, b, kp, acc);
-                sym_mfma<M, NT, false>(PY, PX, nullptr, nullptr, 1.0f, 0.f, 0.f, 0.f, b, kp, acc2);
+                sym_mfma<M, NT, false>(PZ, PZ, nullptr, nullptr, -1.0f, 0.f, 0.f, 0.f, b, kp, acc);
+                sym_mfma<M, NT, false>(PY, PZ, nullptr, nullptr, 1.0f, 0.f, 0.f, 0.f, b, kp, acc2);
                 __syncthreads();
                 sym_store<M, NT>(PY, acc, 1.0f);
-                sym_store<M, NT>(PX, acc2, 2.0f);
+                sym_store<M, NT>(PZ, acc2, 2.0f);
                 __syncthreads();
             }
             nflops += 2.0 * b * b * b * ng;
-            // A_k = U_k A_{k-1}, U = C + i us S (C in Y, S in X)
+            // A_k = U_k A_{k-1}, U = C + i us S (C in Y's plane, S in Z's)
             cf *Ak = A0 + (size_t)k * P.Lsum;
             if (k == 0) {
                 for (int e = tid; e < TM * TM; e += NT) {
                     const int jj = e / TM, qq = e - jj * TM, o = jj * LDK + qq;
-                    const float cr = PY[o], ci = us * PX[o];
+                    const float cr = PY[o], ci = us * PZ[o];
                     PBr[o] = cr; // column jj of A_0 (U_0 is symmetric)
                     PBi[o] = ci;
                     if (jj < b && qq < b) Ak[qq + jj * b] = make_float2(cr, ci);
@@ -1138,7 +1240,7 @@ __global__ void __launch_bounds__(NThr<M>::v) k_expdyson(DevPlan P, int Kb, int 
                     if (t < M * M) {
                         const int oa = (16 * (t / M) + r) * LDK + qq + 4 * g;
                         const int ob = (16 * (t % M) + r) * LDK + qq + 4 * g;
-                        const v4f uc = *(const v4f *)&PY[oa], usn = *(const v4f *)&PX[oa];
+                        const v4f uc = *(const v4f *)&PY[oa], usn = *(const v4f *)&PZ[oa];
                         const v4f br = *(const v4f *)&PBr[ob], bi = *(const v4f *)&PBi[ob];
 #pragma unroll
                         for (int u = 0; u < 4; u++) {
@@ -1629,8 +1731,15 @@ __global__ void __launch_bounds__(NThr<M>::v, (M == 5 && sizeof(R) == 4) ? 3 : 1
                                  bJ, BJ, bJ, T1, smem, CGemm<M, NT, R, SPLIT>::BYTES);
         CGemm<M, NT, R, SPLIT>::run(bI, bJ, bI, BI, bI, true, T1, bI, Qt + eo, bI, smem);
         if (P.rt_from_q) continue; // R is formed from Q in k_corr
-        sparse_rows_times<NT, R>(P.r_rowptr + P.pair_roff[pr], P.r_col, (const cd *)P.r_val, bJ,
-                                 bI, BI, bI, T1, smem, CGemm<M, NT, R, SPLIT>::BYTES);
+        if (P.rho_c) {
+            // prepared rho of this crystallite (pulses before the FID): stored as rho_JI^H
+            // (b_I x b_J, column-major), so T1 = rho_JI B_I is its conjugate transpose times B_I
+            const CT *X = (const CT *)P.rho_c + (size_t)c * P.Etot + P.pair_eoff[pr];
+            CGemm<M, NT, R, SPLIT>::run(bJ, bI, bI, X, bI, true, BI, bI, T1, bJ, smem);
+        } else {
+            sparse_rows_times<NT, R>(P.r_rowptr + P.pair_roff[pr], P.r_col, (const cd *)P.r_val,
+                                     bJ, bI, BI, bI, T1, smem, CGemm<M, NT, R, SPLIT>::BYTES);
+        }
         // R^T(r,s) = sum_b T1(b,r) conj(B_J(b,s)) = (T1^T conj(B_J))(r,s): computed as
         // G = B_J^H T1 (bJ x bI) then transposed into Rt
         CGemm<M, NT, R, SPLIT>::run(bJ, bI, bJ, BJ, bJ, true, T1, bJ, G, bJ, smem);
@@ -2237,6 +2346,44 @@ __global__ void k_zero(double *p, size_t n) {
     for (; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0.0;
 }
 
+// ------------------------------------------------------------------------------ k_rho_prep
+// Pulses before the gamma-COMPUTE FID (SequenceRun::doFID, sequencerun_funcs.cc:693-703): the
+// prepared density operator rho_c = U rho U^H of crystallite c, U its preparation propagator (the
+// pulse plan's A_n, FP64, block diagonal in the pulse plan's blocks: U(x, a) = 0 unless x and a
+// share a block), evaluated on the FID plan's coupled block pairs only.  One thread per
+// (crystallite, pair element e = r + s b_I): x = state of row s of block J, y = state of column r
+// of block I, rho_c(x, y) = sum over rho's nonzeros (a, b, v) of U(x, a) v conj(U(y, b)), stored
+// conjugated (the pair's block rho_JI^H, column-major b_I x b_J) in the storage precision.
+template <typename R>
+__global__ void __launch_bounds__(256) k_rho_prep(DevPlan P, DevPlan PP, int Kb,
+                                                  const cd *__restrict__ Aprep,
+                                                  const int *__restrict__ er,
+                                                  const int *__restrict__ es,
+                                                  const int *__restrict__ sblk,
+                                                  const int *__restrict__ sloc,
+                                                  const int *__restrict__ nzi,
+                                                  const double *__restrict__ nzv, int nnz,
+                                                  typename CX<R>::T *__restrict__ out) {
+    const long idx = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (idx >= (long)Kb * P.Etot) return;
+    const int c = (int)(idx / P.Etot);
+    const long e = idx - (long)c * P.Etot;
+    const int x = P.blk_states[es[e]], y = P.blk_states[er[e]];
+    const int bx = sblk[x], by = sblk[y];
+    const cd *Ac = Aprep + (size_t)c * 2 * PP.Lsum; // the pulse plan keeps two product slots
+    const cd *Ux = Ac + PP.blk_uoff[bx], *Uy = Ac + PP.blk_uoff[by];
+    const int dx = PP.blk_dim[bx], dy = PP.blk_dim[by], lx = sloc[x], ly = sloc[y];
+    cd acc = cmk(0.0, 0.0);
+    for (int z = 0; z < nnz; z++) {
+        const int a = nzi[2 * z], b = nzi[2 * z + 1];
+        if (sblk[a] != bx || sblk[b] != by) continue;
+        const cd v = cmk(nzv[2 * z], nzv[2 * z + 1]);
+        const cd ua = Ux[lx + (size_t)sloc[a] * dx], ub = Uy[ly + (size_t)sloc[b] * dy];
+        cfma(acc, cmul(ua, v), cconj(ub));
+    }
+    out[idx] = CX<R>::from(cconj(acc));
+}
+
 // ------------------------------------------------------------------------------ static path
 // H(0, dt) block of every crystallite (wr = 0: rotor phase 0), complex copy for k_eig.
 __global__ void __launch_bounds__(256) k_static_h(DevPlan P, int Kb, int c0, const int *__restrict__ grp,
@@ -2373,8 +2520,8 @@ extern "C" int blfid_dev_launch_expdyson(int M, const DevPlan *P, int Kb, int c0
     case 1: k_expdyson<1><<<grid, 64, tb, st>>>(*P, Kb, c0, grp, ngrp, A); break;
     case 2: k_expdyson<2><<<grid, 256, tb, st>>>(*P, Kb, c0, grp, ngrp, A); break;
     case 3: k_expdyson<3><<<grid, 256, tb, st>>>(*P, Kb, c0, grp, ngrp, A); break;
-    case 4: k_expdyson<4><<<grid, 256, tb, st>>>(*P, Kb, c0, grp, ngrp, A); break;
-    default: k_expdyson<5><<<grid, 256, tb, st>>>(*P, Kb, c0, grp, ngrp, A); break;
+    case 4: k_expdyson<4><<<grid, XdThreads<4>::v, tb, st>>>(*P, Kb, c0, grp, ngrp, A); break;
+    default: k_expdyson<5><<<grid, XdThreads<5>::v, tb, st>>>(*P, Kb, c0, grp, ngrp, A); break;
     }
     return (int)hipGetLastError();
 }
@@ -2541,6 +2688,22 @@ extern "C" int blfid_dev_launch_corr(int prec, const DevPlan *P, int Kb, const v
         if (prec) launch_corr<float, 8>(grid, P, Kb, Qt, Rt, thetas, er, es, part, st);
         else launch_corr<double, 8>(grid, P, Kb, Qt, Rt, thetas, er, es, part, st);
     }
+    return (int)hipGetLastError();
+}
+
+extern "C" int blfid_dev_launch_rho_prep(int prec, const DevPlan *P, const DevPlan *PP, int Kb,
+                                         const cd *Aprep, const int *sblk, const int *sloc,
+                                         const int *nzi, const double *nzv, int nnz,
+                                         void *rho_out, hipStream_t st) {
+    const long n = (long)Kb * P->Etot;
+    if (n <= 0) return 0;
+    const int grid = (int)((n + 255) / 256);
+    if (prec)
+        k_rho_prep<float><<<grid, 256, 0, st>>>(*P, *PP, Kb, Aprep, P->elem_r, P->elem_s, sblk,
+                                                 sloc, nzi, nzv, nnz, (cf *)rho_out);
+    else
+        k_rho_prep<double><<<grid, 256, 0, st>>>(*P, *PP, Kb, Aprep, P->elem_r, P->elem_s, sblk,
+                                                  sloc, nzi, nzv, nnz, (cd *)rho_out);
     return (int)hipGetLastError();
 }
 

@@ -25,6 +25,8 @@ enum Region {
     R_EIG,       // eigen-solver scratch of blocks beyond the LDS classes
     R_WR,        // W in the storage precision (complex64 plans)
     R_GROUP,     // powder-sum accumulators: per group of CRYST_GROUP crystallites, sum + comp
+    R_RHO,       // prepared density operator of each crystallite: its coupled block pairs
+    R_PREP,      // working storage of the preparation (pulse) propagators
     R_COUNT
 };
 
@@ -45,6 +47,8 @@ struct Shape {
     long Lsum = 0;      // sum of b^2 over blocks
     long Estr = 0;      // Q row stride per step (elements, padded)
     long K = 0;         // crystallites of the powder (sizes the group accumulators)
+    long rho_elems = 0; // prepared-rho elements per crystallite (pulses before the FID), or 0
+    const Shape *prep = nullptr; // layout of the preparation propagators, or none
 };
 
 inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -78,6 +82,14 @@ inline void region_sizes(const Shape &s, long Kb, size_t sz[R_COUNT]) {
     sz[R_PART] = kb * (size_t)s.nparts * 2 * s.npts * d8;
     sz[R_EIG] = (size_t)s.eig_slots * bm2 * c16;
     sz[R_GROUP] = ngroups(s.K) * 2 * (2 * (size_t)s.npts) * d8;
+    sz[R_RHO] = kb * (size_t)s.rho_elems * cs;
+    if (s.prep) {
+        size_t psz[R_COUNT];
+        region_sizes(*s.prep, Kb, psz);
+        size_t o = 0;
+        for (int i = 0; i < R_COUNT; i++) o += align256(psz[i]);
+        sz[R_PREP] = o;
+    }
 }
 
 // Total bytes of the layout for Kb crystallites; off[] receives each region's offset.

@@ -89,6 +89,8 @@ def lib():
         L.blfid_plan_set_powder.argtypes = [vp, _d, _d, _d, _d, ctypes.c_int]
         L.blfid_plan_run.argtypes = [vp, ctypes.c_int, vp]
         L.blfid_plan_run_noalloc.argtypes = [vp, ctypes.c_int, vp, vp]
+        L.blfid_plan_set_prep.argtypes = [vp, ctypes.c_double, ctypes.c_int, _d, _d, ctypes.c_int,
+                                          ctypes.POINTER(_RF)]
         L.blfid_plan_fetch.argtypes = [vp, _d]
         L.blfid_plan_device_fid.argtypes = [vp]
         L.blfid_plan_device_fid.restype = vp
@@ -166,16 +168,7 @@ class Plan:
                                        int(npts), float(t0), prec, ctypes.byref(h)))
         else:
             steps, rf, step_dt, step_t1 = _ptop
-            ids = {}
-            for n_ in sy.isotopes:
-                ids.setdefault(n_, len(ids))
-            tabs, chans = [], []
-            for name, amp, ph, off in rf:
-                t = [np.ascontiguousarray(np.broadcast_to(np.asarray(v, np.float64), (steps,)))
-                     for v in (amp, ph, off)]
-                tabs.append(t)
-                chans.append(_RF(ids.get(name, -1), _p(t[0]), _p(t[1]), _p(t[2])))
-            arr = (_RF * max(1, len(chans)))(*chans)
+            (chans, tabs), arr = _rf_channels(sy, rf, steps)
             self._rf = (tabs, arr)
             if step_dt is None:
                 _check(L.blfid_plan_create_ptop(ctypes.byref(s), float(wr), float(rotor_deg),
@@ -194,6 +187,7 @@ class Plan:
         self.h = h
         self.N = sy.N
         self.npts = int(npts)
+        self._sys = sy
 
     @classmethod
     def ptop(cls, sy: SolidSys, wr: float, rotor_deg: float, steps: int, npts: int, rf=(),
@@ -212,6 +206,21 @@ class Plan:
         dt = np.atleast_1d(np.asarray(step_dt, np.float64))
         return cls(sy, wr, rotor_deg, 0.0, npts, t0, precision=precision,
                    _ptop=(len(dt), list(rf), dt, step_t1))
+
+    def set_prep(self, rotor_deg: float, step_dt, rf=(), step_t1=None):
+        """Pulses applied to rho before the FID (blfid_plan_set_prep): steps of lengths step_dt
+        starting at step_t1 (or consecutive from 0), rf channels (isotope, amp[steps],
+        phase_rad[steps], offset[steps]); the plan's t0 should be the pulses' end.  Operators
+        must be set afterwards.  An empty step list removes the preparation."""
+        dt = np.ascontiguousarray(np.atleast_1d(np.asarray(step_dt, np.float64)))
+        n = len(dt)
+        (chans, tabs), arr = _rf_channels(self._sys, rf, n)
+        t1 = (None if step_t1 is None else
+              np.ascontiguousarray(np.broadcast_to(np.asarray(step_t1, np.float64), (n,))))
+        self._prep = (dt, t1, tabs, arr)
+        _check(lib().blfid_plan_set_prep(self.h, float(rotor_deg), n,
+                                         None if t1 is None else _p(t1),
+                                         _p(dt) if n else None, len(chans), arr))
 
     def close(self):
         if getattr(self, "h", None) and self.h.value:
@@ -319,6 +328,38 @@ def powder_fid(sy, wr, rotor_deg, sw, npts, pw, rho, det, t0=0.0, normalize=True
     try:
         f = p.powder_fid(pw, rho, det, normalize)
         return f, p.sw
+    finally:
+        p.close()
+
+
+def _rf_channels(sy, rf, steps):
+    """((channels, tables), blfid_rf array) of rf channels (isotope name, amp, phase_rad,
+    offset; scalars or one value per step); the array points into the tables."""
+    ids = {}
+    for n_ in sy.isotopes:
+        ids.setdefault(n_, len(ids))
+    tabs, chans = [], []
+    for name, amp, ph, off in rf:
+        t = [np.ascontiguousarray(np.broadcast_to(np.asarray(v, np.float64), (steps,)))
+             for v in (amp, ph, off)]
+        tabs.append(t)
+        chans.append(_RF(ids.get(name, -1), _p(t[0]), _p(t[1]), _p(t[2])))
+    arr = (_RF * max(1, len(chans)))(*chans)
+    return (chans, tabs), arr
+
+
+def prep_fid(sy, wr, rotor_deg, sw, npts, pw, rho, det, step_dt, rf=(), step_t1=None,
+             t0=None, prep_rotor_deg=None, normalize=True, precision="fp64"):
+    """Pulses before a gamma-COMPUTE FID (solid-2.0: pulses, then a plain fid()): per
+    crystallite rho <- U rho U^H with U the steps' propagator, then the powder FID from start
+    time t0 (default: the end of the steps).  Returns (fid, adjusted sw)."""
+    dt = np.atleast_1d(np.asarray(step_dt, np.float64))
+    if t0 is None:
+        t0 = float(dt.sum()) if step_t1 is None else float(np.asarray(step_t1)[-1] + dt[-1])
+    p = Plan(sy, wr, rotor_deg, sw, npts, t0, precision=precision)
+    try:
+        p.set_prep(rotor_deg if prep_rotor_deg is None else prep_rotor_deg, dt, rf, step_t1)
+        return p.powder_fid(pw, rho, det, normalize), p.sw
     finally:
         p.close()
 

@@ -27,9 +27,12 @@ Reference behaviour restated (examples/solid-2.0/src):
     their rotor angle from the sub-sequence's ``rotor()`` with the magic angle as default
     (pulsedata.cc:287-300), pulses of the main section the ``rotor`` parameter; a
     sub-sequence's times start at 0 each time it is used (sequencerun.cc:122-162).
-Everything else (2-D acquisition, indexed ``fid(i)``, main-section ``loop``/``if``, pulses
-before a plain ``fid()``, ``hold``, several spin systems or powders in one program) raises
-BlochLibError: the engine has no per-crystallite CPU path to fall back on.
+  * pulses before a plain ``fid()``: every crystallite's rho is propagated through the used
+    propagators, then its gamma-COMPUTE FID runs from the current time (sequencerun_funcs.cc:
+    693-703) -- the engine's preparation stage (blfid_plan_set_prep).
+Everything else (2-D acquisition, indexed ``fid(i)``, main-section ``loop``/``if``, ``hold``,
+several spin systems or powders in one program) raises BlochLibError: the engine has no
+per-crystallite CPU path to fall back on.
 """
 from __future__ import annotations
 
@@ -503,13 +506,30 @@ class _Runner:
         ro = sy.op(self.ro if self.ro is not None else str(self.p["roeq"]))
         wr, rotor = float(self.p["wr"]), float(self.p["rotor"])
         if not self.ptop:
-            if self.props:
-                raise BlochLibError("pulses before a plain fid() (a prepared density matrix per "
-                                    "crystallite) are not supported by this engine")
-            f, sw = engine.powder_fid(sy, wr, rotor, float(self.p["sw"]), npts, pw, ro, det,
-                                      normalize=True, precision=self.precision)
+            st = self.steps() if self.props else []
+            if st:
+                # pulses before a plain fid(): every crystallite's rho is propagated through the
+                # used sub-sequences, then its gamma-COMPUTE FID runs from the current time
+                # (sequencerun_funcs.cc:693-703, oneFID::setStartTime(currentTime))
+                rots = {x[2] for x in st}
+                if len(rots) > 1:
+                    raise BlochLibError("pulses with different rotor angles in one propagator "
+                                        "are not supported by this engine")
+                t1 = np.array([x[0] for x in st])
+                dt = np.array([x[1] for x in st])
+                f, sw = engine.prep_fid(sy, wr, rotor, float(self.p["sw"]), npts, pw, ro, det,
+                                        dt, rf=self.rf_tables(st), step_t1=t1,
+                                        t0=self.current_time,
+                                        prep_rotor_deg=next(iter(rots)) * RAD2DEG,
+                                        normalize=True, precision=self.precision)
+                self.log(f"fid after {len(st)} pulse steps: {len(pw)} crystallites, {npts} "
+                         f"points, sw {sw}")
+            else:
+                f, sw = engine.powder_fid(sy, wr, rotor, float(self.p["sw"]), npts, pw, ro, det,
+                                          normalize=True, precision=self.precision)
+                self.log(f"fid: {len(pw)} crystallites, {npts} points, sw {sw}")
             self.p["sw"] = sw
-            self.log(f"fid: {len(pw)} crystallites, {npts} points, sw {sw}")
+            self.props = []
         else:
             st = self.steps()
             if not st:

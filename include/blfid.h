@@ -116,6 +116,22 @@ blfid_status blfid_plan_create_pulse(const blfid_system *sys, double wr_hz, doub
                                      int npts, double t0, int nrf, const blfid_rf *rf,
                                      blfid_precision prec, blfid_plan **out);
 
+/* Pulses applied to rho before the FID of a spinning gamma-COMPUTE plan, as solid-2.0 runs a
+ * plain fid() after pulses (SequenceRun::doFID, sequencerun_funcs.cc:693-703): for every
+ * crystallite rho <- U rho U^H with U the propagator of the `nsteps` steps (lengths step_dt[k],
+ * start times step_t1[k] or consecutive from 0 when NULL, rf channels as in
+ * blfid_plan_create_pulse, rotor angle rotor_deg, the plan's spinning speed), computed in FP64;
+ * the FID then runs from the plan's start time t0 (oneFID::setStartTime(currentTime),
+ * sequencerun_funcs.cc:611), i.e. the plan should be created with t0 = the pulses' end.  The
+ * prepared rho is treated as general (the rho == (det + det^H)/2 shortcut of compute.h:75-83 is
+ * not taken; for a prepared rho it is an exact floating-point coincidence).  nsteps == 0 removes
+ * the preparation.  Operators must be set (again) after this call.  Replaces the per-crystallite
+ * PropList[j].propogate(ro) + hFID.FID(ro, ...) of the reference.  Static and pulse-program
+ * plans return BLFID_ERR_UNSUPPORTED. */
+blfid_status blfid_plan_set_prep(blfid_plan *plan, double rotor_deg, int nsteps,
+                                 const double *step_t1, const double *step_dt, int nrf,
+                                 const blfid_rf *rf);
+
 /* Adjusted sweep width n*wr (compute.h:96, oneFID.h:429), steps per rotor period n, N. */
 double blfid_plan_sw(const blfid_plan *plan);
 int blfid_plan_steps(const blfid_plan *plan);

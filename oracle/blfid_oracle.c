@@ -695,27 +695,87 @@ done:
 /* powder driver (threads over crystallites; deterministic ordered final sum)          */
 /* ------------------------------------------------------------------------------------ */
 
+/* pulses applied to rho before a gamma-COMPUTE FID (SequenceRun::doFID, sequencerun_funcs.cc:
+ * 693-703): rho <- U rho U^H with U the sub-sequence propagator of each crystallite */
+typedef struct {
+    double wr, rotb_rad;
+    int nsteps, nrf;
+    const double *step_t1, *step_dt, *rf;
+} orc_prep;
+
 typedef struct {
     int spinning;
     orc_spin_cfg sc;
     orc_static_cfg tc;
+    const orc_prep *prep; /* NULL: rho as given */
     const double *theta, *phi, *gamma, *wt;
     int k0, k1, P;
     cplx *acc; /* P */
     int status;
 } orc_job;
 
+static int pulse_propagator(const orc_sys *sy, const double cp[9], double wr, double rotb_rad,
+                            double t0, double tau, int nsteps, const double *step_t1,
+                            const double *step_dt, const double *rf, int nrf, cplx *U, cplx *H,
+                            cplx *X, cplx *T);
+
+/* rho_c = U rho U^H for one crystallite, and the reference's exact isroSYM test of rho_c */
+static int prep_rho(const orc_prep *pp, const orc_sys *sy, double theta, double phi, double gamma,
+                    const cplx *rho, const cplx *det, cplx *rc, int *rosym) {
+    const int N = sy->N;
+    const size_t NN = (size_t)N * N;
+    cplx *U = malloc(sizeof(cplx) * NN), *H = malloc(sizeof(cplx) * NN);
+    cplx *X = malloc(sizeof(cplx) * NN), *T = malloc(sizeof(cplx) * NN);
+    int st = ORC_ENOMEM;
+    if (U && H && X && T) {
+        double cp[9];
+        rot3(phi, theta, gamma, cp);
+        st = pulse_propagator(sy, cp, pp->wr, pp->rotb_rad, 0.0, 0.0, pp->nsteps, pp->step_t1,
+                              pp->step_dt, pp->rf, pp->nrf, U, H, X, T);
+        if (st == ORC_OK) {
+            zgemm(N, 'N', 'C', rho, U, T); /* rho U^H */
+            zgemm(N, 'N', 'N', U, T, rc);  /* U rho U^H */
+            int sym = 1;
+            for (int i = 0; i < N && sym; i++)
+                for (int j = 0; j < N; j++)
+                    if (EL(rc, i, j, N) != 0.5 * (EL(det, i, j, N) + conj(EL(det, j, i, N)))) {
+                        sym = 0;
+                        break;
+                    }
+            *rosym = sym;
+        }
+    }
+    free(U); free(H); free(X); free(T);
+    return st;
+}
+
 static void *job_run(void *arg) {
     orc_job *jb = (orc_job *)arg;
     cplx *f = malloc(sizeof(cplx) * (jb->P > 0 ? jb->P : 1));
     if (!f) { jb->status = ORC_ENOMEM; return NULL; }
     for (int t = 0; t < jb->P; t++) jb->acc[t] = 0;
+    cplx *rc = NULL;
+    if (jb->prep) {
+        const size_t N = (size_t)jb->sc.sy->N;
+        rc = malloc(sizeof(cplx) * N * N);
+        if (!rc) { free(f); jb->status = ORC_ENOMEM; return NULL; }
+    }
     for (int k = jb->k0; k < jb->k1; k++) {
-        int st = jb->spinning ? spinning_fid_one(&jb->sc, jb->theta[k], jb->phi[k], jb->gamma[k], f)
+        int st;
+        if (jb->prep) {
+            orc_spin_cfg cf = jb->sc;
+            st = prep_rho(jb->prep, cf.sy, jb->theta[k], jb->phi[k], jb->gamma[k], cf.rho, cf.det,
+                          rc, &cf.rosym);
+            cf.rho = rc;
+            if (!st) st = spinning_fid_one(&cf, jb->theta[k], jb->phi[k], jb->gamma[k], f);
+        } else {
+            st = jb->spinning ? spinning_fid_one(&jb->sc, jb->theta[k], jb->phi[k], jb->gamma[k], f)
                               : static_fid_one(&jb->tc, jb->theta[k], jb->phi[k], jb->gamma[k], f);
+        }
         if (st) { jb->status = st; break; }
         for (int t = 0; t < jb->P; t++) jb->acc[t] += jb->wt[k] * f[t];
     }
+    free(rc);
     free(f);
     return NULL;
 }
@@ -727,7 +787,7 @@ static void *job_run(void *arg) {
  *   rho/det: N x N complex, column-major, interleaved re/im
  *   normalize: 1 -> divide by Tr(det det^H) (sequencerun_funcs.cc:698-702); 0 -> oneFID.h:396-402
  *   fid_out: 2*npts doubles; sw_out: adjusted sweep width (compute.h:96) */
-int orc_powder_fid(int S, const int *isotopes, int ncsa, const double *csa, int ndip,
+static int powder_fid_impl(const orc_prep *prep, int S, const int *isotopes, int ncsa, const double *csa, int ndip,
                    const double *dip, int nj, const double *jc, double wr, double rotor_deg,
                    double sw, int npts, double t0, int K, const double *theta, const double *phi,
                    const double *gamma, const double *w, const double *rho, const double *det,
@@ -762,7 +822,7 @@ int orc_powder_fid(int S, const int *isotopes, int ncsa, const double *csa, int 
     cplx *accs = calloc((size_t)nthreads * (npts > 0 ? npts : 1), sizeof(cplx));
     if (!jobs || !th || !accs) { free(jobs); free(th); free(accs); return ORC_ENOMEM; }
     for (int t = 0; t < nthreads; t++) {
-        jobs[t].spinning = spinning; jobs[t].sc = sc; jobs[t].tc = tc;
+        jobs[t].spinning = spinning; jobs[t].sc = sc; jobs[t].tc = tc; jobs[t].prep = prep;
         jobs[t].theta = theta; jobs[t].phi = phi; jobs[t].gamma = gamma; jobs[t].wt = w;
         jobs[t].k0 = (int)((long)K * t / nthreads); jobs[t].k1 = (int)((long)K * (t + 1) / nthreads);
         jobs[t].P = npts; jobs[t].acc = accs + (size_t)t * (npts > 0 ? npts : 1);
@@ -784,6 +844,36 @@ int orc_powder_fid(int S, const int *isotopes, int ncsa, const double *csa, int 
     }
     free(jobs); free(th); free(accs);
     return st;
+}
+
+int orc_powder_fid(int S, const int *isotopes, int ncsa, const double *csa, int ndip,
+                   const double *dip, int nj, const double *jc, double wr, double rotor_deg,
+                   double sw, int npts, double t0, int K, const double *theta, const double *phi,
+                   const double *gamma, const double *w, const double *rho, const double *det,
+                   int normalize, int nthreads, double *fid_out, double *sw_out) {
+    return powder_fid_impl(NULL, S, isotopes, ncsa, csa, ndip, dip, nj, jc, wr, rotor_deg, sw,
+                           npts, t0, K, theta, phi, gamma, w, rho, det, normalize, nthreads,
+                           fid_out, sw_out);
+}
+
+/* Pulses before a gamma-COMPUTE fid() (SequenceRun::doFID, sequencerun_funcs.cc:693-703): per
+ * crystallite rho <- U rho U^H with U the propagator of the nsteps pulse steps (lengths
+ * step_dt, start times step_t1, rf table rf as orc_pulse_fid, rotor angle prep_rotor_deg, the
+ * FID's wr), then the FID of that rho from start time t0 (oneFID::setStartTime(currentTime),
+ * sequencerun_funcs.cc:611), the isroSYM test applied to the prepared rho as the reference does.
+ * wr and rotor_deg must describe a spinning FID. */
+int orc_prep_fid(int S, const int *isotopes, int ncsa, const double *csa, int ndip,
+                 const double *dip, int nj, const double *jc, double wr, double rotor_deg,
+                 double sw, int npts, double t0, int nsteps, const double *step_t1,
+                 const double *step_dt, double prep_rotor_deg, int nrf, const double *rf, int K,
+                 const double *theta, const double *phi, const double *gamma, const double *w,
+                 const double *rho, const double *det, int normalize, int nthreads,
+                 double *fid_out, double *sw_out) {
+    if (wr == 0.0 || rotor_deg == 0.0 || nsteps < 1 || !step_dt) return ORC_EINVAL;
+    orc_prep pp = {wr, prep_rotor_deg * DEG2RAD, nsteps, nrf, step_t1, step_dt, rf};
+    return powder_fid_impl(&pp, S, isotopes, ncsa, csa, ndip, dip, nj, jc, wr, rotor_deg, sw,
+                           npts, t0, K, theta, phi, gamma, w, rho, det, normalize, nthreads,
+                           fid_out, sw_out);
 }
 
 /* ------------------------------------------------------------------------------------ */
@@ -834,6 +924,42 @@ typedef struct {
     cplx *acc;
 } orc_pulse_job;
 
+/* U = prod_k exp(+i 2 pi tau_k (H_k + H_rf,k)) of a pulse sub-sequence for one crystallite
+ * (cartPowder cp): equal steps tau from t0 when step_dt is NULL, else steps of lengths step_dt
+ * starting at step_t1[k] (or consecutive from t0 when step_t1 is NULL) -- Propogation::
+ * propogator, propogation.cc:240-304, U *= X meaning U = X U.  H, X, T are N x N work arrays. */
+static int pulse_propagator(const orc_sys *sy, const double cp[9], double wr, double rotb_rad,
+                            double t0, double tau, int nsteps, const double *step_t1,
+                            const double *step_dt, const double *rf, int nrf, cplx *U, cplx *H,
+                            cplx *X, cplx *T) {
+    const int N = sy->N;
+    const size_t NN = (size_t)N * N;
+    for (size_t q = 0; q < NN; q++) U[q] = 0;
+    for (int i = 0; i < N; i++) U[i + (size_t)i * N] = 1.0;
+    double t1 = (step_dt && step_t1) ? step_t1[0] : t0;
+    double t2 = t1 + (step_dt ? step_dt[0] : tau);
+    for (int k = 0; k < nsteps; k++) {
+        double dt = step_dt ? step_dt[k] : tau;
+        double tmid = (t2 - t1) / 2.0 + t1;
+        build_hamiltonian(sy, cp, rotb_rad, tmid * wr * PI2, H);
+        add_rf(sy, rf + (size_t)k * 4 * nrf, nrf, H);
+        int st = orc_expm_herm(N, H, CMPLX(0.0, 1.0) * dt * PI2, X);
+        if (st) return st;
+        zgemm(N, 'N', 'N', X, U, T);
+        memcpy(U, T, sizeof(cplx) * NN);
+        if (step_dt) {
+            if (k + 1 < nsteps) {
+                t1 = step_t1 ? step_t1[k + 1] : t2;
+                t2 = t1 + step_dt[k + 1];
+            }
+        } else {
+            t1 += tau;
+            t2 += tau;
+        }
+    }
+    return ORC_OK;
+}
+
 /* crystallites k0..k1-1 of a pulse-program powder: acc[m] += w_c Tr(rho_m det) */
 static void *pulse_job_run(void *arg) {
     orc_pulse_job *jb = (orc_pulse_job *)arg;
@@ -851,32 +977,11 @@ static void *pulse_job_run(void *arg) {
     for (int c = jb->k0; c < jb->k1; c++) {
         double cp[9];
         rot3(jb->phi[c], jb->theta[c], jb->gamma[c], cp);
-        for (size_t q = 0; q < NN; q++) U[q] = 0;
-        for (int i = 0; i < N; i++) U[i + (size_t)i * N] = 1.0;
-        const double *sdt = jb->step_dt, *st1 = jb->step_t1;
-        double t1 = (sdt && st1) ? st1[0] : jb->t0;
-        double t2 = t1 + (sdt ? sdt[0] : jb->tau);
-        for (int k = 0; k < jb->nsteps; k++) {
-            double dt = sdt ? sdt[k] : jb->tau;
-            double tmid = (t2 - t1) / 2.0 + t1;
-            build_hamiltonian(sy, cp, jb->rotb_rad, tmid * jb->wr * PI2, H);
-            add_rf(sy, jb->rf + (size_t)k * 4 * jb->nrf, jb->nrf, H);
-            int st = orc_expm_herm(N, H, CMPLX(0.0, 1.0) * dt * PI2, X);
-            if (st) {
-                jb->status = st;
-                goto done;
-            }
-            zgemm(N, 'N', 'N', X, U, T);
-            memcpy(U, T, sizeof(cplx) * NN);
-            if (sdt) {
-                if (k + 1 < jb->nsteps) {
-                    t1 = st1 ? st1[k + 1] : t2;
-                    t2 = t1 + sdt[k + 1];
-                }
-            } else {
-                t1 += jb->tau;
-                t2 += jb->tau;
-            }
+        int st = pulse_propagator(sy, cp, jb->wr, jb->rotb_rad, jb->t0, jb->tau, jb->nsteps,
+                                  jb->step_t1, jb->step_dt, jb->rf, jb->nrf, U, H, X, T);
+        if (st) {
+            jb->status = st;
+            goto done;
         }
         memcpy(r, jb->rho, sizeof(cplx) * NN);
         for (int m = 0; m < jb->npts; m++) {

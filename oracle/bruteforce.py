@@ -163,11 +163,10 @@ def rf_hamiltonian(sy, rf_k):
     return H
 
 
-def ptop_equivalent(sy, theta, phi, gamma, rotor_deg, wr, nsteps, npts, rho, det, rf=(), t0=0.0,
-                    step_dt=None, step_t1=None):
-    """One crystallite: U = prod_k exp(+i 2 pi tau_k (H_k + H_rf,k)) (propogation.cc:240-304,
-    note the + sign), fid[m] = Tr(rho_m det), rho_{m+1} = U rho_m U^H (sequencerun.cc:164-196).
-    Steps are one rotor period of equal length, or consecutive steps of lengths step_dt."""
+def pulse_propagator(sy, theta, phi, gamma, rotor_deg, wr, nsteps, rf=(), t0=0.0, step_dt=None,
+                     step_t1=None):
+    """U = prod_k exp(+i 2 pi tau_k (H_k + H_rf,k)) over one rotor period of equal steps or the
+    given steps (propogation.cc:240-304; U *= X is U = X U), from scipy.linalg.expm."""
     dts = (np.full(nsteps, 1.0 / wr / nsteps) if step_dt is None
            else np.broadcast_to(np.asarray(step_dt, float), (nsteps,)))
     starts = (t0 + np.concatenate([[0.0], np.cumsum(dts)[:-1]]) if step_t1 is None
@@ -180,6 +179,29 @@ def ptop_equivalent(sy, theta, phi, gamma, rotor_deg, wr, nsteps, npts, rho, det
         H = H + rf_hamiltonian(sy, [(n_, np.broadcast_to(a, nsteps)[k], np.broadcast_to(p, nsteps)[k],
                                      np.broadcast_to(o, nsteps)[k]) for n_, a, p, o in rf])
         U = scipy.linalg.expm(2j * np.pi * tau * H) @ U
+    return U
+
+
+def prep_equivalent(sy, theta, phi, gamma, rotor_deg, wr, sw, npts, rho, det, step_dt, rf=(),
+                    step_t1=None, t0_fid=0.0, prep_rotor_deg=None):
+    """Pulses before a gamma-COMPUTE FID (sequencerun_funcs.cc:693-703) for one crystallite:
+    rho <- U rho U^H with U the pulse steps' propagator, then the explicit-stepping equivalent of
+    the reference's gamma-COMPUTE FID of that rho from start time t0_fid."""
+    dts = np.asarray(step_dt, float)
+    U = pulse_propagator(sy, theta, phi, gamma,
+                         rotor_deg if prep_rotor_deg is None else prep_rotor_deg, wr, len(dts), rf,
+                         0.0, dts, step_t1)
+    rc = U @ rho.astype(np.complex128) @ U.conj().T
+    return gamma_compute_equivalent(sy, theta, phi, gamma, rotor_deg, wr, sw, npts, rc, det,
+                                    t0=t0_fid)
+
+
+def ptop_equivalent(sy, theta, phi, gamma, rotor_deg, wr, nsteps, npts, rho, det, rf=(), t0=0.0,
+                    step_dt=None, step_t1=None):
+    """One crystallite: U = prod_k exp(+i 2 pi tau_k (H_k + H_rf,k)) (propogation.cc:240-304,
+    note the + sign), fid[m] = Tr(rho_m det), rho_{m+1} = U rho_m U^H (sequencerun.cc:164-196).
+    Steps are one rotor period of equal length, or consecutive steps of lengths step_dt."""
+    U = pulse_propagator(sy, theta, phi, gamma, rotor_deg, wr, nsteps, rf, t0, step_dt, step_t1)
     r = rho.astype(np.complex128)
     fid = np.zeros(npts, dtype=np.complex128)
     for m in range(npts):

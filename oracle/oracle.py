@@ -47,6 +47,12 @@ def lib():
                                     ctypes.c_int, _d, _d, ctypes.c_int, ctypes.c_double,
                                     ctypes.c_int, _d, ctypes.c_int, _d, _d, _d, _d, _d, _d,
                                     ctypes.c_int, ctypes.c_int, _d]
+        L.orc_prep_fid.restype = ctypes.c_int
+        L.orc_prep_fid.argtypes = ([ctypes.c_int, _i, ctypes.c_int, _d, ctypes.c_int, _d,
+                                    ctypes.c_int, _d, ctypes.c_double, ctypes.c_double,
+                                    ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                    _d, _d, ctypes.c_double, ctypes.c_int, _d, ctypes.c_int]
+                                   + [_d] * 6 + [ctypes.c_int, ctypes.c_int, _d, _d])
         L.orc_hamiltonian.argtypes = [ctypes.c_int, _i, ctypes.c_int, _d, ctypes.c_int, _d,
                                       ctypes.c_int, _d] + [ctypes.c_double] * 5 + [_d]
         L.orc_eig_herm_x.argtypes = [ctypes.c_int, _d, _d, _d]
@@ -181,3 +187,35 @@ def ptop_fid(sy, wr, rotor_deg, nsteps, npts, powder, rho, det, rf=(), t0=0.0, n
     if st != 0:
         raise RuntimeError(f"oracle failed with status {st}")
     return out[: 2 * npts].view(np.complex128).copy()
+
+
+def prep_fid(sy, wr, rotor_deg, sw, npts, powder, rho, det, step_dt, step_t1=None,
+             prep_rotor_deg=None, rf=(), t0=0.0, normalize=True, nthreads=None):
+    """Pulses before a gamma-COMPUTE FID: per crystallite rho <- U rho U^H with U the propagator
+    of the steps (lengths step_dt, start times step_t1 or consecutive from 0, rf channels as
+    ptop_fid, rotor angle prep_rotor_deg, default the FID's), then the powder FID of that rho
+    from start time t0.  Returns (fid, adjusted sw)."""
+    L = lib()
+    iso, nc, csa, nd, dip, nj, jc = _sys(sy)
+    dts = np.ascontiguousarray(np.asarray(step_dt, np.float64).ravel())
+    nsteps = len(dts)
+    tab, nrf = rf_table(sy, list(rf), nsteps)
+    t1s = None if step_t1 is None else np.ascontiguousarray(np.asarray(step_t1, np.float64).ravel())
+    r = _cm(rho)
+    d = _cm(det)
+    out = np.zeros(2 * max(npts, 1))
+    swo = np.zeros(1)
+    th = np.ascontiguousarray(powder.theta, np.float64)
+    ph = np.ascontiguousarray(powder.phi, np.float64)
+    ga = np.ascontiguousarray(powder.gamma, np.float64)
+    w = np.ascontiguousarray(powder.weight, np.float64)
+    st = L.orc_prep_fid(sy.spins, _p(iso, _i), nc, _p(csa), nd, _p(dip), nj, _p(jc), float(wr),
+                        float(rotor_deg), float(sw), int(npts), float(t0), nsteps,
+                        _p(t1s) if t1s is not None else None, _p(dts),
+                        float(rotor_deg if prep_rotor_deg is None else prep_rotor_deg), nrf,
+                        _p(tab), len(th), _p(th), _p(ph), _p(ga), _p(w), _p(r), _p(d),
+                        int(bool(normalize)), int(nthreads or os.cpu_count() or 1), _p(out),
+                        _p(swo))
+    if st != 0:
+        raise RuntimeError(f"oracle failed with status {st}")
+    return out[: 2 * npts].view(np.complex128).copy(), float(swo[0])

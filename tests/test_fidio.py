@@ -63,3 +63,29 @@ def test_matlab_round_trip(tmp_path):
 def test_dispatch(tmp_path, cmd):
     p = fidio.save_fid(cmd, np.ones(8, complex), str(tmp_path / cmd), 1000.0)
     assert os.path.exists(p)
+
+
+def test_matlab_file_read_by_scipy(tmp_path):
+    """The MAT-file is read by an independent MAT v5 reader (scipy.io.loadmat) as the complex
+    column vector vdat (matstream::put("vdat", fid1D), sequencerun_funcs.cc:824-827)."""
+    import scipy.io
+    f = np.exp(2j * np.pi * 0.01 * np.arange(33)) * np.linspace(1.0, 0.2, 33)
+    p = fidio.save_fid_matlab(f, str(tmp_path / "m.mat"))
+    d = scipy.io.loadmat(p)
+    v = d["vdat"]
+    assert v.shape == (33, 1) and np.iscomplexobj(v)
+    assert np.array_equal(v[:, 0], f)
+
+
+def test_binary_layout_of_reference(tmp_path):
+    """savefidbinary's layout (sequencerun_funcs.cc:828-842): the text line "npts1 =" P, endl,
+    then for each point its real and imaginary parts as native doubles, parsed here byte by byte
+    without the module's reader."""
+    import struct
+    f = np.array([1.5 - 2.0j, -0.25 + 0.125j, 3e-7 + 0j])
+    p = fidio.save_fid_binary(f, str(tmp_path / "b"))
+    raw = open(p, "rb").read()
+    head = b"npts1 =3\n"
+    assert raw.startswith(head) and len(raw) == len(head) + 3 * 16
+    vals = struct.unpack("=6d", raw[len(head):])
+    assert vals == (1.5, -2.0, -0.25, 0.125, 3e-7, 0.0)

@@ -322,3 +322,31 @@ def test_explicit_step_start_times_match_explicit_numpy(oracle):
                          normalize=False, step_dt=dts)
     assert np.abs(f1 - f2).max() < 1e-11
     assert np.abs(f1 - f3).max() > 1e-6   # the start times matter under spinning
+
+
+@pytest.mark.parametrize("case", ["homo-x-pulse", "hetero-13C-pulse-and-delay"])
+def test_prep_fid_equals_explicit_propagation(oracle, case):
+    """Pulses before a gamma-COMPUTE fid() (sequencerun_funcs.cc:693-703): the oracle's
+    prepared-rho FID equals, per crystallite, U_prep rho U_prep^H from scipy expm steps (+i sign,
+    rf on the pulsed isotope) followed by the explicit-stepping gamma-COMPUTE equivalent from the
+    pulses' end time."""
+    if case == "homo-x-pulse":
+        sy = SolidSys.from_text("numspin 2\nT 1H 0\nT 1H 1\nC 1000 5000 0.5 0 10 20 30\n"
+                                "C -1000 4000 0.3 1 40 50 60\nD -15000 0 1 0 30 40\n")
+        dts = np.full(3, 0.8e-6)
+        rf = [("1H", np.full(3, 100000.0), np.zeros(3), np.zeros(3))]
+        rho, det = sy.op("Iz"), sy.op("Ip")
+    else:
+        sy = SolidSys.from_text("numspin 2\nT 13C 0\nT 1H 1\nC 200 4000 0.3 0 10 20 30\n"
+                                "C 0 3000 0.6 1 40 50 60\nD -20000 0 1 0 30 40\nJ 140 0 1\n")
+        dts = np.array([1.25e-6, 1.25e-6, 2.0e-6])
+        rf = [("13C", np.array([80000.0, 80000.0, 0.0]), np.array([0.0, 0.3, 0.0]),
+               np.array([500.0, 500.0, 0.0]))]
+        rho, det = sy.op("Iz_0"), sy.op("Ip_0")
+    wr, sw, npts = 10000.0, 80000.0, 20
+    t0 = float(dts.sum())
+    for th, ph, ga in [(0.7, 1.9, 2.3), (1.2, 0.4, 0.0)]:
+        f1, _ = oracle.prep_fid(sy, wr, MAGIC, sw, npts, Powder.single(th, ph, ga), rho, det,
+                                step_dt=dts, rf=rf, t0=t0, normalize=False)
+        f2 = B.prep_equivalent(sy, th, ph, ga, MAGIC, wr, sw, npts, rho, det, dts, rf=rf, t0_fid=t0)
+        assert np.abs(f1 - f2).max() < 1e-11 * max(1.0, np.abs(f2).max()), (case, th)

@@ -99,16 +99,33 @@ def test_config4_ptop_full_length_c64():
     assert err1 < TOL_C64
 
 
-def test_ten_spin_c64_against_fp64_engine():
-    """Config-3 shape (10 spins, N = 1024, 64 steps, 512 points): the complex64 form against the
-    FP64 form of the engine (itself pinned to the oracle) on four crystallites."""
+def test_config3_dense_against_oracle_fixture():
+    """The dense precision configuration at its full shape (10-spin 1H cluster, N = 1024, blocks
+    up to 252 states, MAS 20 kHz, 64 steps, 512 points) on four ZCW 1154 crystallites, against
+    the FP64 CPU oracle's FID of the same crystallites (fixture written by
+    tests/golden/make_config3_fixture.py): the FP64 form to 1e-9, the complex64 form to 1e-5
+    against both the oracle and the FP64 form; the BF16x3 form is reported against both and
+    held to its own 5e-5 bar."""
+    import os
+    fx = np.load(os.path.join(os.path.dirname(__file__), "golden", "config3_oracle_fid.npz"))
     w = workloads.config3()
+    assert str(fx["system"]) == w.system.to_text()
     rho, det = w.operators()
-    pw = w.powder().subset(np.array([0, 300, 600, 900]))
-    print(f"[c64] config3 K={len(pw)}", flush=True)
-    a, _ = engine.powder_fid(w.system, w.wr, w.rotor, w.sw, w.npts, pw, rho, det)
-    b, _ = engine.powder_fid(w.system, w.wr, w.rotor, w.sw, w.npts, pw, rho, det,
-                             precision="c64")
-    err = rel_l2(b, a)
-    print(f"[c64] config3: rel-L2 vs FP64 engine {err:.3e}", flush=True)
-    assert err < TOL_C64
+    pw = w.powder().subset(fx["idx"])
+    ref = fx["fid"]
+    print(f"[c64] config3 K={len(pw)} against the oracle fixture", flush=True)
+    f64, sw = engine.powder_fid(w.system, w.wr, w.rotor, w.sw, w.npts, pw, rho, det)
+    assert sw == float(fx["sw"][0])
+    e64 = rel_l2(f64, ref)
+    print(f"[c64] config3: FP64 form vs oracle {e64:.3e}", flush=True)
+    assert e64 < 1e-9
+    c64, _ = engine.powder_fid(w.system, w.wr, w.rotor, w.sw, w.npts, pw, rho, det,
+                               precision="c64")
+    ec, ecf = rel_l2(c64, ref), rel_l2(c64, f64)
+    print(f"[c64] config3: complex64 vs oracle {ec:.3e}, vs FP64 form {ecf:.3e}", flush=True)
+    assert ec < TOL_C64 and ecf < TOL_C64
+    b3, _ = engine.powder_fid(w.system, w.wr, w.rotor, w.sw, w.npts, pw, rho, det,
+                              precision="bf16x3")
+    eb, ebf = rel_l2(b3, ref), rel_l2(b3, f64)
+    print(f"[c64] config3: BF16x3 vs oracle {eb:.3e}, vs FP64 form {ebf:.3e}", flush=True)
+    assert eb < 5e-5 and ebf < 5e-5

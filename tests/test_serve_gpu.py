@@ -95,3 +95,35 @@ def test_bad_requests_fail_alone():
         assert x.error is None, (i, x.error)
         solo, _ = engine.powder_fid(x.system, x.wr, x.rotor, x.sw, x.npts, x.powder, x.rho, x.det)
         assert np.array_equal(x.fid, solo), i
+
+
+@pytest.mark.parametrize("precision", ["fp64", "c64"])
+def test_config5_stream_prefix(precision):
+    """The serving configuration's own seeded stream (Poisson 200/s, 2-10-spin clusters,
+    ZCW 1154/3722, 32-128 steps, 256-2048 points): its first six requests, which include a
+    9-spin one (N = 512, blocks of 126 states, ZCW 3722), served with their arrival times.  Every
+    request equals its solo run bit for bit, and the two smallest match the oracle (1e-9 in FP64,
+    1e-5 in complex64)."""
+    reqs = serve.config5_requests(6)
+    assert max(r.system.spins for r in reqs) >= 9
+    print(f"[serve] config-5 prefix {precision}: spins {[r.system.spins for r in reqs]}", flush=True)
+    srv = serve.FIDServer(streams=4, precision=precision)
+    try:
+        done = srv.run(reqs)
+    finally:
+        srv.close()
+    assert sorted(r.rid for r in done) == list(range(len(reqs)))
+    for r in done:
+        assert r.error is None, (r.rid, r.error)
+        solo, sw = engine.powder_fid(r.system, r.wr, r.rotor, r.sw, r.npts, r.powder, r.rho, r.det,
+                                     precision=precision)
+        print(f"[serve] request {r.rid} S={r.system.spins}: mix == solo "
+              f"{np.array_equal(r.fid, solo)}", flush=True)
+        assert r.sw_out == sw
+        assert np.array_equal(r.fid, solo), r.rid
+    tol = 1e-9 if precision == "fp64" else 1e-5
+    for r in sorted(done, key=lambda x: x.system.N * len(x.powder))[:2]:
+        ref, _ = O.powder_fid(r.system, r.wr, r.rotor, r.sw, r.npts, r.powder, r.rho, r.det)
+        err = rel(r.fid, ref)
+        print(f"[serve] request {r.rid} S={r.system.spins}: rel-L2 vs oracle {err:.2e}", flush=True)
+        assert err < tol

@@ -166,10 +166,31 @@ def test_unsupported_statements_raise(stmt):
         r.statement(stmt)
 
 
-def test_plain_fid_after_pulses_is_rejected_without_touching_the_engine():
-    sim = SimFile(STATIC.replace("fid()", "1H:pulse(1e-6, 0, 1000)\n    fid()"))
-    with pytest.raises(BlochLibError, match="pulses before a plain fid"):
-        sim.run()
+def test_plain_fid_after_pulses_runs_the_preparation(monkeypatch, tmp_path):
+    """Pulses before a plain fid() (sequencerun_funcs.cc:693-703): the runner hands the pulse
+    steps (maxtstep-split, main-section times, rf tables), the start time (the pulses' end) and
+    the pulses' rotor angle to the engine's preparation; the engine call is intercepted here."""
+    from blochlib_amd import engine
+    seen = {}
+
+    def fake(sy, wr, rotor, sw, npts, pw, rho, det, dt, rf=(), step_t1=None, t0=None,
+             prep_rotor_deg=None, normalize=True, precision="fp64"):
+        seen.update(wr=wr, rotor=rotor, sw=sw, npts=npts, dt=np.asarray(dt), rf=rf,
+                    t1=np.asarray(step_t1), t0=t0, prot=prep_rotor_deg, rho=rho)
+        return np.zeros(npts, complex), 40000.0
+
+    monkeypatch.setattr(engine, "prep_fid", fake)
+    mas = STATIC.replace("wr=0", "wr=10000").replace("rotor=0", "rotor=54.7356")
+    sim = SimFile(mas.replace("fid()", "1H:pulse(12e-6, 90, 25000)\n    fid()"))
+    sim.base_dir = os.path.dirname(__file__)
+    out = sim.run(outdir=str(tmp_path))
+    assert out["sw"] == 40000.0
+    assert seen["wr"] == 10000.0 and math.isclose(seen["rotor"], 54.7356)
+    # 12 us in steps of at most maxtstep = 5 us: three equal steps, from t = 0
+    assert np.allclose(seen["dt"], 4e-6) and np.allclose(seen["t1"], [0.0, 4e-6, 8e-6])
+    assert math.isclose(seen["t0"], 12e-6) and math.isclose(seen["prot"], 54.7356)
+    (iso, amp, ph, off), = seen["rf"]
+    assert iso == "1H" and np.allclose(amp, 25000.0) and np.allclose(ph, math.pi / 2)
 
 
 def test_cli_reports_errors(tmp_path, capsys):
