@@ -22,6 +22,7 @@
 #include <cstring>
 #include <functional>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <numeric>
 #include <string>
@@ -67,6 +68,14 @@ int blfid_dev_launch_prep(int K, const double *th, const double *ph, const doubl
 int blfid_dev_launch_static_h(const DevPlan *P, int Kb, int c0, const int *grp, int ngrp,
                               double *scratch, size_t slot, int nslots, double2 *bufH,
                               hipStream_t st);
+int blfid_dev_launch_state_fill(long K, int N, const double2 *rho0, double2 *S, hipStream_t st);
+int blfid_dev_launch_state_apply(const DevPlan *PP, int Kb, int c0, const double2 *Aprep,
+                                 const int *sblk, const int *sloc, int N, int repeat, double2 *S,
+                                 double2 *T, int nslots, hipStream_t st);
+int blfid_dev_launch_state_trace(int K, int N, const double2 *S, const int *nzi,
+                                 const double *nzv, int nnz, double *part, hipStream_t st);
+int blfid_dev_launch_state_gather(int prec, const DevPlan *P, int Kb, int c0, const double2 *S,
+                                  void *rho_out, hipStream_t st);
 int blfid_dev_launch_static_fid(int M, const DevPlan *P, int Kb, const int *pgrp, int ngrp,
                                 const double2 *bufW, const double *lam, double2 *scratch,
                                 size_t slot, int nslots, double w0, int prune, double *part,
@@ -169,6 +178,19 @@ struct blfid_workspace {
     bool used = false;
 };
 
+// Per-crystallite density operators of one powder, resident in HBM (blfid_state_*): K dense
+// N x N complex128 matrices, column-major, plus the powder's rotations and weights.
+struct blfid_state {
+    int S = 0, N = 0, K = 0;
+    DevBuf<double> d_th, d_ph, d_ga, d_w, d_RP;
+    DevBuf<double2> rho;   // K N^2
+    DevBuf<double2> rho0;  // staging of blfid_state_set / blfid_state_load
+    DevBuf<char> ws;       // propagators and product slots of blfid_state_apply
+    DevBuf<int> sblk, sloc, nzi;
+    DevBuf<double> nzv, part, grp, out;
+    double budget = 0;
+};
+
 struct blfid_plan {
     // input
     int S = 0, N = 0;
@@ -242,6 +264,8 @@ struct blfid_plan {
     DevBuf<int> d_prep_sblk, d_prep_sloc, d_rho_nzi;
     DevBuf<double> d_rho_nzv;
     int rho_nnz = 0;
+    // or: each crystallite's rho taken from a state (blfid_plan_set_state), not owned
+    blfid_state *state = nullptr;
     char *bufRho = nullptr;
     // working storage (carved from the plan's own arena or a shared workspace)
     DevBuf<char> own_ws;
@@ -925,6 +949,7 @@ blfid_status blfid_plan_set_prep(blfid_plan *pl, double rotor_deg, int nsteps,
     delete pl->prep;
     pl->prep = pp;
     pl->prep_rotor = rotor_deg;
+    if (pp) pl->state = nullptr; // a preparation and a state are exclusive sources of rho
     pl->have_ops = false; // the block-pair tables depend on it: operators are set again
     return BLFID_OK;
 }
@@ -959,9 +984,9 @@ blfid_status blfid_plan_set_operators(blfid_plan *pl, const double *rho_d, const
                 rosym = 0;
                 break;
             }
-    // with pulses before the FID every crystallite has its own prepared rho, which couples any
-    // pair of blocks: the general (non-rosym) form, on every block pair that D couples
-    const bool prepared = pl->prep != nullptr;
+    // with pulses before the FID (or rho from a state) every crystallite has its own rho, which
+    // couples any pair of blocks: the general (non-rosym) form, on every block pair that D couples
+    const bool prepared = pl->prep != nullptr || pl->state != nullptr;
     if (prepared) rosym = 0;
     pl->rosym = rosym;
     cplx tr = 0;
@@ -1066,7 +1091,7 @@ blfid_status blfid_plan_set_operators(blfid_plan *pl, const double *rho_d, const
     }
     pl->rt_rev = false;
     for (int v : pl->elem_rev) pl->rt_rev |= v >= 0;
-    if (prepared) {
+    if (pl->prep) {
         // rho's nonzero entries, from which each crystallite's prepared blocks are formed
         std::vector<int> nzi;
         std::vector<double> nzv;
@@ -1176,7 +1201,7 @@ static double pair_flops(const blfid_plan *pl, const std::vector<int> &g) {
     for (int p : g) {
         double bI = pl->blk_dim[pl->pairs[p].I], bJ = pl->blk_dim[pl->pairs[p].J];
         s += 8.0 * (bI * bI * bJ + (pl->rt_from_q ? 0.0 : bJ * bJ * bI));  // Q (and R)
-        if (pl->prep) s += 8.0 * bJ * bI * bI;                              // rho_c B_I
+        if (pl->prep || pl->state) s += 8.0 * bJ * bI * bI;                // rho_c B_I
     }
     return s;
 }
@@ -1223,7 +1248,7 @@ static blfid_layout::Shape shape_of(const blfid_plan *pl) {
     s.Lsum = pl->Lsum;
     s.Estr = pl->Estr;
     s.K = pl->K;
-    s.rho_elems = pl->prep ? pl->Etot : 0;
+    s.rho_elems = (pl->prep || pl->state) ? pl->Etot : 0;
     return s;
 }
 
@@ -1302,7 +1327,7 @@ static void carve(blfid_plan *pl, char *base, const size_t *off) {
     pl->scrEig = (double2 *)(base + off[R_EIG]);
     pl->bufWr = pl->prec && pl->spinning ? base + off[R_WR] : nullptr;
     pl->grp = (double *)(base + off[R_GROUP]);
-    pl->bufRho = pl->prep ? base + off[R_RHO] : nullptr;
+    pl->bufRho = (pl->prep || pl->state) ? base + off[R_RHO] : nullptr;
 }
 
 // Enqueues the whole powder on `st` with the working storage at `base` (regions at off[],
@@ -1312,6 +1337,8 @@ static blfid_status enqueue_run(blfid_plan *pl, const blfid_layout::Shape &sh, c
                                 const size_t *off, long Kb, int normalize, double *out,
                                 hipStream_t st) {
     const int K = pl->K;
+    if (pl->state && pl->state->K != K)
+        return fail(BLFID_ERR_INVALID, "the state and the plan's powder differ in size");
     pl->Kb = (int)Kb;
     carve(pl, base, off);
     refresh_devplan(pl);
@@ -1360,6 +1387,9 @@ static blfid_status enqueue_run(blfid_plan *pl, const blfid_layout::Shape &sh, c
                                                pl->thetas, kv.first, pl->scrEig, bm2,
                                                sh.eig_slots, st));
             }
+            if (pl->state)
+                LAUNCHCHK(blfid_dev_launch_state_gather(0, P, kb, c0, pl->state->rho.p,
+                                                        pl->bufRho, st));
             for (auto &kv : pl->pair_groups) {
                 PhaseScope ps(pl, 7 * 8 + kv.first, kb * (pair_flops(pl, kv.second) + 8.0 * pl->npts * pair_elems(pl, kv.second)),
                               kb * (pair_reads(pl, kv.second) * 16.0 + (double)kv.second.size() * pl->npts * 16.0), st);
@@ -1400,6 +1430,10 @@ static blfid_status enqueue_run(blfid_plan *pl, const blfid_layout::Shape &sh, c
                     (const double2 *)(pp->bufA + (size_t)((pp->n - 1) % 2) * pp->Lsum * 16),
                     pl->d_prep_sblk.p, pl->d_prep_sloc.p, pl->d_rho_nzi.p, pl->d_rho_nzv.p,
                     pl->rho_nnz, pl->bufRho, st));
+            } else if (pl->state) {
+                // each crystallite's rho from the state, on the plan's block pairs
+                LAUNCHCHK(blfid_dev_launch_state_gather(pr, P, kb, c0, pl->state->rho.p,
+                                                        pl->bufRho, st));
             }
             // complex64 matrix-core plans: blocks of <= 80 states take the fused propagator +
             // Dyson chain (U_k never leaves LDS); larger blocks, the FP64 form and the BF16x3
@@ -1486,6 +1520,9 @@ static blfid_status enqueue_run(blfid_plan *pl, const blfid_layout::Shape &sh, c
                                                (int)kv.second.size(), pl->bufU, (size_t)pl->Lsum, 1,
                                                pl->bufW, nullptr, pl->thetas, kv.first,
                                                pl->scrEig, bm2, sh.eig_slots, st));
+            if (pl->state)
+                LAUNCHCHK(blfid_dev_launch_state_gather(0, P, kb, c0, pl->state->rho.p,
+                                                        pl->bufRho, st));
             for (auto &kv : pl->pair_groups) {
                 PhaseScope ps(pl, 7 * 8 + kv.first, 0.0,
                               kb * (pair_reads(pl, kv.second) * 16.0 + (double)kv.second.size() * pl->npts * 16.0), st);
@@ -1553,6 +1590,223 @@ blfid_status blfid_plan_run(blfid_plan *pl, int normalize, void *stream) {
             return fail(BLFID_ERR_DEVICE, std::string("hipEventRecord: ") + hipGetErrorString(e));
     }
     return rs;
+}
+
+// ------------------------------------------------------------------------------ spin state
+blfid_status blfid_state_create(int nspins, int K, const double *theta, const double *phi,
+                                const double *gamma, const double *weight, blfid_state **out) {
+    if (!out) return fail(BLFID_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    if (nspins < 1 || nspins > DEV_MAX_SPINS)
+        return fail(BLFID_ERR_UNSUPPORTED, "spin count outside 1.." + std::to_string(DEV_MAX_SPINS));
+    if (K < 0 || (K > 0 && (!theta || !phi || !gamma || !weight)))
+        return fail(BLFID_ERR_INVALID, "bad powder arguments");
+    std::unique_ptr<blfid_state> st(new (std::nothrow) blfid_state());
+    if (!st) return fail(BLFID_ERR_NOMEM, "host allocation");
+    st->S = nspins;
+    st->N = 1 << nspins;
+    st->K = K;
+    const size_t NN = (size_t)st->N * st->N;
+    HIPCHK(st->rho.alloc(std::max<size_t>((size_t)K * NN, 1)));
+    HIPCHK(hipMemset(st->rho.p, 0, std::max<size_t>((size_t)K * NN, 1) * sizeof(double2)));
+    if (K > 0) {
+        HIPCHK(st->d_th.upload(std::vector<double>(theta, theta + K)));
+        HIPCHK(st->d_ph.upload(std::vector<double>(phi, phi + K)));
+        HIPCHK(st->d_ga.upload(std::vector<double>(gamma, gamma + K)));
+        HIPCHK(st->d_w.upload(std::vector<double>(weight, weight + K)));
+        HIPCHK(st->d_RP.alloc((size_t)9 * K));
+        LAUNCHCHK(blfid_dev_launch_prep(K, st->d_th.p, st->d_ph.p, st->d_ga.p, st->d_RP.p, nullptr));
+        HIPCHK(hipDeviceSynchronize());
+    }
+    *out = st.release();
+    return BLFID_OK;
+}
+
+void blfid_state_destroy(blfid_state *st) {
+    if (!st) return;
+    (void)hipDeviceSynchronize();
+    delete st;
+}
+
+int blfid_state_dim(const blfid_state *st) { return st ? st->N : 0; }
+int blfid_state_size(const blfid_state *st) { return st ? st->K : 0; }
+
+blfid_status blfid_state_set_memory_budget(blfid_state *st, double bytes) {
+    if (!st || !(bytes >= 0)) return fail(BLFID_ERR_INVALID, "bad memory budget");
+    st->budget = bytes;
+    return BLFID_OK;
+}
+
+// Fills every crystallite (all == 0: the one N x N rho; all == 1: K matrices in order).
+static blfid_status state_fill(blfid_state *st, const double *rho, int all, hipStream_t s) {
+    if (!st || !rho) return fail(BLFID_ERR_INVALID, "NULL argument");
+    const size_t NN = (size_t)st->N * st->N;
+    HIPCHK(hipStreamSynchronize(s)); // the staging buffer may still feed earlier work
+    if (all) {
+        if (st->K > 0)
+            HIPCHK(hipMemcpyAsync(st->rho.p, rho, (size_t)st->K * NN * sizeof(double2),
+                                  hipMemcpyHostToDevice, s));
+    } else {
+        HIPCHK(st->rho0.alloc(NN));
+        HIPCHK(hipMemcpy(st->rho0.p, rho, NN * sizeof(double2), hipMemcpyHostToDevice));
+        LAUNCHCHK(blfid_dev_launch_state_fill(st->K, st->N, st->rho0.p, st->rho.p, s));
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    return BLFID_OK;
+}
+
+blfid_status blfid_state_set(blfid_state *st, const double *rho, void *stream) {
+    return state_fill(st, rho, 0, (hipStream_t)stream);
+}
+
+blfid_status blfid_state_load(blfid_state *st, const double *rho_all, void *stream) {
+    return state_fill(st, rho_all, 1, (hipStream_t)stream);
+}
+
+blfid_status blfid_state_fetch(blfid_state *st, double *rho_all, void *stream) {
+    if (!st || !rho_all) return fail(BLFID_ERR_INVALID, "NULL argument");
+    hipStream_t s = (hipStream_t)stream;
+    if (st->K > 0)
+        HIPCHK(hipMemcpyAsync(rho_all, st->rho.p,
+                              (size_t)st->K * st->N * st->N * sizeof(double2),
+                              hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return BLFID_OK;
+}
+
+blfid_status blfid_state_apply(blfid_state *st, const blfid_system *sys, double wr_hz,
+                               double rotor_deg, int nsteps, const double *step_t1,
+                               const double *step_dt, int nrf, const blfid_rf *rf, int repeat,
+                               void *stream) {
+    if (!st || !sys) return fail(BLFID_ERR_INVALID, "NULL argument");
+    if (sys->nspins != st->S)
+        return fail(BLFID_ERR_INVALID, "the system and the state differ in spin count");
+    if (repeat < 0) return fail(BLFID_ERR_INVALID, "negative repeat count");
+    if (nsteps < 1 || !step_dt) return fail(BLFID_ERR_INVALID, "a propagator needs steps");
+    hipStream_t s = (hipStream_t)stream;
+    blfid_plan *ppr = nullptr;
+    blfid_status cs = create_pulse_plan(sys, wr_hz, rotor_deg, nsteps, step_t1, step_dt, 1, 0.0,
+                                        nrf, rf, BLFID_FP64, &ppr);
+    if (cs != BLFID_OK) return cs;
+    std::unique_ptr<blfid_plan> pp(ppr);
+    if (repeat == 0 || st->K == 0) return BLFID_OK;
+    const int N = st->N, K = st->K;
+    std::vector<int> sblk(N), sloc(N);
+    for (size_t b = 0; b < pp->blocks.size(); b++)
+        for (size_t a = 0; a < pp->blocks[b].states.size(); a++) {
+            sblk[pp->blocks[b].states[a]] = (int)b;
+            sloc[pp->blocks[b].states[a]] = (int)a;
+        }
+    HIPCHK(hipStreamSynchronize(s)); // tables and working storage may still feed earlier work
+    HIPCHK(st->sblk.upload(sblk));
+    HIPCHK(st->sloc.upload(sloc));
+    // working storage: the pulse plan's propagator regions for Kb crystallites (its exp scratch
+    // cut to 4096 workgroup slots: a propagation is a small job), then one N x N product slot
+    // per crystallite of the batch (at most 4096 slots)
+    blfid_layout::Shape ps = prep_shape_of(pp.get());
+    ps.exp_slots = std::min(ps.exp_slots, 4096);
+    const size_t slot_bytes = (size_t)N * N * sizeof(double2);
+    auto need = [&](long kb) {
+        return blfid_layout::align256(blfid_layout::layout_bytes(ps, kb)) +
+               (size_t)std::min<long>(kb, 4096) * slot_bytes;
+    };
+    double avail = st->budget;
+    if (avail <= 0) {
+        size_t fre = 0, tot = 0;
+        HIPCHK(hipMemGetInfo(&fre, &tot));
+        avail = 0.4 * (double)(fre + st->ws.n);
+    }
+    if ((double)need(1) > avail)
+        return fail(BLFID_ERR_NOMEM, "one crystallite's propagator does not fit the memory budget");
+    long lo = 1, hi = K;
+    while (lo < hi) {
+        const long mid = lo + (hi - lo + 1) / 2;
+        if ((double)need(mid) <= avail) lo = mid;
+        else hi = mid - 1;
+    }
+    const long Kb = lo;
+    HIPCHK(st->ws.alloc(need(Kb)));
+    size_t poff[blfid_layout::R_COUNT];
+    const size_t pbytes = blfid_layout::align256(blfid_layout::layout_bytes(ps, Kb, poff));
+    carve(pp.get(), st->ws.p, poff);
+    refresh_devplan(pp.get());
+    pp->dp.RP = st->d_RP.p;
+    const DevPlan *PP = &pp->dp;
+    double2 *T = (double2 *)(st->ws.p + pbytes);
+    const int nslots = (int)std::min<long>(Kb, 4096);
+    const size_t pbm2 = (size_t)pp->bmax * pp->bmax;
+    for (int c0 = 0; c0 < K; c0 += (int)Kb) {
+        const int kb = (int)std::min<long>(Kb, K - c0);
+        for (auto &kv : pp->blk_groups)
+            LAUNCHCHK(blfid_dev_launch_exp(0, kv.first, PP, kb, c0, pp->d_bgrp[kv.first].p,
+                                           (int)kv.second.size(), pp->scrE, pbm2, ps.exp_slots,
+                                           pp->bufU, s));
+        for (auto &kv : pp->blk_groups)
+            LAUNCHCHK(blfid_dev_launch_dyson(0, kv.first, PP, kb, pp->d_bgrp[kv.first].p,
+                                             (int)kv.second.size(), pp->bufU, pp->bufA, s));
+        LAUNCHCHK(blfid_dev_launch_state_apply(
+            PP, kb, c0, (const double2 *)(pp->bufA + (size_t)((pp->n - 1) % 2) * pp->Lsum * 16),
+            st->sblk.p, st->sloc.p, N, repeat, st->rho.p, T, nslots, s));
+    }
+    // the pulse plan's tables feed the kernels above: they finish before it is released
+    HIPCHK(hipStreamSynchronize(s));
+    return BLFID_OK;
+}
+
+blfid_status blfid_state_trace(blfid_state *st, const double *det_d, int normalize,
+                               double *out2, void *stream) {
+    if (!st || !det_d || !out2) return fail(BLFID_ERR_INVALID, "NULL argument");
+    const int N = st->N, K = st->K;
+    const cplx *det = (const cplx *)det_d;
+    std::vector<int> nzi;
+    std::vector<double> nzv;
+    cplx trdd = 0;
+    for (int j = 0; j < N; j++)
+        for (int i = 0; i < N; i++) {
+            const cplx v = det[i + (size_t)j * N];
+            trdd += v * std::conj(v);
+            if (v != 0.0) {
+                nzi.push_back(i);
+                nzi.push_back(j);
+                nzv.push_back(v.real());
+                nzv.push_back(v.imag());
+            }
+        }
+    if (normalize && trdd == 0.0) return fail(BLFID_ERR_INVALID, "Tr(det det^H) is zero");
+    out2[0] = out2[1] = 0.0;
+    if (K == 0) return BLFID_OK;
+    hipStream_t s = (hipStream_t)stream;
+    HIPCHK(hipStreamSynchronize(s));
+    HIPCHK(st->nzi.upload(nzi));
+    HIPCHK(st->nzv.upload(nzv));
+    const int G = blfid_layout::CRYST_GROUP;
+    const int ng = (int)blfid_layout::ngroups(K);
+    HIPCHK(st->part.alloc((size_t)2 * K));
+    HIPCHK(st->grp.alloc((size_t)ng * 4));
+    HIPCHK(st->out.alloc(2));
+    const cplx norm = normalize ? trdd : cplx(1.0, 0.0);
+    LAUNCHCHK(blfid_dev_launch_zero(st->grp.p, (size_t)ng * 4, s));
+    LAUNCHCHK(blfid_dev_launch_state_trace(K, N, st->rho.p, st->nzi.p, st->nzv.p,
+                                           (int)(nzv.size() / 2), st->part.p, s));
+    LAUNCHCHK(blfid_dev_launch_reduce(K, 0, 1, 1, G, st->d_w.p, 1.0, st->part.p, st->grp.p, s));
+    LAUNCHCHK(blfid_dev_launch_finish(1, ng, st->grp.p, norm.real(), norm.imag(), st->out.p, s));
+    HIPCHK(hipMemcpyAsync(out2, st->out.p, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return BLFID_OK;
+}
+
+blfid_status blfid_plan_set_state(blfid_plan *pl, blfid_state *st) {
+    if (!pl) return fail(BLFID_ERR_INVALID, "plan is NULL");
+    if (st && st->N != pl->N)
+        return fail(BLFID_ERR_INVALID, "the state and the plan differ in Hilbert dimension");
+    if (pl->last_stream) (void)hipStreamSynchronize(pl->last_stream);
+    if (st) {
+        delete pl->prep;
+        pl->prep = nullptr;
+    }
+    pl->state = st;
+    pl->have_ops = false; // the block-pair tables depend on it: operators are set again
+    return BLFID_OK;
 }
 
 // ------------------------------------------------------------------------------ no-alloc run

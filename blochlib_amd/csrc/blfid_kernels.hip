@@ -2384,6 +2384,96 @@ __global__ void __launch_bounds__(256) k_rho_prep(DevPlan P, DevPlan PP, int Kb,
     out[idx] = CX<R>::from(cconj(acc));
 }
 
+// ------------------------------------------------------------------------------ spin state
+// Per-crystallite density operators carried through a pulse program: solid-2.0 keeps one ro per
+// powder orientation and every used propagator advances it in place (subSequence::propogate,
+// sequencerun.cc:164-196, applied from SequenceRun::doFID, sequencerun_funcs.cc:625-744).
+// rho_c is N x N complex128, column-major: S[c N^2 + x + y N].
+
+// Every crystallite's rho = rho0 (N x N).
+__global__ void __launch_bounds__(256) k_state_fill(long total, long NN, const cd *__restrict__ rho0,
+                                                    cd *__restrict__ S) {
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i < total) S[i] = rho0[i % NN];
+}
+
+// rho_c <- U rho_c U^H, `repeat` times, with U the crystallite's propagator from a pulse plan
+// (FP64, block diagonal in that plan's blocks, read from Aprep as k_rho_prep reads it).  One
+// workgroup per crystallite at a time; Tw is the workgroup's N x N slot: Tw = rho U^H, then
+// rho = U Tw.  Thread e owns element (x, y) = (e mod N, e / N), so consecutive lanes read
+// consecutive rows of rho and of Tw.
+__global__ void __launch_bounds__(256) k_state_apply(DevPlan PP, int Kb, int c0,
+                                                     const cd *__restrict__ Aprep,
+                                                     const int *__restrict__ sblk,
+                                                     const int *__restrict__ sloc, int N,
+                                                     int repeat, cd *__restrict__ S,
+                                                     cd *__restrict__ T) {
+    const long NN = (long)N * N;
+    cd *Tw = T + (size_t)blockIdx.x * NN;
+    for (int c = blockIdx.x; c < Kb; c += gridDim.x) {
+        const cd *Ac = Aprep + (size_t)c * 2 * PP.Lsum; // the pulse plan keeps two product slots
+        cd *R = S + (size_t)(c0 + c) * NN;
+        for (int r = 0; r < repeat; r++) {
+            for (long e = threadIdx.x; e < NN; e += blockDim.x) {
+                const int x = (int)(e % N), y = (int)(e / N);
+                const int by = sblk[y], dy = PP.blk_dim[by], ly = sloc[y];
+                const cd *Uy = Ac + PP.blk_uoff[by];
+                const int *st = PP.blk_states + PP.blk_soff[by];
+                cd acc = cmk(0.0, 0.0);
+                for (int j = 0; j < dy; j++)
+                    cfma(acc, R[x + (size_t)st[j] * N], cconj(Uy[ly + (size_t)j * dy]));
+                Tw[e] = acc;
+            }
+            __syncthreads();
+            for (long e = threadIdx.x; e < NN; e += blockDim.x) {
+                const int x = (int)(e % N), y = (int)(e / N);
+                const int bx = sblk[x], dx = PP.blk_dim[bx], lx = sloc[x];
+                const cd *Ux = Ac + PP.blk_uoff[bx];
+                const int *st = PP.blk_states + PP.blk_soff[bx];
+                cd acc = cmk(0.0, 0.0);
+                for (int j = 0; j < dx; j++)
+                    cfma(acc, Ux[lx + (size_t)j * dx], Tw[st[j] + (size_t)y * N]);
+                R[e] = acc;
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// part[c] = Tr(rho_c det) = sum over det's nonzeros (a, b, v) of rho_c(b, a) v, one thread per
+// crystallite (SequenceRun::doFID's trace(ro, detect), sequencerun_funcs.cc:635, 694).
+__global__ void __launch_bounds__(256) k_state_trace(int K, int N, const cd *__restrict__ S,
+                                                     const int *__restrict__ nzi,
+                                                     const double *__restrict__ nzv, int nnz,
+                                                     double *__restrict__ part) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= K) return;
+    const cd *R = S + (size_t)c * N * N;
+    cd acc = cmk(0.0, 0.0);
+    for (int z = 0; z < nnz; z++) {
+        const int a = nzi[2 * z], b = nzi[2 * z + 1];
+        cfma(acc, R[b + (size_t)a * N], cmk(nzv[2 * z], nzv[2 * z + 1]));
+    }
+    part[2 * (size_t)c] = acc.x;
+    part[2 * (size_t)c + 1] = acc.y;
+}
+
+// The FID plan's per-crystallite rho blocks taken from a state: element e of crystallite c of
+// the batch holds conj(rho_{c0+c}(x, y)), x the state of row s of block J, y of column r of
+// block I -- the layout k_rho_prep writes and k_qr / k_static_fid read.
+template <typename R>
+__global__ void __launch_bounds__(256) k_state_gather(DevPlan P, int Kb, int c0,
+                                                      const cd *__restrict__ S,
+                                                      typename CX<R>::T *__restrict__ out) {
+    const long idx = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (idx >= (long)Kb * P.Etot) return;
+    const int c = (int)(idx / P.Etot);
+    const long e = idx - (long)c * P.Etot;
+    const int x = P.blk_states[P.elem_s[e]], y = P.blk_states[P.elem_r[e]];
+    const cd v = S[(size_t)(c0 + c) * P.N * P.N + x + (size_t)y * P.N];
+    out[idx] = CX<R>::from(cconj(v));
+}
+
 // ------------------------------------------------------------------------------ static path
 // H(0, dt) block of every crystallite (wr = 0: rotor phase 0), complex copy for k_eig.
 __global__ void __launch_bounds__(256) k_static_h(DevPlan P, int Kb, int c0, const int *__restrict__ grp,
@@ -2424,7 +2514,7 @@ __global__ void __launch_bounds__(NThr<M>::v) k_static_fid(DevPlan P, int Kb, co
                                                     double w0, int prune,
                                                     double *__restrict__ part) {
     constexpr int NT = NThr<M>::v;
-    __shared__ __align__(16) cd smem[2 * 16 * 16 * M];
+    __shared__ __align__(16) cd smem[GemmLds<M, cd>::ELEMS];
     const int tid = threadIdx.x;
     cd *T1 = scratch + slot * 3 * blockIdx.x;
     cd *Gq = T1 + slot, *Gr = Gq + slot;
@@ -2441,8 +2531,15 @@ __global__ void __launch_bounds__(NThr<M>::v) k_static_fid(DevPlan P, int Kb, co
         sparse_rows_times<NT, double>(P.d_rowptr + P.pair_doff[pr], P.d_col, (const cd *)P.d_val,
                                       bI, bJ, VJ, bJ, T1, (char *)smem, (int)sizeof(smem));
         wg_gemm<M, NT, cd>(bI, bJ, bI, VI, bI, true, T1, bI, false, Gq, bI, cmk(0, 0), smem);
-        sparse_rows_times<NT, double>(P.r_rowptr + P.pair_roff[pr], P.r_col, (const cd *)P.r_val,
-                                      bJ, bI, VI, bI, T1, (char *)smem, (int)sizeof(smem));
+        if (P.rho_c) {
+            // rho of this crystallite from a state: X = rho_JI^H (b_I x b_J), rho_JI V_I = X^H V_I
+            const cd *X = (const cd *)P.rho_c + (size_t)c * P.Etot + P.pair_eoff[pr];
+            wg_gemm<M, NT, cd>(bJ, bI, bI, X, bI, true, VI, bI, false, T1, bJ, cmk(0, 0), smem);
+        } else {
+            sparse_rows_times<NT, double>(P.r_rowptr + P.pair_roff[pr], P.r_col,
+                                          (const cd *)P.r_val, bJ, bI, VI, bI, T1, (char *)smem,
+                                          (int)sizeof(smem));
+        }
         wg_gemm<M, NT, cd>(bJ, bI, bJ, VJ, bJ, true, T1, bJ, false, Gr, bJ, cmk(0, 0), smem);
         // A(i,j) = Do(i,j) sig0(j,i): i in I, j in J
         for (int e = tid; e < bI * bJ; e += NT) {
@@ -2704,6 +2801,40 @@ extern "C" int blfid_dev_launch_rho_prep(int prec, const DevPlan *P, const DevPl
     else
         k_rho_prep<double><<<grid, 256, 0, st>>>(*P, *PP, Kb, Aprep, P->elem_r, P->elem_s, sblk,
                                                   sloc, nzi, nzv, nnz, (cd *)rho_out);
+    return (int)hipGetLastError();
+}
+
+extern "C" int blfid_dev_launch_state_fill(long K, int N, const cd *rho0, cd *S, hipStream_t st) {
+    const long NN = (long)N * N, total = K * NN;
+    if (total <= 0) return 0;
+    k_state_fill<<<(unsigned)((total + 255) / 256), 256, 0, st>>>(total, NN, rho0, S);
+    return (int)hipGetLastError();
+}
+
+extern "C" int blfid_dev_launch_state_apply(const DevPlan *PP, int Kb, int c0, const cd *Aprep,
+                                            const int *sblk, const int *sloc, int N, int repeat,
+                                            cd *S, cd *T, int nslots, hipStream_t st) {
+    const int grid = Kb < nslots ? Kb : nslots;
+    if (grid <= 0 || repeat <= 0) return 0;
+    k_state_apply<<<grid, 256, 0, st>>>(*PP, Kb, c0, Aprep, sblk, sloc, N, repeat, S, T);
+    return (int)hipGetLastError();
+}
+
+extern "C" int blfid_dev_launch_state_trace(int K, int N, const cd *S, const int *nzi,
+                                            const double *nzv, int nnz, double *part,
+                                            hipStream_t st) {
+    if (K <= 0) return 0;
+    k_state_trace<<<(K + 255) / 256, 256, 0, st>>>(K, N, S, nzi, nzv, nnz, part);
+    return (int)hipGetLastError();
+}
+
+extern "C" int blfid_dev_launch_state_gather(int prec, const DevPlan *P, int Kb, int c0,
+                                             const cd *S, void *rho_out, hipStream_t st) {
+    const long n = (long)Kb * P->Etot;
+    if (n <= 0) return 0;
+    const int grid = (int)((n + 255) / 256);
+    if (prec) k_state_gather<float><<<grid, 256, 0, st>>>(*P, Kb, c0, S, (cf *)rho_out);
+    else k_state_gather<double><<<grid, 256, 0, st>>>(*P, Kb, c0, S, (cd *)rho_out);
     return (int)hipGetLastError();
 }
 

@@ -104,6 +104,21 @@ def lib():
         L.blfid_plan_set_profiling.argtypes = [vp, ctypes.c_int]
         L.blfid_plan_phase_times.argtypes = [vp, _d, _i, _d]
         L.blfid_plan_phase_bytes.argtypes = [vp, _d]
+        L.blfid_state_create.argtypes = [ctypes.c_int, ctypes.c_int, _d, _d, _d, _d,
+                                         ctypes.POINTER(vp)]
+        L.blfid_state_destroy.argtypes = [vp]
+        L.blfid_state_destroy.restype = None
+        L.blfid_state_dim.argtypes = [vp]
+        L.blfid_state_size.argtypes = [vp]
+        L.blfid_state_set_memory_budget.argtypes = [vp, ctypes.c_double]
+        L.blfid_state_set.argtypes = [vp, _d, vp]
+        L.blfid_state_load.argtypes = [vp, _d, vp]
+        L.blfid_state_fetch.argtypes = [vp, _d, vp]
+        L.blfid_state_apply.argtypes = [vp, ctypes.POINTER(_System), ctypes.c_double,
+                                        ctypes.c_double, ctypes.c_int, _d, _d, ctypes.c_int,
+                                        ctypes.POINTER(_RF), ctypes.c_int, vp]
+        L.blfid_state_trace.argtypes = [vp, _d, ctypes.c_int, _d, vp]
+        L.blfid_plan_set_state.argtypes = [vp, vp]
         L.blfid_last_error.restype = ctypes.c_char_p
         L.blfid_version.restype = ctypes.c_char_p
         _lib = L
@@ -243,8 +258,14 @@ class Plan:
 
     def set_operators(self, rho, det):
         self._r = _cm(rho, self.N)
-        self._dt = _cm(det, self.N)
-        _check(lib().blfid_plan_set_operators(self.h, _p(self._r), _p(self._dt)))
+        self._det = _cm(det, self.N)
+        _check(lib().blfid_plan_set_operators(self.h, _p(self._r), _p(self._det)))
+
+    def set_state(self, state: "State | None"):
+        """Take each crystallite's rho from `state` (blfid_plan_set_state); None detaches.
+        Operators must be set afterwards (their rho is then ignored)."""
+        self._state = state
+        _check(lib().blfid_plan_set_state(self.h, state.h if state is not None else None))
 
     def set_powder(self, pw: Powder):
         self._pw = [np.ascontiguousarray(a, np.float64) for a in
@@ -319,6 +340,113 @@ class Plan:
         self.set_powder(pw)
         self.run(normalize)
         return self.fetch()
+
+
+def _system(sy: SolidSys):
+    """(_System, arrays it points into) for one SolidSys."""
+    iso, csa, dip, jc = sy.tables()
+    keep = [np.ascontiguousarray(iso, np.int32), csa if csa.size else np.zeros(1),
+            dip if dip.size else np.zeros(1), jc if jc.size else np.zeros(1)]
+    s = _System(sy.spins, _p(keep[0], _i), len(sy.csa), _p(keep[1]), len(sy.dip), _p(keep[2]),
+                len(sy.jc), _p(keep[3]))
+    return s, keep
+
+
+class State:
+    """Per-crystallite density operators of one powder in device memory (blfid_state_*): the
+    rho that solid-2.0 carries through a pulses{} program for every powder orientation
+    (sequencerun.h:268, subSequence::propogate sequencerun.cc:164-196)."""
+
+    def __init__(self, nspins: int, pw: Powder):
+        self._pw = [np.ascontiguousarray(a, np.float64) for a in
+                    (pw.theta, pw.phi, pw.gamma, pw.weight)]
+        h = ctypes.c_void_p()
+        _check(lib().blfid_state_create(int(nspins), len(pw), *[_p(a) for a in self._pw],
+                                        ctypes.byref(h)))
+        self.h = h
+        self.N = 1 << int(nspins)
+        self.K = len(pw)
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            lib().blfid_state_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set(self, rho):
+        """Every crystallite's rho = rho (ro(op))."""
+        r = _cm(rho, self.N)
+        _check(lib().blfid_state_set(self.h, _p(r), None))
+
+    def load(self, rho_all):
+        """Crystallite k's rho = rho_all[k] (K x N x N)."""
+        a = np.asarray(rho_all, dtype=np.complex128)
+        if a.shape != (self.K, self.N, self.N):
+            raise BlochLibError(f"states must be {self.K}x{self.N}x{self.N}, got {a.shape}")
+        buf = np.ascontiguousarray(np.transpose(a, (0, 2, 1))).view(np.float64)
+        _check(lib().blfid_state_load(self.h, _p(buf), None))
+
+    def fetch(self) -> np.ndarray:
+        """K x N x N array of every crystallite's rho."""
+        buf = np.zeros(2 * self.K * self.N * self.N)
+        _check(lib().blfid_state_fetch(self.h, _p(buf), None))
+        return np.transpose(buf.view(np.complex128).reshape(self.K, self.N, self.N),
+                            (0, 2, 1)).copy()
+
+    def set_memory_budget(self, nbytes: float):
+        _check(lib().blfid_state_set_memory_budget(self.h, float(nbytes)))
+
+    def apply(self, sy: SolidSys, wr: float, rotor_deg: float, step_dt, rf=(), step_t1=None,
+              repeat: int = 1):
+        """rho <- U^repeat rho U^-repeat, U the propagator of the steps (lengths step_dt,
+        starts step_t1 or consecutive from 0, rf channels (isotope, amp[n], phase_rad[n],
+        offset[n])) at every crystallite's orientation."""
+        dt = np.ascontiguousarray(np.atleast_1d(np.asarray(step_dt, np.float64)))
+        n = len(dt)
+        t1 = (None if step_t1 is None else
+              np.ascontiguousarray(np.broadcast_to(np.asarray(step_t1, np.float64), (n,))))
+        (chans, tabs), arr = _rf_channels(sy, rf, n)
+        s, keep = _system(sy)
+        _check(lib().blfid_state_apply(self.h, ctypes.byref(s), float(wr), float(rotor_deg), n,
+                                       None if t1 is None else _p(t1), _p(dt), len(chans), arr,
+                                       int(repeat), None))
+
+    def trace(self, det, normalize: bool = True) -> complex:
+        """sum_k w_k Tr(rho_k det) [/ Tr(det det^H)]."""
+        d = _cm(det, self.N)
+        out = np.zeros(2)
+        _check(lib().blfid_state_trace(self.h, _p(d), int(bool(normalize)), _p(out), None))
+        return complex(out[0], out[1])
+
+
+def state_fid(sy, wr, rotor_deg, sw, npts, pw, state: State, det, t0=0.0, normalize=True,
+              precision="fp64"):
+    """FID(powder, ro_k, det) from each crystallite's current rho (a gamma-COMPUTE or static
+    plan reading `state`); returns (fid, adjusted sw)."""
+    p = Plan(sy, wr, rotor_deg, sw, npts, t0, precision=precision)
+    try:
+        p.set_state(state)
+        f = p.powder_fid(pw, det, det, normalize)
+        return f, p.sw
+    finally:
+        p.close()
+
+
+def state_pulse_fid(sy, wr, rotor_deg, step_dt, npts, pw, state: State, det, rf=(),
+                    step_t1=None, normalize=True):
+    """Point-to-point FID of a sub-sequence from each crystallite's current rho: point m is
+    sum_k w_k Tr(U^m rho_k U^-m det) [/ Tr(det det^H)]."""
+    p = Plan.pulse(sy, wr, rotor_deg, step_dt, npts, rf, 0.0, "fp64", step_t1)
+    try:
+        p.set_state(state)
+        return p.powder_fid(pw, det, det, normalize)
+    finally:
+        p.close()
 
 
 def powder_fid(sy, wr, rotor_deg, sw, npts, pw, rho, det, t0=0.0, normalize=True,

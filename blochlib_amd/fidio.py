@@ -80,9 +80,13 @@ def read_plotter_fid(path: str):
     return d[:, 0], d[:, 1], d[:, 2] + 1j * d[:, 3], d[:, 4] + 1j * d[:, 5]
 
 
-def save_fid_binary(fid, fname: str) -> str:
+def save_fid_binary(fid, fname: str, sw1: str | None = None) -> str:
+    """``npts1 =P`` then P (re, im) doubles; with sw1, a leading ``sw1 =<sw1>`` line (the
+    indexed savefidbinary(name, i), sequencerun_funcs.cc:873-886)."""
     f = np.asarray(fid, dtype=np.complex128)
     with open(fname, "wb") as fh:
+        if sw1 is not None:
+            fh.write(f"sw1 ={sw1}\n".encode())
         fh.write(f"npts1 ={len(f)}\n".encode())
         fh.write(np.ascontiguousarray(np.stack([f.real, f.imag], axis=1)).astype("<f8").tobytes())
     return fname
@@ -91,9 +95,53 @@ def save_fid_binary(fid, fname: str) -> str:
 def read_fid_binary(fname: str) -> np.ndarray:
     with open(fname, "rb") as fh:
         head = fh.readline().decode()
+        if head.startswith("sw1"):
+            head = fh.readline().decode()
         n = int(head.split("=")[1])
         d = np.frombuffer(fh.read(16 * n), dtype="<f8")
     return d[0::2] + 1j * d[1::2]
+
+
+def save_fid2d_binary(fid2, fname: str) -> str:
+    """2-D savefidbinary (sequencerun_funcs.cc:790-808): ``npts1 =rows``, ``npts2 =cols``,
+    then the rows in order, each point as (re, im) doubles."""
+    f = np.asarray(fid2, dtype=np.complex128)
+    with open(fname, "wb") as fh:
+        fh.write(f"npts1 ={f.shape[0]}\nnpts2 ={f.shape[1]}\n".encode())
+        fh.write(np.ascontiguousarray(np.stack([f.real, f.imag], axis=-1)).astype("<f8").tobytes())
+    return fname
+
+
+def read_fid2d_binary(fname: str) -> np.ndarray:
+    with open(fname, "rb") as fh:
+        r = int(fh.readline().decode().split("=")[1])
+        c = int(fh.readline().decode().split("=")[1])
+        d = np.frombuffer(fh.read(16 * r * c), dtype="<f8")
+    return (d[0::2] + 1j * d[1::2]).reshape(r, c)
+
+
+def save_fid2d_text(fid2, fname: str) -> str:
+    """2-D savefid / savefidtext (sequencerun_funcs.cc:809-820): ``npts1 =rows``, ``npts2
+    =cols``, then one line ``p h re im`` per point, rows in order (an existing file is
+    replaced)."""
+    f = np.asarray(fid2, dtype=np.complex128)
+    lines = [f"npts1 ={f.shape[0]}", f"npts2 ={f.shape[1]}"]
+    for p in range(f.shape[0]):
+        for h in range(f.shape[1]):
+            lines.append(f"{p} {h} {_fmt(f[p, h].real)} {_fmt(f[p, h].imag)}")
+    with open(fname, "w") as fh:
+        fh.write("\n".join(lines) + "\n")
+    return fname
+
+
+def read_fid2d_text(fname: str) -> np.ndarray:
+    with open(fname) as fh:
+        r = int(fh.readline().split("=")[1])
+        c = int(fh.readline().split("=")[1])
+        d = np.loadtxt(fh, ndmin=2)
+    out = np.zeros((r, c), dtype=np.complex128)
+    out[d[:, 0].astype(int), d[:, 1].astype(int)] = d[:, 2] + 1j * d[:, 3]
+    return out
 
 
 # MAT-file v5 element types / classes
@@ -107,21 +155,44 @@ def _element(dtype: int, payload: bytes) -> bytes:
     return struct.pack("<ii", dtype, len(payload)) + payload + b"\0" * pad
 
 
-def save_fid_matlab(fid, fname: str, var: str = "vdat") -> str:
-    """MAT v5 file with one complex double column vector named `var`."""
-    f = np.asarray(fid, dtype=np.complex128)
+def _mat_matrix(name: str, a) -> bytes:
+    """One miMATRIX element: a complex array (FIDs, written complex even when real), or a real
+    scalar; a vector is a column, a 2-D array keeps its rows x cols, data column-major."""
+    cplx = np.iscomplexobj(a)
+    v = np.asarray(a, dtype=np.complex128 if cplx else np.float64)
+    if v.ndim == 0:
+        v = v.reshape(1, 1)
+    elif v.ndim == 1:
+        v = v.reshape(-1, 1)
+    rows, cols = v.shape
+    col = v.ravel(order="F")
+    flags = _mxDOUBLE_CLASS | (_COMPLEX_FLAG if cplx else 0)
+    body = (_element(_miUINT32, struct.pack("<II", flags, 0))
+            + _element(_miINT32, struct.pack("<ii", rows, cols))
+            + _element(_miINT8, name.encode())
+            + _element(_miDOUBLE, np.real(col).astype("<f8").tobytes()))
+    if cplx:
+        body += _element(_miDOUBLE, np.imag(col).astype("<f8").tobytes())
+    return struct.pack("<ii", _miMATRIX, len(body)) + body
+
+
+def save_matlab(fname: str, variables: dict) -> str:
+    """MAT v5 file with the given variables in order (matstream::put, src/utils/matlab5.h:
+    682-745): complex arrays (vectors as columns) and real scalars."""
     hdr = ("MATLAB 5.0 MAT-file, Platform: " + os.uname().sysname + ", Created on: "
            + time.ctime() + ", Generated by blochlib_amd")
     hdr = hdr.encode()[:116].ljust(116, b" ")
-    header = hdr + b"\0" * 8 + struct.pack("<H", 0x0100) + b"IM"
-    body = (_element(_miUINT32, struct.pack("<II", _mxDOUBLE_CLASS | _COMPLEX_FLAG, 0))
-            + _element(_miINT32, struct.pack("<ii", len(f), 1))
-            + _element(_miINT8, var.encode())
-            + _element(_miDOUBLE, f.real.astype("<f8").tobytes())
-            + _element(_miDOUBLE, f.imag.astype("<f8").tobytes()))
+    out = hdr + b"\0" * 8 + struct.pack("<H", 0x0100) + b"IM"
+    for name, val in variables.items():
+        out += _mat_matrix(name, val)
     with open(fname, "wb") as fh:
-        fh.write(header + struct.pack("<ii", _miMATRIX, len(body)) + body)
+        fh.write(out)
     return fname
+
+
+def save_fid_matlab(fid, fname: str, var: str = "vdat") -> str:
+    """MAT v5 file with one complex double column vector named `var`."""
+    return save_matlab(fname, {var: np.asarray(fid, dtype=np.complex128)})
 
 
 def read_fid_matlab(fname: str) -> dict:
@@ -148,10 +219,12 @@ def read_fid_matlab(fname: str) -> dict:
         rows, cols = struct.unpack_from("<ii", parts[1][1])
         name = parts[2][1].decode()
         re = np.frombuffer(parts[3][1], dtype="<f8")
-        v = re.astype(np.complex128)
+        v = re.copy()
         if flags & _COMPLEX_FLAG:
-            v = v + 1j * np.frombuffer(parts[4][1], dtype="<f8")
-        out[name] = v.reshape(cols, rows).T if cols > 1 else v
+            v = re + 1j * np.frombuffer(parts[4][1], dtype="<f8")
+        out[name] = v.reshape(cols, rows).T if cols > 1 else (v[0] if rows == 1 and cols == 1
+                                                              and not flags & _COMPLEX_FLAG
+                                                              else v)
     return out
 
 

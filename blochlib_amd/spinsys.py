@@ -126,6 +126,65 @@ class SolidSys:
             if not (0 <= j[1] < S and 0 <= j[2] < S) or j[1] == j[2]:
                 raise BlochLibError(f"J ({j[1]},{j[2]}) invalid")
 
+    def copy(self) -> "SolidSys":
+        return SolidSys(list(self.isotopes), list(self.csa), list(self.dip), list(self.jc))
+
+    def set_spin_param(self, what: str, value: float):
+        """alterSys(what, value): SolidSys::setSpinParam (src/QMspins/sys.cc:433-560) with its
+        availability test (sys.cc:217-280).  'C<i>' and 'C<i>iso' set the isotropic shift of
+        the CSA on spin i, 'C<i>del' / 'C<i>eta' / 'C<i>alpha' / 'C<i>beta' / 'C<i>gamma' (the
+        third character decides) the other tensor values (csa.cc:46-116); 'D<i><j>' the
+        dipolar coupling's del, 'J<i><j>' the J coupling's iso (dip.cc:48-57, j.cc:42-49);
+        spin indices are single digits.  As the reference, a D or J change raises when the
+        last coupling of its list is not the one named."""
+        def avail():
+            if len(what) < 2 or not what[1].isdigit():
+                return False
+            if what[0] == "C":
+                return any(c[3] == int(what[1]) for c in self.csa)
+            if what[0] in "DJ":
+                if len(what) < 3 or not what[2].isdigit():
+                    return False
+                a, b = int(what[1]), int(what[2])
+                lst = self.dip if what[0] == "D" else self.jc
+                return any((x[1] == a and x[2] == b) or (x[1] == b and x[2] == a) for x in lst)
+            return False
+
+        if not avail():
+            raise BlochLibError(f" Your spin parameter {what} Ain't no good")
+        v = float(value)
+        if what[0] == "C":
+            on = int(what[1])
+            field_of = {"e": 2, "d": 1, "i": 0, "a": 4, "b": 5, "g": 6}
+            if len(what) >= 3:
+                if what[2] not in field_of:
+                    if len(what) >= 6 and what[2:5] == "sig" and what[5] in "123":
+                        raise BlochLibError(f"alterSys({what}) (principal values) is not "
+                                            "supported by this engine")
+                    return  # the reference sets nothing
+                k = field_of[what[2]]
+            else:
+                k = 0
+            if k == 2 and (v > 1.0 or v < 0.0):
+                raise BlochLibError(" 'eta' set to a negative or value >1...")
+            for i, c in enumerate(self.csa):
+                if c[3] == on:
+                    c = list(c)
+                    c[k] = v
+                    self.csa[i] = tuple(c)
+                    return
+            return
+        a, b = int(what[1]), int(what[2])
+        lst = self.dip if what[0] == "D" else self.jc
+        for i, x in enumerate(lst):
+            if (x[1] == a and x[2] == b) or (x[1] == b and x[2] == a):
+                x = list(x)
+                x[0] = v
+                lst[i] = tuple(x)
+            elif i + 1 == len(lst):
+                kind = "Dipole" if what[0] == "D" else "J"
+                raise BlochLibError(f"no {kind} coupling exsits between spin {a} and {b}")
+
     def to_text(self) -> str:
         out = [f"numspin {self.spins}"]
         out += [f"T {n} {i}" for i, n in enumerate(self.isotopes)]

@@ -132,6 +132,54 @@ blfid_status blfid_plan_set_prep(blfid_plan *plan, double rotor_deg, int nsteps,
                                  const double *step_t1, const double *step_dt, int nrf,
                                  const blfid_rf *rf);
 
+/* Spin state: the density operator of every crystallite of one powder, resident in device
+ * memory, carried through a pulse program.  Replaces SequenceRun's per-orientation `ro`
+ * (examples/solid-2.0/src/sequencerun.h:268, reset at every powder point,
+ * sequencerun.cc:512-527) and its in-place propagation by the used sub-sequences
+ * (subSequence::propogate, sequencerun.cc:164-196, called from SequenceRun::doFID,
+ * sequencerun_funcs.cc:625-744); ro(op) is blfid_state_set (setRo, sequencerun_funcs.cc:
+ * 315-331), a used propagator is blfid_state_apply, fid(i)'s single point is blfid_state_trace
+ * (sequencerun_funcs.cc:635, 694), and an FID from the current rho is a plan run after
+ * blfid_plan_set_state.  Crystallite k has angles (theta[k], phi[k], gamma[k]) and weight w[k].
+ * Every state call returns after its work on `stream` is done; a plan run that reads the state
+ * must finish before the state is changed (use one stream).  The state holds K N^2 complex128. */
+typedef struct blfid_state blfid_state;
+blfid_status blfid_state_create(int nspins, int K, const double *theta, const double *phi,
+                                const double *gamma, const double *weight, blfid_state **out);
+void blfid_state_destroy(blfid_state *state);
+int blfid_state_dim(const blfid_state *state);  /* N */
+int blfid_state_size(const blfid_state *state); /* K */
+/* Device bytes blfid_state_apply may use for its propagators (0: 40% of the free memory). */
+blfid_status blfid_state_set_memory_budget(blfid_state *state, double bytes);
+/* Every crystallite's rho = rho (N x N, column-major, interleaved). */
+blfid_status blfid_state_set(blfid_state *state, const double *rho, void *stream);
+/* Crystallite k's rho = rho_all[k] (K consecutive N x N matrices); fetch is the reverse. */
+blfid_status blfid_state_load(blfid_state *state, const double *rho_all, void *stream);
+blfid_status blfid_state_fetch(blfid_state *state, double *rho_all, void *stream);
+/* rho_k <- U_k^repeat rho_k U_k^-repeat for every crystallite k, U_k the propagator of the
+ * `nsteps` steps (lengths step_dt, start times step_t1 or consecutive from 0 when NULL, rf
+ * channels as in blfid_plan_create_pulse, rotor angle rotor_deg, spinning speed wr_hz) of
+ * `sys` at crystallite k's orientation, computed in FP64 (Propogation::propogator,
+ * propogation.cc:240-304; a sub-sequence used `repeat` times applies the same U repeat times,
+ * subSequence::propogate, sequencerun.cc:183-195).  repeat == 0 changes nothing. */
+blfid_status blfid_state_apply(blfid_state *state, const blfid_system *sys, double wr_hz,
+                               double rotor_deg, int nsteps, const double *step_t1,
+                               const double *step_dt, int nrf, const blfid_rf *rf, int repeat,
+                               void *stream);
+/* out2 = sum_k w_k Tr(rho_k det) [/ Tr(det det^H) when normalize], summed in crystallite order
+ * in groups as the plan's powder sum (sequencerun_funcs.cc:635, 694: fid1D[idx] +=
+ * weight trace(ro, detect) / trace(detect, adjoint(detect))). */
+blfid_status blfid_state_trace(blfid_state *state, const double *det, int normalize,
+                               double *out2, void *stream);
+/* The plan's FID starts from the state's rho_k for crystallite k of the plan's powder (same K;
+ * the same orientations are the caller's to keep) instead of the plan's rho: a gamma-COMPUTE,
+ * static or pulse-program FID from each crystallite's current rho (sequencerun_funcs.cc:
+ * 699-735).  rho passed to blfid_plan_set_operators is then ignored except for its size; the
+ * rho == (det + det^H)/2 shortcut is not taken.  Detaches a preparation (blfid_plan_set_prep);
+ * NULL detaches the state.  The state must outlive the plan's runs.  Operators must be set
+ * (again) after this call. */
+blfid_status blfid_plan_set_state(blfid_plan *plan, blfid_state *state);
+
 /* Adjusted sweep width n*wr (compute.h:96, oneFID.h:429), steps per rotor period n, N. */
 double blfid_plan_sw(const blfid_plan *plan);
 int blfid_plan_steps(const blfid_plan *plan);

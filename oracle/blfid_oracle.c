@@ -708,6 +708,7 @@ typedef struct {
     orc_spin_cfg sc;
     orc_static_cfg tc;
     const orc_prep *prep; /* NULL: rho as given */
+    const cplx *rho_all;  /* non-NULL: crystallite k's rho is rho_all + k N^2 */
     const double *theta, *phi, *gamma, *wt;
     int k0, k1, P;
     cplx *acc; /* P */
@@ -762,7 +763,28 @@ static void *job_run(void *arg) {
     }
     for (int k = jb->k0; k < jb->k1; k++) {
         int st;
-        if (jb->prep) {
+        if (jb->rho_all) {
+            /* the crystallite's own rho, with the reference's exact isroSYM test on it */
+            const int N = jb->spinning ? jb->sc.sy->N : jb->tc.sy->N;
+            const cplx *rk = jb->rho_all + (size_t)k * N * N;
+            if (jb->spinning) {
+                orc_spin_cfg cf = jb->sc;
+                int sym = 1;
+                for (int i = 0; i < N && sym; i++)
+                    for (int j = 0; j < N; j++)
+                        if (EL(rk, i, j, N) != 0.5 * (EL(cf.det, i, j, N) + conj(EL(cf.det, j, i, N)))) {
+                            sym = 0;
+                            break;
+                        }
+                cf.rho = rk;
+                cf.rosym = sym;
+                st = spinning_fid_one(&cf, jb->theta[k], jb->phi[k], jb->gamma[k], f);
+            } else {
+                orc_static_cfg tf = jb->tc;
+                tf.rho = rk;
+                st = static_fid_one(&tf, jb->theta[k], jb->phi[k], jb->gamma[k], f);
+            }
+        } else if (jb->prep) {
             orc_spin_cfg cf = jb->sc;
             st = prep_rho(jb->prep, cf.sy, jb->theta[k], jb->phi[k], jb->gamma[k], cf.rho, cf.det,
                           rc, &cf.rosym);
@@ -787,7 +809,7 @@ static void *job_run(void *arg) {
  *   rho/det: N x N complex, column-major, interleaved re/im
  *   normalize: 1 -> divide by Tr(det det^H) (sequencerun_funcs.cc:698-702); 0 -> oneFID.h:396-402
  *   fid_out: 2*npts doubles; sw_out: adjusted sweep width (compute.h:96) */
-static int powder_fid_impl(const orc_prep *prep, int S, const int *isotopes, int ncsa, const double *csa, int ndip,
+static int powder_fid_impl(const orc_prep *prep, const double *rho_all, int S, const int *isotopes, int ncsa, const double *csa, int ndip,
                    const double *dip, int nj, const double *jc, double wr, double rotor_deg,
                    double sw, int npts, double t0, int K, const double *theta, const double *phi,
                    const double *gamma, const double *w, const double *rho, const double *det,
@@ -823,6 +845,7 @@ static int powder_fid_impl(const orc_prep *prep, int S, const int *isotopes, int
     if (!jobs || !th || !accs) { free(jobs); free(th); free(accs); return ORC_ENOMEM; }
     for (int t = 0; t < nthreads; t++) {
         jobs[t].spinning = spinning; jobs[t].sc = sc; jobs[t].tc = tc; jobs[t].prep = prep;
+        jobs[t].rho_all = (const cplx *)rho_all;
         jobs[t].theta = theta; jobs[t].phi = phi; jobs[t].gamma = gamma; jobs[t].wt = w;
         jobs[t].k0 = (int)((long)K * t / nthreads); jobs[t].k1 = (int)((long)K * (t + 1) / nthreads);
         jobs[t].P = npts; jobs[t].acc = accs + (size_t)t * (npts > 0 ? npts : 1);
@@ -851,7 +874,7 @@ int orc_powder_fid(int S, const int *isotopes, int ncsa, const double *csa, int 
                    double sw, int npts, double t0, int K, const double *theta, const double *phi,
                    const double *gamma, const double *w, const double *rho, const double *det,
                    int normalize, int nthreads, double *fid_out, double *sw_out) {
-    return powder_fid_impl(NULL, S, isotopes, ncsa, csa, ndip, dip, nj, jc, wr, rotor_deg, sw,
+    return powder_fid_impl(NULL, NULL, S, isotopes, ncsa, csa, ndip, dip, nj, jc, wr, rotor_deg, sw,
                            npts, t0, K, theta, phi, gamma, w, rho, det, normalize, nthreads,
                            fid_out, sw_out);
 }
@@ -871,7 +894,7 @@ int orc_prep_fid(int S, const int *isotopes, int ncsa, const double *csa, int nd
                  double *fid_out, double *sw_out) {
     if (wr == 0.0 || rotor_deg == 0.0 || nsteps < 1 || !step_dt) return ORC_EINVAL;
     orc_prep pp = {wr, prep_rotor_deg * DEG2RAD, nsteps, nrf, step_t1, step_dt, rf};
-    return powder_fid_impl(&pp, S, isotopes, ncsa, csa, ndip, dip, nj, jc, wr, rotor_deg, sw,
+    return powder_fid_impl(&pp, NULL, S, isotopes, ncsa, csa, ndip, dip, nj, jc, wr, rotor_deg, sw,
                            npts, t0, K, theta, phi, gamma, w, rho, det, normalize, nthreads,
                            fid_out, sw_out);
 }
@@ -919,6 +942,7 @@ typedef struct {
     int nsteps, npts, nrf;
     const double *step_t1, *step_dt, *rf;
     const cplx *rho, *det;
+    const cplx *rho_all; /* non-NULL: crystallite c starts from rho_all + c N^2 */
     const double *theta, *phi, *gamma, *wt;
     int k0, k1, status;
     cplx *acc;
@@ -983,7 +1007,7 @@ static void *pulse_job_run(void *arg) {
             jb->status = st;
             goto done;
         }
-        memcpy(r, jb->rho, sizeof(cplx) * NN);
+        memcpy(r, jb->rho_all ? jb->rho_all + (size_t)c * NN : jb->rho, sizeof(cplx) * NN);
         for (int m = 0; m < jb->npts; m++) {
             cplx tr = 0;
             for (int i = 0; i < N; i++)
@@ -1000,7 +1024,7 @@ done:
     return NULL;
 }
 
-int orc_pulse_fid(int S, const int *isotopes, int ncsa, const double *csa, int ndip,
+static int pulse_fid_impl(const double *rho_all, int S, const int *isotopes, int ncsa, const double *csa, int ndip,
                   const double *dip, int nj, const double *jc, double wr, double rotor_deg,
                   int nsteps, const double *step_t1, const double *step_dt, int npts,
                   double t0, int nrf,
@@ -1026,7 +1050,7 @@ int orc_pulse_fid(int S, const int *isotopes, int ncsa, const double *csa, int n
         jb->tau = step_dt ? 0.0 : (1.0 / wr) / nsteps;
         jb->nsteps = nsteps; jb->npts = npts; jb->nrf = nrf;
         jb->step_t1 = step_t1; jb->step_dt = step_dt; jb->rf = rf;
-        jb->rho = (const cplx *)rho; jb->det = D;
+        jb->rho = (const cplx *)rho; jb->det = D; jb->rho_all = (const cplx *)rho_all;
         jb->theta = theta; jb->phi = phi; jb->gamma = gamma; jb->wt = w;
         jb->k0 = (int)((long)K * t / nthreads); jb->k1 = (int)((long)K * (t + 1) / nthreads);
         jb->acc = accs + (size_t)t * P1;
@@ -1047,6 +1071,122 @@ int orc_pulse_fid(int S, const int *isotopes, int ncsa, const double *csa, int n
         fid_out[2 * m + 1] = cimag(v);
     }
     free(jobs); free(th); free(accs);
+    return st;
+}
+
+int orc_pulse_fid(int S, const int *isotopes, int ncsa, const double *csa, int ndip,
+                  const double *dip, int nj, const double *jc, double wr, double rotor_deg,
+                  int nsteps, const double *step_t1, const double *step_dt, int npts,
+                  double t0, int nrf,
+                  const double *rf, int K, const double *theta, const double *phi,
+                  const double *gamma, const double *w, const double *rho, const double *det,
+                  int normalize, int nthreads, double *fid_out) {
+    return pulse_fid_impl(NULL, S, isotopes, ncsa, csa, ndip, dip, nj, jc, wr, rotor_deg, nsteps,
+                          step_t1, step_dt, npts, t0, nrf, rf, K, theta, phi, gamma, w, rho, det,
+                          normalize, nthreads, fid_out);
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* per-crystallite rho carried through a pulse program (solid-2.0's ro per powder point) */
+/* ------------------------------------------------------------------------------------ */
+/* The FID of a pulse sub-sequence (as orc_pulse_fid) from crystallite c's own rho,
+ * rho_all + c N^2 (column-major, interleaved). */
+int orc_state_pulse_fid(int S, const int *isotopes, int ncsa, const double *csa, int ndip,
+                        const double *dip, int nj, const double *jc, double wr, double rotor_deg,
+                        int nsteps, const double *step_t1, const double *step_dt, int npts,
+                        double t0, int nrf, const double *rf, int K, const double *theta,
+                        const double *phi, const double *gamma, const double *w,
+                        const double *rho_all, const double *det, int normalize, int nthreads,
+                        double *fid_out) {
+    if (!rho_all) return ORC_EINVAL;
+    return pulse_fid_impl(rho_all, S, isotopes, ncsa, csa, ndip, dip, nj, jc, wr, rotor_deg,
+                          nsteps, step_t1, step_dt, npts, t0, nrf, rf, K, theta, phi, gamma, w,
+                          rho_all, det, normalize, nthreads, fid_out);
+}
+
+/* The gamma-COMPUTE (spinning) or static FID (as orc_powder_fid) from crystallite c's own
+ * rho (sequencerun_funcs.cc:699-735: hFID.FID(ro, detect, ...) with the propagated ro). */
+int orc_state_fid(int S, const int *isotopes, int ncsa, const double *csa, int ndip,
+                  const double *dip, int nj, const double *jc, double wr, double rotor_deg,
+                  double sw, int npts, double t0, int K, const double *theta, const double *phi,
+                  const double *gamma, const double *w, const double *rho_all, const double *det,
+                  int normalize, int nthreads, double *fid_out, double *sw_out) {
+    if (!rho_all) return ORC_EINVAL;
+    return powder_fid_impl(NULL, rho_all, S, isotopes, ncsa, csa, ndip, dip, nj, jc, wr,
+                           rotor_deg, sw, npts, t0, K, theta, phi, gamma, w, rho_all, det,
+                           normalize, nthreads, fid_out, sw_out);
+}
+
+/* rho_c <- U_c^repeat rho_c U_c^-repeat for every crystallite, U_c the propagator of the
+ * steps at crystallite c's orientation (Propogation::propogator, propogation.cc:240-304; a
+ * sub-sequence used `repeat` times applies its U that many times, sequencerun.cc:183-195). */
+typedef struct {
+    const orc_sys *sy;
+    double wr, rotb_rad;
+    int nsteps, nrf, repeat;
+    const double *step_t1, *step_dt, *rf;
+    const double *theta, *phi, *gamma;
+    cplx *rho_all;
+    int k0, k1, status;
+} orc_apply_job;
+
+static void *apply_job_run(void *arg) {
+    orc_apply_job *jb = (orc_apply_job *)arg;
+    const int N = jb->sy->N;
+    const size_t NN = (size_t)N * N;
+    cplx *H = malloc(sizeof(cplx) * NN), *X = malloc(sizeof(cplx) * NN);
+    cplx *U = malloc(sizeof(cplx) * NN), *T = malloc(sizeof(cplx) * NN);
+    if (!H || !X || !U || !T) {
+        jb->status = ORC_ENOMEM;
+        goto done;
+    }
+    for (int c = jb->k0; c < jb->k1; c++) {
+        double cp[9];
+        rot3(jb->phi[c], jb->theta[c], jb->gamma[c], cp);
+        int st = pulse_propagator(jb->sy, cp, jb->wr, jb->rotb_rad, 0.0, 0.0, jb->nsteps,
+                                  jb->step_t1, jb->step_dt, jb->rf, jb->nrf, U, H, X, T);
+        if (st) {
+            jb->status = st;
+            goto done;
+        }
+        cplx *r = jb->rho_all + (size_t)c * NN;
+        for (int m = 0; m < jb->repeat; m++) {
+            zgemm(N, 'N', 'C', r, U, T); /* r U^H */
+            zgemm(N, 'N', 'N', U, T, r); /* U r U^H */
+        }
+    }
+done:
+    free(H); free(X); free(U); free(T);
+    return NULL;
+}
+
+int orc_state_apply(int S, const int *isotopes, int ncsa, const double *csa, int ndip,
+                    const double *dip, int nj, const double *jc, double wr, double rotor_deg,
+                    int nsteps, const double *step_t1, const double *step_dt, int nrf,
+                    const double *rf, int repeat, int K, const double *theta, const double *phi,
+                    const double *gamma, double *rho_all, int nthreads) {
+    if (S < 1 || S > 12 || nsteps < 1 || !step_dt || repeat < 0 || K < 0) return ORC_EINVAL;
+    orc_sys sy = {S, 1 << S, isotopes, ncsa, csa, ndip, dip, nj, jc};
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > K && K > 0) nthreads = K;
+    if (K == 0) return ORC_OK;
+    orc_apply_job *jobs = calloc(nthreads, sizeof(orc_apply_job));
+    pthread_t *th = calloc(nthreads, sizeof(pthread_t));
+    if (!jobs || !th) { free(jobs); free(th); return ORC_ENOMEM; }
+    for (int t = 0; t < nthreads; t++) {
+        orc_apply_job *jb = &jobs[t];
+        jb->sy = &sy; jb->wr = wr; jb->rotb_rad = rotor_deg * DEG2RAD;
+        jb->nsteps = nsteps; jb->nrf = nrf; jb->repeat = repeat;
+        jb->step_t1 = step_t1; jb->step_dt = step_dt; jb->rf = rf;
+        jb->theta = theta; jb->phi = phi; jb->gamma = gamma; jb->rho_all = (cplx *)rho_all;
+        jb->k0 = (int)((long)K * t / nthreads); jb->k1 = (int)((long)K * (t + 1) / nthreads);
+    }
+    for (int t = 1; t < nthreads; t++) pthread_create(&th[t], NULL, apply_job_run, &jobs[t]);
+    apply_job_run(&jobs[0]);
+    for (int t = 1; t < nthreads; t++) pthread_join(th[t], NULL);
+    int st = ORC_OK;
+    for (int t = 0; t < nthreads; t++) if (jobs[t].status) st = jobs[t].status;
+    free(jobs); free(th);
     return st;
 }
 

@@ -53,6 +53,19 @@ def lib():
                                     ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                     _d, _d, ctypes.c_double, ctypes.c_int, _d, ctypes.c_int]
                                    + [_d] * 6 + [ctypes.c_int, ctypes.c_int, _d, _d])
+        L.orc_state_apply.argtypes = ([ctypes.c_int, _i, ctypes.c_int, _d, ctypes.c_int, _d,
+                                       ctypes.c_int, _d, ctypes.c_double, ctypes.c_double,
+                                       ctypes.c_int, _d, _d, ctypes.c_int, _d, ctypes.c_int,
+                                       ctypes.c_int, _d, _d, _d, _d, ctypes.c_int])
+        L.orc_state_fid.argtypes = ([ctypes.c_int, _i, ctypes.c_int, _d, ctypes.c_int, _d,
+                                     ctypes.c_int, _d, ctypes.c_double, ctypes.c_double,
+                                     ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_int]
+                                    + [_d] * 6 + [ctypes.c_int, ctypes.c_int, _d, _d])
+        L.orc_state_pulse_fid.argtypes = ([ctypes.c_int, _i, ctypes.c_int, _d, ctypes.c_int, _d,
+                                           ctypes.c_int, _d, ctypes.c_double, ctypes.c_double,
+                                           ctypes.c_int, _d, _d, ctypes.c_int, ctypes.c_double,
+                                           ctypes.c_int, _d, ctypes.c_int]
+                                          + [_d] * 6 + [ctypes.c_int, ctypes.c_int, _d])
         L.orc_hamiltonian.argtypes = [ctypes.c_int, _i, ctypes.c_int, _d, ctypes.c_int, _d,
                                       ctypes.c_int, _d] + [ctypes.c_double] * 5 + [_d]
         L.orc_eig_herm_x.argtypes = [ctypes.c_int, _d, _d, _d]
@@ -219,3 +232,81 @@ def prep_fid(sy, wr, rotor_deg, sw, npts, powder, rho, det, step_dt, step_t1=Non
     if st != 0:
         raise RuntimeError(f"oracle failed with status {st}")
     return out[: 2 * npts].view(np.complex128).copy(), float(swo[0])
+
+
+# ------------------------------------------------------------------ per-crystallite rho
+def _states_cm(rho_all):
+    """K x N x N complex -> K consecutive column-major matrices as float64."""
+    a = np.asarray(rho_all, dtype=np.complex128)
+    return np.ascontiguousarray(np.transpose(a, (0, 2, 1))).view(np.float64).ravel()
+
+
+def _powder_arrays(powder):
+    return [np.ascontiguousarray(x, np.float64) for x in
+            (powder.theta, powder.phi, powder.gamma, powder.weight)]
+
+
+def state_apply(sy, wr, rotor_deg, step_dt, powder, rho_all, rf=(), step_t1=None, repeat=1,
+                nthreads=None):
+    """rho_k <- U_k^repeat rho_k U_k^-repeat for every crystallite k of `powder`; returns the
+    new K x N x N array."""
+    L = lib()
+    iso, nc, csa, nd, dip, nj, jc = _sys(sy)
+    dts = np.ascontiguousarray(np.atleast_1d(np.asarray(step_dt, np.float64)))
+    n = len(dts)
+    tab, nrf = rf_table(sy, list(rf), n)
+    t1s = (None if step_t1 is None else
+           np.ascontiguousarray(np.broadcast_to(np.asarray(step_t1, np.float64), (n,))))
+    buf = _states_cm(rho_all).copy()
+    th, ph, ga, _w = _powder_arrays(powder)
+    st = L.orc_state_apply(sy.spins, _p(iso, _i), nc, _p(csa), nd, _p(dip), nj, _p(jc),
+                           float(wr), float(rotor_deg), n, None if t1s is None else _p(t1s),
+                           _p(dts), nrf, _p(tab), int(repeat), len(th), _p(th), _p(ph), _p(ga),
+                           _p(buf), int(nthreads or os.cpu_count() or 1))
+    if st != 0:
+        raise RuntimeError(f"oracle failed with status {st}")
+    K, N = len(th), sy.N
+    return np.transpose(buf.view(np.complex128).reshape(K, N, N), (0, 2, 1)).copy()
+
+
+def state_fid(sy, wr, rotor_deg, sw, npts, powder, rho_all, det, t0=0.0, normalize=True,
+              nthreads=None):
+    """gamma-COMPUTE or static powder FID from each crystallite's own rho; (fid, sw)."""
+    L = lib()
+    iso, nc, csa, nd, dip, nj, jc = _sys(sy)
+    buf = _states_cm(rho_all)
+    d = _cm(det)
+    out = np.zeros(2 * max(npts, 1))
+    swo = np.zeros(1)
+    th, ph, ga, w = _powder_arrays(powder)
+    st = L.orc_state_fid(sy.spins, _p(iso, _i), nc, _p(csa), nd, _p(dip), nj, _p(jc), float(wr),
+                         float(rotor_deg), float(sw), int(npts), float(t0), len(th), _p(th),
+                         _p(ph), _p(ga), _p(w), _p(buf), _p(d), int(bool(normalize)),
+                         int(nthreads or os.cpu_count() or 1), _p(out), _p(swo))
+    if st != 0:
+        raise RuntimeError(f"oracle failed with status {st}")
+    return out[: 2 * npts].view(np.complex128).copy(), float(swo[0])
+
+
+def state_pulse_fid(sy, wr, rotor_deg, step_dt, npts, powder, rho_all, det, rf=(),
+                    step_t1=None, normalize=True, nthreads=None):
+    """Point-to-point FID of a sub-sequence from each crystallite's own rho."""
+    L = lib()
+    iso, nc, csa, nd, dip, nj, jc = _sys(sy)
+    dts = np.ascontiguousarray(np.atleast_1d(np.asarray(step_dt, np.float64)))
+    n = len(dts)
+    tab, nrf = rf_table(sy, list(rf), n)
+    t1s = (None if step_t1 is None else
+           np.ascontiguousarray(np.broadcast_to(np.asarray(step_t1, np.float64), (n,))))
+    buf = _states_cm(rho_all)
+    d = _cm(det)
+    out = np.zeros(2 * max(npts, 1))
+    th, ph, ga, w = _powder_arrays(powder)
+    st = L.orc_state_pulse_fid(sy.spins, _p(iso, _i), nc, _p(csa), nd, _p(dip), nj, _p(jc),
+                               float(wr), float(rotor_deg), n, None if t1s is None else _p(t1s),
+                               _p(dts), int(npts), 0.0, nrf, _p(tab), len(th), _p(th), _p(ph),
+                               _p(ga), _p(w), _p(buf), _p(d), int(bool(normalize)),
+                               int(nthreads or os.cpu_count() or 1), _p(out))
+    if st != 0:
+        raise RuntimeError(f"oracle failed with status {st}")
+    return out[: 2 * npts].view(np.complex128).copy()

@@ -8,6 +8,9 @@ Each check is chosen so that a plausible mistake in the restatement fails it:
   * gamma-COMPUTE    -> explicit step propagation (oracle/bruteforce.py), rosym and general rho,
                         even and odd n (integer-division scale), wrap-around of the rotor period
   * closed forms     -> fid[0] and a single-spin MAS FID from a scalar phase sum
+  * states           -> per-crystallite propagation against scipy expm steps, composition and
+                        repeat 0; FIDs from per-crystallite rho against the uniform-rho and
+                        prepared-rho FIDs, and the pulse FID shifted by one point
   * ptop with rf     -> explicit numpy propagation with the rf Hamiltonian built from Kronecker
                         operators, on- and off-resonance nutation closed forms (propagator
                         sign, phase sense, offsets, the zero-amplitude channel rule)
@@ -350,3 +353,92 @@ def test_prep_fid_equals_explicit_propagation(oracle, case):
                                 step_dt=dts, rf=rf, t0=t0, normalize=False)
         f2 = B.prep_equivalent(sy, th, ph, ga, MAGIC, wr, sw, npts, rho, det, dts, rf=rf, t0_fid=t0)
         assert np.abs(f1 - f2).max() < 1e-11 * max(1.0, np.abs(f2).max()), (case, th)
+
+
+# ------------------------------------------------------------------ per-crystallite rho (states)
+def _state_case():
+    sy = SolidSys.from_text("numspin 3\nT 1H 0\nT 1H 1\nT 13C 2\nC 1000 5000 0.5 0 10 20 30\n"
+                            "C -800 4000 0.3 1 40 50 60\nC 300 6000 0.2 2 5 15 25\n"
+                            "D -15000 0 1 0 30 40\nD 3000 0 2 10 20 30\nJ 60 1 2 0\n")
+    pw = Powder(np.array([0.7, 1.2, 2.1]), np.array([1.9, 0.4, 3.0]), np.array([2.3, 0.0, 1.1]),
+                np.array([0.5, 0.3, 0.2]))
+    dts = np.array([0.8e-6, 0.8e-6, 1.5e-6, 2.0e-6])
+    t1s = np.array([3e-6, 3.8e-6, 4.6e-6, 6.1e-6])
+    rf = [("1H", np.array([90000.0, 90000.0, 0.0, 50000.0]), np.array([0.0, 0.5, 0.0, 1.0]),
+           np.array([300.0, 300.0, 0.0, -200.0])),
+          ("13C", np.array([0.0, 60000.0, 60000.0, 0.0]), np.array([0.0, 0.2, 0.2, 0.0]),
+           np.zeros(4))]
+    return sy, pw, dts, t1s, rf
+
+
+@pytest.mark.parametrize("wr,repeat", [(10000.0, 1), (10000.0, 3), (0.0, 2)])
+def test_state_apply_equals_explicit_propagators(oracle, wr, repeat):
+    """rho_k <- U_k^m rho_k U_k^-m with U_k from scipy expm steps (+i sign, rf per isotope,
+    explicit start times) at each crystallite's orientation; every crystallite starts from its
+    own random Hermitian rho so that a mix-up of crystallites or a dropped U^H fails."""
+    sy, pw, dts, t1s, rf = _state_case()
+    rng = np.random.default_rng(5)
+    rho = np.stack([rand_herm(rng, sy.N) for _ in range(len(pw))])
+    got = oracle.state_apply(sy, wr, MAGIC, dts, pw, rho, rf=rf, step_t1=t1s, repeat=repeat)
+    for k in range(len(pw)):
+        U = B.pulse_propagator(sy, pw.theta[k], pw.phi[k], pw.gamma[k], MAGIC, wr, len(dts), rf,
+                               0.0, dts, t1s)
+        Um = np.linalg.matrix_power(U, repeat)
+        want = Um @ rho[k] @ Um.conj().T
+        assert np.abs(got[k] - want).max() < 1e-12 * np.abs(want).max(), k
+        assert np.isclose(np.trace(got[k]), np.trace(rho[k]))
+
+
+def test_state_apply_repeat_zero_and_composition(oracle):
+    sy, pw, dts, t1s, rf = _state_case()
+    rng = np.random.default_rng(9)
+    rho = np.stack([rand_herm(rng, sy.N) for _ in range(len(pw))])
+    same = oracle.state_apply(sy, 8000.0, MAGIC, dts, pw, rho, rf=rf, step_t1=t1s, repeat=0)
+    assert np.array_equal(same, rho)
+    once = oracle.state_apply(sy, 8000.0, MAGIC, dts, pw, rho, rf=rf, step_t1=t1s)
+    twice = oracle.state_apply(sy, 8000.0, MAGIC, dts, pw, once, rf=rf, step_t1=t1s)
+    both = oracle.state_apply(sy, 8000.0, MAGIC, dts, pw, rho, rf=rf, step_t1=t1s, repeat=2)
+    assert np.abs(twice - both).max() < 1e-13 * np.abs(both).max()
+
+
+@pytest.mark.parametrize("wr", [10000.0, 0.0])
+def test_state_fid_of_a_uniform_rho_is_the_powder_fid(oracle, wr):
+    """Every crystallite at the same rho: the per-crystallite FID is the plain powder FID
+    (gamma-COMPUTE when spinning, the static FID when not), rosym rho included."""
+    sy, pw, _, _, _ = _state_case()
+    det = sy.op("Ip_0+Ip_1")
+    for rho in (sy.op("Ix_0+Ix_2"), 0.5 * (det + det.conj().T)):
+        rho_all = np.stack([rho] * len(pw))
+        f1, sw1 = oracle.state_fid(sy, wr, MAGIC, 80000.0, 32, pw, rho_all, det, t0=2e-6)
+        f2, sw2 = oracle.powder_fid(sy, wr, MAGIC, 80000.0, 32, pw, rho, det, t0=2e-6)
+        assert sw1 == sw2 and np.abs(f1 - f2).max() < 1e-13 * np.abs(f2).max()
+
+
+def test_state_fid_after_apply_is_the_prepared_fid(oracle):
+    """rho_k from state_apply of the pulse steps, then the FID from the pulses' end: the
+    prepared-rho FID (itself pinned against explicit propagation above)."""
+    sy, pw, dts, t1s, rf = _state_case()
+    rho, det = sy.op("Iz_0+Iz_1"), sy.op("Ip_2")
+    rho_all = oracle.state_apply(sy, 10000.0, MAGIC, dts, pw, np.stack([rho] * len(pw)), rf=rf,
+                                 step_t1=t1s)
+    t0 = float(t1s[-1] + dts[-1])
+    f1, _ = oracle.state_fid(sy, 10000.0, MAGIC, 80000.0, 24, pw, rho_all, det, t0=t0)
+    f2, _ = oracle.prep_fid(sy, 10000.0, MAGIC, 80000.0, 24, pw, rho, det, step_dt=dts,
+                            step_t1=t1s, rf=rf, t0=t0)
+    assert np.abs(f1 - f2).max() < 1e-12 * np.abs(f2).max()
+
+
+def test_state_pulse_fid_continues_the_pulse_fid(oracle):
+    """The pulse-program FID from rho_k = U_k rho U_k^H is the plain one shifted by a point;
+    from a uniform rho it is the plain one."""
+    sy, pw, dts, t1s, rf = _state_case()
+    rho, det = sy.op("Ix"), sy.op("Ip_0")
+    base = oracle.ptop_fid(sy, 12000.0, MAGIC, len(dts), 17, pw, rho, det, rf=rf, step_dt=dts,
+                           step_t1=t1s)
+    uni = oracle.state_pulse_fid(sy, 12000.0, MAGIC, dts, 17, pw, np.stack([rho] * len(pw)),
+                                 det, rf=rf, step_t1=t1s)
+    assert np.abs(uni - base).max() < 1e-13 * np.abs(base).max()
+    moved = oracle.state_apply(sy, 12000.0, MAGIC, dts, pw, np.stack([rho] * len(pw)), rf=rf,
+                               step_t1=t1s)
+    f = oracle.state_pulse_fid(sy, 12000.0, MAGIC, dts, 16, pw, moved, det, rf=rf, step_t1=t1s)
+    assert np.abs(f - base[1:]).max() < 1e-12 * np.abs(base).max()

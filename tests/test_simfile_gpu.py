@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 from blochlib_amd import fidio
-from blochlib_amd.simfile import SimFile, _Runner
+from blochlib_amd.simfile import SimFile, _Runner, _rf_tables
 from oracle import oracle as O
 
 from test_simfile import C7, STATIC
@@ -66,11 +66,158 @@ def test_ptop_c7_sim_matches_oracle(tmp_path, precision, tol):
     for it in sim2.pulses.items:
         if isinstance(it, str) and not it.replace(" ", "").startswith(("fid(", "savefid")):
             r.statement(it)
-    st = r.steps()
+    st = r.period(r.proplist).steps
     sy = sim2.system
     ref = O.ptop_fid(sy, 5000.0, np.degrees(st[0][2]), len(st), 24, sim2.powder,
-                     sy.op("Iz_0+Iz_1"), sy.op("Iz_0"), rf=r.rf_tables(st),
+                     sy.op("Iz_0+Iz_1"), sy.op("Iz_0"), rf=_rf_tables(st),
                      step_dt=[s[1] for s in st], step_t1=[s[0] for s in st])
     err = np.linalg.norm(res["fid"] - ref) / np.linalg.norm(ref)
     print(f"[sim] ptop C7 {precision}: rel-L2 {err:.3e}", flush=True)
     assert err < tol
+
+
+# ------------------------------------------------------------------ 2-D and indexed acquisition
+SMALL = dict(zcw="aveType zcw\n thetaStep 34\n phiStep 21")
+MAGIC_DEG = float(np.degrees(np.arccos(1 / np.sqrt(3))))
+
+
+def _small_powder(text):
+    return text.replace("aveType zcw\n        thetaStep 233\n        phiStep 144",
+                        SMALL["zcw"]).replace("aveType ../crystals/rep2000", SMALL["zcw"])
+
+
+def _norm_trace(rho_all, det, pw):
+    tr = np.einsum("kab,ba->k", rho_all, det)
+    return np.sum(pw.weight * tr) / np.trace(det @ det.conj().T)
+
+
+def test_mas_2d_rows_after_accumulating_delays(tmp_path):
+    """mas-2d.sim's program (examples/solid-2.0/examples/mas/mas-2d.sim): fid(i) then a delay,
+    rho carried from row to row; row i is the gamma-COMPUTE FID of rho after i delays (each
+    split into maxtstep steps at its own start time) from t0 = i dwell."""
+    from test_simfile import MAS2D
+    text = _small_powder(MAS2D).replace("npts1D=512", "npts1D=32").replace("sw=40000",
+                                                                             "sw=80000")
+    print("[sim2d] mas-2d", flush=True)
+    sim = SimFile(text)
+    res = sim.run(outdir=str(tmp_path))
+    sy, pw = sim.system, sim.powder
+    det, rho = sy.op("Ip"), sy.op("Ix")
+    rows = []
+    state = np.stack([rho] * len(pw))
+    t = 0.0
+    for i in range(4):
+        if i:
+            # the delay spans [t, t + dwell] and its step count is ceil((t2 - t1) / maxtstep)
+            # in floating point, as Propogation::propogator computes it: 5 steps for the third
+            t1, t2 = t, t + 2e-5
+            m = int(np.ceil((t2 - t1) / 5e-6))
+            d = (t2 - t1) / m
+            state = O.state_apply(sy, 10000.0, 54.7356, np.full(m, d), pw, state,
+                                  step_t1=t1 + np.arange(m) * d)
+            t = t2
+        f, _ = O.state_fid(sy, 10000.0, 54.7356, 80000.0, 32, pw, state, det, t0=t)
+        rows.append(f)
+    ref = np.array(rows)
+    err = np.linalg.norm(res["fid2d"] - ref) / np.linalg.norm(ref)
+    print(f"[sim2d] mas-2d: rel-L2 {err:.3e}", flush=True)
+    assert err < 1e-9
+    m = fidio.read_fid_matlab(res["files"][0])
+    assert np.array_equal(m["vdat"], res["fid2d"])
+
+
+def test_postc7_2d_rows(tmp_path):
+    """postc7-2d.sim's program: reuse(sub1, m), a 13C pulse, fid(m), ro() reset."""
+    from test_simfile import POSTC7_2D
+    text = _small_powder(POSTC7_2D).replace("npts1D=64", "npts1D=24")
+    print("[sim2d] postc7-2d", flush=True)
+    sim = SimFile(text)
+    res = sim.run(outdir=str(tmp_path))
+    sy, pw = sim.system, sim.powder
+    det, rho = sy.op("Iz_0"), sy.op("Iz_0+Iz_1")
+    t360 = 1 / 35000.0
+    T = 7 * (2 * t360)
+    per = int(np.ceil(2 * t360 / 1e-6))
+    dts = np.full(7 * per, 2 * t360 / per)
+    t1 = np.arange(7 * per) * (2 * t360 / per)
+    ph = np.repeat(np.deg2rad(np.arange(7) * 360 / 7), per)
+    rf = [("13C", np.full(7 * per, 35000.0), ph, np.zeros(7 * per))]
+    rows = []
+    for m in range(3):
+        st = np.stack([rho] * len(pw))
+        if m:
+            st = O.state_apply(sy, 5000.0, MAGIC_DEG, dts, pw, st, rf=rf, step_t1=t1, repeat=m)
+        # the pulse spans [m T, m T + 1 us]; (t2 - t1) / maxtstep rounds above 1 for m > 0, so
+        # it is two steps there, as in the reference
+        p1, p2 = m * T, m * T + 1e-6
+        k = int(np.ceil((p2 - p1) / 1e-6))
+        st = O.state_apply(sy, 5000.0, MAGIC_DEG, np.full(k, (p2 - p1) / k), pw, st,
+                           rf=[("13C", np.full(k, 250000.0), np.full(k, np.deg2rad(270.0)),
+                                np.zeros(k))],
+                           step_t1=p1 + np.arange(k) * (p2 - p1) / k)
+        f, _ = O.state_fid(sy, 5000.0, MAGIC_DEG, 20000.0, 24, pw, st, det, t0=p2)
+        rows.append(f)
+    ref = np.array(rows)
+    err = np.linalg.norm(res["fid2d"] - ref) / np.linalg.norm(ref)
+    print(f"[sim2d] postc7-2d: rel-L2 {err:.3e}", flush=True)
+    assert err < 1e-9
+    assert np.array_equal(fidio.read_fid2d_binary(res["files"][0]), res["fid2d"])
+
+
+def test_ptop_2d_rows_and_indexed_points_with_hold(tmp_path):
+    """ptop 2-D fid(m): row m holds the points after 1..npts applications of the used
+    sub-sequence; ptop 1-D fid(i) with a held sub-sequence: one point per call, rho carried."""
+    from test_simfile import C7
+    base = _small_powder(C7).replace("npts1D=64", "npts1D=12")
+    text = base.replace("""    reuse(sub1)
+    fid()
+    savefidtext(c7fid)""", """    2D()
+    loop(m=0:1)
+        use(sub1)
+        fid(m)
+        ro(Iz_0+Iz_1)
+    end
+    savefid(c7rows)""")
+    print("[sim2d] ptop 2-D", flush=True)
+    sim = SimFile(text)
+    res = sim.run(outdir=str(tmp_path))
+    sy, pw = sim.system, sim.powder
+    t360 = 1 / 35000.0
+    per = int(np.ceil(2 * t360 / 1e-6))
+    dts = np.full(7 * per, 2 * t360 / per)
+    t1 = np.arange(7 * per) * (2 * t360 / per)
+    ph = np.repeat(np.deg2rad(np.arange(7) * 360 / 7), per)
+    rf = [("13C", np.full(7 * per, 35000.0), ph, np.zeros(7 * per))]
+    ref = O.ptop_fid(sy, 5000.0, MAGIC_DEG, len(dts), 13, pw, sy.op("Iz_0+Iz_1"), sy.op("Iz_0"),
+                     rf=rf, step_dt=dts, step_t1=t1)[1:]
+    err = np.linalg.norm(res["fid2d"] - np.stack([ref, ref])) / (np.sqrt(2) * np.linalg.norm(ref))
+    print(f"[sim2d] ptop 2-D: rel-L2 {err:.3e}", flush=True)
+    assert err < 1e-9
+    assert np.allclose(fidio.read_fid2d_text(res["files"][0]), res["fid2d"], rtol=2e-5,
+                       atol=1e-7)
+
+    text = base.replace("""    reuse(sub1)
+    fid()
+    savefidtext(c7fid)""", """    ro(Iz_0)
+    loop(i=0:2)
+        reuse(sub1, 2, hold)
+        13C:pulse(1e-6, 0, 250000)
+        fid(i)
+    end
+    savefidtext(tdr)""")
+    print("[sim2d] ptop fid(i) hold", flush=True)
+    sim = SimFile(text)
+    res = sim.run(outdir=str(tmp_path))
+    st = np.stack([sy.op("Iz_0")] * len(pw))
+    pts = []
+    for i in range(3):
+        st = O.state_apply(sy, 5000.0, MAGIC_DEG, dts, pw, st, rf=rf, step_t1=t1, repeat=2)
+        p1, p2 = i * 1e-6, i * 1e-6 + 1e-6
+        k = int(np.ceil((p2 - p1) / 1e-6))
+        st = O.state_apply(sy, 5000.0, MAGIC_DEG, np.full(k, (p2 - p1) / k), pw, st,
+                           rf=[("13C", np.full(k, 250000.0), np.zeros(k), np.zeros(k))],
+                           step_t1=p1 + np.arange(k) * (p2 - p1) / k)
+        pts.append(_norm_trace(st, sy.op("Iz_0"), pw))
+    err = np.abs(res["fid"][:3] - np.array(pts)).max() / np.abs(pts).max()
+    print(f"[sim2d] ptop fid(i) hold: rel {err:.3e}", flush=True)
+    assert err < 1e-10 and res["sw"] == 1e6
