@@ -356,7 +356,8 @@ def bench_serve(args):
     reqs = serve.config5_requests(args.requests, max_spins=args.max_spins)
     cs = sum(len(r.powder) * int(round(r.sw / r.wr)) for r in reqs)
     print(f"[bench] serving {len(reqs)} requests ({cs / 1e6:.1f} M crystallite-steps)", flush=True)
-    srv = serve.FIDServer(streams=4, precision=args.precision)
+    srv = serve.FIDServer(streams=4, precision=args.precision,
+                          slice_flops=args.slice_flops if args.slice_flops > 0 else float("inf"))
     warm = serve.config5_requests(2, seed=999, max_spins=4)
     srv.run(warm)
     t = time.perf_counter()
@@ -385,6 +386,12 @@ def bench_serve(args):
         "latency_ms_per_fid": {"p50": float(np.percentile(lat, 50)),
                                "p99": float(np.percentile(lat, 99)),
                                "note": "arrival to result on the host, queueing included"},
+        "latency_ms_by_spins": {
+            f"{lo}-{hi}": {"requests": int(m.sum()), "p50": float(np.percentile(lat[m], 50)),
+                           "max": float(lat[m].max())}
+            for lo, hi in ((2, 6), (7, 8), (9, 10))
+            for m in [np.array([lo <= r.system.spins <= hi for r in done])] if m.any()},
+        "slice_flops": args.slice_flops,
         "rel_l2_vs_oracle": worst,
     }
     print(json.dumps(out), flush=True)
@@ -472,6 +479,39 @@ def measure(H, wls, ptop, precision, steps, warmup, args, calibrate=False):
     return {"elapsed": elapsed, "lat": lat, "agg": agg, "fids": fids, "streams": nst}
 
 
+def dense_config3(H, precision: str, K: int = 64) -> dict:
+    """The dense 10-spin configuration (N = 1024, blocks up to 252 states, MAS 20 kHz, 64 steps,
+    512 points) on K evenly spaced ZCW 1154 crystallites: one warm-up, one timed run of the
+    requested form and of the FP64 form, and the requested form's FID against the FP64 one."""
+    w = workloads.config3()
+    rho, det = w.operators()
+    pw = w.powder().subset(np.arange(0, w.powder_size, w.powder_size // K)[:K])
+    res = {}
+    for prec in dict.fromkeys((precision, "fp64")):
+        p = engine.Plan(w.system, w.wr, w.rotor, w.sw, w.npts, precision=prec)
+        try:
+            p.set_operators(rho, det)
+            p.set_powder(pw)
+            p.run()
+            p.fetch()
+            H.hipDeviceSynchronize()
+            t = time.perf_counter()
+            p.run()
+            f = p.fetch()
+            res[prec] = (time.perf_counter() - t, f)
+        finally:
+            p.close()
+    el, f = res[precision]
+    el64, f64 = res["fp64"]
+    print(f"[bench] dense config 3: K={K} {precision} {el:.3f}s, fp64 {el64:.3f}s", flush=True)
+    return {"workload": "dense 10-spin 1H cluster (N=1024, blocks up to 252 states), MAS 20 kHz, "
+                        "64 steps/period, 512 points, rho=Ix, detect=I+",
+            "crystallites": K, "precision": precision, "unit": "crystallite-steps/s",
+            "value": K * w.steps / el, "ms": el * 1e3,
+            "fp64_form_value": K * w.steps / el64,
+            "rel_l2_vs_fp64_gpu": float(np.linalg.norm(f - f64) / np.linalg.norm(f64))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -489,6 +529,8 @@ def main():
                     help="one device copy of MIB MiB before the warm-up (counter calibration)")
     ap.add_argument("--requests", type=int, default=48, help="requests of the serving mix")
     ap.add_argument("--max-spins", type=int, default=10, help="largest cluster of the serving mix")
+    ap.add_argument("--slice-flops", type=float, default=4e12,
+                    help="serving: work per powder slice of a request (0: whole requests)")
     ap.add_argument("--sets", type=int, default=8, help="parameter sets per step (ptop)")
     ap.add_argument("--precision", choices=["fp64", "c64", "bf16x3"], default="c64",
                     help="complex64 propagators/products with FP64 eigen-solver, phases and "
@@ -496,6 +538,9 @@ def main():
                          "form with its complex products as three bf16 products each")
     ap.add_argument("--no-fp64-form", action="store_true",
                     help="skip timing the FP64 form beside the c64 headline")
+    ap.add_argument("--dense-k", type=int, default=64,
+                    help="crystallites of the dense 10-spin configuration timed beside the "
+                         "headline (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this engine runs on one GPU; multi-GPU runs are not implemented "
@@ -600,6 +645,8 @@ def main():
                                        "fids": len(errs),
                                        "note": "every FID of the step at full size"},
         }
+    if not ptop and args.dense_k > 0:
+        out["dense_config3"] = dense_config3(H, args.precision, args.dense_k)
     if not args.no_accuracy:
         out["rel_l2_vs_oracle"] = (accuracy_ptop(wls, args.precision) if ptop
                                    else accuracy(wls, args.precision))

@@ -58,6 +58,8 @@ struct DevPlan {
     const int *elem_s;     // Etot: state-list position of each pair element's row (block J)
     const void *rho_c;     // prepared rho per crystallite of the batch on the block pairs (storage
                            // precision, conj(rho_JI) at element r + s b_I), or NULL
+    int corr_auto_chunks;  // leading correlation chunks whose elements have no reverse element
+                           // (rt_from_q plans): they take the autocorrelation kernel
 };
 
 // Correlation work split (k_corr), shared by the host planner and the kernel: a crystallite's
@@ -77,11 +79,17 @@ __device__ __host__ inline int corr_chunk(int n) {
     int ch = a < b ? a : b;
     return ch < 1 ? 1 : ch;
 }
-__device__ __host__ inline int corr_segments(int n, long Etot) {
-    const int CH = corr_chunk(n);
-    const int nchunk = (int)((Etot + CH - 1) / CH);
+__device__ __host__ inline int corr_range_segments(int nchunk) {
+    if (nchunk <= 0) return 0;
     const int spc = corr_seg_chunks(nchunk);
     return (nchunk + spc - 1) / spc;
+}
+// FID partial sums of a correlation: the segments of the autocorrelation chunks [0, A) and of
+// the general chunks [A, nchunk), each range segmented on its own
+__device__ __host__ inline int corr_segments(int n, long Etot, int auto_chunks = 0) {
+    const int CH = corr_chunk(n);
+    const int nchunk = (int)((Etot + CH - 1) / CH);
+    return corr_range_segments(auto_chunks) + corr_range_segments(nchunk - auto_chunks);
 }
 __host__ __device__ inline bool corr_use_fft(int n) {
     return n >= 16 && (n & (n - 1)) == 0 && n <= 2 * CORR_TW;

@@ -239,6 +239,8 @@ struct blfid_plan {
     int nparts = 1;            // FID partial sums per crystallite
     bool rt_from_q = false;    // rosym spinning case: R formed from Q in k_corr
     bool rt_rev = false;       // rt_from_q and some element has a reverse element
+    int corr_auto = 0;         // leading correlation chunks taking the autocorrelation kernel
+    long corr_rev_elems = 0;   // elements with a reverse element (their R reads a second Q)
     std::vector<int> elem_rev;  // per element (r,s) of pair (I,J): element (s,r) of (J,I) or -1
     std::map<int, std::vector<int>> pair_groups; // M -> pairs
     // powder
@@ -289,8 +291,25 @@ struct blfid_plan {
     std::vector<Ev> pending;
     double ph_ms[BLFID_NSLOT] = {0}, ph_flops[BLFID_NSLOT] = {0}, ph_bytes[BLFID_NSLOT] = {0};
     int ph_n[BLFID_NSLOT] = {0};
+    // HIP graph of the whole run (blfid_plan_graph_capture): valid while `gen` is unchanged and
+    // the working storage and FID buffer are the ones it was recorded with
+    unsigned long gen = 0, g_gen = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    hipStream_t cap_stream = nullptr;
+    char *g_base = nullptr;
+    double *g_out = nullptr;
+    // the run in crystallite slices (blfid_plan_run_slice): next slice start, batch size, plan
+    // generation and working storage of the slices so far
+    int sl_next = -1, sl_norm = 1;
+    long sl_Kb = 0;
+    unsigned long sl_gen = 0;
+    char *sl_base = nullptr;
     ~blfid_plan() {
         for (auto &e : pending) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (cap_stream) (void)hipStreamDestroy(cap_stream);
         delete prep;
     }
 };
@@ -634,6 +653,7 @@ static void refresh_devplan(blfid_plan *pl) {
     d.Estr = pl->Estr;
     d.nparts = pl->nparts;
     d.rt_rev = pl->rt_rev ? 1 : 0;
+    d.corr_auto_chunks = pl->corr_auto;
     d.theta = PI2 * pl->tau;
     d.terms = pl->d_terms.p;
     d.ff_off = pl->d_ff_off.p;
@@ -919,6 +939,7 @@ blfid_status blfid_plan_set_prep(blfid_plan *pl, double rotor_deg, int nsteps,
                                  const double *step_t1, const double *step_dt, int nrf,
                                  const blfid_rf *rf) {
     if (!pl) return fail(BLFID_ERR_INVALID, "plan is NULL");
+    pl->gen++; // a recorded graph no longer matches the plan
     if (nsteps < 0) return fail(BLFID_ERR_INVALID, "negative step count");
     if (nsteps > 0 && !step_dt) return fail(BLFID_ERR_INVALID, "step_dt is NULL");
     if (nsteps > 0 && (!pl->spinning || pl->ptop))
@@ -967,12 +988,14 @@ int blfid_plan_dim(const blfid_plan *pl) { return pl ? pl->N : 0; }
 
 blfid_status blfid_plan_set_memory_budget(blfid_plan *pl, double bytes) {
     if (!pl || bytes < 0) return fail(BLFID_ERR_INVALID, "bad plan or budget");
+    pl->gen++; // a recorded graph no longer matches the plan
     pl->budget = bytes;
     return BLFID_OK;
 }
 
 blfid_status blfid_plan_set_operators(blfid_plan *pl, const double *rho_d, const double *det_d) {
     if (!pl || !rho_d || !det_d) return fail(BLFID_ERR_INVALID, "NULL argument");
+    pl->gen++; // a recorded graph no longer matches the plan
     const int N = pl->N;
     const cplx *rho = (const cplx *)rho_d, *det = (const cplx *)det_d;
     auto R = [&](int i, int j) { return rho[i + (size_t)j * N]; };
@@ -1017,6 +1040,14 @@ blfid_status blfid_plan_set_operators(blfid_plan *pl, const double *rho_d, const
     for (int I = 0; I < nb; I++)
         for (int J = 0; J < nb; J++)
             if (dnz[I * nb + J] && rnz[J * nb + I]) pl->pairs.push_back({I, J});
+    if (pl->spinning && rosym) {
+        // pairs without a reverse pair (J, I) first: their elements have R = conj(Q) and form
+        // the leading correlation chunks that take the autocorrelation kernel
+        std::vector<char> has(nb * nb, 0);
+        for (auto &q : pl->pairs) has[q.I * nb + q.J] = 1;
+        std::stable_partition(pl->pairs.begin(), pl->pairs.end(),
+                              [&](const Pair &q) { return !has[q.J * nb + q.I]; });
+    }
     pl->pair_eoff.clear(); pl->pair_doff.clear(); pl->pair_roff.clear();
     pl->d_rowptr.clear(); pl->d_col.clear(); pl->d_val.clear();
     pl->r_rowptr.clear(); pl->r_col.clear(); pl->r_val.clear();
@@ -1065,10 +1096,6 @@ blfid_status blfid_plan_set_operators(blfid_plan *pl, const double *rho_d, const
     }
     pl->Etot = eo;
     pl->Estr = (eo + 15) & ~15L;
-    if (pl->spinning && !pl->ptop)
-        pl->nparts = eo > 0 ? corr_segments(pl->n, eo) : 1;
-    else
-        pl->nparts = std::max<int>(1, (int)pl->pairs.size());
     // rosym spinning case: R_(I,J) = B_J^H (det_JI + det_IJ^H) B_I, i.e. R_(I,J)(s,r) =
     // Q_(J,I)(s,r) + conj(Q_(I,J)(r,s)); the pair (J,I) exists exactly when det_JI != 0, so R
     // needs neither its own products nor storage: k_corr forms it from Q through elem_rev
@@ -1090,7 +1117,24 @@ blfid_status blfid_plan_set_operators(blfid_plan *pl, const double *rho_d, const
         }
     }
     pl->rt_rev = false;
-    for (int v : pl->elem_rev) pl->rt_rev |= v >= 0;
+    long first_rev = eo;
+    for (long e = 0; e < (long)pl->elem_rev.size(); e++)
+        if (pl->elem_rev[e] >= 0) {
+            pl->rt_rev = true;
+            first_rev = std::min(first_rev, e);
+        }
+    // correlation chunks before the first element with a reverse element: autocorrelation
+    {
+        const int CH = corr_chunk(pl->n);
+        const int nchunk = (int)((eo + CH - 1) / CH);
+        pl->corr_auto = !pl->rt_from_q ? 0 : !pl->rt_rev ? nchunk : (int)(first_rev / CH);
+        pl->corr_rev_elems = 0;
+        for (int v : pl->elem_rev) pl->corr_rev_elems += v >= 0;
+    }
+    if (pl->spinning && !pl->ptop)
+        pl->nparts = eo > 0 ? corr_segments(pl->n, eo, corr_use_fft(pl->n) ? pl->corr_auto : 0) : 1;
+    else
+        pl->nparts = std::max<int>(1, (int)pl->pairs.size());
     if (pl->prep) {
         // rho's nonzero entries, from which each crystallite's prepared blocks are formed
         std::vector<int> nzi;
@@ -1136,6 +1180,7 @@ blfid_status blfid_plan_set_powder(blfid_plan *pl, const double *theta, const do
                                    const double *gamma, const double *weight, int K) {
     if (!pl || K < 0 || (K > 0 && (!theta || !phi || !gamma || !weight)))
         return fail(BLFID_ERR_INVALID, "bad powder arguments");
+    pl->gen++; // a recorded graph no longer matches the plan
     pl->K = K;
     if (K == 0) return BLFID_OK;
     std::vector<double> th(theta, theta + K), ph(phi, phi + K), ga(gamma, gamma + K),
@@ -1299,7 +1344,11 @@ blfid_status blfid_plan_stats(const blfid_plan *pl, double *out) {
         ex += 8.0 * s3;         // Dyson
         ex += 8.0 * s3;         // B = A W
         ex += 8.0 * pairsQ;     // Q and R
-        ex += pairs * corr_flops_per_elem(pl->n, pl->npts, pl->rt_from_q && !pl->rt_rev) / n;
+        const double ea = corr_use_fft(pl->n)
+                              ? std::min<double>((double)pl->corr_auto * corr_chunk(pl->n), (double)pl->Etot)
+                              : 0.0;
+        ex += (ea * corr_flops_per_elem(pl->n, pl->npts, true) +
+               ((double)pl->Etot - ea) * corr_flops_per_elem(pl->n, pl->npts, false)) / n;
     }
     out[0] = ex;
     if (pl->ptop) out[1] = 8.0 * N * N * N + 16.0 * N * N * N * P / n;
@@ -1333,9 +1382,12 @@ static void carve(blfid_plan *pl, char *base, const size_t *off) {
 // Enqueues the whole powder on `st` with the working storage at `base` (regions at off[],
 // batches of Kb crystallites) and the normalised FID written to `out` (2 npts doubles, device).
 // Every error return after the first launch leaves the launched work in the stream.
-static blfid_status enqueue_run(blfid_plan *pl, const blfid_layout::Shape &sh, char *base,
-                                const size_t *off, long Kb, int normalize, double *out,
-                                hipStream_t st) {
+// The crystallites [c_begin, c_end) of the powder in batches of Kb; `first` starts the FID (the
+// powder rotations when new, the group sums zeroed), `last` finishes it into `out`.  The group
+// sums carry over between calls, so consecutive ranges give the whole run's FID bit for bit.
+static blfid_status enqueue_range(blfid_plan *pl, const blfid_layout::Shape &sh, char *base,
+                                  const size_t *off, long Kb, int normalize, double *out,
+                                  hipStream_t st, int c_begin, int c_end, bool first, bool last) {
     const int K = pl->K;
     if (pl->state && pl->state->K != K)
         return fail(BLFID_ERR_INVALID, "the state and the plan's powder differ in size");
@@ -1349,14 +1401,14 @@ static blfid_status enqueue_run(blfid_plan *pl, const blfid_layout::Shape &sh, c
     }
     const int G = blfid_layout::CRYST_GROUP;
     const int ng = (int)blfid_layout::ngroups(K);
-    LAUNCHCHK(blfid_dev_launch_zero(pl->grp, (size_t)ng * 2 * 2 * pl->npts, st));
+    if (first) LAUNCHCHK(blfid_dev_launch_zero(pl->grp, (size_t)ng * 2 * 2 * pl->npts, st));
     const size_t bm2 = (size_t)pl->bmax * pl->bmax;
     const int pr = pl->prec;
     const size_t cs = pr ? 8 : 16;
     const int ff = pl->rosym == 1 ? 2 : 1;
     const double scale = (pl->spinning && !pl->ptop) ? 1.0 / (double)(pl->n / ff) : 1.0;
-    for (int c0 = 0; c0 < K; c0 += (int)Kb) {
-        int kb = (int)std::min<long>(Kb, K - c0);
+    for (int c0 = c_begin; c0 < c_end; c0 += (int)Kb) {
+        int kb = (int)std::min<long>(Kb, c_end - c0);
         // the FID parts are all written by the correlation / eigenbasis-FID kernels, unless
         // the operators couple no block pair at all
         if (pl->pairs.empty())
@@ -1503,10 +1555,18 @@ static blfid_status enqueue_run(blfid_plan *pl, const blfid_layout::Shape &sh, c
                                               ns, pl->bufQt, pl->bufRt, st));
             }
             if (pl->Etot > 0) {
-                const double rq = (!pl->rt_from_q || pl->rt_rev) ? 2.0 : 1.0; // Q, and R or reverse Q
-                const bool autoc = pl->rt_from_q && !pl->rt_rev;
-                PhaseScope ps(pl, 5 * 8, kb * (double)pl->Etot * corr_flops_per_elem(pl->n, pl->npts, autoc),
-                              kb * ((double)pl->n * pl->Etot * cs * rq + (double)pl->nparts * pl->npts * 16.0), st);
+                // Q once per element; R too unless it is formed from Q, or the reverse element's
+                // Q where there is one.  Elements of the autocorrelation chunks take that
+                // variant's flop count, the others the general one.
+                const double reads = pl->rt_from_q ? (double)pl->Etot + (double)pl->corr_rev_elems
+                                                   : 2.0 * (double)pl->Etot;
+                const double ea = corr_use_fft(pl->n)
+                                      ? std::min<double>((double)pl->corr_auto * corr_chunk(pl->n), (double)pl->Etot)
+                                      : 0.0;
+                const double fl = ea * corr_flops_per_elem(pl->n, pl->npts, true) +
+                                  ((double)pl->Etot - ea) * corr_flops_per_elem(pl->n, pl->npts, false);
+                PhaseScope ps(pl, 5 * 8, kb * fl,
+                              kb * ((double)pl->n * reads * cs + (double)pl->nparts * pl->npts * 16.0), st);
                 LAUNCHCHK(blfid_dev_launch_corr(pr, P, kb, pl->bufQt, pl->bufRt, pl->thetas,
                                                 pl->d_er.p, pl->d_es.p, pl->part, st));
             }
@@ -1539,9 +1599,17 @@ static blfid_status enqueue_run(blfid_plan *pl, const blfid_layout::Shape &sh, c
                                               pl->part, pl->grp, st));
         }
     }
-    cplx norm = normalize ? pl->trdd : cplx(1.0, 0.0);
-    LAUNCHCHK(blfid_dev_launch_finish(pl->npts, ng, pl->grp, norm.real(), norm.imag(), out, st));
+    if (last) {
+        cplx norm = normalize ? pl->trdd : cplx(1.0, 0.0);
+        LAUNCHCHK(blfid_dev_launch_finish(pl->npts, ng, pl->grp, norm.real(), norm.imag(), out, st));
+    }
     return BLFID_OK;
+}
+
+static blfid_status enqueue_run(blfid_plan *pl, const blfid_layout::Shape &sh, char *base,
+                                const size_t *off, long Kb, int normalize, double *out,
+                                hipStream_t st) {
+    return enqueue_range(pl, sh, base, off, Kb, normalize, out, st, 0, pl->K, true, true);
 }
 
 blfid_status blfid_plan_run(blfid_plan *pl, int normalize, void *stream) {
@@ -1797,6 +1865,7 @@ blfid_status blfid_state_trace(blfid_state *st, const double *det_d, int normali
 
 blfid_status blfid_plan_set_state(blfid_plan *pl, blfid_state *st) {
     if (!pl) return fail(BLFID_ERR_INVALID, "plan is NULL");
+    pl->gen++; // a recorded graph no longer matches the plan
     if (st && st->N != pl->N)
         return fail(BLFID_ERR_INVALID, "the state and the plan differ in Hilbert dimension");
     if (pl->last_stream) (void)hipStreamSynchronize(pl->last_stream);
@@ -1806,6 +1875,123 @@ blfid_status blfid_plan_set_state(blfid_plan *pl, blfid_state *st) {
     }
     pl->state = st;
     pl->have_ops = false; // the block-pair tables depend on it: operators are set again
+    return BLFID_OK;
+}
+
+// ------------------------------------------------------------------------------ HIP graphs
+blfid_status blfid_plan_graph_capture(blfid_plan *pl, int normalize) {
+    if (!pl) return fail(BLFID_ERR_INVALID, "plan is NULL");
+    if (!pl->have_ops) return fail(BLFID_ERR_INVALID, "operators not set");
+    if (normalize && pl->trdd == 0.0) return fail(BLFID_ERR_INVALID, "Tr(det det^H) is zero");
+    if (pl->prof)
+        return fail(BLFID_ERR_UNSUPPORTED, "profiling is on: a graph does not record phase events");
+    if (pl->ws)
+        return fail(BLFID_ERR_UNSUPPORTED, "graph capture needs the plan's own working storage");
+    if (pl->state && pl->state->K != pl->K)
+        return fail(BLFID_ERR_INVALID, "the state and the plan's powder differ in size");
+    if (pl->last_stream) HIPCHK(hipStreamSynchronize(pl->last_stream));
+    blfid_layout::Shape sh, psh;
+    full_shape(pl, sh, psh);
+    double avail = pl->budget;
+    if (avail <= 0) {
+        size_t fre = 0, tot = 0;
+        HIPCHK(hipMemGetInfo(&fre, &tot));
+        avail = 0.4 * (double)(fre + pl->own_ws.n);
+    }
+    const long Kb = blfid_layout::choose_batch(sh, pl->K, avail);
+    if (Kb < 1) return fail(BLFID_ERR_NOMEM, "one crystallite does not fit the memory budget");
+    size_t off[blfid_layout::R_COUNT];
+    const size_t need = blfid_layout::layout_bytes(sh, Kb, off);
+    HIPCHK(pl->own_ws.alloc(need));
+    HIPCHK(pl->fid_out.alloc((size_t)2 * pl->npts));
+    if (pl->gexec) (void)hipGraphExecDestroy(pl->gexec);
+    if (pl->graph) (void)hipGraphDestroy(pl->graph);
+    pl->gexec = nullptr;
+    pl->graph = nullptr;
+    if (!pl->cap_stream) HIPCHK(hipStreamCreateWithFlags(&pl->cap_stream, hipStreamNonBlocking));
+    // the recorded run forms the powder rotations itself; an eager run still forms them too
+    const bool rpd = pl->rp_dirty;
+    HIPCHK(hipStreamBeginCapture(pl->cap_stream, hipStreamCaptureModeThreadLocal));
+    blfid_status rs = enqueue_run(pl, sh, pl->own_ws.p, off, Kb, normalize, pl->fid_out.p,
+                                  pl->cap_stream);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(pl->cap_stream, &g);
+    pl->rp_dirty = rpd;
+    if (rs != BLFID_OK || e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        if (rs != BLFID_OK) return rs;
+        return fail(BLFID_ERR_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    }
+    hipGraphExec_t x = nullptr;
+    e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return fail(BLFID_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
+    pl->graph = g;
+    pl->gexec = x;
+    pl->g_gen = pl->gen;
+    pl->g_base = pl->own_ws.p;
+    pl->g_out = pl->fid_out.p;
+    return BLFID_OK;
+}
+
+blfid_status blfid_plan_graph_launch(blfid_plan *pl, void *stream) {
+    if (!pl) return fail(BLFID_ERR_INVALID, "plan is NULL");
+    if (!pl->gexec || pl->g_gen != pl->gen || pl->own_ws.p != pl->g_base ||
+        pl->fid_out.p != pl->g_out)
+        return fail(BLFID_ERR_INVALID, "no graph recorded for the plan as it is now: capture again");
+    hipStream_t st = (hipStream_t)stream;
+    HIPCHK(hipGraphLaunch(pl->gexec, st));
+    pl->last_stream = st;
+    pl->ran = true;
+    return BLFID_OK;
+}
+
+// ------------------------------------------------------------------------------ slices
+blfid_status blfid_plan_run_slice(blfid_plan *pl, int c_begin, int c_end, int normalize,
+                                  void *stream) {
+    if (!pl) return fail(BLFID_ERR_INVALID, "plan is NULL");
+    if (!pl->have_ops) return fail(BLFID_ERR_INVALID, "operators not set");
+    if (pl->ws)
+        return fail(BLFID_ERR_UNSUPPORTED, "slices need the plan's own working storage");
+    if (c_begin < 0 || c_end > pl->K || c_begin >= c_end)
+        return fail(BLFID_ERR_INVALID, "bad crystallite range");
+    hipStream_t st = (hipStream_t)stream;
+    if (c_begin == 0) {
+        if (normalize && pl->trdd == 0.0) return fail(BLFID_ERR_INVALID, "Tr(det det^H) is zero");
+        blfid_layout::Shape sh, psh;
+        full_shape(pl, sh, psh);
+        double avail = pl->budget;
+        if (avail <= 0) {
+            size_t fre = 0, tot = 0;
+            HIPCHK(hipMemGetInfo(&fre, &tot));
+            avail = 0.4 * (double)(fre + pl->own_ws.n);
+        }
+        const long Kb = blfid_layout::choose_batch(sh, pl->K, avail);
+        if (Kb < 1) return fail(BLFID_ERR_NOMEM, "one crystallite does not fit the memory budget");
+        HIPCHK(pl->own_ws.alloc(blfid_layout::layout_bytes(sh, Kb)));
+        HIPCHK(pl->fid_out.alloc((size_t)2 * pl->npts));
+        pl->sl_Kb = Kb;
+        pl->sl_gen = pl->gen;
+        pl->sl_norm = normalize;
+        pl->sl_base = pl->own_ws.p;
+    } else if (c_begin != pl->sl_next || pl->sl_gen != pl->gen || pl->own_ws.p != pl->sl_base) {
+        return fail(BLFID_ERR_INVALID,
+                    "slices must follow each other from crystallite 0, with the plan unchanged");
+    }
+    blfid_layout::Shape sh, psh;
+    full_shape(pl, sh, psh);
+    size_t off[blfid_layout::R_COUNT];
+    (void)blfid_layout::layout_bytes(sh, pl->sl_Kb, off);
+    pl->last_stream = st;
+    pl->sl_next = -1; // a failure leaves no slice to continue
+    blfid_status rs = enqueue_range(pl, sh, pl->own_ws.p, off, pl->sl_Kb, pl->sl_norm,
+                                    pl->fid_out.p, st, c_begin, c_end, c_begin == 0,
+                                    c_end == pl->K);
+    if (rs != BLFID_OK) return rs;
+    pl->sl_next = c_end == pl->K ? -1 : c_end;
+    if (c_end == pl->K) pl->ran = true;
     return BLFID_OK;
 }
 
@@ -1922,6 +2108,7 @@ void blfid_workspace_destroy(blfid_workspace *w) {
 
 blfid_status blfid_plan_set_workspace(blfid_plan *pl, blfid_workspace *w) {
     if (!pl) return fail(BLFID_ERR_INVALID, "plan is NULL");
+    pl->gen++; // a recorded graph no longer matches the plan
     pl->ws = w;
     if (w) pl->own_ws.release();
     return BLFID_OK;
@@ -1929,6 +2116,7 @@ blfid_status blfid_plan_set_workspace(blfid_plan *pl, blfid_workspace *w) {
 
 blfid_status blfid_plan_set_profiling(blfid_plan *pl, int on) {
     if (!pl) return fail(BLFID_ERR_INVALID, "plan is NULL");
+    pl->gen++; // a recorded graph no longer matches the plan
     collect_profile(pl);
     pl->prof = on != 0;
     for (int i = 0; i < BLFID_NSLOT; i++) {

@@ -1952,7 +1952,8 @@ __global__ void __launch_bounds__(256, (LN > 0 && sizeof(R) == 4 && NPT == 4) ? 
                                               const double *__restrict__ thetas,
                                               const int *__restrict__ elem_r,
                                               const int *__restrict__ elem_s,
-                                              double *__restrict__ part) {
+                                              double *__restrict__ part, int ch_lo, int ch_hi,
+                                              int seg_base) {
     typedef typename CX<R>::T CT;
     constexpr bool FFT = LN > 0;
     typedef CorrFft<FFT ? LN : 4> FF;
@@ -1981,7 +1982,9 @@ __global__ void __launch_bounds__(256, (LN > 0 && sizeof(R) == 4 && NPT == 4) ? 
     const int npg = (n + CORR_PPT - 1) / CORR_PPT;
     const int nseg = 256 / CH;
     const int kpt = (n + nseg - 1) / nseg;
-    const int nchunk = (int)((P.Etot + CH - 1) / CH);
+    // this launch's chunks [ch_lo, ch_hi) of the crystallite's elements; its segments are
+    // partial sums seg_base, seg_base + 1, ...
+    const int nchunk = ch_hi - ch_lo;
     // Task = (crystallite, segment of consecutive chunks); the segmentation depends on the plan
     // only (not on the batch), so a crystallite's FID is summed in the same grouping whatever
     // the batch size.  The segment's FID partial sums (t = tid + 256 m) stay in registers and
@@ -2002,11 +2005,11 @@ __global__ void __launch_bounds__(256, (LN > 0 && sizeof(R) == 4 && NPT == 4) ? 
     for (int m = 0; m < NPT; m++) hoff[m] = FFT ? 0 : (tid + 256 * m) % n; // step of point tid + 256 m
     for (long task = blockIdx.x; task < ntask; task += gridDim.x) {
         const int c = (int)(task / nsg), sg = (int)(task % nsg);
-        const int ch_end = min(nchunk, (sg + 1) * spc);
+        const int ch_end = ch_lo + min(nchunk, (sg + 1) * spc);
         cd hacc[NPT];
 #pragma unroll
         for (int m = 0; m < NPT; m++) hacc[m] = cmk(0, 0);
-        for (int ch = sg * spc; ch < ch_end; ch++) {
+        for (int ch = ch_lo + sg * spc; ch < ch_end; ch++) {
             const long e0 = (long)ch * CH;
             if constexpr (FFT) {
                 // staging fused with the first forward pass (radix 8, L = n): item (el, j) loads
@@ -2250,12 +2253,12 @@ __global__ void __launch_bounds__(256, (LN > 0 && sizeof(R) == 4 && NPT == 4) ? 
                         }
                     }
                     // the first chunk of the segment stores, the later ones add (same thread)
-                    double *dst = part + 2 * (((size_t)c * nsg + sg) * P.npts);
+                    double *dst = part + 2 * (((size_t)c * P.nparts + seg_base + sg) * P.npts);
 #pragma unroll
                     for (int m = 0; m < NPT; m++) {
                         if (m < nt) {
                             const int t = t0 + 256 * m;
-                            if (ch == sg * spc) {
+                            if (ch == ch_lo + sg * spc) {
                                 dst[2 * t] = acc[m].x;
                                 dst[2 * t + 1] = acc[m].y;
                             } else {
@@ -2269,7 +2272,7 @@ __global__ void __launch_bounds__(256, (LN > 0 && sizeof(R) == 4 && NPT == 4) ? 
             __syncthreads();
         }
         if (hold) {
-            double *dst = part + 2 * (((size_t)c * nsg + sg) * P.npts);
+            double *dst = part + 2 * (((size_t)c * P.nparts + seg_base + sg) * P.npts);
 #pragma unroll
             for (int m = 0; m < NPT; m++) {
                 if (m < nth) {
@@ -2738,52 +2741,65 @@ extern "C" int blfid_dev_launch_qr(int prec, int M, const DevPlan *P, int Kb, co
     return (int)hipGetLastError();
 }
 
-// AUTO variant (FFT sizes only) when R = conj(Q): rosym with no reverse elements
+// One launch over the chunks [lo, hi) of every crystallite of the batch, its segments written as
+// partial sums seg_base...; AUTO (FFT sizes only): the autocorrelation variant, for chunks whose
+// elements have R = conj(Q) (rosym, no reverse element)
 template <typename R, int LN, int NPT>
-static void launch_corr_t(int grid, const DevPlan *P, int Kb, const void *Qt, const void *Rt,
+static void launch_corr_t(bool autoc, const DevPlan *P, int Kb, const void *Qt, const void *Rt,
                           const double *thetas, const int *er, const int *es, double *part,
-                          hipStream_t st) {
+                          int lo, int hi, int seg_base, hipStream_t st) {
     typedef typename CX<R>::T CT;
-    if (LN > 0 && P->rt_from_q && !P->rt_rev)
+    const long ntask = (long)Kb * corr_range_segments(hi - lo);
+    const int grid = (int)(ntask < GRID_CAP ? ntask : GRID_CAP);
+    if (grid <= 0) return;
+    if (LN > 0 && autoc)
         k_corr<R, LN, NPT, LN != 0><<<grid, 256, 0, st>>>(*P, Kb, (const CT *)Qt, (const CT *)Rt,
-                                                          thetas, er, es, part);
+                                                          thetas, er, es, part, lo, hi, seg_base);
     else
         k_corr<R, LN, NPT, false><<<grid, 256, 0, st>>>(*P, Kb, (const CT *)Qt, (const CT *)Rt,
-                                                        thetas, er, es, part);
+                                                        thetas, er, es, part, lo, hi, seg_base);
 }
 // FFT variants by log2 n (corr_use_fft(n): 16 <= n <= 512), LN = 0 the direct sum
 template <typename R, int NPT>
-static void launch_corr(int grid, const DevPlan *P, int Kb, const void *Qt, const void *Rt,
+static void launch_corr(bool autoc, const DevPlan *P, int Kb, const void *Qt, const void *Rt,
                         const double *thetas, const int *er, const int *es, double *part,
-                        hipStream_t st) {
+                        int lo, int hi, int seg_base, hipStream_t st) {
     const int ln = corr_use_fft(P->n) ? __builtin_ctz((unsigned)P->n) : 0;
+#define CORR_ARGS autoc, P, Kb, Qt, Rt, thetas, er, es, part, lo, hi, seg_base, st
     switch (ln) {
-    case 4: launch_corr_t<R, 4, NPT>(grid, P, Kb, Qt, Rt, thetas, er, es, part, st); break;
-    case 5: launch_corr_t<R, 5, NPT>(grid, P, Kb, Qt, Rt, thetas, er, es, part, st); break;
-    case 6: launch_corr_t<R, 6, NPT>(grid, P, Kb, Qt, Rt, thetas, er, es, part, st); break;
-    case 7: launch_corr_t<R, 7, NPT>(grid, P, Kb, Qt, Rt, thetas, er, es, part, st); break;
-    case 8: launch_corr_t<R, 8, NPT>(grid, P, Kb, Qt, Rt, thetas, er, es, part, st); break;
-    case 9: launch_corr_t<R, 9, NPT>(grid, P, Kb, Qt, Rt, thetas, er, es, part, st); break;
-    default: launch_corr_t<R, 0, NPT>(grid, P, Kb, Qt, Rt, thetas, er, es, part, st); break;
+    case 4: launch_corr_t<R, 4, NPT>(CORR_ARGS); break;
+    case 5: launch_corr_t<R, 5, NPT>(CORR_ARGS); break;
+    case 6: launch_corr_t<R, 6, NPT>(CORR_ARGS); break;
+    case 7: launch_corr_t<R, 7, NPT>(CORR_ARGS); break;
+    case 8: launch_corr_t<R, 8, NPT>(CORR_ARGS); break;
+    case 9: launch_corr_t<R, 9, NPT>(CORR_ARGS); break;
+    default: launch_corr_t<R, 0, NPT>(CORR_ARGS); break;
     }
+#undef CORR_ARGS
 }
 
+// The correlation of a batch: the leading P->corr_auto_chunks chunks with the autocorrelation
+// variant, the rest with the general one (each range segmented on its own, corr_segments).
 extern "C" int blfid_dev_launch_corr(int prec, const DevPlan *P, int Kb, const void *Qt,
                                      const void *Rt, const double *thetas, const int *er,
                                      const int *es, double *part, hipStream_t st) {
-    int CH = corr_chunk(P->n);
-    int nchunk = (int)((P->Etot + CH - 1) / CH);
-    int spc = corr_seg_chunks(nchunk);
-    long ntask = (long)Kb * ((nchunk + spc - 1) / spc);
-    int grid = (int)(ntask < GRID_CAP ? ntask : GRID_CAP);
-    if (grid <= 0) return 0;
+    const int CH = corr_chunk(P->n);
+    const int nchunk = (int)((P->Etot + CH - 1) / CH);
+    const int A = corr_use_fft(P->n) ? min(max(P->corr_auto_chunks, 0), nchunk) : 0;
+    const int segA = corr_range_segments(A);
     // four points per thread up to 1024 FID points, eight above
-    if (P->npts <= 1024) {
-        if (prec) launch_corr<float, 4>(grid, P, Kb, Qt, Rt, thetas, er, es, part, st);
-        else launch_corr<double, 4>(grid, P, Kb, Qt, Rt, thetas, er, es, part, st);
-    } else {
-        if (prec) launch_corr<float, 8>(grid, P, Kb, Qt, Rt, thetas, er, es, part, st);
-        else launch_corr<double, 8>(grid, P, Kb, Qt, Rt, thetas, er, es, part, st);
+    for (int pass = 0; pass < 2; pass++) {
+        const int lo = pass ? A : 0, hi = pass ? nchunk : A;
+        if (hi <= lo) continue;
+        const bool autoc = pass == 0;
+        const int sb = pass ? segA : 0;
+        if (P->npts <= 1024) {
+            if (prec) launch_corr<float, 4>(autoc, P, Kb, Qt, Rt, thetas, er, es, part, lo, hi, sb, st);
+            else launch_corr<double, 4>(autoc, P, Kb, Qt, Rt, thetas, er, es, part, lo, hi, sb, st);
+        } else {
+            if (prec) launch_corr<float, 8>(autoc, P, Kb, Qt, Rt, thetas, er, es, part, lo, hi, sb, st);
+            else launch_corr<double, 8>(autoc, P, Kb, Qt, Rt, thetas, er, es, part, lo, hi, sb, st);
+        }
     }
     return (int)hipGetLastError();
 }

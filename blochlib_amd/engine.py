@@ -119,6 +119,9 @@ def lib():
                                         ctypes.POINTER(_RF), ctypes.c_int, vp]
         L.blfid_state_trace.argtypes = [vp, _d, ctypes.c_int, _d, vp]
         L.blfid_plan_set_state.argtypes = [vp, vp]
+        L.blfid_plan_graph_capture.argtypes = [vp, ctypes.c_int]
+        L.blfid_plan_graph_launch.argtypes = [vp, vp]
+        L.blfid_plan_run_slice.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp]
         L.blfid_last_error.restype = ctypes.c_char_p
         L.blfid_version.restype = ctypes.c_char_p
         _lib = L
@@ -290,6 +293,21 @@ class Plan:
         _check(lib().blfid_plan_run_noalloc(self.h, int(bool(normalize)),
                                             ctypes.c_void_p(fid_out_dev),
                                             ctypes.c_void_p(stream) if stream else None))
+
+    def run_slice(self, c_begin: int, c_end: int, normalize: bool = True,
+                  stream: int | None = None):
+        """Crystallites [c_begin, c_end) (blfid_plan_run_slice): slices from 0 to K in order give
+        run()'s FID."""
+        _check(lib().blfid_plan_run_slice(self.h, int(c_begin), int(c_end), int(bool(normalize)),
+                                          ctypes.c_void_p(stream) if stream else None))
+
+    def graph_capture(self, normalize: bool = True):
+        """Record the whole run as a HIP graph (blfid_plan_graph_capture); nothing runs."""
+        _check(lib().blfid_plan_graph_capture(self.h, int(bool(normalize))))
+
+    def graph_launch(self, stream: int | None = None):
+        """Run the recorded graph (blfid_plan_graph_launch); then fetch() as after run()."""
+        _check(lib().blfid_plan_graph_launch(self.h, ctypes.c_void_p(stream) if stream else None))
 
     def fetch(self) -> np.ndarray:
         out = np.zeros(2 * self.npts)

@@ -4,10 +4,14 @@ Each request is one powder FID (a spin system, MAS conditions, powder, operators
 keeps a FIFO of arrived requests and admits the head whenever its working storage fits the
 remaining device budget and an execution stream is free; every admitted request gets a plan
 sized to its share (blfid_plan_set_memory_budget, so large powders run in crystallite batches)
-and runs asynchronously on its own HIP stream, so small requests overlap large ones.  Plans are
-cached by (system, acquisition, precision): a repeated system reuses its device tables.
-Completion is observed with HIP events; the FID is fetched and the request's latency (arrival
-to result on the host) recorded.
+and runs asynchronously on its own HIP stream, so small requests overlap large ones.  A request
+runs as slices of its powder (blfid_plan_run_slice) of about `slice_flops` of work each (at
+least `min_slice` crystallites), issued `depth` at a time: a large request's kernels then end
+every few hundred milliseconds and the device's queues interleave the kernels of requests
+admitted after it, instead of those waiting for the whole large FID.  Slices give the
+unsliced FID bit for bit.  Plans are cached by (system, acquisition, precision): a repeated
+system reuses its device tables.  Completion is observed with HIP events; the FID is fetched and
+the request's latency (arrival to result on the host) recorded.
 
 This replaces nothing in BlochLib (which computes one simulation per process); it is the
 service shape of the batched engine: FIDs/s and latency percentiles under an arrival process.
@@ -67,17 +71,21 @@ def _hip():
 @dataclass
 class _Slot:
     stream: ctypes.c_void_p
-    event: ctypes.c_void_p
+    events: list                  # one HIP event per slice in flight (depth of them)
     req: Request | None = None
     plan: engine.Plan | None = None
     key: tuple | None = None
+    next_c: int = 0               # first crystallite of the next slice
+    slice_k: int = 0              # crystallites per slice
+    inflight: list = field(default_factory=list)  # events of the slices issued, oldest first
 
 
 class FIDServer:
     """Admission by device-memory footprint onto `streams` concurrent HIP streams."""
 
     def __init__(self, budget_bytes: float | None = None, streams: int = 4,
-                 precision: str = "fp64", cache_plans: int = 4, min_batch: int = 64):
+                 precision: str = "fp64", cache_plans: int = 4, min_batch: int = 64,
+                 slice_flops: float = 4e12, min_slice: int = 64, depth: int = 2):
         self.hip = _hip()
         if budget_bytes is None:
             free, tot = ctypes.c_size_t(), ctypes.c_size_t()
@@ -87,14 +95,21 @@ class FIDServer:
         self.used = 0.0
         self.precision = precision
         self.min_batch = min_batch
+        self.slice_flops = float(slice_flops)
+        self.min_slice = max(1, int(min_slice))
+        self.depth = max(1, int(depth))
         self.slots = []
         for _ in range(streams):
-            s, e = ctypes.c_void_p(), ctypes.c_void_p()
+            s = ctypes.c_void_p()
             # non-blocking streams (default priority): independent of the null stream on which
             # plan tables are uploaded
             self.hip.hipStreamCreateWithFlags(ctypes.byref(s), 1)
-            self.hip.hipEventCreateWithFlags(ctypes.byref(e), 2)  # hipEventDisableTiming
-            self.slots.append(_Slot(s, e))
+            evs = []
+            for _ in range(self.depth):
+                e = ctypes.c_void_p()
+                self.hip.hipEventCreateWithFlags(ctypes.byref(e), 2)  # hipEventDisableTiming
+                evs.append(e)
+            self.slots.append(_Slot(s, evs))
         self.cache: OrderedDict = OrderedDict()
         self.cache_max = cache_plans
         # evicted plans: freeing device memory synchronises the device, so they are destroyed
@@ -111,7 +126,8 @@ class FIDServer:
         self.cache.clear()
         self._bury()
         for sl in self.slots:
-            self.hip.hipEventDestroy(sl.event)
+            for e in sl.events:
+                self.hip.hipEventDestroy(e)
             self.hip.hipStreamDestroy(sl.stream)
         self.slots = []
 
@@ -156,6 +172,23 @@ class FIDServer:
         min_batch or the whole powder, whichever is smaller)."""
         st = p.stats()
         return st["fixed_bytes"] + st["bytes_per_crystallite"] * min(K, self.min_batch)
+
+    def slice_size(self, p: engine.Plan, K: int) -> int:
+        """Crystallites per slice: about slice_flops of the plan's work, at least min_slice (so
+        that every launch still spreads over the device), at most the whole powder."""
+        per = max(p.stats()["flops_per_cs"] * max(p.steps, 1), 1.0)
+        return int(min(max(K, 1), max(self.min_slice, self.slice_flops // per)))
+
+    def _issue(self, sl: _Slot):
+        """Issue the request's next slices while fewer than `depth` are in flight."""
+        K = len(sl.req.powder)
+        while sl.next_c < K and len(sl.inflight) < self.depth:
+            c1 = min(K, sl.next_c + sl.slice_k)
+            sl.plan.run_slice(sl.next_c, c1, True, stream=sl.stream.value)
+            ev = next(e for e in sl.events if all(e.value != x.value for x in sl.inflight))
+            self.hip.hipEventRecord(ev, sl.stream)
+            sl.inflight.append(ev)
+            sl.next_c = c1
 
     def _fail(self, r: Request, err: Exception, t: float, done: list, on_done):
         """Completes a request with its error (bad system, unsupported input, out of memory):
@@ -204,26 +237,45 @@ class FIDServer:
                 share = max(need, min(room, need * 4))
                 queue.popleft()
                 sl = free[0]
+                sl.req, sl.plan, sl.key = r, p, k
+                sl.next_c, sl.inflight = 0, []
                 try:
                     p.set_memory_budget(share)
                     p.set_operators(r.rho, r.det)
                     p.set_powder(r.powder)
-                    p.run(True, stream=sl.stream.value)
+                    sl.slice_k = self.slice_size(p, len(r.powder))
+                    self._issue(sl)
                 except engine.BlochLibError as e:
                     # the plan may hold work enqueued before the failure: it is retired, not
                     # cached, and destroyed once nothing is in flight
                     self.graveyard.append((p, 0.0))
                     self._fail(r, e, clock() - t0, done, on_done)
+                    sl.req, sl.plan, sl.key, sl.inflight = None, None, None, []
                     progressed = True
                     continue
-                self.hip.hipEventRecord(sl.event, sl.stream)
-                sl.req, sl.plan, sl.key = r, p, k
                 r.footprint = share
                 r.t_admit = clock() - t0
                 self.used += share
                 progressed = True
             for sl in self.slots:
-                if sl.req is not None and self.hip.hipEventQuery(sl.event) == 0:
+                if sl.req is None:
+                    continue
+                while sl.inflight and self.hip.hipEventQuery(sl.inflight[0]) == 0:
+                    sl.inflight.pop(0)
+                    progressed = True
+                K = len(sl.req.powder)
+                if sl.next_c < K:
+                    try:
+                        self._issue(sl)
+                    except engine.BlochLibError as e:
+                        r = sl.req
+                        self.used -= r.footprint
+                        self.graveyard.append((sl.plan, 0.0))
+                        self._fail(r, e, clock() - t0, done, on_done)
+                        sl.req, sl.plan, sl.key, sl.inflight = None, None, None, []
+                        progressed = True
+                    continue
+                if not sl.inflight:
                     r = sl.req
                     self.used -= r.footprint
                     try:

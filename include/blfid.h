@@ -180,6 +180,28 @@ blfid_status blfid_state_trace(blfid_state *state, const double *det, int normal
  * (again) after this call. */
 blfid_status blfid_plan_set_state(blfid_plan *plan, blfid_state *state);
 
+/* The plan's whole run (every crystallite batch's launches) recorded once as a HIP graph and
+ * launched as one: no reference counterpart (the reference has no device launches), for callers
+ * that repeat one plan (a fixed acquisition over many parameter sets, the benchmark's steps).
+ * Capture allocates the plan's working storage and records, without running anything; launch
+ * runs the recorded work on `stream` and blfid_plan_fetch reads the result, identical to
+ * blfid_plan_run's.  A state attached to the plan is read at launch time.  Any later change to
+ * the plan (operators, powder, preparation, state, workspace, budget, profiling) makes launch
+ * return BLFID_ERR_INVALID until the next capture.  Capture needs the plan's own working storage
+ * (no shared workspace) and profiling off.  Graph launches and runs of one plan share its
+ * buffers: keep them on one stream. */
+blfid_status blfid_plan_graph_capture(blfid_plan *plan, int normalize);
+blfid_status blfid_plan_graph_launch(blfid_plan *plan, void *stream);
+
+/* The run in slices of the powder: crystallites [c_begin, c_end) of the plan's powder, c_begin
+ * == 0 starting the FID and c_end == K finishing it (then blfid_plan_fetch as after a run).
+ * Consecutive slices (each starting where the previous one ended) give blfid_plan_run's FID bit
+ * for bit.  No reference counterpart: it lets a server interleave the slices of a large request
+ * with other requests on the device.  Needs the plan's own working storage; the plan must not
+ * change, or be run otherwise, between its first and last slice. */
+blfid_status blfid_plan_run_slice(blfid_plan *plan, int c_begin, int c_end, int normalize,
+                                  void *stream);
+
 /* Adjusted sweep width n*wr (compute.h:96, oneFID.h:429), steps per rotor period n, N. */
 double blfid_plan_sw(const blfid_plan *plan);
 int blfid_plan_steps(const blfid_plan *plan);

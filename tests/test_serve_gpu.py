@@ -32,8 +32,10 @@ def rel(a, b):
     return np.linalg.norm(a - b) / np.linalg.norm(b)
 
 
-@pytest.mark.parametrize("budget_factor", [None, 1.5])
-def test_mix_equals_solo_runs(budget_factor):
+@pytest.mark.parametrize("budget_factor,sliced", [(None, False), (1.5, False), (None, True)])
+def test_mix_equals_solo_runs(budget_factor, sliced):
+    """sliced: every request cut into slices of 7 crystallites (blfid_plan_run_slice), two in
+    flight per stream, so the slices of the four streams interleave on the device."""
     reqs = small_mix()
     budget = None
     if budget_factor:
@@ -46,7 +48,8 @@ def test_mix_equals_solo_runs(budget_factor):
             p.close()
         budget = budget_factor * max(fps)
     print(f"[serve] {len(reqs)} requests, budget {budget}", flush=True)
-    srv = serve.FIDServer(budget_bytes=budget, streams=4)
+    kw = dict(slice_flops=1.0, min_slice=7) if sliced else {}
+    srv = serve.FIDServer(budget_bytes=budget, streams=4, **kw)
     try:
         done = srv.run(reqs)
     finally:
