@@ -356,14 +356,17 @@ def bench_serve(args):
     reqs = serve.config5_requests(args.requests, max_spins=args.max_spins)
     cs = sum(len(r.powder) * int(round(r.sw / r.wr)) for r in reqs)
     print(f"[bench] serving {len(reqs)} requests ({cs / 1e6:.1f} M crystallite-steps)", flush=True)
-    srv = serve.FIDServer(streams=4, precision=args.precision,
+    srv = serve.FIDServer(streams=4, precision=args.precision, policy=args.serve_policy,
                           slice_flops=args.slice_flops if args.slice_flops > 0 else float("inf"))
     warm = serve.config5_requests(2, seed=999, max_spins=4)
     srv.run(warm)
     t = time.perf_counter()
     done = srv.run(reqs, on_done=lambda r: print(f"[bench] request {r.rid} S={r.system.spins} "
-                                                   f"done at {r.t_done:.2f}s", flush=True))
+                                                   f"arrived {r.arrival:.2f}s admitted "
+                                                   f"{r.t_admit:.2f}s done at {r.t_done:.2f}s",
+                                                   flush=True))
     span = time.perf_counter() - t
+    switches = srv.switches
     srv.close()
     lat = np.array([r.latency for r in done]) * 1e3
     # accuracy: the smallest requests against the oracle
@@ -391,7 +394,8 @@ def bench_serve(args):
                            "max": float(lat[m].max())}
             for lo, hi in ((2, 6), (7, 8), (9, 10))
             for m in [np.array([lo <= r.system.spins <= hi for r in done])] if m.any()},
-        "slice_flops": args.slice_flops,
+        "slice_flops": args.slice_flops, "policy": args.serve_policy,
+        "stream_switches": switches,
         "rel_l2_vs_oracle": worst,
     }
     print(json.dumps(out), flush=True)
@@ -424,6 +428,14 @@ def measure(H, wls, ptop, precision, steps, warmup, args, calibrate=False):
         plans.append(p)
     print(f"[bench] {precision}: {len(plans)} plans resident in "
           f"{time.perf_counter() - t0:.1f}s", flush=True)
+    tuning = None
+    if not ptop and args.tune_crystallites > 0:
+        # launch choices timed on the leading crystallites of every plan (outside the timed
+        # region, like plan creation); the same FIDs, bit for bit or within the c64 tolerance
+        t1 = time.perf_counter()
+        tuning = engine.autotune(plans, crystallites=args.tune_crystallites)
+        tuning["seconds"] = time.perf_counter() - t1
+        print(f"[bench] {precision}: autotuned in {tuning['seconds']:.1f}s: {tuning}", flush=True)
 
     def step(events=None):
         for i, p in enumerate(plans):
@@ -476,7 +488,8 @@ def measure(H, wls, ptop, precision, steps, warmup, args, calibrate=False):
         H.hipEventDestroy(e)
     for st in streams:
         H.hipStreamDestroy(st)
-    return {"elapsed": elapsed, "lat": lat, "agg": agg, "fids": fids, "streams": nst}
+    return {"elapsed": elapsed, "lat": lat, "agg": agg, "fids": fids, "streams": nst,
+            "tuning": tuning}
 
 
 def dense_config3(H, precision: str, K: int = 64) -> dict:
@@ -531,6 +544,9 @@ def main():
     ap.add_argument("--max-spins", type=int, default=10, help="largest cluster of the serving mix")
     ap.add_argument("--slice-flops", type=float, default=4e12,
                     help="serving: work per powder slice of a request (0: whole requests)")
+    ap.add_argument("--serve-policy", choices=["srpt", "fifo"], default="srpt",
+                    help="serving: streams go to the least remaining work (switched at slice "
+                         "boundaries) or to requests in arrival order")
     ap.add_argument("--sets", type=int, default=8, help="parameter sets per step (ptop)")
     ap.add_argument("--precision", choices=["fp64", "c64", "bf16x3"], default="c64",
                     help="complex64 propagators/products with FP64 eigen-solver, phases and "
@@ -538,6 +554,9 @@ def main():
                          "form with its complex products as three bf16 products each")
     ap.add_argument("--no-fp64-form", action="store_true",
                     help="skip timing the FP64 form beside the c64 headline")
+    ap.add_argument("--tune-crystallites", type=int, default=32,
+                    help="autotune the launch choices on this many leading crystallites of "
+                         "every plan before the warm-up (0: the default choices)")
     ap.add_argument("--dense-k", type=int, default=64,
                     help="crystallites of the dense 10-spin configuration timed beside the "
                          "headline (0: skip)")
@@ -614,6 +633,7 @@ def main():
                         "tflops": v["flops"] / max(v["ms"], 1e-9) / 1e9}
                     for k, v in sorted(agg.items(), key=lambda kv: -kv[1]["ms"])},
         "roofline": roof,
+        "tuning": m.get("tuning"),
         "families": {k: family_roofline(k, v, args.precision)
                      for k, v in sorted(fams.items(), key=lambda kv: -kv[1]["ms"])},
     }

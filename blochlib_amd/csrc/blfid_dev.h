@@ -23,6 +23,9 @@ struct DevPlan {
     const double *ffconst; // S x S: constant flip-flop coefficients (strong J / 2)
     const int *ff_off;     // S x S + 1: per ordered spin pair, its run in ff_term
     const int *ff_term;    // terms with a flip-flop part, listed under both orders of their pair
+    const double *xd_cst;  // N x 7: diagonal of each basis state as (1, w) . xd_cst, w the six
+                           // quadratic products of v = R_P^T u (fused kernel's H build)
+    const double *xd_fpr;  // S x S x 7: flip-flop coefficient of each ordered pair, likewise
     const double *u3;      // n x 3: row 2 of cartSpin at each step midpoint
     const double *RP;      // K x 9: cartPowder of each crystallite (row-major)
     const int *blk_dim;    // nblocks
@@ -60,7 +63,22 @@ struct DevPlan {
                            // precision, conj(rho_JI) at element r + s b_I), or NULL
     int corr_auto_chunks;  // leading correlation chunks whose elements have no reverse element
                            // (rt_from_q plans): they take the autocorrelation kernel
+    // launch choices of the autotuner (0 everywhere: the defaults)
+    int tn_cap[9];         // persistent-grid cap (workgroups) per kernel family, indexed like the
+                           // profiling phases: exp 0, dyson 1, eig 2, bmul 3, qr 4, corr 5,
+                           // reduce 6, static 7, expdyson 8
+    int tn_corr;           // k_corr variant: 0 four FID points per thread up to 1024 points
+                           // (eight above) at the default occupancy, 1 the same at two waves
+                           // per SIMD (no register spill), 2 eight points per thread
 };
+
+// Persistent-grid cap of a kernel family: the tuned value or 4096 workgroups; single-wave
+// workgroups (tile class 1) take four times as many.
+#define GRID_CAP 4096
+inline long tuned_cap(const DevPlan *P, int fam, int M) {
+    const long c = P->tn_cap[fam] > 0 ? P->tn_cap[fam] : GRID_CAP;
+    return M == 1 ? 4 * c : c;
+}
 
 // Correlation work split (k_corr), shared by the host planner and the kernel: a crystallite's
 // Etot elements are cut into chunks of CH elements (CH = min(CORR_SPAN / n, 256 / ceil(n / 8)))

@@ -105,6 +105,22 @@ static blfid_status fail(blfid_status s, const std::string &msg) {
                                               hipGetErrorString((hipError_t)_e));             \
     } while (0)
 
+// Process-wide launch tuning per device (blfid_tune_*): all zero (the defaults) at start.
+#define TUNE_MAX_DEVICES 64
+static std::mutex g_tune_mu;
+static int g_tune[TUNE_MAX_DEVICES][BLFID_TUNE_LEN];
+
+static int tune_device() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= TUNE_MAX_DEVICES) dev = 0;
+    return dev;
+}
+
+static void tune_from_process(int *vals) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    std::memcpy(vals, g_tune[tune_device()], sizeof(int) * BLFID_TUNE_LEN);
+}
+
 static const double PI2 = 6.283185307179586476925286766559;
 static const double DEG2RAD = 0.017453292519943295769236907684886;
 
@@ -213,6 +229,7 @@ struct blfid_plan {
     double sw_adj = 0, tau = 0;
     std::vector<double> terms, dconst, ffconst, u3;
     std::vector<int> ff_off, ff_term; // flip-flop terms of each ordered spin pair (CSR)
+    std::vector<double> xd_cst, xd_fpr; // quadratic-form tables of the fused kernel's H build
     std::vector<Block> blocks;
     std::vector<int> blk_dim, blk_soff, blk_states;
     std::vector<long> blk_uoff;
@@ -249,6 +266,7 @@ struct blfid_plan {
     // device
     DevBuf<double> d_terms, d_dconst, d_ffconst, d_u3, d_RP, d_th, d_ph, d_ga, d_w;
     DevBuf<int> d_ff_off, d_ff_term;
+    DevBuf<double> d_xd_cst, d_xd_fpr;
     DevBuf<int> d_blk_dim, d_blk_soff, d_blk_states, d_pair_I, d_pair_J, d_pair_doff,
         d_pair_roff, d_drow, d_dcol, d_rrow, d_rcol, d_er, d_es, d_erev;
     DevBuf<long> d_blk_uoff, d_pair_eoff;
@@ -285,6 +303,8 @@ struct blfid_plan {
     hipStream_t last_stream = nullptr;
     bool ran = false;
     DevPlan dp{};
+    // launch choices (blfid_tune_*): taken from the device's process table at creation
+    int tn[BLFID_TUNE_LEN] = {0};
     // profiling
     bool prof = false;
     struct Ev { int phase; hipEvent_t a, b; double flops, bytes; };
@@ -440,6 +460,32 @@ static blfid_status build_tables(blfid_plan *pl) {
         for (size_t q = 0; q < lists.size(); q++) {
             pl->ff_term.insert(pl->ff_term.end(), lists[q].begin(), lists[q].end());
             pl->ff_off[q + 1] = (int)pl->ff_term.size();
+        }
+        // The same Hamiltonian as quadratic forms in v = R_P^T u for the fused kernel: with
+        // w = (vx^2, vy^2, vz^2, 2 vx vy, 2 vx vz, 2 vy vz) every term's zz_t = C_t . w, so the
+        // diagonal of basis state a is xd_cst[a] . (1, w) and the flip-flop coefficient of the
+        // ordered pair (i, j) is xd_fpr[i S + j] . (1, w) (entry 0 holds the constant part).
+        const int nt = (int)(pl->terms.size() / DEV_TERM_STRIDE);
+        pl->xd_cst.assign((size_t)N * 7, 0.0);
+        for (int a = 0; a < N; a++) {
+            double *o = &pl->xd_cst[(size_t)a * 7];
+            o[0] = pl->dconst[a];
+            for (int t = 0; t < nt; t++) {
+                const double *c = &pl->terms[DEV_TERM_STRIDE * t];
+                const int i = (int)c[8], j = (int)c[9];
+                double f = c[6] * mz(a, i);
+                if (j >= 0) f *= mz(a, j);
+                for (int q = 0; q < 6; q++) o[1 + q] += f * c[q];
+            }
+        }
+        pl->xd_fpr.assign((size_t)S * S * 7, 0.0);
+        for (size_t q = 0; q < lists.size(); q++) {
+            double *o = &pl->xd_fpr[q * 7];
+            o[0] = pl->ffconst[q];
+            for (int t : lists[q]) {
+                const double *c = &pl->terms[DEV_TERM_STRIDE * t];
+                for (int k = 0; k < 6; k++) o[1 + k] += c[7] * c[k];
+            }
         }
     }
     // blocks: union-find over states, edges = flip-flops of coupled pairs
@@ -614,6 +660,8 @@ static blfid_status upload_tables(blfid_plan *pl) {
     HIPCHK(pl->d_ffconst.upload(pl->ffconst));
     HIPCHK(pl->d_ff_off.upload(pl->ff_off));
     HIPCHK(pl->d_ff_term.upload(pl->ff_term));
+    HIPCHK(pl->d_xd_cst.upload(pl->xd_cst));
+    HIPCHK(pl->d_xd_fpr.upload(pl->xd_fpr));
     HIPCHK(pl->d_u3.upload(pl->u3));
     HIPCHK(pl->d_blk_dim.upload(pl->blk_dim));
     HIPCHK(pl->d_blk_soff.upload(pl->blk_soff));
@@ -658,6 +706,8 @@ static void refresh_devplan(blfid_plan *pl) {
     d.terms = pl->d_terms.p;
     d.ff_off = pl->d_ff_off.p;
     d.ff_term = pl->d_ff_term.p;
+    d.xd_cst = pl->d_xd_cst.p;
+    d.xd_fpr = pl->d_xd_fpr.p;
     d.dconst = pl->d_dconst.p;
     d.ffconst = pl->d_ffconst.p;
     d.u3 = pl->d_u3.p;
@@ -688,6 +738,8 @@ static void refresh_devplan(blfid_plan *pl) {
     d.rho_c = pl->bufRho;
     d.elem_r = pl->d_er.p;
     d.elem_s = pl->d_es.p;
+    for (int f = 0; f < 9; f++) d.tn_cap[f] = pl->tn[6 + f];
+    d.tn_corr = pl->tn[5];
 }
 
 // ------------------------------------------------------------------------------ C ABI
@@ -754,6 +806,7 @@ static blfid_plan *new_plan(const blfid_system *sys, double wr_hz, double rotor_
     pl->rotor_deg = rotor_deg;
     pl->t0 = t0;
     pl->npts = npts;
+    tune_from_process(pl->tn);
     return pl;
 }
 
@@ -1489,19 +1542,22 @@ static blfid_status enqueue_range(blfid_plan *pl, const blfid_layout::Shape &sh,
             }
             // complex64 matrix-core plans: blocks of <= 80 states take the fused propagator +
             // Dyson chain (U_k never leaves LDS); larger blocks, the FP64 form and the BF16x3
-            // form take the propagator and the Dyson chain as two kernels
+            // form take the propagator and the Dyson chain as two kernels; a tuning may send a
+            // tile class of the fused path to the two kernels
             const bool fused = pr == 1 && !pl->bf16x3;
+            // (tile variant, device group list, host group list) of the two-kernel path
+            std::vector<std::pair<int, std::pair<const int *, const std::vector<int> *>>> two;
             if (fused) {
                 for (auto &kv : pl->fuse_groups) {
+                    if (pl->tn[kv.first - 1] == 1) {
+                        two.push_back({kv.first, {pl->d_fgrp[kv.first].p, &kv.second}});
+                        continue;
+                    }
                     PhaseScope ps(pl, 8 * 8 + kv.first, 0.0,
                                   kb * (double)pl->n * group_b2(pl, kv.second) * cs, st); // flops: device
                     LAUNCHCHK(blfid_dev_launch_expdyson(kv.first, P, kb, c0, pl->d_fgrp[kv.first].p,
                                                         (int)kv.second.size(), pl->bufA, st));
                 }
-            }
-            // (tile variant, device group list, host group list) of the two-kernel path
-            std::vector<std::pair<int, std::pair<const int *, const std::vector<int> *>>> two;
-            if (fused) {
                 if (!pl->big_blocks.empty()) two.push_back({5, {pl->d_bigb.p, &pl->big_blocks}});
             } else {
                 for (auto &kv : pl->blk_groups) two.push_back({kv.first, {pl->d_bgrp[kv.first].p, &kv.second}});
@@ -1612,11 +1668,18 @@ static blfid_status enqueue_run(blfid_plan *pl, const blfid_layout::Shape &sh, c
     return enqueue_range(pl, sh, base, off, Kb, normalize, out, st, 0, pl->K, true, true);
 }
 
+// The run of crystallites [0, c_end) on the plan's working storage (the whole powder for
+// blfid_plan_run; a leading sample for the autotuner, whose FID is not a result).
+static blfid_status run_upto(blfid_plan *pl, int normalize, hipStream_t st, int c_end);
+
 blfid_status blfid_plan_run(blfid_plan *pl, int normalize, void *stream) {
     if (!pl) return fail(BLFID_ERR_INVALID, "plan is NULL");
+    return run_upto(pl, normalize, (hipStream_t)stream, pl->K);
+}
+
+static blfid_status run_upto(blfid_plan *pl, int normalize, hipStream_t st, int c_end) {
     if (!pl->have_ops) return fail(BLFID_ERR_INVALID, "operators not set");
     if (normalize && pl->trdd == 0.0) return fail(BLFID_ERR_INVALID, "Tr(det det^H) is zero");
-    hipStream_t st = (hipStream_t)stream;
     pl->last_stream = st;
     blfid_layout::Shape sh, psh;
     full_shape(pl, sh, psh);
@@ -1647,8 +1710,9 @@ blfid_status blfid_plan_run(blfid_plan *pl, int normalize, void *stream) {
         base = pl->own_ws.p;
     }
     HIPCHK(pl->fid_out.alloc((size_t)2 * pl->npts));
-    blfid_status rs = enqueue_run(pl, sh, base, off, Kb, normalize, pl->fid_out.p, st);
-    if (rs == BLFID_OK) pl->ran = true;
+    blfid_status rs = enqueue_range(pl, sh, base, off, Kb, normalize, pl->fid_out.p, st, 0,
+                                    c_end, true, true);
+    pl->ran = rs == BLFID_OK && c_end == pl->K;
     if (pl->ws) {
         // the next user of the workspace waits for whatever was enqueued, on every exit
         hipError_t e = hipEventRecord(pl->ws->ev, st);
@@ -2155,6 +2219,184 @@ blfid_status blfid_plan_phase_bytes(blfid_plan *pl, double *bytes) {
     if (!pl || !bytes) return fail(BLFID_ERR_INVALID, "NULL argument");
     collect_profile(pl);
     for (int i = 0; i < BLFID_NSLOT; i++) bytes[i] = pl->ph_bytes[i];
+    return BLFID_OK;
+}
+
+// ------------------------------------------------------------------------------ launch autotuner
+static blfid_status check_tuning(const int *v) {
+    if (!v) return fail(BLFID_ERR_INVALID, "tuning is NULL");
+    for (int M = 0; M < 5; M++)
+        if (v[M] != 0 && v[M] != 1) return fail(BLFID_ERR_INVALID, "propagator method is 0 or 1");
+    if (v[5] < 0 || v[5] > 2) return fail(BLFID_ERR_INVALID, "correlation variant is 0, 1 or 2");
+    for (int f = 6; f < 15; f++)
+        if (v[f] < 0 || v[f] > (1 << 20))
+            return fail(BLFID_ERR_INVALID, "grid cap outside 0..1048576 workgroups");
+    if (v[15] != 0) return fail(BLFID_ERR_INVALID, "reserved tuning entry must be 0");
+    return BLFID_OK;
+}
+
+blfid_status blfid_tune_get(int *vals) {
+    if (!vals) return fail(BLFID_ERR_INVALID, "vals is NULL");
+    tune_from_process(vals);
+    return BLFID_OK;
+}
+
+blfid_status blfid_tune_set(const int *vals) {
+    blfid_status s = check_tuning(vals);
+    if (s != BLFID_OK) return s;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    std::memcpy(g_tune[tune_device()], vals, sizeof(int) * BLFID_TUNE_LEN);
+    return BLFID_OK;
+}
+
+blfid_status blfid_plan_tune_get(const blfid_plan *pl, int *vals) {
+    if (!pl || !vals) return fail(BLFID_ERR_INVALID, "NULL argument");
+    std::memcpy(vals, pl->tn, sizeof(int) * BLFID_TUNE_LEN);
+    return BLFID_OK;
+}
+
+blfid_status blfid_plan_tune_set(blfid_plan *pl, const int *vals) {
+    if (!pl) return fail(BLFID_ERR_INVALID, "plan is NULL");
+    blfid_status s = check_tuning(vals);
+    if (s != BLFID_OK) return s;
+    pl->gen++; // a recorded graph no longer matches the plan
+    if (pl->last_stream) HIPCHK(hipStreamSynchronize(pl->last_stream));
+    std::memcpy(pl->tn, vals, sizeof(int) * BLFID_TUNE_LEN);
+    refresh_devplan(pl);
+    return BLFID_OK;
+}
+
+// Crystallites a plan is tuned on: at least `kc`, and enough that its per-(crystallite, block)
+// launches (the fused propagator's, the eigen-solver's) have 2048 workgroups, as the full runs'
+// batches do -- a small sample would favour the kernels with more tasks per crystallite.
+static int tune_sample(const blfid_plan *pl, int kc) {
+    const int nb = std::max<int>(1, (int)pl->blocks.size());
+    return std::min(pl->K, std::max(kc, (2048 + nb - 1) / nb));
+}
+
+// Kernel time per profiling slot, summed over the plans, of one candidate tuning: each plan runs
+// its tuning sample twice and keeps, per slot, the faster of the two.
+static blfid_status tune_pass(blfid_plan **plans, int nplans, int kc, hipStream_t st,
+                              const int *vals, double *slot_ms) {
+    for (int i = 0; i < BLFID_NSLOT; i++) slot_ms[i] = 0;
+    for (int p = 0; p < nplans; p++) {
+        blfid_plan *pl = plans[p];
+        std::memcpy(pl->tn, vals, sizeof(int) * BLFID_TUNE_LEN);
+        pl->gen++;
+        double best[BLFID_NSLOT];
+        for (int rep = 0; rep < 2; rep++) {
+            blfid_status s = blfid_plan_set_profiling(pl, 1);
+            if (s == BLFID_OK) s = run_upto(pl, 0, st, tune_sample(pl, kc));
+            if (s != BLFID_OK) return s;
+            collect_profile(pl);
+            for (int i = 0; i < BLFID_NSLOT; i++)
+                best[i] = rep ? std::min(best[i], pl->ph_ms[i]) : pl->ph_ms[i];
+        }
+        for (int i = 0; i < BLFID_NSLOT; i++) slot_ms[i] += best[i];
+    }
+    return BLFID_OK;
+}
+
+blfid_status blfid_autotune(blfid_plan **plans, int nplans, int crystallites, void *stream,
+                            double *report) {
+    if (!plans || nplans < 1) return fail(BLFID_ERR_INVALID, "no plans");
+    if (crystallites < 1) return fail(BLFID_ERR_INVALID, "crystallites must be positive");
+    for (int p = 0; p < nplans; p++) {
+        const blfid_plan *pl = plans[p];
+        if (!pl) return fail(BLFID_ERR_INVALID, "plan is NULL");
+        if (!pl->spinning || pl->ptop)
+            return fail(BLFID_ERR_UNSUPPORTED, "the autotuner takes gamma-COMPUTE plans");
+        if (!pl->have_ops) return fail(BLFID_ERR_INVALID, "operators not set");
+        if (pl->K < 1) return fail(BLFID_ERR_INVALID, "empty powder");
+    }
+    hipStream_t st = (hipStream_t)stream;
+    const double keep = 0.97; // a candidate replaces the current choice when 3% faster
+    int cur[BLFID_TUNE_LEN];
+    std::memcpy(cur, plans[0]->tn, sizeof(cur));
+    std::vector<int> orig((size_t)nplans * BLFID_TUNE_LEN);
+    std::vector<char> prof0(nplans);
+    for (int p = 0; p < nplans; p++) {
+        std::memcpy(&orig[(size_t)p * BLFID_TUNE_LEN], plans[p]->tn, sizeof(cur));
+        prof0[p] = plans[p]->prof;
+    }
+    std::vector<double> T(BLFID_TUNE_PASSES * BLFID_NSLOT, 0.0);
+    auto slot = [&](int pass, int i) -> double & { return T[(size_t)pass * BLFID_NSLOT + i]; };
+    blfid_status s = BLFID_OK;
+    // passes 0-2: fused (0, 2) or two-kernel (1) propagator per tile class, correlation variant
+    // 0, 1, 2
+    for (int c = 0; c < 3 && s == BLFID_OK; c++) {
+        int v[BLFID_TUNE_LEN];
+        std::memcpy(v, cur, sizeof(v));
+        for (int M = 0; M < 5; M++) v[M] = c == 1 ? 1 : 0;
+        v[5] = c;
+        s = tune_pass(plans, nplans, crystallites, st, v, &slot(c, 0));
+    }
+    if (s == BLFID_OK) {
+        for (int M = 1; M <= 5; M++) {
+            const double fused = std::min(slot(0, 64 + M), slot(2, 64 + M));
+            // the two-kernel cost of the class: its exp and Dyson slots beyond what the fused
+            // pass spent there (blocks above 80 states share class 5's slots)
+            const double two = slot(1, M) + slot(1, 8 + M) - slot(0, M) - slot(0, 8 + M);
+            if (fused <= 0.0) continue; // no block of this class is fused in these plans
+            const double now = cur[M - 1] ? two : fused, alt = cur[M - 1] ? fused : two;
+            if (alt < keep * now) cur[M - 1] ^= 1;
+        }
+        double cbest = slot(cur[5], 40);
+        for (int c = 0; c < 3; c++)
+            if (slot(c, 40) < keep * cbest) {
+                cbest = slot(c, 40);
+                cur[5] = c;
+            }
+    }
+    // passes 3-6: one grid cap for every tuned family; each family then takes its fastest
+    const int caps[4] = {1024, 2048, 4096, 8192};
+    const int fams[7] = {0, 1, 2, 3, 4, 5, 8};
+    for (int j = 0; j < 4 && s == BLFID_OK; j++) {
+        int v[BLFID_TUNE_LEN];
+        std::memcpy(v, cur, sizeof(v));
+        for (int f : fams) v[6 + f] = caps[j];
+        s = tune_pass(plans, nplans, crystallites, st, v, &slot(3 + j, 0));
+    }
+    if (s == BLFID_OK) {
+        for (int f : fams) {
+            double fam[4];
+            for (int j = 0; j < 4; j++) {
+                fam[j] = 0;
+                for (int M = 0; M < 8; M++) fam[j] += slot(3 + j, f * 8 + M);
+            }
+            const int c0 = cur[6 + f] ? cur[6 + f] : 4096;
+            int jc = 2;
+            for (int j = 0; j < 4; j++)
+                if (caps[j] == c0) jc = j;
+            int jb = jc;
+            for (int j = 0; j < 4; j++)
+                if (fam[j] < keep * fam[jb]) jb = j;
+            cur[6 + f] = caps[jb] == 4096 ? 0 : caps[jb];
+        }
+    }
+    // every plan leaves with the chosen tuning (its own after a failure), its profiling switch
+    // as it was and its counters cleared
+    for (int p = 0; p < nplans; p++) {
+        blfid_plan *pl = plans[p];
+        std::memcpy(pl->tn, s == BLFID_OK ? cur : &orig[(size_t)p * BLFID_TUNE_LEN], sizeof(cur));
+        pl->gen++;
+        if (pl->last_stream) (void)hipStreamSynchronize(pl->last_stream);
+        collect_profile(pl);
+        const blfid_status s2 = blfid_plan_set_profiling(pl, prof0[p]);
+        if (s == BLFID_OK) s = s2;
+        refresh_devplan(pl);
+        pl->ran = false;
+    }
+    if (s != BLFID_OK) return s;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        std::memcpy(g_tune[tune_device()], cur, sizeof(cur));
+    }
+    if (report)
+        for (int j = 0; j < BLFID_TUNE_PASSES; j++) {
+            report[j] = 0;
+            for (int i = 0; i < BLFID_NSLOT; i++) report[j] += slot(j, i);
+        }
     return BLFID_OK;
 }
 

@@ -980,74 +980,47 @@ __device__ __forceinline__ void sym_store(float *PL, const v4f (&acc)[NS], float
 }
 
 // Hamiltonian of one step for the fused kernel, in three phases separated by barriers:
-//   A: v = R_P^T u (every thread), zz_t = v^T C_t v for every anisotropic term and the flip-flop
-//      coefficient of every ordered spin pair (its terms' zz recomputed in place, so A has no
-//      internal dependency);
-//   B: diagonal of each row (terms split over lanes, combined by shuffles) and its row sum of
-//      absolute values (diagonal plus every flip-flop partner), the block's infinity norm per wave;
+//   A: v = R_P^T u and its six quadratic products w (every thread, in registers); the diagonal
+//      of each row, xd_cst[a] . (1, w) for its basis state a, and the flip-flop coefficient of
+//      every ordered spin pair, xd_fpr . (1, w) (both tables formed on the host from the same
+//      terms: seven FP64 multiply-adds per entry instead of a pass over every term);
+//   B: each row's sum of absolute values (diagonal plus every flip-flop partner), the block's
+//      infinity norm per wave;
 //   C: X = theta H / 2^s rounded once from FP64 -- its diagonal and the block's fixed flip-flop
 //      positions (xd_list) -- scattered into the plane, s the least with ||X||_inf <= rmax.
 // Every other entry of the plane is zero and stays zero.  Returns ||H||_inf and s.  Same
-// values as build_h_block (sys.cc:749-770; csa.cc:289-329, dip.cc:171-207).
+// Hamiltonian as build_h_block (sys.cc:749-770; csa.cc:289-329, dip.cc:171-207), its sums
+// associated differently.
 template <int NT>
-__device__ double xd_build_h(const DevPlan &P, const double *terms, const double *RPc,
-                             const double *u, int blk, float *PX, int LDK, double *s_zz,
-                             double *s_ff, double *s_diag, double *s_red, double theta,
-                             double rmax_x, int &sq) {
-    const int tid = threadIdx.x, S = P.S, nt = P.nterms;
+__device__ double xd_build_h(const DevPlan &P, const double *RPc, const double *u, int blk,
+                             float *PX, int LDK, double *s_ff, double *s_diag, double *s_red,
+                             double theta, double rmax_x, int &sq) {
+    const int tid = threadIdx.x, S = P.S;
     const double vx = RPc[0] * u[0] + RPc[3] * u[1] + RPc[6] * u[2];
     const double vy = RPc[1] * u[0] + RPc[4] * u[1] + RPc[7] * u[2];
     const double vz = RPc[2] * u[0] + RPc[5] * u[1] + RPc[8] * u[2];
-    auto zzf = [&](int t) {
-        const double *c = terms + DEV_TERM_STRIDE * t;
-        return c[0] * vx * vx + c[1] * vy * vy + c[2] * vz * vz +
-               2.0 * (c[3] * vx * vy + c[4] * vx * vz + c[5] * vy * vz);
+    const double w0 = vx * vx, w1 = vy * vy, w2 = vz * vz;
+    const double w3 = 2.0 * vx * vy, w4 = 2.0 * vx * vz, w5 = 2.0 * vy * vz;
+    auto form = [&](const double *c) {
+        return c[0] + c[1] * w0 + c[2] * w1 + c[3] * w2 + c[4] * w3 + c[5] * w4 + c[6] * w5;
     };
-    for (int x = tid; x < nt + S * S; x += NT) {
-        if (x < nt) {
-            s_zz[x] = zzf(x);
-        } else {
-            const int q = x - nt;
-            double f = P.ffconst[q];
-            for (int z = P.ff_off[q]; z < P.ff_off[q + 1]; z++) {
-                const int t = P.ff_term[z];
-                f += zzf(t) * terms[DEV_TERM_STRIDE * t + 7];
-            }
-            s_ff[q] = f;
-        }
-    }
-    __syncthreads();
     const int b = P.blk_dim[blk];
     const int *st = P.blk_states + P.blk_soff[blk];
-    int tpr = 1;
-    while (tpr < 16 && tpr * 2 * b <= NT) tpr *= 2;
+    for (int x = tid; x < b + S * S; x += NT) {
+        if (x < b) s_diag[x] = form(P.xd_cst + 7 * (size_t)st[x]);
+        else s_ff[x - b] = form(P.xd_fpr + 7 * (x - b));
+    }
+    __syncthreads();
     double rmax = 0.0;
-    for (int r0 = 0; r0 < b; r0 += NT / tpr) {
-        const int r = r0 + tid / tpr, part = tid % tpr;
-        double h = 0.0;
-        int ga = 0;
-        if (r < b) {
-            ga = st[r];
-            for (int t = part; t < nt; t += tpr) {
-                const double *c = terms + DEV_TERM_STRIDE * t;
-                const int i = (int)c[8], j = (int)c[9];
-                double f = c[6] * (((ga >> (S - 1 - i)) & 1) ? -0.5 : 0.5);
-                if (j >= 0) f *= ((ga >> (S - 1 - j)) & 1) ? -0.5 : 0.5;
-                h += s_zz[t] * f;
-            }
+    for (int r = tid; r < b; r += NT) {
+        const int ga = st[r];
+        double sm = fabs(s_diag[r]);
+        for (int i = 0; i < S; i++) {
+            const int bi = (ga >> (S - 1 - i)) & 1;
+            for (int j = i + 1; j < S; j++)
+                if (bi != ((ga >> (S - 1 - j)) & 1)) sm += fabs(s_ff[i * S + j]);
         }
-        for (int off = tpr / 2; off > 0; off >>= 1) h += __shfl_xor(h, off);
-        if (r < b && part == 0) {
-            h += P.dconst[ga];
-            s_diag[r] = h;
-            double sm = fabs(h);
-            for (int i = 0; i < S; i++) {
-                const int bi = (ga >> (S - 1 - i)) & 1;
-                for (int j = i + 1; j < S; j++)
-                    if (bi != ((ga >> (S - 1 - j)) & 1)) sm += fabs(s_ff[i * S + j]);
-            }
-            rmax = fmax(rmax, sm);
-        }
+        rmax = fmax(rmax, sm);
     }
     for (int off = 32; off > 0; off >>= 1) rmax = fmax(rmax, __shfl_xor(rmax, off));
     if ((tid & 63) == 0) s_red[tid >> 6] = rmax;
@@ -1097,12 +1070,10 @@ __global__ void __launch_bounds__(XdThreads<M>::v) k_expdyson(DevPlan P, int Kb,
     __shared__ __align__(16) float pl[5 * T::PLANE];
     float *PX = pl, *PY = PX + T::PLANE, *PZ = PY + T::PLANE, *PBr = PZ + T::PLANE,
           *PBi = PBr + T::PLANE;
-    __shared__ double s_zz[DEV_MAX_TERMS];
     __shared__ double s_ff[DEV_MAX_SPINS * DEV_MAX_SPINS];
     __shared__ double s_red[NT / 64];
     __shared__ double s_diag[DEV_LDS_BLOCK];
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63, g = lane >> 4, r = lane & 15;
-    const double *terms = stage_terms<NT>(P);
     const float us = (float)P.usign;
     constexpr double RMAX = 0.75, TOL = 2e-8;
     double nflops = 0.0;
@@ -1119,9 +1090,8 @@ __global__ void __launch_bounds__(XdThreads<M>::v) k_expdyson(DevPlan P, int Kb,
         for (int k = 0; k < P.n; k++) {
             const double theta = P.theta;
             int sq = 0;
-            const double hn = xd_build_h<NT>(P, terms, P.RP + 9 * (size_t)(c0 + c), P.u3 + 3 * k,
-                                             blk, PX, LDK, s_zz, s_ff, s_diag, s_red, theta, RMAX,
-                                             sq);
+            const double hn = xd_build_h<NT>(P, P.RP + 9 * (size_t)(c0 + c), P.u3 + 3 * k, blk,
+                                             PX, LDK, s_ff, s_diag, s_red, theta, RMAX, sq);
             const double rx = hn * theta / (double)(1 << sq), r2 = rx * rx;
             const int m = (r2 * r2 / 24.0 <= TOL) ? 1 : (r2 * r2 * r2 / 720.0 <= TOL) ? 2 : 4;
             int ng = 1 + 3 * sq; // products of this step (Y, the squarings; more below)
@@ -1944,9 +1914,9 @@ __device__ __forceinline__ void corr_mid_pass(typename CX<R>::T *sQ, const typen
 // variant for n = 2^LN (CorrFft<LN>), LN = 0 the direct sum for any n.  The complex64 FFT variant
 // with four points is held to three waves per SIMD (168 registers: at four it spilled, and its
 // sixteen loads per chunk then waited on the spill reloads); the others are bound by LDS or
-// registers at two.
-template <typename R, int LN, int NPT, bool AUTO>
-__global__ void __launch_bounds__(256, (LN > 0 && sizeof(R) == 4 && NPT == 4) ? (AUTO ? 4 : 3) : 2) k_corr(DevPlan P, int Kb,
+// registers at two.  OCC > 0 replaces that bound (a variant the launch autotuner may choose).
+template <typename R, int LN, int NPT, bool AUTO, int OCC = 0>
+__global__ void __launch_bounds__(256, OCC > 0 ? OCC : (LN > 0 && sizeof(R) == 4 && NPT == 4) ? (AUTO ? 4 : 3) : 2) k_corr(DevPlan P, int Kb,
                                               const typename CX<R>::T *__restrict__ Qt,
                                               const typename CX<R>::T *__restrict__ Rt,
                                               const double *__restrict__ thetas,
@@ -2579,7 +2549,6 @@ __global__ void __launch_bounds__(NThr<M>::v) k_static_fid(DevPlan P, int Kb, co
     } while (0)
 // prec: 0 = FP64 storage (double2), 1 = complex64 storage (float2) of U, A, B, Q, R and the exp
 // scratch; the eigen-solver, phases, FIDs and reductions are FP64 in both.
-#define GRID_CAP 4096
 
 template <typename R>
 static void launch_exp(int M, const DevPlan *P, int Kb, int c0, const int *grp, int ngrp,
@@ -2600,7 +2569,8 @@ extern "C" int blfid_dev_launch_exp(int prec, int M, const DevPlan *P, int Kb, i
                                     const int *grp, int ngrp, void *scratch, size_t slot,
                                     int nslots, void *bufU, hipStream_t st) {
     long ntask = (long)Kb * P->n * ngrp;
-    int grid = (int)(ntask < nslots ? ntask : nslots);
+    const long cap = P->tn_cap[0] > 0 ? std::min<long>(tuned_cap(P, 0, M), nslots) : nslots;
+    int grid = (int)(ntask < cap ? ntask : cap);
     if (grid <= 0) return 0;
     if (prec) launch_exp<float>(M, P, Kb, c0, grp, ngrp, scratch, slot, grid, bufU, st);
     else launch_exp<double>(M, P, Kb, c0, grp, ngrp, scratch, slot, grid, bufU, st);
@@ -2611,11 +2581,11 @@ extern "C" int blfid_dev_launch_exp(int prec, int M, const DevPlan *P, int Kb, i
 extern "C" int blfid_dev_launch_expdyson(int M, const DevPlan *P, int Kb, int c0, const int *grp,
                                          int ngrp, void *bufA, hipStream_t st) {
     const long ntask = (long)Kb * ngrp;
-    const long cap = M == 1 ? 4 * GRID_CAP : GRID_CAP;
+    const long cap = tuned_cap(P, 8, M);
     const int grid = (int)(ntask < cap ? ntask : cap);
     if (grid <= 0) return 0;
     cf *A = (cf *)bufA;
-    const size_t tb = sizeof(double) * DEV_TERM_STRIDE * (size_t)P->nterms; // staged terms
+    const size_t tb = 0; // the H build reads its quadratic-form tables from global memory
     switch (M) {
     case 1: k_expdyson<1><<<grid, 64, tb, st>>>(*P, Kb, c0, grp, ngrp, A); break;
     case 2: k_expdyson<2><<<grid, 256, tb, st>>>(*P, Kb, c0, grp, ngrp, A); break;
@@ -2644,7 +2614,7 @@ static void launch_dyson(int M, const DevPlan *P, int Kb, const int *grp, int ng
 extern "C" int blfid_dev_launch_dyson(int prec, int M, const DevPlan *P, int Kb, const int *grp,
                                       int ngrp, const void *bufU, void *bufA, hipStream_t st) {
     long ntask = (long)Kb * ngrp;
-    const long cap = M == 1 ? 4 * GRID_CAP : GRID_CAP; // single-wave workgroups: four times as many
+    const long cap = tuned_cap(P, 1, M);
     int grid = (int)(ntask < cap ? ntask : cap);
     if (grid <= 0) return 0;
     if (prec) launch_dyson<float>(M, P, Kb, grp, ngrp, grid, bufU, bufA, st);
@@ -2678,7 +2648,7 @@ extern "C" int blfid_dev_launch_eig(int prec, const DevPlan *P, int Kb, const in
                                     void *bufWr, double *outv, int cls, cd *gscr, size_t gslot,
                                     int nslots, hipStream_t st) {
     long ntask = (long)Kb * ngrp;
-    int cap = cls >= 5 ? nslots : GRID_CAP;
+    long cap = cls >= 5 ? nslots : tuned_cap(P, 2, 2);
     int grid = (int)(ntask < cap ? ntask : cap);
     if (grid <= 0) return 0;
     if (prec) launch_eig<float>(P, Kb, grp, ngrp, grid, src, src_stride, mode, bufW, bufWr, outv, cls, gscr, gslot, st);
@@ -2706,7 +2676,7 @@ extern "C" int blfid_dev_launch_bmul(int prec, int M, const DevPlan *P, int Kb, 
                                      int ngrp, const void *bufA, const void *bufW, void *bufB,
                                      hipStream_t st) {
     long ntask = (long)Kb * P->n * ngrp;
-    const long cap = M == 1 ? 4 * GRID_CAP : GRID_CAP;
+    const long cap = tuned_cap(P, 3, M);
     int grid = (int)(ntask < cap ? ntask : cap);
     if (grid <= 0) return 0;
     if (prec) launch_bmul<float>(M, P, Kb, grp, ngrp, grid, bufA, bufW, bufB, st);
@@ -2734,7 +2704,8 @@ extern "C" int blfid_dev_launch_qr(int prec, int M, const DevPlan *P, int Kb, co
                                    int ngrp, const void *bufB, void *scratch, size_t slot,
                                    int nslots, void *Qt, void *Rt, hipStream_t st) {
     long ntask = (long)Kb * P->n * ngrp;
-    int grid = (int)(ntask < nslots ? ntask : nslots);
+    const long cap = P->tn_cap[4] > 0 ? std::min<long>(tuned_cap(P, 4, M), nslots) : nslots;
+    int grid = (int)(ntask < cap ? ntask : cap);
     if (grid <= 0) return 0;
     if (prec) launch_qr<float>(M, P, Kb, pgrp, ngrp, grid, bufB, scratch, slot, Qt, Rt, st);
     else launch_qr<double>(M, P, Kb, pgrp, ngrp, grid, bufB, scratch, slot, Qt, Rt, st);
@@ -2744,36 +2715,37 @@ extern "C" int blfid_dev_launch_qr(int prec, int M, const DevPlan *P, int Kb, co
 // One launch over the chunks [lo, hi) of every crystallite of the batch, its segments written as
 // partial sums seg_base...; AUTO (FFT sizes only): the autocorrelation variant, for chunks whose
 // elements have R = conj(Q) (rosym, no reverse element)
-template <typename R, int LN, int NPT>
+template <typename R, int LN, int NPT, int OCC>
 static void launch_corr_t(bool autoc, const DevPlan *P, int Kb, const void *Qt, const void *Rt,
                           const double *thetas, const int *er, const int *es, double *part,
                           int lo, int hi, int seg_base, hipStream_t st) {
     typedef typename CX<R>::T CT;
     const long ntask = (long)Kb * corr_range_segments(hi - lo);
-    const int grid = (int)(ntask < GRID_CAP ? ntask : GRID_CAP);
+    const long cap = tuned_cap(P, 5, 2);
+    const int grid = (int)(ntask < cap ? ntask : cap);
     if (grid <= 0) return;
     if (LN > 0 && autoc)
-        k_corr<R, LN, NPT, LN != 0><<<grid, 256, 0, st>>>(*P, Kb, (const CT *)Qt, (const CT *)Rt,
+        k_corr<R, LN, NPT, LN != 0, OCC><<<grid, 256, 0, st>>>(*P, Kb, (const CT *)Qt, (const CT *)Rt,
                                                           thetas, er, es, part, lo, hi, seg_base);
     else
-        k_corr<R, LN, NPT, false><<<grid, 256, 0, st>>>(*P, Kb, (const CT *)Qt, (const CT *)Rt,
+        k_corr<R, LN, NPT, false, OCC><<<grid, 256, 0, st>>>(*P, Kb, (const CT *)Qt, (const CT *)Rt,
                                                         thetas, er, es, part, lo, hi, seg_base);
 }
 // FFT variants by log2 n (corr_use_fft(n): 16 <= n <= 512), LN = 0 the direct sum
-template <typename R, int NPT>
+template <typename R, int NPT, int OCC = 0>
 static void launch_corr(bool autoc, const DevPlan *P, int Kb, const void *Qt, const void *Rt,
                         const double *thetas, const int *er, const int *es, double *part,
                         int lo, int hi, int seg_base, hipStream_t st) {
     const int ln = corr_use_fft(P->n) ? __builtin_ctz((unsigned)P->n) : 0;
 #define CORR_ARGS autoc, P, Kb, Qt, Rt, thetas, er, es, part, lo, hi, seg_base, st
     switch (ln) {
-    case 4: launch_corr_t<R, 4, NPT>(CORR_ARGS); break;
-    case 5: launch_corr_t<R, 5, NPT>(CORR_ARGS); break;
-    case 6: launch_corr_t<R, 6, NPT>(CORR_ARGS); break;
-    case 7: launch_corr_t<R, 7, NPT>(CORR_ARGS); break;
-    case 8: launch_corr_t<R, 8, NPT>(CORR_ARGS); break;
-    case 9: launch_corr_t<R, 9, NPT>(CORR_ARGS); break;
-    default: launch_corr_t<R, 0, NPT>(CORR_ARGS); break;
+    case 4: launch_corr_t<R, 4, NPT, OCC>(CORR_ARGS); break;
+    case 5: launch_corr_t<R, 5, NPT, OCC>(CORR_ARGS); break;
+    case 6: launch_corr_t<R, 6, NPT, OCC>(CORR_ARGS); break;
+    case 7: launch_corr_t<R, 7, NPT, OCC>(CORR_ARGS); break;
+    case 8: launch_corr_t<R, 8, NPT, OCC>(CORR_ARGS); break;
+    case 9: launch_corr_t<R, 9, NPT, OCC>(CORR_ARGS); break;
+    default: launch_corr_t<R, 0, NPT, OCC>(CORR_ARGS); break;
     }
 #undef CORR_ARGS
 }
@@ -2787,14 +2759,17 @@ extern "C" int blfid_dev_launch_corr(int prec, const DevPlan *P, int Kb, const v
     const int nchunk = (int)((P->Etot + CH - 1) / CH);
     const int A = corr_use_fft(P->n) ? min(max(P->corr_auto_chunks, 0), nchunk) : 0;
     const int segA = corr_range_segments(A);
-    // four points per thread up to 1024 FID points, eight above
+    // four points per thread up to 1024 FID points, eight above (variants 0 and 1: variant 1
+    // runs the complex64 four-point kernels at two waves per SIMD); variant 2: eight throughout
     for (int pass = 0; pass < 2; pass++) {
         const int lo = pass ? A : 0, hi = pass ? nchunk : A;
         if (hi <= lo) continue;
         const bool autoc = pass == 0;
         const int sb = pass ? segA : 0;
-        if (P->npts <= 1024) {
-            if (prec) launch_corr<float, 4>(autoc, P, Kb, Qt, Rt, thetas, er, es, part, lo, hi, sb, st);
+        if (P->npts <= 1024 && P->tn_corr != 2) {
+            if (prec && P->tn_corr == 1)
+                launch_corr<float, 4, 2>(autoc, P, Kb, Qt, Rt, thetas, er, es, part, lo, hi, sb, st);
+            else if (prec) launch_corr<float, 4>(autoc, P, Kb, Qt, Rt, thetas, er, es, part, lo, hi, sb, st);
             else launch_corr<double, 4>(autoc, P, Kb, Qt, Rt, thetas, er, es, part, lo, hi, sb, st);
         } else {
             if (prec) launch_corr<float, 8>(autoc, P, Kb, Qt, Rt, thetas, er, es, part, lo, hi, sb, st);

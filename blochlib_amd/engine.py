@@ -122,6 +122,11 @@ def lib():
         L.blfid_plan_graph_capture.argtypes = [vp, ctypes.c_int]
         L.blfid_plan_graph_launch.argtypes = [vp, vp]
         L.blfid_plan_run_slice.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp]
+        L.blfid_autotune.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.c_int, vp, _d]
+        L.blfid_tune_get.argtypes = [_i]
+        L.blfid_tune_set.argtypes = [_i]
+        L.blfid_plan_tune_get.argtypes = [vp, _i]
+        L.blfid_plan_tune_set.argtypes = [vp, _i]
         L.blfid_last_error.restype = ctypes.c_char_p
         L.blfid_version.restype = ctypes.c_char_p
         _lib = L
@@ -353,11 +358,78 @@ class Plan:
                              "flops": float(fl[slot]), "bytes": float(by[slot])}
         return out
 
+    def tuning(self) -> dict:
+        """The plan's launch choices (blfid_plan_tune_get) as a dict (see tuning_to_dict)."""
+        v = np.zeros(TUNE_LEN, np.int32)
+        _check(lib().blfid_plan_tune_get(self.h, _p(v, _i)))
+        return tuning_to_dict(v)
+
+    def set_tuning(self, tuning):
+        """Set the plan's launch choices (blfid_plan_tune_set) from a dict or TUNE_LEN ints."""
+        v = tuning_to_array(tuning)
+        _check(lib().blfid_plan_tune_set(self.h, _p(v, _i)))
+
     def powder_fid(self, pw: Powder, rho, det, normalize=True) -> np.ndarray:
         self.set_operators(rho, det)
         self.set_powder(pw)
         self.run(normalize)
         return self.fetch()
+
+
+TUNE_LEN = 16          # BLFID_TUNE_LEN
+TUNE_FAMILIES = ("exp", "dyson", "eig", "bmul", "qr", "corr", "reduce", "static", "expdyson")
+
+
+def tuning_to_dict(v) -> dict:
+    """BLFID_TUNE_LEN ints -> {"method": {M: "fused"|"two"}, "corr": 0..2, "grid": {family: cap}}
+    (cap 0: the default)."""
+    v = [int(x) for x in v]
+    return {"method": {M: ("two" if v[M - 1] else "fused") for M in range(1, 6)},
+            "corr": v[5], "grid": {f: v[6 + i] for i, f in enumerate(TUNE_FAMILIES)}}
+
+
+def tuning_to_array(t) -> np.ndarray:
+    if isinstance(t, dict):
+        v = np.zeros(TUNE_LEN, np.int32)
+        for M, m in t.get("method", {}).items():
+            v[int(M) - 1] = 1 if m in ("two", 1) else 0
+        v[5] = int(t.get("corr", 0))
+        for i, f in enumerate(TUNE_FAMILIES):
+            v[6 + i] = int(t.get("grid", {}).get(f, 0))
+        return v
+    v = np.ascontiguousarray(t, np.int32)
+    if v.shape != (TUNE_LEN,):
+        raise ValueError(f"a tuning is {TUNE_LEN} ints")
+    return v
+
+
+def tuning() -> dict:
+    """The current device's process-wide launch choices (blfid_tune_get)."""
+    v = np.zeros(TUNE_LEN, np.int32)
+    _check(lib().blfid_tune_get(_p(v, _i)))
+    return tuning_to_dict(v)
+
+
+def set_tuning(t):
+    """Set the current device's process-wide launch choices (blfid_tune_set): plans created
+    afterwards take them (e.g. a tuning saved as JSON by an earlier process)."""
+    v = tuning_to_array(t)
+    _check(lib().blfid_tune_set(_p(v, _i)))
+
+
+def autotune(plans, crystallites: int = 64, stream: int | None = None) -> dict:
+    """Time the launch candidates on the first `crystallites` crystallites of each plan and set
+    the fastest on every plan and on the process table (blfid_autotune); returns the chosen
+    tuning with the summed kernel time of each candidate pass ("pass_ms").  The plans' last FIDs
+    are overwritten: run them again before fetching."""
+    plans = list(plans)
+    arr = (ctypes.c_void_p * len(plans))(*[p.h.value for p in plans])
+    rep = np.zeros(7)
+    _check(lib().blfid_autotune(arr, len(plans), int(crystallites),
+                                ctypes.c_void_p(stream) if stream else None, _p(rep)))
+    out = plans[0].tuning()
+    out["pass_ms"] = [float(x) for x in rep]
+    return out
 
 
 def _system(sy: SolidSys):

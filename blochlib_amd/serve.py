@@ -1,17 +1,20 @@
 """Serving loop: powder-FID requests admitted onto one GPU by their device-memory footprint.
 
-Each request is one powder FID (a spin system, MAS conditions, powder, operators).  The server
-keeps a FIFO of arrived requests and admits the head whenever its working storage fits the
-remaining device budget and an execution stream is free; every admitted request gets a plan
-sized to its share (blfid_plan_set_memory_budget, so large powders run in crystallite batches)
-and runs asynchronously on its own HIP stream, so small requests overlap large ones.  A request
-runs as slices of its powder (blfid_plan_run_slice) of about `slice_flops` of work each (at
-least `min_slice` crystallites), issued `depth` at a time: a large request's kernels then end
-every few hundred milliseconds and the device's queues interleave the kernels of requests
-admitted after it, instead of those waiting for the whole large FID.  Slices give the
-unsliced FID bit for bit.  Plans are cached by (system, acquisition, precision): a repeated
-system reuses its device tables.  Completion is observed with HIP events; the FID is fetched and
-the request's latency (arrival to result on the host) recorded.
+Each request is one powder FID (a spin system, MAS conditions, powder, operators).  Arrived
+requests wait in a queue ordered by an estimate of their work; the server admits them while
+their working storage fits the remaining device budget (a plan sized to one slice of the
+request, blfid_plan_set_memory_budget) and keeps every admitted request resident.  A request
+runs as slices of its powder (blfid_plan_run_slice) of about `slice_flops` of work each, issued
+at most `depth` at a time on one of `streams` HIP streams.  Streams go to the admitted requests
+with the least remaining work: at every slice boundary a running request whose remaining work is
+larger than that of a request waiting for a stream stops issuing, drains its slices in flight
+and yields the stream (its plan, group accumulators included, stays resident, and it resumes
+where it stopped, on whichever stream comes free).  Small requests that arrive while a 10-spin
+FID is running therefore finish within a few slices instead of behind the whole large FID.
+Slices give the unsliced FID bit for bit, whatever the interleaving.  Plans are cached by
+(system, acquisition, precision): a repeated system reuses its device tables.  Completion is
+observed with HIP events; the FID is fetched and the request's latency (arrival to result on
+the host) recorded.
 
 This replaces nothing in BlochLib (which computes one simulation per process); it is the
 service shape of the batched engine: FIDs/s and latency percentiles under an arrival process.
@@ -54,6 +57,15 @@ class Request:
     def latency(self) -> float:
         return self.t_done - self.arrival
 
+    def estimate(self) -> float:
+        """Work estimate used to order the queue before a plan exists: per crystallite-step,
+        the cube of every magnetisation sector of the spins (the blocks of a homonuclear
+        system), times crystallites and steps per period."""
+        from math import comb
+        S = self.system.spins
+        n = max(1, int(round(self.sw / self.wr))) if self.wr else 1
+        return float(sum(comb(S, k) ** 3 for k in range(S + 1))) * len(self.powder) * n
+
 
 def _hip():
     L = ctypes.CDLL("libamdhip64.so")
@@ -69,23 +81,44 @@ def _hip():
 
 
 @dataclass
+class _Active:
+    """An admitted request: its plan stays resident until the last slice completes."""
+    req: Request
+    plan: engine.Plan
+    key: tuple
+    share: float                  # device bytes counted against the budget
+    per_c: float                  # work per crystallite (flops, all steps of a period)
+    slice_k: int                  # crystallites per slice
+    next_c: int = 0               # first crystallite of the next slice
+    slot: "_Slot | None" = None   # the stream it runs on, or None while it waits
+
+    @property
+    def K(self) -> int:
+        return len(self.req.powder)
+
+    @property
+    def remaining(self) -> float:
+        return (self.K - self.next_c) * self.per_c
+
+
+@dataclass
 class _Slot:
     stream: ctypes.c_void_p
     events: list                  # one HIP event per slice in flight (depth of them)
-    req: Request | None = None
-    plan: engine.Plan | None = None
-    key: tuple | None = None
-    next_c: int = 0               # first crystallite of the next slice
-    slice_k: int = 0              # crystallites per slice
+    act: _Active | None = None
     inflight: list = field(default_factory=list)  # events of the slices issued, oldest first
 
 
 class FIDServer:
-    """Admission by device-memory footprint onto `streams` concurrent HIP streams."""
+    """Admission by device-memory footprint; streams go to the admitted requests with the least
+    remaining work, switched at slice boundaries."""
 
     def __init__(self, budget_bytes: float | None = None, streams: int = 4,
-                 precision: str = "fp64", cache_plans: int = 4, min_batch: int = 64,
-                 slice_flops: float = 4e12, min_slice: int = 64, depth: int = 2):
+                 precision: str = "fp64", cache_plans: int = 4, max_batch: int = 256,
+                 slice_flops: float = 4e12, min_slice: int = 4, fill_tasks: int = 1024,
+                 depth: int = 2, max_active: int = 32, policy: str = "srpt"):
+        if policy not in ("srpt", "fifo"):
+            raise ValueError("policy is 'srpt' (least remaining work first) or 'fifo'")
         self.hip = _hip()
         if budget_bytes is None:
             free, tot = ctypes.c_size_t(), ctypes.c_size_t()
@@ -94,10 +127,13 @@ class FIDServer:
         self.budget = float(budget_bytes)
         self.used = 0.0
         self.precision = precision
-        self.min_batch = min_batch
+        self.max_batch = max(1, int(max_batch))
         self.slice_flops = float(slice_flops)
         self.min_slice = max(1, int(min_slice))
+        self.fill_tasks = max(1, int(fill_tasks))
         self.depth = max(1, int(depth))
+        self.max_active = max(1, int(max_active))
+        self.policy = policy
         self.slots = []
         for _ in range(streams):
             s = ctypes.c_void_p()
@@ -115,12 +151,14 @@ class FIDServer:
         # evicted plans: freeing device memory synchronises the device, so they are destroyed
         # only while nothing is in flight (their bytes stay counted until then)
         self.graveyard = []
+        self.switches = 0             # times a running request yielded its stream
 
     def close(self):
         for sl in self.slots:
-            if sl.plan is not None:
-                sl.plan.fetch()
-                sl.plan.close()
+            if sl.act is not None:
+                sl.act.plan.fetch()
+                sl.act.plan.close()
+                sl.act = None
         for p, _ in self.cache.values():
             p.close()
         self.cache.clear()
@@ -149,8 +187,12 @@ class FIDServer:
 
     def _release(self, k, p, nbytes):
         """Keep a finished plan (its device arena stays allocated and counted) or drop it."""
-        self.cache[k] = (p, nbytes)
-        self.used += nbytes
+        if k in self.cache:               # a twin request's plan is cached already
+            self.graveyard.append((p, nbytes))
+            self.used += nbytes
+        else:
+            self.cache[k] = (p, nbytes)
+            self.used += nbytes
         while len(self.cache) > self.cache_max:
             self._evict()
 
@@ -167,28 +209,35 @@ class FIDServer:
             self.used -= nb
         self.graveyard = []
 
-    def footprint(self, p: engine.Plan, K: int) -> float:
-        """Device bytes to admit a request: fixed scratch + a batch of crystallites (at least
-        min_batch or the whole powder, whichever is smaller)."""
-        st = p.stats()
-        return st["fixed_bytes"] + st["bytes_per_crystallite"] * min(K, self.min_batch)
-
     def slice_size(self, p: engine.Plan, K: int) -> int:
-        """Crystallites per slice: about slice_flops of the plan's work, at least min_slice (so
-        that every launch still spreads over the device), at most the whole powder."""
-        per = max(p.stats()["flops_per_cs"] * max(p.steps, 1), 1.0)
-        return int(min(max(K, 1), max(self.min_slice, self.slice_flops // per)))
+        """Crystallites per slice: about slice_flops of the plan's work, but enough that the
+        per-(crystallite, block) launches (the eigen-solver's, one workgroup each) still have
+        fill_tasks workgroups, at least min_slice, at most the whole powder."""
+        st = p.stats()
+        per = max(st["flops_per_cs"] * max(p.steps, 1), 1.0)
+        fill = -(-self.fill_tasks // max(int(st["blocks"]), 1))
+        return int(min(max(K, 1), max(self.min_slice, fill, self.slice_flops // per)))
 
-    def _issue(self, sl: _Slot):
-        """Issue the request's next slices while fewer than `depth` are in flight."""
-        K = len(sl.req.powder)
-        while sl.next_c < K and len(sl.inflight) < self.depth:
-            c1 = min(K, sl.next_c + sl.slice_k)
-            sl.plan.run_slice(sl.next_c, c1, True, stream=sl.stream.value)
+    def footprint(self, p: engine.Plan, K: int, slice_k: int | None = None) -> float:
+        """Device bytes to admit a request: fixed scratch + one batch of crystallites (a slice,
+        at most max_batch crystallites, at most the whole powder)."""
+        st = p.stats()
+        kb = min(K, self.max_batch, slice_k if slice_k else K)
+        return st["fixed_bytes"] + st["bytes_per_crystallite"] * max(kb, 1)
+
+    def _issue(self, sl: _Slot, rival: float):
+        """Issue the bound request's next slices while fewer than `depth` are in flight and no
+        waiting request has less remaining work than it (`rival`)."""
+        a = sl.act
+        while a.next_c < a.K and len(sl.inflight) < self.depth:
+            if self.policy == "srpt" and rival < a.remaining:
+                return
+            c1 = min(a.K, a.next_c + a.slice_k)
+            a.plan.run_slice(a.next_c, c1, True, stream=sl.stream.value)
             ev = next(e for e in sl.events if all(e.value != x.value for x in sl.inflight))
             self.hip.hipEventRecord(ev, sl.stream)
             sl.inflight.append(ev)
-            sl.next_c = c1
+            a.next_c = c1
 
     def _fail(self, r: Request, err: Exception, t: float, done: list, on_done):
         """Completes a request with its error (bad system, unsupported input, out of memory):
@@ -200,104 +249,144 @@ class FIDServer:
         if on_done:
             on_done(r)
 
+    def _order(self, r: Request):
+        return (r.estimate(), r.arrival) if self.policy == "srpt" else (r.arrival,)
+
+    def _admit(self, queue: list, active: list, now, done, on_done) -> bool:
+        """Admit queued requests (least estimated work first) while their footprint fits the
+        budget; returns whether anything changed."""
+        progressed = False
+        queue.sort(key=self._order)
+        while queue and len(active) < self.max_active:
+            r = queue[0]
+            try:
+                k, p, held = self._plan(r)
+                slice_k = self.slice_size(p, len(r.powder))
+                need = self.footprint(p, len(r.powder), slice_k)
+            except engine.BlochLibError as e:
+                queue.pop(0)
+                self._fail(r, e, now(), done, on_done)
+                progressed = True
+                continue
+            while need > self.budget - self.used and self._evict():
+                pass
+            if need > self.budget - self.used and not any(x.act for x in self.slots):
+                self._bury()
+            if need > self.budget - self.used and (active or self.used > 0):
+                self._release(k, p, held)   # waits for memory; the arena goes back to the cache
+                break
+            queue.pop(0)
+            try:
+                p.set_memory_budget(need)
+                p.set_operators(r.rho, r.det)
+                p.set_powder(r.powder)
+            except engine.BlochLibError as e:
+                # the plan may hold work enqueued before the failure: it is retired, not
+                # cached, and destroyed once nothing is in flight
+                self.graveyard.append((p, 0.0))
+                self._fail(r, e, now(), done, on_done)
+                progressed = True
+                continue
+            per = max(p.stats()["flops_per_cs"] * max(p.steps, 1), 1.0)
+            r.footprint = need
+            r.t_admit = now()
+            self.used += need
+            active.append(_Active(r, p, k, need, per, slice_k))
+            progressed = True
+        return progressed
+
+    def _retire(self, a: _Active, err, now, done, on_done):
+        """Take a request off the device: completed (err None) or failed."""
+        r = a.req
+        self.used -= a.share
+        if a.slot is not None:
+            a.slot.act = None
+            a.slot.inflight = []
+            a.slot = None
+        if err is None:
+            try:
+                r.fid = a.plan.fetch()
+                r.sw_out = a.plan.sw
+                r.t_done = now()
+                self._release(a.key, a.plan, a.share)
+                done.append(r)
+                if on_done:
+                    on_done(r)
+                return
+            except engine.BlochLibError as e:
+                err = e
+        self.graveyard.append((a.plan, 0.0))
+        self._fail(r, err, now(), done, on_done)
+
     def run(self, requests, clock=time.perf_counter, on_done=None) -> list:
         """Serve `requests` (arrival times in seconds from now); returns them completed (a
         request that failed has `error` set and no FID)."""
         todo = deque(sorted(requests, key=lambda r: r.arrival))
-        queue: deque = deque()
+        queue: list = []
+        active: list = []
         done = []
         t0 = clock()
-        while todo or queue or any(sl.req is not None for sl in self.slots):
-            now = clock() - t0
-            while todo and todo[0].arrival <= now:
+
+        def now():
+            return clock() - t0
+
+        while todo or queue or active:
+            t = now()
+            while todo and todo[0].arrival <= t:
                 queue.append(todo.popleft())
-            progressed = False
-            # admit the queue head while it fits
-            while queue:
-                free = [sl for sl in self.slots if sl.req is None]
-                if not free:
-                    break
-                r = queue[0]
-                try:
-                    k, p, held = self._plan(r)
-                    need = self.footprint(p, len(r.powder))
-                except engine.BlochLibError as e:
-                    queue.popleft()
-                    self._fail(r, e, clock() - t0, done, on_done)
-                    progressed = True
-                    continue
-                while need > self.budget - self.used and self._evict():
-                    pass
-                if need > self.budget - self.used and not any(x.req for x in self.slots):
-                    self._bury()
-                room = self.budget - self.used
-                if need > room and self.used > 0:
-                    self._release(k, p, held)
-                    break
-                share = max(need, min(room, need * 4))
-                queue.popleft()
-                sl = free[0]
-                sl.req, sl.plan, sl.key = r, p, k
-                sl.next_c, sl.inflight = 0, []
-                try:
-                    p.set_memory_budget(share)
-                    p.set_operators(r.rho, r.det)
-                    p.set_powder(r.powder)
-                    sl.slice_k = self.slice_size(p, len(r.powder))
-                    self._issue(sl)
-                except engine.BlochLibError as e:
-                    # the plan may hold work enqueued before the failure: it is retired, not
-                    # cached, and destroyed once nothing is in flight
-                    self.graveyard.append((p, 0.0))
-                    self._fail(r, e, clock() - t0, done, on_done)
-                    sl.req, sl.plan, sl.key, sl.inflight = None, None, None, []
-                    progressed = True
-                    continue
-                r.footprint = share
-                r.t_admit = clock() - t0
-                self.used += share
-                progressed = True
+            progressed = self._admit(queue, active, now, done, on_done) if queue else False
+            # slices that completed; requests whose last slice completed
             for sl in self.slots:
-                if sl.req is None:
-                    continue
                 while sl.inflight and self.hip.hipEventQuery(sl.inflight[0]) == 0:
                     sl.inflight.pop(0)
                     progressed = True
-                K = len(sl.req.powder)
-                if sl.next_c < K:
-                    try:
-                        self._issue(sl)
-                    except engine.BlochLibError as e:
-                        r = sl.req
-                        self.used -= r.footprint
-                        self.graveyard.append((sl.plan, 0.0))
-                        self._fail(r, e, clock() - t0, done, on_done)
-                        sl.req, sl.plan, sl.key, sl.inflight = None, None, None, []
-                        progressed = True
-                    continue
-                if not sl.inflight:
-                    r = sl.req
-                    self.used -= r.footprint
-                    try:
-                        r.fid = sl.plan.fetch()
-                        r.sw_out = sl.plan.sw
-                        r.t_done = clock() - t0
-                        self._release(sl.key, sl.plan, r.footprint)
-                        done.append(r)
-                        if on_done:
-                            on_done(r)
-                    except engine.BlochLibError as e:
-                        self.graveyard.append((sl.plan, 0.0))
-                        self._fail(r, e, clock() - t0, done, on_done)
-                    sl.req, sl.plan, sl.key = None, None, None
+                a = sl.act
+                if a is not None and not sl.inflight and a.next_c >= a.K:
+                    active.remove(a)
+                    self._retire(a, None, now, done, on_done)
                     progressed = True
-            if self.graveyard and not any(sl.req is not None for sl in self.slots):
+            waiting = sorted((a for a in active if a.slot is None),
+                             key=(lambda a: (a.remaining, a.req.arrival)) if self.policy == "srpt"
+                             else (lambda a: a.req.arrival))
+            # a stream whose request drained at a slice boundary goes to a waiting request with
+            # less remaining work
+            if self.policy == "srpt":
+                for sl in self.slots:
+                    a = sl.act
+                    if a is None or sl.inflight or not waiting:
+                        continue
+                    if waiting[0].remaining < a.remaining:
+                        a.slot, sl.act = None, None
+                        waiting.append(a)
+                        waiting.sort(key=lambda x: (x.remaining, x.req.arrival))
+                        self.switches += 1
+                        progressed = True
+            for sl in self.slots:
+                if sl.act is None and waiting:
+                    a = waiting.pop(0)
+                    sl.act, a.slot, sl.inflight = a, sl, []
+                    progressed = True
+            rival = waiting[0].remaining if waiting else float("inf")
+            for sl in self.slots:
+                a = sl.act
+                if a is None:
+                    continue
+                before = a.next_c
+                try:
+                    self._issue(sl, rival)
+                except engine.BlochLibError as e:
+                    active.remove(a)
+                    self._retire(a, e, now, done, on_done)
+                    progressed = True
+                    continue
+                progressed = progressed or a.next_c != before
+            if self.graveyard and not any(sl.act is not None for sl in self.slots):
                 self._bury()
             if not progressed:
-                if queue or any(sl.req is not None for sl in self.slots):
+                if queue or active:
                     time.sleep(0.0002)
                 elif todo:
-                    time.sleep(max(0.0, min(0.01, todo[0].arrival - (clock() - t0))))
+                    time.sleep(max(0.0, min(0.01, todo[0].arrival - now())))
         return done
 
 

@@ -292,6 +292,42 @@ blfid_status blfid_plan_phase_times(blfid_plan *plan, double *ms, int *launches,
  * powder sum).  Beside counter-measured traffic it shows re-reads and spills. */
 blfid_status blfid_plan_phase_bytes(blfid_plan *plan, double *bytes);
 
+/* Launch autotuner (no counterpart in the reference, whose CPU loops have no launch shape).
+ * A tuning is BLFID_TUNE_LEN ints:
+ *   [0..4]  propagator method of tile class M = 1..5 (blocks of 16(M-1)+1 .. 16M states, class 5
+ *           up to 80) in complex64 gamma-COMPUTE plans: 0 the fused propagator + Dyson chain
+ *           kernel, 1 the propagator and the Dyson chain as two kernels;
+ *   [5]     correlation kernel variant: 0 four FID points per thread up to 1024 points, eight
+ *           above; 1 the same with the complex64 four-point kernel at two waves per SIMD; 2 eight
+ *           points per thread throughout;
+ *   [6..14] persistent-grid cap (workgroups) of the kernel families in profiling-phase order
+ *           (exp, dyson, eig, bmul, qr, corr, reduce, static, expdyson); 0 the default 4096;
+ *   [15]    reserved, 0.
+ * Every choice gives the same FID bit for bit except [0..4], which switch between two complex64
+ * forms (both within the complex64 tolerance of the FP64 form).  Each device has a process-wide
+ * table (all zero at start); a plan takes the table of the current device when it is created.
+ * blfid_autotune times the candidates of every entry on the leading crystallites of each plan --
+ * `crystallites` of them, or more where that many would leave its per-(crystallite, block)
+ * launches with fewer than 2048 workgroups (a sample smaller than the runs' batches would favour
+ * the kernels with more tasks per crystallite); it times them with profiling events per kernel
+ * family and tile class, the faster of two runs per candidate, in seven candidate passes (three
+ * for [0..5], four grid caps), keeps a candidate only
+ * when it is at least 3% faster than the current choice, writes the result to every plan and to
+ * the process table of the current device, and stores the summed time of each pass (ms) in
+ * report[0..6] when report is not NULL.  The plans must be gamma-COMPUTE plans with operators
+ * and powder set; their profiling counters are cleared (the profiling switch is kept), their
+ * last FID is overwritten (run them again before fetching) and their recorded graphs
+ * invalidated.  Work is enqueued on `stream` and waited for.  A failure leaves every plan and the
+ * process table with the tuning they had. */
+#define BLFID_TUNE_LEN 16
+#define BLFID_TUNE_PASSES 7
+blfid_status blfid_autotune(blfid_plan **plans, int nplans, int crystallites, void *stream,
+                            double *report);
+blfid_status blfid_tune_get(int *vals);
+blfid_status blfid_tune_set(const int *vals);
+blfid_status blfid_plan_tune_get(const blfid_plan *plan, int *vals);
+blfid_status blfid_plan_tune_set(blfid_plan *plan, const int *vals);
+
 /* Thread-local description of the last error. */
 const char *blfid_last_error(void);
 
