@@ -42,6 +42,10 @@ sys.path.insert(0, ROOT)
 
 from blochlib_amd import engine, workloads  # noqa: E402
 
+# launch choices of the autotuner saved for this device (blfid_autotune on the batched workload)
+TUNING_FILE = os.path.join(ROOT, "blochlib_amd", "tuning", "mi355x.json")
+TUNING_SOURCE = ["defaults (no saved table)"]
+
 FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
@@ -428,14 +432,24 @@ def measure(H, wls, ptop, precision, steps, warmup, args, calibrate=False):
         plans.append(p)
     print(f"[bench] {precision}: {len(plans)} plans resident in "
           f"{time.perf_counter() - t0:.1f}s", flush=True)
-    tuning = None
     if not ptop and args.tune_crystallites > 0:
         # launch choices timed on the leading crystallites of every plan (outside the timed
         # region, like plan creation); the same FIDs, bit for bit or within the c64 tolerance
         t1 = time.perf_counter()
         tuning = engine.autotune(plans, crystallites=args.tune_crystallites)
         tuning["seconds"] = time.perf_counter() - t1
+        tuning["source"] = f"blfid_autotune in this run ({precision} plans)"
         print(f"[bench] {precision}: autotuned in {tuning['seconds']:.1f}s: {tuning}", flush=True)
+        if args.save_tuning and precision == args.precision:
+            with open(args.save_tuning, "w") as f:
+                json.dump({"tuning": {k: tuning[k] for k in ("method", "corr", "grid", "qr_flip")},
+                           "made_by": "python bench.py --tune-crystallites "
+                                      f"{args.tune_crystallites} --save-tuning <file>",
+                           "workload": ("batched: 64 seeded 1H clusters of 4-8 spins, MAS 15 kHz, "
+                                        "128 steps/period, ZCW 3722, 1024 points; " + precision),
+                           "pass_ms": tuning["pass_ms"]}, f, indent=1)
+    else:
+        tuning = dict(engine.tuning(), source=TUNING_SOURCE[0])
 
     def step(events=None):
         for i, p in enumerate(plans):
@@ -554,9 +568,15 @@ def main():
                          "form with its complex products as three bf16 products each")
     ap.add_argument("--no-fp64-form", action="store_true",
                     help="skip timing the FP64 form beside the c64 headline")
-    ap.add_argument("--tune-crystallites", type=int, default=32,
-                    help="autotune the launch choices on this many leading crystallites of "
-                         "every plan before the warm-up (0: the default choices)")
+    ap.add_argument("--tune-crystallites", type=int, default=0,
+                    help="run the launch autotuner on (at least) this many leading crystallites "
+                         "of every plan before the warm-up (0: use the saved table, "
+                         "blochlib_amd/tuning/mi355x.json, or the defaults without it)")
+    ap.add_argument("--save-tuning", default="", metavar="FILE",
+                    help="write the table the autotuner chose (with --tune-crystallites) as JSON")
+    ap.add_argument("--tuning", default="",
+                    help="launch choices as JSON (the engine.tuning() dict), set instead of the "
+                         "saved table")
     ap.add_argument("--dense-k", type=int, default=64,
                     help="crystallites of the dense 10-spin configuration timed beside the "
                          "headline (0: skip)")
@@ -567,6 +587,13 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if args.tuning:
+        engine.set_tuning(json.loads(args.tuning))
+        TUNING_SOURCE[0] = "--tuning"
+    elif args.tune_crystallites <= 0 and os.path.exists(TUNING_FILE):
+        with open(TUNING_FILE) as f:
+            engine.set_tuning(json.load(f)["tuning"])
+        TUNING_SOURCE[0] = os.path.relpath(TUNING_FILE, ROOT) + " (saved blfid_autotune table)"
     if args.workload == "serve":
         return bench_serve(args)
     H = hip()

@@ -70,6 +70,8 @@ struct DevPlan {
     int tn_corr;           // k_corr variant: 0 four FID points per thread up to 1024 points
                            // (eight above) at the default occupancy, 1 the same at two waves
                            // per SIMD (no register spill), 2 eight points per thread
+    int tn_qr;             // k_qr form per complex64 tile class: bit M-1 set takes the other form
+                           // (per-step for classes 1-2, step-walking for 3-4)
 };
 
 // Persistent-grid cap of a kernel family: the tuned value or 4096 workgroups; single-wave

@@ -740,6 +740,7 @@ static void refresh_devplan(blfid_plan *pl) {
     d.elem_s = pl->d_es.p;
     for (int f = 0; f < 9; f++) d.tn_cap[f] = pl->tn[6 + f];
     d.tn_corr = pl->tn[5];
+    d.tn_qr = pl->tn[15];
 }
 
 // ------------------------------------------------------------------------------ C ABI
@@ -2231,7 +2232,7 @@ static blfid_status check_tuning(const int *v) {
     for (int f = 6; f < 15; f++)
         if (v[f] < 0 || v[f] > (1 << 20))
             return fail(BLFID_ERR_INVALID, "grid cap outside 0..1048576 workgroups");
-    if (v[15] != 0) return fail(BLFID_ERR_INVALID, "reserved tuning entry must be 0");
+    if (v[15] < 0 || v[15] > 15) return fail(BLFID_ERR_INVALID, "Q-product form bits are 0..15");
     return BLFID_OK;
 }
 
@@ -2266,16 +2267,24 @@ blfid_status blfid_plan_tune_set(blfid_plan *pl, const int *vals) {
     return BLFID_OK;
 }
 
-// Crystallites a plan is tuned on: at least `kc`, and enough that its per-(crystallite, block)
-// launches (the fused propagator's, the eigen-solver's) have 2048 workgroups, as the full runs'
-// batches do -- a small sample would favour the kernels with more tasks per crystallite.
+// Crystallites a plan is tuned on: at least `kc`, and enough that each of its launches with one
+// task per crystallite and block (or block pair) of a tile class -- the fused propagator, the
+// step-walking Q products -- has 2048 workgroups, as the full runs' batches do: a smaller
+// sample starves those launches and favours the forms with a task per step.
 static int tune_sample(const blfid_plan *pl, int kc) {
-    const int nb = std::max<int>(1, (int)pl->blocks.size());
-    return std::min(pl->K, std::max(kc, (2048 + nb - 1) / nb));
+    size_t m = 0;
+    auto least = [&](const std::map<int, std::vector<int>> &g) {
+        for (auto &kv : g)
+            if (!kv.second.empty()) m = m ? std::min(m, kv.second.size()) : kv.second.size();
+    };
+    least(pl->blk_groups);
+    least(pl->pair_groups);
+    const int per = (int)std::max<size_t>(m, 1);
+    return std::min(pl->K, std::max(kc, (2048 + per - 1) / per));
 }
 
 // Kernel time per profiling slot, summed over the plans, of one candidate tuning: each plan runs
-// its tuning sample twice and keeps, per slot, the faster of the two.
+// its tuning sample once.
 static blfid_status tune_pass(blfid_plan **plans, int nplans, int kc, hipStream_t st,
                               const int *vals, double *slot_ms) {
     for (int i = 0; i < BLFID_NSLOT; i++) slot_ms[i] = 0;
@@ -2283,16 +2292,11 @@ static blfid_status tune_pass(blfid_plan **plans, int nplans, int kc, hipStream_
         blfid_plan *pl = plans[p];
         std::memcpy(pl->tn, vals, sizeof(int) * BLFID_TUNE_LEN);
         pl->gen++;
-        double best[BLFID_NSLOT];
-        for (int rep = 0; rep < 2; rep++) {
-            blfid_status s = blfid_plan_set_profiling(pl, 1);
-            if (s == BLFID_OK) s = run_upto(pl, 0, st, tune_sample(pl, kc));
-            if (s != BLFID_OK) return s;
-            collect_profile(pl);
-            for (int i = 0; i < BLFID_NSLOT; i++)
-                best[i] = rep ? std::min(best[i], pl->ph_ms[i]) : pl->ph_ms[i];
-        }
-        for (int i = 0; i < BLFID_NSLOT; i++) slot_ms[i] += best[i];
+        blfid_status s = blfid_plan_set_profiling(pl, 1);
+        if (s == BLFID_OK) s = run_upto(pl, 0, st, tune_sample(pl, kc));
+        if (s != BLFID_OK) return s;
+        collect_profile(pl);
+        for (int i = 0; i < BLFID_NSLOT; i++) slot_ms[i] += pl->ph_ms[i];
     }
     return BLFID_OK;
 }
@@ -2322,13 +2326,14 @@ blfid_status blfid_autotune(blfid_plan **plans, int nplans, int crystallites, vo
     std::vector<double> T(BLFID_TUNE_PASSES * BLFID_NSLOT, 0.0);
     auto slot = [&](int pass, int i) -> double & { return T[(size_t)pass * BLFID_NSLOT + i]; };
     blfid_status s = BLFID_OK;
-    // passes 0-2: fused (0, 2) or two-kernel (1) propagator per tile class, correlation variant
-    // 0, 1, 2
+    // passes 0-2: fused (0, 2) or two-kernel (1) propagator per tile class, default (0, 2) or
+    // other (1) Q-product form per tile class, correlation variant 0, 1, 2
     for (int c = 0; c < 3 && s == BLFID_OK; c++) {
         int v[BLFID_TUNE_LEN];
         std::memcpy(v, cur, sizeof(v));
         for (int M = 0; M < 5; M++) v[M] = c == 1 ? 1 : 0;
         v[5] = c;
+        v[15] = c == 1 ? 15 : 0;
         s = tune_pass(plans, nplans, crystallites, st, v, &slot(c, 0));
     }
     if (s == BLFID_OK) {
@@ -2340,6 +2345,13 @@ blfid_status blfid_autotune(blfid_plan **plans, int nplans, int crystallites, vo
             if (fused <= 0.0) continue; // no block of this class is fused in these plans
             const double now = cur[M - 1] ? two : fused, alt = cur[M - 1] ? fused : two;
             if (alt < keep * now) cur[M - 1] ^= 1;
+        }
+        for (int M = 1; M <= 4; M++) {
+            const double dflt = std::min(slot(0, 32 + M), slot(2, 32 + M)), other = slot(1, 32 + M);
+            if (dflt <= 0.0 || other <= 0.0) continue; // no pair of this class
+            const bool on = (cur[15] >> (M - 1)) & 1;
+            const double now = on ? other : dflt, alt = on ? dflt : other;
+            if (alt < keep * now) cur[15] ^= 1 << (M - 1);
         }
         double cbest = slot(cur[5], 40);
         for (int c = 0; c < 3; c++)

@@ -304,6 +304,7 @@ __device__ __forceinline__ void wg_cgemm_mfma(int m, int n, int k, const cf *__r
     constexpr int TM = 16 * M, LDK = KC + 4, NW = NT / 64, NSUB = (M * M + NW - 1) / NW;
     float *Ar = smem, *Ai = Ar + TM * LDK, *Br = Ai + TM * LDK, *Bi = Br + TM * LDK;
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63, g = lane >> 4, r = lane & 15;
+    __syncthreads(); // operands the workgroup wrote before the call are visible to every thread
 #pragma unroll 1
     for (int i0 = 0; i0 < m; i0 += TM) {
 #pragma unroll 1
@@ -317,38 +318,62 @@ __device__ __forceinline__ void wg_cgemm_mfma(int m, int n, int k, const cf *__r
 #pragma unroll 1
             for (int q0 = 0; q0 < k; q0 += KC) {
                 const int kc = min(KC, k - q0), kp = (kc + 15) & ~15;
-                __syncthreads();
-                if (ca) {
-                    for (int idx = tid; idx < TM * kp; idx += NT) {
-                        const int q = idx % kp, i = idx / kp;
-                        cf v = make_float2(0.0f, 0.0f);
-                        if (i0 + i < m && q < kc) {
-                            v = A[(q0 + q) + (size_t)(i0 + i) * lda];
-                            v.y = -v.y;
+                // staging: each thread issues its loads of both operands in groups of G (the
+                // first group before the barrier that ends the previous slice's products, so
+                // those loads overlap them), then stores the group to LDS; columns q >= kc are
+                // zero.  The group sizes (four for tile class 3, two for 5, one otherwise) keep
+                // each kernel's registers within its occupancy class (measured per class).
+                constexpr int PER = TM * KC / NT, G = M == 3 ? 4 : M == 5 ? 2 : 1;
+                static_assert(PER * NT == TM * KC, "staging split");
+#pragma unroll 1
+                for (int u0 = 0; u0 < PER; u0 += G) {
+                    cf va[G], vb[G];
+#pragma unroll
+                    for (int t = 0; t < G; t++) {
+                        const int idx = tid + (u0 + t) * NT;
+                        cf v = make_float2(0.0f, 0.0f), wv = make_float2(0.0f, 0.0f);
+                        if (u0 + t < PER) {
+                            if (ca) {
+                                const int q = idx % KC, i = idx / KC;
+                                if (i0 + i < m && q < kc) {
+                                    v = A[(q0 + q) + (i0 + i) * lda]; // 32-bit offsets
+                                    v.y = -v.y;
+                                }
+                            } else {
+                                // A(i, q) is contiguous in i.  Each 16 lanes load 16 consecutive
+                                // rows of one column (a 128-byte segment) and a wave covers 4
+                                // columns, so the 64 stores i * (KC + 4) + q of a wave spread
+                                // over the banks.
+                                const int rest = idx >> 6;
+                                const int i = (idx & 15) + 16 * (rest % M), q = ((idx >> 4) & 3) + 4 * (rest / M);
+                                if (i0 + i < m && q < kc) v = A[(i0 + i) + (q0 + q) * lda];
+                            }
+                            const int qb = idx % KC, j = idx / KC;
+                            if (j0 + j < n && qb < kc) wv = B[(q0 + qb) + (j0 + j) * ldb];
                         }
-                        Ar[i * LDK + q] = v.x;
-                        Ai[i * LDK + q] = v.y;
+                        va[t] = v;
+                        vb[t] = wv;
                     }
-                } else {
-                    // A(i, q) is contiguous in i.  Each 16 lanes load 16 consecutive rows of one
-                    // column (a 128-byte segment) and a wave covers 4 columns, so the 64 stores
-                    // i * (KC + 4) + q of a wave spread over the banks (consecutive lanes on
-                    // consecutive rows of one column would share a quarter of them).
-                    for (int idx = tid; idx < TM * kp; idx += NT) {
-                        const int rest = idx >> 6;
-                        const int i = (idx & 15) + 16 * (rest % M), q = ((idx >> 4) & 3) + 4 * (rest / M);
-                        cf v = make_float2(0.0f, 0.0f);
-                        if (i0 + i < m && q < kc) v = A[(i0 + i) + (size_t)(q0 + q) * lda];
-                        Ar[i * LDK + q] = v.x;
-                        Ai[i * LDK + q] = v.y;
+                    if (u0 == 0) __syncthreads();
+#pragma unroll
+                    for (int t = 0; t < G; t++) {
+                        if (u0 + t >= PER) continue;
+                        const int idx = tid + (u0 + t) * NT;
+                        int i, q;
+                        if (ca) {
+                            q = idx % KC;
+                            i = idx / KC;
+                        } else {
+                            const int rest = idx >> 6;
+                            i = (idx & 15) + 16 * (rest % M);
+                            q = ((idx >> 4) & 3) + 4 * (rest / M);
+                        }
+                        Ar[i * LDK + q] = va[t].x;
+                        Ai[i * LDK + q] = va[t].y;
+                        const int qb = idx % KC, j = idx / KC;
+                        Br[j * LDK + qb] = vb[t].x;
+                        Bi[j * LDK + qb] = vb[t].y;
                     }
-                }
-                for (int idx = tid; idx < TM * kp; idx += NT) {
-                    const int q = idx % kp, j = idx / kp;
-                    cf v = make_float2(0.0f, 0.0f);
-                    if (j0 + j < n && q < kc) v = B[(q0 + q) + (size_t)(j0 + j) * ldb];
-                    Br[j * LDK + q] = v.x;
-                    Bi[j * LDK + q] = v.y;
                 }
                 __syncthreads();
 #pragma unroll 1
@@ -1643,7 +1668,9 @@ __device__ __forceinline__ bool qr_chain_mfma(int bI, int bJ, int n, const int *
 // (rho_eff = det + det^H when rho == (det+det^H)/2, compute.h:381-382).  Stored per element
 // e = r + s*b_I at offset pair_eoff, layout [c][k][e] with rows of Estr elements:
 // Qs = Q(r,s), Rs = R(s,r).
-template <int M, typename R, bool SPLIT = false>
+// FLIP (complex64, tile classes 1-4; a variant the launch autotuner may choose): the other of
+// the two forms -- the per-step form for classes 1-2, the step-walking form for 3-4.
+template <int M, typename R, bool SPLIT = false, bool FLIP = false>
 __global__ void __launch_bounds__(NThr<M>::v, (M == 5 && sizeof(R) == 4) ? 3 : 1) k_qr(DevPlan P, int Kb, const int *__restrict__ pgrp,
                                                    int ngrp,
                                                    const typename CX<R>::T *__restrict__ bufB,
@@ -1654,7 +1681,7 @@ __global__ void __launch_bounds__(NThr<M>::v, (M == 5 && sizeof(R) == 4) ? 3 : 1
     typedef typename CX<R>::T CT;
     // complex64 pairs of small blocks with R formed from Q: one task per (crystallite, pair)
     // walks all steps (qr_chain_mfma); otherwise one task per (crystallite, step, pair)
-    constexpr bool CHAIN = M <= 2 && std::is_same<R, float>::value && !SPLIT;
+    constexpr bool CHAIN = ((M <= 2) != FLIP) && M <= 4 && std::is_same<R, float>::value && !SPLIT;
     constexpr int GB = CGemm<M, NT, R, SPLIT>::BYTES, CB = QrChainLds<M>::BYTES;
     __shared__ __align__(16) char smem[CHAIN && CB > GB ? CB : GB];
     const int tid = threadIdx.x;
@@ -2691,6 +2718,17 @@ static void launch_qr(int M, const DevPlan *P, int Kb, const int *pgrp, int ngrp
     typedef typename CX<R>::T CT;
     const CT *B = (const CT *)bufB;
     CT *sc = (CT *)scratch, *Q = (CT *)Qt, *Rr = (CT *)Rt;
+    if constexpr (std::is_same<R, float>::value) {
+        if (!P->bf16x3 && M <= 4 && ((P->tn_qr >> (M - 1)) & 1)) {
+            switch (M) {
+            case 1: k_qr<1, R, false, true><<<grid, 64, 0, st>>>(*P, Kb, pgrp, ngrp, B, sc, slot, Q, Rr); break;
+            case 2: k_qr<2, R, false, true><<<grid, 256, 0, st>>>(*P, Kb, pgrp, ngrp, B, sc, slot, Q, Rr); break;
+            case 3: k_qr<3, R, false, true><<<grid, 256, 0, st>>>(*P, Kb, pgrp, ngrp, B, sc, slot, Q, Rr); break;
+            default: k_qr<4, R, false, true><<<grid, 256, 0, st>>>(*P, Kb, pgrp, ngrp, B, sc, slot, Q, Rr); break;
+            }
+            return;
+        }
+    }
     switch (M) {
     case 1: LAUNCH_SPLIT(k_qr, 1, R, grid, 64, st, *P, Kb, pgrp, ngrp, B, sc, slot, Q, Rr); break;
     case 2: LAUNCH_SPLIT(k_qr, 2, R, grid, 256, st, *P, Kb, pgrp, ngrp, B, sc, slot, Q, Rr); break;

@@ -381,11 +381,12 @@ TUNE_FAMILIES = ("exp", "dyson", "eig", "bmul", "qr", "corr", "reduce", "static"
 
 
 def tuning_to_dict(v) -> dict:
-    """BLFID_TUNE_LEN ints -> {"method": {M: "fused"|"two"}, "corr": 0..2, "grid": {family: cap}}
-    (cap 0: the default)."""
+    """BLFID_TUNE_LEN ints -> {"method": {M: "fused"|"two"}, "corr": 0..2, "grid": {family: cap}
+    (cap 0: the default), "qr_flip": {M: the other Q-product form}}."""
     v = [int(x) for x in v]
     return {"method": {M: ("two" if v[M - 1] else "fused") for M in range(1, 6)},
-            "corr": v[5], "grid": {f: v[6 + i] for i, f in enumerate(TUNE_FAMILIES)}}
+            "corr": v[5], "grid": {f: v[6 + i] for i, f in enumerate(TUNE_FAMILIES)},
+            "qr_flip": {M: bool((v[15] >> (M - 1)) & 1) for M in range(1, 5)}}
 
 
 def tuning_to_array(t) -> np.ndarray:
@@ -396,6 +397,7 @@ def tuning_to_array(t) -> np.ndarray:
         v[5] = int(t.get("corr", 0))
         for i, f in enumerate(TUNE_FAMILIES):
             v[6 + i] = int(t.get("grid", {}).get(f, 0))
+        v[15] = sum(1 << (int(M) - 1) for M, on in t.get("qr_flip", {}).items() if on)
         return v
     v = np.ascontiguousarray(t, np.int32)
     if v.shape != (TUNE_LEN,):

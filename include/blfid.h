@@ -302,16 +302,19 @@ blfid_status blfid_plan_phase_bytes(blfid_plan *plan, double *bytes);
  *           points per thread throughout;
  *   [6..14] persistent-grid cap (workgroups) of the kernel families in profiling-phase order
  *           (exp, dyson, eig, bmul, qr, corr, reduce, static, expdyson); 0 the default 4096;
- *   [15]    reserved, 0.
- * Every choice gives the same FID bit for bit except [0..4], which switch between two complex64
- * forms (both within the complex64 tolerance of the FP64 form).  Each device has a process-wide
+ *   [15]    Q-product (k_qr) form of the complex64 tile classes 1-4: bit M-1 set takes the other
+ *           form (one task per step for classes 1-2, whose default walks the steps of a
+ *           crystallite with the pair in LDS; the step-walking form for classes 3-4).
+ * Every choice gives the same FID bit for bit except [0..4] and [15], which switch between two
+ * complex64 forms (both within the complex64 tolerance of the FP64 form).  Each device has a process-wide
  * table (all zero at start); a plan takes the table of the current device when it is created.
  * blfid_autotune times the candidates of every entry on the leading crystallites of each plan --
- * `crystallites` of them, or more where that many would leave its per-(crystallite, block)
- * launches with fewer than 2048 workgroups (a sample smaller than the runs' batches would favour
- * the kernels with more tasks per crystallite); it times them with profiling events per kernel
- * family and tile class, the faster of two runs per candidate, in seven candidate passes (three
- * for [0..5], four grid caps), keeps a candidate only
+ * `crystallites` of them, or more where that many would leave a launch with one task per
+ * crystallite and block (or block pair) of a tile class with fewer than 2048 workgroups (a
+ * sample smaller than the runs' batches would favour the forms with a task per step), up to the
+ * whole powder; it times them with profiling events per kernel family and tile class, one run
+ * per candidate, in seven candidate passes (three for [0..5] and [15], four grid caps), keeps a
+ * candidate only
  * when it is at least 3% faster than the current choice, writes the result to every plan and to
  * the process table of the current device, and stores the summed time of each pass (ms) in
  * report[0..6] when report is not NULL.  The plans must be gamma-COMPUTE plans with operators

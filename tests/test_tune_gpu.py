@@ -59,6 +59,13 @@ def test_forced_choices_keep_the_fid():
               f"{rel(two, ref):.2e}, fused vs FP64 {rel(base, ref):.2e}", flush=True)
         assert rel(two, ref) < 1e-5 and rel(base, ref) < 1e-5
         assert p.tuning()["method"][3] == "two"
+        p.set_tuning({"qr_flip": {M: True for M in range(1, 5)}})
+        p.run()
+        flip = p.fetch()
+        print(f"[tune] c64 other Q-product forms: vs default {rel(flip, base):.2e}, vs FP64 "
+              f"{rel(flip, ref):.2e}", flush=True)
+        assert rel(flip, ref) < 1e-5 and rel(flip, base) < 2e-6
+        assert p.tuning()["qr_flip"] == {1: True, 2: True, 3: True, 4: True}
     finally:
         p.close()
         d.close()
@@ -108,7 +115,7 @@ def test_bad_tunings_are_refused():
         with pytest.raises(engine.EngineError):
             engine.set_tuning(v)
         v = ZERO.copy()
-        v[15] = 1
+        v[15] = 16
         with pytest.raises(engine.EngineError):
             p.set_tuning(v)
         assert p.tuning() == engine.tuning_to_dict(ZERO)
