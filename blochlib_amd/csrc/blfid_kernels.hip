@@ -1278,10 +1278,8 @@ __global__ void __launch_bounds__(XdThreads<M>::v) k_expdyson(DevPlan P, int Kb,
 // mode 0: Mx = A_n block (unitary) -> lambda_r = Mx_rr, out phase = arg(lambda_r)
 // mode 1: Mx = H block (Hermitian, static path) -> out value = Re(Mx_rr)
 #define EIG_MAXB 80
-#define EIG_MAXPAIR 128 /* blocks up to 256 */
-// MB > 0: the block lives in LDS (b <= MB), with LDS and threads sized for that class so that
-// workgroups of small blocks share a CU; MB == 0: in a global scratch slot of b*b complex per
-// workgroup (any b <= 2*EIG_MAXPAIR), same algorithm.
+// The block lives in LDS (b <= MB), with LDS and threads sized for that class so that
+// workgroups of small blocks share a CU; larger blocks take k_eig_blk.
 // The source block may be stored in FP32 (complex64 plans); the solver always runs in FP64 and
 // W stays FP64, with a storage-precision copy Wr for the B = A W products when R != double.
 template <int MB, int NT, typename R>
@@ -1296,14 +1294,14 @@ __global__ void __launch_bounds__(NT) k_eig(DevPlan P, int Kb, const int *__rest
     // complex64 period propagator carries ~1e-7 relative error, so its solve stops earlier.
     constexpr double tol_rot = __is_same(R, double) ? 1e-15 : 1e-10;
     constexpr double tol_off = __is_same(R, double) ? 1e-17 : 1e-12;
-    constexpr bool BIG = MB == 0;
-    constexpr int MP = BIG ? EIG_MAXPAIR : (MB + 1) / 2;
+    static_assert(MB > 0 && MB <= EIG_MAXB, "blocks above 80 states take k_eig_blk");
+    constexpr int MP = (MB + 1) / 2;
     // complex64 eigenvectors of the two largest LDS classes accumulate in LDS (FP32) when only
     // the storage-precision copy is wanted (the spinning path): the rotations then never wait on
     // global memory.  The FP64 copy W is not written in that case.  (Both classes already run
     // one 1024-thread workgroup per CU, so the extra LDS costs no occupancy.)
     constexpr bool WLDS = (MB == EIG_MAXB || MB == 64) && __is_same(R, float);
-    __shared__ cd Mlds[BIG ? 1 : MB * (MB + 1)];
+    __shared__ cd Mlds[MB * (MB + 1)];
     __shared__ cf Wl[WLDS ? MB * MB : 1];
     const bool wl = WLDS && bufWr != nullptr;
     __shared__ double rc[MP];
@@ -1311,12 +1309,12 @@ __global__ void __launch_bounds__(NT) k_eig(DevPlan P, int Kb, const int *__rest
     __shared__ int rp[MP], rq[MP];
     __shared__ int s_any;
     const int tid = threadIdx.x;
-    cd *Ms = BIG ? gscr + gslot * blockIdx.x : Mlds;
+    cd *Ms = Mlds;
     const long ntask = (long)Kb * ngrp;
     for (long task = blockIdx.x; task < ntask; task += gridDim.x) {
         const int g = (int)(task % ngrp), c = (int)(task / ngrp);
         const int blk = grp[g], b = P.blk_dim[blk];
-        const int ld = BIG ? b : b + 1;
+        const int ld = b + 1;
         const CT *S0 = src + (size_t)c * src_stride + P.blk_uoff[blk];
         cd *W = bufW + (size_t)c * P.Lsum + P.blk_uoff[blk];
         for (int e = tid; e < b * b; e += NT) {
@@ -1447,6 +1445,232 @@ __global__ void __launch_bounds__(NT) k_eig(DevPlan P, int Kb, const int *__rest
             } else {
                 for (int e = tid; e < b * b; e += NT) Wr[e] = CX<R>::from(W[e]);
             }
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------------------ k_eig_blk
+// Joint rotation of the index pair (p, q) of a normal-matrix iterate: the Cardoso-Souloumiac
+// angle for H1 = (M + M^H)/2, H2 = (M - M^H)/2i, as k_eig forms it.  Returns false (identity)
+// when the pair's off-diagonal part is below tol_off of its diagonal; big: |s| > tol_rot.
+__device__ __forceinline__ bool jj_rotation(cd mpp, cd mqq, cd mpq, cd mqp, double tol_off,
+                                            double tol_rot, double &cth, cd &sth, bool &big) {
+    cth = 1.0;
+    sth = cmk(0, 0);
+    big = false;
+    const cd h1pq = cscale(cadd(mpq, cconj(mqp)), 0.5);
+    const cd d2 = csub(mpq, cconj(mqp));
+    const cd h2pq = cmk(0.5 * d2.y, -0.5 * d2.x);
+    const double off = fabs(h1pq.x) + fabs(h1pq.y) + fabs(h2pq.x) + fabs(h2pq.y);
+    const double scale = fabs(mpp.x) + fabs(mpp.y) + fabs(mqq.x) + fabs(mqq.y) + 1e-300;
+    if (!(off > tol_off * scale)) return false;
+    const double h1[3] = {mpp.x - mqq.x, 2.0 * h1pq.x, 2.0 * h1pq.y};
+    const double h2[3] = {mpp.y - mqq.y, 2.0 * h2pq.x, 2.0 * h2pq.y};
+    const double ka = h1[0] * h1[0] + h1[1] * h1[1] + h1[2] * h1[2];
+    const double kd = h2[0] * h2[0] + h2[1] * h2[1] + h2[2] * h2[2];
+    const double kb = h1[0] * h2[0] + h1[1] * h2[1] + h1[2] * h2[2];
+    const double hm = 0.5 * (ka + kd);
+    const double lam = hm + sqrt(0.25 * (ka - kd) * (ka - kd) + kb * kb);
+    const double a1 = lam - kd, b1 = kb, a2 = kb, b2 = lam - ka;
+    double al, be;
+    if (a1 * a1 + b1 * b1 >= a2 * a2 + b2 * b2) { al = a1; be = b1; }
+    else { al = a2; be = b2; }
+    if (al == 0.0 && be == 0.0) { al = 1.0; be = 0.0; }
+    double ux = al * h1[0] + be * h2[0];
+    double uy = al * h1[1] + be * h2[1];
+    double uz = al * h1[2] + be * h2[2];
+    if (ux < 0) { ux = -ux; uy = -uy; uz = -uz; }
+    const double r = sqrt(ux * ux + uy * uy + uz * uz);
+    if (!(r > 0.0)) return false;
+    cth = sqrt((ux + r) / (2.0 * r));
+    const double den = sqrt(2.0 * r * (ux + r));
+    sth = cmk(uy / den, -uz / den);
+    big = fabs(sth.x) + fabs(sth.y) > tol_rot;
+    return true;
+}
+
+// Blocks of more than 80 states (b <= 256): block Jacobi with the same rotations.  The b x b
+// iterate M lives in the workgroup's global scratch slot (ld = b), W in bufW.  The indices are
+// cut into column blocks of BJ_NB (the last one padded with absent indices) and the blocks
+// paired by the circle method.  In each round every pair (P, Q) of the round -- up to 16 at
+// once -- copies its 2 BJ_NB x 2 BJ_NB principal submatrix into LDS and runs one joint-Jacobi
+// sweep on it there (15 rounds of 8 rotations), accumulating its rotation V; then
+// M <- V^H M on the pair's rows (every pair), and M <- M V, W <- W V on its columns.  Sweeps
+// repeat until no rotation of a sweep exceeds tol_rot.  A round reads and writes M twice and W
+// once, for b / BJ_NB - 1 rounds per sweep where the scalar form (k_eig with MB = 0) takes b - 1
+// rounds over M and W.  mode as k_eig.
+#define BJ_NB 8
+#define BJ_NS (2 * BJ_NB)
+#define BJ_MAXPAIR 16 /* b <= 16 BJ_MAXPAIR */
+template <typename R>
+__global__ void __launch_bounds__(512) k_eig_blk(DevPlan P, int Kb, const int *__restrict__ grp,
+                                                 int ngrp, const typename CX<R>::T *__restrict__ src,
+                                                 size_t src_stride, int mode, cd *__restrict__ bufW,
+                                                 typename CX<R>::T *__restrict__ bufWr,
+                                                 double *__restrict__ outv, cd *__restrict__ gscr,
+                                                 size_t gslot) {
+    typedef typename CX<R>::T CT;
+    constexpr int NT = 512, NB = BJ_NB, NS = BJ_NS, LS = NS + 1;
+    constexpr int RW = NB * NB + NB * NS; // rotation work items per pair and sub-round
+    constexpr double tol_rot = __is_same(R, double) ? 1e-15 : 1e-10;
+    constexpr double tol_off = __is_same(R, double) ? 1e-17 : 1e-12;
+    __shared__ cd sS[BJ_MAXPAIR * NS * LS];
+    __shared__ cd sV[BJ_MAXPAIR * NS * NS];
+    __shared__ double rc[BJ_MAXPAIR * NB];
+    __shared__ cd rs[BJ_MAXPAIR * NB];
+    __shared__ int rp[BJ_MAXPAIR * NB], rq[BJ_MAXPAIR * NB];
+    __shared__ int s_gi[BJ_MAXPAIR * NS]; // global index of each pair's local index, -1 absent
+    __shared__ int s_rot[BJ_MAXPAIR];
+    __shared__ int s_any;
+    const int tid = threadIdx.x;
+    cd *Ms = gscr + gslot * blockIdx.x;
+    const long ntask = (long)Kb * ngrp;
+    for (long task = blockIdx.x; task < ntask; task += gridDim.x) {
+        const int g = (int)(task % ngrp), c = (int)(task / ngrp);
+        const int blk = grp[g], b = P.blk_dim[blk];
+        const CT *S0 = src + (size_t)c * src_stride + P.blk_uoff[blk];
+        cd *W = bufW + (size_t)c * P.Lsum + P.blk_uoff[blk];
+        for (int e = tid; e < b * b; e += NT) {
+            Ms[e] = to_cd(S0[e]);
+            W[e] = cmk(e % b == e / b ? 1.0 : 0.0, 0.0);
+        }
+        const int nbk = 2 * ((b + NS - 1) / NS); // even number of column blocks
+        const int npr = nbk / 2;                 // pairs per round, <= BJ_MAXPAIR
+        __syncthreads();
+        for (int sweep = 0; sweep < 40 && b > 1; sweep++) {
+            if (tid == 0) s_any = 0;
+            for (int round = 0; round < nbk - 1; round++) {
+                for (int x = tid; x < npr * NS; x += NT) {
+                    const int pi = x / NS, l = x % NS;
+                    const int pa = pi, pb = nbk - 1 - pi;
+                    const int ia = pa == 0 ? 0 : ((pa - 1 + round) % (nbk - 1)) + 1;
+                    const int ib = pb == 0 ? 0 : ((pb - 1 + round) % (nbk - 1)) + 1;
+                    const int Pb = min(ia, ib), Qb = max(ia, ib);
+                    const int gi = l < NB ? Pb * NB + l : Qb * NB + (l - NB);
+                    s_gi[x] = gi < b ? gi : -1;
+                }
+                if (tid < BJ_MAXPAIR) s_rot[tid] = 0;
+                __syncthreads();
+                for (int x = tid; x < npr * NS * NS; x += NT) {
+                    const int pi = x / (NS * NS), e = x % (NS * NS), i = e % NS, j = e / NS;
+                    const int gi = s_gi[pi * NS + i], gj = s_gi[pi * NS + j];
+                    sS[pi * NS * LS + i + j * LS] =
+                        (gi >= 0 && gj >= 0) ? Ms[gi + (size_t)gj * b] : cmk(0, 0);
+                    sV[pi * NS * NS + e] = cmk(i == j ? 1.0 : 0.0, 0.0);
+                }
+                __syncthreads();
+                // one joint-Jacobi sweep of every pair's submatrix (circle method over NS)
+                for (int sr = 0; sr < NS - 1; sr++) {
+                    for (int x = tid; x < npr * NB; x += NT) {
+                        const int pi = x / NB, k = x % NB;
+                        const int pa = k, pb = NS - 1 - k;
+                        const int ia = pa == 0 ? 0 : ((pa - 1 + sr) % (NS - 1)) + 1;
+                        const int ib = pb == 0 ? 0 : ((pb - 1 + sr) % (NS - 1)) + 1;
+                        const int p = min(ia, ib), q = max(ia, ib);
+                        const cd *Sp = sS + pi * NS * LS;
+                        double cth = 1.0;
+                        cd sth = cmk(0, 0);
+                        bool big = false;
+                        if (s_gi[pi * NS + p] >= 0 && s_gi[pi * NS + q] >= 0 &&
+                            jj_rotation(Sp[p + p * LS], Sp[q + q * LS], Sp[p + q * LS],
+                                        Sp[q + p * LS], tol_off, tol_rot, cth, sth, big)) {
+                            s_rot[pi] = 1;
+                            if (big) s_any = 1;
+                        }
+                        rc[x] = cth;
+                        rs[x] = sth;
+                        rp[x] = p;
+                        rq[x] = q;
+                    }
+                    __syncthreads();
+                    // S <- R^H S R on every 2 x 2 block of rotation pairs; V <- V R
+                    for (int x = tid; x < npr * RW; x += NT) {
+                        const int pi = x / RW, e = x % RW;
+                        if (e < NB * NB) {
+                            cd *Sp = sS + pi * NS * LS;
+                            const int ki = pi * NB + e % NB, kj = pi * NB + e / NB;
+                            const int p = rp[ki], q = rq[ki], p2 = rp[kj], q2 = rq[kj];
+                            const double ci = rc[ki], cj = rc[kj];
+                            const cd si = rs[ki], sj = rs[kj];
+                            const cd a = Sp[p + p2 * LS], bb2 = Sp[p + q2 * LS];
+                            const cd c2 = Sp[q + p2 * LS], d = Sp[q + q2 * LS];
+                            const cd a1 = cadd(cscale(a, ci), cmul(cconj(si), c2));
+                            const cd c1 = csub(cscale(c2, ci), cmul(si, a));
+                            const cd b1 = cadd(cscale(bb2, ci), cmul(cconj(si), d));
+                            const cd d1 = csub(cscale(d, ci), cmul(si, bb2));
+                            Sp[p + p2 * LS] = cadd(cscale(a1, cj), cmul(sj, b1));
+                            Sp[p + q2 * LS] = csub(cscale(b1, cj), cmul(cconj(sj), a1));
+                            Sp[q + p2 * LS] = cadd(cscale(c1, cj), cmul(sj, d1));
+                            Sp[q + q2 * LS] = csub(cscale(d1, cj), cmul(cconj(sj), c1));
+                        } else {
+                            cd *Vp = sV + pi * NS * NS;
+                            const int f = e - NB * NB, row = f % NS, k = pi * NB + f / NS;
+                            const int p = rp[k], q = rq[k];
+                            const double cth = rc[k];
+                            const cd sth = rs[k];
+                            const cd wp = Vp[row + p * NS], wq = Vp[row + q * NS];
+                            Vp[row + p * NS] = cadd(cscale(wp, cth), cmul(sth, wq));
+                            Vp[row + q * NS] = csub(cscale(wq, cth), cmul(cconj(sth), wp));
+                        }
+                    }
+                    __syncthreads();
+                }
+                // rows of every rotated pair: M(idx, col) <- V^H M(idx, col)
+                for (int x = tid; x < npr * b; x += NT) {
+                    const int pi = x / b, col = x % b;
+                    if (!s_rot[pi]) continue;
+                    const int *gi = s_gi + pi * NS;
+                    const cd *Vp = sV + pi * NS * NS;
+                    cd m[NS];
+#pragma unroll
+                    for (int l = 0; l < NS; l++)
+                        m[l] = gi[l] >= 0 ? Ms[gi[l] + (size_t)col * b] : cmk(0, 0);
+#pragma unroll
+                    for (int i = 0; i < NS; i++) {
+                        if (gi[i] < 0) continue;
+                        cd acc = cmk(0, 0);
+#pragma unroll
+                        for (int l = 0; l < NS; l++) cfma(acc, cconj(Vp[l + i * NS]), m[l]);
+                        Ms[gi[i] + (size_t)col * b] = acc;
+                    }
+                }
+                __syncthreads();
+                // columns of every rotated pair: M(row, idx) <- M(row, idx) V, W likewise
+                for (int x = tid; x < 2 * npr * b; x += NT) {
+                    const int which = x / (npr * b), y = x % (npr * b), pi = y / b, row = y % b;
+                    if (!s_rot[pi]) continue;
+                    const int *gi = s_gi + pi * NS;
+                    const cd *Vp = sV + pi * NS * NS;
+                    cd *X = which ? W : Ms;
+                    cd m[NS];
+#pragma unroll
+                    for (int l = 0; l < NS; l++)
+                        m[l] = gi[l] >= 0 ? X[row + (size_t)gi[l] * b] : cmk(0, 0);
+#pragma unroll
+                    for (int j = 0; j < NS; j++) {
+                        if (gi[j] < 0) continue;
+                        cd acc = cmk(0, 0);
+#pragma unroll
+                        for (int l = 0; l < NS; l++) cfma(acc, m[l], Vp[l + j * NS]);
+                        X[row + (size_t)gi[j] * b] = acc;
+                    }
+                }
+                __syncthreads();
+            }
+            // every thread reads the flag before the next sweep's reset by thread 0
+            const int any = s_any;
+            __syncthreads();
+            if (!any) break;
+        }
+        double *o = outv + (size_t)c * P.N + P.blk_soff[blk];
+        for (int r = tid; r < b; r += NT) {
+            const cd l = Ms[r + (size_t)r * b];
+            o[r] = (mode == 0) ? atan2(l.y, l.x) : l.x;
+        }
+        if (bufWr) {
+            CT *Wr = bufWr + (size_t)c * P.Lsum + P.blk_uoff[blk];
+            for (int e = tid; e < b * b; e += NT) Wr[e] = CX<R>::from(W[e]);
         }
         __syncthreads();
     }
@@ -2663,7 +2887,7 @@ static void launch_eig(const DevPlan *P, int Kb, const int *grp, int ngrp, int g
     case 2: k_eig<48, 256, R><<<grid, 256, 0, st>>>(EIG_ARGS); break;
     case 3: k_eig<64, 1024, R><<<grid, 1024, 0, st>>>(EIG_ARGS); break;
     case 4: k_eig<80, 1024, R><<<grid, 1024, 0, st>>>(EIG_ARGS); break;
-    default: k_eig<0, 1024, R><<<grid, 1024, 0, st>>>(EIG_ARGS); break;
+    default: k_eig_blk<R><<<grid, 512, 0, st>>>(EIG_ARGS); break;
     }
 #undef EIG_ARGS
 }

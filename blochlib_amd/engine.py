@@ -351,7 +351,7 @@ class Plan:
             if n[slot]:
                 ph, M = divmod(slot, 8)
                 if ph == 2:   # eigen-solver size classes
-                    name = "k_eig<" + ("16", "32", "48", "64", "80", "global")[M - 1] + ">"
+                    name = "k_eig<" + ("16", "32", "48", "64", "80", "blk")[M - 1] + ">"
                 else:
                     name = self.KERNELS[ph] + (f"<{M}>" if M else "")
                 out[name] = {"ms": float(ms[slot]), "launches": int(n[slot]),
