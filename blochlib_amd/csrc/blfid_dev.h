@@ -7,6 +7,12 @@
 #define DEV_MAX_SPINS 12
 #define DEV_MAX_BLOCK 256 /* largest Hilbert-space block (k_eig pair tables) */
 #define DEV_LDS_BLOCK 80  /* blocks up to this size keep k_eig's matrix in LDS */
+// Row padding (floats) of the LDS-resident product planes (chain_mfma, SymTile): rows of
+// 16 M + LDS_PAD floats, a multiple of four so that every row starts on 16 bytes (8, 12 and 20
+// measured no faster on the batched clusters).
+#ifndef LDS_PAD
+#define LDS_PAD 4
+#endif
 
 struct DevPlan {
     int S, N;      // spins, Hilbert dimension

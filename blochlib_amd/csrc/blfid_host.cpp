@@ -239,7 +239,7 @@ struct blfid_plan {
     std::map<int, std::vector<int>> fuse_groups; // M -> blocks of <= DEV_LDS_BLOCK states
     std::vector<int> big_blocks;                 // blocks beyond DEV_LDS_BLOCK states
     // fused kernel: per block of <= DEV_LDS_BLOCK states, its flip-flop positions in the LDS plane
-    // of its tile variant M, (r LDK + q) << 8 | (i S + j), LDK = 16 M + 4 (CSR over blocks)
+    // of its tile variant M, (r LDK + q) << 8 | (i S + j), LDK = 16 M + LDS_PAD (CSR over blocks)
     std::vector<int> xd_off, xd_list;
     std::map<int, std::vector<int>> eig_cls;    // eigen-solver size class -> blocks
     int exp_slots = 0, qr_slots = 0, eig_slots = 0;
@@ -591,7 +591,7 @@ static blfid_status build_tables(blfid_plan *pl) {
         const auto &st = pl->blocks[b].states;
         const int d = (int)st.size();
         if (d <= DEV_LDS_BLOCK) {
-            const int LDK = 16 * ((d + 15) / 16) + 4;
+            const int LDK = 16 * ((d + 15) / 16) + LDS_PAD;
             for (int r = 0; r < d; r++)
                 for (int q = 0; q < d; q++) {
                     const int x = st[r] ^ st[q];

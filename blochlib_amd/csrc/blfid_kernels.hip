@@ -797,12 +797,12 @@ __global__ void __launch_bounds__(NThr<M>::v, (M == 1 && sizeof(R) == 4) ? 4 : 1
 //   !DYSON: C_k = X_k W, k = 0..n-1 -- W stays the B operand for every k.
 // Each step then loads only its A operand (U_k or X_k), prefetched into registers one step
 // ahead.  Operand layout, fragments and products as in wg_cgemm_mfma.  X_k = X0 + k xstride,
-// output k at C0 + (k % ckeep) cstride.  `smem` holds 4 * 16 M * (16 M + 4) floats.
+// output k at C0 + (k % ckeep) cstride.  `smem` holds 4 * 16 M * (16 M + LDS_PAD) floats.
 template <int M, int NT, bool DYSON>
 __device__ __forceinline__ void chain_mfma(int b, int n, const cf *__restrict__ X0, size_t xstride,
                                            const cf *__restrict__ W, cf *__restrict__ C0,
                                            size_t cstride, int ckeep, float *smem) {
-    constexpr int TM = 16 * M, LDK = TM + 4, NW = NT / 64, NSUB = (M * M + NW - 1) / NW;
+    constexpr int TM = 16 * M, LDK = TM + LDS_PAD, NW = NT / 64, NSUB = (M * M + NW - 1) / NW;
     constexpr int PER = TM * TM / NT; // operand elements staged per thread
     float *Ar = smem, *Ai = Ar + TM * LDK, *Br = Ai + TM * LDK, *Bi = Br + TM * LDK;
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63, g = lane >> 4, r = lane & 15;
@@ -902,7 +902,7 @@ __global__ void __launch_bounds__(NThr<M>::v, (M == 3 && sizeof(R) == 4) ? 3 : (
     constexpr int NT = NThr<M>::v;
     typedef typename CX<R>::T CT;
     constexpr bool CHAIN = std::is_same<R, float>::value && !SPLIT;
-    constexpr int CHB = 4 * 16 * M * (16 * M + 4) * 4, GB = CGemm<M, NT, R, SPLIT>::BYTES;
+    constexpr int CHB = 4 * 16 * M * (16 * M + LDS_PAD) * 4, GB = CGemm<M, NT, R, SPLIT>::BYTES;
     __shared__ __align__(16) char smem[CHAIN && CHB > GB ? CHB : GB];
     const long ntask = (long)Kb * ngrp;
     for (long task = blockIdx.x; task < ntask; task += gridDim.x) {
@@ -932,7 +932,7 @@ __global__ void __launch_bounds__(NThr<M>::v, (M == 3 && sizeof(R) == 4) ? 3 : (
 
 // ------------------------------------------------------------------------------ k_expdyson
 // LDS-resident real symmetric products on FP32 matrix cores.  A plane holds a 16M x 16M symmetric
-// matrix (zero past the block size b) in rows of LDK = 16M + 4 floats: plane[r LDK + q] =
+// matrix (zero past the block size b) in rows of LDK = 16M + LDS_PAD floats: plane[r LDK + q] =
 // X(r, q) = X(q, r), so one layout is both the k-contiguous A operand (rows) and B operand
 // (columns) of v_mfma_f32_16x16x4f32 (lane 16 g + r reads the float4 at row r, k = 4g .. 4g + 3
 // of each 16-wide k slice, as wg_cgemm_mfma).  Every polynomial in X, and cos X, sin X, is
@@ -940,7 +940,7 @@ __global__ void __launch_bounds__(NThr<M>::v, (M == 3 && sizeof(R) == 4) ? 3 : (
 // float4 runs, plane[col][row .. row + 3], with no exchange.  Sub-tiles t = w + NW s of the
 // output are dealt round robin to the NW waves.
 template <int M, int NT> struct SymTile {
-    static constexpr int TM = 16 * M, LDK = TM + 4, NW = NT / 64, NSUB = (M * M + NW - 1) / NW;
+    static constexpr int TM = 16 * M, LDK = TM + LDS_PAD, NW = NT / 64, NSUB = (M * M + NW - 1) / NW;
     static constexpr int PLANE = TM * LDK; // floats
 };
 
@@ -1299,8 +1299,12 @@ __global__ void __launch_bounds__(NT) k_eig(DevPlan P, int Kb, const int *__rest
     // complex64 eigenvectors of the two largest LDS classes accumulate in LDS (FP32) when only
     // the storage-precision copy is wanted (the spinning path): the rotations then never wait on
     // global memory.  The FP64 copy W is not written in that case.  (Both classes already run
-    // one 1024-thread workgroup per CU, so the extra LDS costs no occupancy.)
-    constexpr bool WLDS = (MB == EIG_MAXB || MB == 64) && __is_same(R, float);
+    // one 1024-thread workgroup per CU, so the extra LDS costs no occupancy; for the 16-48-state
+    // classes it does, and holding W in LDS there made k_eig 3% slower on the batched clusters.)
+#ifndef EIG_WLDS_MIN
+#define EIG_WLDS_MIN 64
+#endif
+    constexpr bool WLDS = MB >= EIG_WLDS_MIN && __is_same(R, float);
     __shared__ cd Mlds[MB * (MB + 1)];
     __shared__ cf Wl[WLDS ? MB * MB : 1];
     const bool wl = WLDS && bufWr != nullptr;
@@ -1689,7 +1693,7 @@ __global__ void __launch_bounds__(NThr<M>::v, (M == 3 && sizeof(R) == 4) ? 3 : (
     // complex64: one task per (crystallite, block) walks all n steps with W resident in LDS;
     // complex128 (and the BF16x3 form): one task per (crystallite, step, block)
     constexpr bool CHAIN = std::is_same<R, float>::value && !SPLIT;
-    constexpr int CHB = 4 * 16 * M * (16 * M + 4) * 4, GB = CGemm<M, NT, R, SPLIT>::BYTES;
+    constexpr int CHB = 4 * 16 * M * (16 * M + LDS_PAD) * 4, GB = CGemm<M, NT, R, SPLIT>::BYTES;
     __shared__ __align__(16) char smem[CHAIN && CHB > GB ? CHB : GB];
     if constexpr (CHAIN) {
         // blocks above 80 states (M = 5) exceed the chain's tile: one task per step for them
