@@ -116,3 +116,26 @@ def test_family_roofline_bound_and_units():
 def test_cpu_info_caps_at_one_gpu_share():
     cores, model = bench.cpu_info()
     assert 1 <= cores <= 16 and isinstance(model, str)
+
+
+def test_tuning_table_round_trip_and_saved_file():
+    """The 16-int launch table <-> dict used by bench.py and the saved autotuner table: every
+    entry survives the round trip (JSON keys as strings too), and the committed table parses."""
+    import json
+    import os
+    from blochlib_amd import engine
+    v = np.zeros(engine.TUNE_LEN, np.int32)
+    v[[1, 3]] = 1                      # two-kernel propagator for tile classes 2 and 4
+    v[5] = 2                           # correlation variant
+    v[6 + engine.TUNE_FAMILIES.index("qr")] = 2048
+    v[15] = 0b0101                     # other Q-product form for classes 1 and 3
+    d = engine.tuning_to_dict(v)
+    assert d["method"][2] == "two" and d["method"][1] == "fused"
+    assert d["grid"]["qr"] == 2048 and d["qr_flip"] == {1: True, 2: False, 3: True, 4: False}
+    assert np.array_equal(engine.tuning_to_array(d), v)
+    assert np.array_equal(engine.tuning_to_array(json.loads(json.dumps(d))), v)
+    with open(bench.TUNING_FILE) as f:
+        saved = json.load(f)
+    t = engine.tuning_to_array(saved["tuning"])
+    assert t.shape == (engine.TUNE_LEN,) and t[15] in range(16) and os.path.basename(
+        bench.TUNING_FILE) == "mi355x.json"

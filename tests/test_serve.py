@@ -159,3 +159,26 @@ def test_server_prefers_least_remaining_work_on_free_streams(monkeypatch):
     order = [r.rid for r in done]
     assert order[-1] == 0 and sorted(order) == [0, 1, 2, 3, 4]
     assert all(sl.act is None and not sl.inflight for sl in srv.slots)
+
+
+def test_slice_size_keeps_per_block_launches_full(monkeypatch):
+    """Slices follow slice_flops, but never leave a per-(crystallite, block) launch with fewer than
+    fill_tasks workgroups, and never exceed the powder."""
+    srv, clock, log = _fake_server(monkeypatch, streams=1)
+
+    class P:
+        steps = 128
+
+        def __init__(self, per, blocks):
+            self._s = {"flops_per_cs": per, "blocks": blocks}
+
+        def stats(self):
+            return self._s
+
+    srv.fill_tasks, srv.min_slice = 1024, 4
+    srv.slice_flops = 4e12
+    assert srv.slice_size(P(1.5e9, 11), 3722) == 94        # fill bound: ceil(1024 / 11)
+    assert srv.slice_size(P(1e6, 11), 3722) == 3722        # cheap plan: the whole powder
+    assert srv.slice_size(P(1e7, 11), 3722) == 3125        # flops bound: 4e12 / 1.28e9
+    assert srv.slice_size(P(1.5e9, 2000), 3722) == 20      # flops bound above fill and minimum
+    assert srv.slice_size(P(1e12, 2000), 10) == 4          # at least min_slice
