@@ -1290,9 +1290,13 @@ __global__ void __launch_bounds__(NT) k_eig(DevPlan P, int Kb, const int *__rest
                                              double *__restrict__ outv, cd *__restrict__ gscr,
                                              size_t gslot) {
     typedef typename CX<R>::T CT;
-    // convergence: rotations below tol_rot end the sweeps; pairs below tol_off are skipped.  A
-    // complex64 period propagator carries ~1e-7 relative error, so its solve stops earlier.
+    // convergence: a sweep whose rotations all stay below tol_stop = sqrt(tol_rot) ends the
+    // sweeps (Jacobi converges quadratically: the off-diagonal part such a sweep leaves is of
+    // order tol_rot, which the confirming sweep of a rotations-below-tol_rot test would only
+    // square); pairs below tol_off are skipped.  A complex64 period propagator carries ~1e-7
+    // relative error, so its solve stops earlier.
     constexpr double tol_rot = __is_same(R, double) ? 1e-15 : 1e-10;
+    constexpr double tol_stop = __is_same(R, double) ? 3.1622776601683795e-8 : 1e-5;
     constexpr double tol_off = __is_same(R, double) ? 1e-17 : 1e-12;
     static_assert(MB > 0 && MB <= EIG_MAXB, "blocks above 80 states take k_eig_blk");
     constexpr int MP = (MB + 1) / 2;
@@ -1373,7 +1377,7 @@ __global__ void __launch_bounds__(NT) k_eig(DevPlan P, int Kb, const int *__rest
                                 cth = sqrt((ux + r) / (2.0 * r));
                                 double den = sqrt(2.0 * r * (ux + r));
                                 sth = cmk(uy / den, -uz / den);
-                                if (fabs(sth.x) + fabs(sth.y) > tol_rot) s_any = 1;
+                                if (fabs(sth.x) + fabs(sth.y) > tol_stop) s_any = 1;
                             }
                         }
                     }
@@ -1457,9 +1461,9 @@ __global__ void __launch_bounds__(NT) k_eig(DevPlan P, int Kb, const int *__rest
 // ------------------------------------------------------------------------------ k_eig_blk
 // Joint rotation of the index pair (p, q) of a normal-matrix iterate: the Cardoso-Souloumiac
 // angle for H1 = (M + M^H)/2, H2 = (M - M^H)/2i, as k_eig forms it.  Returns false (identity)
-// when the pair's off-diagonal part is below tol_off of its diagonal; big: |s| > tol_rot.
+// when the pair's off-diagonal part is below tol_off of its diagonal; big: |s| > tol_big.
 __device__ __forceinline__ bool jj_rotation(cd mpp, cd mqq, cd mpq, cd mqp, double tol_off,
-                                            double tol_rot, double &cth, cd &sth, bool &big) {
+                                            double tol_big, double &cth, cd &sth, bool &big) {
     cth = 1.0;
     sth = cmk(0, 0);
     big = false;
@@ -1490,7 +1494,7 @@ __device__ __forceinline__ bool jj_rotation(cd mpp, cd mqq, cd mpq, cd mqp, doub
     cth = sqrt((ux + r) / (2.0 * r));
     const double den = sqrt(2.0 * r * (ux + r));
     sth = cmk(uy / den, -uz / den);
-    big = fabs(sth.x) + fabs(sth.y) > tol_rot;
+    big = fabs(sth.x) + fabs(sth.y) > tol_big;
     return true;
 }
 
@@ -1501,9 +1505,9 @@ __device__ __forceinline__ bool jj_rotation(cd mpp, cd mqq, cd mpq, cd mqp, doub
 // once -- copies its 2 BJ_NB x 2 BJ_NB principal submatrix into LDS and runs one joint-Jacobi
 // sweep on it there (15 rounds of 8 rotations), accumulating its rotation V; then
 // M <- V^H M on the pair's rows (every pair), and M <- M V, W <- W V on its columns.  Sweeps
-// repeat until no rotation of a sweep exceeds tol_rot.  A round reads and writes M twice and W
-// once, for b / BJ_NB - 1 rounds per sweep where the scalar form (k_eig with MB = 0) takes b - 1
-// rounds over M and W.  mode as k_eig.
+// repeat until no rotation of a sweep exceeds sqrt(tol_rot) (as in k_eig).  A round reads and
+// writes M twice and W once, for b / BJ_NB - 1 rounds per sweep where the scalar rotations take
+// b - 1 passes over M and W.  mode as k_eig.
 #define BJ_NB 8
 #define BJ_NS (2 * BJ_NB)
 #define BJ_MAXPAIR 16 /* b <= 16 BJ_MAXPAIR */
@@ -1517,7 +1521,7 @@ __global__ void __launch_bounds__(512) k_eig_blk(DevPlan P, int Kb, const int *_
     typedef typename CX<R>::T CT;
     constexpr int NT = 512, NB = BJ_NB, NS = BJ_NS, LS = NS + 1;
     constexpr int RW = NB * NB + NB * NS; // rotation work items per pair and sub-round
-    constexpr double tol_rot = __is_same(R, double) ? 1e-15 : 1e-10;
+    constexpr double tol_stop = __is_same(R, double) ? 3.1622776601683795e-8 : 1e-5;
     constexpr double tol_off = __is_same(R, double) ? 1e-17 : 1e-12;
     __shared__ cd sS[BJ_MAXPAIR * NS * LS];
     __shared__ cd sV[BJ_MAXPAIR * NS * NS];
@@ -1578,7 +1582,7 @@ __global__ void __launch_bounds__(512) k_eig_blk(DevPlan P, int Kb, const int *_
                         bool big = false;
                         if (s_gi[pi * NS + p] >= 0 && s_gi[pi * NS + q] >= 0 &&
                             jj_rotation(Sp[p + p * LS], Sp[q + q * LS], Sp[p + q * LS],
-                                        Sp[q + p * LS], tol_off, tol_rot, cth, sth, big)) {
+                                        Sp[q + p * LS], tol_off, tol_stop, cth, sth, big)) {
                             s_rot[pi] = 1;
                             if (big) s_any = 1;
                         }
