@@ -942,15 +942,42 @@ __global__ void __launch_bounds__(NThr<M>::v, (M == 3 && sizeof(R) == 4) ? 3 : (
 template <int M, int NT> struct SymTile {
     static constexpr int TM = 16 * M, LDK = TM + LDS_PAD, NW = NT / 64, NSUB = (M * M + NW - 1) / NW;
     static constexpr int PLANE = TM * LDK; // floats
+    // symmetric products of the classes whose M x M tiles do not divide evenly over the waves
+    // (3 x 3 over 4, 5 x 5 over 8) compute only the tiles I <= J and store each twice: 6 and 15
+    // tiles take two per wave where 9 and 25 took three and four
+    static constexpr bool UP = (M == 3 || M == 5);
 };
+
+// Output tile (I, J) (16 x 16 blocks of the 16M x 16M result) of sub-tile s of wave w: tile
+// t = w + NW s of the M x M tiles, or of the M(M+1)/2 tiles I <= J (row by row) when UP.
+template <int M, int NT, bool UP>
+__device__ __forceinline__ bool sym_tile(int w, int s, int &I, int &J) {
+    const int u = w + SymTile<M, NT>::NW * s;
+    if constexpr (UP) {
+        if (u >= M * (M + 1) / 2) return false;
+        int i = 0;
+#pragma unroll
+        for (int q = 1; q < M; q++)
+            if (u >= q * M - q * (q - 1) / 2) i = q;
+        I = i;
+        J = u - (i * M - i * (i - 1) / 2) + i;
+        return true;
+    } else {
+        if (u >= M * M) return false;
+        I = u / M;
+        J = u % M;
+        return true;
+    }
+}
 
 template <int NS> __device__ __forceinline__ void sym_zero(v4f (&acc)[NS]) {
 #pragma unroll
     for (int s = 0; s < NS; s++) acc[s] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
 }
 
-// acc += sa PA PB over k < kp (the wave's sub-tiles).  VIRT: the B operand is formed on the fly
-// from two planes, ca I + cb PY + cc PZ (the identity masked to k < b), instead of read from PB.
+// acc += sa PA PB over k < kp (the wave's sub-tiles; only tiles I <= J when the result is
+// symmetric and SymTile::UP).  VIRT: the B operand is formed on the fly from two planes,
+// ca I + cb PY + cc PZ (the identity masked to k < b), instead of read from PB.
 template <int M, int NT, bool VIRT, int NS>
 __device__ __forceinline__ void sym_mfma(const float *PA, const float *PB, const float *PY,
                                          const float *PZ, float sa, float ca, float cb, float cc,
@@ -961,10 +988,10 @@ __device__ __forceinline__ void sym_mfma(const float *PA, const float *PB, const
     for (int qq = 0; qq < kp; qq += 16) {
 #pragma unroll
         for (int s = 0; s < T::NSUB; s++) {
-            const int t = w + T::NW * s;
-            if (t < M * M) {
-                const int col = 16 * (t % M) + r;
-                const int oa = (16 * (t / M) + r) * T::LDK + qq + 4 * g;
+            int I, J;
+            if (sym_tile<M, NT, T::UP>(w, s, I, J)) {
+                const int col = 16 * J + r;
+                const int oa = (16 * I + r) * T::LDK + qq + 4 * g;
                 const int ob = col * T::LDK + qq + 4 * g;
                 const v4f av = *(const v4f *)&PA[oa];
                 v4f bv;
@@ -986,6 +1013,18 @@ __device__ __forceinline__ void sym_mfma(const float *PA, const float *PB, const
     }
 }
 
+// Store v (a lane's four accumulator values of tile (I, J) of a symmetric result) into the
+// plane: transposed as one float4 at block (J, I) and, for I != J, directly at block (I, J).
+template <int M, int NT>
+__device__ __forceinline__ void sym_put(float *PL, int I, int J, int g, int r, const v4f &v) {
+    typedef SymTile<M, NT> T;
+    *(v4f *)&PL[(16 * J + r) * T::LDK + 16 * I + 4 * g] = v;
+    if (T::UP && I != J) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) PL[(16 * I + 4 * g + j) * T::LDK + 16 * J + r] = v[j];
+    }
+}
+
 // plane = sc * acc (symmetric result, full 16M x 16M tile including the zero padding)
 template <int M, int NT, int NS>
 __device__ __forceinline__ void sym_store(float *PL, const v4f (&acc)[NS], float sc) {
@@ -994,12 +1033,12 @@ __device__ __forceinline__ void sym_store(float *PL, const v4f (&acc)[NS], float
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63, g = lane >> 4, r = lane & 15;
 #pragma unroll
     for (int s = 0; s < T::NSUB; s++) {
-        const int t = w + T::NW * s;
-        if (t < M * M) {
+        int I, J;
+        if (sym_tile<M, NT, T::UP>(w, s, I, J)) {
             v4f v;
 #pragma unroll
             for (int j = 0; j < 4; j++) v[j] = sc * acc[s][j];
-            *(v4f *)&PL[(16 * (t % M) + r) * T::LDK + 16 * (t / M) + 4 * g] = v;
+            sym_put<M, NT>(PL, I, J, g, r, v);
         }
     }
 }
@@ -1128,9 +1167,9 @@ __global__ void __launch_bounds__(XdThreads<M>::v) k_expdyson(DevPlan P, int Kb,
                 // C = I - Y/2 into Y's plane, P = I - Y/6 into Z's (no operand of this product)
 #pragma unroll
                 for (int s = 0; s < NSUB; s++) {
-                    const int t = w + NW * s;
-                    if (t < M * M) {
-                        const int col = 16 * (t % M) + r, row0 = 16 * (t / M) + 4 * g;
+                    int I, J;
+                    if (sym_tile<M, NT, T::UP>(w, s, I, J)) {
+                        const int col = 16 * J + r, row0 = 16 * I + 4 * g;
                         v4f cv, pv;
 #pragma unroll
                         for (int j = 0; j < 4; j++) {
@@ -1138,8 +1177,8 @@ __global__ void __launch_bounds__(XdThreads<M>::v) k_expdyson(DevPlan P, int Kb,
                             cv[j] = d - 0.5f * acc[s][j];
                             pv[j] = d - (float)(1.0 / 6.0) * acc[s][j];
                         }
-                        *(v4f *)&PY[col * LDK + row0] = cv;
-                        *(v4f *)&PZ[col * LDK + row0] = pv;
+                        sym_put<M, NT>(PY, I, J, g, r, cv);
+                        sym_put<M, NT>(PZ, I, J, g, r, pv);
                     }
                 }
                 __syncthreads();
@@ -1171,9 +1210,9 @@ __global__ void __launch_bounds__(XdThreads<M>::v) k_expdyson(DevPlan P, int Kb,
                 const float as = m == 2 ? (float)(1.0 / 120.0) : 1.0f;
 #pragma unroll
                 for (int s = 0; s < NSUB; s++) {
-                    const int t = w + NW * s;
-                    if (t < M * M) {
-                        const int col = 16 * (t % M) + r, row0 = 16 * (t / M) + 4 * g;
+                    int I, J;
+                    if (sym_tile<M, NT, T::UP>(w, s, I, J)) {
+                        const int col = 16 * J + r, row0 = 16 * I + 4 * g;
                         const int o = col * LDK + row0;
                         const v4f y = *(const v4f *)&PY[o];
                         v4f cv, pv;
@@ -1186,8 +1225,8 @@ __global__ void __launch_bounds__(XdThreads<M>::v) k_expdyson(DevPlan P, int Kb,
                             cv[j] = (d - 0.5f * y[j]) + ac * acc[s][j];
                             pv[j] = (d - (float)(1.0 / 6.0) * y[j]) + as * p2;
                         }
-                        *(v4f *)&PY[o] = cv;
-                        *(v4f *)&PZ[o] = pv;
+                        sym_put<M, NT>(PY, I, J, g, r, cv);
+                        sym_put<M, NT>(PZ, I, J, g, r, pv);
                     }
                 }
                 __syncthreads();
