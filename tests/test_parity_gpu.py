@@ -2,7 +2,7 @@
 
 Tolerance: relative L2 <= 1e-9 for the FP64 engine on these sizes (the engine's exp is a
 Taylor/scaling-squaring evaluation and its eigen-solver is joint Jacobi, so agreement is to
-rounding, not bitwise).  The north-star bar is 1e-5.
+rounding, not bitwise).  The accuracy bar of the workload is 1e-5.
 """
 import sys
 

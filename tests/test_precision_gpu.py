@@ -1,6 +1,6 @@
 """GPU parity of the complex64 form (BLFID_C64: FP32 propagators, Dyson products, basis changes
 and correlation sums; FP64 eigen-solver, phases, FID points and powder reduction) against the
-FP64 oracle, at the north-star bar: relative L2 <= 1e-5 over the powder-summed FID."""
+FP64 oracle, at the workload's accuracy bar: relative L2 <= 1e-5 over the powder-summed FID."""
 import numpy as np
 import pytest
 
