@@ -1,38 +1,33 @@
-"""solid-2.0 input files (``.sim``) for the GPU engine: the reader and a runner of the subset of
-the pulse language that the engine computes.
+"""solid-2.0 input files (``.sim``) for the GPU engine: the reader and a runner of the pulse
+language over the whole powder at once.
 
 Reference behaviour restated (examples/solid-2.0/src):
   * sections ``spins{}``, ``parameters{}``, ``pulses{}`` (solid.cc:115-193); ``spins{}`` is
-    SolidSys::read (src/QMspins/sys.cc:120-204); several spin systems are not supported;
+    SolidSys::read (src/QMspins/sys.cc:120-204), its subsections ``spin1{}``, ``spin2{}`` ...
+    named systems of the global lines plus their own (MultiSystem, multisystem.cc:104-181);
   * ``parameters{}``: ``name=expr`` assignments over the ParamSet defaults (paramset.cc:44-57)
     and ``powder{}`` / ``powder1{}`` ... sections with ``aveType`` (a file or a built-in
     generator), ``thetaStep``, ``phiStep``, ``gammaStep`` (multipowder.cc:56-114); the first
     powder is the default one (multipowder.cc:140-154);
-  * expressions: + - * / ^ with the Parser's functions and constants e, pi, Pi, PI, deg2rad,
-    rad2deg (src/utils/parser.cc:77-102, 219-246, 640-720);
-  * ``pulses{}`` statements: assignments, ``detect(op)``, ``ro(op)``, ``amplitude(a)``,
-    ``offset(o)``, ``wr=``, ``rotor=``, ``fid()``, ``savefid/savefidtext/savefidbinary/
-    savefidmatlab(name)`` (sequencerun_funcs.cc:752-894), ``ptop()``, sub-sequence sections
-    ``name{...}`` used by ``use(name[, repeat])`` / ``reuse(name[, repeat])``
-    (sequencerun_funcs.cc:465-551), and pulse lines ``ISO:pulse(t[, phase_deg[, amp[,
-    offset]]]) | ISO:delay(t)`` (pulsedata.cc:166-280) inside sub-sequences or, in ptop mode,
-    in the main section (doPulseData, sequencerun_funcs.cc:101-135); ``loop(v=a:b) ... end``
-    inside sub-sequences;
-  * a plain ``fid()`` (no pulses before it) is oneFID's powder FID divided by Tr(det det^H)
-    (sequencerun_funcs.cc:685-705, the whole powder in one engine call) and sets sw to the
-    adjusted value; a ptop ``fid()`` records Tr(rho det)/Tr(det det^H) and applies every used
-    propagator between points (sequencerun_funcs.cc:641-655);
+  * expressions: the Parser's grammar, functions and constants (src/utils/parser.cc);
+  * ``pulses{}`` statements in SequenceRun::decide's order: assignments, ``detect(op)``,
+    ``ro(op)``, ``amplitude(a)``, ``offset(o)``, ``wr=``, ``rotor=``, ``spinsys(name)``,
+    ``powder(name)``, ``alterSys(what, value)``, ``loop``/``if``/``elseif``/``else`` with
+    ScriptParse's resume rule, ``fid()`` and indexed ``fid(i)`` in 1-D, 2-D (``2D()``) and ptop
+    (``ptop()``) modes, every ``savefid*`` form (sequencerun_funcs.cc:553-894), sub-sequence
+    sections used by ``use(name[, repeat[, hold]])`` / ``reuse(...)`` (sequencerun_funcs.cc:
+    465-551; SequenceParse's unrolling and readPulses' evaluation, pulsedata.cc:287-378), and
+    pulse lines ``ISO:pulse(t[, phase_deg[, amp[, offset]]]) | ISO:delay(t)``
+    (pulsedata.cc:166-280);
   * propagators split each pulse of length tau into ceil(tau/maxtstep) steps unless
-    tau <= mintstep or wr == 0 (propogation.cc:257-300); pulses read inside a sub-sequence take
-    their rotor angle from the sub-sequence's ``rotor()`` with the magic angle as default
-    (pulsedata.cc:287-300), pulses of the main section the ``rotor`` parameter; a
-    sub-sequence's times start at 0 each time it is used (sequencerun.cc:122-162).
-  * pulses before a plain ``fid()``: every crystallite's rho is propagated through the used
-    propagators, then its gamma-COMPUTE FID runs from the current time (sequencerun_funcs.cc:
-    693-703) -- the engine's preparation stage (blfid_plan_set_prep).
-Everything else (2-D acquisition, indexed ``fid(i)``, main-section ``loop``/``if``, ``hold``,
-several spin systems or powders in one program) raises BlochLibError: the engine has no
-per-crystallite CPU path to fall back on.
+    tau <= mintstep or wr == 0 (propogation.cc:257-300); each pulse propagates with the spin
+    system named before it (pulsedata.cc:313-317, 362);
+  * every orientation's rho is a device state (blfid_state_*) that the used propagators advance
+    and every fid() reads; a plain fid() is the gamma-COMPUTE (or static) FID from it.
+Not supported (BlochLibError names the statement): ``cycler()``, ``reset()``, quadrupolar
+interactions, a ``powder()`` switch after the program has pulsed or acquired, a powder switch
+inside a sub-sequence, and a held sub-sequence whose pulses use several spin systems.  The
+engine has no per-crystallite CPU path to fall back on.
 """
 from __future__ import annotations
 
@@ -471,11 +466,13 @@ _ASSIGN = re.compile(r"^([A-Za-z_][A-Za-z0-9_]*)\s*=\s*(.+)$")
 
 @dataclass
 class Segment:
-    """One pulse line: its time span and per-channel (isotope, amp Hz, phase rad, offset Hz)."""
+    """One pulse line: its time span and per-channel (isotope, amp Hz, phase rad, offset Hz),
+    and the spin system it propagates with (a spinsys() name, 'default' the first)."""
     t1: float
     t2: float
     rotor_rad: float
     channels: list
+    sys: str = "default"
 
 
 # ------------------------------------------------------------------------------ program
@@ -489,18 +486,20 @@ class SimFile:
         if not spins:
             raise BlochLibError("no spins{} section")
         if len(spins) > 1:
-            raise BlochLibError("several spin systems (spins1{}, spins2{} ...) are not supported")
-        self.system = SolidSys.from_text("\n".join(_flatten(spins[0])))
+            raise BlochLibError("one spins{} section (several systems are its spin1{}, spin2{} "
+                                "... subsections)")
+        self.systems = self._systems(spins[0])
+        self.system = self.systems[min(self.systems)]
         self.params = dict(PARAM_DEFAULTS)
+        self.powders = {}
         self.powder = None
         par = root.sub("parameters")
         if par is not None:
-            pows = [s for s in par.items if isinstance(s, Section) and s.name.startswith("powder")]
-            pows.sort(key=lambda s: s.name)
             for ln in par.lines():
                 self._param_line(ln)
-            if pows:
-                self.powder = self._powder(pows[0])
+            self.powders = self._powders(par)
+            if self.powders:
+                self.powder = self.powders[min(self.powders)]
         self.pulses = root.sub("pulses")
         if self.pulses is None:
             raise BlochLibError("no pulses{} section")
@@ -529,6 +528,52 @@ class SimFile:
             self.params[name] = evaluate(val, self.params)
         except BlochLibError:
             self.params[name] = val
+
+    @staticmethod
+    def _numbered(sec: Section, base: str, limit_key: str) -> list:
+        """The subsections base1, base2, ... present in order (counted while the next one
+        exists, up to the section's own `limit_key` value, 100 by default), as MultiSystem::
+        parse and MultiPowder::parse count them (multisystem.cc:104-132,
+        multipowder.cc:56-87)."""
+        limit = 100
+        for ln in sec.lines():
+            tok = ln.replace("=", " ").split()
+            if len(tok) >= 2 and tok[0] == limit_key:
+                try:
+                    limit = int(float(tok[1]))
+                except ValueError:
+                    pass
+        out = []
+        while sec.sub(f"{base}{len(out) + 1}") is not None and len(out) <= limit:
+            out.append(sec.sub(f"{base}{len(out) + 1}"))
+        return out
+
+    def _systems(self, spins: Section) -> dict:
+        """MultiSystem (multisystem.cc:104-152): the spins{} lines outside subsections are the
+        global part; each subsection spin1{}, spin2{} ... is a system read from the global
+        lines followed by its own; without subsections the global lines are the one system
+        'spin1'.  Every system must keep the Hilbert dimension.  'default' is the first name
+        in order (the std::map's first), spin1."""
+        glob = spins.lines()
+        subs = self._numbered(spins, "spin", "numsys")
+        out = {}
+        if not subs:
+            out["spin1"] = SolidSys.from_text("\n".join(glob))
+        for i, sec in enumerate(subs):
+            out[f"spin{i + 1}"] = SolidSys.from_text("\n".join(glob + _flatten(sec)))
+        dims = {sy.N for sy in out.values()}
+        if len(dims) > 1:
+            raise BlochLibError("the spin systems of spins{} differ in Hilbert dimension")
+        return out
+
+    def _powders(self, par: Section) -> dict:
+        """MultiPowder::parse (multipowder.cc:56-87): subsections powder1{}, powder2{} ... or
+        one powder{}."""
+        subs = self._numbered(par, "powder", "numpows")
+        if subs:
+            return {f"powder{i + 1}": self._powder(sec) for i, sec in enumerate(subs)}
+        one = par.sub("powder")
+        return {"powder": self._powder(one)} if one is not None else {}
 
     def _powder(self, sec: Section) -> Powder:
         kv = {}
@@ -767,7 +812,11 @@ class _Runner:
         self.precision = precision
         self.verbose = verbose
         self.subs = {it.name: it for it in sim.pulses.items if isinstance(it, Section)}
-        self.sys = sim.system.copy()
+        # the named spin systems (alterSys changes the current one in place) and the current
+        # system and powder names (sequencerun.cc:229 'default', spinsys() / powder())
+        self.systems = {k: v.copy() for k, v in sim.systems.items()}
+        self.usesys = "default"
+        self.usepow = "default"
         self.sub_cache = {}      # sub-sequence -> [segments of its last parse, _Prop or None]
         self.ptop = False
         self.is2D = False
@@ -794,6 +843,18 @@ class _Runner:
     def log(self, *a):
         if self.verbose:
             print("[sim]", *a, flush=True)
+
+    def system(self, use: str) -> SolidSys:
+        """MultiSystem::getSystem (multisystem.cc:165-181): 'default' is the first system."""
+        if use == "default":
+            return self.systems[min(self.systems)]
+        if use not in self.systems:
+            raise BlochLibError(f"SoildSys '{use}' does not exsist")
+        return self.systems[use]
+
+    @property
+    def sys(self) -> SolidSys:
+        return self.system(self.usesys)
 
     def run(self):
         lines = [it for it in self.sim.pulses.items if isinstance(it, str)]
@@ -826,10 +887,15 @@ class _Runner:
             if not inner:
                 raise BlochLibError(f" Bad 'offset' usage for \"{s}\"")
             self.offset = evaluate(_split_args(inner)[0], self.p)
-        elif "spinsys(" in s or "powder(" in s:
-            if _inside(w) not in ("", "default"):
-                raise BlochLibError("several spin systems / powders are not supported by this "
-                                    "engine")
+        elif "spinsys(" in s:
+            # setSystem (sequencerun_funcs.cc:179-193): later pulses and fid() use it
+            if not _inside(w):
+                raise BlochLibError(f" Bad 'spinsys' usage for \"{s}\"\n should be "
+                                    "\"spinsys(sys)\" ")
+            self.system(_inside(w))
+            self.usesys = _inside(w)
+        elif "powder(" in s:
+            self.set_powder(w, s)
         elif "cycler(" in s:
             raise BlochLibError("cycler() is not supported by this engine")
         elif "dumpState()" in s:
@@ -958,14 +1024,17 @@ class _Runner:
 
         run_script(sec.lines(), record, self.p)
         segs = []
-        t, rot, amp, off = 0.0, MAGIC_RAD, 0.0, 0.0
+        t, rot, amp, off, use = 0.0, MAGIC_RAD, 0.0, 0.0, "default"
         for w in rec:
             if "rotor(" in w:
                 rot = evaluate(_inside(w), self.p) * DEG2RAD
-            elif "spinsys(" in w or "powder(" in w:
-                if _inside(w) not in ("", "default"):
-                    raise BlochLibError("several spin systems / powders are not supported by "
-                                        "this engine")
+            elif "spinsys(" in w:
+                use = _inside(w) or "default"     # readPulses (pulsedata.cc:313-317)
+                self.system(use)
+            elif "powder(" in w:
+                if (_inside(w) or "default") not in ("default", self.usepow):
+                    raise BlochLibError("a powder() switch inside a sub-sequence (orientations "
+                                        "of another powder per pulse) is not supported")
             elif "amplitude(" in w:
                 amp = evaluate(_inside(w), self.p)
             elif "offset(" in w:
@@ -977,9 +1046,22 @@ class _Runner:
                 self.p[lhs] = evaluate(rhs, self.p)
             elif "pulse(" in w or "delay(" in w:
                 sg = self.segment(w, t, rot, amp, off)
+                sg.sys = use
                 segs.append(sg)
                 t = sg.t2
         return segs
+
+    def seq_props(self, segs) -> list:
+        """A sub-sequence's propagators: one per run of consecutive pulses on the same spin
+        system, each with that system (alterSys changes applied so far included)."""
+        out = []
+        for sg in segs:
+            if out and out[-1][0] == sg.sys:
+                out[-1][1].append(sg)
+            else:
+                out.append((sg.sys, [sg]))
+        return [_Prop(self.split_steps(g), self.system(u).copy(), float(self.p["wr"]))
+                for u, g in out]
 
     def use(self, w: str):
         """showUsed (sequencerun_funcs.cc:465-551): use(seq[, repeat[, hold]]) parses the
@@ -1006,9 +1088,18 @@ class _Runner:
             self.sub_cache[args[0]] = ent
         if repeat > 0 and self.to_run:
             if ent[1] is None:
-                ent[1] = _Prop(self.split_steps(ent[0]), self.sys.copy(), float(self.p["wr"]))
-            pr = ent[1]
-            self.proplist.append(_Prop(pr.steps, pr.sys, pr.wr, repeat, held))
+                ent[1] = self.seq_props(ent[0])
+            props = ent[1]
+            if len(props) == 1:
+                pr = props[0]
+                self.proplist.append(_Prop(pr.steps, pr.sys, pr.wr, repeat, held))
+            elif held:
+                raise BlochLibError("a held sub-sequence whose pulses use several spin systems "
+                                    "is not supported")
+            else:
+                # the whole sequence `repeat` times, its parts in order
+                for _ in range(repeat):
+                    self.proplist.extend(_Prop(pr.steps, pr.sys, pr.wr) for pr in props)
         if not held and ent[0]:
             self.current_time = ent[0][-1].t2 * repeat
 
@@ -1081,9 +1172,34 @@ class _Runner:
         return self.state
 
     def powder(self) -> Powder:
-        if self.sim.powder is None:
+        if not self.sim.powders:
             raise BlochLibError("no powder{} section in parameters")
-        return self.sim.powder
+        if self.usepow == "default":
+            return self.sim.powders[min(self.sim.powders)]
+        if self.usepow not in self.sim.powders:
+            raise BlochLibError(f"Powder '{self.usepow}' does not exsist")
+        return self.sim.powders[self.usepow]
+
+    def set_powder(self, w: str, s: str):
+        """setPowder (sequencerun_funcs.cc:212-228): a different powder restarts the FID
+        accumulation.  The orientation loop of the reference restarts with it; here every
+        orientation of the program is a device state, so a switch is accepted only before any
+        pulse, propagator or fid() of the program."""
+        inner = _inside(w)
+        if not inner:
+            raise BlochLibError(f" Bad 'powder' usage for \"{s}\"\n should be \"powder(pow)\" ")
+        if inner == self.usepow:
+            return
+        if self.proplist or self.ops or self.state is not None or np.any(self.fid1):
+            raise BlochLibError("powder() after pulses or fid() (a second powder within one "
+                                "program) is not supported")
+        old, self.usepow = self.usepow, inner
+        try:
+            self.powder()
+        except BlochLibError:
+            self.usepow = old
+            raise
+        self.fid1[:] = 0
 
     def det(self):
         return self.sys.op(str(self.p["detect"]))

@@ -327,7 +327,7 @@ def fake_engine(monkeypatch):
     def fid_stub(kind):
         def f(sy, wr, rotor, *a, **kw):
             npts = a[1] if kind in ("powder_fid", "state_fid") else a[1]
-            calls.append((kind, dict(wr=wr, rotor=rotor, t0=kw.get("t0"), args=a)))
+            calls.append((kind, dict(wr=wr, rotor=rotor, t0=kw.get("t0"), args=a, sys=sy)))
             n = a[1] if kind in ("powder_fid", "state_fid") else a[1]
             out = np.full(n, len(calls), dtype=complex)
             return (out, 40000.0) if kind in ("powder_fid", "state_fid") else out
@@ -488,3 +488,119 @@ def test_cli_reports_errors(tmp_path, capsys):
     p.write_text(C7.replace("reuse(sub1)", "reuse(nosub)"))
     assert solid.main([str(p)]) == 1
     assert "does NOT exsist" in capsys.readouterr().err
+
+
+MULTI = """
+spins{
+    numspin 2
+    T 1H 0
+    T 1H 1
+    C 1000 2000 0 0
+    spin1{
+        D -3000 0 1
+    }
+    spin2{
+        D -6000 0 1
+        J 50 0 1
+    }
+}
+parameters{
+    powder1{
+        aveType ../crystals/rep2000
+    }
+    powder2{
+        aveType zcw
+        thetaStep 233
+        phiStep 144
+    }
+    npts1D=64
+    sw=40000
+    roeq= Iz
+}
+pulses{
+    sub1{
+        amplitude(100000)
+        1H:pulse(2.5e-6, 0)
+        spinsys(spin2)
+        1H:delay(10e-6)
+    }
+    wr=0
+    rotor=0
+    PROGRAM
+}
+"""
+
+
+def multi(program):
+    return SimFile(MULTI.replace("PROGRAM", program))
+
+
+def test_spin_systems_and_powders_are_named_like_multisystem():
+    """MultiSystem / MultiPowder: spin1{}, spin2{} extend the global lines; powder1{},
+    powder2{} are named powders; 'default' is the first of each; without subsections the
+    global lines are 'spin1'."""
+    sim = multi("fid()")
+    assert sorted(sim.systems) == ["spin1", "spin2"] and sorted(sim.powders) == ["powder1", "powder2"]
+    s1, s2 = sim.systems["spin1"], sim.systems["spin2"]
+    assert len(s1.csa) == len(s2.csa) == 1 and len(s1.dip) == len(s2.dip) == 1
+    assert s1.dip[0][0] == -3000 and s2.dip[0][0] == -6000 and len(s2.jc) == 1 and not s1.jc
+    assert sim.system is s1 and sim.powder is sim.powders["powder1"]
+    one = SimFile(STATIC)
+    assert list(one.systems) == ["spin1"] and list(one.powders) == ["powder"]
+    bad = MULTI.replace("        J 50 0 1\n", "        T 1H 2\n        J 50 0 2\n")
+    with pytest.raises(BlochLibError):
+        SimFile(bad.replace("PROGRAM", "fid()"))
+
+
+def test_spinsys_switches_the_system_of_later_pulses_and_fids(monkeypatch, tmp_path):
+    """Main section: a pulse before spinsys(spin2) propagates with spin1, the fid() after it
+    uses spin2 (sequencerun_funcs.cc:101-134, 179-193, 594-595)."""
+    calls = fake_engine(monkeypatch)
+    sim = multi("""detect(Ip)
+    ro(Ix)
+    amplitude(50000)
+    1H:pulse(5e-6)
+    spinsys(spin2)
+    fid()""")
+    sim.run(outdir=str(tmp_path))
+    applies = [c for c in calls if c[0] == "apply"]
+    fids = [c for c in calls if c[0] in ("powder_fid", "state_fid")]
+    assert len(applies) == 1 and applies[0][1] == sim.systems["spin1"]
+    assert len(fids) == 1 and fids[0][1]["sys"] == sim.systems["spin2"]
+    with pytest.raises(BlochLibError):
+        multi("spinsys(spin3)\n    fid()").run(outdir=str(tmp_path))
+
+
+def test_spinsys_inside_a_subsequence_splits_its_propagator(monkeypatch, tmp_path):
+    """readPulses gives each pulse the system named before it in the sequence (pulsedata.cc:
+    313-317, 362): use(sub1, 2) applies [pulse on spin1, delay on spin2] twice, in order."""
+    calls = fake_engine(monkeypatch)
+    sim = multi("""detect(Ip)
+    ro(Ix)
+    use(sub1, 2)
+    fid()""")
+    sim.run(outdir=str(tmp_path))
+    applies = [c for c in calls if c[0] == "apply"]
+    systems = [c[1] for c in applies]
+    s1, s2 = sim.systems["spin1"], sim.systems["spin2"]
+    assert systems == [s1, s2, s1, s2]
+    assert all(c[7] == 1 for c in applies)
+    with pytest.raises(BlochLibError):          # held: one propagator per sequence
+        multi("use(sub1, 2, hold)\n    fid()").run(outdir=str(tmp_path))
+
+
+def test_powder_selects_a_named_powder_before_the_program_runs(monkeypatch, tmp_path):
+    """powder(powder2) before any pulse or fid() makes the FID use that powder; a switch after
+    a fid() (a second powder within one program) is refused."""
+    calls = fake_engine(monkeypatch)
+    sim = multi("""powder(powder2)
+    detect(Ip)
+    ro(Ix)
+    fid()""")
+    sim.run(outdir=str(tmp_path))
+    fids = [c for c in calls if c[0] in ("powder_fid", "state_fid")]
+    assert len(fids[0][1]["args"][2]) == len(sim.powders["powder2"]) != len(sim.powders["powder1"])
+    with pytest.raises(BlochLibError):
+        multi("fid()\n    powder(powder2)\n    fid()").run(outdir=str(tmp_path))
+    with pytest.raises(BlochLibError):
+        multi("powder(powder7)\n    fid()").run(outdir=str(tmp_path))

@@ -221,3 +221,39 @@ def test_ptop_2d_rows_and_indexed_points_with_hold(tmp_path):
     err = np.abs(res["fid"][:3] - np.array(pts)).max() / np.abs(pts).max()
     print(f"[sim2d] ptop fid(i) hold: rel {err:.3e}", flush=True)
     assert err < 1e-10 and res["sw"] == 1e6
+
+
+def test_two_spin_systems_against_oracle(tmp_path):
+    """MultiSystem: a sub-sequence whose pulse runs on spin1 and whose delay runs on spin2
+    (spinsys() inside it), used twice, then spinsys(spin2) and a MAS fid(); against the
+    oracle's state functions applied with each part's own system in the same order."""
+    from test_simfile import MULTI
+    text = MULTI.replace("PROGRAM", """wr=10000
+    rotor=54.7356
+    detect(Ip)
+    ro(Ix)
+    use(sub1, 2)
+    spinsys(spin2)
+    fid()""").replace("wr=0\n    rotor=0\n", "").replace("npts1D=64", "npts1D=48\n    maxtstep=5e-6")
+    text = text.replace("aveType ../crystals/rep2000", SMALL["zcw"])
+    print("[sim] two spin systems", flush=True)
+    sim = SimFile(text)
+    res = sim.run(outdir=str(tmp_path))
+    sim2 = SimFile(text)
+    r = _Runner(sim2, ".", "fp64", False)
+    for it in sim2.pulses.items:
+        if isinstance(it, str) and not it.replace(" ", "").startswith(("fid(", "savefid")):
+            r.statement(it)
+    s1, s2 = sim2.systems["spin1"], sim2.systems["spin2"]
+    assert [p.sys for p in r.proplist] == [s1, s2, s1, s2]
+    pw = sim2.powder
+    state = np.stack([s1.op("Ix")] * len(pw))
+    for pr in r.proplist:
+        state = O.state_apply(pr.sys, 10000.0, np.degrees(pr.steps[0][2]),
+                              [x[1] for x in pr.steps], pw, state, rf=_rf_tables(pr.steps),
+                              step_t1=[x[0] for x in pr.steps], repeat=pr.repeat)
+    ref, sw = O.state_fid(s2, 10000.0, 54.7356, 40000.0, 48, pw, state, s2.op("Ip"),
+                          t0=r.current_time)
+    err = np.linalg.norm(res["fid"] - ref) / np.linalg.norm(ref)
+    print(f"[sim] two spin systems: rel-L2 {err:.3e}", flush=True)
+    assert err < 1e-9 and res["sw"] == sw
