@@ -68,6 +68,8 @@ int blfid_dev_launch_prep(int K, const double *th, const double *ph, const doubl
 int blfid_dev_launch_static_h(const DevPlan *P, int Kb, int c0, const int *grp, int ngrp,
                               double *scratch, size_t slot, int nslots, double2 *bufH,
                               hipStream_t st);
+int blfid_dev_launch_state_project(long K, int N, const double2 *C, double inv_nrm, double2 *S,
+                                   hipStream_t st);
 int blfid_dev_launch_state_fill(long K, int N, const double2 *rho0, double2 *S, hipStream_t st);
 int blfid_dev_launch_state_apply(const DevPlan *PP, int Kb, int c0, const double2 *Aprep,
                                  const int *sblk, const int *sloc, int N, int repeat, double2 *S,
@@ -1790,6 +1792,21 @@ static blfid_status state_fill(blfid_state *st, const double *rho, int all, hipS
 
 blfid_status blfid_state_set(blfid_state *st, const double *rho, void *stream) {
     return state_fill(st, rho, 0, (hipStream_t)stream);
+}
+
+blfid_status blfid_state_project(blfid_state *st, const double *op, void *stream) {
+    if (!st || !op) return fail(BLFID_ERR_INVALID, "NULL argument");
+    const size_t NN = (size_t)st->N * st->N;
+    double nrm = 0.0;
+    for (size_t e = 0; e < NN; e++) nrm += op[2 * e] * op[2 * e] + op[2 * e + 1] * op[2 * e + 1];
+    if (!(nrm > 0.0)) return fail(BLFID_ERR_INVALID, "the cycler operator is zero");
+    hipStream_t s = (hipStream_t)stream;
+    HIPCHK(hipStreamSynchronize(s)); // the staging buffer may still feed earlier work
+    HIPCHK(st->rho0.alloc(NN));
+    HIPCHK(hipMemcpy(st->rho0.p, op, NN * sizeof(double2), hipMemcpyHostToDevice));
+    LAUNCHCHK(blfid_dev_launch_state_project(st->K, st->N, st->rho0.p, 1.0 / nrm, st->rho.p, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return BLFID_OK;
 }
 
 blfid_status blfid_state_load(blfid_state *st, const double *rho_all, void *stream) {

@@ -2669,6 +2669,29 @@ __global__ void __launch_bounds__(256) k_state_fill(long total, long NN, const c
     if (i < total) S[i] = rho0[i % NN];
 }
 
+// rho_c <- (Tr(rho_c C^H) / nrm) C for every crystallite c (nrm = Tr(C C^H), formed on the
+// host): one workgroup per crystallite at a time, the trace reduced in FP64 over the N x N
+// elements in a fixed order (per thread, then a tree over the workgroup).
+__global__ void __launch_bounds__(256) k_state_project(long K, long NN, const cd *__restrict__ C,
+                                                       double inv_nrm, cd *__restrict__ S) {
+    __shared__ cd red[256];
+    const int tid = threadIdx.x;
+    for (long c = blockIdx.x; c < K; c += gridDim.x) {
+        cd *R = S + c * NN;
+        cd acc = cmk(0, 0);
+        for (long e = tid; e < NN; e += 256) cfma(acc, R[e], cconj(C[e]));
+        red[tid] = acc;
+        __syncthreads();
+        for (int h = 128; h > 0; h >>= 1) {
+            if (tid < h) red[tid] = cadd(red[tid], red[tid + h]);
+            __syncthreads();
+        }
+        const cd t = cscale(red[0], inv_nrm);
+        for (long e = tid; e < NN; e += 256) R[e] = cmul(t, C[e]);
+        __syncthreads(); // red[0] read by every thread before the next crystallite's writes
+    }
+}
+
 // rho_c <- U rho_c U^H, `repeat` times, with U the crystallite's propagator from a pulse plan
 // (FP64, block diagonal in that plan's blocks, read from Aprep as k_rho_prep reads it).  One
 // workgroup per crystallite at a time; Tw is the workgroup's N x N slot: Tw = rho U^H, then
@@ -3102,6 +3125,14 @@ extern "C" int blfid_dev_launch_rho_prep(int prec, const DevPlan *P, const DevPl
     else
         k_rho_prep<double><<<grid, 256, 0, st>>>(*P, *PP, Kb, Aprep, P->elem_r, P->elem_s, sblk,
                                                   sloc, nzi, nzv, nnz, (cd *)rho_out);
+    return (int)hipGetLastError();
+}
+
+extern "C" int blfid_dev_launch_state_project(long K, int N, const cd *C, double inv_nrm, cd *S,
+                                              hipStream_t st) {
+    if (K <= 0) return 0;
+    const unsigned grid = (unsigned)(K < 4096 ? K : 4096);
+    k_state_project<<<grid, 256, 0, st>>>(K, (long)N * N, C, inv_nrm, S);
     return (int)hipGetLastError();
 }
 

@@ -118,6 +118,7 @@ def lib():
                                         ctypes.c_double, ctypes.c_int, _d, _d, ctypes.c_int,
                                         ctypes.POINTER(_RF), ctypes.c_int, vp]
         L.blfid_state_trace.argtypes = [vp, _d, ctypes.c_int, _d, vp]
+        L.blfid_state_project.argtypes = [vp, _d, vp]
         L.blfid_plan_set_state.argtypes = [vp, vp]
         L.blfid_plan_graph_capture.argtypes = [vp, ctypes.c_int]
         L.blfid_plan_graph_launch.argtypes = [vp, vp]
@@ -507,6 +508,11 @@ class State:
         _check(lib().blfid_state_apply(self.h, ctypes.byref(s), float(wr), float(rotor_deg), n,
                                        None if t1 is None else _p(t1), _p(dt), len(chans), arr,
                                        int(repeat), None))
+
+    def project(self, op):
+        """rho_k <- Tr(rho_k op^H) op / Tr(op op^H) for every crystallite (cycler(op))."""
+        o = _cm(op, self.N)
+        _check(lib().blfid_state_project(self.h, _p(o), None))
 
     def trace(self, det, normalize: bool = True) -> complex:
         """sum_k w_k Tr(rho_k det) [/ Tr(det det^H)]."""

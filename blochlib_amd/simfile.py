@@ -24,7 +24,11 @@ Reference behaviour restated (examples/solid-2.0/src):
     system named before it (pulsedata.cc:313-317, 362);
   * every orientation's rho is a device state (blfid_state_*) that the used propagators advance
     and every fid() reads; a plain fid() is the gamma-COMPUTE (or static) FID from it.
-Not supported (BlochLibError names the statement): ``cycler()``, ``reset()``, quadrupolar
+A main-section ``cycler(op)`` is a propagator-list entry that projects every orientation's rho
+onto op where the list is applied (sequencerun.cc:164-178); inside a sub-sequence it has no
+effect, as the reference's propagator of a used sequence never applies it.
+Not supported (BlochLibError names the statement): ``reset()`` (inside the reference's
+orientation loop it discards the powder sum and disables every later save), quadrupolar
 interactions, a ``powder()`` switch after the program has pulsed or acquired, a powder switch
 inside a sub-sequence, and a held sub-sequence whose pulses use several spin systems.  The
 engine has no per-crystallite CPU path to fall back on.
@@ -776,6 +780,8 @@ class _Prop:
     wr: float
     repeat: int = 1
     hold: bool = False
+    # a cycler(op) entry instead: no steps, rho <- Tr(rho op^H) op / Tr(op op^H)
+    proj: np.ndarray | None = field(default=None, compare=False)
 
 
 def _rf_tables(st):
@@ -897,7 +903,15 @@ class _Runner:
         elif "powder(" in s:
             self.set_powder(w, s)
         elif "cycler(" in s:
-            raise BlochLibError("cycler() is not supported by this engine")
+            # doCycler (sequencerun_funcs.cc:232-256): a propagator-list entry that projects
+            # ro onto the operator where the list is applied (sequencerun.cc:164-178)
+            inner = _inside(w)
+            if not inner:
+                raise BlochLibError(f" Bad 'cycler' usage for \"{s}\"\n should be "
+                                    "\"cycler(SpinOp)\" ")
+            if self.to_run and inner not in ("Fe", "Ie"):
+                self.proplist.append(_Prop([], self.sys.copy(), float(self.p["wr"]),
+                                           proj=self.sys.op(inner)))
         elif "dumpState()" in s:
             return
         elif _is_assignment(s):
@@ -929,7 +943,11 @@ class _Runner:
         elif "use(" in s:
             self.use(w)
         elif "reset(" in s:
-            raise BlochLibError("reset() is not supported by this engine")
+            # ResetPars (sequencerun_funcs.cc:349-362) runs inside the orientation loop: it
+            # zeroes the accumulated powder FID and clears canSave, so the reference keeps one
+            # orientation's data and saves nothing afterwards
+            raise BlochLibError("reset() is not supported by this engine (the reference's "
+                                "reset() discards the powder sum and disables later saves)")
         elif "alterSys(" in s:
             args = _split_args(_inside(w))
             if not _inside(w) or len(args) < 2:
@@ -1040,7 +1058,9 @@ class _Runner:
             elif "offset(" in w:
                 off = evaluate(_inside(w), self.p)
             elif "cycler(" in w:
-                raise BlochLibError("cycler() is not supported by this engine")
+                # recorded on the next pulse (pulsedata.cc:337-340, 364), but the propagator
+                # of a used sequence (Propogation::propogator) never applies it: no effect
+                continue
             elif "=" in w and w.find("==") != w.find("=") and w.find("!=") != w.find("=") + 1:
                 lhs, rhs = w.split("=", 1)
                 self.p[lhs] = evaluate(rhs, self.p)
@@ -1115,8 +1135,9 @@ class _Runner:
         return not self.ops
 
     def queue(self, props, times: int = 1):
-        """Apply the propagators in order, the whole list `times` times (lazily)."""
-        props = [p for p in props if p.steps and p.repeat > 0]
+        """Apply the propagators (and cycler projections) in order, the whole list `times`
+        times (lazily)."""
+        props = [p for p in props if (p.steps or p.proj is not None) and p.repeat > 0]
         if not props or times <= 0:
             return
         if len(props) == 1:
@@ -1133,7 +1154,10 @@ class _Runner:
 
     def period(self, props):
         """The propagators applied once each, in order, as one _Prop (their repeats expanded),
-        or None when they differ in spin system, spinning speed or rotor angle."""
+        or None when they differ in spin system, spinning speed or rotor angle or include a
+        cycler projection."""
+        if any(p.proj is not None for p in props):
+            return None
         props = [p for p in props if p.steps and p.repeat > 0]
         if not props:
             return None
@@ -1157,6 +1181,9 @@ class _Runner:
             self.has_base = True
             self.synced = 0
         for pr, reps in self.ops[self.synced:]:
+            if pr.proj is not None:
+                self.state.project(pr.proj)      # idempotent: once for any repeat count
+                continue
             groups = []
             for x in pr.steps:
                 if groups and groups[-1][0] == x[2]:
@@ -1231,7 +1258,7 @@ class _Runner:
         out = np.zeros(npts, dtype=np.complex128)
         if npts <= 0:
             return out
-        live = [p for p in props if p.steps and p.repeat > 0]
+        live = [p for p in props if (p.steps or p.proj is not None) and p.repeat > 0]
         per = self.period(live)
         if not live:
             out[:] = self.trace()
@@ -1307,6 +1334,8 @@ class _Runner:
             raise BlochLibError("fid() without an index after the first 2-D row with "
                                 "propagators before it is not supported (the reference "
                                 "applies a propagator it has not computed)")
+        else:
+            self.queue([p for p in self.proplist if p.proj is not None], 1)
         row = idx if idx != -1 else self.pt2D
         if row >= self.fid2.shape[0]:
             raise BlochLibError(f"2-D row {row} beyond npts2D = {self.fid2.shape[0]}")

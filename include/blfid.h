@@ -171,6 +171,11 @@ blfid_status blfid_state_apply(blfid_state *state, const blfid_system *sys, doub
  * weight trace(ro, detect) / trace(detect, adjoint(detect))). */
 blfid_status blfid_state_trace(blfid_state *state, const double *det, int normalize,
                                double *out2, void *stream);
+/* rho_k <- Tr(rho_k op^H) op / Tr(op op^H) for every crystallite: the main-section cycler(op)
+ * entry of the propagator list, which projects ro onto op where the list is applied
+ * (subSequence::propogate with docycler, sequencerun.cc:164-178; doCycler,
+ * sequencerun_funcs.cc:232-256).  op: N x N complex128, column-major, not zero. */
+blfid_status blfid_state_project(blfid_state *state, const double *op, void *stream);
 /* The plan's FID starts from the state's rho_k for crystallite k of the plan's powder (same K;
  * the same orientations are the caller's to keep) instead of the plan's rho: a gamma-COMPUTE,
  * static or pulse-program FID from each crystallite's current rho (sequencerun_funcs.cc:

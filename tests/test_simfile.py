@@ -166,7 +166,7 @@ def test_main_section_pulses_in_ptop_mode_are_timed_by_current_time():
     assert (iso, a, iso2, a2) == ("1H", 50000.0, "13C", 0.0) and math.isclose(ph, math.pi / 2)
 
 
-@pytest.mark.parametrize("stmt", ["reset()", "cycler(Ix)", "spinsys(other)", "powder(p2)",
+@pytest.mark.parametrize("stmt", ["reset()", "cycler()", "spinsys(other)", "powder(p2)",
                                   "alterSys(Q1del, 3)", "alterSys(C7iso, 3)", "use(nosub)",
                                   "use(sub1, -1)", "1H:pulse(1e-6):x"])
 def test_unsupported_or_bad_statements_raise(stmt):
@@ -320,6 +320,9 @@ def fake_engine(monkeypatch):
         def trace(self, det, normalize=True):
             calls.append(("trace",))
             return 1.0 + 0j
+
+        def project(self, op):
+            calls.append(("project", np.array(op)))
 
         def close(self):
             pass
@@ -604,3 +607,26 @@ def test_powder_selects_a_named_powder_before_the_program_runs(monkeypatch, tmp_
         multi("fid()\n    powder(powder2)\n    fid()").run(outdir=str(tmp_path))
     with pytest.raises(BlochLibError):
         multi("powder(powder7)\n    fid()").run(outdir=str(tmp_path))
+
+
+def test_cycler_projects_rho_where_the_list_is_applied(monkeypatch, tmp_path):
+    """Main-section cycler(op) is a propagator-list entry (doCycler, sequencerun_funcs.cc:
+    232-256): the fid() after it applies the pulse, then the projection onto op; cycler(Ie)
+    adds nothing; a cycler line inside a sub-sequence has no effect."""
+    calls = fake_engine(monkeypatch)
+    sim = multi("""detect(Ip)
+    ro(Ix)
+    use(sub1)
+    cycler(Iz)
+    cycler(Ie)
+    fid()""".replace("use(sub1)", "amplitude(50000)\n    1H:pulse(5e-6)"))
+    sim.run(outdir=str(tmp_path))
+    kinds = [c[0] for c in calls]
+    assert kinds == ["create", "set", "apply", "project", "state_fid"], kinds
+    proj = [c[1] for c in calls if c[0] == "project"][0]
+    assert np.array_equal(proj, sim.system.op("Iz"))
+    calls.clear()
+    sub = MULTI.replace("        spinsys(spin2)\n", "        cycler(Iz)\n")
+    sim = SimFile(sub.replace("PROGRAM", "detect(Ip)\n    ro(Ix)\n    use(sub1)\n    fid()"))
+    sim.run(outdir=str(tmp_path))
+    assert "project" not in [c[0] for c in calls]

@@ -257,3 +257,52 @@ def test_two_spin_systems_against_oracle(tmp_path):
     err = np.linalg.norm(res["fid"] - ref) / np.linalg.norm(ref)
     print(f"[sim] two spin systems: rel-L2 {err:.3e}", flush=True)
     assert err < 1e-9 and res["sw"] == sw
+
+
+def test_cycler_between_pulses_against_oracle(tmp_path):
+    """MAS: pulse, delay, cycler(op), pulse, fid(); against the oracle's state functions with
+    the projection rho <- Tr(rho op^H) op / Tr(op op^H) (subSequence::propogate,
+    sequencerun.cc:164-178) written in numpy between them.  The delay dephases each
+    orientation differently, so every crystallite keeps its own projection weight."""
+    text = STATIC.replace("npts1D=512", "npts1D=32").replace("sw=40000", "sw=80000")
+    text = text.replace("aveType ../crystals/rep2000", SMALL["zcw"])
+    text = text.replace("""    wr=0
+    rotor=0
+    detect(Ip)
+    ro(Ix)
+    fid()""", """    wr=10000
+    rotor=54.7356
+    detect(Ip)
+    ro(Iz)
+    amplitude(50000)
+    1H:pulse(5e-6)
+    1H:delay(12e-6)
+    cycler(Ix_0+Iy_1)
+    1H:pulse(2e-6, 90)
+    fid()""")
+    print("[sim] cycler", flush=True)
+    sim = SimFile(text)
+    res = sim.run(outdir=str(tmp_path))
+    sim2 = SimFile(text)
+    r = _Runner(sim2, ".", "fp64", False)
+    for it in sim2.pulses.items:
+        if isinstance(it, str) and not it.replace(" ", "").startswith(("fid(", "savefid")):
+            r.statement(it)
+    sy, pw = sim2.system, sim2.powder
+    assert [p.proj is not None for p in r.proplist] == [False, False, True, False]
+    op = sy.op("Ix_0+Iy_1")
+    opH = op.conj().T
+    state = np.stack([sy.op("Iz")] * len(pw))
+    for pr in r.proplist:
+        if pr.proj is not None:
+            t = np.einsum("kab,ba->k", state, opH) / np.trace(op @ opH)
+            state = t[:, None, None] * op[None]
+            continue
+        state = O.state_apply(sy, 10000.0, np.degrees(pr.steps[0][2]), [x[1] for x in pr.steps],
+                              pw, state, rf=_rf_tables(pr.steps),
+                              step_t1=[x[0] for x in pr.steps], repeat=pr.repeat)
+    ref, sw = O.state_fid(sy, 10000.0, 54.7356, 80000.0, 32, pw, state, sy.op("Ip"),
+                          t0=r.current_time)
+    err = np.linalg.norm(res["fid"] - ref) / np.linalg.norm(ref)
+    print(f"[sim] cycler: rel-L2 {err:.3e}", flush=True)
+    assert err < 1e-9 and res["sw"] == sw

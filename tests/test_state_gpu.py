@@ -114,6 +114,41 @@ def test_trace_matches_numpy():
     assert abs(got - want / np.trace(det @ det.conj().T)) < 1e-12 * abs(got)
 
 
+def project_numpy(rho, op):
+    """cycler(op): rho_k <- Tr(rho_k op^H) op / Tr(op op^H) (subSequence::propogate,
+    sequencerun.cc:164-178), written with numpy's matrix trace."""
+    opH = op.conj().T
+    t = np.einsum("kab,ba->k", rho, opH) / np.trace(op @ opH)
+    return t[:, None, None] * op[None]
+
+
+@pytest.mark.parametrize("K,expr", [(51, "Ip_0+Iz_2"), (5000, "Ix_1")])
+def test_project_matches_numpy(K, expr):
+    """A non-Hermitian operator checks the conjugation; K=5000 exceeds the launch's 4096
+    workgroups, so the crystallite loop runs twice on some of them."""
+    sy, pw, *_ = case3()
+    rng = np.random.default_rng(11)
+    if K != len(pw):
+        pw = Powder(rng.uniform(0, np.pi, K), rng.uniform(0, 2 * np.pi, K),
+                    rng.uniform(0, 2 * np.pi, K), np.full(K, 1.0 / K), "random")
+    rho = rand_herm_stack(rng, K, sy.N)
+    op = sy.op(expr)
+    print(f"[state] project K={K} op={expr}", flush=True)
+    st = engine.State(sy.spins, pw)
+    try:
+        st.load(rho)
+        st.project(op)
+        got = st.fetch()
+        with pytest.raises(engine.EngineError):
+            st.project(np.zeros_like(op))
+    finally:
+        st.close()
+    want = project_numpy(rho, op)
+    assert rel(got, want) < 1e-13
+    # a projection is idempotent: the projected state is a multiple of op
+    assert rel(project_numpy(got, op), got) < 1e-13
+
+
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("c64", 1e-5)])
 def test_gamma_compute_fid_from_the_state(precision, tol):
     """Random Hermitian rho per crystallite: every block pair D couples is populated."""
