@@ -803,7 +803,9 @@ __device__ __forceinline__ void chain_mfma(int b, int n, const cf *__restrict__ 
                                            const cf *__restrict__ W, cf *__restrict__ C0,
                                            size_t cstride, int ckeep, float *smem) {
     constexpr int TM = 16 * M, LDK = TM + LDS_PAD, NW = NT / 64, NSUB = (M * M + NW - 1) / NW;
-    constexpr int PER = TM * TM / NT; // operand elements staged per thread
+    // operand elements staged per thread (the last round partial when NT does not divide TM^2)
+    constexpr int PER = (TM * TM + NT - 1) / NT;
+    constexpr bool EVEN = PER * NT == TM * TM;
     float *Ar = smem, *Ai = Ar + TM * LDK, *Br = Ai + TM * LDK, *Bi = Br + TM * LDK;
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63, g = lane >> 4, r = lane & 15;
     const int kp = (b + 15) & ~15;
@@ -829,7 +831,8 @@ __device__ __forceinline__ void chain_mfma(int b, int n, const cf *__restrict__ 
 #pragma unroll
         for (int u = 0; u < PER; u++) {
             const int i = at_i(u), q = at_q(u);
-            pre[u] = (i < b && q < b) ? Xk[i + q * b] : make_float2(0.0f, 0.0f);
+            const bool in = EVEN || tid + u * NT < TM * TM;
+            pre[u] = (in && i < b && q < b) ? Xk[i + q * b] : make_float2(0.0f, 0.0f);
         }
     };
     const int k0 = DYSON ? 1 : 0;
@@ -837,6 +840,7 @@ __device__ __forceinline__ void chain_mfma(int b, int n, const cf *__restrict__ 
     for (int k = k0; k < n; k++) {
 #pragma unroll
         for (int u = 0; u < PER; u++) {
+            if (!EVEN && tid + u * NT >= TM * TM) continue;
             const int i = at_i(u), q = at_q(u);
             Ar[i * LDK + q] = pre[u].x;
             Ai[i * LDK + q] = pre[u].y;
@@ -1726,13 +1730,19 @@ __global__ void __launch_bounds__(512) k_eig_blk(DevPlan P, int Kb, const int *_
 
 // ------------------------------------------------------------------------------ k_bmul
 // B_k = A_k W for every (crystallite, step, block) of a group; writes into bufB.
+// The complex64 chain of 65-80-state blocks runs eight waves (two per SIMD; its LDS allows one
+// workgroup per CU): at four waves it needed 432 registers and ran one wave per SIMD, and eight
+// waves took 18% off its time on 8-spin clusters (class 4 measured the same either way).
+template <int M, typename R, bool SPLIT> struct BmThreads {
+    static constexpr int v = (M == 5 && std::is_same<R, float>::value && !SPLIT) ? 512 : NThr<M>::v;
+};
 template <int M, typename R, bool SPLIT = false>
-__global__ void __launch_bounds__(NThr<M>::v, (M == 3 && sizeof(R) == 4) ? 3 : (M == 4 && sizeof(R) == 4) ? 2 : 1) k_bmul(DevPlan P, int Kb, const int *__restrict__ grp,
+__global__ void __launch_bounds__((BmThreads<M, R, SPLIT>::v), (BmThreads<M, R, SPLIT>::v == 512) ? 1 : (M == 3 && sizeof(R) == 4) ? 3 : (M == 4 && sizeof(R) == 4) ? 2 : 1) k_bmul(DevPlan P, int Kb, const int *__restrict__ grp,
                                                      int ngrp,
                                                      const typename CX<R>::T *__restrict__ bufA,
                                                      const typename CX<R>::T *__restrict__ bufW,
                                                      typename CX<R>::T *__restrict__ bufB) {
-    constexpr int NT = NThr<M>::v;
+    constexpr int NT = BmThreads<M, R, SPLIT>::v;
     typedef typename CX<R>::T CT;
     // complex64: one task per (crystallite, block) walks all n steps with W resident in LDS;
     // complex128 (and the BF16x3 form): one task per (crystallite, step, block)
@@ -1942,14 +1952,20 @@ __device__ __forceinline__ bool qr_chain_mfma(int bI, int bJ, int n, const int *
 // Qs = Q(r,s), Rs = R(s,r).
 // FLIP (complex64, tile classes 1-4; a variant the launch autotuner may choose): the other of
 // the two forms -- the per-step form for classes 1-2, the step-walking form for 3-4.
+// The complex64 per-step products of tile classes 4-5 (blocks of 49-80 states) run eight waves,
+// two workgroups per CU (class 4: 79 registers, 19% faster than four waves; class 5: 117
+// registers, 3% faster); class 3 measured 13% slower at eight waves and keeps four.
+template <int M, typename R, bool SPLIT, bool FLIP = false> struct QrThreads {
+    static constexpr int v = (M >= 4 && !FLIP && std::is_same<R, float>::value && !SPLIT) ? 512 : NThr<M>::v;
+};
 template <int M, typename R, bool SPLIT = false, bool FLIP = false>
-__global__ void __launch_bounds__(NThr<M>::v, (M == 5 && sizeof(R) == 4) ? 3 : 1) k_qr(DevPlan P, int Kb, const int *__restrict__ pgrp,
+__global__ void __launch_bounds__((QrThreads<M, R, SPLIT, FLIP>::v), (QrThreads<M, R, SPLIT, FLIP>::v == 512) ? 2 : (M == 5 && sizeof(R) == 4) ? 3 : 1) k_qr(DevPlan P, int Kb, const int *__restrict__ pgrp,
                                                    int ngrp,
                                                    const typename CX<R>::T *__restrict__ bufB,
                                                    typename CX<R>::T *__restrict__ scratch,
                                                    size_t slot, typename CX<R>::T *__restrict__ Qt,
                                                    typename CX<R>::T *__restrict__ Rt) {
-    constexpr int NT = NThr<M>::v;
+    constexpr int NT = QrThreads<M, R, SPLIT, FLIP>::v;
     typedef typename CX<R>::T CT;
     // complex64 pairs of small blocks with R formed from Q: one task per (crystallite, pair)
     // walks all steps (qr_chain_mfma); otherwise one task per (crystallite, step, pair)
@@ -2869,6 +2885,15 @@ __global__ void __launch_bounds__(NThr<M>::v) k_static_fid(DevPlan P, int Kb, co
         else                                                                                     \
             kern<M_, R_, false><<<grid_, nt_, 0, st_>>>(__VA_ARGS__);                            \
     } while (0)
+// As LAUNCH_SPLIT, with the workgroup size of each variant from the kernel's thread traits TH.
+#define LAUNCH_SPLIT_TH(kern, TH, M_, R_, grid_, st_, ...)                                       \
+    do {                                                                                         \
+        constexpr bool SP_ = std::is_same<R_, float>::value;                                     \
+        if (SP_ && P->bf16x3)                                                                    \
+            kern<M_, R_, SP_><<<grid_, TH<M_, R_, SP_>::v, 0, st_>>>(__VA_ARGS__);               \
+        else                                                                                     \
+            kern<M_, R_, false><<<grid_, TH<M_, R_, false>::v, 0, st_>>>(__VA_ARGS__);           \
+    } while (0)
 // prec: 0 = FP64 storage (double2), 1 = complex64 storage (float2) of U, A, B, Q, R and the exp
 // scratch; the eigen-solver, phases, FIDs and reductions are FP64 in both.
 
@@ -2986,10 +3011,10 @@ static void launch_bmul(int M, const DevPlan *P, int Kb, const int *grp, int ngr
     CT *B = (CT *)bufB;
     switch (M) {
     case 1: LAUNCH_SPLIT(k_bmul, 1, R, grid, 64, st, *P, Kb, grp, ngrp, A, W, B); break;
-    case 2: LAUNCH_SPLIT(k_bmul, 2, R, grid, 256, st, *P, Kb, grp, ngrp, A, W, B); break;
-    case 3: LAUNCH_SPLIT(k_bmul, 3, R, grid, 256, st, *P, Kb, grp, ngrp, A, W, B); break;
-    case 4: LAUNCH_SPLIT(k_bmul, 4, R, grid, 256, st, *P, Kb, grp, ngrp, A, W, B); break;
-    default: LAUNCH_SPLIT(k_bmul, 5, R, grid, 256, st, *P, Kb, grp, ngrp, A, W, B); break;
+    case 2: LAUNCH_SPLIT_TH(k_bmul, BmThreads, 2, R, grid, st, *P, Kb, grp, ngrp, A, W, B); break;
+    case 3: LAUNCH_SPLIT_TH(k_bmul, BmThreads, 3, R, grid, st, *P, Kb, grp, ngrp, A, W, B); break;
+    case 4: LAUNCH_SPLIT_TH(k_bmul, BmThreads, 4, R, grid, st, *P, Kb, grp, ngrp, A, W, B); break;
+    default: LAUNCH_SPLIT_TH(k_bmul, BmThreads, 5, R, grid, st, *P, Kb, grp, ngrp, A, W, B); break;
     }
 }
 
@@ -3026,10 +3051,10 @@ static void launch_qr(int M, const DevPlan *P, int Kb, const int *pgrp, int ngrp
     }
     switch (M) {
     case 1: LAUNCH_SPLIT(k_qr, 1, R, grid, 64, st, *P, Kb, pgrp, ngrp, B, sc, slot, Q, Rr); break;
-    case 2: LAUNCH_SPLIT(k_qr, 2, R, grid, 256, st, *P, Kb, pgrp, ngrp, B, sc, slot, Q, Rr); break;
-    case 3: LAUNCH_SPLIT(k_qr, 3, R, grid, 256, st, *P, Kb, pgrp, ngrp, B, sc, slot, Q, Rr); break;
-    case 4: LAUNCH_SPLIT(k_qr, 4, R, grid, 256, st, *P, Kb, pgrp, ngrp, B, sc, slot, Q, Rr); break;
-    default: LAUNCH_SPLIT(k_qr, 5, R, grid, 256, st, *P, Kb, pgrp, ngrp, B, sc, slot, Q, Rr); break;
+    case 2: LAUNCH_SPLIT_TH(k_qr, QrThreads, 2, R, grid, st, *P, Kb, pgrp, ngrp, B, sc, slot, Q, Rr); break;
+    case 3: LAUNCH_SPLIT_TH(k_qr, QrThreads, 3, R, grid, st, *P, Kb, pgrp, ngrp, B, sc, slot, Q, Rr); break;
+    case 4: LAUNCH_SPLIT_TH(k_qr, QrThreads, 4, R, grid, st, *P, Kb, pgrp, ngrp, B, sc, slot, Q, Rr); break;
+    default: LAUNCH_SPLIT_TH(k_qr, QrThreads, 5, R, grid, st, *P, Kb, pgrp, ngrp, B, sc, slot, Q, Rr); break;
     }
 }
 
