@@ -56,7 +56,7 @@ J 50 0 1 1
           zcw(1154).subset(np.arange(0, 1154, 97)), sy.op(rho_name), sy.op("Ip"), normalize=False)
 
 
-@pytest.mark.parametrize("S", [4, 6, 8])
+@pytest.mark.parametrize("S", [4, 6, 7, 8])
 def test_config2_clusters_full_points_c64(S):
     """Config-2 clusters (MAS 15 kHz, 128 steps, 1024 points) on a few ZCW 3722 crystallites."""
     w = next(x for x in workloads.config2() if x.meta["spins"] == S)
