@@ -1128,10 +1128,11 @@ __device__ double xd_build_h(const DevPlan &P, const double *RPc, const double *
 // Replaces k_exp + k_dyson
 // for these blocks (matexp.h:239-252, compute.h:257-274).  Blocks of 49-80 states run eight
 // waves (two per SIMD; the LDS allows one workgroup per CU).  Tile class 3 is held to three
-// workgroups per CU (167 registers instead of 195, no spills; 23% faster than two).
+// workgroups per CU (167 registers instead of 195, no spills; 23% faster than two) and class 2
+// to five (96 registers, 7 of them spilled to scratch; still 11% faster than four).
 template <int M> struct XdThreads { static constexpr int v = M >= 4 ? 512 : NThr<M>::v; };
 template <int M>
-__global__ void __launch_bounds__(XdThreads<M>::v, M == 3 ? 3 : 1) k_expdyson(DevPlan P, int Kb, int c0,
+__global__ void __launch_bounds__(XdThreads<M>::v, M == 3 ? 3 : M == 2 ? 5 : 1) k_expdyson(DevPlan P, int Kb, int c0,
                                                               const int *__restrict__ grp,
                                                               int ngrp, cf *__restrict__ bufA) {
     constexpr int NT = XdThreads<M>::v;
