@@ -1956,12 +1956,13 @@ __device__ __forceinline__ bool qr_chain_mfma(int bI, int bJ, int n, const int *
 // the two forms -- the per-step form for classes 1-2, the step-walking form for 3-4.
 // The complex64 per-step products of tile classes 4-5 (blocks of 49-80 states) run eight waves,
 // two workgroups per CU (class 4: 79 registers, 19% faster than four waves; class 5: 117
-// registers, 3% faster); class 3 measured 13% slower at eight waves and keeps four.
+// registers, 3% faster); class 3 measured 13% slower at eight waves and keeps four, held to five
+// workgroups per CU (90 registers, no spills; 11% faster than four).
 template <int M, typename R, bool SPLIT, bool FLIP = false> struct QrThreads {
     static constexpr int v = (M >= 4 && !FLIP && std::is_same<R, float>::value && !SPLIT) ? 512 : NThr<M>::v;
 };
 template <int M, typename R, bool SPLIT = false, bool FLIP = false>
-__global__ void __launch_bounds__((QrThreads<M, R, SPLIT, FLIP>::v), (QrThreads<M, R, SPLIT, FLIP>::v == 512) ? 2 : (M == 5 && sizeof(R) == 4) ? 3 : 1) k_qr(DevPlan P, int Kb, const int *__restrict__ pgrp,
+__global__ void __launch_bounds__((QrThreads<M, R, SPLIT, FLIP>::v), (QrThreads<M, R, SPLIT, FLIP>::v == 512) ? 2 : (M == 5 && sizeof(R) == 4) ? 3 : (M == 3 && sizeof(R) == 4 && !FLIP && !SPLIT) ? 5 : 1) k_qr(DevPlan P, int Kb, const int *__restrict__ pgrp,
                                                    int ngrp,
                                                    const typename CX<R>::T *__restrict__ bufB,
                                                    typename CX<R>::T *__restrict__ scratch,
