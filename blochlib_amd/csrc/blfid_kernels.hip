@@ -631,9 +631,11 @@ template <> struct ExpTol<double> { static constexpr double tol = 1e-17, rmax = 
 template <> struct ExpTol<float> { static constexpr double tol = 2e-8, rmax = 2.0; };
 
 // The 16 x 16 complex64 variant is held to four waves per SIMD (128 registers, no spills; three
-// without the bound).
+// without the bound).  The FP64 49-64-state variant (like k_dyson's and k_qr's) is held to three
+// workgroups per CU: 168 registers instead of 234 with 17 spilled, still 16% faster than two
+// (k_dyson 14%, no spills; k_qr 21%, 6 spilled).
 template <int M, typename R>
-__global__ void __launch_bounds__(NThr<M>::v, (M == 1 && sizeof(R) == 4) ? 4 : 1) k_exp(DevPlan P, int Kb, int c0,
+__global__ void __launch_bounds__(NThr<M>::v, (M == 1 && sizeof(R) == 4) ? 4 : (M == 4 && sizeof(R) == 8) ? 3 : 1) k_exp(DevPlan P, int Kb, int c0,
                                                     const int *__restrict__ grp, int ngrp,
                                                     R *__restrict__ scratch, size_t slot,
                                                     typename CX<R>::T *__restrict__ bufU) {
@@ -899,7 +901,7 @@ __device__ __forceinline__ void chain_mfma(int b, int n, const cf *__restrict__ 
 // crystallite's akeep slots: all of them for gamma-COMPUTE, two alternating ones when only
 // the period propagator A_n is needed.
 template <int M, typename R, bool SPLIT = false>
-__global__ void __launch_bounds__(NThr<M>::v, (M == 3 && sizeof(R) == 4) ? 3 : (M == 4 && sizeof(R) == 4) ? 2 : 1) k_dyson(DevPlan P, int Kb, const int *__restrict__ grp,
+__global__ void __launch_bounds__(NThr<M>::v, (M == 3 && sizeof(R) == 4) ? 3 : (M == 4 && sizeof(R) == 4) ? 2 : (M == 4 && sizeof(R) == 8) ? 3 : 1) k_dyson(DevPlan P, int Kb, const int *__restrict__ grp,
                                                       int ngrp,
                                                       const typename CX<R>::T *__restrict__ bufU,
                                                       typename CX<R>::T *__restrict__ bufA) {
@@ -1962,7 +1964,7 @@ template <int M, typename R, bool SPLIT, bool FLIP = false> struct QrThreads {
     static constexpr int v = (M >= 4 && !FLIP && std::is_same<R, float>::value && !SPLIT) ? 512 : NThr<M>::v;
 };
 template <int M, typename R, bool SPLIT = false, bool FLIP = false>
-__global__ void __launch_bounds__((QrThreads<M, R, SPLIT, FLIP>::v), (QrThreads<M, R, SPLIT, FLIP>::v == 512) ? 2 : (M == 5 && sizeof(R) == 4) ? 3 : (M == 3 && sizeof(R) == 4 && !FLIP && !SPLIT) ? 5 : 1) k_qr(DevPlan P, int Kb, const int *__restrict__ pgrp,
+__global__ void __launch_bounds__((QrThreads<M, R, SPLIT, FLIP>::v), (QrThreads<M, R, SPLIT, FLIP>::v == 512) ? 2 : (M == 5 && sizeof(R) == 4) ? 3 : (M == 3 && sizeof(R) == 4 && !FLIP && !SPLIT) ? 5 : (M == 4 && sizeof(R) == 8) ? 3 : 1) k_qr(DevPlan P, int Kb, const int *__restrict__ pgrp,
                                                    int ngrp,
                                                    const typename CX<R>::T *__restrict__ bufB,
                                                    typename CX<R>::T *__restrict__ scratch,
