@@ -306,3 +306,40 @@ def test_cycler_between_pulses_against_oracle(tmp_path):
     err = np.linalg.norm(res["fid"] - ref) / np.linalg.norm(ref)
     print(f"[sim] cycler: rel-L2 {err:.3e}", flush=True)
     assert err < 1e-9 and res["sw"] == sw
+
+
+def test_ptop_cycler_points_against_oracle(tmp_path):
+    """ptop fid() whose list holds a used sub-sequence and then cycler(op): after each point the
+    period propagator is applied, then the projection (subSequence::propogate,
+    sequencerun.cc:164-196); against the oracle's state functions with the projection in numpy
+    between the points."""
+    from test_simfile import C7
+    text = _small_powder(C7).replace("npts1D=64", "npts1D=6").replace(
+        "    reuse(sub1)\n    fid()", "    reuse(sub1)\n    cycler(Iz_0)\n    fid()")
+    print("[sim] ptop cycler", flush=True)
+    sim = SimFile(text)
+    res = sim.run(outdir=str(tmp_path))
+    sim2 = SimFile(text)
+    r = _Runner(sim2, ".", "fp64", False)
+    for it in sim2.pulses.items:
+        if isinstance(it, str) and not it.replace(" ", "").startswith(("fid(", "savefid")):
+            r.statement(it)
+    props = r.proplist
+    assert [p.proj is not None for p in props] == [False, True]
+    sy, pw = sim2.system, sim2.powder
+    op, det = sy.op("Iz_0"), sy.op("Iz_0")
+    opH = op.conj().T
+    pr = props[0]
+    state = np.stack([sy.op("Iz_0+Iz_1")] * len(pw))
+    pts = []
+    for m in range(6):
+        pts.append(_norm_trace(state, det, pw))
+        state = O.state_apply(sy, 5000.0, MAGIC_DEG, [x[1] for x in pr.steps], pw, state,
+                              rf=_rf_tables(pr.steps), step_t1=[x[0] for x in pr.steps],
+                              repeat=pr.repeat)
+        t = np.einsum("kab,ba->k", state, opH) / np.trace(op @ opH)
+        state = t[:, None, None] * op[None]
+    ref = np.array(pts)
+    err = np.abs(res["fid"] - ref).max() / np.abs(ref).max()
+    print(f"[sim] ptop cycler: rel {err:.3e}", flush=True)
+    assert err < 1e-10
